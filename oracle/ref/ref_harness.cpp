@@ -1,0 +1,412 @@
+// ref_harness.cpp - extern "C" entry points over the REAL reference library
+// (AuroraPIC sources compiled unmodified from the reference tree by
+// oracle/ref/Makefile into oracle/_ref/).  Used only by tests (to validate the
+// oracle restatement and write golden fixtures) and by bench.py's
+// cpu_baseline leg (to time the reference's own 3D step on the host).
+//
+// This file is this repository's code; it contains no reference source.
+#include "pic/Collision.hpp"
+#include "pic/FieldSolver.hpp"
+#include "pic/GasDataset.hpp"
+#include "pic/Mesh2D.hpp"
+#include "pic/Mesh3D.hpp"
+#include "pic/PrescribedField.hpp"
+#include "pic/Pusher.hpp"
+#include "pic/Simulation3D.hpp"
+#include "pic/Species.hpp"
+#include "pic/Swarm.hpp"
+
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <random>
+#include <string>
+#include <unistd.h>
+#include <vector>
+
+namespace {
+
+pic::Boundary to_bc(int bc) {
+    return bc == 0 ? pic::Boundary::Periodic : pic::Boundary::Dirichlet;
+}
+
+void set_error(char* err, int len, const std::string& what) {
+    if (!err || len <= 0) return;
+    std::snprintf(err, static_cast<std::size_t>(len), "%s", what.c_str());
+}
+
+std::filesystem::path temp_dir() {
+    auto dir = std::filesystem::temp_directory_path() /
+               ("aurora_ref_" + std::to_string(::getpid()));
+    std::filesystem::create_directories(dir);
+    return dir;
+}
+
+std::filesystem::path write_table(const std::string& stem, int64_t rows,
+                                  const double* e, const double* s) {
+    auto path = temp_dir() / (stem + ".dat");
+    std::ofstream out(path);
+    out << std::setprecision(17);
+    for (int64_t i = 0; i < rows; ++i) out << e[i] << ' ' << s[i] << '\n';
+    return path;
+}
+
+pic::CollisionProcessKind to_kind(int k) {
+    switch (k) {
+        case 0: return pic::CollisionProcessKind::Elastic;
+        case 1: return pic::CollisionProcessKind::Excitation;
+        case 2: return pic::CollisionProcessKind::Ionization;
+        default: return pic::CollisionProcessKind::Attachment;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_deposit(int dim, const int64_t* n, const double* L, int bc,
+                int64_t np, const double* pos, const uint8_t* alive,
+                double charge, double weight, double* rho) {
+    try {
+        if (dim == 1) {
+            pic::Grid grid(n[0], L[0], to_bc(bc));
+            pic::SpeciesConfig cfg;
+            cfg.charge = charge;
+            cfg.weight = weight;
+            cfg.particles = static_cast<std::size_t>(np);
+            pic::Species sp(cfg, 1);
+            sp.particles().resize(static_cast<std::size_t>(np));
+            for (int64_t p = 0; p < np; ++p) {
+                sp.particles()[p].x = pos[p];
+                sp.particles()[p].alive = alive ? alive[p] != 0 : true;
+            }
+            grid.clear_charge();
+            sp.deposit_charge(grid);
+            std::memcpy(rho, grid.rho().data(), n[0] * sizeof(double));
+        } else if (dim == 2) {
+            pic::Mesh2D mesh(n[0], n[1], L[0], L[1], to_bc(bc));
+            std::vector<pic::Particle2D> ps(static_cast<std::size_t>(np));
+            for (int64_t p = 0; p < np; ++p) {
+                ps[p].position = {pos[2 * p], pos[2 * p + 1]};
+                ps[p].alive = alive ? alive[p] != 0 : true;
+            }
+            pic::deposit_charge_cic(mesh, ps, charge, weight);
+            std::memcpy(rho, mesh.rho().data(), mesh.size() * sizeof(double));
+        } else {
+            pic::Mesh3D mesh(n[0], n[1], n[2], L[0], L[1], L[2], to_bc(bc));
+            std::vector<pic::Particle3D> ps(static_cast<std::size_t>(np));
+            for (int64_t p = 0; p < np; ++p) {
+                ps[p].position = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+                ps[p].alive = alive ? alive[p] != 0 : true;
+            }
+            pic::deposit_charge_cic(mesh, ps, charge, weight);
+            std::memcpy(rho, mesh.rho().data(), mesh.size() * sizeof(double));
+        }
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+int ref_gather(int dim, const int64_t* n, const double* L, int bc,
+               int64_t np, const double* pos, const double* E, double* out) {
+    try {
+        if (dim == 1) {
+            pic::Grid grid(n[0], L[0], to_bc(bc));
+            std::memcpy(grid.electric().data(), E, n[0] * sizeof(double));
+            for (int64_t p = 0; p < np; ++p) out[p] = pic::interpolate_electric(grid, pos[p]);
+        } else if (dim == 2) {
+            pic::Mesh2D mesh(n[0], n[1], L[0], L[1], to_bc(bc));
+            const std::size_t nn = mesh.size();
+            std::memcpy(mesh.electric_x().data(), E, nn * sizeof(double));
+            std::memcpy(mesh.electric_y().data(), E + nn, nn * sizeof(double));
+            for (int64_t p = 0; p < np; ++p) {
+                auto e = pic::interpolate_electric(mesh, pic::Vec2{pos[2 * p], pos[2 * p + 1]});
+                out[2 * p] = e.x;
+                out[2 * p + 1] = e.y;
+            }
+        } else {
+            pic::Mesh3D mesh(n[0], n[1], n[2], L[0], L[1], L[2], to_bc(bc));
+            const std::size_t nn = mesh.size();
+            std::memcpy(mesh.electric_x().data(), E, nn * sizeof(double));
+            std::memcpy(mesh.electric_y().data(), E + nn, nn * sizeof(double));
+            std::memcpy(mesh.electric_z().data(), E + 2 * nn, nn * sizeof(double));
+            for (int64_t p = 0; p < np; ++p) {
+                auto e = pic::interpolate_electric(
+                    mesh, pic::Vec3{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]});
+                out[3 * p] = e.x;
+                out[3 * p + 1] = e.y;
+                out[3 * p + 2] = e.z;
+            }
+        }
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// E: dim arrays concatenated; phi: n-node array.
+int ref_solve(int dim, const int64_t* n, const double* L, int bc, double eps,
+              const double* rho, double phi_left, double phi_right,
+              double* phi, double* E) {
+    try {
+        pic::FieldSolver solver(eps);
+        if (dim == 1) {
+            pic::Grid grid(n[0], L[0], to_bc(bc));
+            std::memcpy(grid.rho().data(), rho, n[0] * sizeof(double));
+            solver.solve(grid, phi_left, phi_right);
+            std::memcpy(phi, grid.phi().data(), n[0] * sizeof(double));
+            std::memcpy(E, grid.electric().data(), n[0] * sizeof(double));
+        } else if (dim == 2) {
+            pic::Mesh2D mesh(n[0], n[1], L[0], L[1], to_bc(bc));
+            const std::size_t nn = mesh.size();
+            std::memcpy(mesh.rho().data(), rho, nn * sizeof(double));
+            solver.solve(mesh);
+            std::memcpy(phi, mesh.phi().data(), nn * sizeof(double));
+            std::memcpy(E, mesh.electric_x().data(), nn * sizeof(double));
+            std::memcpy(E + nn, mesh.electric_y().data(), nn * sizeof(double));
+        } else {
+            pic::Mesh3D mesh(n[0], n[1], n[2], L[0], L[1], L[2], to_bc(bc));
+            const std::size_t nn = mesh.size();
+            std::memcpy(mesh.rho().data(), rho, nn * sizeof(double));
+            solver.solve(mesh);
+            std::memcpy(phi, mesh.phi().data(), nn * sizeof(double));
+            std::memcpy(E, mesh.electric_x().data(), nn * sizeof(double));
+            std::memcpy(E + nn, mesh.electric_y().data(), nn * sizeof(double));
+            std::memcpy(E + 2 * nn, mesh.electric_z().data(), nn * sizeof(double));
+        }
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// one Boris kick of size h (Pusher.cpp kick_boris) on a 3D particle
+void ref_boris(const double* v, const double* E, const double* B, double qm,
+               double h, double* out) {
+    pic::Particle3D p{};
+    p.velocity_half = {v[0], v[1], v[2]};
+    pic::kick_boris(p, {E[0], E[1], E[2]}, {B[0], B[1], B[2]}, qm, h);
+    out[0] = p.velocity_half.x;
+    out[1] = p.velocity_half.y;
+    out[2] = p.velocity_half.z;
+}
+
+int ref_table_eval(int64_t rows, const double* coords, const double* values,
+                   double q, double* out3) {
+    try {
+        std::vector<double> c(coords, coords + rows);
+        std::vector<pic::Vec3> v(static_cast<std::size_t>(rows));
+        for (int64_t i = 0; i < rows; ++i)
+            v[i] = {values[3 * i], values[3 * i + 1], values[3 * i + 2]};
+        pic::TabulatedVectorField1D table(pic::CoordinateAxis::X, c, v);
+        auto r = table.evaluate({q, 0.0, 0.0});
+        out3[0] = r.x;
+        out3[1] = r.y;
+        out3[2] = r.z;
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+int ref_xs_eval(int64_t rows, const double* e, const double* s, int64_t nq,
+                const double* q, double* out) {
+    try {
+        auto path = write_table("xs", rows, e, s);
+        pic::CrossSectionTable table(path);
+        for (int64_t i = 0; i < nq; ++i) out[i] = table.evaluate(q[i]);
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// Runs NullCollisionModel::collide (3V) on np particles with one
+// mt19937_64(seed) stream, in particle order.  counts: [candidates, nulls,
+// chan0..chanN-1, secondaries]; energy_change: per channel sums.
+int ref_mcc_collide(int nchan, const int* kind, const double* threshold,
+                    const int64_t* rows, const double* const* energy,
+                    const double* const* sigma, double neutral_density,
+                    double max_frequency, int64_t max_candidates,
+                    double particle_mass, double neutral_mass,
+                    double neutral_temperature, int64_t np,
+                    const double* v_in, double dt, uint64_t seed,
+                    double* v_out, uint64_t* counts, double* energy_change,
+                    char* err, int errlen) {
+    try {
+        pic::CollisionConfig cfg;
+        cfg.enabled = true;
+        cfg.model = pic::CollisionModelKind::NullCollision;
+        cfg.neutral_density = neutral_density;
+        cfg.species = "electrons";
+        cfg.max_frequency = max_frequency;
+        cfg.max_candidates_per_particle = static_cast<std::size_t>(max_candidates);
+        cfg.neutral_mass = neutral_mass;
+        cfg.neutral_temperature = neutral_temperature;
+        if (neutral_temperature > 0.0) cfg.gas_data_units = pic::UnitSystem::SI;
+        for (int c = 0; c < nchan; ++c) {
+            pic::CollisionChannelConfig ch;
+            ch.name = "ch" + std::to_string(c);
+            ch.process = to_kind(kind[c]);
+            ch.cross_section_file = write_table(ch.name, rows[c], energy[c], sigma[c]);
+            ch.threshold_energy = threshold[c];
+            if (ch.process == pic::CollisionProcessKind::Ionization) {
+                ch.secondary_species = "electrons";
+                ch.ion_species = "ions";
+            }
+            if (ch.process == pic::CollisionProcessKind::Attachment)
+                ch.attachment_species = "negative_ions";
+            cfg.channels.push_back(ch);
+        }
+        pic::NullCollisionModel model(cfg, particle_mass);
+        std::mt19937_64 rng(seed);
+        std::memset(counts, 0, sizeof(uint64_t) * static_cast<std::size_t>(nchan + 3));
+        for (int c = 0; c < nchan; ++c) energy_change[c] = 0.0;
+        for (int64_t p = 0; p < np; ++p) {
+            pic::Vec3 v{v_in[3 * p], v_in[3 * p + 1], v_in[3 * p + 2]};
+            auto st = model.collide(v, dt, rng);
+            v_out[3 * p] = v.x;
+            v_out[3 * p + 1] = v.y;
+            v_out[3 * p + 2] = v.z;
+            counts[0] += st.candidates;
+            counts[1] += st.null_collisions;
+            for (int c = 0; c < nchan; ++c) {
+                counts[2 + c] += st.channel_collisions[c];
+                energy_change[c] += st.channel_projectile_energy_change[c];
+            }
+            counts[2 + nchan] += st.secondaries.size();
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(err, errlen, e.what());
+        return -1;
+    }
+}
+
+// Swarm scan over one reduced field through run_swarm_benchmark
+// (fixed population).  out: drift, drift_se, mean energy, its se, ionization
+// rate, its se, candidates, collisions per channel (nchan).
+int ref_swarm(int nchan, const int* kind, const double* threshold,
+              const int64_t* rows, const double* const* energy,
+              const double* const* sigma, double neutral_mass,
+              double neutral_density, double max_frequency, double timestep,
+              int64_t steps, int64_t burn_in, int64_t particles,
+              int64_t blocks, double initial_mean_energy_ev,
+              double max_energy_ev, double reduced_field_td, uint64_t seed,
+              double* out, char* err, int errlen) {
+    try {
+        auto dir = temp_dir();
+        auto manifest = dir / "swarm.gas";
+        {
+            std::ofstream m(manifest);
+            m << std::setprecision(17);
+            m << "gas_data_version = 2\nunits = si\ngas = harness_gas\n"
+              << "neutral_mass = " << neutral_mass << "\n"
+              << "dataset_id = harness\ndataset_version = 1\n"
+              << "data_provenance = harness\ncitation = harness\n"
+              << "retrieved = 2026-01-01\nlicense = synthetic\n";
+            const char* names[] = {"elastic", "excitation", "ionization", "attachment"};
+            for (int c = 0; c < nchan; ++c) {
+                auto path = write_table("swarm_ch" + std::to_string(c), rows[c],
+                                        energy[c], sigma[c]);
+                m << "\n[collision.c" << c << "]\ntype = " << names[kind[c]]
+                  << "\ncross_section_file = " << path.string() << "\n";
+                if (kind[c] == 1 || kind[c] == 2)
+                    m << "threshold_energy = " << threshold[c] << "\n";
+            }
+        }
+        pic::SwarmBenchmarkConfig cfg;
+        cfg.gas_data_file = manifest;
+        cfg.neutral_density = neutral_density;
+        cfg.reduced_fields_td = {reduced_field_td};
+        cfg.max_frequency = max_frequency;
+        cfg.timestep = timestep;
+        cfg.steps = static_cast<std::size_t>(steps);
+        cfg.burn_in_steps = static_cast<std::size_t>(burn_in);
+        cfg.particles = static_cast<std::size_t>(particles);
+        cfg.uncertainty_blocks = static_cast<std::size_t>(blocks);
+        cfg.initial_mean_energy_ev = initial_mean_energy_ev;
+        cfg.max_energy_ev = max_energy_ev;
+        cfg.seed = seed;
+        cfg.work_item_limit = 4000000000ULL;
+        cfg.output_file = dir / "swarm.csv";
+        auto results = pic::run_swarm_benchmark(cfg);
+        const auto& r = results.front();
+        out[0] = r.electron_drift_velocity_m_s;
+        out[1] = r.mean_velocity_x_standard_error_m_s;
+        out[2] = r.mean_energy_ev;
+        out[3] = r.mean_energy_standard_error_ev;
+        out[4] = r.ionization_rate_s;
+        out[5] = r.ionization_rate_standard_error_s;
+        out[6] = static_cast<double>(r.collision_candidates);
+        for (int c = 0; c < nchan; ++c) out[7 + c] = static_cast<double>(r.channels[c].collisions);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(err, errlen, e.what());
+        return -1;
+    }
+}
+
+// Times the reference's own Simulation3D::step() on a periodic mesh with
+// species initialised by the reference's initializer.  species: nspecies
+// rows of (charge, mass, weight, particles, thermal_velocity).  B profile
+// along x (brows rows) or none.  Returns seconds per step (wall).
+int ref_sim3d_time(const int64_t* n, const double* L, double dt,
+                   double permittivity_rel, int si_units, int nspecies,
+                   const double* species, int64_t brows,
+                   const double* bcoords, const double* bvalues,
+                   int steps, double* seconds_per_step,
+                   int64_t* live_particles, char* err, int errlen) {
+    try {
+        pic::Simulation3DConfig cfg;
+        cfg.units.system = si_units ? pic::UnitSystem::SI : pic::UnitSystem::Normalized;
+        cfg.units.relative_permittivity = permittivity_rel;
+        cfg.nx = n[0];
+        cfg.ny = n[1];
+        cfg.nz = n[2];
+        cfg.length_x = L[0];
+        cfg.length_y = L[1];
+        cfg.length_z = L[2];
+        cfg.dt = dt;
+        cfg.steps = static_cast<std::size_t>(steps);
+        cfg.output_interval = 1000000;
+        cfg.boundary = pic::Boundary::Periodic;
+        if (brows > 0) {
+            std::vector<double> c(bcoords, bcoords + brows);
+            std::vector<pic::Vec3> v(static_cast<std::size_t>(brows));
+            for (int64_t i = 0; i < brows; ++i)
+                v[i] = {bvalues[3 * i], bvalues[3 * i + 1], bvalues[3 * i + 2]};
+            cfg.magnetic_field_profile = pic::TabulatedVectorField1D(pic::CoordinateAxis::X, c, v);
+        }
+        for (int s = 0; s < nspecies; ++s) {
+            pic::Species3DConfig sc;
+            sc.name = "s" + std::to_string(s);
+            sc.charge = species[5 * s];
+            sc.mass = species[5 * s + 1];
+            sc.weight = species[5 * s + 2];
+            sc.particles = static_cast<std::size_t>(species[5 * s + 3]);
+            sc.thermal_velocity = species[5 * s + 4];
+            cfg.species.push_back(sc);
+        }
+        pic::Simulation3D sim(cfg);
+        sim.initialize();
+        auto t0 = std::chrono::steady_clock::now();
+        for (int s = 0; s < steps; ++s) sim.step();
+        auto t1 = std::chrono::steady_clock::now();
+        *seconds_per_step = std::chrono::duration<double>(t1 - t0).count() / steps;
+        int64_t live = 0;
+        for (const auto& sp : sim.species()) live += static_cast<int64_t>(sp.live_count());
+        *live_particles = live;
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(err, errlen, e.what());
+        return -1;
+    }
+}
+
+}  // extern "C"

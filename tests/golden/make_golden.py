@@ -1,0 +1,131 @@
+"""Write the golden fixtures in tests/golden/ from the REAL reference.
+
+Runs only in a container that holds the reference tree: it loads
+oracle/_ref/libaurora_ref.so (the reference's own sources compiled unmodified
+by oracle/Makefile, plus oracle/ref/ref_harness.cpp) and records inputs and
+the reference's outputs as small .npz files.  The fixtures travel; the
+reference does not.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+sys.path.insert(0, os.path.join(REPO, "aurora-pic_amd"))
+
+import oracle_py  # noqa: E402
+from aurora_pic import workloads  # noqa: E402
+
+
+def save(name, **arrays):
+    np.savez_compressed(os.path.join(HERE, name), **arrays)
+    print("wrote", name)
+
+
+def mesh_cases():
+    # (name, n, L, bc)
+    return [
+        ("1d_periodic", [64], [2.0], 0),
+        ("1d_dirichlet", [33], [0.067], 1),
+        ("2d_periodic", [12, 10], [1.2, 1.0], 0),
+        ("2d_dirichlet", [9, 7], [1.0, 0.75], 1),
+        ("3d_periodic", [8, 6, 5], [1.0, 0.8, 0.6], 0),
+        ("3d_dirichlet", [6, 5, 4], [1.2, 1.0, 0.8], 1),
+    ]
+
+
+def main():
+    ref = oracle_py.Reference()
+    rng = np.random.default_rng(20260731)
+
+    for name, n, L, bc in mesh_cases():
+        dim = len(n)
+        npart = 400 * dim
+        # positions inside and (periodic) outside the box, plus exact nodes
+        lo = -0.5 if bc == 0 else 0.0
+        hi = 1.5 if bc == 0 else 1.0
+        pos = rng.uniform(lo, hi, size=(npart, dim)) * np.asarray(L)
+        pos[0] = 0.0
+        pos[1] = np.asarray(L)  # upper wall / wrap point
+        alive = (rng.random(npart) > 0.1).astype(np.uint8)
+        rho = ref.deposit(n, L, bc, pos, -1.5, 0.75, alive)
+        nn = int(np.prod(n))
+        E = rng.normal(size=(dim, nn))
+        gathered = ref.gather(n, L, bc, pos, E)
+        save(f"deposit_gather_{name}.npz", n=np.asarray(n), L=np.asarray(L), bc=bc, pos=pos,
+             alive=alive, charge=-1.5, weight=0.75, rho=rho, E=E, gathered=gathered)
+
+    # Poisson solves, including non power-of-two periodic extents
+    for name, n, L, bc, eps in [
+        ("1d_periodic", [64], [2.0], 0, 1.0),
+        ("1d_dirichlet", [65], [0.067], 1, 8.8541878128e-12),
+        ("2d_pow2", [16, 32], [1.0, 1.5], 0, 1.0),
+        ("2d_mixed", [12, 10], [1.0, 1.25], 0, 2.0),
+        ("3d_pow2", [8, 16, 8], [1.0, 1.25, 1.5], 0, 1.0),
+        ("3d_mixed", [8, 10, 12], [1.0, 1.25, 1.5], 0, 1.0),
+    ]:
+        nn = int(np.prod(n))
+        rho = rng.normal(size=nn)
+        phi_l, phi_r = (0.0, 0.0) if bc == 0 else (3.0, -7.5)
+        phi, E = ref.solve(n, L, bc, eps, rho, phi_l, phi_r)
+        save(f"solve_{name}.npz", n=np.asarray(n), L=np.asarray(L), bc=bc, eps=eps, rho=rho,
+             phi_left=phi_l, phi_right=phi_r, phi=phi, E=E)
+
+    # Boris kicks
+    v = rng.normal(size=(200, 3))
+    E = rng.normal(size=(200, 3))
+    B = rng.normal(size=(200, 3))
+    qm = rng.uniform(-2, 2, size=200)
+    h = rng.uniform(-0.2, 0.2, size=200)
+    out = np.stack([ref.boris(v[i], E[i], B[i], qm[i], h[i]) for i in range(200)])
+    save("boris.npz", v=v, E=E, B=B, qm=qm, h=h, out=out)
+
+    # tabulated vector field (non-uniform coordinates)
+    coords = np.cumsum(rng.uniform(0.1, 1.0, size=40))
+    coords -= coords[0]
+    values = rng.normal(size=(40, 3))
+    q = np.concatenate([rng.uniform(-0.5, coords[-1] + 0.5, size=300), coords[[0, 7, -1]]])
+    outs = []
+    valid = []
+    for x in q:
+        r = ref.table_eval(coords, values, x)
+        valid.append(r is not None)
+        outs.append(r if r is not None else np.zeros(3))
+    save("table.npz", coords=coords, values=values, q=q, out=np.stack(outs),
+         valid=np.asarray(valid))
+
+    # cross-section interpolation on the synthetic gas tables
+    tables = workloads.synthetic_gas_tables(rows=200)
+    for c, (kind, thr, e, s) in enumerate(tables):
+        qs = np.concatenate([np.geomspace(e[0] / 10, e[-1] * 10, 500), e[[0, 5, -1]]])
+        save(f"xs_{c}.npz", e=e, s=s, q=qs, out=ref.xs_eval(e, s, qs))
+
+    # MCC statistics on the synthetic tables: the reference's own stream
+    model = workloads.synthetic_mcc_model(rows=200)
+    n = 6000
+    speeds = np.sqrt(2 * np.geomspace(1.0, 400.0, n) * workloads.EV / workloads.ME)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    # float32-representable inputs keep the fixture small and exact
+    vin = (dirs * speeds[:, None]).astype(np.float32).astype(np.float64)
+    dt = 2.0e-9
+    vout, counts, ech = ref.mcc_collide(model, vin, dt, seed=8128)
+    save("mcc_stats.npz", vin=vin.astype(np.float32), dt=dt, counts=counts.astype(np.int64), energy_change=ech,
+         vout_energy=np.sum(vout ** 2, axis=1))
+
+    # swarm, one reduced field, fixed population
+    cfg = dict(neutral_density=1.0e21, max_frequency=model.max_frequency,
+               timestep=0.099 / model.max_frequency, steps=3000, burn_in=1000, particles=2000,
+               blocks=10, initial_mean_energy_ev=1.0, max_energy_ev=900.0,
+               reduced_field_td=300.0)
+    out = ref.swarm(model, workloads.HELIUM_MASS, cfg, seed=271828)
+    save("swarm_300td.npz", out=out, **{k: np.asarray(v) for k, v in cfg.items()})
+
+
+if __name__ == "__main__":
+    main()
