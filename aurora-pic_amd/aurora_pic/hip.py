@@ -1,0 +1,421 @@
+"""ctypes binding of libaurora_hip.so (the C ABI in include/aurora_hip.h).
+
+Fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REPO = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "lib", "libaurora_hip.so")
+HEADER = os.path.join(REPO, "include", "aurora_hip.h")
+
+PERIODIC, DIRICHLET = 0, 1
+PB_ABSORB, PB_REFLECT, PB_PERIODIC = 1, 2, 3
+VELOCITY_HALF, VELOCITY_CENTRED = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_ip = C.POINTER(C.c_int)
+
+
+class HipError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+class SwarmRequest(C.Structure):
+    _fields_ = [
+        ("reduced_field_td", C.c_double),
+        ("timestep", C.c_double),
+        ("particles", C.c_int64),
+        ("steps", C.c_int64),
+        ("burn_in", C.c_int64),
+        ("blocks", C.c_int64),
+        ("initial_mean_energy_ev", C.c_double),
+        ("max_energy_ev", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class SwarmResult(C.Structure):
+    _fields_ = [
+        ("drift", C.c_double),
+        ("drift_se", C.c_double),
+        ("mean_energy_ev", C.c_double),
+        ("mean_energy_se", C.c_double),
+        ("ionization_rate", C.c_double),
+        ("ionization_rate_se", C.c_double),
+        ("rate", C.c_double * 8),
+        ("rate_se", C.c_double * 8),
+        ("collisions", C.c_uint64 * 8),
+        ("candidates", C.c_uint64),
+        ("nulls", C.c_uint64),
+        ("max_energy_ev", C.c_double),
+        ("status", C.c_int),
+    ]
+
+
+def declared_symbols(header: str = HEADER):
+    """Function names declared in include/aurora_hip.h."""
+    text = open(header).read()
+    return sorted(set(re.findall(r"\b(aurora_hip_\w+)\s*\(", text)))
+
+
+_LIB = None
+
+
+def load(path: str = LIB_PATH):
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `python aurora-pic_amd/build.py` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(path)
+    ctx = C.c_void_p
+    sig = {
+        "aurora_hip_create": [C.c_int, C.POINTER(C.c_void_p)],
+        "aurora_hip_destroy": [ctx],
+        "aurora_hip_synchronize": [ctx],
+        "aurora_hip_device_info": [ctx, C.c_char_p, C.c_size_t],
+        "aurora_hip_mesh_bind": [ctx, C.c_int, _i64p, _dp, C.c_int, C.c_double, C.c_double],
+        "aurora_hip_particle_boundaries": [ctx, _ip],
+        "aurora_hip_species_add": [ctx, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int,
+                                   _ip],
+        "aurora_hip_species_upload": [ctx, C.c_int, C.c_int64, _dp, _dp, _u8p, C.c_int],
+        "aurora_hip_species_init_maxwellian": [ctx, C.c_int, C.c_int64, C.c_double, _dp,
+                                               C.c_uint64],
+        "aurora_hip_species_count": [ctx, C.c_int, _i64p, _i64p],
+        "aurora_hip_species_download": [ctx, C.c_int, C.c_int64, _dp, _dp, _u8p, _u32p, _i64p],
+        "aurora_hip_species_download_raw": [ctx, C.c_int, C.c_int64, _i32p, _dp, _dp, _i64p],
+        "aurora_hip_species_velocity": [ctx, C.c_int, C.c_double, C.c_int64, _dp],
+        "aurora_hip_species_charge_fixed": [ctx, C.c_int, _i64p],
+        "aurora_hip_set_uniform_b": [ctx, _dp],
+        "aurora_hip_set_b_profile": [ctx, C.c_int, C.c_int64, _dp, _dp],
+        "aurora_hip_set_electrodes": [ctx, _dp, _dp],
+        "aurora_hip_set_electric": [ctx, _dp],
+        "aurora_hip_get_fields": [ctx, _dp, _dp, _dp],
+        "aurora_hip_set_rho": [ctx, _dp],
+        "aurora_hip_deposit": [ctx],
+        "aurora_hip_field_solve": [ctx, C.c_double, C.c_double, C.c_int],
+        "aurora_hip_push": [ctx, C.c_double],
+        "aurora_hip_initialize": [ctx, C.c_double],
+        "aurora_hip_step": [ctx, C.c_double, C.c_int64],
+        "aurora_hip_energy": [ctx, C.c_double, _dp, _dp, _dp, _i64p],
+        "aurora_hip_boundary_losses": [ctx, C.c_int, _i64p, _dp],
+        "aurora_hip_time": [ctx, _dp, _i64p],
+        "aurora_hip_set_sort_interval": [ctx, C.c_int64],
+        "aurora_hip_sort": [ctx],
+        "aurora_hip_mcc_load": [ctx, C.c_int, C.c_int, _ip, _dp, _i64p, C.POINTER(_dp),
+                                C.POINTER(_dp), C.c_double, C.c_double, C.c_int64, C.c_double,
+                                C.c_double, _ip, C.c_uint64],
+        "aurora_hip_mcc_stats": [ctx, _u64p, _dp],
+        "aurora_hip_mcc_collide": [ctx, C.c_int64, _dp, C.c_double, C.c_uint64, C.c_uint32,
+                                   _u64p, _dp],
+        "aurora_hip_swarm_gas": [ctx, C.c_int, _ip, _dp, _i64p, C.POINTER(_dp), C.POINTER(_dp),
+                                 C.c_double, C.c_double, C.c_int64, C.c_double],
+        "aurora_hip_swarm_submit": [ctx, C.POINTER(SwarmRequest), C.c_uint64, _i64p],
+        "aurora_hip_swarm_advance": [ctx, C.c_int64, _i64p],
+        "aurora_hip_swarm_poll": [ctx, C.c_int64, _ip, C.POINTER(SwarmResult)],
+        "aurora_hip_kernel_timing": [ctx, C.c_int],
+        "aurora_hip_kernel_time": [ctx, C.c_char_p, _dp, _i64p],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.aurora_hip_last_error.argtypes = [ctx]
+    L.aurora_hip_last_error.restype = C.c_char_p
+    L.aurora_hip_swarm_footprint.argtypes = [C.POINTER(SwarmRequest)]
+    L.aurora_hip_swarm_footprint.restype = C.c_uint64
+    _LIB = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One device context: a mesh, its species, fields and MCC/swarm state."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.aurora_hip_create(device, C.byref(h))
+        if rc:
+            raise HipError(rc, "aurora_hip_create failed (no usable HIP device?)")
+        self.h = h
+        self.dim = 0
+        self.nnodes = 0
+        self.species = []
+
+    def close(self):
+        if self.h:
+            self.lib.aurora_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise HipError(rc, self.lib.aurora_hip_last_error(self.h).decode())
+
+    def info(self):
+        buf = C.create_string_buffer(256)
+        self._check(self.lib.aurora_hip_device_info(self.h, buf, 256))
+        return buf.value.decode()
+
+    # ---- mesh / species
+    def mesh(self, n, L, bc=PERIODIC, eps=1.0, depth=1.0):
+        n = np.ascontiguousarray(n, dtype=np.int64)
+        L = _f64(L)
+        self._check(self.lib.aurora_hip_mesh_bind(self.h, len(n), _p(n, _i64p), _p(L, _dp), bc,
+                                                  eps, depth))
+        self.dim = len(n)
+        self.nnodes = int(np.prod(n))
+        self.species = []
+
+    def boundaries(self, policy):
+        pol = (C.c_int * 6)(*(list(policy) + [3] * (6 - len(policy))))
+        self._check(self.lib.aurora_hip_particle_boundaries(self.h, pol))
+
+    def add_species(self, charge, mass, weight, capacity, precision=32):
+        sid = C.c_int()
+        self._check(self.lib.aurora_hip_species_add(self.h, charge, mass, weight, int(capacity),
+                                                     precision, C.byref(sid)))
+        self.species.append(sid.value)
+        return sid.value
+
+    def upload(self, sid, pos, vel, alive=None, kind=VELOCITY_HALF):
+        pos = _f64(pos).reshape(-1)
+        vel = _f64(vel).reshape(-1)
+        count = vel.size // 3
+        al = None if alive is None else np.ascontiguousarray(alive, dtype=np.uint8)
+        self._check(self.lib.aurora_hip_species_upload(self.h, sid, count, _p(pos, _dp),
+                                                       _p(vel, _dp),
+                                                       None if al is None else _p(al, _u8p),
+                                                       kind))
+
+    def init_maxwellian(self, sid, count, vth, seed, drift=(0.0, 0.0, 0.0)):
+        d = _f64(drift)
+        self._check(self.lib.aurora_hip_species_init_maxwellian(self.h, sid, int(count), vth,
+                                                                _p(d, _dp), seed))
+
+    def count(self, sid):
+        s, l = C.c_int64(), C.c_int64()
+        self._check(self.lib.aurora_hip_species_count(self.h, sid, C.byref(s), C.byref(l)))
+        return s.value, l.value
+
+    def download(self, sid):
+        slots, _ = self.count(sid)
+        pos = np.zeros(max(slots, 1) * self.dim)
+        vel = np.zeros(max(slots, 1) * 3)
+        alive = np.zeros(max(slots, 1), dtype=np.uint8)
+        uid = np.zeros(max(slots, 1), dtype=np.uint32)
+        n = C.c_int64()
+        self._check(self.lib.aurora_hip_species_download(self.h, sid, max(slots, 1),
+                                                         _p(pos, _dp), _p(vel, _dp),
+                                                         _p(alive, _u8p), _p(uid, _u32p),
+                                                         C.byref(n)))
+        k = n.value
+        return (pos[:k * self.dim].reshape(k, self.dim), vel[:k * 3].reshape(k, 3), alive[:k],
+                uid[:k])
+
+    def download_raw(self, sid):
+        slots, _ = self.count(sid)
+        cell = np.zeros(max(slots, 1), dtype=np.int32)
+        off = np.zeros(max(slots, 1) * self.dim)
+        vel = np.zeros(max(slots, 1) * 3)
+        n = C.c_int64()
+        self._check(self.lib.aurora_hip_species_download_raw(self.h, sid, max(slots, 1),
+                                                             _p(cell, _i32p), _p(off, _dp),
+                                                             _p(vel, _dp), C.byref(n)))
+        k = n.value
+        return cell[:k], off[:k * self.dim].reshape(k, self.dim), vel[:k * 3].reshape(k, 3)
+
+    def velocity(self, sid, dt):
+        slots, _ = self.count(sid)
+        v = np.zeros(max(slots, 1) * 3)
+        self._check(self.lib.aurora_hip_species_velocity(self.h, sid, dt, max(slots, 1),
+                                                         _p(v, _dp)))
+        return v[:slots * 3].reshape(slots, 3)
+
+    def charge_fixed(self, sid):
+        acc = np.zeros(self.nnodes, dtype=np.int64)
+        self._check(self.lib.aurora_hip_species_charge_fixed(self.h, sid, _p(acc, _i64p)))
+        return acc
+
+    # ---- fields
+    def uniform_b(self, b):
+        b = _f64(b)
+        self._check(self.lib.aurora_hip_set_uniform_b(self.h, _p(b, _dp)))
+
+    def b_profile(self, axis, coords, values):
+        coords, values = _f64(coords), _f64(values).reshape(-1)
+        self._check(self.lib.aurora_hip_set_b_profile(self.h, axis, coords.size,
+                                                      _p(coords, _dp), _p(values, _dp)))
+
+    def electrodes(self, left4, right4):
+        l4, r4 = _f64(left4), _f64(right4)
+        self._check(self.lib.aurora_hip_set_electrodes(self.h, _p(l4, _dp), _p(r4, _dp)))
+
+    def set_electric(self, E):
+        E = _f64(E).reshape(-1)
+        self._check(self.lib.aurora_hip_set_electric(self.h, _p(E, _dp)))
+
+    def set_rho(self, rho):
+        rho = _f64(rho)
+        self._check(self.lib.aurora_hip_set_rho(self.h, _p(rho, _dp)))
+
+    def fields(self):
+        rho = np.zeros(self.nnodes)
+        phi = np.zeros(self.nnodes)
+        E = np.zeros(self.nnodes * self.dim)
+        self._check(self.lib.aurora_hip_get_fields(self.h, _p(rho, _dp), _p(phi, _dp),
+                                                   _p(E, _dp)))
+        return rho, phi, E.reshape(self.dim, self.nnodes)
+
+    # ---- operations
+    def deposit(self):
+        self._check(self.lib.aurora_hip_deposit(self.h))
+
+    def solve(self, phi_left=0.0, phi_right=0.0, want_phi=True):
+        self._check(self.lib.aurora_hip_field_solve(self.h, phi_left, phi_right, int(want_phi)))
+
+    def push(self, dt):
+        self._check(self.lib.aurora_hip_push(self.h, dt))
+
+    def initialize(self, dt):
+        self._check(self.lib.aurora_hip_initialize(self.h, dt))
+
+    def step(self, dt, n=1):
+        self._check(self.lib.aurora_hip_step(self.h, dt, n))
+
+    def sync(self):
+        self._check(self.lib.aurora_hip_synchronize(self.h))
+
+    def energy(self, dt):
+        ke = np.zeros(max(len(self.species), 1))
+        fe, l1 = C.c_double(), C.c_double()
+        live = C.c_int64()
+        self._check(self.lib.aurora_hip_energy(self.h, dt, _p(ke, _dp), C.byref(fe), C.byref(l1),
+                                               C.byref(live)))
+        return ke[:len(self.species)], fe.value, l1.value, live.value
+
+    def losses(self, sid):
+        a = np.zeros(6, dtype=np.int64)
+        e = np.zeros(2)
+        self._check(self.lib.aurora_hip_boundary_losses(self.h, sid, _p(a, _i64p), _p(e, _dp)))
+        return a, e
+
+    def time(self):
+        t, s = C.c_double(), C.c_int64()
+        self._check(self.lib.aurora_hip_time(self.h, C.byref(t), C.byref(s)))
+        return t.value, s.value
+
+    def sort_interval(self, k):
+        self._check(self.lib.aurora_hip_set_sort_interval(self.h, k))
+
+    def sort(self):
+        self._check(self.lib.aurora_hip_sort(self.h))
+
+    # ---- timing
+    def timing(self, enable=True):
+        self._check(self.lib.aurora_hip_kernel_timing(self.h, int(enable)))
+
+    def kernel_time(self, name):
+        ms, n = C.c_double(), C.c_int64()
+        self._check(self.lib.aurora_hip_kernel_time(self.h, name.encode(), C.byref(ms),
+                                                    C.byref(n)))
+        return ms.value, n.value
+
+    # ---- MCC
+    def _tables(self, channels):
+        nch = len(channels)
+        kinds = (C.c_int * nch)(*[int(c[0]) for c in channels])
+        thr = _f64([c[1] for c in channels])
+        rows = np.ascontiguousarray([len(c[2]) for c in channels], dtype=np.int64)
+        self._keep = [(_f64(c[2]), _f64(c[3])) for c in channels]
+        ep = (_dp * nch)(*[_p(e, _dp) for e, _ in self._keep])
+        sp = (_dp * nch)(*[_p(s, _dp) for _, s in self._keep])
+        return nch, kinds, thr, rows, ep, sp
+
+    def mcc_load(self, target, gas, product_species=(-1, -1), seed=1, neutral_temperature=0.0):
+        nch, kinds, thr, rows, ep, sp = self._tables(gas["channels"])
+        prod = (C.c_int * 2)(*product_species)
+        self._check(self.lib.aurora_hip_mcc_load(
+            self.h, target, nch, kinds, _p(thr, _dp), _p(rows, _i64p), ep, sp,
+            gas["neutral_density"], gas["max_frequency"], gas.get("max_candidates", 64),
+            gas["neutral_mass"], neutral_temperature, prod, seed))
+        self._nchan = nch
+
+    def mcc_stats(self):
+        counts = np.zeros(3 + self._nchan, dtype=np.uint64)
+        ech = np.zeros(self._nchan)
+        self._check(self.lib.aurora_hip_mcc_stats(self.h, _p(counts, _u64p), _p(ech, _dp)))
+        return counts, ech
+
+    def mcc_collide(self, vel, dt, step, stream_id=0):
+        v = _f64(vel).copy().reshape(-1)
+        counts = np.zeros(3 + self._nchan, dtype=np.uint64)
+        ech = np.zeros(self._nchan)
+        self._check(self.lib.aurora_hip_mcc_collide(self.h, v.size // 3, _p(v, _dp), dt, step,
+                                                    stream_id, _p(counts, _u64p),
+                                                    _p(ech, _dp)))
+        return v.reshape(-1, 3), counts, ech
+
+    # ---- swarms
+    def swarm_gas(self, gas):
+        nch, kinds, thr, rows, ep, sp = self._tables(gas["channels"])
+        self._check(self.lib.aurora_hip_swarm_gas(
+            self.h, nch, kinds, _p(thr, _dp), _p(rows, _i64p), ep, sp, gas["neutral_density"],
+            gas["max_frequency"], gas.get("max_candidates", 64), gas["neutral_mass"]))
+        self._nchan = nch
+
+    def swarm_submit(self, request: SwarmRequest, hbm_budget=0):
+        h = C.c_int64()
+        self._check(self.lib.aurora_hip_swarm_submit(self.h, C.byref(request), hbm_budget,
+                                                     C.byref(h)))
+        return h.value
+
+    def swarm_advance(self, max_steps):
+        a = C.c_int64()
+        self._check(self.lib.aurora_hip_swarm_advance(self.h, max_steps, C.byref(a)))
+        return a.value
+
+    def swarm_poll(self, handle):
+        done = C.c_int()
+        r = SwarmResult()
+        self._check(self.lib.aurora_hip_swarm_poll(self.h, handle, C.byref(done), C.byref(r)))
+        return bool(done.value), r
+
+    def swarm_footprint(self, request: SwarmRequest):
+        return int(self.lib.aurora_hip_swarm_footprint(C.byref(request)))

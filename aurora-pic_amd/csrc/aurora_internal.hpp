@@ -1,0 +1,319 @@
+// aurora_internal.hpp - context state and device helpers shared by the
+// HIP translation units of libaurora_hip.so (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "aurora_hip.h"
+
+namespace aurora {
+
+// ------------------------------------------------------------------ errors
+struct Failure {
+    int code;
+    std::string message;
+};
+
+#define AUR_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess)                                                \
+            throw ::aurora::Failure{AURORA_ERR_HIP,                          \
+                                    std::string(#call) + ": " +              \
+                                        hipGetErrorString(e_)};              \
+    } while (0)
+
+#define AUR_REQUIRE(cond, code, msg)                                         \
+    do {                                                                     \
+        if (!(cond)) throw ::aurora::Failure{(code), (msg)};                 \
+    } while (0)
+
+// Device error bits (set with atomicOr, read back at synchronisation).
+enum : uint32_t {
+    DEV_ERR_B_RANGE = 1u,
+    DEV_ERR_RATE = 2u,
+    DEV_ERR_CANDIDATES = 4u,
+    DEV_ERR_CAPACITY = 8u,
+    DEV_ERR_ENERGY = 16u,
+    DEV_ERR_NONFINITE = 32u,
+};
+
+constexpr int kMaxChannels = 8;
+
+// --------------------------------------------------------- device memory
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t count = 0;
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        AUR_HIP(hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T)));
+        count = n;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), count(o.count) { o.ptr = nullptr; o.count = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); ptr = o.ptr; count = o.count; o.ptr = nullptr; o.count = 0; }
+        return *this;
+    }
+};
+
+// ------------------------------------------------------------- state
+struct Species {
+    double charge = 0, mass = 1, weight = 1;
+    int precision = 32;  // offsets and velocities: 32 or 64 bit
+    int64_t capacity = 0;
+    int64_t slots = 0;    // slots in use (live + dead)
+    uint32_t next_uid = 0;
+    DevBuf<int32_t> cell;     // linear cell index, -1 = dead
+    DevBuf<uint8_t> off[3];   // offsets in cell units (float or double)
+    DevBuf<uint8_t> vel[3];   // v^{n-1/2} (float or double)
+    DevBuf<uint32_t> uid;
+    DevBuf<int64_t> acc;      // fixed-point charge, one per node
+    // sort double buffers (allocated on first sort)
+    DevBuf<int32_t> cell2;
+    DevBuf<uint8_t> off2[3], vel2[3];
+    DevBuf<uint32_t> uid2;
+    // boundary ledger on device: [6 counts][2 energies as int64-free doubles]
+    DevBuf<unsigned long long> absorbed;  // 6
+    DevBuf<double> absorbed_energy;       // per-block partials (2 per block)
+    int64_t absorbed_host[6] = {0, 0, 0, 0, 0, 0};
+    double absorbed_energy_host[2] = {0, 0};
+    size_t elem() const { return precision == 64 ? 8 : 4; }
+};
+
+struct MccModelHost {
+    bool loaded = false;
+    int target = -1;
+    int nchan = 0;
+    int kind[kMaxChannels] = {};
+    double threshold[kMaxChannels] = {};
+    int64_t rows[kMaxChannels] = {};
+    int64_t row_offset[kMaxChannels] = {};
+    bool shared_grid = false;
+    double neutral_density = 0, max_frequency = 0, neutral_mass = 0, neutral_vstd = 0;
+    double particle_mass = 0;
+    int64_t max_candidates = 64;
+    int secondary_species = -1, ion_species = -1;
+    uint64_t seed = 0;
+    DevBuf<float> energy;   // all channels concatenated (FP32 tables, LDS-staged)
+    DevBuf<float> sigma;
+    DevBuf<double> energy64;  // FP64 copies (thermal bound scan)
+    DevBuf<double> sigma64;
+    DevBuf<unsigned long long> counts;  // [candidates, nulls, chan...]
+    DevBuf<double> energy_change;       // per channel, per-block partials
+    DevBuf<int32_t> product_count;      // per target particle
+    DevBuf<int64_t> product_offset;
+    DevBuf<float> product_vel;          // secondaries: up to 2 per particle x 6 floats
+    int64_t product_slots = 0;
+};
+
+struct TimerEntry {
+    double total_ms = 0;
+    int64_t launches = 0;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    std::string info;
+    int cu_count = 256;
+
+    // mesh
+    int dim = 0;
+    int64_t n[3] = {1, 1, 1};
+    double L[3] = {1, 1, 1};
+    double d[3] = {1, 1, 1};
+    int bc = AURORA_PERIODIC;
+    double eps = 1.0, depth = 1.0;
+    int64_t nnodes = 0;
+    int pb[6] = {AURORA_PB_PERIODIC, AURORA_PB_PERIODIC, AURORA_PB_PERIODIC,
+                 AURORA_PB_PERIODIC, AURORA_PB_PERIODIC, AURORA_PB_PERIODIC};
+
+    DevBuf<double> rho, phi, E64;   // E64: dim arrays
+    DevBuf<float> E32;              // packed: stride 4 (3D), 2 (2D), 1 (1D)
+    bool phi_valid = false;
+
+    // magnetic field
+    double Buni[3] = {0, 0, 0};
+    bool has_profile = false;
+    int baxis = 0;
+    int64_t brows = 0;
+    DevBuf<double> bcoords, bvalues;
+    DevBuf<int32_t> bhint;          // per cell along the table axis
+
+    // 1D electrodes: offset, amplitude, frequency, phase
+    double left4[4] = {0, 0, 0, 0};
+    double right4[4] = {0, 0, 0, 0};
+
+    double time = 0.0;
+    int64_t step_count = 0;
+
+    // FFT workspace
+    DevBuf<double2> fa, fb;
+    DevBuf<double2> tw[3];          // per-axis twiddles e^{-2 pi i k / n}
+    // 1D Dirichlet precomputed Thomas coefficients
+    DevBuf<double> thomas_cp, thomas_inv;
+    DevBuf<double> dft_cos, dft_sin;
+
+    // reductions
+    DevBuf<double> partials;
+    DevBuf<uint32_t> dev_error;
+
+    // sorting
+    int64_t sort_interval = 0;
+    int tile[3] = {8, 8, 8};
+    DevBuf<int32_t> tile_count, tile_offset;
+    DevBuf<unsigned> sort_keys, sort_keys2, sort_vals, sort_vals2;
+    DevBuf<uint8_t> sort_temp;
+
+    std::vector<Species> species;
+    MccModelHost mcc;
+
+    // swarm batch state lives in swarm.hip
+    void* swarm = nullptr;
+
+    bool timing = false;
+    std::map<std::string, TimerEntry> timers;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+
+    int64_t cells_total() const {
+        int64_t c = 1;
+        for (int a = 0; a < dim; ++a) c *= (bc == AURORA_PERIODIC ? n[a] : n[a] - 1);
+        return c;
+    }
+    int64_t cells_axis(int a) const { return bc == AURORA_PERIODIC ? n[a] : n[a] - 1; }
+};
+
+// Scoped kernel timer: records HIP events around a launch when enabled.
+struct KernelScope {
+    Context& ctx;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    KernelScope(Context& c, const char* nm) : ctx(c), name(nm) {
+        if (ctx.timing) {
+            AUR_HIP(hipEventCreate(&a));
+            AUR_HIP(hipEventCreate(&b));
+            AUR_HIP(hipEventRecord(a, ctx.stream));
+        }
+    }
+    ~KernelScope() {
+        if (ctx.timing && a) {
+            (void)hipEventRecord(b, ctx.stream);
+            ctx.pending.push_back({name, {a, b}});
+        }
+    }
+};
+void flush_timers(Context& ctx);
+
+void check_device_error(Context& ctx);
+inline dim3 grid_for(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 2147483647LL) g = 2147483647LL;
+    return dim3(static_cast<unsigned>(g));
+}
+
+// host-side API pieces implemented in separate translation units
+void solve_fields(Context& ctx, double phi_left, double phi_right, bool want_phi);
+void prepare_solver(Context& ctx);
+void deposit_all(Context& ctx);
+void convert_rho(Context& ctx);
+void push_all(Context& ctx, double dt, bool fuse_deposit);
+void sort_species(Context& ctx, Species& sp);
+void mcc_step(Context& ctx, double dt);
+void init_half_all(Context& ctx, double dt);
+void set_b_profile_device(Context& ctx);
+void swarm_destroy(Context& ctx);
+
+// ------------------------------------------------------- device helpers
+// Fixed-point quantisation: round-to-nearest-even of s * 2^40 (s in [0,1])
+// via one correctly rounded fma against 2^52.
+__device__ __forceinline__ long long quantise(double s) {
+    double t = __fma_rn(s, 0x1p40, 0x1p52);
+    return __double_as_longlong(t) - 0x4330000000000000LL;
+}
+
+// Philox4x32-10 (Salmon et al., SC'11)
+struct Philox {
+    uint32_t key0, key1;
+    uint32_t c0, c1, c2, c3;
+    uint32_t buf[4];
+    int used;
+    __device__ __forceinline__ void init(uint64_t seed, uint32_t stream, uint32_t uid,
+                                         uint32_t step) {
+        key0 = static_cast<uint32_t>(seed);
+        key1 = static_cast<uint32_t>(seed >> 32);
+        c0 = 0; c1 = uid; c2 = step; c3 = stream;
+        used = 4;
+    }
+    __device__ __forceinline__ void refill() {
+        uint32_t x0 = c0, x1 = c1, x2 = c2, x3 = c3, k0 = key0, k1 = key1;
+#pragma unroll
+        for (int r = 0; r < 10; ++r) {
+            if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+            const uint32_t lo0 = 0xD2511F53u * x0;
+            const uint32_t hi0 = __umulhi(0xD2511F53u, x0);
+            const uint32_t lo1 = 0xCD9E8D57u * x2;
+            const uint32_t hi1 = __umulhi(0xCD9E8D57u, x2);
+            const uint32_t n0 = hi1 ^ x1 ^ k0;
+            const uint32_t n2 = hi0 ^ x3 ^ k1;
+            x0 = n0; x1 = lo1; x2 = n2; x3 = lo0;
+        }
+        buf[0] = x0; buf[1] = x1; buf[2] = x2; buf[3] = x3;
+        c0 += 1;
+        used = 0;
+    }
+    // odd multiples of 2^-24 in (0,1), exact in FP32
+    __device__ __forceinline__ float u01() {
+        if (used == 4) refill();
+        const uint32_t u = buf[used++];
+        return static_cast<float>((u >> 9) * 2u + 1u) * 0x1p-24f;
+    }
+};
+
+}  // namespace aurora
+
+struct aurora_hip_ctx {
+    aurora::Context c;
+};
+
+namespace aurora {
+// Runs f(context) and maps exceptions to status codes + last_error.
+template <typename F>
+int guarded(aurora_hip_ctx* h, F&& f) {
+    if (!h) return AURORA_ERR_INVALID_ARGUMENT;
+    try {
+        h->c.last_error.clear();
+        f(h->c);
+        return AURORA_OK;
+    } catch (const Failure& e) {
+        h->c.last_error = e.message;
+        return e.code;
+    } catch (const std::exception& e) {
+        h->c.last_error = e.what();
+        return AURORA_ERR_INVALID_ARGUMENT;
+    }
+}
+inline Species& species_at(Context& c, int sid) {
+    if (sid < 0 || sid >= static_cast<int>(c.species.size()))
+        throw Failure{AURORA_ERR_NOT_FOUND, "unknown species id"};
+    return c.species[sid];
+}
+}  // namespace aurora

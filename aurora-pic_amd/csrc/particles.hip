@@ -1,0 +1,890 @@
+// particles.hip - particle kernels of the per-step loop:
+//   deposit   CIC charge into int64 fixed point (Species.cpp:161-185,
+//             Mesh2D.cpp:90-144, Mesh3D.cpp:62-126)
+//   push      gather (FieldSolver.cpp:909-1038) + leapfrog/Boris kick
+//             (Pusher.cpp:20-186) + drift + particle boundaries
+//             (Simulation3D.cpp:30-84, Simulation.cpp:1039-1077), optionally
+//             fused with the deposit at x^{n+1}
+//   sync      time-centred velocity / FP64 kinetic energy
+//             (Species3D.cpp:168-176)
+// Positions are (linear cell index, in-cell offsets in [0,1]); the cell
+// index strides are node counts, so node (i,j,k) of a cell is its corner.
+#include <cmath>
+
+#include "aurora_internal.hpp"
+
+namespace aurora {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename R>
+struct SpeciesView {
+    int32_t* cell;
+    R* off0;
+    R* off1;
+    R* off2;
+    R* v0;
+    R* v1;
+    R* v2;
+    int64_t slots;
+};
+
+struct MeshView {
+    int nx, ny, nz;
+    int periodic;          // field boundary periodic
+    int cx, cy, cz;        // cells per axis
+    int pb[6];             // particle boundary per face
+};
+
+struct BView {
+    int mode;              // 0 none, 1 uniform, 2 table
+    double bx, by, bz;
+    int axis;
+    int rows;
+    const double* coords;
+    const double* values;
+    const int32_t* hint;   // per cell along axis: first row with coord > cell start
+    double d_axis;
+};
+
+template <typename R>
+struct PushArgs {
+    SpeciesView<R> sp;
+    MeshView m;
+    BView b;
+    const float* E32;
+    const double* E64;
+    int64_t nnodes;
+    R half_qm_dt;          // 0.5 * qm * dt (Boris)
+    R qm_dt;               // qm * dt (leapfrog)
+    R dtd0, dtd1, dtd2;    // dt / d per axis
+    long long* acc;        // deposit target (fused)
+    unsigned long long* absorbed;  // 6 face counters
+    double* absorbed_energy;       // 2 sides (1D), atomics
+    double half_mw;        // 0.5 * m * w (1D loss ledger)
+    uint32_t* err;
+};
+
+__device__ __forceinline__ int wrap_index(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+// ---------------------------------------------------------------- B field
+__device__ __forceinline__ void b_table(const BView& b, double q, int cell_axis, double& Bx,
+                                        double& By, double& Bz, uint32_t* err) {
+    // PrescribedField.cpp:89-113: exact endpoints, upper_bound + lerp,
+    // out-of-range is an error.
+    const double* c = b.coords;
+    const double* v = b.values;
+    const int rows = b.rows;
+    if (!(q >= c[0] && q <= c[rows - 1])) {
+        atomicOr(err, DEV_ERR_B_RANGE);
+        const int r = q < c[0] ? 0 : rows - 1;
+        Bx = v[3 * r]; By = v[3 * r + 1]; Bz = v[3 * r + 2];
+        return;
+    }
+    if (q == c[0]) { Bx = v[0]; By = v[1]; Bz = v[2]; return; }
+    if (q == c[rows - 1]) {
+        Bx = v[3 * (rows - 1)]; By = v[3 * (rows - 1) + 1]; Bz = v[3 * (rows - 1) + 2];
+        return;
+    }
+    int r = b.hint ? b.hint[cell_axis] : 0;
+    if (r < 1) r = 1;
+    // the hint never exceeds upper_bound(q); walk forward to it
+    while (r < rows && c[r] <= q) ++r;
+    // and never undershoot when the hint is stale (no hint = r from 1)
+    const int right = r, left = r - 1;
+    const double f = (q - c[left]) / (c[right] - c[left]);
+    Bx = v[3 * left] + f * (v[3 * right] - v[3 * left]);
+    By = v[3 * left + 1] + f * (v[3 * right + 1] - v[3 * left + 1]);
+    Bz = v[3 * left + 2] + f * (v[3 * right + 2] - v[3 * left + 2]);
+}
+
+template <int DIM, typename R>
+__device__ __forceinline__ void magnetic(const BView& b, const int* idx, const R* off,
+                                         R& Bx, R& By, R& Bz, uint32_t* err) {
+    if (b.mode == 1) {
+        Bx = static_cast<R>(b.bx); By = static_cast<R>(b.by); Bz = static_cast<R>(b.bz);
+        return;
+    }
+    double q = 0.0;
+    int ca = 0;
+    if (b.axis < DIM) {
+        ca = idx[b.axis];
+        q = (static_cast<double>(ca) + static_cast<double>(off[b.axis])) * b.d_axis;
+    }
+    double x, y, z;
+    b_table(b, q, ca, x, y, z, err);
+    Bx = static_cast<R>(x); By = static_cast<R>(y); Bz = static_cast<R>(z);
+}
+
+// ------------------------------------------------------------------ gather
+template <int DIM, typename R>
+__device__ __forceinline__ void gather(const PushArgs<R>& a, const int* idx, const R* off,
+                                       R* E) {
+    const MeshView& m = a.m;
+    const R one = R(1);
+    const int i0 = idx[0];
+    const int i1 = (i0 + 1 == m.nx) ? 0 : i0 + 1;
+    const R wx[2] = {one - off[0], off[0]};
+    if constexpr (DIM == 1) {
+        if constexpr (sizeof(R) == 8) {
+            E[0] = a.E64[i0] * wx[0] + a.E64[i1] * wx[1];
+        } else {
+            E[0] = a.E32[i0] * wx[0] + a.E32[i1] * wx[1];
+        }
+        return;
+    } else {
+        const int j0 = idx[1];
+        const int j1 = (j0 + 1 == m.ny) ? 0 : j0 + 1;
+        const R wy[2] = {one - off[1], off[1]};
+        if constexpr (DIM == 2) {
+            const int n00 = j0 * m.nx + i0, n10 = j0 * m.nx + i1;
+            const int n01 = j1 * m.nx + i0, n11 = j1 * m.nx + i1;
+            const R w00 = wx[0] * wy[0], w10 = wx[1] * wy[0];
+            const R w01 = wx[0] * wy[1], w11 = wx[1] * wy[1];
+            if constexpr (sizeof(R) == 8) {
+                const double* ex = a.E64;
+                const double* ey = a.E64 + a.nnodes;
+                E[0] = ex[n00] * w00 + ex[n10] * w10 + ex[n01] * w01 + ex[n11] * w11;
+                E[1] = ey[n00] * w00 + ey[n10] * w10 + ey[n01] * w01 + ey[n11] * w11;
+            } else {
+                const float2* e2 = reinterpret_cast<const float2*>(a.E32);
+                const float2 e00 = e2[n00], e10 = e2[n10], e01 = e2[n01], e11 = e2[n11];
+                E[0] = e00.x * w00 + e10.x * w10 + e01.x * w01 + e11.x * w11;
+                E[1] = e00.y * w00 + e10.y * w10 + e01.y * w01 + e11.y * w11;
+            }
+            return;
+        } else {
+            const int k0 = idx[2];
+            const int k1 = (k0 + 1 == m.nz) ? 0 : k0 + 1;
+            const R wz[2] = {one - off[2], off[2]};
+            const int ii[2] = {i0, i1}, jj[2] = {j0, j1}, kk[2] = {k0, k1};
+            R ex = 0, ey = 0, ez = 0;
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int aa = 0; aa < 2; ++aa) {
+                        const R shape = wx[aa] * wy[b] * wz[c];
+                        const int node = (kk[c] * m.ny + jj[b]) * m.nx + ii[aa];
+                        if constexpr (sizeof(R) == 8) {
+                            ex += a.E64[node] * shape;
+                            ey += a.E64[a.nnodes + node] * shape;
+                            ez += a.E64[2 * a.nnodes + node] * shape;
+                        } else {
+                            const float4 e = reinterpret_cast<const float4*>(a.E32)[node];
+                            ex += e.x * shape;
+                            ey += e.y * shape;
+                            ez += e.z * shape;
+                        }
+                    }
+            E[0] = ex; E[1] = ey; E[2] = ez;
+        }
+    }
+}
+
+// ------------------------------------------------------------------- kick
+template <typename R>
+__device__ __forceinline__ void boris(R* v, const R* E, R Bx, R By, R Bz, R h) {
+    // Pusher.cpp:20-29 with h = 0.5 * qm * dt
+    R vm0 = v[0] + E[0] * h, vm1 = v[1] + E[1] * h, vm2 = v[2] + E[2] * h;
+    const R t0 = Bx * h, t1 = By * h, t2 = Bz * h;
+    const R tt = t0 * t0 + t1 * t1 + t2 * t2;
+    const R sf = R(2) / (R(1) + tt);
+    const R s0 = t0 * sf, s1 = t1 * sf, s2 = t2 * sf;
+    const R p0 = vm0 + (vm1 * t2 - vm2 * t1);
+    const R p1 = vm1 + (vm2 * t0 - vm0 * t2);
+    const R p2 = vm2 + (vm0 * t1 - vm1 * t0);
+    const R q0 = vm0 + (p1 * s2 - p2 * s1);
+    const R q1 = vm1 + (p2 * s0 - p0 * s2);
+    const R q2 = vm2 + (p0 * s1 - p1 * s0);
+    v[0] = q0 + E[0] * h;
+    v[1] = q1 + E[1] * h;
+    v[2] = q2 + E[2] * h;
+}
+
+// ------------------------------------------------------------------ drift
+// Mirror (ni + f) into [0, ncells] (reflect_coordinate, Simulation3D.cpp:30-40)
+// in integer + fraction form.  A particle that would need more than 64
+// reflections (a non-finite or absurd velocity) is treated as lost.
+template <typename R>
+__device__ __forceinline__ bool reflect_into(int& ni, R& f, R& v, int ncells) {
+    for (int it = 0; it < 64; ++it) {
+        if (!(ni < 0 || ni > ncells || (ni == ncells && f > R(0)))) return true;
+        if (ni < -ncells || ni > 2 * ncells) return false;
+        if (ni < 0) {
+            if (f > R(0)) { ni = -ni - 1; f = R(1) - f; } else { ni = -ni; }
+        } else {
+            if (f > R(0)) { ni = 2 * ncells - ni - 1; f = R(1) - f; } else { ni = 2 * ncells - ni; }
+        }
+        v = -v;
+    }
+    return false;
+}
+
+// Moves one axis by dx (cell units) with the face policies of the axis.
+// Returns false when the particle is absorbed (face set).
+template <typename R>
+__device__ __forceinline__ bool drift_axis(int& idx, R& off, R& v, R dx, int ncells, int n,
+                                           int pol_lo, int pol_hi, int& face_hit) {
+    R x = off + dx;
+    if (x >= R(0) && x < R(1)) {
+        off = x;
+        return true;
+    }
+    R fl = floor(x);
+    int shift = static_cast<int>(fl);
+    R f = x - fl;
+    if (f >= R(1)) { f = R(0); shift += 1; }
+    int ni = idx + shift;
+    if (ni < 0) {
+        if (pol_lo == AURORA_PB_ABSORB) { face_hit = 0; return false; }
+        if (pol_lo == AURORA_PB_PERIODIC) {
+            idx = wrap_index(ni, ncells);
+            off = f;
+            return true;
+        }
+        // reflect about x = 0 (Simulation3D.cpp:30-40): -(ni + f)
+        if (!reflect_into(ni, f, v, ncells)) { face_hit = 0; return false; }
+    } else if (ni > ncells || (ni == ncells && f > R(0))) {
+        if (pol_hi == AURORA_PB_ABSORB) { face_hit = 1; return false; }
+        if (pol_hi == AURORA_PB_PERIODIC) {
+            idx = wrap_index(ni, ncells);
+            off = f;
+            return true;
+        }
+        if (!reflect_into(ni, f, v, ncells)) { face_hit = 1; return false; }
+    }
+    // x == L exactly on a Dirichlet axis: last cell with offset 1
+    if (ni == ncells) {
+        if (ncells == n) { ni = 0; f = R(0); }       // periodic mesh: wraps to node 0
+        else { ni = ncells - 1; f = R(1); }
+    }
+    idx = ni;
+    off = f;
+    return true;
+}
+
+// ---------------------------------------------------------------- deposit
+template <int DIM>
+__device__ __forceinline__ void deposit_one(long long* acc, const MeshView& m, const int* idx,
+                                            const double* o) {
+    const int i0 = idx[0];
+    const int i1 = (i0 + 1 == m.nx) ? 0 : i0 + 1;
+    const double wx[2] = {1.0 - o[0], o[0]};
+    if constexpr (DIM == 1) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(acc + i0),
+                  static_cast<unsigned long long>(quantise(wx[0])));
+        atomicAdd(reinterpret_cast<unsigned long long*>(acc + i1),
+                  static_cast<unsigned long long>(quantise(wx[1])));
+    } else {
+        const int j0 = idx[1];
+        const int j1 = (j0 + 1 == m.ny) ? 0 : j0 + 1;
+        const double wy[2] = {1.0 - o[1], o[1]};
+        if constexpr (DIM == 2) {
+            const int jj[2] = {j0, j1}, ii[2] = {i0, i1};
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(acc + jj[b] * m.nx + ii[a]),
+                              static_cast<unsigned long long>(quantise(wx[a] * wy[b])));
+        } else {
+            const int k0 = idx[2];
+            const int k1 = (k0 + 1 == m.nz) ? 0 : k0 + 1;
+            const double wz[2] = {1.0 - o[2], o[2]};
+            const int ii[2] = {i0, i1}, jj[2] = {j0, j1}, kk[2] = {k0, k1};
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int a = 0; a < 2; ++a)
+                        atomicAdd(reinterpret_cast<unsigned long long*>(
+                                      acc + (kk[c] * m.ny + jj[b]) * m.nx + ii[a]),
+                                  static_cast<unsigned long long>(
+                                      quantise(wx[a] * wy[b] * wz[c])));
+        }
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void decompose(int c, const MeshView& m, int* idx) {
+    idx[0] = c % m.nx;
+    if constexpr (DIM > 1) {
+        const int t = c / m.nx;
+        idx[1] = t % m.ny;
+        if constexpr (DIM > 2) idx[2] = t / m.ny;
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ int compose(const MeshView& m, const int* idx) {
+    int c = idx[0];
+    if constexpr (DIM > 1) c += m.nx * idx[1];
+    if constexpr (DIM > 2) c += m.nx * m.ny * idx[2];
+    return c;
+}
+
+template <int DIM, typename R>
+__global__ void __launch_bounds__(kBlock) k_deposit(SpeciesView<R> sp, MeshView m,
+                                                    long long* acc) {
+    for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < sp.slots;
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int c = sp.cell[p];
+        if (c < 0) continue;
+        int idx[3];
+        decompose<DIM>(c, m, idx);
+        double o[3];
+        o[0] = static_cast<double>(sp.off0[p]);
+        if constexpr (DIM > 1) o[1] = static_cast<double>(sp.off1[p]);
+        if constexpr (DIM > 2) o[2] = static_cast<double>(sp.off2[p]);
+        deposit_one<DIM>(acc, m, idx, o);
+    }
+}
+
+template <int DIM, typename R, int BMODE, bool DEPOSIT>
+__global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
+    const SpeciesView<R>& sp = a.sp;
+    for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < sp.slots;
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int c = sp.cell[p];
+        if (c < 0) continue;
+        int idx[3] = {0, 0, 0};
+        decompose<DIM>(c, a.m, idx);
+        R off[3] = {sp.off0[p], R(0), R(0)};
+        if constexpr (DIM > 1) off[1] = sp.off1[p];
+        if constexpr (DIM > 2) off[2] = sp.off2[p];
+        R v[3] = {sp.v0[p], sp.v1[p], sp.v2[p]};
+        R E[3] = {R(0), R(0), R(0)};
+        gather<DIM, R>(a, idx, off, E);
+        if constexpr (BMODE == 0) {
+            // leapfrog kick on the mesh axes (Pusher.cpp:70-72, 126-129, 162-166)
+#pragma unroll
+            for (int ax = 0; ax < DIM; ++ax) v[ax] += a.qm_dt * E[ax];
+        } else {
+            R Bx, By, Bz;
+            magnetic<DIM, R>(a.b, idx, off, Bx, By, Bz, a.err);
+            boris<R>(v, E, Bx, By, Bz, a.half_qm_dt);
+        }
+        // drift + particle boundaries, axis by axis (Simulation3D.cpp:309-340)
+        const R dtd[3] = {a.dtd0, a.dtd1, a.dtd2};
+        const int nn[3] = {a.m.nx, a.m.ny, a.m.nz};
+        const int nc[3] = {a.m.cx, a.m.cy, a.m.cz};
+        bool alive = true;
+        int face = -1;
+#pragma unroll
+        for (int ax = 0; ax < DIM; ++ax) {
+            int hit = 0;
+            if (!drift_axis<R>(idx[ax], off[ax], v[ax], v[ax] * dtd[ax], nc[ax], nn[ax],
+                               a.m.pb[2 * ax], a.m.pb[2 * ax + 1], hit)) {
+                alive = false;
+                face = 2 * ax + hit;
+                break;
+            }
+        }
+        sp.v0[p] = v[0];
+        sp.v1[p] = v[1];
+        sp.v2[p] = v[2];
+        if (!alive) {
+            sp.cell[p] = -1;
+            atomicAdd(a.absorbed + face, 1ull);
+            if constexpr (DIM == 1) {
+                // Simulation.cpp:1059-1076: represented energy at the wall
+                const double e = a.half_mw * (static_cast<double>(v[0]) * v[0] +
+                                              static_cast<double>(v[1]) * v[1] +
+                                              static_cast<double>(v[2]) * v[2]);
+                atomicAdd(a.absorbed_energy + face, e);
+            }
+            continue;
+        }
+        sp.cell[p] = compose<DIM>(a.m, idx);
+        sp.off0[p] = off[0];
+        if constexpr (DIM > 1) sp.off1[p] = off[1];
+        if constexpr (DIM > 2) sp.off2[p] = off[2];
+        if constexpr (DEPOSIT) {
+            double o[3] = {static_cast<double>(off[0]), static_cast<double>(off[1]),
+                           static_cast<double>(off[2])};
+            deposit_one<DIM>(a.acc, a.m, idx, o);
+        }
+    }
+}
+
+// sign = +1: synchronize (v from v_half); -1: half-step init (v_half from v)
+template <int DIM, typename R, int BMODE>
+__global__ void __launch_bounds__(kBlock) k_half_kick(PushArgs<R> a, R sign_half, R sign_qm_dt,
+                                                      double* vout, int write_back,
+                                                      double half_mw, double* partial) {
+    __shared__ double red[kBlock];
+    double ke = 0.0;
+    const SpeciesView<R>& sp = a.sp;
+    for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < sp.slots;
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int c = sp.cell[p];
+        if (c < 0) {
+            if (vout) { vout[3 * p] = 0; vout[3 * p + 1] = 0; vout[3 * p + 2] = 0; }
+            continue;
+        }
+        int idx[3] = {0, 0, 0};
+        decompose<DIM>(c, a.m, idx);
+        R off[3] = {sp.off0[p], R(0), R(0)};
+        if constexpr (DIM > 1) off[1] = sp.off1[p];
+        if constexpr (DIM > 2) off[2] = sp.off2[p];
+        R v[3] = {sp.v0[p], sp.v1[p], sp.v2[p]};
+        R E[3] = {R(0), R(0), R(0)};
+        gather<DIM, R>(a, idx, off, E);
+        if constexpr (BMODE == 0) {
+            // half leapfrog kick: v +/- 0.5 qm E dt (Pusher.cpp:66-68, 82-86)
+#pragma unroll
+            for (int ax = 0; ax < DIM; ++ax) v[ax] += sign_qm_dt * E[ax];
+        } else {
+            R Bx, By, Bz;
+            magnetic<DIM, R>(a.b, idx, off, Bx, By, Bz, a.err);
+            boris<R>(v, E, Bx, By, Bz, sign_half);
+        }
+        if (write_back) {
+            sp.v0[p] = v[0];
+            sp.v1[p] = v[1];
+            sp.v2[p] = v[2];
+        }
+        if (vout) {
+            vout[3 * p] = v[0];
+            vout[3 * p + 1] = v[1];
+            vout[3 * p + 2] = v[2];
+        }
+        const double v0 = v[0], v1 = v[1], v2 = v[2];
+        ke += half_mw * (v0 * v0 + v1 * v1 + v2 * v2);
+    }
+    if (partial) {
+        red[threadIdx.x] = ke;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+    }
+}
+
+__global__ void k_fill_i64(long long* p, int64_t n) {
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[i] = 0;
+}
+
+struct RhoArgs {
+    const long long* acc[kMaxChannels];
+    double cw[kMaxChannels];   // charge * weight * 2^-40 (per unit depth in 2D)
+    int nspecies;
+    int nx, ny, nz;
+    int periodic;
+    double dx, dy, dz;
+    double vol;                // periodic node volume
+    int dim;
+};
+
+__global__ void k_rho(RhoArgs a, double* rho, int64_t nnodes) {
+#pragma clang fp contract(off)
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < nnodes;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        double vol = a.vol;
+        if (!a.periodic) {
+            // Grid.cpp:26-30, Mesh2D.cpp:77-88, Mesh3D.cpp:52-60
+            const int ix = static_cast<int>(i % a.nx);
+            const int iy = static_cast<int>((i / a.nx) % a.ny);
+            const int iz = static_cast<int>(i / (static_cast<int64_t>(a.nx) * a.ny));
+            const double wx = (ix == 0 || ix + 1 == a.nx) ? 0.5 * a.dx : a.dx;
+            if (a.dim == 1) vol = wx;
+            else {
+                const double wy = (iy == 0 || iy + 1 == a.ny) ? 0.5 * a.dy : a.dy;
+                if (a.dim == 2) vol = wx * wy;
+                else vol = wx * wy * ((iz == 0 || iz + 1 == a.nz) ? 0.5 * a.dz : a.dz);
+            }
+        }
+        double r = 0.0;
+        for (int s = 0; s < a.nspecies; ++s)
+            r += (static_cast<double>(a.acc[s][i]) * a.cw[s]) / vol;
+        rho[i] = r;
+    }
+}
+
+// Uniform positions and Maxwellian velocities from Philox (Box-Muller).
+template <int DIM, typename R>
+__global__ void k_init_maxwellian(SpeciesView<R> sp, MeshView m, uint32_t* uid, int64_t count,
+                                  double vth, double dx0, double dx1, double dx2, uint64_t seed) {
+    for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < count;
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        Philox rng;
+        rng.init(seed, 0x5EEDu, static_cast<uint32_t>(p), 0xFFFFFFFFu);
+        int idx[3] = {0, 0, 0};
+        R off[3] = {R(0), R(0), R(0)};
+        const int nc[3] = {m.cx, m.cy, m.cz};
+#pragma unroll
+        for (int ax = 0; ax < DIM; ++ax) {
+            // uniform over [0, cells): integer cell and offset from two draws
+            const float u = rng.u01();
+            int ci = static_cast<int>(u * static_cast<float>(nc[ax]));
+            if (ci >= nc[ax]) ci = nc[ax] - 1;
+            idx[ax] = ci;
+            off[ax] = static_cast<R>(rng.u01());
+        }
+        const float u1 = rng.u01(), u2 = rng.u01(), u3 = rng.u01(), u4 = rng.u01();
+        const float r1 = sqrtf(-2.0f * logf(u1)), r2 = sqrtf(-2.0f * logf(u3));
+        float s1, c1, s2, c2;
+        sincosf(6.28318530717958647692f * u2, &s1, &c1);
+        sincosf(6.28318530717958647692f * u4, &s2, &c2);
+        sp.cell[p] = compose<DIM>(m, idx);
+        sp.off0[p] = off[0];
+        if constexpr (DIM > 1) sp.off1[p] = off[1];
+        if constexpr (DIM > 2) sp.off2[p] = off[2];
+        sp.v0[p] = static_cast<R>(dx0 + vth * r1 * c1);
+        sp.v1[p] = static_cast<R>(dx1 + vth * r1 * s1);
+        sp.v2[p] = static_cast<R>(dx2 + vth * r2 * c2);
+        uid[p] = static_cast<uint32_t>(p);
+    }
+}
+
+template <typename R>
+SpeciesView<R> view(Species& sp) {
+    SpeciesView<R> v;
+    v.cell = sp.cell.ptr;
+    v.off0 = reinterpret_cast<R*>(sp.off[0].ptr);
+    v.off1 = reinterpret_cast<R*>(sp.off[1].ptr);
+    v.off2 = reinterpret_cast<R*>(sp.off[2].ptr);
+    v.v0 = reinterpret_cast<R*>(sp.vel[0].ptr);
+    v.v1 = reinterpret_cast<R*>(sp.vel[1].ptr);
+    v.v2 = reinterpret_cast<R*>(sp.vel[2].ptr);
+    v.slots = sp.slots;
+    return v;
+}
+
+MeshView mesh_view(const Context& c) {
+    MeshView m;
+    m.nx = static_cast<int>(c.n[0]);
+    m.ny = static_cast<int>(c.n[1]);
+    m.nz = static_cast<int>(c.n[2]);
+    m.periodic = c.bc == AURORA_PERIODIC;
+    m.cx = static_cast<int>(c.cells_axis(0));
+    m.cy = c.dim > 1 ? static_cast<int>(c.cells_axis(1)) : 1;
+    m.cz = c.dim > 2 ? static_cast<int>(c.cells_axis(2)) : 1;
+    for (int f = 0; f < 6; ++f) m.pb[f] = c.pb[f];
+    return m;
+}
+
+BView b_view(const Context& c) {
+    BView b{};
+    if (c.has_profile) {
+        b.mode = 2;
+        b.axis = c.baxis;
+        b.rows = static_cast<int>(c.brows);
+        b.coords = c.bcoords.ptr;
+        b.values = c.bvalues.ptr;
+        b.hint = c.baxis < c.dim ? c.bhint.ptr : nullptr;
+        b.d_axis = c.baxis < c.dim ? c.d[c.baxis] : 1.0;
+    } else if (c.Buni[0] != 0.0 || c.Buni[1] != 0.0 || c.Buni[2] != 0.0) {
+        b.mode = 1;
+    }
+    b.bx = c.Buni[0];
+    b.by = c.Buni[1];
+    b.bz = c.Buni[2];
+    return b;
+}
+
+template <typename R>
+PushArgs<R> push_args(Context& c, Species& sp, double dt) {
+    PushArgs<R> a{};
+    a.sp = view<R>(sp);
+    a.m = mesh_view(c);
+    a.b = b_view(c);
+    a.E32 = c.E32.ptr;
+    a.E64 = c.E64.ptr;
+    a.nnodes = c.nnodes;
+    const double qm = sp.charge / sp.mass;
+    a.half_qm_dt = static_cast<R>(0.5 * qm * dt);
+    a.qm_dt = static_cast<R>(qm * dt);
+    a.dtd0 = static_cast<R>(dt / c.d[0]);
+    a.dtd1 = static_cast<R>(dt / c.d[1]);
+    a.dtd2 = static_cast<R>(dt / c.d[2]);
+    a.acc = reinterpret_cast<long long*>(sp.acc.ptr);
+    a.absorbed = sp.absorbed.ptr;
+    a.absorbed_energy = sp.absorbed_energy.ptr;
+    a.half_mw = 0.5 * sp.mass * sp.weight;
+    a.err = c.dev_error.ptr;
+    return a;
+}
+
+int launch_blocks(const Context& c, int64_t n) {
+    int64_t g = (n + kBlock - 1) / kBlock;
+    const int64_t cap = static_cast<int64_t>(c.cu_count) * 32;
+    if (g > cap) g = cap;
+    return static_cast<int>(g < 1 ? 1 : g);
+}
+
+template <int DIM, typename R, int BMODE>
+void launch_push_b(Context& c, Species& sp, double dt, bool deposit) {
+    auto a = push_args<R>(c, sp, dt);
+    const int g = launch_blocks(c, sp.slots);
+    if (deposit) {
+        KernelScope ks(c, "push_deposit");
+        k_push<DIM, R, BMODE, true><<<g, kBlock, 0, c.stream>>>(a);
+    } else {
+        KernelScope ks(c, "push");
+        k_push<DIM, R, BMODE, false><<<g, kBlock, 0, c.stream>>>(a);
+    }
+    AUR_HIP(hipGetLastError());
+}
+
+template <int DIM, typename R>
+void launch_push(Context& c, Species& sp, double dt, bool deposit) {
+    const int mode = b_view(c).mode;
+    if (mode == 0) launch_push_b<DIM, R, 0>(c, sp, dt, deposit);
+    else if (mode == 1) launch_push_b<DIM, R, 1>(c, sp, dt, deposit);
+    else launch_push_b<DIM, R, 2>(c, sp, dt, deposit);
+}
+
+template <typename F>
+void dispatch_dim_prec(Context& c, Species& sp, F&& f) {
+    if (sp.precision == 64) {
+        if (c.dim == 1) f.template operator()<1, double>();
+        else if (c.dim == 2) f.template operator()<2, double>();
+        else f.template operator()<3, double>();
+    } else {
+        if (c.dim == 1) f.template operator()<1, float>();
+        else if (c.dim == 2) f.template operator()<2, float>();
+        else f.template operator()<3, float>();
+    }
+}
+
+template <int DIM, typename R, int BMODE>
+void launch_half_b(Context& c, Species& sp, double dt, double sign, double* vout, int write_back,
+                   double* partial, int grid) {
+    auto a = push_args<R>(c, sp, dt);
+    const double qm = sp.charge / sp.mass;
+    const R sign_half = static_cast<R>(sign * 0.5 * qm * 0.5 * dt);
+    const R sign_qm_dt = static_cast<R>(sign * 0.5 * qm * dt);
+    KernelScope ks(c, "half_kick");
+    k_half_kick<DIM, R, BMODE><<<grid, kBlock, 0, c.stream>>>(
+        a, sign_half, sign_qm_dt, vout, write_back, 0.5 * sp.mass * sp.weight, partial);
+    AUR_HIP(hipGetLastError());
+}
+
+template <int DIM, typename R>
+void launch_half(Context& c, Species& sp, double dt, double sign, double* vout, int write_back,
+                 double* partial, int grid) {
+    const int mode = b_view(c).mode;
+    if (mode == 0) launch_half_b<DIM, R, 0>(c, sp, dt, sign, vout, write_back, partial, grid);
+    else if (mode == 1) launch_half_b<DIM, R, 1>(c, sp, dt, sign, vout, write_back, partial, grid);
+    else launch_half_b<DIM, R, 2>(c, sp, dt, sign, vout, write_back, partial, grid);
+}
+
+}  // namespace
+
+void set_b_profile_device(Context& c) {
+    if (!c.has_profile || c.baxis >= c.dim) return;
+    // per-cell hint: first table row with coordinate > cell start
+    std::vector<double> coords(static_cast<size_t>(c.brows));
+    AUR_HIP(hipMemcpy(coords.data(), c.bcoords.ptr, c.brows * sizeof(double),
+                      hipMemcpyDeviceToHost));
+    const int64_t cells = c.n[c.baxis];
+    std::vector<int32_t> hint(static_cast<size_t>(cells + 1));
+    for (int64_t i = 0; i <= cells; ++i) {
+        const double x0 = static_cast<double>(i) * c.d[c.baxis];
+        int64_t r = 0;
+        while (r < c.brows && coords[r] <= x0) ++r;
+        hint[i] = static_cast<int32_t>(r);
+    }
+    c.bhint.alloc(cells + 1);
+    AUR_HIP(hipMemcpy(c.bhint.ptr, hint.data(), hint.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+}
+
+void deposit_all(Context& c) {
+    const MeshView m = mesh_view(c);
+    for (auto& sp : c.species) {
+        {
+            KernelScope ks(c, "clear_charge");
+            k_fill_i64<<<launch_blocks(c, c.nnodes), kBlock, 0, c.stream>>>(
+                reinterpret_cast<long long*>(sp.acc.ptr), c.nnodes);
+        }
+        if (sp.slots == 0) continue;
+        KernelScope ks(c, "deposit");
+        const int g = launch_blocks(c, sp.slots);
+        auto acc = reinterpret_cast<long long*>(sp.acc.ptr);
+        dispatch_dim_prec(c, sp, [&]<int DIM, typename R>() {
+            k_deposit<DIM, R><<<g, kBlock, 0, c.stream>>>(view<R>(sp), m, acc);
+        });
+        AUR_HIP(hipGetLastError());
+    }
+}
+
+void convert_rho(Context& c) {
+    AUR_REQUIRE(c.species.size() <= static_cast<size_t>(kMaxChannels), AURORA_ERR_UNSUPPORTED,
+                "at most 8 species per context");
+    RhoArgs a{};
+    a.nspecies = static_cast<int>(c.species.size());
+    for (int s = 0; s < a.nspecies; ++s) {
+        a.acc[s] = reinterpret_cast<const long long*>(c.species[s].acc.ptr);
+        const double w = c.dim == 2 ? c.species[s].weight / c.depth : c.species[s].weight;
+        a.cw[s] = c.species[s].charge * w * 0x1p-40;
+    }
+    a.nx = static_cast<int>(c.n[0]);
+    a.ny = static_cast<int>(c.n[1]);
+    a.nz = static_cast<int>(c.n[2]);
+    a.periodic = c.bc == AURORA_PERIODIC;
+    a.dx = c.d[0];
+    a.dy = c.d[1];
+    a.dz = c.d[2];
+    a.dim = c.dim;
+    a.vol = c.dim == 1 ? c.d[0] : (c.dim == 2 ? c.d[0] * c.d[1] : c.d[0] * c.d[1] * c.d[2]);
+    KernelScope ks(c, "rho");
+    k_rho<<<launch_blocks(c, c.nnodes), kBlock, 0, c.stream>>>(a, c.rho.ptr, c.nnodes);
+    AUR_HIP(hipGetLastError());
+}
+
+void push_all(Context& c, double dt, bool fuse_deposit) {
+    if (fuse_deposit) {
+        for (auto& sp : c.species) {
+            KernelScope ks(c, "clear_charge");
+            k_fill_i64<<<launch_blocks(c, c.nnodes), kBlock, 0, c.stream>>>(
+                reinterpret_cast<long long*>(sp.acc.ptr), c.nnodes);
+        }
+    }
+    for (auto& sp : c.species) {
+        if (sp.slots == 0) continue;
+        if (c.dim == 1 && !sp.absorbed_energy.ptr) {
+            sp.absorbed_energy.alloc(2);
+            AUR_HIP(hipMemset(sp.absorbed_energy.ptr, 0, 2 * sizeof(double)));
+        }
+        dispatch_dim_prec(c, sp, [&]<int DIM, typename R>() {
+            launch_push<DIM, R>(c, sp, dt, fuse_deposit);
+        });
+    }
+}
+
+void init_half_all(Context& c, double dt) {
+    for (auto& sp : c.species) {
+        if (sp.slots == 0) continue;
+        const int g = launch_blocks(c, sp.slots);
+        dispatch_dim_prec(c, sp, [&]<int DIM, typename R>() {
+            launch_half<DIM, R>(c, sp, dt, -1.0, nullptr, 1, nullptr, g);
+        });
+    }
+}
+
+}  // namespace aurora
+
+using namespace aurora;
+
+extern "C" {
+
+int aurora_hip_species_init_maxwellian(aurora_hip_ctx* h, int sid, int64_t count,
+                                       double thermal_velocity, const double* drift3,
+                                       uint64_t seed) {
+    return guarded(h, [&](Context& c) {
+        AUR_REQUIRE(sid >= 0 && sid < static_cast<int>(c.species.size()), AURORA_ERR_NOT_FOUND,
+                    "unknown species id");
+        Species& sp = c.species[sid];
+        AUR_REQUIRE(count >= 0 && count <= sp.capacity, AURORA_ERR_CAPACITY,
+                    "particle count exceeds max_particles_per_species");
+        AUR_REQUIRE(std::isfinite(thermal_velocity) && thermal_velocity >= 0.0,
+                    AURORA_ERR_INVALID_ARGUMENT, "thermal velocity must be finite and >= 0");
+        const double dr[3] = {drift3 ? drift3[0] : 0.0, drift3 ? drift3[1] : 0.0,
+                              drift3 ? drift3[2] : 0.0};
+        sp.slots = count;
+        sp.next_uid = static_cast<uint32_t>(count);
+        if (count == 0) return;
+        const MeshView m = mesh_view(c);
+        const int g = launch_blocks(c, count);
+        dispatch_dim_prec(c, sp, [&]<int DIM, typename R>() {
+            k_init_maxwellian<DIM, R><<<g, kBlock, 0, c.stream>>>(
+                view<R>(sp), m, sp.uid.ptr, count, thermal_velocity, dr[0], dr[1], dr[2], seed);
+        });
+        AUR_HIP(hipGetLastError());
+        AUR_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int aurora_hip_species_velocity(aurora_hip_ctx* h, int sid, double dt, int64_t capacity,
+                                double* velocity) {
+    return guarded(h, [&](Context& c) {
+        AUR_REQUIRE(sid >= 0 && sid < static_cast<int>(c.species.size()), AURORA_ERR_NOT_FOUND,
+                    "unknown species id");
+        Species& sp = c.species[sid];
+        AUR_REQUIRE(capacity >= sp.slots && velocity, AURORA_ERR_INVALID_ARGUMENT,
+                    "output arrays are smaller than the species slot count");
+        if (sp.slots == 0) return;
+        DevBuf<double> out;
+        out.alloc(sp.slots * 3);
+        const int g = launch_blocks(c, sp.slots);
+        dispatch_dim_prec(c, sp, [&]<int DIM, typename R>() {
+            launch_half<DIM, R>(c, sp, dt, 1.0, out.ptr, 0, nullptr, g);
+        });
+        AUR_HIP(hipStreamSynchronize(c.stream));
+        check_device_error(c);
+        AUR_HIP(hipMemcpy(velocity, out.ptr, sp.slots * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int aurora_hip_energy(aurora_hip_ctx* h, double dt, double* kinetic, double* field_energy,
+                      double* charge_l1, int64_t* live) {
+    return guarded(h, [&](Context& c) {
+        AUR_REQUIRE(c.dim > 0, AURORA_ERR_INVALID_ARGUMENT, "bind a mesh first");
+        int64_t nlive = 0;
+        for (size_t s = 0; s < c.species.size(); ++s) {
+            Species& sp = c.species[s];
+            double ke = 0.0;
+            if (sp.slots > 0) {
+                const int g = launch_blocks(c, sp.slots);
+                DevBuf<double> part;
+                part.alloc(g);
+                dispatch_dim_prec(c, sp, [&]<int DIM, typename R>() {
+                    launch_half<DIM, R>(c, sp, dt, 1.0, nullptr, 0, part.ptr, g);
+                });
+                std::vector<double> hp(g);
+                AUR_HIP(hipStreamSynchronize(c.stream));
+                AUR_HIP(hipMemcpy(hp.data(), part.ptr, g * sizeof(double), hipMemcpyDeviceToHost));
+                for (double v : hp) ke += v;
+                std::vector<int32_t> cell(static_cast<size_t>(sp.slots));
+                AUR_HIP(hipMemcpy(cell.data(), sp.cell.ptr, sp.slots * sizeof(int32_t),
+                                  hipMemcpyDeviceToHost));
+                for (auto v : cell) nlive += v >= 0;
+            }
+            if (kinetic) kinetic[s] = ke;
+        }
+        check_device_error(c);
+        if (field_energy || charge_l1) {
+            // FP64 node sums in node order (Simulation3D.cpp:564-576)
+            std::vector<double> rho(c.nnodes), E(c.nnodes * c.dim);
+            AUR_HIP(hipMemcpy(rho.data(), c.rho.ptr, c.nnodes * sizeof(double),
+                              hipMemcpyDeviceToHost));
+            AUR_HIP(hipMemcpy(E.data(), c.E64.ptr, c.nnodes * c.dim * sizeof(double),
+                              hipMemcpyDeviceToHost));
+            double fe = 0.0, l1 = 0.0;
+            for (int64_t i = 0; i < c.nnodes; ++i) {
+                double e2 = 0.0;
+                for (int a = 0; a < c.dim; ++a) e2 += E[a * c.nnodes + i] * E[a * c.nnodes + i];
+                double vol = 1.0;
+                int64_t rem = i;
+                for (int a = 0; a < c.dim; ++a) {
+                    const int64_t k = rem % c.n[a];
+                    rem /= c.n[a];
+                    const double w = (c.bc == AURORA_PERIODIC || (k != 0 && k + 1 != c.n[a]))
+                                         ? c.d[a] : 0.5 * c.d[a];
+                    vol *= w;
+                }
+                fe += 0.5 * c.eps * e2 * vol;
+                l1 += std::abs(rho[i]) * vol;
+            }
+            if (field_energy) *field_energy = fe;
+            if (charge_l1) *charge_l1 = l1;
+        }
+        if (live) *live = nlive;
+    });
+}
+
+}  // extern "C"

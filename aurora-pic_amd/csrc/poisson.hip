@@ -1,0 +1,437 @@
+// poisson.hip - FP64 field solves (FieldSolver.cpp:384-759):
+//   1D periodic   direct spectral sum with continuous k (407-440)
+//   1D Dirichlet  Thomas algorithm, same recurrence (733-759)
+//   2D/3D periodic FFT Poisson with spectral E (442-499, 659-731)
+// The 2D/3D solve runs one forward complex transform of rho and two
+// inverse transforms: (Ex + i Ey) and (Ez + i phi).  Each spectral field is
+// first projected onto its Hermitian part (zeroing the own-axis Nyquist
+// plane of E_a), which leaves its real inverse - the quantity the reference
+// keeps - unchanged and lets two real fields share one complex inverse.
+#include <cmath>
+#include <vector>
+
+#include "aurora_internal.hpp"
+
+namespace aurora {
+
+namespace {
+
+constexpr int kFftLds = 2048;   // complex elements per block (32 KiB)
+
+struct AxisPlan {
+    int n;
+    int64_t elem_stride;
+    int64_t inner_count;
+    int64_t outer_stride;
+    int64_t nlines;
+    int pow2;
+    int log2n;
+};
+
+__device__ __forceinline__ int bitrev(int x, int bits) {
+    return static_cast<int>(__brev(static_cast<unsigned>(x)) >> (32 - bits));
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Batched 1D transforms of B lines of one axis per block, staged in LDS.
+// Forward: X[k] = sum x[j] e^{-2 pi i jk/n}; inverse: conjugate twiddles and
+// a 1/n scale (FieldSolver.cpp:171-214).
+template <bool POW2>
+__global__ void __launch_bounds__(256) k_fft_axis(double2* data, AxisPlan p, int lines_per_block,
+                                                  const double2* tw, int inverse) {
+    __shared__ double2 lds[kFftLds];
+    __shared__ double2 tmp[POW2 ? 1 : kFftLds];
+    const int n = p.n;
+    const int B = lines_per_block;
+    const int64_t line0 = static_cast<int64_t>(blockIdx.x) * B;
+    const int total = B * n;
+    const bool contiguous = p.elem_stride == 1;
+    // load (bit-reversed slot for the radix-2 path)
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        int b, e;
+        if (contiguous) { b = t / n; e = t % n; } else { b = t % B; e = t / B; }
+        const int64_t line = line0 + b;
+        double2 v = make_double2(0.0, 0.0);
+        if (line < p.nlines) {
+            const int64_t start = (line / p.inner_count) * p.outer_stride + (line % p.inner_count);
+            v = data[start + e * p.elem_stride];
+        }
+        const int slot = POW2 ? bitrev(e, p.log2n) : e;
+        lds[b * n + slot] = v;
+    }
+    __syncthreads();
+    if constexpr (POW2) {
+        for (int half = 1; half < n; half <<= 1) {
+            const int tstride = n / (2 * half);
+            for (int t = threadIdx.x; t < total / 2; t += blockDim.x) {
+                const int b = t / (n / 2);
+                const int q = t % (n / 2);
+                const int grp = q / half, pos = q % half;
+                const int i0 = b * n + grp * 2 * half + pos;
+                const int i1 = i0 + half;
+                double2 w = tw[pos * tstride];
+                if (inverse) w.y = -w.y;
+                const double2 u = lds[i0];
+                const double2 x = cmul(lds[i1], w);
+                lds[i0] = make_double2(u.x + x.x, u.y + x.y);
+                lds[i1] = make_double2(u.x - x.x, u.y - x.y);
+            }
+            __syncthreads();
+        }
+    } else {
+        for (int t = threadIdx.x; t < total; t += blockDim.x) {
+            const int b = t / n, k = t % n;
+            double sr = 0.0, si = 0.0;
+            for (int j = 0; j < n; ++j) {
+                double2 w = tw[(static_cast<int64_t>(j) * k) % n];
+                if (inverse) w.y = -w.y;
+                const double2 x = lds[b * n + j];
+                sr += x.x * w.x - x.y * w.y;
+                si += x.x * w.y + x.y * w.x;
+            }
+            tmp[t] = make_double2(sr, si);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < total; t += blockDim.x) lds[t] = tmp[t];
+        __syncthreads();
+    }
+    const double scale = inverse ? 1.0 / static_cast<double>(n) : 1.0;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        int b, e;
+        if (contiguous) { b = t / n; e = t % n; } else { b = t % B; e = t / B; }
+        const int64_t line = line0 + b;
+        if (line >= p.nlines) continue;
+        const int64_t start = (line / p.inner_count) * p.outer_stride + (line % p.inner_count);
+        double2 v = lds[b * n + e];
+        v.x *= scale;
+        v.y *= scale;
+        data[start + e * p.elem_stride] = v;
+    }
+}
+
+__global__ void k_load_real(const double* rho, double2* out, int64_t n) {
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        out[i] = make_double2(rho[i], 0.0);
+}
+
+struct GreenArgs {
+    int nx, ny, nz, dim;
+    double twopi_over_L[3];
+    double eps;
+    int want_phi;
+};
+
+__device__ __forceinline__ double mode_of(int idx, int n) {
+    // Nyquist index counts as +n/2 (FieldSolver.cpp:421, 471, 694)
+    return idx <= n / 2 ? static_cast<double>(idx) : -static_cast<double>(n - idx);
+}
+
+// fa: rho_hat in, (Ex + i Ey)_hat out; fb: (Ez + i phi)_hat out
+__global__ void k_green(GreenArgs g, double2* fa, double2* fb, int64_t total) {
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int ix = static_cast<int>(i % g.nx);
+        const int iy = static_cast<int>((i / g.nx) % g.ny);
+        const int iz = static_cast<int>(i / (static_cast<int64_t>(g.nx) * g.ny));
+        if (ix == 0 && iy == 0 && iz == 0) {
+            fa[i] = make_double2(0.0, 0.0);
+            if (fb) fb[i] = make_double2(0.0, 0.0);
+            continue;
+        }
+        const double kx = g.twopi_over_L[0] * mode_of(ix, g.nx);
+        const double ky = g.dim > 1 ? g.twopi_over_L[1] * mode_of(iy, g.ny) : 0.0;
+        const double kz = g.dim > 2 ? g.twopi_over_L[2] * mode_of(iz, g.nz) : 0.0;
+        double k2 = kx * kx + ky * ky;
+        if (g.dim > 2) k2 += kz * kz;
+        const double2 r = fa[i];
+        const double den = g.eps * k2;
+        const double pr = r.x / den, pi = r.y / den;
+        // E_a = -i k_a phi (FieldSolver.cpp:486-488, 717-720), own-axis Nyquist
+        // plane removed (Hermitian part)
+        const bool nyx = (g.nx % 2 == 0) && ix == g.nx / 2;
+        const bool nyy = (g.ny % 2 == 0) && iy == g.ny / 2;
+        const bool nyz = (g.nz % 2 == 0) && iz == g.nz / 2;
+        const double exr = nyx ? 0.0 : kx * pi, exi = nyx ? 0.0 : -kx * pr;
+        const double eyr = nyy ? 0.0 : ky * pi, eyi = nyy ? 0.0 : -ky * pr;
+        fa[i] = make_double2(exr - eyi, exi + eyr);
+        if (fb) {
+            double ezr = 0.0, ezi = 0.0;
+            if (g.dim > 2 && !nyz) { ezr = kz * pi; ezi = -kz * pr; }
+            if (g.want_phi) fb[i] = make_double2(ezr - pi, ezi + pr);
+            else fb[i] = make_double2(ezr, ezi);
+        }
+    }
+}
+
+__global__ void k_unpack(const double2* fa, const double2* fb, double* E64, double* phi,
+                         float* E32, int dim, int want_phi, int64_t nn) {
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < nn;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double2 a = fa[i];
+        E64[i] = a.x;
+        E64[nn + i] = a.y;
+        if (dim == 3) {
+            const double2 b = fb[i];
+            E64[2 * nn + i] = b.x;
+            if (want_phi) phi[i] = b.y;
+            reinterpret_cast<float4*>(E32)[i] = make_float4(static_cast<float>(a.x),
+                                                            static_cast<float>(a.y),
+                                                            static_cast<float>(b.x), 0.0f);
+        } else {
+            if (want_phi) phi[i] = fb[i].y;
+            reinterpret_cast<float2*>(E32)[i] =
+                make_float2(static_cast<float>(a.x), static_cast<float>(a.y));
+        }
+    }
+}
+
+// 1D periodic: direct spectral sum (FieldSolver.cpp:407-440) in one block.
+__global__ void __launch_bounds__(1024) k_spectral_1d(const double* rho, double* phi, double* E,
+                                                      float* E32, const double* cs,
+                                                      const double* sn, int n, double L,
+                                                      double eps) {
+#pragma clang fp contract(off)
+    extern __shared__ double sh[];
+    double* r = sh;              // n
+    double* pr = sh + n;         // phi_k re
+    double* pim = sh + 2 * n;    // phi_k im
+    __shared__ double mean_s;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) r[j] = rho[j];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int j = 0; j < n; ++j) m += r[j];
+        mean_s = m / static_cast<double>(n);
+    }
+    __syncthreads();
+    const double mean = mean_s;
+    const double twopi = 6.283185307179586476925286766559;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        if (k == 0) { pr[0] = 0.0; pim[0] = 0.0; continue; }
+        double rr = 0.0, ri = 0.0;
+        int m = 0;
+        for (int j = 0; j < n; ++j) {
+            const double x = r[j] - mean;
+            rr += x * cs[m];
+            ri -= x * sn[m];
+            m += k;
+            if (m >= n) m -= n;
+        }
+        rr /= static_cast<double>(n);
+        ri /= static_cast<double>(n);
+        const double kk = (k <= n / 2) ? static_cast<double>(k) : -static_cast<double>(n - k);
+        const double wave = twopi * kk / L;
+        const double den = eps * wave * wave;
+        pr[k] = rr / den;
+        pim[k] = ri / den;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        double ph = 0.0, e = 0.0;
+        int m = j;
+        for (int k = 1; k < n; ++k) {
+            const double c = cs[m], s = sn[m];
+            const double kk = (k <= n / 2) ? static_cast<double>(k) : -static_cast<double>(n - k);
+            const double wave = twopi * kk / L;
+            ph += pr[k] * c - pim[k] * s;
+            // E_k = -i wave phi_k = (wave pim, -wave pr)
+            e += (wave * pim[k]) * c - (-wave * pr[k]) * s;
+            m += j;
+            if (m >= n) m -= n;
+        }
+        phi[j] = ph;
+        E[j] = e;
+        E32[j] = static_cast<float>(e);
+    }
+}
+
+// 1D Dirichlet Thomas solve (FieldSolver.cpp:733-759): the coefficients
+// cp and denominators depend only on the index and are precomputed on the
+// host with the reference's operations; the data recurrence runs in one
+// lane, then the field is differenced in parallel.
+__global__ void __launch_bounds__(256) k_thomas_1d(const double* rho, double* phi, double* E,
+                                                   float* E32, const double* cp,
+                                                   const double* denom, double* dp, int n,
+                                                   double dx, double eps, double phi_left,
+                                                   double phi_right) {
+#pragma clang fp contract(off)
+    if (threadIdx.x == 0) {
+        const int m = n - 2;
+        const double dx2 = dx * dx;
+        phi[0] = phi_left;
+        phi[n - 1] = phi_right;
+        double prev = 0.0;
+        for (int i = 0; i < m; ++i) {
+            double di = rho[i + 1] * dx2 / eps;
+            if (i == 0) di += phi_left;
+            if (i == m - 1) di += phi_right;
+            prev = (i == 0) ? di / 2.0 : (di + prev) / denom[i];
+            dp[i] = prev;
+        }
+        double nxt = dp[m - 1];
+        phi[n - 2] = nxt;
+        for (int ii = m - 1; ii-- > 0;) {
+            nxt = dp[ii] - cp[ii] * nxt;
+            phi[ii + 1] = nxt;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double e;
+        if (i == 0) e = -(phi[1] - phi[0]) / dx;
+        else if (i == n - 1) e = -(phi[n - 1] - phi[n - 2]) / dx;
+        else e = -(phi[i + 1] - phi[i - 1]) / (2.0 * dx);
+        E[i] = e;
+        E32[i] = static_cast<float>(e);
+    }
+}
+
+AxisPlan plan_axis(const Context& c, int axis) {
+    AxisPlan p{};
+    p.n = static_cast<int>(c.n[axis]);
+    p.pow2 = (p.n & (p.n - 1)) == 0;
+    p.log2n = 0;
+    while ((1 << p.log2n) < p.n) ++p.log2n;
+    const int64_t nx = c.n[0], ny = c.n[1], nz = c.n[2];
+    if (axis == 0) {
+        p.elem_stride = 1; p.inner_count = 1; p.outer_stride = nx; p.nlines = ny * nz;
+    } else if (axis == 1) {
+        p.elem_stride = nx; p.inner_count = nx; p.outer_stride = nx * ny; p.nlines = nx * nz;
+    } else {
+        p.elem_stride = nx * ny; p.inner_count = nx * ny; p.outer_stride = 0; p.nlines = nx * ny;
+    }
+    return p;
+}
+
+void run_axis(Context& c, double2* data, int axis, int inverse) {
+    const AxisPlan p = plan_axis(c, axis);
+    int B = kFftLds / p.n;
+    if (B > 64) B = 64;
+    const int64_t blocks = (p.nlines + B - 1) / B;
+    KernelScope ks(c, "fft_axis");
+    if (p.pow2)
+        k_fft_axis<true><<<static_cast<unsigned>(blocks), 256, 0, c.stream>>>(
+            data, p, B, c.tw[axis].ptr, inverse);
+    else
+        k_fft_axis<false><<<static_cast<unsigned>(blocks), 256, 0, c.stream>>>(
+            data, p, B, c.tw[axis].ptr, inverse);
+    AUR_HIP(hipGetLastError());
+}
+
+int blocks_for(const Context& c, int64_t n) {
+    int64_t g = (n + 255) / 256;
+    const int64_t cap = static_cast<int64_t>(c.cu_count) * 16;
+    return static_cast<int>(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+}  // namespace
+
+void prepare_solver(Context& c) {
+    const double twopi = 6.283185307179586476925286766559;
+    if (c.dim == 1 && c.bc == AURORA_PERIODIC) {
+        const int n = static_cast<int>(c.n[0]);
+        AUR_REQUIRE(n <= 2048, AURORA_ERR_UNSUPPORTED,
+                    "1D periodic spectral solve supports at most 2048 nodes");
+        std::vector<double> cs(n), sn(n);
+        for (int m = 0; m < n; ++m) {
+            const double a = twopi * static_cast<double>(m) / static_cast<double>(n);
+            cs[m] = std::cos(a);
+            sn[m] = std::sin(a);
+        }
+        c.dft_cos.alloc(n);
+        c.dft_sin.alloc(n);
+        AUR_HIP(hipMemcpy(c.dft_cos.ptr, cs.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        AUR_HIP(hipMemcpy(c.dft_sin.ptr, sn.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    } else if (c.dim == 1) {
+        const int n = static_cast<int>(c.n[0]);
+        const int m = n - 2;
+        std::vector<double> cp(m), denom(m, 2.0);
+        cp[0] = -1.0 / 2.0;
+        for (int i = 1; i < m; ++i) {
+            denom[i] = 2.0 + cp[i - 1];
+            cp[i] = (i == m - 1) ? 0.0 : -1.0 / denom[i];
+        }
+        c.thomas_cp.alloc(m);
+        c.thomas_inv.alloc(2 * m);  // denominators + dp scratch
+        AUR_HIP(hipMemcpy(c.thomas_cp.ptr, cp.data(), m * sizeof(double), hipMemcpyHostToDevice));
+        AUR_HIP(hipMemcpy(c.thomas_inv.ptr, denom.data(), m * sizeof(double),
+                          hipMemcpyHostToDevice));
+    } else {
+        for (int a = 0; a < c.dim; ++a) {
+            const int n = static_cast<int>(c.n[a]);
+            AUR_REQUIRE(n <= kFftLds, AURORA_ERR_UNSUPPORTED,
+                        "periodic FFT axes longer than 2048 nodes are not supported");
+            const bool pow2 = (n & (n - 1)) == 0;
+            AUR_REQUIRE(pow2 || n <= 1024, AURORA_ERR_UNSUPPORTED,
+                        "non power-of-two FFT axes longer than 1024 are not supported");
+            std::vector<double2> tw(n);
+            for (int k = 0; k < n; ++k) {
+                const double ang = twopi * static_cast<double>(k) / static_cast<double>(n);
+                tw[k] = make_double2(std::cos(ang), -std::sin(ang));
+            }
+            c.tw[a].alloc(n);
+            AUR_HIP(hipMemcpy(c.tw[a].ptr, tw.data(), n * sizeof(double2), hipMemcpyHostToDevice));
+        }
+        c.fa.alloc(c.nnodes);
+        c.fb.alloc(c.nnodes);
+    }
+}
+
+void solve_fields(Context& c, double phi_left, double phi_right, bool want_phi) {
+    if (c.dim == 1) {
+        const int n = static_cast<int>(c.n[0]);
+        if (c.bc == AURORA_PERIODIC) {
+            KernelScope ks(c, "solve_spectral_1d");
+            const size_t shmem = 3 * static_cast<size_t>(n) * sizeof(double);
+            k_spectral_1d<<<1, 1024, shmem, c.stream>>>(c.rho.ptr, c.phi.ptr, c.E64.ptr, c.E32.ptr,
+                                                        c.dft_cos.ptr, c.dft_sin.ptr, n, c.L[0],
+                                                        c.eps);
+        } else {
+            KernelScope ks(c, "solve_thomas_1d");
+            const int m = n - 2;
+            k_thomas_1d<<<1, 256, 0, c.stream>>>(c.rho.ptr, c.phi.ptr, c.E64.ptr, c.E32.ptr,
+                                                 c.thomas_cp.ptr, c.thomas_inv.ptr,
+                                                 c.thomas_inv.ptr + m, n, c.d[0], c.eps,
+                                                 phi_left, phi_right);
+        }
+        AUR_HIP(hipGetLastError());
+        c.phi_valid = true;
+        return;
+    }
+    const int64_t nn = c.nnodes;
+    const int g = blocks_for(c, nn);
+    {
+        KernelScope ks(c, "fft_load");
+        k_load_real<<<g, 256, 0, c.stream>>>(c.rho.ptr, c.fa.ptr, nn);
+    }
+    for (int a = 0; a < c.dim; ++a) run_axis(c, c.fa.ptr, a, 0);
+    GreenArgs ga{};
+    ga.nx = static_cast<int>(c.n[0]);
+    ga.ny = static_cast<int>(c.n[1]);
+    ga.nz = static_cast<int>(c.n[2]);
+    ga.dim = c.dim;
+    for (int a = 0; a < 3; ++a) ga.twopi_over_L[a] = 6.283185307179586476925286766559 / c.L[a];
+    ga.eps = c.eps;
+    ga.want_phi = want_phi ? 1 : 0;
+    const bool use_b = c.dim == 3 || want_phi;
+    {
+        KernelScope ks(c, "green");
+        k_green<<<g, 256, 0, c.stream>>>(ga, c.fa.ptr, use_b ? c.fb.ptr : nullptr, nn);
+    }
+    for (int a = c.dim - 1; a >= 0; --a) run_axis(c, c.fa.ptr, a, 1);
+    if (use_b)
+        for (int a = c.dim - 1; a >= 0; --a) run_axis(c, c.fb.ptr, a, 1);
+    {
+        KernelScope ks(c, "fft_unpack");
+        k_unpack<<<g, 256, 0, c.stream>>>(c.fa.ptr, c.fb.ptr, c.E64.ptr, c.phi.ptr, c.E32.ptr,
+                                          c.dim, want_phi ? 1 : 0, nn);
+    }
+    AUR_HIP(hipGetLastError());
+    c.phi_valid = want_phi;
+}
+
+}  // namespace aurora

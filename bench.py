@@ -1,0 +1,188 @@
+"""Benchmark: particle-updates/s of the 3D3V periodic plasma with tabulated B
+(128^3 cells, 16 electrons + 16 ions per cell = 67,108,864 particles, cell
+sort every 20 steps) on one MI355X, with the reference CPU path timed on a
+bounded sample beside it.
+
+One step = gather + Boris push (tabulated B_y(x)) + drift + periodic
+boundary + fixed-point deposit for every particle, FP64 FFT Poisson solve.
+Inputs are generated on the device and resident in HBM before timing.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "aurora-pic_amd"))
+
+# bytes per particle-update of the dominant kernel: read + write of the
+# 28-byte SoA state (int32 cell, 3 x FP32 offset, 3 x FP32 velocity)
+BYTES_PER_UPDATE = 56
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(cfg, ppc_sample: int):
+    """Reference Simulation3D::step() (compiled from the reference sources into
+    oracle/_ref) on the same mesh with fewer particles per cell, one step."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_py
+    from aurora_pic import workloads as wl
+
+    cells = cfg["n"][0]
+    sample = wl.config_plasma_3d(cells=cells, ppc=ppc_sample)
+    sp = []
+    for s in sample["species"]:
+        sp += [s["charge"], s["mass"], s["weight"], s["particles"],
+               wl.maxwellian_sigma(s["temperature_ev"], s["mass"])]
+    ax, bx, bv = sample["btable"]
+    desc = (f"reference Simulation3D::step() (oracle/_ref, -O3, 1 thread) on the "
+            f"{cells}^3 mesh with {ppc_sample}+{ppc_sample} particles per cell "
+            f"({2 * sample['species'][0]['particles']} particles), tabulated B_y(x), 1 step "
+            f"after initialize(); host {cpu_model()}")
+    if oracle_py.reference_available():
+        ref = oracle_py.Reference()
+        sec, live = ref.sim3d_time(sample["n"], sample["L"], sample["dt"], 1.0, 1, sp,
+                                   (bx, bv), 1)
+        return {"value": live / sec, "unit": "particle-updates/s", "cores": 1,
+                "kind": "reference", "sample": desc}
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cells", type=int, default=128)
+    ap.add_argument("--ppc", type=int, default=16)
+    ap.add_argument("--timing-steps", type=int, default=20)
+    ap.add_argument("--cpu-ppc", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the per-step loop of one 3D3V instance runs on a "
+              "single MI355X; multi-GPU sharding is a design note only (see DESIGN.md).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from aurora_pic import hip
+    from aurora_pic import workloads as wl
+
+    cfg = wl.config_plasma_3d(cells=args.cells, ppc=args.ppc)
+    dt = cfg["dt"]
+    ctx = hip.Context(0)
+    log("device:", ctx.info())
+    ctx.mesh(cfg["n"], cfg["L"], 0, cfg["eps"])
+    axis, bx, bv = cfg["btable"]
+    ctx.b_profile(axis, bx, bv)
+    sids = []
+    for k, s in enumerate(cfg["species"]):
+        sid = ctx.add_species(s["charge"], s["mass"], s["weight"], s["particles"])
+        ctx.init_maxwellian(sid, s["particles"], wl.maxwellian_sigma(s["temperature_ev"],
+                                                                     s["mass"]), cfg["seed"] + k)
+        sids.append(sid)
+    total = sum(s["particles"] for s in cfg["species"])
+    ctx.sort()
+    ctx.initialize(dt)
+    ctx.sort_interval(cfg["sort_interval"])
+    log(f"particles {total}, warmup {args.warmup} steps")
+    ctx.step(dt, args.warmup)
+    ctx.sync()
+
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    ctx.step(dt, args.steps)
+    ctx.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = total * args.steps / elapsed
+    log(f"{ms_per_step:.3f} ms/step, {value:.4g} particle-updates/s")
+
+    # per-kernel HIP-event timing on the context stream
+    ctx.timing(True)
+    ctx.step(dt, args.timing_steps)
+    ctx.sync()
+    kernels = {}
+    for name in ["push_deposit", "rho", "fft_axis", "green", "fft_load", "fft_unpack",
+                 "clear_charge", "sort"]:
+        ms, n = ctx.kernel_time(name)
+        if n:
+            kernels[name] = {"avg_ms": ms / n, "launches": n}
+    ctx.timing(False)
+    pd = kernels.get("push_deposit", {"avg_ms": float("nan")})
+    per_launch_particles = total / len(sids)
+    achieved = BYTES_PER_UPDATE * per_launch_particles / (pd["avg_ms"] * 1e-3) / 1e9
+    log("kernels:", json.dumps(kernels))
+    ke, fe, l1, live = ctx.energy(dt)
+    log(f"live {live}, kinetic {ke.sum():.6e} J, field {fe:.6e} J")
+    ctx.close()
+
+    traffic = None
+    tfile = os.path.join(REPO, "profiles", "push_deposit_plasma3d_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    base = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (reference, bounded sample)...")
+        try:
+            base = cpu_baseline(cfg, args.cpu_ppc)
+        except Exception as e:  # the baseline must not hide the GPU result
+            log("cpu baseline failed:", e)
+    out = {
+        "metric": "particle-updates/s (gather+push+deposit+MCC) vs CPU; batched swarm "
+                  "p50/p99 latency",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plasma_3d3v: 128^3 cells, 16 e + 16 i per cell "
+                               "(67,108,864 particles), Te 5 eV, Ti 0.1 eV, 131 amu ions, "
+                               "tabulated B_y(x) 512 rows, FFT Poisson (FP64), sort every 20 "
+                               "steps" if args.cells == 128 and args.ppc == 16 else
+                               f"plasma_3d3v {args.cells}^3 x {args.ppc}+{args.ppc} ppc",
+                   "particles": total, "cells": args.cells ** 3, "dt_s": dt,
+                   "kernels_ms": kernels},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "push_deposit (gather+Boris+drift+wrap+int64 deposit)",
+                     "bytes_per_update": BYTES_PER_UPDATE},
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

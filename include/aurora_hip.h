@@ -1,0 +1,293 @@
+/*
+ * aurora_hip.h - C ABI of the MI355X-native AuroraPIC particle loop.
+ *
+ * The reference (AuroraPIC, C++20) has no FFI: its hot path is the call
+ * surface of Species / Pusher / NullCollisionModel / FieldSolver classes,
+ * dispatched by a RuntimePolicy whose "gpu" backend is reserved and
+ * rejected (include/pic/Runtime.hpp:9, src/Runtime.cpp:73-74).  These entry
+ * points are what that backend binds: plain pointers and sizes, FP64 at the
+ * boundary, device-resident state between calls.  Each declaration names
+ * the reference interface it replaces (paths relative to the reference
+ * repository root).
+ *
+ * Conventions
+ *  - Every function returns an aurora_status (0 = success).  On failure
+ *    aurora_hip_last_error() returns the message; messages reuse the
+ *    reference's exception text so callers can keep matching substrings.
+ *  - A context is driven by one host thread.  Work is queued on the
+ *    context's HIP stream; calls that return host data synchronise it.
+ *  - Positions are in the reference's units (metres in SI, normalised
+ *    otherwise), origin at node 0.  Velocities are 3 components per
+ *    particle (1D3V and 2D3V carry the out-of-plane components).
+ *  - Particle state on the device: SoA, integer cell index + FP32 (or FP64)
+ *    in-cell offsets + FP32 (or FP64) staggered velocity v^{n-1/2}.
+ *  - Charge density is accumulated in int64 fixed point (2^-40 of a
+ *    particle per node), so it is bit-exact and independent of particle
+ *    order, sorting and batching.
+ */
+#ifndef AURORA_HIP_H
+#define AURORA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct aurora_hip_ctx aurora_hip_ctx;
+
+typedef enum aurora_status {
+    AURORA_OK = 0,
+    AURORA_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+    AURORA_ERR_HIP = 2,              /* HIP runtime failure */
+    AURORA_ERR_RATE_ABOVE_BOUND = 3, /* Collision.cpp:617-623, 677-684 */
+    AURORA_ERR_CANDIDATE_LIMIT = 4,  /* Collision.cpp:864-870 */
+    AURORA_ERR_CAPACITY = 5,         /* Simulation.cpp:945-963 */
+    AURORA_ERR_B_OUT_OF_RANGE = 6,   /* PrescribedField.cpp:95-100 */
+    AURORA_ERR_ENERGY_ABOVE_MAX = 7, /* Swarm.cpp:1021-1044 */
+    AURORA_ERR_UNSUPPORTED = 8,      /* configuration outside this build */
+    AURORA_ERR_NOT_FOUND = 9         /* unknown species / request handle */
+} aurora_status;
+
+enum { AURORA_PERIODIC = 0, AURORA_DIRICHLET = 1 };            /* Types.hpp:31 */
+enum { AURORA_PB_ABSORB = 1, AURORA_PB_REFLECT = 2,
+       AURORA_PB_PERIODIC = 3 };                               /* Types.hpp:33 */
+enum { AURORA_VELOCITY_HALF = 0, AURORA_VELOCITY_CENTRED = 1 };
+enum { AURORA_ELASTIC = 0, AURORA_EXCITATION = 1, AURORA_IONIZATION = 2,
+       AURORA_ATTACHMENT = 3 };                                /* Config.hpp:42 */
+
+/* ---------------------------------------------------------------- context */
+/* Runtime.cpp:22-76 (RuntimePolicy with backend = gpu).  device: HIP
+ * ordinal.  The context owns a HIP stream at default priority. */
+int aurora_hip_create(int device, aurora_hip_ctx** out);
+int aurora_hip_destroy(aurora_hip_ctx* ctx);
+/* Message of the last failure on this context (empty when none). */
+const char* aurora_hip_last_error(const aurora_hip_ctx* ctx);
+int aurora_hip_synchronize(aurora_hip_ctx* ctx);
+/* Library and device description ("gfx950, 256 CUs ...") */
+int aurora_hip_device_info(aurora_hip_ctx* ctx, char* buf, size_t len);
+
+/* ------------------------------------------------------------------- mesh */
+/* Grid(nx, L, bc) (Grid.cpp:19-22), Mesh2D (Mesh2D.cpp:49-70),
+ * Mesh3D (Mesh3D.cpp:39-50) and FieldSolver(permittivity)
+ * (FieldSolver.cpp:375-382).  n: nodes per axis; length: domain extents;
+ * depth: out-of-plane depth used by the 2D deposit (Species2D.cpp:160-170),
+ * ignored otherwise.  Binding a mesh drops all species. */
+int aurora_hip_mesh_bind(aurora_hip_ctx* ctx, int dim, const int64_t* n,
+                         const double* length, int boundary,
+                         double permittivity, double depth);
+/* Particle boundary policy per face (left,right,bottom,top,back,front),
+ * ParticleBoundaryConfig3D (Simulation3D.hpp:17-24), already resolved. */
+int aurora_hip_particle_boundaries(aurora_hip_ctx* ctx, const int* policy);
+
+/* ---------------------------------------------------------------- species */
+/* Species*(cfg) (Species.hpp:11, Species3D.hpp:27).  precision: 32 or 64
+ * (offsets and velocities).  capacity: maximum particle slots, the
+ * max_particles_per_species of the reference (Config.hpp:191). */
+int aurora_hip_species_add(aurora_hip_ctx* ctx, double charge, double mass,
+                           double weight, int64_t capacity, int precision,
+                           int* species_id);
+/* Replaces the species' particles (Species*::particles() assignment /
+ * external-state ingestion, Simulation3D.cpp:270-286).  position:
+ * count*dim, velocity: count*3, alive: count (NULL = all alive).
+ * velocity_kind selects v^{n-1/2} (AURORA_VELOCITY_HALF) or the
+ * time-centred v (AURORA_VELOCITY_CENTRED, then v_half = v until
+ * aurora_hip_initialize).  Particle uids are assigned 0..count-1. */
+int aurora_hip_species_upload(aurora_hip_ctx* ctx, int species_id,
+                              int64_t count, const double* position,
+                              const double* velocity, const uint8_t* alive,
+                              int velocity_kind);
+/* Species*::initialize (Species3D.cpp:28-162) for the measured workloads:
+ * count particles uniform over the domain, Maxwellian velocities
+ * (per-component sigma thermal_velocity, plus drift3), generated on the
+ * device from a Philox4x32-10 stream keyed by seed.  Velocities are
+ * time-centred until aurora_hip_initialize. */
+int aurora_hip_species_init_maxwellian(aurora_hip_ctx* ctx, int species_id,
+                                       int64_t count, double thermal_velocity,
+                                       const double* drift3, uint64_t seed);
+/* Slots in use (live + dead) and live particles (Species::live_count,
+ * Species.cpp:353-355). */
+int aurora_hip_species_count(aurora_hip_ctx* ctx, int species_id,
+                             int64_t* slots, int64_t* live);
+/* Copies the state back in storage order: position (slots*dim),
+ * velocity_half (slots*3), alive (slots), uid (slots).  Any output may be
+ * NULL.  capacity is the number of slots the caller's arrays hold. */
+int aurora_hip_species_download(aurora_hip_ctx* ctx, int species_id,
+                                int64_t capacity, double* position,
+                                double* velocity_half, uint8_t* alive,
+                                uint32_t* uid, int64_t* slots);
+/* Time-centred velocity (synchronize_boris / synchronize_leapfrog,
+ * Pusher.cpp:172-186) from the current field, storage order, slots*3. */
+int aurora_hip_species_velocity(aurora_hip_ctx* ctx, int species_id,
+                                double dt, int64_t capacity,
+                                double* velocity);
+/* Raw device state in storage order: cell (slots; -1 = dead), offsets
+ * (slots*dim, the stored values widened exactly to double), velocity_half
+ * (slots*3).  The representation the fixed-point deposit is defined on. */
+int aurora_hip_species_download_raw(aurora_hip_ctx* ctx, int species_id,
+                                    int64_t capacity, int32_t* cell,
+                                    double* offsets, double* velocity_half,
+                                    int64_t* slots);
+/* Fixed-point charge accumulator of one species (node count int64). */
+int aurora_hip_species_charge_fixed(aurora_hip_ctx* ctx, int species_id,
+                                    int64_t* acc);
+
+/* -------------------------------------------------------------- fields */
+/* Uniform B for the Boris pusher (Simulation3DConfig::magnetic_field,
+ * Simulation3D.hpp:46).  Zero with no profile keeps the leapfrog kick. */
+int aurora_hip_set_uniform_b(aurora_hip_ctx* ctx, const double* b3);
+/* TabulatedVectorField1D(axis, coordinates, values)
+ * (PrescribedField.cpp:60-87); axis 0/1/2 = x/y/z; values rows*3. */
+int aurora_hip_set_b_profile(aurora_hip_ctx* ctx, int axis, int64_t rows,
+                             const double* coordinates, const double* values);
+/* Sinusoidal electrode drive of the 1D Dirichlet solve,
+ * Simulation::electrode_potential (Simulation.cpp:678-685): each side
+ * offset + amplitude * sin(2 pi f t + phase). */
+int aurora_hip_set_electrodes(aurora_hip_ctx* ctx, const double* left4,
+                              const double* right4);
+/* Overwrites the node electric field (E: dim arrays of node count),
+ * e.g. to drive a kernel with a prescribed field. */
+int aurora_hip_set_electric(aurora_hip_ctx* ctx, const double* E);
+/* Node charge density, potential and field (any may be NULL).  phi is
+ * only valid after a solve with want_phi (or aurora_hip_initialize). */
+int aurora_hip_get_fields(aurora_hip_ctx* ctx, double* rho, double* phi,
+                          double* E);
+/* Overwrites rho (node count) for a standalone field solve. */
+int aurora_hip_set_rho(aurora_hip_ctx* ctx, const double* rho);
+
+/* ---------------------------------------------------------- operations */
+/* Mesh::clear_charge + Species*::deposit_charge for every species
+ * (Species.cpp:161-185, Mesh2D.cpp:90-144, Mesh3D.cpp:62-126). */
+int aurora_hip_deposit(aurora_hip_ctx* ctx);
+/* FieldSolver::solve (FieldSolver.cpp:384-405): 1D periodic spectral,
+ * 1D Dirichlet tridiagonal with (phi_left, phi_right), 2D/3D periodic FFT.
+ * want_phi also produces the potential (the field always). */
+int aurora_hip_field_solve(aurora_hip_ctx* ctx, double phi_left,
+                           double phi_right, int want_phi);
+/* Gather + kick + drift + particle boundaries for every species
+ * (Simulation3D.cpp:344-352, Simulation.cpp:1025-1079). */
+int aurora_hip_push(aurora_hip_ctx* ctx, double dt);
+/* Simulation*::initialize after the particles exist
+ * (Simulation3D.cpp:288-299, Simulation.cpp:759-767): deposit, solve at
+ * t = 0, then v_half from the time-centred velocity. */
+int aurora_hip_initialize(aurora_hip_ctx* ctx, double dt);
+/* nsteps of Simulation*::step (Simulation3D.cpp:342-365,
+ * Simulation2D.cpp:1337-1364, Simulation.cpp:1008-1138): fused push and
+ * deposit, field solve at t+dt, MCC where a model is loaded.  The
+ * diagnostic synchronisation is computed on demand instead of every step
+ * (aurora_hip_species_velocity, aurora_hip_energy). */
+int aurora_hip_step(aurora_hip_ctx* ctx, double dt, int64_t nsteps);
+/* Simulation3D::sample (Simulation3D.cpp:551-579), FP64: kinetic energy
+ * per species (time-centred v), field energy, charge L1, live count. */
+int aurora_hip_energy(aurora_hip_ctx* ctx, double dt, double* kinetic,
+                      double* field_energy, double* charge_l1,
+                      int64_t* live);
+/* Boundary-loss ledger of a species since the last reset: absorbed
+ * counts per face (6) and, in 1D, absorbed kinetic energy per side (2)
+ * (BoundaryLoss1D, Simulation.hpp:20-25). */
+int aurora_hip_boundary_losses(aurora_hip_ctx* ctx, int species_id,
+                               int64_t* absorbed6, double* energy2);
+/* Simulation time and step count kept by the context. */
+int aurora_hip_time(aurora_hip_ctx* ctx, double* time, int64_t* step);
+/* Cell sort every interval steps (0 = never) and an immediate sort. */
+int aurora_hip_set_sort_interval(aurora_hip_ctx* ctx, int64_t interval);
+int aurora_hip_sort(aurora_hip_ctx* ctx);
+
+/* ------------------------------------------------------------------ MCC */
+/* NullCollisionModel(config, particle_mass) (Collision.cpp:315-577) for
+ * the target species.  Tables: per channel rows energies (J, strictly
+ * increasing) and cross sections (m^2).  product_species: for ionization
+ * the (secondary electron, ion) species ids, else -1.  neutral_temperature
+ * > 0 samples bounded-Maxwellian neutrals (Collision.cpp:332-363).
+ * seed keys the Philox stream (seed, particle uid, step). */
+int aurora_hip_mcc_load(aurora_hip_ctx* ctx, int target_species, int nchan,
+                        const int* kind, const double* threshold,
+                        const int64_t* rows, const double* const* energy,
+                        const double* const* sigma, double neutral_density,
+                        double max_frequency, int64_t max_candidates,
+                        double neutral_mass, double neutral_temperature,
+                        const int* product_species, uint64_t seed);
+/* Accumulated CollisionStepStatistics (Collision.hpp:50-63) since load:
+ * counts = [candidates, nulls, per-channel collisions (nchan)],
+ * energy_change per channel (J, weighted by the species weight). */
+int aurora_hip_mcc_stats(aurora_hip_ctx* ctx, uint64_t* counts,
+                         double* energy_change);
+/* NullCollisionModel::collide on an array of 3V velocities (in place) with
+ * the stream keyed (seed, stream_id, uid = index, step): the same kernel
+ * the step uses, exposed for per-particle parity checks.  stats as above
+ * plus secondaries at counts[2+nchan]; sec_velocity receives up to
+ * count*max_secondaries secondary velocities (may be NULL). */
+int aurora_hip_mcc_collide(aurora_hip_ctx* ctx, int64_t count,
+                           double* velocity, double dt, uint64_t step,
+                           uint32_t stream_id, uint64_t* counts,
+                           double* energy_change);
+
+/* ------------------------------------------------------------ swarms */
+/* Homogeneous electron swarm requests (run_field, Swarm.cpp:940-1250,
+ * fixed population), batched: many requests of uneven particle count
+ * advance together in persistent launches; each result is identical
+ * whether the request runs alone or mixed with others. */
+typedef struct aurora_hip_swarm_request {
+    double reduced_field_td;      /* E/N (Td) */
+    double timestep;              /* s; max_frequency * timestep <= 0.1 */
+    int64_t particles;
+    int64_t steps;
+    int64_t burn_in;
+    int64_t blocks;               /* uncertainty_blocks */
+    double initial_mean_energy_ev;
+    double max_energy_ev;
+    uint64_t seed;
+} aurora_hip_swarm_request;
+
+typedef struct aurora_hip_swarm_result {
+    double drift;                 /* m/s, -<v_x> */
+    double drift_se;
+    double mean_energy_ev;
+    double mean_energy_se;
+    double ionization_rate;       /* 1/s per electron */
+    double ionization_rate_se;
+    double rate[8];
+    double rate_se[8];
+    uint64_t collisions[8];
+    uint64_t candidates;
+    uint64_t nulls;
+    double max_energy_ev;
+    int status;                   /* aurora_status of the request */
+} aurora_hip_swarm_result;
+
+/* The gas of every swarm request on this context (GasDataset +
+ * collision_config, Swarm.cpp:561-593). */
+int aurora_hip_swarm_gas(aurora_hip_ctx* ctx, int nchan, const int* kind,
+                         const double* threshold, const int64_t* rows,
+                         const double* const* energy,
+                         const double* const* sigma, double neutral_density,
+                         double max_frequency, int64_t max_candidates,
+                         double neutral_mass);
+/* Admits a request when its device footprint fits the HBM budget
+ * (bytes; 0 = no limit); returns AURORA_ERR_CAPACITY otherwise. */
+int aurora_hip_swarm_submit(aurora_hip_ctx* ctx,
+                            const aurora_hip_swarm_request* request,
+                            uint64_t hbm_budget, int64_t* handle);
+/* Advances every admitted, unfinished request by up to max_steps steps in
+ * one launch; *active receives the number still running. */
+int aurora_hip_swarm_advance(aurora_hip_ctx* ctx, int64_t max_steps,
+                             int64_t* active);
+/* done = 1 once the request finished; result then holds its outputs and
+ * the request's device memory is released. */
+int aurora_hip_swarm_poll(aurora_hip_ctx* ctx, int64_t handle, int* done,
+                          aurora_hip_swarm_result* result);
+/* Device bytes a request of this size holds while admitted. */
+uint64_t aurora_hip_swarm_footprint(const aurora_hip_swarm_request* request);
+
+/* -------------------------------------------------------------- timing */
+/* Per-kernel HIP-event timing on the context stream (off by default). */
+int aurora_hip_kernel_timing(aurora_hip_ctx* ctx, int enable);
+/* Total milliseconds and launch count of kernel `name` since enabled. */
+int aurora_hip_kernel_time(aurora_hip_ctx* ctx, const char* name,
+                           double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AURORA_HIP_H */

@@ -1,0 +1,287 @@
+"""GPU parity of the deposit, gather + push and field solves through the
+C ABI, against the oracle (and the reference's golden vectors).
+
+Bars: charge density bit-exact against the oracle's fixed-point definition
+on the device's own (cell, offset) state; fields and one-step push within
+1e-6 relative (max-norm over the array, relative to the array's scale) of
+the FP64 oracle restatement."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["1d_periodic", "1d_dirichlet", "2d_periodic", "3d_periodic"]
+
+
+@pytest.fixture
+def hip_ctx():
+    from aurora_pic import hip
+    c = hip.Context(0)
+    yield c
+    c.close()
+
+
+def _setup_case(ctx, g, precision=32):
+    n, L, bc = g["n"], g["L"], int(g["bc"])
+    ctx.mesh(n, L, bc, 1.0)
+    sid = ctx.add_species(float(g["charge"]), 1.0, float(g["weight"]), len(g["alive"]) + 16,
+                          precision)
+    vel = np.zeros((len(g["alive"]), 3))
+    ctx.upload(sid, g["pos"], vel, g["alive"])
+    return sid
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("precision", [32, 64])
+def test_deposit_bit_exact_against_fixed_point_oracle(oracle, hip_ctx, case, precision):
+    print(f"case={case} precision={precision}", flush=True)
+    g = golden(f"deposit_gather_{case}.npz")
+    sid = _setup_case(hip_ctx, g, precision)
+    hip_ctx.deposit()
+    acc = hip_ctx.charge_fixed(sid)
+    cell, off, _ = hip_ctx.download_raw(sid)
+    expect = oracle.deposit_fixed(g["n"], int(g["bc"]), cell, off)
+    assert np.array_equal(acc, expect)
+    rho, _, _ = hip_ctx.fields()
+    rho_fx = oracle.fixed_to_rho(g["n"], g["L"], int(g["bc"]), [expect],
+                                 [float(g["charge"]) * float(g["weight"])])
+    assert np.array_equal(rho, rho_fx)
+    # against the reference's FP64 deposit of the same input positions: the
+    # difference is the offset rounding (FP32: 2^-24 per weight) plus 2^-40
+    ulp = 2.0 ** -24 if precision == 32 else 2.0 ** -52
+    dim = len(g["n"])
+    touch = oracle.deposit_fixed(g["n"], int(g["bc"]), cell, np.full_like(off, 0.5))
+    count = touch / (2.0 ** 40 / 2 ** dim)
+    d = [g["L"][a] / (g["n"][a] if int(g["bc"]) == 0 else g["n"][a] - 1) for a in range(dim)]
+    vmin = np.prod(d) / 2 ** dim
+    bound = count * (dim * 2 * ulp + 2.0 ** -40) * abs(float(g["charge"]) * float(g["weight"])) / vmin
+    assert np.all(np.abs(rho - g["rho"]) <= bound + 1e-300)
+
+
+def test_deposit_order_independent(oracle, hip_ctx):
+    g = golden("deposit_gather_3d_periodic.npz")
+    sid = _setup_case(hip_ctx, g)
+    hip_ctx.deposit()
+    a = hip_ctx.charge_fixed(sid)
+    perm = np.random.default_rng(0).permutation(len(g["alive"]))
+    hip_ctx.upload(sid, g["pos"][perm], np.zeros((len(perm), 3)), g["alive"][perm])
+    hip_ctx.deposit()
+    assert np.array_equal(a, hip_ctx.charge_fixed(sid))
+
+
+@pytest.mark.parametrize("case", ["1d_periodic", "1d_dirichlet", "2d_pow2", "2d_mixed",
+                                  "3d_pow2", "3d_mixed"])
+def test_field_solve_matches_reference_golden(hip_ctx, case):
+    print(f"case={case}", flush=True)
+    g = golden(f"solve_{case}.npz")
+    hip_ctx.mesh(g["n"], g["L"], int(g["bc"]), float(g["eps"]))
+    hip_ctx.set_rho(g["rho"])
+    hip_ctx.solve(float(g["phi_left"]), float(g["phi_right"]), True)
+    _, phi, E = hip_ctx.fields()
+    assert np.max(np.abs(E - g["E"])) <= 1e-10 * np.max(np.abs(g["E"]))
+    assert np.max(np.abs(phi - g["phi"])) <= 1e-10 * np.max(np.abs(g["phi"]))
+
+
+def test_field_solve_3d_analytic_mode(hip_ctx):
+    # test_core.cpp:1360-1438 on a 32^3 power-of-two mesh
+    n, L = [32, 32, 32], [1.0, 1.25, 1.5]
+    kx, ky, kz = 2 * np.pi / L[0], 4 * np.pi / L[1], 6 * np.pi / L[2]
+    k2 = kx * kx + ky * ky + kz * kz
+    z, y, x = [np.arange(n[a]) * L[a] / n[a] for a in (2, 1, 0)]
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    rho = np.sin(kx * X) * np.cos(ky * Y) * np.sin(kz * Z)
+    hip_ctx.mesh(n, L, 0, 1.0)
+    hip_ctx.set_rho(rho.reshape(-1))
+    hip_ctx.solve(0, 0, True)
+    _, phi, E = hip_ctx.fields()
+    assert np.max(np.abs(phi - (rho / k2).reshape(-1))) < 1e-12
+    ex = -kx * np.cos(kx * X) * np.cos(ky * Y) * np.sin(kz * Z) / k2
+    ez = -kz * np.sin(kx * X) * np.cos(ky * Y) * np.cos(kz * Z) / k2
+    assert np.max(np.abs(E[0] - ex.reshape(-1))) < 1e-12
+    assert np.max(np.abs(E[2] - ez.reshape(-1))) < 1e-12
+
+
+def _smooth_field(n, L, dim, rng):
+    coords = [np.arange(n[a]) * L[a] / n[a] for a in range(dim)]
+    grids = np.meshgrid(*coords[::-1], indexing="ij")[::-1]
+    E = []
+    for a in range(dim):
+        f = 0.3 + np.zeros_like(grids[0])
+        for b in range(dim):
+            f = f + rng.uniform(-1, 1) * np.sin(2 * np.pi * grids[b] / L[b] + rng.uniform(0, 6))
+        E.append(f.reshape(-1))
+    return np.stack(E)
+
+
+PUSH_CASES = [
+    ("1d_periodic", [64], [2.0], 0, None, None),
+    ("1d_dirichlet", [65], [1.0], 1, None, None),
+    ("2d_uniform_bz", [32, 24], [1.0, 0.75], 0, [0.0, 0.0, 2.5], None),
+    ("3d_leapfrog", [16, 12, 10], [1.0, 0.75, 0.6], 0, None, None),
+    ("3d_table_by", [16, 12, 10], [1.0, 0.75, 0.6], 0, None, "table"),
+]
+
+
+@pytest.mark.parametrize("name,n,L,bc,B,btable", PUSH_CASES)
+def test_one_step_push_matches_oracle(oracle, hip_ctx, name, n, L, bc, B, btable):
+    print(f"case={name}", flush=True)
+    rng = np.random.default_rng(7)
+    dim = len(n)
+    N = 4000
+    pos = rng.uniform(0, 1, size=(N, dim)) * np.asarray(L)
+    vel = rng.normal(0, 0.4, size=(N, 3))
+    E = _smooth_field(n, L, dim, rng)
+    qm, dt = -1.3, 0.05
+    hip_ctx.mesh(n, L, bc, 1.0)
+    if B is not None:
+        hip_ctx.uniform_b(B)
+    bt = None
+    if btable == "table":
+        x = np.linspace(0, L[0], 57)
+        vals = np.stack([0.2 * np.cos(x), 1.5 + np.sin(3 * x), 0.4 * x], axis=1)
+        hip_ctx.b_profile(0, x, vals)
+        bt = (0, x, vals)
+    sid = hip_ctx.add_species(qm, 1.0, 1.0, N)
+    hip_ctx.upload(sid, pos, vel)
+    hip_ctx.set_electric(E)
+    pos0, vel0, alive0, _ = hip_ctx.download(sid)
+    hip_ctx.push(dt)
+    pos1, vel1, alive1, _ = hip_ctx.download(sid)
+    opos, ovel, oalive, _ = oracle.push(n, L, bc, pos0, vel0, alive0, E, qm, dt, B=B, btable=bt)
+    # absorption decisions agree except within rounding of a wall
+    scale_x = np.asarray(L)
+    mismatch = alive1 != oalive
+    if mismatch.any():
+        near = np.min(np.minimum(np.abs(opos[mismatch]), np.abs(opos[mismatch] - scale_x)), axis=1)
+        assert np.all(near <= 1e-6 * np.max(scale_x))
+    keep = (alive1 == 1) & (oalive == 1)
+    dx = pos1[keep] - opos[keep]
+    if bc == 0:
+        dx = (dx + 0.5 * scale_x) % scale_x - 0.5 * scale_x
+    assert np.max(np.abs(dx)) <= 1e-6 * np.max(scale_x)
+    vscale = np.max(np.abs(ovel[keep]))
+    assert np.max(np.abs(vel1[keep] - ovel[keep])) <= 1e-6 * vscale
+
+
+def test_step_3d_matches_oracle(oracle, hip_ctx):
+    """Initialize + one full step (push, deposit, FFT solve) of a two-species
+    3D plasma with a tabulated B against the FP64 oracle from the same state."""
+    rng = np.random.default_rng(3)
+    n, L = [16, 16, 16], [1.0, 1.0, 1.0]
+    N = 16 * 16 * 16 * 4
+    dt = 0.02
+    species = [(-1.0, 1.0, 1.0 / N), (1.0, 50.0, 1.0 / N)]
+    x = np.linspace(0, 1.0, 33)
+    bvals = np.stack([np.zeros(33), 2.0 * np.exp(-((x - 0.5) / 0.2) ** 2), np.zeros(33)], axis=1)
+    hip_ctx.mesh(n, L, 0, 1.0)
+    hip_ctx.b_profile(0, x, bvals)
+    sids = []
+    states = []
+    for q, m, w in species:
+        pos = rng.uniform(0, 1, size=(N, 3))
+        vel = rng.normal(0, 0.3 / math.sqrt(m), size=(N, 3))
+        sid = hip_ctx.add_species(q, m, w, N)
+        hip_ctx.upload(sid, pos, vel, kind=1)
+        sids.append(sid)
+    for sid in sids:
+        p, v, a, _ = hip_ctx.download(sid)
+        states.append([p, v, a])
+    hip_ctx.initialize(dt)
+    bt = (0, x, bvals)
+
+    def oracle_fields(sts):
+        rho = np.zeros(4096)
+        for (q, m, w), (p, v, a) in zip(species, sts):
+            rho += oracle.deposit_cic(n, L, 0, p, q, w, a)
+        _, E = oracle.poisson(n, L, 0, 1.0, rho)
+        return rho, E
+
+    _, E0 = oracle_fields(states)
+    _, _, Eg0 = hip_ctx.fields()
+    assert np.max(np.abs(Eg0 - E0)) <= 1e-6 * np.max(np.abs(E0))
+    for (q, m, w), st in zip(species, states):
+        st[1] = oracle.init_half(n, L, 0, st[0], st[1], st[2], E0, q / m, dt, btable=bt)
+    hip_ctx.step(dt, 1)
+    for (q, m, w), st in zip(species, states):
+        st[0], st[1], st[2], _ = oracle.push(n, L, 0, st[0], st[1], st[2], E0, q / m, dt,
+                                             btable=bt)
+    rho1, E1 = oracle_fields(states)
+    rg, _, Eg1 = hip_ctx.fields()
+    assert np.max(np.abs(Eg1 - E1)) <= 1e-6 * np.max(np.abs(E1))
+    for sid, (q, m, w), st in zip(sids, species, states):
+        p, v, a, _ = hip_ctx.download(sid)
+        d = (p - st[0] + 0.5) % 1.0 - 0.5
+        assert np.max(np.abs(d)) <= 1e-6
+        assert np.max(np.abs(v - st[1])) <= 1e-6 * np.max(np.abs(st[1]))
+        # fused deposit is bit-exact on the device's own post-drift state
+        cell, off, _ = hip_ctx.download_raw(sid)
+        assert np.array_equal(hip_ctx.charge_fixed(sid), oracle.deposit_fixed(n, 0, cell, off))
+
+
+def test_sort_preserves_particles_and_charge(hip_ctx):
+    rng = np.random.default_rng(9)
+    n, L = [16, 8, 8], [2.0, 1.0, 1.0]
+    N = 20000
+    hip_ctx.mesh(n, L, 0, 1.0)
+    sid = hip_ctx.add_species(-1.0, 1.0, 1.0, N)
+    hip_ctx.upload(sid, rng.uniform(0, 1, (N, 3)) * L, rng.normal(0, 1, (N, 3)))
+    hip_ctx.set_electric(np.zeros((3, 16 * 64)))
+    hip_ctx.step(0.01, 3)
+    acc = hip_ctx.charge_fixed(sid)
+    p0, v0, _, u0 = hip_ctx.download(sid)
+    hip_ctx.sort()
+    hip_ctx.deposit()
+    assert np.array_equal(acc, hip_ctx.charge_fixed(sid))
+    p1, v1, _, u1 = hip_ctx.download(sid)
+    o0, o1 = np.argsort(u0), np.argsort(u1)
+    assert np.array_equal(u0[o0], u1[o1])
+    assert np.array_equal(p0[o0], p1[o1]) and np.array_equal(v0[o0], v1[o1])
+    # sorted by tile-major cell key: cells are non-decreasing within tiles order
+    cell, _, _ = hip_ctx.download_raw(sid)
+    assert len(cell) == N
+
+
+def test_full_size_3d_invariants(hip_ctx):
+    """The 3D3V workload at full size (128^3 cells, 16 + 16 particles per
+    cell): charge conservation of the fixed-point deposit, fused deposit equal
+    to a separate deposit of the same state, live count."""
+    import aurora_pic.workloads as wl
+    cfg = wl.config_plasma_3d()
+    hip_ctx.mesh(cfg["n"], cfg["L"], 0, cfg["eps"])
+    axis, bx, bv = cfg["btable"]
+    hip_ctx.b_profile(axis, bx, bv)
+    sids = []
+    for k, s in enumerate(cfg["species"]):
+        sid = hip_ctx.add_species(s["charge"], s["mass"], s["weight"], s["particles"])
+        hip_ctx.init_maxwellian(sid, s["particles"], wl.maxwellian_sigma(s["temperature_ev"],
+                                                                         s["mass"]), 100 + k)
+        sids.append(sid)
+    hip_ctx.initialize(cfg["dt"])
+    hip_ctx.step(cfg["dt"], 2)
+    N = cfg["species"][0]["particles"]
+    for sid in sids:
+        acc = hip_ctx.charge_fixed(sid)
+        assert abs(int(acc.sum()) - N * 2 ** 40) <= N * 4
+        slots, live = hip_ctx.count(sid)
+        assert live == N
+    fused = [hip_ctx.charge_fixed(s) for s in sids]
+    hip_ctx.deposit()
+    for s, a in zip(sids, fused):
+        assert np.array_equal(a, hip_ctx.charge_fixed(s))
+    ke, fe, l1, live = hip_ctx.energy(cfg["dt"])
+    assert live == 2 * N and np.all(np.isfinite(ke)) and np.isfinite(fe)
+
+
+def test_error_paths(hip_ctx):
+    from aurora_pic import hip
+    hip_ctx.mesh([8, 8, 8], [1, 1, 1], 0, 1.0)
+    with pytest.raises(hip.HipError, match="does not cover the simulation domain"):
+        hip_ctx.b_profile(0, [0.1, 0.5], [[0, 0, 1], [0, 0, 2]])
+    with pytest.raises(hip.HipError, match="strictly increasing"):
+        hip_ctx.b_profile(0, [0.0, 0.0, 1.0], [[0, 0, 1]] * 3)
+    with pytest.raises(hip.HipError, match="permittivity"):
+        hip_ctx.mesh([8], [1.0], 0, 0.0)
