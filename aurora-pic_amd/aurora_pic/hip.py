@@ -119,6 +119,7 @@ def load(path: str = LIB_PATH):
         "aurora_hip_time": [ctx, _dp, _i64p],
         "aurora_hip_set_sort_interval": [ctx, C.c_int64],
         "aurora_hip_sort": [ctx],
+        "aurora_hip_set_tiling": [ctx, C.c_int, C.c_int, C.c_int, C.c_int],
         "aurora_hip_mcc_load": [ctx, C.c_int, C.c_int, _ip, _dp, _i64p, C.POINTER(_dp),
                                 C.POINTER(_dp), C.c_double, C.c_double, C.c_int64, C.c_double,
                                 C.c_double, _ip, C.c_uint64],
@@ -343,6 +344,9 @@ class Context:
 
     def sort_interval(self, k):
         self._check(self.lib.aurora_hip_set_sort_interval(self.h, k))
+
+    def tiling(self, tx, ty=0, tz=0, halo=0):
+        self._check(self.lib.aurora_hip_set_tiling(self.h, tx, ty, tz, halo))
 
     def sort(self):
         self._check(self.lib.aurora_hip_sort(self.h))

@@ -25,6 +25,9 @@ FLAGS = [
     "-O3", "-std=c++20", "--offload-arch=gfx950", "-fPIC",
     # packed FP32 instructions are kept out of generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # contract a*b+c only within one source expression: the same expression
+    # rounds the same way in every kernel that inlines it
+    "-ffp-contract=on",
     "-Wall", "-Wno-unused-result", "-Wno-unused-command-line-argument",
     f"-I{os.path.join(REPO, 'include')}", f"-I{CSRC}",
 ]

@@ -361,6 +361,7 @@ int aurora_hip_species_upload(aurora_hip_ctx* h, int sid, int64_t count, const d
         }
         sp.slots = count;
         sp.next_uid = static_cast<uint32_t>(count);
+        sp.tiled = false;
     });
 }
 
@@ -611,6 +612,27 @@ int aurora_hip_set_sort_interval(aurora_hip_ctx* h, int64_t interval) {
     return guarded(h, [&](Context& c) {
         AUR_REQUIRE(interval >= 0, AURORA_ERR_INVALID_ARGUMENT, "sort interval must be >= 0");
         c.sort_interval = interval;
+    });
+}
+
+int aurora_hip_set_tiling(aurora_hip_ctx* h, int tx, int ty, int tz, int halo) {
+    return guarded(h, [&](Context& c) {
+        if (tx == 0) {
+            c.tiled_enabled = false;
+            return;
+        }
+        static const int shapes[][4] = {{8, 8, 8, 2}, {8, 8, 8, 1}, {8, 8, 4, 2}, {4, 4, 4, 2},
+                                        {16, 8, 4, 2}};
+        bool known = false;
+        for (const auto& s : shapes)
+            known = known || (s[0] == tx && s[1] == ty && s[2] == tz && s[3] == halo);
+        AUR_REQUIRE(known, AURORA_ERR_UNSUPPORTED, "tile shape is not built into this library");
+        c.tiled_enabled = true;
+        c.tile[0] = tx;
+        c.tile[1] = ty;
+        c.tile[2] = tz;
+        c.halo = halo;
+        for (auto& sp : c.species) sp.tiled = false;
     });
 }
 

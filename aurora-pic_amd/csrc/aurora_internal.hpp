@@ -92,6 +92,8 @@ struct Species {
     // boundary ledger on device: [6 counts][2 energies as int64-free doubles]
     DevBuf<unsigned long long> absorbed;  // 6
     DevBuf<double> absorbed_energy;       // per-block partials (2 per block)
+    DevBuf<int32_t> tile_start;           // ntiles + 1 segment starts after a sort
+    bool tiled = false;                   // storage is tile-sorted (tile_start valid)
     int64_t absorbed_host[6] = {0, 0, 0, 0, 0, 0};
     double absorbed_energy_host[2] = {0, 0};
     size_t elem() const { return precision == 64 ? 8 : 4; }
@@ -179,6 +181,8 @@ struct Context {
     // sorting
     int64_t sort_interval = 0;
     int tile[3] = {8, 8, 8};
+    int halo = 2;
+    bool tiled_enabled = true;
     DevBuf<int32_t> tile_count, tile_offset;
     DevBuf<unsigned> sort_keys, sort_keys2, sort_vals, sort_vals2;
     DevBuf<uint8_t> sort_temp;
@@ -199,6 +203,25 @@ struct Context {
         return c;
     }
     int64_t cells_axis(int a) const { return bc == AURORA_PERIODIC ? n[a] : n[a] - 1; }
+    // tiles of the sort key (and of the LDS-tiled push) per axis
+    int tiles_axis(int a) const {
+        if (a >= dim) return 1;
+        const int t = tile[a] < n[a] ? tile[a] : static_cast<int>(n[a]);
+        return static_cast<int>((n[a] + t - 1) / t);
+    }
+    int64_t ntiles() const {
+        return static_cast<int64_t>(tiles_axis(0)) * tiles_axis(1) * tiles_axis(2);
+    }
+    // the LDS-tiled 3D push applies on periodic meshes whose extents the
+    // tiles divide and that hold a whole tile box without self-overlap
+    bool tiled_push_ok() const {
+        if (!tiled_enabled || dim != 3 || bc != AURORA_PERIODIC) return false;
+        for (int f = 0; f < 6; ++f)
+            if (pb[f] != AURORA_PB_PERIODIC) return false;
+        for (int a = 0; a < 3; ++a)
+            if (n[a] % tile[a] != 0 || n[a] < tile[a] + 1 + 2 * halo) return false;
+        return true;
+    }
 };
 
 // Scoped kernel timer: records HIP events around a launch when enabled.

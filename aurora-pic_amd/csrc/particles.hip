@@ -122,6 +122,32 @@ __device__ __forceinline__ void magnetic(const BView& b, const int* idx, const R
 }
 
 // ------------------------------------------------------------------ gather
+// Trilinear accumulation shared by the global and the LDS-tiled gathers
+// (FieldSolver.cpp:1024-1037 order: z, y, x corners); fetch(a, b, c) returns
+// the node field of corner (a, b, c).
+template <typename Fetch>
+__device__ __forceinline__ void trilinear_f32(const float* off, Fetch fetch, float* E) {
+    const float wx[2] = {1.0f - off[0], off[0]};
+    const float wy[2] = {1.0f - off[1], off[1]};
+    const float wz[2] = {1.0f - off[2], off[2]};
+    float ex = 0.f, ey = 0.f, ez = 0.f;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const float shape = wx[a] * wy[b] * wz[c];
+                const float4 e = fetch(a, b, c);
+                ex += e.x * shape;
+                ey += e.y * shape;
+                ez += e.z * shape;
+            }
+    E[0] = ex;
+    E[1] = ey;
+    E[2] = ez;
+}
+
 template <int DIM, typename R>
 __device__ __forceinline__ void gather(const PushArgs<R>& a, const int* idx, const R* off,
                                        R* E) {
@@ -161,8 +187,15 @@ __device__ __forceinline__ void gather(const PushArgs<R>& a, const int* idx, con
         } else {
             const int k0 = idx[2];
             const int k1 = (k0 + 1 == m.nz) ? 0 : k0 + 1;
-            const R wz[2] = {one - off[2], off[2]};
             const int ii[2] = {i0, i1}, jj[2] = {j0, j1}, kk[2] = {k0, k1};
+            if constexpr (sizeof(R) == 4) {
+                const float4* E4 = reinterpret_cast<const float4*>(a.E32);
+                trilinear_f32(off, [&](int aa, int b, int c) {
+                    return E4[(kk[c] * m.ny + jj[b]) * m.nx + ii[aa]];
+                }, E);
+                return;
+            }
+            const R wz[2] = {one - off[2], off[2]};
             R ex = 0, ey = 0, ez = 0;
 #pragma unroll
             for (int c = 0; c < 2; ++c)
@@ -625,6 +658,149 @@ int launch_blocks(const Context& c, int64_t n) {
     return static_cast<int>(g < 1 ? 1 : g);
 }
 
+// ------------------------------------------------------- LDS-tiled 3D push
+// One workgroup per TX x TY x TZ tile of a tile-sorted species.  The E field
+// of the tile plus a halo of H cells is staged in LDS as float4, and the
+// tile's int64 charge is accumulated in LDS; particles that drifted beyond
+// the halo since the last sort use the global field and global atomics.
+// The result is identical to the untiled kernel: integer adds commute.
+__device__ __forceinline__ int rel_index(int i, int o, int n) {
+    int r = i - o;
+    if (r < 0) r += n;
+    else if (r >= n) r -= n;
+    return r;
+}
+
+template <int TX, int TY, int TZ, int H, int BMODE>
+__global__ void __launch_bounds__(512) k_push_tile3d(PushArgs<float> a,
+                                                     const int32_t* __restrict__ tile_start,
+                                                     int ntx, int nty) {
+    constexpr int BX = TX + 1 + 2 * H, BY = TY + 1 + 2 * H, BZ = TZ + 1 + 2 * H;
+    constexpr int BN = BX * BY * BZ;
+    __shared__ float4 sE[BN];
+    __shared__ unsigned long long sQ[BN];
+    const MeshView& m = a.m;
+    const SpeciesView<float>& sp = a.sp;
+    const int t = blockIdx.x;
+    const int ox = (t % ntx) * TX - H;
+    const int oy = ((t / ntx) % nty) * TY - H;
+    const int oz = (t / (ntx * nty)) * TZ - H;
+    const float4* E4 = reinterpret_cast<const float4*>(a.E32);
+    for (int q = threadIdx.x; q < BN; q += blockDim.x) {
+        const int bx = q % BX, by = (q / BX) % BY, bz = q / (BX * BY);
+        const int gx = wrap_index(ox + bx, m.nx);
+        const int gy = wrap_index(oy + by, m.ny);
+        const int gz = wrap_index(oz + bz, m.nz);
+        sE[q] = E4[(gz * m.ny + gy) * m.nx + gx];
+        sQ[q] = 0ull;
+    }
+    __syncthreads();
+    const int p0 = tile_start[t], p1 = tile_start[t + 1];
+    for (int p = p0 + static_cast<int>(threadIdx.x); p < p1; p += blockDim.x) {
+        const int c = sp.cell[p];
+        if (c < 0) continue;
+        int idx[3];
+        decompose<3>(c, m, idx);
+        float off[3] = {sp.off0[p], sp.off1[p], sp.off2[p]};
+        float v[3] = {sp.v0[p], sp.v1[p], sp.v2[p]};
+        float E[3];
+        {
+            const int rx = rel_index(idx[0], ox, m.nx);
+            const int ry = rel_index(idx[1], oy, m.ny);
+            const int rz = rel_index(idx[2], oz, m.nz);
+            if (rx <= BX - 2 && ry <= BY - 2 && rz <= BZ - 2) {
+                const int base = (rz * BY + ry) * BX + rx;
+                trilinear_f32(off, [&](int aa, int b, int cc) {
+                    return sE[base + (cc * BY + b) * BX + aa];
+                }, E);
+            } else {
+                gather<3, float>(a, idx, off, E);
+            }
+        }
+        if constexpr (BMODE == 0) {
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) v[ax] += a.qm_dt * E[ax];
+        } else {
+            float Bx, By, Bz;
+            magnetic<3, float>(a.b, idx, off, Bx, By, Bz, a.err);
+            boris<float>(v, E, Bx, By, Bz, a.half_qm_dt);
+        }
+        const float dtd[3] = {a.dtd0, a.dtd1, a.dtd2};
+        const int nn[3] = {m.nx, m.ny, m.nz};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            int hit = 0;
+            drift_axis<float>(idx[ax], off[ax], v[ax], v[ax] * dtd[ax], nn[ax], nn[ax],
+                              AURORA_PB_PERIODIC, AURORA_PB_PERIODIC, hit);
+        }
+        sp.v0[p] = v[0];
+        sp.v1[p] = v[1];
+        sp.v2[p] = v[2];
+        sp.cell[p] = compose<3>(m, idx);
+        sp.off0[p] = off[0];
+        sp.off1[p] = off[1];
+        sp.off2[p] = off[2];
+        const double o[3] = {static_cast<double>(off[0]), static_cast<double>(off[1]),
+                             static_cast<double>(off[2])};
+        const int rx = rel_index(idx[0], ox, m.nx);
+        const int ry = rel_index(idx[1], oy, m.ny);
+        const int rz = rel_index(idx[2], oz, m.nz);
+        if (rx <= BX - 2 && ry <= BY - 2 && rz <= BZ - 2) {
+            const double wx[2] = {1.0 - o[0], o[0]};
+            const double wy[2] = {1.0 - o[1], o[1]};
+            const double wz[2] = {1.0 - o[2], o[2]};
+            const int base = (rz * BY + ry) * BX + rx;
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int aa = 0; aa < 2; ++aa)
+                        atomicAdd(&sQ[base + (cc * BY + b) * BX + aa],
+                                  static_cast<unsigned long long>(
+                                      quantise(wx[aa] * wy[b] * wz[cc])));
+        } else {
+            deposit_one<3>(a.acc, m, idx, o);
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < BN; q += blockDim.x) {
+        const unsigned long long val = sQ[q];
+        if (val == 0ull) continue;
+        const int bx = q % BX, by = (q / BX) % BY, bz = q / (BX * BY);
+        const int gx = wrap_index(ox + bx, m.nx);
+        const int gy = wrap_index(oy + by, m.ny);
+        const int gz = wrap_index(oz + bz, m.nz);
+        atomicAdd(reinterpret_cast<unsigned long long*>(a.acc) + (gz * m.ny + gy) * m.nx + gx,
+                  val);
+    }
+}
+
+template <int TX, int TY, int TZ, int H>
+bool launch_tile3d_t(Context& c, Species& sp, const PushArgs<float>& a, int mode) {
+    if (c.tile[0] != TX || c.tile[1] != TY || c.tile[2] != TZ || c.halo != H) return false;
+    const int ntx = c.tiles_axis(0), nty = c.tiles_axis(1);
+    const unsigned grid = static_cast<unsigned>(c.ntiles());
+    KernelScope ks(c, "push_deposit");
+    if (mode == 0)
+        k_push_tile3d<TX, TY, TZ, H, 0><<<grid, 512, 0, c.stream>>>(a, sp.tile_start.ptr, ntx, nty);
+    else if (mode == 1)
+        k_push_tile3d<TX, TY, TZ, H, 1><<<grid, 512, 0, c.stream>>>(a, sp.tile_start.ptr, ntx, nty);
+    else
+        k_push_tile3d<TX, TY, TZ, H, 2><<<grid, 512, 0, c.stream>>>(a, sp.tile_start.ptr, ntx, nty);
+    AUR_HIP(hipGetLastError());
+    return true;
+}
+
+// the tile shapes built into the library (the autotuner picks among them)
+bool launch_tile3d(Context& c, Species& sp, double dt) {
+    auto a = push_args<float>(c, sp, dt);
+    const int mode = a.b.mode;
+    return launch_tile3d_t<8, 8, 8, 2>(c, sp, a, mode) || launch_tile3d_t<8, 8, 8, 1>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 4, 2>(c, sp, a, mode) || launch_tile3d_t<4, 4, 4, 2>(c, sp, a, mode) ||
+           launch_tile3d_t<16, 8, 4, 2>(c, sp, a, mode);
+}
+
 template <int DIM, typename R, int BMODE>
 void launch_push_b(Context& c, Species& sp, double dt, bool deposit) {
     auto a = push_args<R>(c, sp, dt);
@@ -760,6 +936,9 @@ void push_all(Context& c, double dt, bool fuse_deposit) {
             sp.absorbed_energy.alloc(2);
             AUR_HIP(hipMemset(sp.absorbed_energy.ptr, 0, 2 * sizeof(double)));
         }
+        if (fuse_deposit && sp.tiled && sp.precision == 32 && c.tiled_push_ok() &&
+            launch_tile3d(c, sp, dt))
+            continue;
         dispatch_dim_prec(c, sp, [&]<int DIM, typename R>() {
             launch_push<DIM, R>(c, sp, dt, fuse_deposit);
         });
@@ -797,6 +976,7 @@ int aurora_hip_species_init_maxwellian(aurora_hip_ctx* h, int sid, int64_t count
                               drift3 ? drift3[2] : 0.0};
         sp.slots = count;
         sp.next_uid = static_cast<uint32_t>(count);
+        sp.tiled = false;
         if (count == 0) return;
         const MeshView m = mesh_view(c);
         const int g = launch_blocks(c, count);

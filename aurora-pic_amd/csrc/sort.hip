@@ -22,7 +22,8 @@ struct KeyArgs {
     unsigned sentinel;
 };
 
-// tile-major key: tile id * tile volume + cell within tile
+// key = tile id; a stable sort keeps the previous order inside a tile, so
+// neighbouring slots of a tile hold particles of different cells
 __global__ void k_sort_keys(const int32_t* cell, unsigned* keys, unsigned* vals, int64_t n,
                             KeyArgs k) {
     for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < n;
@@ -34,14 +35,23 @@ __global__ void k_sort_keys(const int32_t* cell, unsigned* keys, unsigned* vals,
             const int t = c / k.nx;
             const int j = t % k.ny;
             const int kk = t / k.ny;
-            const unsigned tile = static_cast<unsigned>(((kk / k.tz) * k.nty + j / k.ty) * k.ntx +
-                                                        i / k.tx);
-            const unsigned in_tile = static_cast<unsigned>(((kk % k.tz) * k.ty + j % k.ty) * k.tx +
-                                                           i % k.tx);
-            key = tile * static_cast<unsigned>(k.tx * k.ty * k.tz) + in_tile;
+            key = static_cast<unsigned>(((kk / k.tz) * k.nty + j / k.ty) * k.ntx + i / k.tx);
         }
         keys[p] = key;
         vals[p] = static_cast<unsigned>(p);
+    }
+}
+
+// segment starts: tile_start[t] = first sorted slot with key >= t, t <= ntiles
+__global__ void k_tile_start(const unsigned* keys, int64_t n, int32_t* tile_start,
+                             unsigned ntiles) {
+    for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p <= n;
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const unsigned kp = p < n ? keys[p] : ntiles;
+        const long long prev = p == 0 ? -1 : static_cast<long long>(keys[p - 1]);
+        for (long long t = prev + 1; t <= static_cast<long long>(kp) &&
+                                     t <= static_cast<long long>(ntiles); ++t)
+            tile_start[t] = static_cast<int32_t>(p);
     }
 }
 
@@ -85,9 +95,8 @@ void sort_species(Context& c, Species& sp) {
     k.ntx = (k.nx + k.tx - 1) / k.tx;
     k.nty = (k.ny + k.ty - 1) / k.ty;
     const int ntz = (k.nz + k.tz - 1) / k.tz;
-    const uint64_t keyspace =
-        static_cast<uint64_t>(k.ntx) * k.nty * ntz * k.tx * k.ty * k.tz;
-    AUR_REQUIRE(keyspace < 0xFFFFFFFFull, AURORA_ERR_UNSUPPORTED, "sort key space too large");
+    const uint64_t keyspace = static_cast<uint64_t>(k.ntx) * k.nty * ntz;
+    AUR_REQUIRE(keyspace < 0x7FFFFFFFull, AURORA_ERR_UNSUPPORTED, "sort key space too large");
     k.sentinel = static_cast<unsigned>(keyspace);
     unsigned bits = 1;
     while ((1ull << bits) <= keyspace) ++bits;
@@ -128,22 +137,17 @@ void sort_species(Context& c, Species& sp) {
         }
     }
     AUR_HIP(hipGetLastError());
-    // live count = position of the first sentinel key
-    unsigned last = 0;
-    AUR_HIP(hipMemcpyAsync(&last, keys2.ptr + (n - 1), sizeof last, hipMemcpyDeviceToHost,
-                           c.stream));
+    // segment starts; the live count is the start of the sentinel segment
+    if (sp.tile_start.count < static_cast<size_t>(keyspace + 1)) sp.tile_start.alloc(keyspace + 1);
+    k_tile_start<<<blocks(c, n + 1), 256, 0, c.stream>>>(keys2.ptr, n, sp.tile_start.ptr,
+                                                         static_cast<unsigned>(keyspace));
+    AUR_HIP(hipGetLastError());
+    int32_t live = 0;
+    AUR_HIP(hipMemcpyAsync(&live, sp.tile_start.ptr + keyspace, sizeof live,
+                           hipMemcpyDeviceToHost, c.stream));
     AUR_HIP(hipStreamSynchronize(c.stream));
-    if (last == k.sentinel) {
-        // binary search for the first sentinel on the host-visible copy
-        std::vector<unsigned> hk(static_cast<size_t>(n));
-        AUR_HIP(hipMemcpy(hk.data(), keys2.ptr, n * sizeof(unsigned), hipMemcpyDeviceToHost));
-        int64_t lo = 0, hi = n;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) / 2;
-            if (hk[mid] == k.sentinel) hi = mid; else lo = mid + 1;
-        }
-        sp.slots = lo;
-    }
+    sp.slots = live;
+    sp.tiled = true;
 }
 
 }  // namespace aurora

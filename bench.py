@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--cpu-ppc", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the per-step loop of one 3D3V instance runs on a "
@@ -108,6 +109,13 @@ def main():
     ctx.sort()
     ctx.initialize(dt)
     ctx.sort_interval(cfg["sort_interval"])
+    tuned = None
+    if not args.no_autotune:
+        from aurora_pic import autotune
+        best, table = autotune.tune(ctx, dt, sort_intervals=(cfg["sort_interval"],), log=log)
+        tuned = {"tile": list(best[1][:3]), "halo": best[1][3], "sort_interval": best[2],
+                 "candidates_ms_per_step": {f"{s[0]}x{s[1]}x{s[2]}h{s[3]}": round(ms, 4)
+                                            for ms, s, _ in table}}
     log(f"particles {total}, warmup {args.warmup} steps")
     ctx.step(dt, args.warmup)
     ctx.sync()
@@ -174,7 +182,7 @@ def main():
                                "steps" if args.cells == 128 and args.ppc == 16 else
                                f"plasma_3d3v {args.cells}^3 x {args.ppc}+{args.ppc} ppc",
                    "particles": total, "cells": args.cells ** 3, "dt_s": dt,
-                   "kernels_ms": kernels},
+                   "autotune": tuned, "kernels_ms": kernels},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "push_deposit (gather+Boris+drift+wrap+int64 deposit)",

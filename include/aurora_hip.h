@@ -190,6 +190,12 @@ int aurora_hip_boundary_losses(aurora_hip_ctx* ctx, int species_id,
                                int64_t* absorbed6, double* energy2);
 /* Simulation time and step count kept by the context. */
 int aurora_hip_time(aurora_hip_ctx* ctx, double* time, int64_t* step);
+/* Tile shape (cells per axis) of the cell sort and of the LDS-tiled 3D
+ * push, and its halo in cells; tx = 0 keeps the sort but disables the tiled
+ * push.  Built shapes: 8x8x8 (halo 1, 2), 8x8x4, 4x4x4, 16x8x4 (halo 2).
+ * No reference counterpart (the reference keeps particles unsorted). */
+int aurora_hip_set_tiling(aurora_hip_ctx* ctx, int tx, int ty, int tz,
+                          int halo);
 /* Cell sort every interval steps (0 = never) and an immediate sort. */
 int aurora_hip_set_sort_interval(aurora_hip_ctx* ctx, int64_t interval);
 int aurora_hip_sort(aurora_hip_ctx* ctx);
