@@ -1,30 +1,567 @@
-// swarm.hip - batched homogeneous electron swarms (Swarm.cpp:940-1250).
-#include "aurora_internal.hpp"
+// swarm.hip - batched homogeneous electron swarms (run_field,
+// Swarm.cpp:940-1250, fixed population), many requests of uneven particle
+// count per launch.
+//
+// Work is split into chunks of kChunk electrons of one request.  A
+// persistent kernel's workgroups pull chunks from a queue; each chunk's
+// velocities stay in registers for the launch's step range (half kick,
+// energy guard, MCC, half kick, guard - Swarm.cpp:998-1060).  Sampling
+// statistics go to per-(chunk, uncertainty block) integer fixed-point
+// partials (v_x in 1/16 m/s, energy in 2^-20 eV) and integer counters;
+// integer sums are associative, so a request's result does not depend on
+// which other requests share its launches or on how its steps are split
+// into launches.  Positions are not tracked (diffusion is not reported).
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "mcc_device.hpp"
 
 namespace aurora {
-void swarm_destroy(Context& c) { c.swarm = nullptr; }
+
+namespace {
+
+constexpr int kChunk = 256;
+constexpr double kElectronMass = 9.1093837139e-31;   // Swarm.cpp:23
+constexpr double kElementaryCharge = 1.602176634e-19; // Swarm.cpp:24
+constexpr double kTownsend = 1.0e-21;                 // Swarm.cpp:26
+
+struct RequestDesc {
+    float* vx;
+    float* vy;
+    float* vz;
+    long long* part;              // [chunk][block][2] fixed-point sums of v_x, energy
+    unsigned long long* counts;   // [candidates, nulls, chan...]
+    unsigned* max_energy_bits;
+    uint32_t* status;
+    int64_t particles;
+    float half_kick;
+    float dt;
+    float max_energy_ev;
+    float ev_per_v2;              // 0.5 m / e
+    int step0;                    // first step of this launch
+    int nsteps;                   // steps this launch
+    int burn_in;
+    int sampling;                 // steps - burn_in
+    int blocks;
+    uint64_t seed;
+};
+
+struct WorkItem {
+    int request;
+    int chunk;
+};
+
+__global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc* __restrict__ descs,
+                                                  const WorkItem* __restrict__ items, int nitems,
+                                                  int* head) {
+    __shared__ MccLds lds;
+    __shared__ int item_s;
+    __shared__ long long red_v[kChunk];
+    __shared__ long long red_e[kChunk];
+    stage_tables(m, lds);
+    const float* E = m.total_rows <= kMccLdsRows ? lds.E : m.energy;
+    const float* S = m.total_rows <= kMccLdsRows ? lds.S : m.sigma;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = atomicAdd(head, 1);
+        __syncthreads();
+        const int item = item_s;
+        __syncthreads();
+        if (item >= nitems) break;
+        const WorkItem w = items[item];
+        const RequestDesc d = descs[w.request];
+        const int64_t p = static_cast<int64_t>(w.chunk) * kChunk + threadIdx.x;
+        const bool active = p < d.particles;
+        float v[3] = {0.f, 0.f, 0.f};
+        if (active) { v[0] = d.vx[p]; v[1] = d.vy[p]; v[2] = d.vz[p]; }
+        uint32_t cand = 0, nulls = 0, chan[kMaxChannels] = {};
+        uint32_t err = 0;
+        float emax = 0.f;
+        long long acc_v = 0, acc_e = 0;
+        int group = -1;
+        for (int s = d.step0; s < d.step0 + d.nsteps; ++s) {
+            const bool sampling = s >= d.burn_in;
+            if (active && !err) {
+                v[0] += d.half_kick;
+                const float e_pre = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                emax = fmaxf(emax, e_pre);
+                if (e_pre > d.max_energy_ev) err |= DEV_ERR_ENERGY;
+                Philox rng;
+                rng.init(d.seed, 0u, static_cast<uint32_t>(p), static_cast<uint32_t>(s));
+                MccTally t;
+                mcc_collide(m, E, S, v, d.dt, rng, t);
+                err |= t.err;
+                v[0] += d.half_kick;
+                const float e = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                emax = fmaxf(emax, e);
+                if (e > d.max_energy_ev) err |= DEV_ERR_ENERGY;
+                if (sampling) {
+                    cand += t.candidates;
+                    nulls += t.nulls;
+                    for (int c = 0; c < m.nchan; ++c) chan[c] += t.chan[c];
+                }
+            }
+            if (sampling) {
+                // uncertainty block of sample k: max b with floor(b S / B) <= k
+                const int k = s - d.burn_in;
+                const int g = static_cast<int>(
+                    ((static_cast<int64_t>(k) + 1) * d.blocks - 1) / d.sampling);
+                if (g != group && group >= 0) {
+                    red_v[threadIdx.x] = acc_v;
+                    red_e[threadIdx.x] = acc_e;
+                    __syncthreads();
+                    for (int st = kChunk / 2; st > 0; st >>= 1) {
+                        if (threadIdx.x < st) {
+                            red_v[threadIdx.x] += red_v[threadIdx.x + st];
+                            red_e[threadIdx.x] += red_e[threadIdx.x + st];
+                        }
+                        __syncthreads();
+                    }
+                    if (threadIdx.x == 0) {
+                        long long* q = d.part + (static_cast<int64_t>(w.chunk) * d.blocks + group) * 2;
+                        q[0] += red_v[0];
+                        q[1] += red_e[0];
+                    }
+                    __syncthreads();
+                    acc_v = 0;
+                    acc_e = 0;
+                }
+                group = g;
+                if (active) {
+                    const float e = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                    acc_v += __float2ll_rn(v[0] * 16.0f);
+                    acc_e += __float2ll_rn(e * 1048576.0f);
+                }
+            }
+        }
+        if (group >= 0) {
+            red_v[threadIdx.x] = acc_v;
+            red_e[threadIdx.x] = acc_e;
+            __syncthreads();
+            for (int st = kChunk / 2; st > 0; st >>= 1) {
+                if (threadIdx.x < st) {
+                    red_v[threadIdx.x] += red_v[threadIdx.x + st];
+                    red_e[threadIdx.x] += red_e[threadIdx.x + st];
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                long long* q = d.part + (static_cast<int64_t>(w.chunk) * d.blocks + group) * 2;
+                q[0] += red_v[0];
+                q[1] += red_e[0];
+            }
+            __syncthreads();
+        }
+        if (active) {
+            d.vx[p] = v[0];
+            d.vy[p] = v[1];
+            d.vz[p] = v[2];
+            if (cand) atomicAdd(d.counts, static_cast<unsigned long long>(cand));
+            if (nulls) atomicAdd(d.counts + 1, static_cast<unsigned long long>(nulls));
+            for (int c = 0; c < m.nchan; ++c)
+                if (chan[c]) atomicAdd(d.counts + 2 + c, static_cast<unsigned long long>(chan[c]));
+            atomicMax(d.max_energy_bits, __float_as_uint(emax));
+            if (err) atomicOr(d.status, err);
+        }
+    }
+}
+
+// initial Maxwellian velocities with mean energy E0 (Swarm.cpp:951-957),
+// Box-Muller from the request's Philox stream (step slot 0xFFFFFFFF)
+__global__ void k_swarm_init(float* vx, float* vy, float* vz, int64_t n, float sigma,
+                             uint64_t seed) {
+    for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < n;
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        Philox rng;
+        rng.init(seed, 0u, static_cast<uint32_t>(p), 0xFFFFFFFFu);
+        const float u1 = rng.u01(), u2 = rng.u01(), u3 = rng.u01(), u4 = rng.u01();
+        const float r1 = sqrtf(-2.0f * logf(u1)), r2 = sqrtf(-2.0f * logf(u3));
+        float s1, c1, s2, c2;
+        sincosf(6.28318530717958647692f * u2, &s1, &c1);
+        sincosf(6.28318530717958647692f * u4, &s2, &c2);
+        vx[p] = sigma * r1 * c1;
+        vy[p] = sigma * r1 * s1;
+        vz[p] = sigma * r2 * c2;
+    }
+}
+
+struct RequestState {
+    aurora_hip_swarm_request req{};
+    int64_t chunks = 0;
+    int64_t steps_done = 0;
+    uint64_t footprint = 0;
+    DevBuf<float> v;                       // 3 x particles (SoA)
+    DevBuf<long long> part;
+    DevBuf<unsigned long long> counts;
+    DevBuf<unsigned> maxe;
+    DevBuf<uint32_t> status;
+};
+
+struct SwarmState {
+    MccModelHost gas;
+    bool gas_loaded = false;
+    std::map<int64_t, std::unique_ptr<RequestState>> active;
+    std::map<int64_t, aurora_hip_swarm_result> finished;
+    int64_t next_handle = 1;
+    uint64_t bytes_in_use = 0;
+    double table_max_ev = 0.0;
+    DevBuf<RequestDesc> descs;
+    DevBuf<WorkItem> items;
+    DevBuf<int> head;
+};
+
+SwarmState& state(Context& c) {
+    if (!c.swarm) c.swarm = new SwarmState();
+    return *static_cast<SwarmState*>(c.swarm);
+}
+
+uint64_t footprint_of(const aurora_hip_swarm_request& r) {
+    const uint64_t chunks = (static_cast<uint64_t>(r.particles) + kChunk - 1) / kChunk;
+    return static_cast<uint64_t>(r.particles) * 12u + chunks * static_cast<uint64_t>(r.blocks) * 16u +
+           (kMaxChannels + 3) * 8u + 64u;
+}
+
+MccParams swarm_params(const SwarmState& s) {
+    const MccModelHost& h = s.gas;
+    MccParams m{};
+    m.nchan = h.nchan;
+    int total = 0;
+    for (int k = 0; k < h.nchan; ++k) {
+        m.kind[k] = h.kind[k];
+        m.threshold[k] = static_cast<float>(h.threshold[k]);
+        m.rows[k] = static_cast<int>(h.rows[k]);
+        m.offset[k] = static_cast<int>(h.row_offset[k]);
+        total += m.rows[k];
+    }
+    m.total_rows = total;
+    m.shared_grid = h.shared_grid ? 1 : 0;
+    m.energy = h.energy.ptr;
+    m.sigma = h.sigma.ptr;
+    m.neutral_density = static_cast<float>(h.neutral_density);
+    m.max_frequency = static_cast<float>(h.max_frequency);
+    m.inv_max_frequency = static_cast<float>(1.0 / h.max_frequency);
+    m.max_candidates = static_cast<int>(h.max_candidates);
+    m.mass = static_cast<float>(kElectronMass);
+    m.half_mass = static_cast<float>(0.5 * kElectronMass);
+    m.neutral_mass = static_cast<float>(h.neutral_mass);
+    m.neutral_vstd = 0.f;
+    return m;
+}
+
+void block_mean_error(const std::vector<double>& means, double& mean, double& se) {
+    // Swarm.cpp:103-134 on block means
+    const size_t b = means.size();
+    double m = 0.0;
+    for (double v : means) m += v;
+    m /= static_cast<double>(b);
+    double sq = 0.0;
+    for (double v : means) sq += (v - m) * (v - m);
+    mean = m;
+    se = b == 1 ? 0.0 : std::sqrt(sq / (static_cast<double>(b) * static_cast<double>(b - 1)));
+}
+
+void finalize(Context& c, SwarmState& s, int64_t handle, RequestState& r) {
+    aurora_hip_swarm_result out{};
+    const auto& q = r.req;
+    const int64_t B = q.blocks, S = q.steps - q.burn_in, N = q.particles;
+    std::vector<long long> part(static_cast<size_t>(r.chunks * B * 2));
+    unsigned long long counts[kMaxChannels + 3];
+    unsigned maxe = 0;
+    uint32_t status = 0;
+    AUR_HIP(hipStreamSynchronize(c.stream));
+    AUR_HIP(hipMemcpy(part.data(), r.part.ptr, part.size() * sizeof(long long),
+                      hipMemcpyDeviceToHost));
+    AUR_HIP(hipMemcpy(counts, r.counts.ptr, sizeof counts, hipMemcpyDeviceToHost));
+    AUR_HIP(hipMemcpy(&maxe, r.maxe.ptr, sizeof maxe, hipMemcpyDeviceToHost));
+    AUR_HIP(hipMemcpy(&status, r.status.ptr, sizeof status, hipMemcpyDeviceToHost));
+    std::vector<double> vm(static_cast<size_t>(B)), em(static_cast<size_t>(B));
+    for (int64_t b = 0; b < B; ++b) {
+        double sv = 0.0, se = 0.0;
+        for (int64_t ch = 0; ch < r.chunks; ++ch) {   // fixed chunk order
+            sv += static_cast<double>(part[(ch * B + b) * 2]) * (1.0 / 16.0);
+            se += static_cast<double>(part[(ch * B + b) * 2 + 1]) * 0x1p-20;
+        }
+        const int64_t len = (b + 1) * S / B - b * S / B;
+        vm[b] = sv / (static_cast<double>(N) * static_cast<double>(len));
+        em[b] = se / (static_cast<double>(N) * static_cast<double>(len));
+    }
+    double mv, sev, me, see;
+    block_mean_error(vm, mv, sev);
+    block_mean_error(em, me, see);
+    out.drift = -mv;
+    out.drift_se = sev;
+    out.mean_energy_ev = me;
+    out.mean_energy_se = see;
+    const double exposure = static_cast<double>(N) * q.timestep * static_cast<double>(S);
+    out.candidates = counts[0];
+    out.nulls = counts[1];
+    for (int k = 0; k < s.gas.nchan; ++k) {
+        out.collisions[k] = counts[2 + k];
+        out.rate[k] = static_cast<double>(counts[2 + k]) / exposure;
+        out.rate_se[k] = std::sqrt(static_cast<double>(counts[2 + k])) / exposure;
+        if (s.gas.kind[k] == AURORA_IONIZATION) {
+            out.ionization_rate += out.rate[k];
+            out.ionization_rate_se = std::hypot(out.ionization_rate_se, out.rate_se[k]);
+        }
+    }
+    float mx;
+    std::memcpy(&mx, &maxe, sizeof mx);
+    out.max_energy_ev = mx;
+    out.status = AURORA_OK;
+    if (status & DEV_ERR_ENERGY) out.status = AURORA_ERR_ENERGY_ABOVE_MAX;
+    else if (status & DEV_ERR_RATE) out.status = AURORA_ERR_RATE_ABOVE_BOUND;
+    else if (status & DEV_ERR_CANDIDATES) out.status = AURORA_ERR_CANDIDATE_LIMIT;
+    else if (status) out.status = AURORA_ERR_INVALID_ARGUMENT;
+    s.finished[handle] = out;
+    s.bytes_in_use -= r.footprint;
+}
+
+}  // namespace
+
+void swarm_destroy(Context& c) {
+    if (c.swarm) {
+        delete static_cast<SwarmState*>(c.swarm);
+        c.swarm = nullptr;
+    }
+}
+
 }  // namespace aurora
 
 using namespace aurora;
 
 extern "C" {
 
-int aurora_hip_swarm_gas(aurora_hip_ctx* h, int, const int*, const double*, const int64_t*,
-                         const double* const*, const double* const*, double, double, int64_t,
-                         double) {
-    return guarded(h, [](Context&) { throw Failure{AURORA_ERR_UNSUPPORTED, "swarm not built yet"}; });
+int aurora_hip_swarm_gas(aurora_hip_ctx* h, int nchan, const int* kind, const double* threshold,
+                         const int64_t* rows, const double* const* energy,
+                         const double* const* sigma, double neutral_density,
+                         double max_frequency, int64_t max_candidates, double neutral_mass) {
+    return guarded(h, [&](Context& c) {
+        SwarmState& s = state(c);
+        AUR_REQUIRE(s.active.empty(), AURORA_ERR_INVALID_ARGUMENT,
+                    "the swarm gas cannot change while requests are running");
+        AUR_REQUIRE(std::isfinite(neutral_mass) && neutral_mass > 0.0,
+                    AURORA_ERR_INVALID_ARGUMENT, "gas dataset requires positive neutral_mass");
+        bool elastic = false;
+        for (int k = 0; k < nchan; ++k) {
+            elastic = elastic || kind[k] == AURORA_ELASTIC;
+            AUR_REQUIRE(kind[k] != AURORA_ATTACHMENT, AURORA_ERR_UNSUPPORTED,
+                        "fixed-population swarm does not support attachment; use "
+                        "population_model = branching_resampled");
+        }
+        AUR_REQUIRE(elastic, AURORA_ERR_INVALID_ARGUMENT,
+                    "electron swarm benchmark requires an elastic channel");
+        // model checks and tables (shared with the 1D MCC model loader)
+        MccModelHost& g = s.gas;
+        auto fin_pos = [](double v) { return std::isfinite(v) && v > 0.0; };
+        AUR_REQUIRE(fin_pos(neutral_density), AURORA_ERR_INVALID_ARGUMENT,
+                    "MCC neutral_density must be positive and finite");
+        AUR_REQUIRE(fin_pos(max_frequency), AURORA_ERR_INVALID_ARGUMENT,
+                    "MCC max_frequency must be positive and finite");
+        AUR_REQUIRE(max_candidates > 0, AURORA_ERR_INVALID_ARGUMENT,
+                    "MCC max_candidates_per_particle must be positive");
+        AUR_REQUIRE(nchan > 0 && nchan <= kMaxChannels, AURORA_ERR_INVALID_ARGUMENT,
+                    "MCC requires at least one collision channel");
+        std::vector<float> e32, s32;
+        int64_t off = 0;
+        for (int k = 0; k < nchan; ++k) {
+            AUR_REQUIRE(rows[k] >= 2, AURORA_ERR_INVALID_ARGUMENT,
+                        "collision cross-section table requires at least two rows");
+            for (int64_t i = 0; i < rows[k]; ++i) {
+                AUR_REQUIRE(i == 0 || energy[k][i] > energy[k][i - 1],
+                            AURORA_ERR_INVALID_ARGUMENT,
+                            "collision table energies must be strictly increasing");
+                e32.push_back(static_cast<float>(energy[k][i]));
+                s32.push_back(static_cast<float>(sigma[k][i]));
+            }
+            g.kind[k] = kind[k];
+            g.threshold[k] = threshold[k];
+            g.rows[k] = rows[k];
+            g.row_offset[k] = off;
+            off += rows[k];
+        }
+        g.nchan = nchan;
+        g.shared_grid = true;
+        for (int k = 1; k < nchan; ++k) {
+            if (rows[k] != rows[0]) { g.shared_grid = false; break; }
+            for (int64_t i = 0; i < rows[k]; ++i)
+                if (static_cast<float>(energy[k][i]) != static_cast<float>(energy[0][i])) {
+                    g.shared_grid = false;
+                    break;
+                }
+        }
+        g.neutral_density = neutral_density;
+        g.max_frequency = max_frequency;
+        g.max_candidates = max_candidates;
+        g.neutral_mass = neutral_mass;
+        g.particle_mass = kElectronMass;
+        g.energy.alloc(e32.size());
+        g.sigma.alloc(s32.size());
+        AUR_HIP(hipMemcpy(g.energy.ptr, e32.data(), e32.size() * sizeof(float), hipMemcpyHostToDevice));
+        AUR_HIP(hipMemcpy(g.sigma.ptr, s32.data(), s32.size() * sizeof(float), hipMemcpyHostToDevice));
+        // energy ceiling of the tables (Swarm.cpp:517-526), eV
+        double emax_ev = 1e300;
+        for (int k = 0; k < nchan; ++k)
+            emax_ev = std::min(emax_ev, energy[k][rows[k] - 1] / kElementaryCharge);
+        s.table_max_ev = emax_ev;
+        s.gas_loaded = true;
+    });
 }
-int aurora_hip_swarm_submit(aurora_hip_ctx* h, const aurora_hip_swarm_request*, uint64_t, int64_t*) {
-    return guarded(h, [](Context&) { throw Failure{AURORA_ERR_UNSUPPORTED, "swarm not built yet"}; });
-}
-int aurora_hip_swarm_advance(aurora_hip_ctx* h, int64_t, int64_t*) {
-    return guarded(h, [](Context&) { throw Failure{AURORA_ERR_UNSUPPORTED, "swarm not built yet"}; });
-}
-int aurora_hip_swarm_poll(aurora_hip_ctx* h, int64_t, int*, aurora_hip_swarm_result*) {
-    return guarded(h, [](Context&) { throw Failure{AURORA_ERR_UNSUPPORTED, "swarm not built yet"}; });
-}
+
 uint64_t aurora_hip_swarm_footprint(const aurora_hip_swarm_request* r) {
-    return r ? static_cast<uint64_t>(r->particles) * 12u : 0u;
+    return r ? footprint_of(*r) : 0;
+}
+
+int aurora_hip_swarm_submit(aurora_hip_ctx* h, const aurora_hip_swarm_request* r,
+                            uint64_t hbm_budget, int64_t* handle) {
+    return guarded(h, [&](Context& c) {
+        SwarmState& s = state(c);
+        AUR_REQUIRE(s.gas_loaded, AURORA_ERR_INVALID_ARGUMENT, "load the swarm gas first");
+        AUR_REQUIRE(r, AURORA_ERR_INVALID_ARGUMENT, "null request");
+        // validate_config (Swarm.cpp:277-559)
+        AUR_REQUIRE(std::isfinite(r->timestep) && r->timestep > 0.0, AURORA_ERR_INVALID_ARGUMENT,
+                    "swarm timestep must be positive and finite");
+        AUR_REQUIRE(s.gas.max_frequency * r->timestep <= 0.1, AURORA_ERR_INVALID_ARGUMENT,
+                    "swarm max_frequency * timestep must not exceed 0.1");
+        AUR_REQUIRE(r->steps > 0 && r->burn_in < r->steps, AURORA_ERR_INVALID_ARGUMENT,
+                    "swarm steps must exceed burn_in_steps");
+        AUR_REQUIRE(r->steps < 2147483647, AURORA_ERR_INVALID_ARGUMENT, "swarm steps too large");
+        AUR_REQUIRE(r->particles > 0, AURORA_ERR_INVALID_ARGUMENT,
+                    "swarm particles must be positive");
+        AUR_REQUIRE(r->blocks > 0 && r->blocks <= r->steps - r->burn_in,
+                    AURORA_ERR_INVALID_ARGUMENT,
+                    "swarm block statistics require at least one sample per block");
+        AUR_REQUIRE(std::isfinite(r->reduced_field_td) && r->reduced_field_td >= 0.0,
+                    AURORA_ERR_INVALID_ARGUMENT,
+                    "swarm reduced fields must be finite and non-negative");
+        AUR_REQUIRE(std::isfinite(r->initial_mean_energy_ev) && r->initial_mean_energy_ev >= 0.0,
+                    AURORA_ERR_INVALID_ARGUMENT,
+                    "swarm initial_mean_energy_ev must be finite and non-negative");
+        AUR_REQUIRE(std::isfinite(r->max_energy_ev) && r->max_energy_ev > 0.0,
+                    AURORA_ERR_INVALID_ARGUMENT, "swarm max_energy_ev must be positive and finite");
+        AUR_REQUIRE(r->max_energy_ev <= s.table_max_ev, AURORA_ERR_INVALID_ARGUMENT,
+                    "swarm max_energy_ev exceeds channel table coverage");
+        const uint64_t fp = footprint_of(*r);
+        if (hbm_budget && s.bytes_in_use + fp > hbm_budget)
+            throw Failure{AURORA_ERR_CAPACITY, "swarm request does not fit the HBM budget now"};
+        auto st = std::make_unique<RequestState>();
+        st->req = *r;
+        st->chunks = (r->particles + kChunk - 1) / kChunk;
+        st->footprint = fp;
+        st->v.alloc(static_cast<size_t>(3 * r->particles));
+        st->part.alloc(static_cast<size_t>(st->chunks * r->blocks * 2));
+        st->counts.alloc(kMaxChannels + 3);
+        st->maxe.alloc(1);
+        st->status.alloc(1);
+        AUR_HIP(hipMemsetAsync(st->part.ptr, 0, st->part.count * sizeof(long long), c.stream));
+        AUR_HIP(hipMemsetAsync(st->counts.ptr, 0, (kMaxChannels + 3) * 8, c.stream));
+        AUR_HIP(hipMemsetAsync(st->maxe.ptr, 0, 4, c.stream));
+        AUR_HIP(hipMemsetAsync(st->status.ptr, 0, 4, c.stream));
+        const double sigma =
+            std::sqrt(2.0 * r->initial_mean_energy_ev * kElementaryCharge / (3.0 * kElectronMass));
+        const int64_t n = r->particles;
+        int64_t g = (n + 255) / 256;
+        if (g > 4096) g = 4096;
+        k_swarm_init<<<static_cast<unsigned>(g), 256, 0, c.stream>>>(
+            st->v.ptr, st->v.ptr + n, st->v.ptr + 2 * n, n, static_cast<float>(sigma), r->seed);
+        AUR_HIP(hipGetLastError());
+        s.bytes_in_use += fp;
+        const int64_t id = s.next_handle++;
+        s.active[id] = std::move(st);
+        if (handle) *handle = id;
+    });
+}
+
+int aurora_hip_swarm_advance(aurora_hip_ctx* h, int64_t max_steps, int64_t* active_out) {
+    return guarded(h, [&](Context& c) {
+        SwarmState& s = state(c);
+        AUR_REQUIRE(max_steps > 0, AURORA_ERR_INVALID_ARGUMENT, "max_steps must be positive");
+        std::vector<RequestDesc> descs;
+        std::vector<WorkItem> items;
+        std::vector<std::pair<int64_t, RequestState*>> launched;
+        for (auto& kv : s.active) {
+            RequestState& r = *kv.second;
+            const auto& q = r.req;
+            const int64_t remaining = q.steps - r.steps_done;
+            if (remaining <= 0) continue;
+            const int64_t ns = std::min(max_steps, remaining);
+            RequestDesc d{};
+            const int64_t n = q.particles;
+            d.vx = r.v.ptr;
+            d.vy = r.v.ptr + n;
+            d.vz = r.v.ptr + 2 * n;
+            d.part = r.part.ptr;
+            d.counts = r.counts.ptr;
+            d.max_energy_bits = r.maxe.ptr;
+            d.status = r.status.ptr;
+            d.particles = n;
+            const double efield = q.reduced_field_td * kTownsend * s.gas.neutral_density;
+            const double accel = -kElementaryCharge * efield / kElectronMass;
+            d.half_kick = static_cast<float>(0.5 * accel * q.timestep);
+            d.dt = static_cast<float>(q.timestep);
+            d.max_energy_ev = static_cast<float>(q.max_energy_ev);
+            d.ev_per_v2 = static_cast<float>(0.5 * kElectronMass / kElementaryCharge);
+            d.step0 = static_cast<int>(r.steps_done);
+            d.nsteps = static_cast<int>(ns);
+            d.burn_in = static_cast<int>(q.burn_in);
+            d.sampling = static_cast<int>(q.steps - q.burn_in);
+            d.blocks = static_cast<int>(q.blocks);
+            d.seed = q.seed;
+            const int ridx = static_cast<int>(descs.size());
+            descs.push_back(d);
+            for (int64_t ch = 0; ch < r.chunks; ++ch)
+                items.push_back(WorkItem{ridx, static_cast<int>(ch)});
+            launched.push_back({kv.first, &r});
+        }
+        if (!items.empty()) {
+            if (s.descs.count < descs.size()) s.descs.alloc(descs.size() * 2);
+            if (s.items.count < items.size()) s.items.alloc(items.size() * 2);
+            if (!s.head.ptr) s.head.alloc(1);
+            AUR_HIP(hipMemcpyAsync(s.descs.ptr, descs.data(), descs.size() * sizeof(RequestDesc),
+                                   hipMemcpyHostToDevice, c.stream));
+            AUR_HIP(hipMemcpyAsync(s.items.ptr, items.data(), items.size() * sizeof(WorkItem),
+                                   hipMemcpyHostToDevice, c.stream));
+            AUR_HIP(hipMemsetAsync(s.head.ptr, 0, sizeof(int), c.stream));
+            const MccParams m = swarm_params(s);
+            int64_t grid = static_cast<int64_t>(c.cu_count) * 4;
+            if (grid > static_cast<int64_t>(items.size())) grid = static_cast<int64_t>(items.size());
+            {
+                KernelScope ks(c, "swarm");
+                k_swarm<<<static_cast<unsigned>(grid), kChunk, 0, c.stream>>>(
+                    m, s.descs.ptr, s.items.ptr, static_cast<int>(items.size()), s.head.ptr);
+                AUR_HIP(hipGetLastError());
+            }
+            // the host arrays are reused next call: wait for the copies
+            AUR_HIP(hipStreamSynchronize(c.stream));
+        }
+        for (auto& lr : launched) {
+            RequestState& r = *lr.second;
+            r.steps_done += std::min(max_steps, r.req.steps - r.steps_done);
+            if (r.steps_done >= r.req.steps) {
+                finalize(c, s, lr.first, r);
+                s.active.erase(lr.first);
+            }
+        }
+        if (active_out) *active_out = static_cast<int64_t>(s.active.size());
+        flush_timers(c);
+    });
+}
+
+int aurora_hip_swarm_poll(aurora_hip_ctx* h, int64_t handle, int* done,
+                          aurora_hip_swarm_result* result) {
+    return guarded(h, [&](Context& c) {
+        SwarmState& s = state(c);
+        auto it = s.finished.find(handle);
+        if (it != s.finished.end()) {
+            if (done) *done = 1;
+            if (result) *result = it->second;
+            s.finished.erase(it);
+            return;
+        }
+        AUR_REQUIRE(s.active.count(handle), AURORA_ERR_NOT_FOUND, "unknown swarm request handle");
+        if (done) *done = 0;
+    });
 }
 
 }  // extern "C"

@@ -82,6 +82,10 @@ def main():
     ap.add_argument("--cpu-ppc", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tile", default=None,
+                    help="fixed tile shape tx,ty,tz,halo (skips the autotuner)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the per-step loop of one 3D3V instance runs on a "
@@ -92,6 +96,14 @@ def main():
     from aurora_pic import hip
     from aurora_pic import workloads as wl
 
+    if args.copy_calibration:
+        import torch
+        x = torch.empty(2 ** 28, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB write")
+        del x, y
     cfg = wl.config_plasma_3d(cells=args.cells, ppc=args.ppc)
     dt = cfg["dt"]
     ctx = hip.Context(0)
@@ -110,7 +122,13 @@ def main():
     ctx.initialize(dt)
     ctx.sort_interval(cfg["sort_interval"])
     tuned = None
-    if not args.no_autotune:
+    if args.tile:
+        shape = [int(v) for v in args.tile.split(",")]
+        ctx.tiling(*shape)
+        ctx.sort()
+        tuned = {"tile": shape[:3], "halo": shape[3], "sort_interval": cfg["sort_interval"],
+                 "fixed": True}
+    elif not args.no_autotune:
         from aurora_pic import autotune
         best, table = autotune.tune(ctx, dt, sort_intervals=(cfg["sort_interval"],), log=log)
         tuned = {"tile": list(best[1][:3]), "halo": best[1][3], "sort_interval": best[2],

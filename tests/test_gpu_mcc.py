@@ -1,0 +1,186 @@
+"""GPU parity of the null-collision MCC (FP32, Philox uniforms) against the
+FP64 oracle fed the same uniform stream, the reference's closed forms, and
+the reference's own statistics (tests/golden/mcc_stats.npz).
+
+Bar: per particle, velocities agree to 1e-4 relative for >= 99 % of the
+particles (the remainder are branch flips at a waiting-time or channel
+boundary, where FP32 and FP64 round to different sides); aggregate counts
+agree within 5 Poisson sigma."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_py
+from aurora_pic import workloads
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def ctx():
+    from aurora_pic import hip
+    c = hip.Context(0)
+    c.mesh([17], [1.0], 1, 1.0)
+    yield c
+    c.close()
+
+
+def _load(ctx, channels, density, max_frequency, mass, max_candidates=64, neutral_mass=0.0,
+          seed=8128):
+    sid = ctx.add_species(-1.0, mass, 1.0, 64)
+    extra = ctx.add_species(1.0, 1000.0, 1.0, 64)
+    gas = dict(channels=channels, neutral_density=density, max_frequency=max_frequency,
+               neutral_mass=neutral_mass, max_candidates=max_candidates)
+    ctx.mcc_load(sid, gas, product_species=(sid, extra), seed=seed)
+    return gas
+
+
+def test_collide_matches_oracle_same_stream(ctx, oracle):
+    g = golden("mcc_stats.npz")
+    gas = workloads.synthetic_gas(rows=200)
+    model = workloads.synthetic_mcc_model(rows=200)
+    _load(ctx, gas["channels"], gas["neutral_density"], gas["max_frequency"], workloads.ME,
+          neutral_mass=gas["neutral_mass"], seed=8128)
+    vin = g["vin"].astype(np.float64)
+    dt = float(g["dt"])
+    vout, counts, ech = ctx.mcc_collide(vin, dt, step=0)
+    ocounts = np.zeros(6, dtype=np.int64)
+    close = 0
+    for p in range(vin.shape[0]):
+        ov, st = oracle.collide(model, vin[p], dt, 8128, 0, p, 0)
+        ocounts += [st.candidates, st.nulls, st.chan[0], st.chan[1], st.chan[2], st.nsec]
+        if np.max(np.abs(vout[p] - ov)) <= 1e-4 * np.linalg.norm(ov) + 1e-3:
+            close += 1
+    assert close >= 0.99 * vin.shape[0], close
+    for k in range(6):
+        assert abs(int(counts[k]) - int(ocounts[k])) <= 5 * math.sqrt(max(ocounts[k], 1)) + 2
+    # and against the reference's own mt19937_64 statistics on the same inputs
+    ref = g["counts"]
+    for k in range(6):
+        assert abs(int(counts[k]) - int(ref[k])) <= 5 * math.sqrt(max(ref[k], 1)) + 2, k
+
+
+def test_elastic_rates_and_energy_conservation(ctx):
+    # test_core.cpp:353-367 closed forms on the device
+    e, s = [0.0, 1.0, 2.0], [0.0, 0.5, 1.0]
+    _load(ctx, [(workloads.ELASTIC, 0.0, e, s)], 1.0, 2.0, 1.0)
+    v = np.tile([math.sqrt(2.0), 0.0, 0.0], (10000, 1))
+    vout, counts, _ = ctx.mcc_collide(v, 0.1, step=3)
+    assert 1800 < counts[0] < 2200
+    assert 620 < counts[2] < 800
+    assert np.max(np.abs(np.linalg.norm(vout, axis=1) - math.sqrt(2.0))) < 1e-6
+
+
+def test_excitation_removes_threshold(ctx):
+    e, s = [0.0, 1.0, 2.0], [0.0, 1.0, 2.0]
+    _load(ctx, [(workloads.EXCITATION, 0.5, e, s)], 1.0, 2.0, 1.0)
+    v = np.tile([2.0, 0.0, 0.0], (200, 1))
+    vout, counts, ech = ctx.mcc_collide(v, 2.0, step=1)
+    e1 = 0.5 * np.sum(vout ** 2, axis=1)
+    loss = 2.0 - e1
+    assert counts[2] > 0
+    assert abs(loss.sum() - 0.5 * counts[2]) < 1e-4 * counts[2]
+    assert abs(ech[0] + loss.sum()) < 1e-4 * counts[2]
+
+
+def test_ionization_counts_secondaries(ctx):
+    e, s = [0.0, 1.0, 2.0], [0.0, 0.5, 1.0]
+    _load(ctx, [(workloads.IONIZATION, 0.5, e, s)], 1.0, 10.0, 1.0, max_candidates=256)
+    v = np.tile([4.0, 0.0, 0.0], (500, 1))
+    vout, counts, ech = ctx.mcc_collide(v, 1.0, step=0)
+    assert counts[2] > 0 and counts[3] == counts[2]      # one secondary per event
+
+
+def test_rate_above_bound_error(ctx):
+    from aurora_pic import hip
+    e, s = [0.0, 1.0, 2.0], [0.0, 0.5, 1.0]
+    _load(ctx, [(workloads.ELASTIC, 0.0, e, s)], 1.0, 0.1, 1.0)
+    with pytest.raises(hip.HipError, match="exceeds configured max_frequency"):
+        ctx.mcc_collide(np.tile([math.sqrt(2.0), 0, 0], (10, 1)), 0.1, step=0)
+
+
+def _oracle_step_1d(oracle, model, state, n, L, eps, dt, t1, phi_r, qm, w, seed, step):
+    """One reference-semantics 1D step (Simulation.cpp:1008-1138) in FP64:
+    push all species, deposit, Dirichlet solve at t+dt, then MCC on
+    species 0 with products appended (secondary -> species 0, ion -> 1)."""
+    for s, st in enumerate(state):
+        st["pos"], st["vh"], st["alive"], _ = oracle.push([n], [L], 1, st["pos"], st["vh"],
+                                                         st["alive"], st["E"], qm[s], dt)
+    rho = np.zeros(n)
+    for s, st in enumerate(state):
+        rho += oracle.deposit_cic([n], [L], 1, st["pos"], qm[s] * 0 + st["q"], w[s], st["alive"])
+    _, E = oracle.poisson([n], [L], 1, eps, rho, 0.0, phi_r)
+    e0 = state[0]
+    Ep = oracle.gather([n], [L], 1, e0["pos"], E[0])
+    products = []
+    for p in range(len(e0["alive"])):
+        if not e0["alive"][p]:
+            continue
+        v = e0["vh"][p].copy()
+        v[0] += 0.5 * qm[0] * Ep[p] * dt
+        v, st = oracle.collide(model, v, dt, seed, 0, int(e0["uid"][p]), step)
+        for k in range(st.nsec):
+            products.append((p, np.array(st.sec_v[k]), np.array(st.ion_v[k])))
+        v[0] -= 0.5 * qm[0] * Ep[p] * dt
+        e0["vh"][p] = v
+    return E, rho, products
+
+
+def test_step_1d_with_mcc_matches_oracle(oracle):
+    """Config-[1]-shaped 1D3V RF discharge step: push, deposit, Thomas solve
+    with the RF electrode at t+dt, MCC with ionization products."""
+    from aurora_pic import hip
+    rng = np.random.default_rng(5)
+    n, L = 65, 0.067
+    eps = workloads.EPS0
+    dt = 1.0 / (400 * 13.56e6)
+    N = 3000
+    gas = workloads.synthetic_gas(rows=1000, neutral_density=1.0e22)
+    model = workloads.synthetic_mcc_model(rows=1000, neutral_density=1.0e22)
+    species = [(-workloads.EV, workloads.ME, 8.0), (workloads.EV, 6.67e-27, 0.026)]
+    w = [2.56e14 * L / N] * 2
+    state = []
+    c = hip.Context(0)
+    c.mesh([n], [L], 1, eps)
+    c.electrodes([0, 0, 0, 0], [0.0, 200.0, 13.56e6, 0.0])
+    sids = []
+    for q, m, te in species:
+        sid = c.add_species(q, m, w[0], 4 * N)
+        pos = rng.uniform(0.1 * L, 0.9 * L, size=N)
+        vel = rng.normal(0, math.sqrt(te * workloads.EV / m), size=(N, 3))
+        c.upload(sid, pos, vel)
+        sids.append(sid)
+    c.mcc_load(sids[0], gas, product_species=(sids[0], sids[1]), seed=77)
+    c.deposit()
+    c.solve(0.0, 0.0, True)
+    for sid, (q, m, te) in zip(sids, species):
+        p, v, a, u = c.download(sid)
+        state.append(dict(pos=p[:, 0].copy(), vh=v.copy(), alive=a.copy(), uid=u.copy(), q=q))
+    _, _, E0 = c.fields()
+    for st in state:
+        st["E"] = E0[0]
+    c.step(dt, 1)
+    qm = [q / m for q, m, _ in species]
+    phi_r = 200.0 * math.sin(2 * math.pi * 13.56e6 * dt)
+    E1, rho1, products = _oracle_step_1d(oracle, model, state, n, L, eps, dt, dt, phi_r, qm, w,
+                                         77, 0)
+    rg, _, Eg = c.fields()
+    assert np.max(np.abs(Eg[0] - E1[0])) <= 1e-6 * np.max(np.abs(E1[0]))
+    # electrons: the original particles keep their slots; products follow
+    p, v, a, u = c.download(sids[0])
+    e0 = state[0]
+    keep = e0["alive"] == 1
+    close = np.max(np.abs(v[:N][keep] - e0["vh"][keep]), axis=1) <= (
+        1e-4 * np.linalg.norm(e0["vh"][keep], axis=1) + 1.0)
+    assert close.mean() >= 0.99
+    slots_e, _ = c.count(sids[0])
+    slots_i, _ = c.count(sids[1])
+    assert abs((slots_e - N) - len(products)) <= max(2, 0.05 * len(products))
+    assert slots_i - N == slots_e - N
+    counts, ech = c.mcc_stats()
+    assert counts[2 + 2] == slots_e - N          # ionizations = products
+    # product uids continue after the uploaded ones, in parent order
+    assert np.array_equal(u[N:slots_e], np.arange(N, slots_e, dtype=np.uint32))
+    c.close()

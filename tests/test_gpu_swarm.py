@@ -1,0 +1,150 @@
+"""GPU batched swarms: closed form, parity with the oracle (same Philox
+streams) and the reference (statistical), and identical results alone and
+in a mix (continuous batching)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_py
+from aurora_pic import workloads
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def ctx():
+    from aurora_pic import hip
+    c = hip.Context(0)
+    yield c
+    c.close()
+
+
+def _request(td, particles, steps, burn, blocks, dt, seed, e0=1.0, emax=900.0):
+    from aurora_pic import hip
+    return hip.SwarmRequest(reduced_field_td=td, timestep=dt, particles=particles, steps=steps,
+                            burn_in=burn, blocks=blocks, initial_mean_energy_ev=e0,
+                            max_energy_ev=emax, seed=seed)
+
+
+def _run_all(ctx, requests, chunk):
+    handles = [ctx.swarm_submit(r) for r in requests]
+    results = {}
+    while len(results) < len(handles):
+        ctx.swarm_advance(chunk)
+        for h in handles:
+            if h in results:
+                continue
+            done, r = ctx.swarm_poll(h)
+            if done:
+                results[h] = r
+    return [results[h] for h in handles]
+
+
+def test_swarm_zero_cross_section_closed_form(ctx):
+    e = np.array([0.0, 1.0e-16])
+    gas = dict(channels=[(workloads.ELASTIC, 0.0, e, np.zeros(2))], neutral_density=1e20,
+               max_frequency=1e7, neutral_mass=workloads.HELIUM_MASS)
+    ctx.swarm_gas(gas)
+    r = _run_all(ctx, [_request(1.0, 300, 40, 10, 5, 1e-9, 3, e0=0.0, emax=600.0)], 64)[0]
+    a_dt = workloads.EV * 1.0e-21 * 1e20 / workloads.ME * 1e-9
+    s1 = np.arange(11, 41, dtype=float)
+    assert r.status == 0
+    # FP32 velocity accumulation and 1/16 m/s statistics quantisation
+    assert abs(r.drift - a_dt * s1.mean()) <= 1e-5 * a_dt * s1.mean() + 0.07
+    e_exp = 0.5 * workloads.ME * a_dt ** 2 * np.mean(s1 ** 2) / workloads.EV
+    assert abs(r.mean_energy_ev - e_exp) <= 1e-5 * e_exp
+    assert r.collisions[0] == 0
+
+
+def _initial_velocities(oracle, n, e0, seed):
+    """The device's initial Maxwellian (Box-Muller on the Philox stream at
+    step slot 0xFFFFFFFF), restated in FP64."""
+    sigma = math.sqrt(2.0 * e0 * workloads.EV / (3.0 * workloads.ME))
+    v = np.zeros((n, 3))
+    for p in range(n):
+        s = oracle.stream(seed, 0, p, 0xFFFFFFFF)
+        u1, u2, u3, u4 = [oracle.stream_u01(s) for _ in range(4)]
+        r1, r2 = math.sqrt(-2 * math.log(u1)), math.sqrt(-2 * math.log(u3))
+        v[p] = [sigma * r1 * math.cos(2 * math.pi * u2), sigma * r1 * math.sin(2 * math.pi * u2),
+                sigma * r2 * math.cos(2 * math.pi * u4)]
+    return v
+
+
+def test_swarm_matches_oracle_same_stream(ctx, oracle):
+    gas = workloads.synthetic_gas(rows=200)
+    model = workloads.synthetic_mcc_model(rows=200)
+    ctx.swarm_gas(gas)
+    dt = 0.099 / gas["max_frequency"]
+    n, steps, burn, blocks, seed = 512, 1200, 400, 8, 4242
+    r = _run_all(ctx, [_request(300.0, n, steps, burn, blocks, dt, seed)], 300)[0]
+    vel0 = _initial_velocities(oracle, n, 1.0, seed)
+    cfg = dict(neutral_density=gas["neutral_density"], max_frequency=gas["max_frequency"],
+               timestep=dt, reduced_field_td=300.0, steps=steps, burn_in=burn, particles=n,
+               blocks=blocks, initial_mean_energy_ev=1.0, max_energy_ev=900.0)
+    o = oracle.swarm(model, cfg, vel0, seed, 0)
+    assert r.status == 0 and o.status == 0
+    for mine, se, theirs, tse in [(r.drift, r.drift_se, o.drift, o.drift_se),
+                                  (r.mean_energy_ev, r.mean_energy_se, o.mean_energy_ev,
+                                   o.mean_energy_se)]:
+        assert abs(mine - theirs) <= 4 * math.hypot(se, tse) + 0.01 * abs(theirs)
+    assert abs(r.ionization_rate - o.ion_rate) <= 5 * math.hypot(r.ionization_rate_se,
+                                                                 o.ion_rate_se) + 1.0
+
+
+def test_swarm_matches_reference_golden(ctx):
+    g = golden("swarm_300td.npz")
+    gas = workloads.synthetic_gas(rows=200)
+    ctx.swarm_gas(gas)
+    req = _request(float(g["reduced_field_td"]), int(g["particles"]), int(g["steps"]),
+                   int(g["burn_in"]), int(g["blocks"]), float(g["timestep"]), 99,
+                   e0=float(g["initial_mean_energy_ev"]), emax=float(g["max_energy_ev"]))
+    r = _run_all(ctx, [req], 1000)[0]
+    ref = g["out"]
+    assert r.status == 0
+    for mine, se, theirs, tse in [(r.drift, r.drift_se, ref[0], ref[1]),
+                                  (r.mean_energy_ev, r.mean_energy_se, ref[2], ref[3])]:
+        assert abs(mine - theirs) <= 5 * math.hypot(se, tse) + 0.02 * abs(theirs)
+    assert abs(r.ionization_rate - ref[4]) <= 5 * math.hypot(r.ionization_rate_se, ref[5]) + 1.0
+
+
+def test_swarm_alone_equals_mixed():
+    """Continuous batching: a request's result is bit-identical alone and
+    mixed with others of different sizes, advanced in different launch
+    chunks (the statistics are integer sums)."""
+    from aurora_pic import hip
+    gas = workloads.synthetic_gas(rows=200)
+    dt = 0.099 / gas["max_frequency"]
+    target = _request(100.0, 1000, 600, 200, 10, dt, 11)
+    others = [_request(30.0, 3000, 900, 300, 10, dt, 12), _request(700.0, 257, 350, 50, 5, dt, 13)]
+    with hip.Context(0) as a:
+        a.swarm_gas(gas)
+        alone = _run_all(a, [target], 600)[0]
+    with hip.Context(0) as b:
+        b.swarm_gas(gas)
+        mixed = _run_all(b, [others[0], target, others[1]], 77)[1]
+    for f in ["drift", "drift_se", "mean_energy_ev", "mean_energy_se", "ionization_rate",
+              "candidates", "nulls", "max_energy_ev"]:
+        assert getattr(alone, f) == getattr(mixed, f), f
+    assert list(alone.collisions) == list(mixed.collisions)
+
+
+def test_swarm_admission_budget(ctx):
+    from aurora_pic import hip
+    gas = workloads.synthetic_gas(rows=200)
+    ctx.swarm_gas(gas)
+    dt = 0.099 / gas["max_frequency"]
+    r = _request(10.0, 10000, 100, 10, 5, dt, 1)
+    fp = ctx.swarm_footprint(r)
+    h1 = ctx.swarm_submit(r, hbm_budget=fp + 10)
+    with pytest.raises(hip.HipError, match="HBM budget"):
+        ctx.swarm_submit(r, hbm_budget=fp + 10)
+    while True:
+        ctx.swarm_advance(100)
+        done, res = ctx.swarm_poll(h1)
+        if done:
+            break
+    ctx.swarm_submit(r, hbm_budget=fp + 10)    # memory freed: admitted now
+    with pytest.raises(hip.HipError, match="max_frequency \\* timestep"):
+        ctx.swarm_submit(_request(10.0, 100, 100, 10, 5, 1.0, 1))
