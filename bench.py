@@ -168,10 +168,12 @@ def main():
     ctx.close()
 
     traffic = None
-    tfile = os.path.join(REPO, "profiles", "push_deposit_plasma3d_traffic.json")
+    tfile = os.path.join(REPO, "profiles", "push_deposit_plasma3d_traffic.txt")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            for line in f:
+                if line.startswith("bytes_per_launch:"):
+                    traffic = int(line.split(":", 1)[1])
 
     base = None
     if not args.no_cpu_baseline:

@@ -75,7 +75,7 @@ def test_elastic_rates_and_energy_conservation(ctx):
 
 def test_excitation_removes_threshold(ctx):
     e, s = [0.0, 1.0, 2.0], [0.0, 1.0, 2.0]
-    _load(ctx, [(workloads.EXCITATION, 0.5, e, s)], 1.0, 2.0, 1.0)
+    _load(ctx, [(workloads.EXCITATION, 0.5, e, s)], 1.0, 4.0, 1.0)
     v = np.tile([2.0, 0.0, 0.0], (200, 1))
     vout, counts, ech = ctx.mcc_collide(v, 2.0, step=1)
     e1 = 0.5 * np.sum(vout ** 2, axis=1)

@@ -76,13 +76,17 @@ def test_elastic_candidate_and_acceptance_rates(oracle):
 def test_excitation_removes_threshold(oracle):
     # test_core.cpp:596-634
     m = _unit_table_model(workloads.EXCITATION, threshold=0.5, sigma_scale=2.0,
-                          max_frequency=2.0)
+                          max_frequency=4.0)
+    accepted = 0
     for p in range(50):
         v, st = oracle.collide(m, [2.0, 0.0, 0.0], 2.0, 77, 0, p, 0)
+        assert st.status == 0
+        accepted += st.chan[0]
         e0 = 0.5 * 4.0
         e1 = 0.5 * float(np.dot(v, v))
         assert abs((e0 - e1) - 0.5 * st.chan[0]) < 1e-13
         assert abs(st.energy_change[0] - (e1 - e0)) < 1e-13
+    assert accepted > 0
 
 
 def test_ionization_energy_partition_closes(oracle):
@@ -92,6 +96,7 @@ def test_ionization_energy_partition_closes(oracle):
     events = 0
     for p in range(40):
         v, st = oracle.collide(m, [4.0, 0.0, 0.0], 10.0, 314159, 0, p, 0)
+        assert st.status == 0
         if st.nsec > 8:
             continue
         prod = 0.5 * float(np.dot(v, v))
