@@ -91,6 +91,12 @@ void locate_host(double x, int64_t n, double L, double d, int bc, int32_t& cell,
     }
 }
 
+int64_t unpack_axis(int32_t c, int dim, int a) {
+    if (dim == 1) return c;
+    const int shift = pack_shift(dim);
+    return (c >> (shift * a)) & ((1 << shift) - 1);
+}
+
 template <typename R>
 void upload_species(Context& ctx, Species& sp, int64_t count, const double* pos,
                     const double* vel, const uint8_t* alive) {
@@ -109,16 +115,16 @@ void upload_species(Context& ctx, Species& sp, int64_t count, const double* pos,
             cell[p] = -1;
             continue;
         }
-        int64_t lin = 0, stride = 1;
+        int64_t packed = 0;
+        const int shift = pack_shift(dim);
         for (int a = 0; a < dim; ++a) {
             int32_t c;
             R o;
             locate_host<R>(pos[dim * p + a], ctx.n[a], ctx.L[a], ctx.d[a], ctx.bc, c, o);
             off[a][p] = o;
-            lin += static_cast<int64_t>(c) * stride;
-            stride *= ctx.n[a];
+            packed |= static_cast<int64_t>(c) << (shift * a);
         }
-        cell[p] = static_cast<int32_t>(lin);
+        cell[p] = static_cast<int32_t>(packed);
     }
     AUR_HIP(hipMemcpy(sp.cell.ptr, cell.data(), count * sizeof(int32_t), hipMemcpyHostToDevice));
     AUR_HIP(hipMemcpy(sp.uid.ptr, uid.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -152,11 +158,9 @@ void download_species(Context& ctx, Species& sp, double* pos, double* vel, uint8
         for (int a = 0; a < dim; ++a) {
             AUR_HIP(hipMemcpy(buf.data(), sp.off[a].ptr, count * sizeof(R),
                               hipMemcpyDeviceToHost));
-            int64_t stride = 1;
-            for (int b = 0; b < a; ++b) stride *= ctx.n[b];
             for (int64_t p = 0; p < count; ++p) {
                 if (cell[p] < 0) { pos[dim * p + a] = 0.0; continue; }
-                const int64_t idx = (cell[p] / stride) % ctx.n[a];
+                const int64_t idx = unpack_axis(cell[p], dim, a);
                 pos[dim * p + a] =
                     (static_cast<double>(idx) + static_cast<double>(buf[p])) * ctx.d[a];
             }
@@ -258,6 +262,9 @@ int aurora_hip_mesh_bind(aurora_hip_ctx* h, int dim, const int64_t* n, const dou
         }
         AUR_REQUIRE(nodes < (int64_t{1} << 31), AURORA_ERR_UNSUPPORTED,
                     "mesh node count exceeds the int32 cell index of this build");
+        for (int a = 0; a < dim; ++a)
+            AUR_REQUIRE(dim == 1 || n[a] <= (int64_t{1} << pack_shift(dim)), AURORA_ERR_UNSUPPORTED,
+                        "mesh extent exceeds the packed cell index (3D: 1024, 2D: 32768 per axis)");
         AUR_HIP(hipSetDevice(c.device));
         AUR_HIP(hipStreamSynchronize(c.stream));
         c.species.clear();
@@ -406,7 +413,19 @@ int aurora_hip_species_download_raw(aurora_hip_ctx* h, int sid, int64_t capacity
         const int64_t n = sp.slots;
         if (n == 0) return;
         AUR_HIP(hipStreamSynchronize(c.stream));
-        if (cell) AUR_HIP(hipMemcpy(cell, sp.cell.ptr, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (cell) {
+            AUR_HIP(hipMemcpy(cell, sp.cell.ptr, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+            // packed corner coordinates -> linear node index (k ny + j) nx + i
+            for (int64_t p = 0; p < n; ++p) {
+                if (cell[p] < 0) continue;
+                int64_t lin = 0, stride = 1;
+                for (int a = 0; a < c.dim; ++a) {
+                    lin += unpack_axis(cell[p], c.dim, a) * stride;
+                    stride *= c.n[a];
+                }
+                cell[p] = static_cast<int32_t>(lin);
+            }
+        }
         std::vector<uint8_t> buf(static_cast<size_t>(n) * sp.elem());
         auto widen = [&](const DevBuf<uint8_t>& src, double* dst, int stride, int col) {
             AUR_HIP(hipMemcpy(buf.data(), src.ptr, buf.size(), hipMemcpyDeviceToHost));

@@ -46,6 +46,16 @@ enum : uint32_t {
 
 constexpr int kMaxChannels = 8;
 
+// Particle cell = packed corner-node coordinates: 1D i; 2D i | j << 15
+// (extents <= 32768); 3D i | j << 10 | k << 20 (extents <= 1024).
+// Unpacking is shifts and masks, not integer division.
+template <int DIM>
+struct CellPack {
+    static constexpr int shift = DIM == 3 ? 10 : (DIM == 2 ? 15 : 31);
+    static constexpr int mask = DIM == 1 ? 0x7FFFFFFF : (1 << shift) - 1;
+};
+inline int pack_shift(int dim) { return dim == 3 ? 10 : (dim == 2 ? 15 : 31); }
+
 // --------------------------------------------------------- device memory
 template <typename T>
 struct DevBuf {
@@ -159,6 +169,9 @@ struct Context {
     int64_t brows = 0;
     DevBuf<double> bcoords, bvalues;
     DevBuf<int32_t> bhint;          // per cell along the table axis
+    DevBuf<float4> brow32;          // FP32 rows (coordinate, Bx, By, Bz)
+    DevBuf<float4> bslope32;        // per-row slopes dB/dq to the next row
+    DevBuf<int32_t> bhint32;        // first FP32 row above the FP32 cell start
 
     // 1D electrodes: offset, amplitude, frequency, phase
     double left4[4] = {0, 0, 0, 0};

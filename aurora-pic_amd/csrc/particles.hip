@@ -47,6 +47,10 @@ struct BView {
     const double* values;
     const int32_t* hint;   // per cell along axis: first row with coord > cell start
     double d_axis;
+    const float4* row32;   // FP32 table (coordinate, Bx, By, Bz) and slopes
+    const float4* slope32;
+    const int32_t* hint32;
+    float d_axis32;
 };
 
 template <typename R>
@@ -103,11 +107,44 @@ __device__ __forceinline__ void b_table(const BView& b, double q, int cell_axis,
     Bz = v[3 * left + 2] + f * (v[3 * right + 2] - v[3 * left + 2]);
 }
 
+// FP32 evaluation of the same piecewise-linear table for FP32 particles:
+// the query is clamped to the table (its coverage of the domain is checked
+// when the table is set) and the row hint is computed from the FP32 cell
+// start, so the forward walk always ends at upper_bound(q).
+__device__ __forceinline__ void b_table32(const BView& b, float q, int cell_axis, float& Bx,
+                                          float& By, float& Bz) {
+    const float4* row = b.row32;
+    const int rows = b.rows;
+    const float lo = row[0].x, hi = row[rows - 1].x;
+    q = fminf(fmaxf(q, lo), hi);
+    if (q == lo) { Bx = row[0].y; By = row[0].z; Bz = row[0].w; return; }
+    if (q == hi) { Bx = row[rows - 1].y; By = row[rows - 1].z; Bz = row[rows - 1].w; return; }
+    int r = b.hint32 ? b.hint32[cell_axis] : 1;
+    if (r < 1) r = 1;
+    while (r < rows && row[r].x <= q) ++r;
+    const float4 L = row[r - 1];
+    const float4 S = b.slope32[r - 1];
+    const float t = q - L.x;
+    Bx = L.y + t * S.x;
+    By = L.z + t * S.y;
+    Bz = L.w + t * S.z;
+}
+
 template <int DIM, typename R>
 __device__ __forceinline__ void magnetic(const BView& b, const int* idx, const R* off,
                                          R& Bx, R& By, R& Bz, uint32_t* err) {
     if (b.mode == 1) {
         Bx = static_cast<R>(b.bx); By = static_cast<R>(b.by); Bz = static_cast<R>(b.bz);
+        return;
+    }
+    if constexpr (sizeof(R) == 4) {
+        float q = 0.f;
+        int ca = 0;
+        if (b.axis < DIM) {
+            ca = idx[b.axis];
+            q = (static_cast<float>(ca) + off[b.axis]) * b.d_axis32;
+        }
+        b_table32(b, q, ca, Bx, By, Bz);
         return;
     }
     double q = 0.0;
@@ -348,20 +385,28 @@ __device__ __forceinline__ void deposit_one(long long* acc, const MeshView& m, c
 
 template <int DIM>
 __device__ __forceinline__ void decompose(int c, const MeshView& m, int* idx) {
-    idx[0] = c % m.nx;
-    if constexpr (DIM > 1) {
-        const int t = c / m.nx;
-        idx[1] = t % m.ny;
-        if constexpr (DIM > 2) idx[2] = t / m.ny;
+    (void)m;
+    if constexpr (DIM == 1) {
+        idx[0] = c;
+    } else {
+        constexpr int S = CellPack<DIM>::shift, M = CellPack<DIM>::mask;
+        idx[0] = c & M;
+        idx[1] = (c >> S) & M;
+        if constexpr (DIM > 2) idx[2] = c >> (2 * S);
     }
 }
 
 template <int DIM>
 __device__ __forceinline__ int compose(const MeshView& m, const int* idx) {
-    int c = idx[0];
-    if constexpr (DIM > 1) c += m.nx * idx[1];
-    if constexpr (DIM > 2) c += m.nx * m.ny * idx[2];
-    return c;
+    (void)m;
+    if constexpr (DIM == 1) {
+        return idx[0];
+    } else {
+        constexpr int S = CellPack<DIM>::shift;
+        int c = idx[0] | (idx[1] << S);
+        if constexpr (DIM > 2) c |= idx[2] << (2 * S);
+        return c;
+    }
 }
 
 template <int DIM, typename R>
@@ -619,6 +664,10 @@ BView b_view(const Context& c) {
         b.values = c.bvalues.ptr;
         b.hint = c.baxis < c.dim ? c.bhint.ptr : nullptr;
         b.d_axis = c.baxis < c.dim ? c.d[c.baxis] : 1.0;
+        b.row32 = c.brow32.ptr;
+        b.slope32 = c.bslope32.ptr;
+        b.hint32 = c.baxis < c.dim ? c.bhint32.ptr : nullptr;
+        b.d_axis32 = static_cast<float>(b.d_axis);
     } else if (c.Buni[0] != 0.0 || c.Buni[1] != 0.0 || c.Buni[2] != 0.0) {
         b.mode = 1;
     }
@@ -861,7 +910,47 @@ void launch_half(Context& c, Species& sp, double dt, double sign, double* vout, 
 }  // namespace
 
 void set_b_profile_device(Context& c) {
-    if (!c.has_profile || c.baxis >= c.dim) return;
+    if (!c.has_profile) return;
+    {
+        std::vector<double> co(static_cast<size_t>(c.brows)), va(static_cast<size_t>(3 * c.brows));
+        AUR_HIP(hipMemcpy(co.data(), c.bcoords.ptr, c.brows * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        AUR_HIP(hipMemcpy(va.data(), c.bvalues.ptr, 3 * c.brows * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        std::vector<float4> rows(static_cast<size_t>(c.brows)), slopes(static_cast<size_t>(c.brows));
+        for (int64_t r = 0; r < c.brows; ++r) {
+            rows[r] = make_float4(static_cast<float>(co[r]), static_cast<float>(va[3 * r]),
+                                  static_cast<float>(va[3 * r + 1]),
+                                  static_cast<float>(va[3 * r + 2]));
+            float sl[3] = {0.f, 0.f, 0.f};
+            if (r + 1 < c.brows)
+                for (int a = 0; a < 3; ++a)
+                    sl[a] = static_cast<float>((va[3 * (r + 1) + a] - va[3 * r + a]) /
+                                               (co[r + 1] - co[r]));
+            slopes[r] = make_float4(sl[0], sl[1], sl[2], 0.f);
+        }
+        c.brow32.alloc(c.brows);
+        c.bslope32.alloc(c.brows);
+        AUR_HIP(hipMemcpy(c.brow32.ptr, rows.data(), rows.size() * sizeof(float4),
+                          hipMemcpyHostToDevice));
+        AUR_HIP(hipMemcpy(c.bslope32.ptr, slopes.data(), slopes.size() * sizeof(float4),
+                          hipMemcpyHostToDevice));
+        if (c.baxis < c.dim) {
+            const int64_t cells = c.n[c.baxis];
+            const float d32 = static_cast<float>(c.d[c.baxis]);
+            std::vector<int32_t> hint(static_cast<size_t>(cells + 1));
+            for (int64_t i = 0; i <= cells; ++i) {
+                const float x0 = static_cast<float>(i) * d32;
+                int64_t r = 0;
+                while (r < c.brows && rows[r].x <= x0) ++r;
+                hint[i] = static_cast<int32_t>(r);
+            }
+            c.bhint32.alloc(cells + 1);
+            AUR_HIP(hipMemcpy(c.bhint32.ptr, hint.data(), hint.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice));
+        }
+    }
+    if (c.baxis >= c.dim) return;
     // per-cell hint: first table row with coordinate > cell start
     std::vector<double> coords(static_cast<size_t>(c.brows));
     AUR_HIP(hipMemcpy(coords.data(), c.bcoords.ptr, c.brows * sizeof(double),

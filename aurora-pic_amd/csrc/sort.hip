@@ -16,6 +16,7 @@ namespace aurora {
 namespace {
 
 struct KeyArgs {
+    int shift, mask;       // packed cell layout
     int nx, ny, nz;
     int tx, ty, tz;        // tile extent in cells
     int ntx, nty;          // tiles per axis
@@ -31,10 +32,9 @@ __global__ void k_sort_keys(const int32_t* cell, unsigned* keys, unsigned* vals,
         const int c = cell[p];
         unsigned key = k.sentinel;
         if (c >= 0) {
-            const int i = c % k.nx;
-            const int t = c / k.nx;
-            const int j = t % k.ny;
-            const int kk = t / k.ny;
+            const int i = c & k.mask;
+            const int j = (c >> k.shift) & k.mask;
+            const int kk = c >> (2 * k.shift);
             key = static_cast<unsigned>(((kk / k.tz) * k.nty + j / k.ty) * k.ntx + i / k.tx);
         }
         keys[p] = key;
@@ -86,6 +86,9 @@ void sort_species(Context& c, Species& sp) {
     if (n == 0) return;
     KernelScope ks(c, "sort");
     KeyArgs k{};
+    k.shift = c.dim == 1 ? 0 : pack_shift(c.dim);
+    k.mask = c.dim == 1 ? 0x7FFFFFFF : (1 << k.shift) - 1;
+    if (c.dim == 1) k.shift = 31;
     k.nx = static_cast<int>(c.n[0]);
     k.ny = static_cast<int>(c.n[1]);
     k.nz = static_cast<int>(c.n[2]);

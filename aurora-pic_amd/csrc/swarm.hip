@@ -7,7 +7,9 @@
 // velocities stay in registers for the launch's step range (half kick,
 // energy guard, MCC, half kick, guard - Swarm.cpp:998-1060).  Sampling
 // statistics go to per-(chunk, uncertainty block) integer fixed-point
-// partials (v_x in 1/16 m/s, energy in 2^-20 eV) and integer counters;
+// partials (power-of-two scales per request, sized from its energy ceiling
+// and block length so a chunk's block sum cannot overflow) and integer
+// counters;
 // integer sums are associative, so a request's result does not depend on
 // which other requests share its launches or on how its steps are split
 // into launches.  Positions are not tracked (diffusion is not reported).
@@ -41,6 +43,8 @@ struct RequestDesc {
     float dt;
     float max_energy_ev;
     float ev_per_v2;              // 0.5 m / e
+    float scale_v;                // fixed-point scales (powers of two)
+    float scale_e;
     int step0;                    // first step of this launch
     int nsteps;                   // steps this launch
     int burn_in;
@@ -131,8 +135,8 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
                 group = g;
                 if (active) {
                     const float e = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                    acc_v += __float2ll_rn(v[0] * 16.0f);
-                    acc_e += __float2ll_rn(e * 1048576.0f);
+                    acc_v += __float2ll_rn(v[0] * d.scale_v);
+                    acc_e += __float2ll_rn(e * d.scale_e);
                 }
             }
         }
@@ -250,6 +254,23 @@ MccParams swarm_params(const SwarmState& s) {
     return m;
 }
 
+// power-of-two fixed-point scales for the v_x and energy sums of one
+// (chunk, uncertainty block): kChunk x (block length) values each bounded by
+// the request's energy ceiling (with 2x headroom) stay below 2^61
+std::pair<double, double> scales(const aurora_hip_swarm_request& q) {
+    const double S = static_cast<double>(q.steps - q.burn_in);
+    const double per_block = std::ceil(S / static_cast<double>(q.blocks));
+    const double emax = 2.0 * q.max_energy_ev;
+    const double vmax = std::sqrt(2.0 * emax * kElementaryCharge / kElectronMass);
+    const double room = std::ldexp(1.0, 61) / (kChunk * per_block);
+    auto pow2_floor = [](double x) {
+        int e = static_cast<int>(std::floor(std::log2(x)));
+        e = std::max(-120, std::min(120, e));
+        return std::ldexp(1.0, e);
+    };
+    return {pow2_floor(room / vmax), pow2_floor(room / emax)};
+}
+
 void block_mean_error(const std::vector<double>& means, double& mean, double& se) {
     // Swarm.cpp:103-134 on block means
     const size_t b = means.size();
@@ -280,8 +301,8 @@ void finalize(Context& c, SwarmState& s, int64_t handle, RequestState& r) {
     for (int64_t b = 0; b < B; ++b) {
         double sv = 0.0, se = 0.0;
         for (int64_t ch = 0; ch < r.chunks; ++ch) {   // fixed chunk order
-            sv += static_cast<double>(part[(ch * B + b) * 2]) * (1.0 / 16.0);
-            se += static_cast<double>(part[(ch * B + b) * 2 + 1]) * 0x1p-20;
+            sv += static_cast<double>(part[(ch * B + b) * 2]) / scales(q).first;
+            se += static_cast<double>(part[(ch * B + b) * 2 + 1]) / scales(q).second;
         }
         const int64_t len = (b + 1) * S / B - b * S / B;
         vm[b] = sv / (static_cast<double>(N) * static_cast<double>(len));
@@ -502,6 +523,8 @@ int aurora_hip_swarm_advance(aurora_hip_ctx* h, int64_t max_steps, int64_t* acti
             d.dt = static_cast<float>(q.timestep);
             d.max_energy_ev = static_cast<float>(q.max_energy_ev);
             d.ev_per_v2 = static_cast<float>(0.5 * kElectronMass / kElementaryCharge);
+            d.scale_v = static_cast<float>(scales(q).first);
+            d.scale_e = static_cast<float>(scales(q).second);
             d.step0 = static_cast<int>(r.steps_done);
             d.nsteps = static_cast<int>(ns);
             d.burn_in = static_cast<int>(q.burn_in);

@@ -122,7 +122,8 @@ int aurora_hip_species_download(aurora_hip_ctx* ctx, int species_id,
 int aurora_hip_species_velocity(aurora_hip_ctx* ctx, int species_id,
                                 double dt, int64_t capacity,
                                 double* velocity);
-/* Raw device state in storage order: cell (slots; -1 = dead), offsets
+/* Raw device state in storage order: cell (slots; linear index
+ * (k ny + j) nx + i of the cell's lower corner node, -1 = dead), offsets
  * (slots*dim, the stored values widened exactly to double), velocity_half
  * (slots*3).  The representation the fixed-point deposit is defined on. */
 int aurora_hip_species_download_raw(aurora_hip_ctx* ctx, int species_id,

@@ -51,8 +51,8 @@ def test_swarm_zero_cross_section_closed_form(ctx):
     a_dt = workloads.EV * 1.0e-21 * 1e20 / workloads.ME * 1e-9
     s1 = np.arange(11, 41, dtype=float)
     assert r.status == 0
-    # FP32 velocity accumulation and 1/16 m/s statistics quantisation
-    assert abs(r.drift - a_dt * s1.mean()) <= 1e-5 * a_dt * s1.mean() + 0.07
+    # FP32 velocity accumulation over 40 half-kick pairs
+    assert abs(r.drift - a_dt * s1.mean()) <= 1e-5 * a_dt * s1.mean()
     e_exp = 0.5 * workloads.ME * a_dt ** 2 * np.mean(s1 ** 2) / workloads.EV
     assert abs(r.mean_energy_ev - e_exp) <= 1e-5 * e_exp
     assert r.collisions[0] == 0
