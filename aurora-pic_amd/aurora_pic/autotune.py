@@ -9,8 +9,11 @@ from __future__ import annotations
 
 import time
 
-# tile shapes built into libaurora_hip.so: (tx, ty, tz, halo)
-SHAPES = [(8, 8, 8, 2), (8, 8, 8, 1), (8, 8, 4, 2), (16, 8, 4, 2), (4, 4, 4, 2)]
+# (tx, ty, tz, halo, block, prefetch) combinations built into libaurora_hip.so
+SHAPES = [(8, 8, 8, 2, 512, 0), (8, 8, 8, 1, 512, 0), (8, 8, 4, 2, 512, 0),
+          (16, 8, 4, 2, 512, 0), (4, 4, 4, 2, 512, 0), (8, 8, 4, 2, 512, 1),
+          (8, 8, 4, 2, 256, 1), (8, 8, 4, 2, 256, 0), (8, 8, 8, 2, 512, 1),
+          (8, 8, 8, 2, 256, 1)]
 
 
 def tune(ctx, dt, shapes=SHAPES, sort_intervals=(20,), steps=20, log=None):
@@ -20,7 +23,8 @@ def tune(ctx, dt, shapes=SHAPES, sort_intervals=(20,), steps=20, log=None):
     table = []
     for shape in shapes:
         try:
-            ctx.tiling(*shape)
+            ctx.tiling(*shape[:4])
+            ctx.push_variant(*(shape[4:] or (512, 0)))
         except Exception:  # shape not applicable to this mesh
             continue
         for interval in sort_intervals:
@@ -36,7 +40,8 @@ def tune(ctx, dt, shapes=SHAPES, sort_intervals=(20,), steps=20, log=None):
             if log:
                 log(f"autotune {shape} sort every {interval}: {ms:.3f} ms/step")
     best = min(table)
-    ctx.tiling(*best[1])
+    ctx.tiling(*best[1][:4])
+    ctx.push_variant(*(best[1][4:] or (512, 0)))
     ctx.sort()
     ctx.sort_interval(best[2])
     return best, table

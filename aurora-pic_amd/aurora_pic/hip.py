@@ -120,6 +120,7 @@ def load(path: str = LIB_PATH):
         "aurora_hip_set_sort_interval": [ctx, C.c_int64],
         "aurora_hip_sort": [ctx],
         "aurora_hip_set_tiling": [ctx, C.c_int, C.c_int, C.c_int, C.c_int],
+        "aurora_hip_set_push_variant": [ctx, C.c_int, C.c_int],
         "aurora_hip_mcc_load": [ctx, C.c_int, C.c_int, _ip, _dp, _i64p, C.POINTER(_dp),
                                 C.POINTER(_dp), C.c_double, C.c_double, C.c_int64, C.c_double,
                                 C.c_double, _ip, C.c_uint64],
@@ -347,6 +348,9 @@ class Context:
 
     def tiling(self, tx, ty=0, tz=0, halo=0):
         self._check(self.lib.aurora_hip_set_tiling(self.h, tx, ty, tz, halo))
+
+    def push_variant(self, block, prefetch):
+        self._check(self.lib.aurora_hip_set_push_variant(self.h, block, prefetch))
 
     def sort(self):
         self._check(self.lib.aurora_hip_sort(self.h))

@@ -655,6 +655,15 @@ int aurora_hip_set_tiling(aurora_hip_ctx* h, int tx, int ty, int tz, int halo) {
     });
 }
 
+int aurora_hip_set_push_variant(aurora_hip_ctx* h, int block, int prefetch) {
+    return guarded(h, [&](Context& c) {
+        AUR_REQUIRE((block == 256 || block == 512) && (prefetch == 0 || prefetch == 1),
+                    AURORA_ERR_UNSUPPORTED, "push variant is not built into this library");
+        c.push_block = block;
+        c.push_prefetch = prefetch;
+    });
+}
+
 int aurora_hip_sort(aurora_hip_ctx* h) {
     return guarded(h, [&](Context& c) {
         for (auto& sp : c.species) sort_species(c, sp);

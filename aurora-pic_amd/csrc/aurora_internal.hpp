@@ -195,6 +195,8 @@ struct Context {
     int64_t sort_interval = 0;
     int tile[3] = {8, 8, 8};
     int halo = 2;
+    int push_block = 512;        // tiled push workgroup size
+    int push_prefetch = 0;       // tiled push loads the next particle early
     bool tiled_enabled = true;
     DevBuf<int32_t> tile_count, tile_offset;
     DevBuf<unsigned> sort_keys, sort_keys2, sort_vals, sort_vals2;

@@ -720,10 +720,12 @@ __device__ __forceinline__ int rel_index(int i, int o, int n) {
     return r;
 }
 
-template <int TX, int TY, int TZ, int H, int BMODE>
-__global__ void __launch_bounds__(512) k_push_tile3d(PushArgs<float> a,
-                                                     const int32_t* __restrict__ tile_start,
-                                                     int ntx, int nty) {
+// BLOCK threads per workgroup; PF = 1 loads the next particle's state
+// while the current one is processed (more memory-level parallelism).
+template <int TX, int TY, int TZ, int H, int BMODE, int BLOCK, int PF>
+__global__ void __launch_bounds__(BLOCK) k_push_tile3d(PushArgs<float> a,
+                                                       const int32_t* __restrict__ tile_start,
+                                                       int ntx, int nty) {
     constexpr int BX = TX + 1 + 2 * H, BY = TY + 1 + 2 * H, BZ = TZ + 1 + 2 * H;
     constexpr int BN = BX * BY * BZ;
     __shared__ float4 sE[BN];
@@ -735,7 +737,7 @@ __global__ void __launch_bounds__(512) k_push_tile3d(PushArgs<float> a,
     const int oy = ((t / ntx) % nty) * TY - H;
     const int oz = (t / (ntx * nty)) * TZ - H;
     const float4* E4 = reinterpret_cast<const float4*>(a.E32);
-    for (int q = threadIdx.x; q < BN; q += blockDim.x) {
+    for (int q = threadIdx.x; q < BN; q += BLOCK) {
         const int bx = q % BX, by = (q / BX) % BY, bz = q / (BX * BY);
         const int gx = wrap_index(ox + bx, m.nx);
         const int gy = wrap_index(oy + by, m.ny);
@@ -744,14 +746,29 @@ __global__ void __launch_bounds__(512) k_push_tile3d(PushArgs<float> a,
         sQ[q] = 0ull;
     }
     __syncthreads();
+    struct State {
+        int c;
+        float o0, o1, o2, v0, v1, v2;
+    };
+    auto load = [&](int p) {
+        State st;
+        st.c = sp.cell[p];
+        st.o0 = sp.off0[p]; st.o1 = sp.off1[p]; st.o2 = sp.off2[p];
+        st.v0 = sp.v0[p]; st.v1 = sp.v1[p]; st.v2 = sp.v2[p];
+        return st;
+    };
     const int p0 = tile_start[t], p1 = tile_start[t + 1];
-    for (int p = p0 + static_cast<int>(threadIdx.x); p < p1; p += blockDim.x) {
-        const int c = sp.cell[p];
-        if (c < 0) continue;
+    int p = p0 + static_cast<int>(threadIdx.x);
+    State next{};
+    if (PF && p < p1) next = load(p);
+    for (; p < p1; p += BLOCK) {
+        const State st = PF ? next : load(p);
+        if (PF && p + BLOCK < p1) next = load(p + BLOCK);
+        if (st.c < 0) continue;
         int idx[3];
-        decompose<3>(c, m, idx);
-        float off[3] = {sp.off0[p], sp.off1[p], sp.off2[p]};
-        float v[3] = {sp.v0[p], sp.v1[p], sp.v2[p]};
+        decompose<3>(st.c, m, idx);
+        float off[3] = {st.o0, st.o1, st.o2};
+        float v[3] = {st.v0, st.v1, st.v2};
         float E[3];
         {
             const int rx = rel_index(idx[0], ox, m.nx);
@@ -813,7 +830,7 @@ __global__ void __launch_bounds__(512) k_push_tile3d(PushArgs<float> a,
         }
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < BN; q += blockDim.x) {
+    for (int q = threadIdx.x; q < BN; q += BLOCK) {
         const unsigned long long val = sQ[q];
         if (val == 0ull) continue;
         const int bx = q % BX, by = (q / BX) % BY, bz = q / (BX * BY);
@@ -825,29 +842,39 @@ __global__ void __launch_bounds__(512) k_push_tile3d(PushArgs<float> a,
     }
 }
 
-template <int TX, int TY, int TZ, int H>
+template <int TX, int TY, int TZ, int H, int BLOCK, int PF>
 bool launch_tile3d_t(Context& c, Species& sp, const PushArgs<float>& a, int mode) {
-    if (c.tile[0] != TX || c.tile[1] != TY || c.tile[2] != TZ || c.halo != H) return false;
+    if (c.tile[0] != TX || c.tile[1] != TY || c.tile[2] != TZ || c.halo != H ||
+        c.push_block != BLOCK || c.push_prefetch != PF)
+        return false;
     const int ntx = c.tiles_axis(0), nty = c.tiles_axis(1);
     const unsigned grid = static_cast<unsigned>(c.ntiles());
     KernelScope ks(c, "push_deposit");
+    const int32_t* ts = sp.tile_start.ptr;
     if (mode == 0)
-        k_push_tile3d<TX, TY, TZ, H, 0><<<grid, 512, 0, c.stream>>>(a, sp.tile_start.ptr, ntx, nty);
+        k_push_tile3d<TX, TY, TZ, H, 0, BLOCK, PF><<<grid, BLOCK, 0, c.stream>>>(a, ts, ntx, nty);
     else if (mode == 1)
-        k_push_tile3d<TX, TY, TZ, H, 1><<<grid, 512, 0, c.stream>>>(a, sp.tile_start.ptr, ntx, nty);
+        k_push_tile3d<TX, TY, TZ, H, 1, BLOCK, PF><<<grid, BLOCK, 0, c.stream>>>(a, ts, ntx, nty);
     else
-        k_push_tile3d<TX, TY, TZ, H, 2><<<grid, 512, 0, c.stream>>>(a, sp.tile_start.ptr, ntx, nty);
+        k_push_tile3d<TX, TY, TZ, H, 2, BLOCK, PF><<<grid, BLOCK, 0, c.stream>>>(a, ts, ntx, nty);
     AUR_HIP(hipGetLastError());
     return true;
 }
 
-// the tile shapes built into the library (the autotuner picks among them)
+// the tile shapes and variants built into the library (the autotuner picks)
 bool launch_tile3d(Context& c, Species& sp, double dt) {
     auto a = push_args<float>(c, sp, dt);
     const int mode = a.b.mode;
-    return launch_tile3d_t<8, 8, 8, 2>(c, sp, a, mode) || launch_tile3d_t<8, 8, 8, 1>(c, sp, a, mode) ||
-           launch_tile3d_t<8, 8, 4, 2>(c, sp, a, mode) || launch_tile3d_t<4, 4, 4, 2>(c, sp, a, mode) ||
-           launch_tile3d_t<16, 8, 4, 2>(c, sp, a, mode);
+    return launch_tile3d_t<8, 8, 8, 2, 512, 0>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 8, 1, 512, 0>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 4, 2, 512, 0>(c, sp, a, mode) ||
+           launch_tile3d_t<4, 4, 4, 2, 512, 0>(c, sp, a, mode) ||
+           launch_tile3d_t<16, 8, 4, 2, 512, 0>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 4, 2, 512, 1>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 4, 2, 256, 1>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 4, 2, 256, 0>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 8, 2, 512, 1>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 8, 2, 256, 1>(c, sp, a, mode);
 }
 
 template <int DIM, typename R, int BMODE>

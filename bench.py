@@ -124,16 +124,20 @@ def main():
     tuned = None
     if args.tile:
         shape = [int(v) for v in args.tile.split(",")]
-        ctx.tiling(*shape)
+        ctx.tiling(*shape[:4])
+        if len(shape) > 4:
+            ctx.push_variant(shape[4], shape[5])
         ctx.sort()
         tuned = {"tile": shape[:3], "halo": shape[3], "sort_interval": cfg["sort_interval"],
                  "fixed": True}
     elif not args.no_autotune:
         from aurora_pic import autotune
         best, table = autotune.tune(ctx, dt, sort_intervals=(cfg["sort_interval"],), log=log)
-        tuned = {"tile": list(best[1][:3]), "halo": best[1][3], "sort_interval": best[2],
-                 "candidates_ms_per_step": {f"{s[0]}x{s[1]}x{s[2]}h{s[3]}": round(ms, 4)
-                                            for ms, s, _ in table}}
+        tuned = {"tile": list(best[1][:3]), "halo": best[1][3], "block": best[1][4],
+                 "prefetch": best[1][5], "sort_interval": best[2],
+                 "candidates_ms_per_step": {
+                     f"{s[0]}x{s[1]}x{s[2]}h{s[3]}b{s[4]}p{s[5]}": round(ms, 4)
+                     for ms, s, _ in table}}
     log(f"particles {total}, warmup {args.warmup} steps")
     ctx.step(dt, args.warmup)
     ctx.sync()

@@ -245,7 +245,10 @@ def test_sort_preserves_particles_and_charge(hip_ctx):
     assert len(cell) == N
 
 
-@pytest.mark.parametrize("shape", [(8, 8, 8, 2), (8, 8, 8, 1), (4, 4, 4, 2), (16, 8, 4, 2)])
+@pytest.mark.parametrize("shape", [(8, 8, 8, 2, 512, 0), (8, 8, 8, 1, 512, 0),
+                                   (4, 4, 4, 2, 512, 0), (16, 8, 4, 2, 512, 0),
+                                   (8, 8, 4, 2, 256, 1), (8, 8, 4, 2, 512, 1),
+                                   (8, 8, 8, 2, 256, 1)])
 def test_tiled_push_bit_identical_to_untiled(shape):
     """The LDS-tiled 3D push performs the same operations as the untiled
     kernel: particle state and fixed-point charge agree bit for bit, also
@@ -265,7 +268,8 @@ def test_tiled_push_bit_identical_to_untiled(shape):
             c.mesh(n, L, 0, 1.0)
             c.b_profile(0, x, bvals)
             if tiled:
-                c.tiling(*shape)
+                c.tiling(*shape[:4])
+                c.push_variant(*shape[4:])
             else:
                 c.tiling(0)
             sid = c.add_species(-1.0, 1.0, 1.0 / N, N)
