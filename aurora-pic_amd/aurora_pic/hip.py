@@ -18,6 +18,8 @@ HEADER = os.path.join(REPO, "include", "aurora_hip.h")
 PERIODIC, DIRICHLET = 0, 1
 PB_ABSORB, PB_REFLECT, PB_PERIODIC = 1, 2, 3
 VELOCITY_HALF, VELOCITY_CENTRED = 0, 1
+(OK, ERR_INVALID_ARGUMENT, ERR_HIP, ERR_RATE_ABOVE_BOUND, ERR_CANDIDATE_LIMIT, ERR_CAPACITY,
+ ERR_B_OUT_OF_RANGE, ERR_ENERGY_ABOVE_MAX, ERR_UNSUPPORTED, ERR_NOT_FOUND) = range(10)
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
@@ -132,6 +134,17 @@ def load(path: str = LIB_PATH):
         "aurora_hip_swarm_submit": [ctx, C.POINTER(SwarmRequest), C.c_uint64, _i64p],
         "aurora_hip_swarm_advance": [ctx, C.c_int64, _i64p],
         "aurora_hip_swarm_poll": [ctx, C.c_int64, _ip, C.POINTER(SwarmResult)],
+        "aurora_hip_ensemble_create": [ctx, C.c_int64, _i64p, _i64p],
+        "aurora_hip_ensemble_load": [ctx, C.c_int64, C.c_int64, C.c_uint64],
+        "aurora_hip_ensemble_graph": [ctx, C.c_int64, C.c_int],
+        "aurora_hip_ensemble_step": [ctx, C.c_int64, C.c_double, C.c_int64],
+        "aurora_hip_ensemble_count": [ctx, C.c_int64, C.c_int64, C.c_int, _i64p, _i64p],
+        "aurora_hip_ensemble_download": [ctx, C.c_int64, C.c_int64, C.c_int, C.c_int64, _dp, _dp,
+                                         _u8p, _u32p, _i64p],
+        "aurora_hip_ensemble_stats": [ctx, C.c_int64, C.c_int64, _u64p, _dp, _u64p, _dp],
+        "aurora_hip_ensemble_fields": [ctx, C.c_int64, C.c_int64, _dp, _dp, _dp],
+        "aurora_hip_ensemble_time": [ctx, C.c_int64, _dp, _i64p],
+        "aurora_hip_ensemble_destroy": [ctx, C.c_int64],
         "aurora_hip_kernel_timing": [ctx, C.c_int],
         "aurora_hip_kernel_time": [ctx, C.c_char_p, _dp, _i64p],
     }
@@ -427,3 +440,71 @@ class Context:
 
     def swarm_footprint(self, request: SwarmRequest):
         return int(self.lib.aurora_hip_swarm_footprint(C.byref(request)))
+
+    # ---- batched 1D ensembles
+    def ensemble_create(self, instances, capacity):
+        cap = np.ascontiguousarray(capacity, dtype=np.int64)
+        h = C.c_int64()
+        self._check(self.lib.aurora_hip_ensemble_create(self.h, instances, _p(cap, _i64p),
+                                                        C.byref(h)))
+        return Ensemble(self, h.value, instances, len(cap))
+
+
+class Ensemble:
+    """Handle of a batched 1D ensemble owned by a Context."""
+
+    def __init__(self, ctx, handle, instances, nspecies):
+        self.ctx, self.h, self.instances, self.nspecies = ctx, handle, instances, nspecies
+
+    def _call(self, name, *args):
+        self.ctx._check(getattr(self.ctx.lib, name)(self.ctx.h, self.h, *args))
+
+    def load(self, instance, mcc_seed):
+        self._call("aurora_hip_ensemble_load", instance, mcc_seed)
+
+    def graph(self, steps_per_graph):
+        self._call("aurora_hip_ensemble_graph", steps_per_graph)
+
+    def step(self, dt, n=1):
+        self._call("aurora_hip_ensemble_step", dt, n)
+
+    def count(self, instance, sid):
+        s, l = C.c_int64(), C.c_int64()
+        self._call("aurora_hip_ensemble_count", instance, sid, C.byref(s), C.byref(l))
+        return s.value, l.value
+
+    def download(self, instance, sid):
+        slots, _ = self.count(instance, sid)
+        m = max(slots, 1)
+        pos, vel = np.zeros(m), np.zeros(3 * m)
+        alive, uid = np.zeros(m, dtype=np.uint8), np.zeros(m, dtype=np.uint32)
+        n = C.c_int64()
+        self._call("aurora_hip_ensemble_download", instance, sid, m, _p(pos, _dp), _p(vel, _dp),
+                   _p(alive, _u8p), _p(uid, _u32p), C.byref(n))
+        k = n.value
+        return pos[:k].reshape(k, 1), vel[:3 * k].reshape(k, 3), alive[:k], uid[:k]
+
+    def stats(self, instance, nchan):
+        counts = np.zeros(nchan + 3, dtype=np.uint64)
+        ech = np.zeros(max(nchan, 1))
+        ab = np.zeros(2 * self.nspecies, dtype=np.uint64)
+        abe = np.zeros(2 * self.nspecies)
+        self._call("aurora_hip_ensemble_stats", instance, _p(counts, _u64p), _p(ech, _dp),
+                   _p(ab, _u64p), _p(abe, _dp))
+        return counts, ech[:nchan], ab.reshape(self.nspecies, 2), abe.reshape(self.nspecies, 2)
+
+    def fields(self, instance):
+        n = self.ctx.nnodes
+        rho, phi, E = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._call("aurora_hip_ensemble_fields", instance, _p(rho, _dp), _p(phi, _dp), _p(E, _dp))
+        return rho, phi, E
+
+    def time(self):
+        t, s = C.c_double(), C.c_int64()
+        self._call("aurora_hip_ensemble_time", C.byref(t), C.byref(s))
+        return t.value, s.value
+
+    def destroy(self):
+        if self.h:
+            self._call("aurora_hip_ensemble_destroy")
+            self.h = 0

@@ -1,0 +1,167 @@
+"""Continuous-batching request loop for electron-swarm solves.
+
+Requests arrive over time (each with an arrival timestamp).  The loop admits
+an arrived request as soon as its HBM footprint fits the budget
+(``aurora_hip_swarm_submit`` refuses it with AURORA_ERR_CAPACITY otherwise,
+and it stays queued), advances every admitted request together by a bounded
+number of steps per launch (``aurora_hip_swarm_advance``: one persistent
+kernel over the chunks of all active requests), and collects results as
+requests finish.  State stays on the device from admission to the result.
+
+A request's result does not depend on what it is batched with (integer
+statistics, per-request RNG streams), so the batched answer equals the one a
+sequential single-request run gives, bit for bit; the loop reports
+per-request latency (arrival -> result) as p50/p99 beside throughput.
+
+The reference's swarm driver (run_swarm_benchmark, Swarm.cpp:1513-1716)
+runs field points one after another on the CPU; seeds follow its rule
+``seed = base + index`` (Swarm.cpp:1513-1515).
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import hip
+
+
+@dataclass
+class Served:
+    index: int
+    arrival_s: float
+    request: object
+    admit_s: float = float("nan")
+    done_s: float = float("nan")
+    result: object = None
+
+    @property
+    def latency_s(self):
+        return self.done_s - self.arrival_s
+
+
+@dataclass
+class ServingReport:
+    served: list
+    wall_s: float
+    electron_steps: int
+    launches: int
+    max_active: int
+    refused_for_budget: int
+    timeline: list = field(default_factory=list)
+
+    def latencies(self):
+        return np.array([s.latency_s for s in self.served])
+
+    def summary(self):
+        lat = self.latencies()
+        return {
+            "requests": len(self.served),
+            "wall_s": self.wall_s,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()),
+            "requests_per_s": len(self.served) / self.wall_s,
+            "electron_steps_per_s": self.electron_steps / self.wall_s,
+            "launches": self.launches,
+            "max_active": self.max_active,
+            "budget_refusals": self.refused_for_budget,
+        }
+
+
+def make_requests(cfg, count=None, steps=None, burn_in=None, particle_scale=1.0):
+    """SwarmRequest objects for a config_swarm_requests() workload
+    (optionally truncated / shortened for a bounded run)."""
+    n = len(cfg["arrival_s"]) if count is None else count
+    steps = cfg["steps"] if steps is None else steps
+    burn = cfg["burn_in"] if burn_in is None else burn_in
+    out = []
+    for i in range(n):
+        parts = max(256, int(round(cfg["particles"][i] * particle_scale)))
+        out.append(hip.SwarmRequest(
+            reduced_field_td=float(cfg["reduced_field_td"][i]), timestep=cfg["timestep"],
+            particles=parts, steps=steps, burn_in=burn, blocks=cfg["blocks"],
+            initial_mean_energy_ev=cfg["initial_mean_energy_ev"],
+            max_energy_ev=cfg["max_energy_ev"], seed=cfg["seed"] + i))
+    return out
+
+
+def serve(ctx, requests, arrivals_s, hbm_budget, steps_per_launch=1000, clock=time.perf_counter,
+          on_launch=None):
+    """Run the request loop until every request has a result.
+
+    ctx: hip.Context with the swarm gas loaded.  arrivals_s: arrival time of
+    each request relative to the start of the loop (non-decreasing).
+    on_launch: optional callable run after every swarm launch (e.g. to
+    interleave other device work).  Returns a ServingReport."""
+    order = np.argsort(np.asarray(arrivals_s), kind="stable")
+    pending = collections.deque(Served(int(i), float(arrivals_s[i]), requests[i]) for i in order)
+    queued = collections.deque()
+    active = {}
+    served = []
+    launches = max_active = refused = 0
+    electron_steps = 0
+    t0 = clock()
+    while pending or queued or active:
+        now = clock() - t0
+        while pending and pending[0].arrival_s <= now:
+            queued.append(pending.popleft())
+        while queued:
+            s = queued[0]
+            try:
+                h = ctx.swarm_submit(s.request, hbm_budget=hbm_budget)
+            except hip.HipError as e:
+                if e.code != hip.ERR_CAPACITY:
+                    raise
+                refused += 1
+                break
+            s.admit_s = clock() - t0
+            active[h] = s
+            queued.popleft()
+        if not active:
+            if pending:
+                wait = pending[0].arrival_s - (clock() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+            continue
+        max_active = max(max_active, len(active))
+        for s in active.values():
+            done_steps = getattr(s, "_steps", 0)
+            k = min(steps_per_launch, s.request.steps - done_steps)
+            s._steps = done_steps + k
+            electron_steps += k * s.request.particles
+        ctx.swarm_advance(steps_per_launch)
+        launches += 1
+        if on_launch is not None:
+            on_launch()
+        t_done = clock() - t0
+        for h in list(active):
+            done, r = ctx.swarm_poll(h)
+            if done:
+                s = active.pop(h)
+                s.done_s = t_done
+                s.result = r
+                served.append(s)
+    wall = clock() - t0
+    served.sort(key=lambda s: s.index)
+    return ServingReport(served, wall, electron_steps, launches, max_active, refused)
+
+
+def run_sequential(ctx, requests, steps_per_launch=None):
+    """Each request alone on the GPU, one after another (the single-instance
+    baseline).  Returns (results, wall seconds, electron-steps)."""
+    results = []
+    steps_total = 0
+    t0 = time.perf_counter()
+    for r in requests:
+        h = ctx.swarm_submit(r)
+        while True:
+            ctx.swarm_advance(steps_per_launch or r.steps)
+            done, res = ctx.swarm_poll(h)
+            if done:
+                break
+        results.append(res)
+        steps_total += r.steps * r.particles
+    return results, time.perf_counter() - t0, steps_total

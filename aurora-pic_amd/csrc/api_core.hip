@@ -211,6 +211,7 @@ int aurora_hip_destroy(aurora_hip_ctx* h) {
     (void)hipSetDevice(h->c.device);
     if (h->c.stream) (void)hipStreamSynchronize(h->c.stream);
     swarm_destroy(h->c);
+    ensembles_destroy(h->c);
     for (auto& p : h->c.pending) {
         (void)hipEventDestroy(p.second.first);
         (void)hipEventDestroy(p.second.second);

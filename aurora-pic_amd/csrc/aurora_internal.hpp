@@ -205,8 +205,9 @@ struct Context {
     std::vector<Species> species;
     MccModelHost mcc;
 
-    // swarm batch state lives in swarm.hip
+    // swarm batch state lives in swarm.hip, batched 1D ensembles in ensemble.hip
     void* swarm = nullptr;
+    void* ensembles = nullptr;
 
     bool timing = false;
     std::map<std::string, TimerEntry> timers;
@@ -279,6 +280,7 @@ void mcc_step(Context& ctx, double dt);
 void init_half_all(Context& ctx, double dt);
 void set_b_profile_device(Context& ctx);
 void swarm_destroy(Context& ctx);
+void ensembles_destroy(Context& ctx);
 
 // ------------------------------------------------------- device helpers
 // Fixed-point quantisation: round-to-nearest-even of s * 2^40 (s in [0,1])

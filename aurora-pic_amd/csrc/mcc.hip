@@ -17,6 +17,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include "mcc_device.hpp"
+#include "push_device.hpp"
 
 namespace aurora {
 
@@ -24,68 +25,6 @@ namespace {
 
 constexpr int kBlk = 256;
 
-MccParams params(const Context& c) {
-    const MccModelHost& h = c.mcc;
-    MccParams m{};
-    m.nchan = h.nchan;
-    int total = 0;
-    for (int k = 0; k < h.nchan; ++k) {
-        m.kind[k] = h.kind[k];
-        m.threshold[k] = static_cast<float>(h.threshold[k]);
-        m.rows[k] = static_cast<int>(h.rows[k]);
-        m.offset[k] = static_cast<int>(h.row_offset[k]);
-        total += m.rows[k];
-    }
-    m.total_rows = total;
-    m.shared_grid = h.shared_grid ? 1 : 0;
-    m.energy = h.energy.ptr;
-    m.sigma = h.sigma.ptr;
-    m.neutral_density = static_cast<float>(h.neutral_density);
-    m.max_frequency = static_cast<float>(h.max_frequency);
-    m.inv_max_frequency = static_cast<float>(1.0 / h.max_frequency);
-    m.max_candidates = static_cast<int>(h.max_candidates);
-    m.mass = static_cast<float>(h.particle_mass);
-    m.half_mass = static_cast<float>(0.5 * h.particle_mass);
-    m.neutral_mass = static_cast<float>(h.neutral_mass);
-    m.neutral_vstd = static_cast<float>(h.neutral_vstd);
-    return m;
-}
-
-// block-level tally reduction into global counters
-struct TallyOut {
-    unsigned long long* counts;  // [candidates, nulls, chan..., secondaries]
-    double* energy_change;       // per channel (J, unweighted)
-};
-
-__device__ void flush_tally(const MccParams& m, const MccTally& t, TallyOut out, bool active) {
-    // one atomic per counter per wave (wave-level reduction via ballot-free
-    // shuffles would reorder nothing: integer adds commute)
-    unsigned long long cand = active ? t.candidates : 0, nul = active ? t.nulls : 0;
-    unsigned long long sec = active ? static_cast<unsigned long long>(t.nsec) : 0;
-    for (int off = 32; off > 0; off >>= 1) {
-        cand += __shfl_down(cand, off, 64);
-        nul += __shfl_down(nul, off, 64);
-        sec += __shfl_down(sec, off, 64);
-    }
-    const bool lead = (threadIdx.x & 63) == 0;
-    if (lead) {
-        if (cand) atomicAdd(out.counts, cand);
-        if (nul) atomicAdd(out.counts + 1, nul);
-        if (sec) atomicAdd(out.counts + 2 + m.nchan, sec);
-    }
-    for (int c = 0; c < m.nchan; ++c) {
-        unsigned long long n = active ? t.chan[c] : 0;
-        double e = active ? static_cast<double>(t.energy_change[c]) : 0.0;
-        for (int off = 32; off > 0; off >>= 1) {
-            n += __shfl_down(n, off, 64);
-            e += __shfl_down(e, off, 64);
-        }
-        if (lead) {
-            if (n) atomicAdd(out.counts + 2 + c, n);
-            if (e != 0.0) atomicAdd(out.energy_change + c, e);
-        }
-    }
-}
 
 // standalone collide of count velocities (parity checks): uid = index
 __global__ void __launch_bounds__(kBlk) k_collide(MccParams m, float* v, int64_t count,
@@ -150,47 +89,14 @@ __global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, Tally
             active = c >= 0;
         }
         MccTally t{};
-        if (active) {
-            // E(x) at the particle (FieldSolver.cpp:909-924, 1D gather)
-            const int i0 = c;
-            const int i1 = (i0 + 1 == a.nx) ? 0 : i0 + 1;
-            const float o = a.off[p];
-            const float ex = a.E32[i0] * (1.0f - o) + a.E32[i1] * o;
-            const float kick = a.half_qm_dt * ex;
-            // synchronize_leapfrog (Pusher.cpp:74-76): v = v_half + 0.5 qm E dt
-            float u[3] = {a.v0[p] + kick, a.v1[p], a.v2[p]};
-            Philox rng;
-            rng.init(a.seed, 0u, a.uid[p], a.step);
-            mcc_collide(m, E, S, u, a.dt, rng, t);
-            if (t.err) atomicOr(a.err, t.err);
-            // initialize_leapfrog_half_step (Pusher.cpp:66-68)
-            a.v0[p] = u[0] - kick;
-            a.v1[p] = u[1];
-            a.v2[p] = u[2];
-            const int np = t.nsec < 2 ? t.nsec : 2;
-            if (t.nsec > 2) atomicOr(a.err, DEV_ERR_CAPACITY);
-            a.nprod[p] = np;
-            for (int k = 0; k < np; ++k)
-                for (int q = 0; q < 3; ++q) {
-                    a.prod[12 * p + 6 * k + q] = t.sec[k][q];
-                    a.prod[12 * p + 6 * k + 3 + q] = t.ion[k][q];
-                }
-        }
+        if (active)
+            a.nprod[p] = mcc_1d_particle(m, E, S, c, a.off[p], a.v0 + p, a.v1 + p, a.v2 + p, a.uid[p],
+                                         a.E32, a.nx, a.half_qm_dt, a.dt, a.seed, a.step,
+                                         a.prod + 12 * p, t, a.err);
         flush_tally(m, t, out, active);
     }
 }
 
-struct AppendSide {
-    int32_t* cell;
-    float* off;
-    float* v0;
-    float* v1;
-    float* v2;
-    uint32_t* uid;
-    int64_t base_slot;
-    uint32_t base_uid;
-    float half_qm_dt;
-};
 
 // write the products of parent p at slots base + offset[p] + k
 __global__ void k_append_products(const int32_t* pcell, const float* poff,
@@ -201,25 +107,7 @@ __global__ void k_append_products(const int32_t* pcell, const float* poff,
          p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int np = nprod[p];
         if (np == 0) continue;
-        const int c = pcell[p];
-        const float o = poff[p];
-        const int i1 = (c + 1 == nx) ? 0 : c + 1;
-        const float ex = E32[c] * (1.0f - o) + E32[i1] * o;
-        for (int k = 0; k < np; ++k) {
-            const int64_t j = offset[p] + k;
-            const float* s = prod + 12 * p + 6 * k;
-            AppendSide sides[2] = {sec, ion};
-            for (int q = 0; q < 2; ++q) {
-                const AppendSide& d = sides[q];
-                const int64_t slot = d.base_slot + j;
-                d.cell[slot] = c;
-                d.off[slot] = o;
-                d.v0[slot] = s[3 * q] - d.half_qm_dt * ex;
-                d.v1[slot] = s[3 * q + 1];
-                d.v2[slot] = s[3 * q + 2];
-                d.uid[slot] = d.base_uid + static_cast<uint32_t>(j);
-            }
-        }
+        append_products_of(pcell[p], poff[p], np, offset[p], prod + 12 * p, E32, nx, sec, ion);
     }
 }
 
@@ -235,6 +123,33 @@ TallyOut tally_out(Context& c) {
 
 }  // namespace
 
+MccParams mcc_params(const Context& c) {
+    const MccModelHost& h = c.mcc;
+    MccParams m{};
+    m.nchan = h.nchan;
+    int total = 0;
+    for (int k = 0; k < h.nchan; ++k) {
+        m.kind[k] = h.kind[k];
+        m.threshold[k] = static_cast<float>(h.threshold[k]);
+        m.rows[k] = static_cast<int>(h.rows[k]);
+        m.offset[k] = static_cast<int>(h.row_offset[k]);
+        total += m.rows[k];
+    }
+    m.total_rows = total;
+    m.shared_grid = h.shared_grid ? 1 : 0;
+    m.energy = h.energy.ptr;
+    m.sigma = h.sigma.ptr;
+    m.neutral_density = static_cast<float>(h.neutral_density);
+    m.max_frequency = static_cast<float>(h.max_frequency);
+    m.inv_max_frequency = static_cast<float>(1.0 / h.max_frequency);
+    m.max_candidates = static_cast<int>(h.max_candidates);
+    m.mass = static_cast<float>(h.particle_mass);
+    m.half_mass = static_cast<float>(0.5 * h.particle_mass);
+    m.neutral_mass = static_cast<float>(h.neutral_mass);
+    m.neutral_vstd = static_cast<float>(h.neutral_vstd);
+    return m;
+}
+
 void mcc_step(Context& c, double dt) {
     MccModelHost& h = c.mcc;
     AUR_REQUIRE(c.dim == 1, AURORA_ERR_UNSUPPORTED,
@@ -248,7 +163,7 @@ void mcc_step(Context& c, double dt) {
         h.product_offset.alloc(n + 1);
         h.product_vel.alloc(static_cast<size_t>(n) * 12);
     }
-    const MccParams m = params(c);
+    const MccParams m = mcc_params(c);
     Mcc1dArgs a{};
     a.cell = sp.cell.ptr;
     a.off = reinterpret_cast<float*>(sp.off[0].ptr);
@@ -503,7 +418,7 @@ int aurora_hip_mcc_collide(aurora_hip_ctx* hctx, int64_t count, double* velocity
         ech.alloc(kMaxChannels);
         AUR_HIP(hipMemset(cnt.ptr, 0, (kMaxChannels + 3) * sizeof(unsigned long long)));
         AUR_HIP(hipMemset(ech.ptr, 0, kMaxChannels * sizeof(double)));
-        const MccParams m = params(c);
+        const MccParams m = mcc_params(c);
         {
             KernelScope ks(c, "collide");
             k_collide<<<blocks(c, count), kBlk, 0, c.stream>>>(

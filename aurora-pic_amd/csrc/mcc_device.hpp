@@ -9,6 +9,7 @@
 #pragma once
 
 #include "aurora_internal.hpp"
+#include "push_device.hpp"
 
 namespace aurora {
 
@@ -233,6 +234,110 @@ __device__ void mcc_collide(const MccParams& m, const float* E, const float* S, 
         if (m.kind[c] == AURORA_ATTACHMENT) break;
     }
 }
+
+// block-level tally reduction into global counters
+struct TallyOut {
+    unsigned long long* counts;  // [candidates, nulls, chan..., secondaries]
+    double* energy_change;       // per channel (J, unweighted)
+};
+
+__device__ void flush_tally(const MccParams& m, const MccTally& t, TallyOut out, bool active) {
+    // one atomic per counter per wave (wave-level reduction via ballot-free
+    // shuffles would reorder nothing: integer adds commute)
+    unsigned long long cand = active ? t.candidates : 0, nul = active ? t.nulls : 0;
+    unsigned long long sec = active ? static_cast<unsigned long long>(t.nsec) : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        cand += __shfl_down(cand, off, 64);
+        nul += __shfl_down(nul, off, 64);
+        sec += __shfl_down(sec, off, 64);
+    }
+    const bool lead = (threadIdx.x & 63) == 0;
+    if (lead) {
+        if (cand) atomicAdd(out.counts, cand);
+        if (nul) atomicAdd(out.counts + 1, nul);
+        if (sec) atomicAdd(out.counts + 2 + m.nchan, sec);
+    }
+    for (int c = 0; c < m.nchan; ++c) {
+        unsigned long long n = active ? t.chan[c] : 0;
+        double e = active ? static_cast<double>(t.energy_change[c]) : 0.0;
+        for (int off = 32; off > 0; off >>= 1) {
+            n += __shfl_down(n, off, 64);
+            e += __shfl_down(e, off, 64);
+        }
+        if (lead) {
+            if (n) atomicAdd(out.counts + 2 + c, n);
+            if (e != 0.0) atomicAdd(out.energy_change + c, e);
+        }
+    }
+}
+
+// One live 1D target particle of the MCC step (Simulation.cpp:900-926):
+// synchronise v from v_half with E(x) at t+dt (Pusher.cpp:74-76), collide,
+// rebuild v_half (Pusher.cpp:66-68).  Up to two ionization product pairs
+// (secondary v, ion v) are staged at prod[0..12); returns their number.
+__device__ __forceinline__ int mcc_1d_particle(const MccParams& m, const float* E, const float* S,
+                                               int c, float o, float* v0, float* v1, float* v2,
+                                               uint32_t uid, const float* E32, int nx,
+                                               float half_qm_dt, float dt, uint64_t seed,
+                                               uint32_t step, float* prod, MccTally& t,
+                                               uint32_t* err) {
+    const float ex = gather_1d_f32(E32, nx, c, o);
+    const float kick = half_qm_dt * ex;
+    float u[3] = {*v0 + kick, *v1, *v2};
+    Philox rng;
+    rng.init(seed, 0u, uid, step);
+    mcc_collide(m, E, S, u, dt, rng, t);
+    if (t.err) atomicOr(err, t.err);
+    *v0 = u[0] - kick;
+    *v1 = u[1];
+    *v2 = u[2];
+    const int np = t.nsec < 2 ? t.nsec : 2;
+    if (t.nsec > 2) atomicOr(err, DEV_ERR_CAPACITY);
+    for (int k = 0; k < np; ++k)
+        for (int q = 0; q < 3; ++q) {
+            prod[6 * k + q] = t.sec[k][q];
+            prod[6 * k + 3 + q] = t.ion[k][q];
+        }
+    return np;
+}
+
+struct AppendSide {
+    int32_t* cell;
+    float* off;
+    float* v0;
+    float* v1;
+    float* v2;
+    uint32_t* uid;
+    int64_t base_slot;
+    uint32_t base_uid;
+    float half_qm_dt;
+};
+
+// Write the np product pairs of one parent (cell c, offset o) at product
+// index j0.. of the secondary and ion species; their v_half comes from the
+// current field (Simulation.cpp:989-994).
+__device__ __forceinline__ void append_products_of(int c, float o, int np, int64_t j0,
+                                                   const float* prod, const float* E32, int nx,
+                                                   const AppendSide& sec, const AppendSide& ion) {
+    const float ex = gather_1d_f32(E32, nx, c, o);
+    for (int k = 0; k < np; ++k) {
+        const int64_t j = j0 + k;
+        const float* s = prod + 6 * k;
+        const AppendSide* sides[2] = {&sec, &ion};
+        for (int q = 0; q < 2; ++q) {
+            const AppendSide& d = *sides[q];
+            const int64_t slot = d.base_slot + j;
+            d.cell[slot] = c;
+            d.off[slot] = o;
+            d.v0[slot] = s[3 * q] - d.half_qm_dt * ex;
+            d.v1[slot] = s[3 * q + 1];
+            d.v2[slot] = s[3 * q + 2];
+            d.uid[slot] = d.base_uid + static_cast<uint32_t>(j);
+        }
+    }
+}
+
+MccParams mcc_params(const Context& c);
 
 // Stage the tables in LDS when they fit (kMccLdsRows), else read global.
 struct MccLds {

@@ -11,7 +11,8 @@
 // index strides are node counts, so node (i,j,k) of a cell is its corner.
 #include <cmath>
 
-#include "aurora_internal.hpp"
+#include "field_device.hpp"
+#include "push_device.hpp"
 
 namespace aurora {
 
@@ -70,11 +71,6 @@ struct PushArgs {
     double half_mw;        // 0.5 * m * w (1D loss ledger)
     uint32_t* err;
 };
-
-__device__ __forceinline__ int wrap_index(int i, int n) {
-    i %= n;
-    return i < 0 ? i + n : i;
-}
 
 // ---------------------------------------------------------------- B field
 __device__ __forceinline__ void b_table(const BView& b, double q, int cell_axis, double& Bx,
@@ -197,7 +193,7 @@ __device__ __forceinline__ void gather(const PushArgs<R>& a, const int* idx, con
         if constexpr (sizeof(R) == 8) {
             E[0] = a.E64[i0] * wx[0] + a.E64[i1] * wx[1];
         } else {
-            E[0] = a.E32[i0] * wx[0] + a.E32[i1] * wx[1];
+            E[0] = gather_1d_f32(a.E32, m.nx, i0, off[0]);
         }
         return;
     } else {
@@ -276,68 +272,6 @@ __device__ __forceinline__ void boris(R* v, const R* E, R Bx, R By, R Bz, R h) {
     v[0] = q0 + E[0] * h;
     v[1] = q1 + E[1] * h;
     v[2] = q2 + E[2] * h;
-}
-
-// ------------------------------------------------------------------ drift
-// Mirror (ni + f) into [0, ncells] (reflect_coordinate, Simulation3D.cpp:30-40)
-// in integer + fraction form.  A particle that would need more than 64
-// reflections (a non-finite or absurd velocity) is treated as lost.
-template <typename R>
-__device__ __forceinline__ bool reflect_into(int& ni, R& f, R& v, int ncells) {
-    for (int it = 0; it < 64; ++it) {
-        if (!(ni < 0 || ni > ncells || (ni == ncells && f > R(0)))) return true;
-        if (ni < -ncells || ni > 2 * ncells) return false;
-        if (ni < 0) {
-            if (f > R(0)) { ni = -ni - 1; f = R(1) - f; } else { ni = -ni; }
-        } else {
-            if (f > R(0)) { ni = 2 * ncells - ni - 1; f = R(1) - f; } else { ni = 2 * ncells - ni; }
-        }
-        v = -v;
-    }
-    return false;
-}
-
-// Moves one axis by dx (cell units) with the face policies of the axis.
-// Returns false when the particle is absorbed (face set).
-template <typename R>
-__device__ __forceinline__ bool drift_axis(int& idx, R& off, R& v, R dx, int ncells, int n,
-                                           int pol_lo, int pol_hi, int& face_hit) {
-    R x = off + dx;
-    if (x >= R(0) && x < R(1)) {
-        off = x;
-        return true;
-    }
-    R fl = floor(x);
-    int shift = static_cast<int>(fl);
-    R f = x - fl;
-    if (f >= R(1)) { f = R(0); shift += 1; }
-    int ni = idx + shift;
-    if (ni < 0) {
-        if (pol_lo == AURORA_PB_ABSORB) { face_hit = 0; return false; }
-        if (pol_lo == AURORA_PB_PERIODIC) {
-            idx = wrap_index(ni, ncells);
-            off = f;
-            return true;
-        }
-        // reflect about x = 0 (Simulation3D.cpp:30-40): -(ni + f)
-        if (!reflect_into(ni, f, v, ncells)) { face_hit = 0; return false; }
-    } else if (ni > ncells || (ni == ncells && f > R(0))) {
-        if (pol_hi == AURORA_PB_ABSORB) { face_hit = 1; return false; }
-        if (pol_hi == AURORA_PB_PERIODIC) {
-            idx = wrap_index(ni, ncells);
-            off = f;
-            return true;
-        }
-        if (!reflect_into(ni, f, v, ncells)) { face_hit = 1; return false; }
-    }
-    // x == L exactly on a Dirichlet axis: last cell with offset 1
-    if (ni == ncells) {
-        if (ncells == n) { ni = 0; f = R(0); }       // periodic mesh: wraps to node 0
-        else { ni = ncells - 1; f = R(1); }
-    }
-    idx = ni;
-    off = f;
-    return true;
 }
 
 // ---------------------------------------------------------------- deposit
@@ -555,40 +489,10 @@ __global__ void k_fill_i64(long long* p, int64_t n) {
         p[i] = 0;
 }
 
-struct RhoArgs {
-    const long long* acc[kMaxChannels];
-    double cw[kMaxChannels];   // charge * weight * 2^-40 (per unit depth in 2D)
-    int nspecies;
-    int nx, ny, nz;
-    int periodic;
-    double dx, dy, dz;
-    double vol;                // periodic node volume
-    int dim;
-};
-
 __global__ void k_rho(RhoArgs a, double* rho, int64_t nnodes) {
-#pragma clang fp contract(off)
     for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < nnodes;
-         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        double vol = a.vol;
-        if (!a.periodic) {
-            // Grid.cpp:26-30, Mesh2D.cpp:77-88, Mesh3D.cpp:52-60
-            const int ix = static_cast<int>(i % a.nx);
-            const int iy = static_cast<int>((i / a.nx) % a.ny);
-            const int iz = static_cast<int>(i / (static_cast<int64_t>(a.nx) * a.ny));
-            const double wx = (ix == 0 || ix + 1 == a.nx) ? 0.5 * a.dx : a.dx;
-            if (a.dim == 1) vol = wx;
-            else {
-                const double wy = (iy == 0 || iy + 1 == a.ny) ? 0.5 * a.dy : a.dy;
-                if (a.dim == 2) vol = wx * wy;
-                else vol = wx * wy * ((iz == 0 || iz + 1 == a.nz) ? 0.5 * a.dz : a.dz);
-            }
-        }
-        double r = 0.0;
-        for (int s = 0; s < a.nspecies; ++s)
-            r += (static_cast<double>(a.acc[s][i]) * a.cw[s]) / vol;
-        rho[i] = r;
-    }
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        rho[i] = rho_at(a, i);
 }
 
 // Uniform positions and Maxwellian velocities from Philox (Box-Muller).

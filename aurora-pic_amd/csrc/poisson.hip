@@ -10,7 +10,7 @@
 #include <cmath>
 #include <vector>
 
-#include "aurora_internal.hpp"
+#include "field_device.hpp"
 
 namespace aurora {
 
@@ -194,100 +194,18 @@ __global__ void __launch_bounds__(1024) k_spectral_1d(const double* rho, double*
                                                       float* E32, const double* cs,
                                                       const double* sn, int n, double L,
                                                       double eps) {
-#pragma clang fp contract(off)
     extern __shared__ double sh[];
-    double* r = sh;              // n
-    double* pr = sh + n;         // phi_k re
-    double* pim = sh + 2 * n;    // phi_k im
     __shared__ double mean_s;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) r[j] = rho[j];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = 0.0;
-        for (int j = 0; j < n; ++j) m += r[j];
-        mean_s = m / static_cast<double>(n);
-    }
-    __syncthreads();
-    const double mean = mean_s;
-    const double twopi = 6.283185307179586476925286766559;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        if (k == 0) { pr[0] = 0.0; pim[0] = 0.0; continue; }
-        double rr = 0.0, ri = 0.0;
-        int m = 0;
-        for (int j = 0; j < n; ++j) {
-            const double x = r[j] - mean;
-            rr += x * cs[m];
-            ri -= x * sn[m];
-            m += k;
-            if (m >= n) m -= n;
-        }
-        rr /= static_cast<double>(n);
-        ri /= static_cast<double>(n);
-        const double kk = (k <= n / 2) ? static_cast<double>(k) : -static_cast<double>(n - k);
-        const double wave = twopi * kk / L;
-        const double den = eps * wave * wave;
-        pr[k] = rr / den;
-        pim[k] = ri / den;
-    }
-    __syncthreads();
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        double ph = 0.0, e = 0.0;
-        int m = j;
-        for (int k = 1; k < n; ++k) {
-            const double c = cs[m], s = sn[m];
-            const double kk = (k <= n / 2) ? static_cast<double>(k) : -static_cast<double>(n - k);
-            const double wave = twopi * kk / L;
-            ph += pr[k] * c - pim[k] * s;
-            // E_k = -i wave phi_k = (wave pim, -wave pr)
-            e += (wave * pim[k]) * c - (-wave * pr[k]) * s;
-            m += j;
-            if (m >= n) m -= n;
-        }
-        phi[j] = ph;
-        E[j] = e;
-        E32[j] = static_cast<float>(e);
-    }
+    spectral_solve_1d(rho, phi, E, E32, cs, sn, n, L, eps, sh, &mean_s);
 }
 
-// 1D Dirichlet Thomas solve (FieldSolver.cpp:733-759): the coefficients
-// cp and denominators depend only on the index and are precomputed on the
-// host with the reference's operations; the data recurrence runs in one
-// lane, then the field is differenced in parallel.
+// 1D Dirichlet Thomas solve (FieldSolver.cpp:733-759)
 __global__ void __launch_bounds__(256) k_thomas_1d(const double* rho, double* phi, double* E,
                                                    float* E32, const double* cp,
                                                    const double* denom, double* dp, int n,
                                                    double dx, double eps, double phi_left,
                                                    double phi_right) {
-#pragma clang fp contract(off)
-    if (threadIdx.x == 0) {
-        const int m = n - 2;
-        const double dx2 = dx * dx;
-        phi[0] = phi_left;
-        phi[n - 1] = phi_right;
-        double prev = 0.0;
-        for (int i = 0; i < m; ++i) {
-            double di = rho[i + 1] * dx2 / eps;
-            if (i == 0) di += phi_left;
-            if (i == m - 1) di += phi_right;
-            prev = (i == 0) ? di / 2.0 : (di + prev) / denom[i];
-            dp[i] = prev;
-        }
-        double nxt = dp[m - 1];
-        phi[n - 2] = nxt;
-        for (int ii = m - 1; ii-- > 0;) {
-            nxt = dp[ii] - cp[ii] * nxt;
-            phi[ii + 1] = nxt;
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double e;
-        if (i == 0) e = -(phi[1] - phi[0]) / dx;
-        else if (i == n - 1) e = -(phi[n - 1] - phi[n - 2]) / dx;
-        else e = -(phi[i + 1] - phi[i - 1]) / (2.0 * dx);
-        E[i] = e;
-        E32[i] = static_cast<float>(e);
-    }
+    thomas_solve_1d(rho, phi, E, E32, cp, denom, dp, n, dx, eps, phi_left, phi_right);
 }
 
 AxisPlan plan_axis(const Context& c, int axis) {

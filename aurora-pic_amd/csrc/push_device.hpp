@@ -1,0 +1,86 @@
+// push_device.hpp - device helpers of the particle push shared by the
+// single-context kernels (particles.hip, mcc.hip) and the batched 1D
+// ensembles (ensemble.hip).  Sharing one definition keeps the FP32
+// operation sequence, and so the results, identical between them.
+#pragma once
+
+#include "aurora_internal.hpp"
+
+namespace aurora {
+
+__device__ __forceinline__ int wrap_index(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+// ------------------------------------------------------------------ drift
+// Mirror (ni + f) into [0, ncells] (reflect_coordinate, Simulation3D.cpp:30-40)
+// in integer + fraction form.  A particle that would need more than 64
+// reflections (a non-finite or absurd velocity) is treated as lost.
+template <typename R>
+__device__ __forceinline__ bool reflect_into(int& ni, R& f, R& v, int ncells) {
+    for (int it = 0; it < 64; ++it) {
+        if (!(ni < 0 || ni > ncells || (ni == ncells && f > R(0)))) return true;
+        if (ni < -ncells || ni > 2 * ncells) return false;
+        if (ni < 0) {
+            if (f > R(0)) { ni = -ni - 1; f = R(1) - f; } else { ni = -ni; }
+        } else {
+            if (f > R(0)) { ni = 2 * ncells - ni - 1; f = R(1) - f; } else { ni = 2 * ncells - ni; }
+        }
+        v = -v;
+    }
+    return false;
+}
+
+// Moves one axis by dx (cell units) with the face policies of the axis.
+// Returns false when the particle is absorbed (face set).
+template <typename R>
+__device__ __forceinline__ bool drift_axis(int& idx, R& off, R& v, R dx, int ncells, int n,
+                                           int pol_lo, int pol_hi, int& face_hit) {
+    R x = off + dx;
+    if (x >= R(0) && x < R(1)) {
+        off = x;
+        return true;
+    }
+    R fl = floor(x);
+    int shift = static_cast<int>(fl);
+    R f = x - fl;
+    if (f >= R(1)) { f = R(0); shift += 1; }
+    int ni = idx + shift;
+    if (ni < 0) {
+        if (pol_lo == AURORA_PB_ABSORB) { face_hit = 0; return false; }
+        if (pol_lo == AURORA_PB_PERIODIC) {
+            idx = wrap_index(ni, ncells);
+            off = f;
+            return true;
+        }
+        // reflect about x = 0 (Simulation3D.cpp:30-40): -(ni + f)
+        if (!reflect_into(ni, f, v, ncells)) { face_hit = 0; return false; }
+    } else if (ni > ncells || (ni == ncells && f > R(0))) {
+        if (pol_hi == AURORA_PB_ABSORB) { face_hit = 1; return false; }
+        if (pol_hi == AURORA_PB_PERIODIC) {
+            idx = wrap_index(ni, ncells);
+            off = f;
+            return true;
+        }
+        if (!reflect_into(ni, f, v, ncells)) { face_hit = 1; return false; }
+    }
+    // x == L exactly on a Dirichlet axis: last cell with offset 1
+    if (ni == ncells) {
+        if (ncells == n) { ni = 0; f = R(0); }       // periodic mesh: wraps to node 0
+        else { ni = ncells - 1; f = R(1); }
+    }
+    idx = ni;
+    off = f;
+    return true;
+}
+
+// 1D linear gather of the FP32 node field (FieldSolver.cpp:909-924); the
+// right node wraps to 0 on a periodic mesh (nx nodes, cell nx-1 ends at
+// node nx-1 on Dirichlet meshes, so the wrap never triggers there).
+__device__ __forceinline__ float gather_1d_f32(const float* E32, int nx, int i0, float o) {
+    const int i1 = (i0 + 1 == nx) ? 0 : i0 + 1;
+    return E32[i0] * (1.0f - o) + E32[i1] * o;
+}
+
+}  // namespace aurora

@@ -292,6 +292,55 @@ int aurora_hip_swarm_poll(aurora_hip_ctx* ctx, int64_t handle, int* done,
 /* Device bytes a request of this size holds while admitted. */
 uint64_t aurora_hip_swarm_footprint(const aurora_hip_swarm_request* request);
 
+/* ---------------------------------------------------------- ensembles */
+/* Batched 1D ensembles: many independent instances of one 1D discharge
+ * (Simulation::step, Simulation.cpp:1008-1138, with the MCC step and
+ * ionization products, Simulation.cpp:771-1006) advanced together.  The
+ * reference runs one Simulation per process and has no batched form; the
+ * per-instance operations are the context's own step, so an instance's
+ * particles are bit-identical to running it alone on a context.
+ * The template is the context's 1D mesh, FP32 species, particle
+ * boundaries, electrodes and MCC model at creation.  capacity: slots per
+ * instance, one per species. */
+int aurora_hip_ensemble_create(aurora_hip_ctx* ctx, int64_t instances,
+                               const int64_t* capacity, int64_t* handle);
+/* Copies the context's current particles and fields (e.g. after
+ * aurora_hip_initialize) into one instance; mcc_seed keys its collision
+ * RNG streams.  All instances must be loaded at the same time/step. */
+int aurora_hip_ensemble_load(aurora_hip_ctx* ctx, int64_t handle,
+                             int64_t instance, uint64_t mcc_seed);
+/* Steps captured per HIP graph (default 50; 0 = direct launches).  Dead
+ * slots are compacted (stably) after every graph. */
+int aurora_hip_ensemble_graph(aurora_hip_ctx* ctx, int64_t handle,
+                              int steps_per_graph);
+/* Advances every instance nsteps steps (field solves at t + dt with the
+ * context's electrode potentials). */
+int aurora_hip_ensemble_step(aurora_hip_ctx* ctx, int64_t handle, double dt,
+                             int64_t nsteps);
+int aurora_hip_ensemble_count(aurora_hip_ctx* ctx, int64_t handle,
+                              int64_t instance, int species, int64_t* slots,
+                              int64_t* live);
+/* One instance's particles: position (m), v_half (3 per particle), alive,
+ * uid; slots receives the slot count (live + dead). */
+int aurora_hip_ensemble_download(aurora_hip_ctx* ctx, int64_t handle,
+                                 int64_t instance, int species,
+                                 int64_t capacity, double* position,
+                                 double* velocity_half, uint8_t* alive,
+                                 uint32_t* uid, int64_t* slots);
+/* MCC counters [candidates, nulls, per channel..., secondaries], per
+ * channel energy change (J), wall counts and energies [species][left,right]
+ * of one instance (CollisionStepStatistics, BoundaryLoss1D). */
+int aurora_hip_ensemble_stats(aurora_hip_ctx* ctx, int64_t handle,
+                              int64_t instance, uint64_t* counts,
+                              double* energy_change, uint64_t* absorbed,
+                              double* absorbed_energy);
+int aurora_hip_ensemble_fields(aurora_hip_ctx* ctx, int64_t handle,
+                               int64_t instance, double* rho, double* phi,
+                               double* E);
+int aurora_hip_ensemble_time(aurora_hip_ctx* ctx, int64_t handle,
+                             double* time, int64_t* step);
+int aurora_hip_ensemble_destroy(aurora_hip_ctx* ctx, int64_t handle);
+
 /* -------------------------------------------------------------- timing */
 /* Per-kernel HIP-event timing on the context stream (off by default). */
 int aurora_hip_kernel_timing(aurora_hip_ctx* ctx, int enable);
