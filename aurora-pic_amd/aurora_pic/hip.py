@@ -18,6 +18,9 @@ HEADER = os.path.join(REPO, "include", "aurora_hip.h")
 PERIODIC, DIRICHLET = 0, 1
 PB_ABSORB, PB_REFLECT, PB_PERIODIC = 1, 2, 3
 VELOCITY_HALF, VELOCITY_CENTRED = 0, 1
+ELASTIC, EXCITATION, IONIZATION, ATTACHMENT, CHARGE_EXCHANGE = range(5)
+SCATTER_ISOTROPIC, SCATTER_BACKWARD, SCATTER_HENYEY_GREENSTEIN = range(3)
+FRAME_PROJECTILE, FRAME_CENTER_OF_MASS = range(2)
 (OK, ERR_INVALID_ARGUMENT, ERR_HIP, ERR_RATE_ABOVE_BOUND, ERR_CANDIDATE_LIMIT, ERR_CAPACITY,
  ERR_B_OUT_OF_RANGE, ERR_ENERGY_ABOVE_MAX, ERR_UNSUPPORTED, ERR_NOT_FOUND) = range(10)
 
@@ -127,6 +130,8 @@ def load(path: str = LIB_PATH):
                                 C.POINTER(_dp), C.c_double, C.c_double, C.c_int64, C.c_double,
                                 C.c_double, _ip, C.c_uint64],
         "aurora_hip_mcc_stats": [ctx, _u64p, _dp],
+        "aurora_hip_mcc_channel_options": [ctx, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64,
+                                           _dp, _dp],
         "aurora_hip_mcc_collide": [ctx, C.c_int64, _dp, C.c_double, C.c_uint64, C.c_uint32,
                                    _u64p, _dp],
         "aurora_hip_swarm_gas": [ctx, C.c_int, _ip, _dp, _i64p, C.POINTER(_dp), C.POINTER(_dp),
@@ -397,6 +402,19 @@ class Context:
             gas["neutral_density"], gas["max_frequency"], gas.get("max_candidates", 64),
             gas["neutral_mass"], neutral_temperature, prod, seed))
         self._nchan = nch
+
+    def mcc_channel_options(self, channel, angular=0, frame=0, mc_energy=None, mc_value=None,
+                            model=0):
+        """CollisionChannelConfig angular model / energy frame of one channel of
+        the context's MCC model (model=0) or the swarm gas (model=1)."""
+        if mc_energy is None:
+            self._check(self.lib.aurora_hip_mcc_channel_options(self.h, model, channel, angular,
+                                                                frame, 0, None, None))
+            return
+        e, v = _f64(mc_energy), _f64(mc_value)
+        self._check(self.lib.aurora_hip_mcc_channel_options(self.h, model, channel, angular,
+                                                            frame, len(e), _p(e, _dp),
+                                                            _p(v, _dp)))
 
     def mcc_stats(self):
         counts = np.zeros(3 + self._nchan, dtype=np.uint64)

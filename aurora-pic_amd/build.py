@@ -42,6 +42,34 @@ def _compile(src: str) -> str:
     return obj
 
 
+HOST = os.path.join(HERE, "host")
+HOST_SO = os.path.join(LIB, "libaurora_gpu.so")
+HOST_TESTS = os.path.join(LIB, "aurora_host_tests")
+CXX = os.environ.get("CXX", "g++")
+
+
+def build_host() -> str:
+    """C++20 host classes (host/aurora_gpu.cpp) over the C ABI, and their
+    test program; plain g++, linked against libaurora_hip.so."""
+    inc = [f"-I{os.path.join(REPO, 'include')}", f"-I{HOST}"]
+    rpath = "-Wl,-rpath,$ORIGIN"
+    cmd = [CXX, "-std=c++20", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", *inc,
+           os.path.join(HOST, "aurora_gpu.cpp"), "-o", HOST_SO + ".tmp", f"-L{LIB}",
+           "-laurora_hip", rpath]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host library build failed:\n{r.stderr[-6000:]}")
+    os.replace(HOST_SO + ".tmp", HOST_SO)
+    cmd = [CXX, "-std=c++20", "-O2", "-Wall", "-Wextra", *inc,
+           os.path.join(HOST, "tests", "host_tests.cpp"), "-o", HOST_TESTS + ".tmp", f"-L{LIB}",
+           "-laurora_gpu", "-laurora_hip", rpath]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host tests build failed:\n{r.stderr[-6000:]}")
+    os.replace(HOST_TESTS + ".tmp", HOST_TESTS)
+    return HOST_SO
+
+
 def build(jobs: int = 8) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
@@ -54,6 +82,7 @@ def build(jobs: int = 8) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, SO)
+    build_host()
     return SO
 
 

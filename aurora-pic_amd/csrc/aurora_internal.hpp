@@ -117,7 +117,16 @@ struct MccModelHost {
     double threshold[kMaxChannels] = {};
     int64_t rows[kMaxChannels] = {};
     int64_t row_offset[kMaxChannels] = {};
-    bool shared_grid = false;
+    bool shared_grid = false;            // every channel has channel 0's energy grid
+    int angular[kMaxChannels] = {};      // SCATTER_* per channel
+    int frame[kMaxChannels] = {};        // 0 projectile, 1 centre of mass
+    double table_lo[kMaxChannels] = {};  // first / last energy of each table
+    double table_hi[kMaxChannels] = {};
+    std::vector<double> mc_energy_host[kMaxChannels];   // mean-cosine tables
+    std::vector<double> mc_value_host[kMaxChannels];
+    DevBuf<float> mc_energy, mc_value;   // concatenated FP32 mean-cosine tables
+    int64_t mc_rows[kMaxChannels] = {};
+    int64_t mc_offset[kMaxChannels] = {};
     double neutral_density = 0, max_frequency = 0, neutral_mass = 0, neutral_vstd = 0;
     double particle_mass = 0;
     int64_t max_candidates = 64;
@@ -125,8 +134,9 @@ struct MccModelHost {
     uint64_t seed = 0;
     DevBuf<float> energy;   // all channels concatenated (FP32 tables, LDS-staged)
     DevBuf<float> sigma;
-    DevBuf<double> energy64;  // FP64 copies (thermal bound scan)
-    DevBuf<double> sigma64;
+    double safe_speed = 0.0;             // rate bound cannot fail below it (cold neutral)
+    std::vector<double> energy_host[kMaxChannels];   // FP64 tables as loaded
+    std::vector<double> sigma_host[kMaxChannels];
     DevBuf<unsigned long long> counts;  // [candidates, nulls, chan...]
     DevBuf<double> energy_change;       // per channel, per-block partials
     DevBuf<int32_t> product_count;      // per target particle
@@ -280,6 +290,8 @@ void mcc_step(Context& ctx, double dt);
 void init_half_all(Context& ctx, double dt);
 void set_b_profile_device(Context& ctx);
 void swarm_destroy(Context& ctx);
+struct MccModelHost;
+MccModelHost& swarm_gas_model(Context& ctx);
 void ensembles_destroy(Context& ctx);
 
 // ------------------------------------------------------- device helpers
@@ -294,7 +306,7 @@ __device__ __forceinline__ long long quantise(double s) {
 struct Philox {
     uint32_t key0, key1;
     uint32_t c0, c1, c2, c3;
-    uint32_t buf[4];
+    uint32_t b0, b1, b2, b3;   // current block; b0 is the next draw
     int used;
     __device__ __forceinline__ void init(uint64_t seed, uint32_t stream, uint32_t uid,
                                          uint32_t step) {
@@ -316,14 +328,19 @@ struct Philox {
             const uint32_t n2 = hi0 ^ x3 ^ k1;
             x0 = n0; x1 = lo1; x2 = n2; x3 = lo0;
         }
-        buf[0] = x0; buf[1] = x1; buf[2] = x2; buf[3] = x3;
+        b0 = x0; b1 = x1; b2 = x2; b3 = x3;
         c0 += 1;
         used = 0;
     }
-    // odd multiples of 2^-24 in (0,1), exact in FP32
+    // odd multiples of 2^-24 in (0,1), exact in FP32; the block is consumed
+    // by shifting (no dynamically indexed array, which would live in scratch)
     __device__ __forceinline__ float u01() {
         if (used == 4) refill();
-        const uint32_t u = buf[used++];
+        const uint32_t u = b0;
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        ++used;
         return static_cast<float>((u >> 9) * 2u + 1u) * 0x1p-24f;
     }
 };

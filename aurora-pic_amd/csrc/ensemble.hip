@@ -218,12 +218,11 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_
 
 // MCC on the target species: one workgroup per 256 target slots of one
 // instance; product pairs are staged per parent with in-chunk offsets.
+template <int NCH>
 __global__ void __launch_bounds__(kEnsThreads) k_ens_mcc(EnsArgs a, MccParams m) {
     __shared__ MccLds lds;
     __shared__ int scan_sh[kEnsThreads];
-    stage_tables(m, lds);
-    const float* E = m.total_rows <= kMccLdsRows ? lds.E : m.energy;
-    const float* S = m.total_rows <= kMccLdsRows ? lds.S : m.sigma;
+    const XsView X = stage_tables(m, lds);
     const int inst = blockIdx.x / a.mchunks;
     const int chunk = blockIdx.x - inst * a.mchunks;
     const EnsSpecies& sp = a.sp[a.target];
@@ -238,14 +237,14 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_mcc(EnsArgs a, MccParams m)
     const bool in = q < n;
     const int c = in ? sp.cell[p] : -1;
     const bool active = c >= 0;
-    MccTally t{};
+    MccTally<NCH> t{};
     int np = 0;
     if (active)
-        np = mcc_1d_particle(m, E, S, c, sp.off[p], sp.v0 + p, sp.v1 + p, sp.v2 + p, sp.uid[p],
+        np = mcc_1d_particle<NCH>(m, X, lds.chan, c, sp.off[p], sp.v0 + p, sp.v1 + p, sp.v2 + p, sp.uid[p],
                              a.E32 + static_cast<int64_t>(inst) * a.nx, a.nx, sp.half_qm_dt,
                              a.dt, a.seeds[inst], static_cast<uint32_t>(*a.step),
                              a.prod + 12 * p, t, a.err + inst);
-    flush_tally(m, t,
+    flush_tally<NCH>(t,
                 TallyOut{a.counts + static_cast<int64_t>(inst) * (kMaxChannels + 3),
                          a.ech + static_cast<int64_t>(inst) * kMaxChannels},
                 active);
@@ -636,7 +635,9 @@ void enqueue_step(Context& c, Ensemble& e, const EnsArgs& a, const MccParams& m)
     k_ens_field<<<a.M, kEnsThreads, shmem, c.stream>>>(a, 1);
     if (a.target >= 0) {
         const int g = a.M * a.mchunks;
-        k_ens_mcc<<<g, kEnsThreads, 0, c.stream>>>(a, m);
+        with_nchan(m.nchan, [&]<int NCH>() {
+            k_ens_mcc<NCH><<<g, kEnsThreads, 0, c.stream>>>(a, m);
+        });
         if (a.sec >= 0) {
             k_ens_products<<<a.M, kEnsThreads, 0, c.stream>>>(a);
             k_ens_append<<<g, kEnsThreads, 0, c.stream>>>(a);

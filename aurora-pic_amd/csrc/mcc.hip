@@ -9,7 +9,10 @@
 //   the capacity check is all-or-nothing (Simulation.cpp:929-964), products
 //   never collide in their birth step, and their v_half is initialised from
 //   the current field (Simulation.cpp:965-1005).
+#include <algorithm>
 #include <cmath>
+#include <limits>
+#include <string>
 #include <cstring>
 #include <string.h>
 #include <vector>
@@ -27,29 +30,28 @@ constexpr int kBlk = 256;
 
 
 // standalone collide of count velocities (parity checks): uid = index
+template <int NCH>
 __global__ void __launch_bounds__(kBlk) k_collide(MccParams m, float* v, int64_t count,
                                                   float dt, uint64_t seed, uint32_t stream,
                                                   uint32_t step, TallyOut out, uint32_t* err) {
     __shared__ MccLds lds;
-    stage_tables(m, lds);
-    const float* E = m.total_rows <= kMccLdsRows ? lds.E : m.energy;
-    const float* S = m.total_rows <= kMccLdsRows ? lds.S : m.sigma;
+    const XsView X = stage_tables(m, lds);
     for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < count;
          base += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int64_t p = base + threadIdx.x;
         const bool active = p < count;
-        MccTally t{};
+        MccTally<NCH> t{};
         if (active) {
             float u[3] = {v[3 * p], v[3 * p + 1], v[3 * p + 2]};
             Philox rng;
             rng.init(seed, stream, static_cast<uint32_t>(p), step);
-            mcc_collide(m, E, S, u, dt, rng, t);
+            mcc_collide<NCH>(m, X, lds.chan, u, dt, rng, t);
             if (t.err) atomicOr(err, t.err);
             v[3 * p] = u[0];
             v[3 * p + 1] = u[1];
             v[3 * p + 2] = u[2];
         }
-        flush_tally(m, t, out, active);
+        flush_tally<NCH>(t, out, active);
     }
 }
 
@@ -73,11 +75,10 @@ struct Mcc1dArgs {
 };
 
 // sync -> collide -> rebuild v_half for every live target particle
+template <int NCH>
 __global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, TallyOut out) {
     __shared__ MccLds lds;
-    stage_tables(m, lds);
-    const float* E = m.total_rows <= kMccLdsRows ? lds.E : m.energy;
-    const float* S = m.total_rows <= kMccLdsRows ? lds.S : m.sigma;
+    const XsView X = stage_tables(m, lds);
     for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < a.slots;
          base += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int64_t p = base + threadIdx.x;
@@ -88,12 +89,12 @@ __global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, Tally
             a.nprod[p] = 0;
             active = c >= 0;
         }
-        MccTally t{};
+        MccTally<NCH> t{};
         if (active)
-            a.nprod[p] = mcc_1d_particle(m, E, S, c, a.off[p], a.v0 + p, a.v1 + p, a.v2 + p, a.uid[p],
+            a.nprod[p] = mcc_1d_particle<NCH>(m, X, lds.chan, c, a.off[p], a.v0 + p, a.v1 + p, a.v2 + p, a.uid[p],
                                          a.E32, a.nx, a.half_qm_dt, a.dt, a.seed, a.step,
                                          a.prod + 12 * p, t, a.err);
-        flush_tally(m, t, out, active);
+        flush_tally<NCH>(t, out, active);
     }
 }
 
@@ -123,22 +124,94 @@ TallyOut tally_out(Context& c) {
 
 }  // namespace
 
-MccParams mcc_params(const Context& c) {
-    const MccModelHost& h = c.mcc;
+// Largest relative speed below which the total collision rate of the model
+// (cold neutral) stays under max_frequency (1 - 1e-5) for every speed: the
+// per-candidate bound check (Collision.cpp:617-623) cannot fail there, with
+// room for the FP32 evaluation's rounding.  Per interval between the speed
+// breakpoints (table rows and thresholds of every channel) each sigma is
+// linear in E = h g^2, so the rate is n (A g + B g^3), whose maximum is at an
+// end or at g^2 = -A / (3 B); the first interval whose maximum reaches the
+// limit ends the safe range.  Beyond the last breakpoint sigma is constant.
+double rate_safe_speed(const MccModelHost& h) {
+    const double limit = h.max_frequency * (1.0 - 1.0e-5);
+    double hc[kMaxChannels];
+    std::vector<double> br = {0.0};
+    for (int c = 0; c < h.nchan; ++c) {
+        const double em = h.frame[c] == AURORA_FRAME_CENTER_OF_MASS
+                              ? h.particle_mass * h.neutral_mass / (h.particle_mass + h.neutral_mass)
+                              : h.particle_mass;
+        hc[c] = 0.5 * em;
+        for (double e : h.energy_host[c]) br.push_back(std::sqrt(e / hc[c]));
+        br.push_back(std::sqrt(h.threshold[c] / hc[c]));
+    }
+    std::sort(br.begin(), br.end());
+    br.erase(std::unique(br.begin(), br.end()), br.end());
+    // linear form n (A g + B g^3) of the rate on the interval holding g_mid
+    auto form = [&](double g_mid, double& A, double& B) {
+        A = B = 0.0;
+        for (int c = 0; c < h.nchan; ++c) {
+            const double e = hc[c] * g_mid * g_mid;
+            if (e < h.threshold[c]) continue;
+            const auto& E = h.energy_host[c];
+            const auto& S = h.sigma_host[c];
+            const size_t n = E.size();
+            if (e <= E[0]) { A += S[0]; continue; }
+            if (e >= E[n - 1]) { A += S[n - 1]; continue; }
+            const size_t i = static_cast<size_t>(std::upper_bound(E.begin(), E.end(), e) -
+                                                 E.begin()) - 1;
+            const double k = (S[i + 1] - S[i]) / (E[i + 1] - E[i]);
+            A += S[i] - k * E[i];
+            B += k * hc[c];
+        }
+        A *= h.neutral_density;
+        B *= h.neutral_density;
+    };
+    for (size_t j = 0; j + 1 < br.size(); ++j) {
+        const double g0 = br[j], g1 = br[j + 1];
+        double A, B;
+        form(0.5 * (g0 + g1), A, B);
+        auto R = [&](double g) { return A * g + B * g * g * g; };
+        double mx = std::max(R(g0), R(g1));
+        if (B != 0.0) {
+            const double gc2 = -A / (3.0 * B);
+            if (gc2 > g0 * g0 && gc2 < g1 * g1) mx = std::max(mx, R(std::sqrt(gc2)));
+        }
+        if (!(mx < limit)) return g0;
+    }
+    double A, B;
+    form(br.back() * 2.0 + 1.0, A, B);        // constant sigmas past the tables
+    return A > 0.0 ? std::max(br.back(), limit / A) : std::numeric_limits<double>::infinity();
+}
+
+MccParams mcc_params_of(const MccModelHost& h) {
     MccParams m{};
     m.nchan = h.nchan;
     int total = 0;
+    bool same_frame = true;
     for (int k = 0; k < h.nchan; ++k) {
         m.kind[k] = h.kind[k];
+        m.angular[k] = h.angular[k];
         m.threshold[k] = static_cast<float>(h.threshold[k]);
+        // Collision.cpp:579-594: projectile mass or reduced mass
+        const double emass = h.frame[k] == AURORA_FRAME_CENTER_OF_MASS
+                                 ? h.particle_mass * h.neutral_mass /
+                                       (h.particle_mass + h.neutral_mass)
+                                 : h.particle_mass;
+        m.half_emass[k] = static_cast<float>(0.5 * emass);
         m.rows[k] = static_cast<int>(h.rows[k]);
         m.offset[k] = static_cast<int>(h.row_offset[k]);
+        m.mc_rows[k] = static_cast<int>(h.mc_rows[k]);
+        m.mc_offset[k] = static_cast<int>(h.mc_offset[k]);
         total += m.rows[k];
+        same_frame = same_frame && h.frame[k] == h.frame[0];
     }
     m.total_rows = total;
-    m.shared_grid = h.shared_grid ? 1 : 0;
+    m.shared_grid = h.shared_grid && same_frame ? 1 : 0;
+    m.grid_shared = h.shared_grid ? 1 : 0;
     m.energy = h.energy.ptr;
     m.sigma = h.sigma.ptr;
+    m.mc_energy = h.mc_energy.ptr;
+    m.mc_value = h.mc_value.ptr;
     m.neutral_density = static_cast<float>(h.neutral_density);
     m.max_frequency = static_cast<float>(h.max_frequency);
     m.inv_max_frequency = static_cast<float>(1.0 / h.max_frequency);
@@ -147,8 +220,14 @@ MccParams mcc_params(const Context& c) {
     m.half_mass = static_cast<float>(0.5 * h.particle_mass);
     m.neutral_mass = static_cast<float>(h.neutral_mass);
     m.neutral_vstd = static_cast<float>(h.neutral_vstd);
+    const double gs = h.safe_speed;
+    m.safe_speed = std::isfinite(gs) && gs < 3.0e38
+                       ? std::nextafter(static_cast<float>(gs), 0.0f) * (1.0f - 1.0e-6f)
+                       : 3.0e38f;
     return m;
 }
+
+MccParams mcc_params(const Context& c) { return mcc_params_of(c.mcc); }
 
 void mcc_step(Context& c, double dt) {
     MccModelHost& h = c.mcc;
@@ -183,7 +262,9 @@ void mcc_step(Context& c, double dt) {
     a.err = c.dev_error.ptr;
     {
         KernelScope ks(c, "mcc");
-        k_mcc_1d<<<blocks(c, n), kBlk, 0, c.stream>>>(m, a, tally_out(c));
+        with_nchan(m.nchan, [&]<int NCH>() {
+            k_mcc_1d<NCH><<<blocks(c, n), kBlk, 0, c.stream>>>(m, a, tally_out(c));
+        });
         AUR_HIP(hipGetLastError());
     }
     if (h.secondary_species < 0) return;
@@ -256,9 +337,9 @@ void mcc_step(Context& c, double dt) {
 
 using namespace aurora;
 
-namespace {
+namespace aurora {
 
-void load_model(Context& c, MccModelHost& h, int nchan, const int* kind, const double* threshold,
+void load_mcc_model(MccModelHost& h, int nchan, const int* kind, const double* threshold,
                 const int64_t* rows, const double* const* energy, const double* const* sigma,
                 double neutral_density, double max_frequency, int64_t max_candidates,
                 double particle_mass, double neutral_mass, double neutral_temperature) {
@@ -293,12 +374,20 @@ void load_model(Context& c, MccModelHost& h, int nchan, const int* kind, const d
     std::vector<float> e32, s32;
     int64_t off = 0;
     for (int k = 0; k < nchan; ++k) {
-        AUR_REQUIRE(kind[k] >= AURORA_ELASTIC && kind[k] <= AURORA_ATTACHMENT,
+        AUR_REQUIRE(kind[k] >= AURORA_ELASTIC && kind[k] <= AURORA_CHARGE_EXCHANGE,
                     AURORA_ERR_INVALID_ARGUMENT, "unknown MCC process kind");
         AUR_REQUIRE(std::isfinite(threshold[k]) && threshold[k] >= 0.0,
                     AURORA_ERR_INVALID_ARGUMENT,
                     "MCC threshold_energy must be finite and non-negative");
-        if (kind[k] == AURORA_ELASTIC || kind[k] == AURORA_ATTACHMENT)
+        if (kind[k] == AURORA_CHARGE_EXCHANGE) {
+            // Collision.cpp:520-533
+            const double tol = 64.0 * 2.220446049250313e-16 * std::max(particle_mass, neutral_mass);
+            AUR_REQUIRE(neutral_mass > 0.0 && std::abs(particle_mass - neutral_mass) <= tol,
+                        AURORA_ERR_INVALID_ARGUMENT,
+                        "resonant charge exchange requires projectile mass equal to neutral_mass");
+        }
+        if (kind[k] == AURORA_ELASTIC || kind[k] == AURORA_ATTACHMENT ||
+            kind[k] == AURORA_CHARGE_EXCHANGE)
             AUR_REQUIRE(threshold[k] == 0.0, AURORA_ERR_INVALID_ARGUMENT,
                         "elastic, attachment, and charge-exchange MCC channel "
                         "threshold_energy must be zero");
@@ -317,12 +406,24 @@ void load_model(Context& c, MccModelHost& h, int nchan, const int* kind, const d
             e32.push_back(static_cast<float>(energy[k][i]));
             s32.push_back(static_cast<float>(sigma[k][i]));
         }
+        h.energy_host[k].assign(energy[k], energy[k] + rows[k]);
+        h.sigma_host[k].assign(sigma[k], sigma[k] + rows[k]);
         h.kind[k] = kind[k];
         h.threshold[k] = threshold[k];
         h.rows[k] = rows[k];
         h.row_offset[k] = off;
+        h.table_lo[k] = energy[k][0];
+        h.table_hi[k] = energy[k][rows[k] - 1];
+        h.angular[k] = SCATTER_ISOTROPIC;
+        h.frame[k] = AURORA_FRAME_PROJECTILE;
+        h.mc_rows[k] = 0;
+        h.mc_offset[k] = 0;
+        h.mc_energy_host[k].clear();
+        h.mc_value_host[k].clear();
         off += rows[k];
     }
+    h.mc_energy.release();
+    h.mc_value.release();
     h.shared_grid = true;
     for (int k = 1; k < nchan; ++k) {
         if (rows[k] != rows[0]) { h.shared_grid = false; break; }
@@ -332,6 +433,7 @@ void load_model(Context& c, MccModelHost& h, int nchan, const int* kind, const d
                 break;
             }
     }
+    h.safe_speed = rate_safe_speed(h);
     h.energy.alloc(e32.size());
     h.sigma.alloc(s32.size());
     AUR_HIP(hipMemcpy(h.energy.ptr, e32.data(), e32.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -340,10 +442,80 @@ void load_model(Context& c, MccModelHost& h, int nchan, const int* kind, const d
     h.energy_change.alloc(kMaxChannels);
     AUR_HIP(hipMemset(h.counts.ptr, 0, (kMaxChannels + 3) * sizeof(unsigned long long)));
     AUR_HIP(hipMemset(h.energy_change.ptr, 0, kMaxChannels * sizeof(double)));
-    (void)c;
 }
 
-}  // namespace
+void set_channel_options(MccModelHost& h, int channel, int angular, int frame, int64_t rows,
+                         const double* energy, const double* mean_cosine) {
+    AUR_REQUIRE(h.nchan > 0, AURORA_ERR_INVALID_ARGUMENT, "no MCC model loaded");
+    AUR_REQUIRE(channel >= 0 && channel < h.nchan, AURORA_ERR_NOT_FOUND,
+                "MCC channel index out of range");
+    AUR_REQUIRE(angular >= SCATTER_ISOTROPIC && angular <= SCATTER_HENYEY_GREENSTEIN,
+                AURORA_ERR_INVALID_ARGUMENT, "unknown angular scattering model");
+    AUR_REQUIRE(frame == AURORA_FRAME_PROJECTILE || frame == AURORA_FRAME_CENTER_OF_MASS,
+                AURORA_ERR_INVALID_ARGUMENT, "unknown collision energy frame");
+    const int kind = h.kind[channel];
+    // Collision.cpp:455-500
+    AUR_REQUIRE(angular == SCATTER_ISOTROPIC || kind == AURORA_ELASTIC,
+                AURORA_ERR_INVALID_ARGUMENT,
+                "anisotropic MCC scattering is supported only for elastic channels");
+    if (angular == SCATTER_HENYEY_GREENSTEIN) {
+        AUR_REQUIRE(rows >= 2 && energy && mean_cosine, AURORA_ERR_INVALID_ARGUMENT,
+                    "Henyey-Greenstein scattering requires mean_cosine_file");
+        for (int64_t i = 0; i < rows; ++i) {
+            AUR_REQUIRE(std::isfinite(energy[i]) && energy[i] >= 0.0 &&
+                            std::isfinite(mean_cosine[i]) && std::abs(mean_cosine[i]) < 1.0,
+                        AURORA_ERR_INVALID_ARGUMENT,
+                        "mean-cosine table requires finite non-negative energies and mean "
+                        "cosines strictly between -1 and 1");
+            AUR_REQUIRE(i == 0 || energy[i] > energy[i - 1], AURORA_ERR_INVALID_ARGUMENT,
+                        "mean-cosine table energies must be strictly increasing");
+        }
+        AUR_REQUIRE(energy[0] <= h.table_lo[channel] && energy[rows - 1] >= h.table_hi[channel],
+                    AURORA_ERR_INVALID_ARGUMENT,
+                    "mean-cosine energy range must cover the cross-section table for channel " +
+                        std::to_string(channel));
+    } else {
+        AUR_REQUIRE(rows == 0, AURORA_ERR_INVALID_ARGUMENT,
+                    "mean-cosine data requires angular_model = henyey_greenstein");
+    }
+    if (frame == AURORA_FRAME_CENTER_OF_MASS) {
+        AUR_REQUIRE(h.neutral_mass > 0.0, AURORA_ERR_INVALID_ARGUMENT,
+                    "center-of-mass collision energy requires positive neutral_mass");
+        AUR_REQUIRE(kind == AURORA_ELASTIC || kind == AURORA_CHARGE_EXCHANGE,
+                    AURORA_ERR_INVALID_ARGUMENT,
+                    "center-of-mass collision energy is supported only for elastic and "
+                    "charge-exchange channels");
+    }
+    h.angular[channel] = angular;
+    h.frame[channel] = frame;
+    h.safe_speed = rate_safe_speed(h);
+    h.mc_energy_host[channel].assign(energy && rows ? energy : nullptr,
+                                     energy && rows ? energy + rows : nullptr);
+    h.mc_value_host[channel].assign(mean_cosine && rows ? mean_cosine : nullptr,
+                                    mean_cosine && rows ? mean_cosine + rows : nullptr);
+    // re-concatenate every channel's mean-cosine table (FP32)
+    std::vector<float> e32, v32;
+    for (int k = 0; k < h.nchan; ++k) {
+        h.mc_offset[k] = static_cast<int64_t>(e32.size());
+        h.mc_rows[k] = static_cast<int64_t>(h.mc_energy_host[k].size());
+        for (size_t i = 0; i < h.mc_energy_host[k].size(); ++i) {
+            e32.push_back(static_cast<float>(h.mc_energy_host[k][i]));
+            v32.push_back(static_cast<float>(h.mc_value_host[k][i]));
+        }
+    }
+    h.mc_energy.release();
+    h.mc_value.release();
+    if (!e32.empty()) {
+        h.mc_energy.alloc(e32.size());
+        h.mc_value.alloc(v32.size());
+        AUR_HIP(hipMemcpy(h.mc_energy.ptr, e32.data(), e32.size() * sizeof(float),
+                          hipMemcpyHostToDevice));
+        AUR_HIP(hipMemcpy(h.mc_value.ptr, v32.data(), v32.size() * sizeof(float),
+                          hipMemcpyHostToDevice));
+    }
+}
+
+}  // namespace aurora
 
 extern "C" {
 
@@ -357,7 +529,7 @@ int aurora_hip_mcc_load(aurora_hip_ctx* hctx, int target, int nchan, const int* 
         Species& sp = species_at(c, target);
         MccModelHost& h = c.mcc;
         h.loaded = false;
-        load_model(c, h, nchan, kind, threshold, rows, energy, sigma, neutral_density,
+        load_mcc_model(h, nchan, kind, threshold, rows, energy, sigma, neutral_density,
                    max_frequency, max_candidates, sp.mass, neutral_mass, neutral_temperature);
         h.target = target;
         h.seed = seed;
@@ -375,6 +547,18 @@ int aurora_hip_mcc_load(aurora_hip_ctx* hctx, int target, int nchan, const int* 
             h.ion_species = product_species[1];
         }
         h.loaded = true;
+    });
+}
+
+int aurora_hip_mcc_channel_options(aurora_hip_ctx* hctx, int model, int channel, int angular,
+                                   int energy_frame, int64_t rows, const double* energy,
+                                   const double* mean_cosine) {
+    return guarded(hctx, [&](Context& c) {
+        AUR_REQUIRE(model == 0 || model == 1, AURORA_ERR_INVALID_ARGUMENT,
+                    "model must be 0 (context MCC) or 1 (swarm gas)");
+        AUR_HIP(hipStreamSynchronize(c.stream));
+        set_channel_options(model == 0 ? c.mcc : swarm_gas_model(c), channel, angular,
+                            energy_frame, rows, energy, mean_cosine);
     });
 }
 
@@ -421,9 +605,11 @@ int aurora_hip_mcc_collide(aurora_hip_ctx* hctx, int64_t count, double* velocity
         const MccParams m = mcc_params(c);
         {
             KernelScope ks(c, "collide");
-            k_collide<<<blocks(c, count), kBlk, 0, c.stream>>>(
-                m, dv.ptr, count, static_cast<float>(dt), h.seed, stream_id,
-                static_cast<uint32_t>(step), TallyOut{cnt.ptr, ech.ptr}, c.dev_error.ptr);
+            with_nchan(m.nchan, [&]<int NCH>() {
+                k_collide<NCH><<<blocks(c, count), kBlk, 0, c.stream>>>(
+                    m, dv.ptr, count, static_cast<float>(dt), h.seed, stream_id,
+                    static_cast<uint32_t>(step), TallyOut{cnt.ptr, ech.ptr}, c.dev_error.ptr);
+            });
             AUR_HIP(hipGetLastError());
         }
         AUR_HIP(hipStreamSynchronize(c.stream));

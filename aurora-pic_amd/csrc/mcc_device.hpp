@@ -1,11 +1,19 @@
 // mcc_device.hpp - device side of the null-collision MCC (FP32), shared by
-// the 1D collision step (mcc.hip) and the batched swarm kernel (swarm.hip).
+// the standalone collide (mcc.hip), the 1D collision step (mcc.hip,
+// ensemble.hip) and the batched swarm kernel (swarm.hip).
 //
 // Restates NullCollisionModel::collide (Collision.cpp:907-993) with
+// collision_energy (579-594: projectile or centre-of-mass frame),
 // rates_for_speed (596-625), the thermal frequency bound (627-685),
-// apply_channel for isotropic scattering (766-829) and the ionization
-// secondary (966-977).  Uniforms come from Philox4x32-10 in a fixed draw
-// order, the same order as the oracle's or_mcc_collide_3v.
+// apply_channel (738-829: attachment, resonant charge exchange, isotropic,
+// backward and Henyey-Greenstein scattering, finite-mass elastic
+// kinematics) and the ionization secondary (966-977).  Uniforms come from
+// Philox4x32-10 in a fixed draw order, the order of the oracle's
+// or_mcc_collide_3v.
+//
+// Everything is templated on the channel count NCH so per-channel rates and
+// counters live in registers (fully unrolled loops); launch sites dispatch
+// on the model's channel count with with_nchan().
 #pragma once
 
 #include "aurora_internal.hpp"
@@ -13,18 +21,27 @@
 
 namespace aurora {
 
-constexpr int kMccLdsRows = 4096;  // total rows over all channels staged in LDS
+constexpr int kMccLdsWords = 4096;  // FP32 table words staged in LDS (16 KiB)
+
+enum : int { SCATTER_ISOTROPIC = 0, SCATTER_BACKWARD = 1, SCATTER_HENYEY_GREENSTEIN = 2 };
 
 struct MccParams {
     int nchan;
     int kind[kMaxChannels];
+    int angular[kMaxChannels];
     float threshold[kMaxChannels];
+    float half_emass[kMaxChannels];  // half the collision-energy mass (projectile or reduced)
     int rows[kMaxChannels];
     int offset[kMaxChannels];
-    int shared_grid;         // every channel uses channel 0's energy grid
+    int mc_rows[kMaxChannels];       // Henyey-Greenstein mean-cosine rows (0: none)
+    int mc_offset[kMaxChannels];
+    int shared_grid;         // every channel uses channel 0's energy grid and energy frame
+    int grid_shared;         // every channel's energies equal channel 0's (one copy suffices)
     int total_rows;
     const float* energy;     // concatenated tables (global)
     const float* sigma;
+    const float* mc_energy;  // concatenated mean-cosine tables (global)
+    const float* mc_value;
     float neutral_density;
     float max_frequency;
     float inv_max_frequency;
@@ -33,51 +50,87 @@ struct MccParams {
     float half_mass;
     float neutral_mass;      // 0: infinitely heavy neutral
     float neutral_vstd;      // 0: cold neutral
+    float safe_speed;        // below it the total rate is provably under max_frequency
 };
 
+template <int NCH>
 struct MccTally {
     uint32_t candidates;
     uint32_t nulls;
-    uint32_t chan[kMaxChannels];
-    float energy_change[kMaxChannels];
+    uint32_t chan[NCH];
+    float energy_change[NCH];
     int nsec;                // ionization events this call
     float sec[2][3];         // first two secondary velocities
     float ion[2][3];         // matching ion velocities
     uint32_t err;
 };
 
-// Linear interpolation with constant extrapolation (Collision.cpp:221-237)
-// on table rows [o, o + n) of arrays E, S (LDS or global).
-__device__ __forceinline__ int xs_upper(const float* E, int o, int n, float x) {
+// Host-side dispatch on the channel count: f.template operator()<NCH>().
+template <typename F>
+void with_nchan(int nchan, F&& f) {
+    switch (nchan) {
+        case 1: f.template operator()<1>(); break;
+        case 2: f.template operator()<2>(); break;
+        case 3: f.template operator()<3>(); break;
+        case 4: f.template operator()<4>(); break;
+        case 5: f.template operator()<5>(); break;
+        case 6: f.template operator()<6>(); break;
+        case 7: f.template operator()<7>(); break;
+        default: f.template operator()<8>(); break;
+    }
+}
+
+// Linear interpolation with constant extrapolation (Collision.cpp:221-237,
+// MeanCosineTable::evaluate 296-313) of a table whose n energies start at
+// E + eo and values at S + so (LDS or global).
+__device__ __forceinline__ int xs_upper(const float* E, int eo, int n, float x) {
     int lo = 0, hi = n;  // first index with E > x
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (x < E[o + mid]) hi = mid; else lo = mid + 1;
+        if (x < E[eo + mid]) hi = mid; else lo = mid + 1;
     }
     return lo;
 }
 
-__device__ __forceinline__ float xs_at(const float* E, const float* S, int o, int n, int up,
-                                       float x) {
-    if (x <= E[o]) return S[o];
-    if (x >= E[o + n - 1]) return S[o + n - 1];
-    const int hi = o + up, lo = hi - 1;
-    const float f = (x - E[lo]) / (E[hi] - E[lo]);
-    return S[lo] + f * (S[hi] - S[lo]);
+__device__ __forceinline__ float xs_at(const float* E, const float* S, int eo, int so, int n,
+                                       int up, float x) {
+    if (x <= E[eo]) return S[so];
+    if (x >= E[eo + n - 1]) return S[so + n - 1];
+    const float e0 = E[eo + up - 1], e1 = E[eo + up];
+    const float s0 = S[so + up - 1], s1 = S[so + up];
+    const float f = (x - e0) / (e1 - e0);
+    return s0 + f * (s1 - s0);
+}
+
+// Cross-section tables as the kernels read them: E holds one energy grid
+// for all channels when eshared (LDS staging of a shared grid), else the
+// concatenated per-channel grids.
+struct XsView {
+    const float* E;
+    const float* S;
+    int eshared;
+};
+
+// collision energy of channel c at relative speed g (Collision.cpp:579-594)
+__device__ __forceinline__ float collision_energy(const MccParams& m, int c, float g) {
+    return m.half_emass[c] * g * g;
 }
 
 // per-channel rates n sigma(E) g for relative speed g; returns the total
-__device__ __forceinline__ float mcc_rates(const MccParams& m, const float* E, const float* S,
-                                           float g, float* rate) {
-    const float e = m.half_mass * g * g;
+template <int NCH>
+__device__ __forceinline__ float mcc_rates(const MccParams& m, const XsView& X, float g,
+                                           float* rate) {
     float total = 0.f;
     int up0 = -1;
-    if (m.shared_grid) up0 = xs_upper(E, 0, m.rows[0], e);
-    for (int c = 0; c < m.nchan; ++c) {
+    if (m.shared_grid) up0 = xs_upper(X.E, 0, m.rows[0], collision_energy(m, 0, g));
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const float e = collision_energy(m, c, g);
         float r = 0.f;
         if (e >= m.threshold[c]) {
-            const int up = m.shared_grid ? up0 : xs_upper(E, m.offset[c], m.rows[c], e);
-            r = m.neutral_density * xs_at(E, S, m.offset[c], m.rows[c], up, e) * g;
+            const int eo = X.eshared ? 0 : m.offset[c];
+            const int up = m.shared_grid ? up0 : xs_upper(X.E, eo, m.rows[c], e);
+            r = m.neutral_density * xs_at(X.E, X.S, eo, m.offset[c], m.rows[c], up, e) * g;
         }
         rate[c] = r;
         total += r;
@@ -92,23 +145,25 @@ __device__ __forceinline__ bool above_bound(const MccParams& m, float total) {
 }
 
 // Collision.cpp:627-685 for thermal neutrals: bound over the speed window
-__device__ bool thermal_bound_exceeded(const MccParams& m, const float* E, const float* S,
-                                       float speed) {
+template <int NCH>
+__device__ bool thermal_bound_exceeded(const MccParams& m, const XsView& X, float speed) {
     const float lim = 8.0f * m.neutral_vstd;
     const float vmax = speed + lim;
     float bound = 0.f;
-    for (int c = 0; c < m.nchan; ++c) {
-        const float emax = m.half_mass * vmax * vmax;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const float emax = collision_energy(m, c, vmax);
         if (emax < m.threshold[c]) continue;
         const float vmin = speed - lim > 0.f ? speed - lim : 0.f;
-        float emin = m.half_mass * vmin * vmin;
+        float emin = collision_energy(m, c, vmin);
         if (emin < m.threshold[c]) emin = m.threshold[c];
-        const int o = m.offset[c], n = m.rows[c];
-        float best = xs_at(E, S, o, n, xs_upper(E, o, n, emin), emin);
-        const float smax = xs_at(E, S, o, n, xs_upper(E, o, n, emax), emax);
+        const int so = m.offset[c], eo = X.eshared ? 0 : so, n = m.rows[c];
+        float best = xs_at(X.E, X.S, eo, so, n, xs_upper(X.E, eo, n, emin), emin);
+        const float smax = xs_at(X.E, X.S, eo, so, n, xs_upper(X.E, eo, n, emax), emax);
         if (smax > best) best = smax;
         for (int i = 0; i < n; ++i)
-            if (E[o + i] >= emin && E[o + i] <= emax && S[o + i] > best) best = S[o + i];
+            if (X.E[eo + i] >= emin && X.E[eo + i] <= emax && X.S[so + i] > best)
+                best = X.S[so + i];
         bound += m.neutral_density * best * vmax;
     }
     return above_bound(m, bound);
@@ -125,6 +180,45 @@ __device__ __forceinline__ void isotropic(float speed, Philox& rng, float* out) 
     out[0] = speed * sine * ca;
     out[1] = speed * sine * sa;
     out[2] = speed * cosine;
+}
+
+// Henyey-Greenstein deflection of the relative velocity g0 (Collision.cpp:
+// 66-134): inverse-CDF cosine for mean cosine mc, uniform azimuth about g0.
+__device__ void henyey_greenstein(const float* g0, float speed, float mc, Philox& rng,
+                                  float* out) {
+    const float inc = sqrtf(g0[0] * g0[0] + g0[1] * g0[1] + g0[2] * g0[2]);
+    if (!(inc > 0.f)) {
+        isotropic(speed, rng, out);
+        return;
+    }
+    const float sample = rng.u01();
+    float cosine = 2.0f * sample - 1.0f;
+    if (mc != 0.f) {
+        const float num = 1.0f - mc * mc;
+        const float den = 1.0f - mc + 2.0f * mc * sample;
+        const float ratio = num / den;
+        cosine = (1.0f + mc * mc - ratio * ratio) / (2.0f * mc);
+        cosine = fminf(fmaxf(cosine, -1.0f), 1.0f);
+    }
+    const float s2 = 1.0f - cosine * cosine;
+    const float sine = sqrtf(s2 > 0.f ? s2 : 0.f);
+    const float az = 6.28318530717958647692f * rng.u01();
+    const float d[3] = {g0[0] / inc, g0[1] / inc, g0[2] / inc};
+    const bool zaxis = fabsf(d[2]) < 0.9f;
+    const float h[3] = {0.f, zaxis ? 0.f : 1.f, zaxis ? 1.f : 0.f};
+    float t[3] = {h[1] * d[2] - h[2] * d[1], h[2] * d[0] - h[0] * d[2],
+                  h[0] * d[1] - h[1] * d[0]};
+    const float tn = sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    t[0] /= tn;
+    t[1] /= tn;
+    t[2] /= tn;
+    const float b[3] = {d[1] * t[2] - d[2] * t[1], d[2] * t[0] - d[0] * t[2],
+                        d[0] * t[1] - d[1] * t[0]};
+    float sa, ca;
+    sincosf(az, &sa, &ca);
+    const float tf = sine * ca, bf = sine * sa;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[a] = speed * (cosine * d[a] + tf * t[a] + bf * b[a]);
 }
 
 // bounded Maxwellian neutral velocity: Box-Muller, rejected above 8 sigma
@@ -145,12 +239,74 @@ __device__ __forceinline__ void neutral_velocity(const MccParams& m, Philox& rng
     }
 }
 
+// Parameters of one channel; the per-channel table lives in LDS so the
+// accepted channel is read with a dynamic index without touching scratch.
+struct ChannelSel {
+    int kind;
+    int angular;
+    float threshold;
+    float half_emass;
+    int mc_offset;
+    int mc_rows;
+};
+
+// apply_channel (Collision.cpp:738-829) for the accepted channel ch:
+// relative velocity g (speed gs) against neutral velocity w
+__device__ void apply_channel(const MccParams& m, const ChannelSel& ch, float* v, const float* w,
+                              const float* g, float gs, Philox& rng) {
+    const int kind = ch.kind;
+    if (kind == AURORA_ATTACHMENT) {
+        v[0] = v[1] = v[2] = 0.f;
+        return;
+    }
+    if (kind == AURORA_CHARGE_EXCHANGE) {
+        v[0] = w[0];
+        v[1] = w[1];
+        v[2] = w[2];
+        return;
+    }
+    float energy = m.half_mass * gs * gs;
+    if (kind == AURORA_EXCITATION || kind == AURORA_IONIZATION) {
+        energy -= ch.threshold;
+        if (energy < 0.f) energy = 0.f;
+    }
+    if (kind == AURORA_IONIZATION) energy *= 0.5f;
+    const float sp = sqrtf(energy / m.half_mass);
+    float sc[3];
+    if (ch.angular == SCATTER_BACKWARD) {
+        sc[0] = -g[0];
+        sc[1] = -g[1];
+        sc[2] = -g[2];
+    } else if (ch.angular == SCATTER_HENYEY_GREENSTEIN) {
+        const float ec = ch.half_emass * gs * gs;     // collision_energy
+        const int o = ch.mc_offset, n = ch.mc_rows;
+        const float mc =
+            xs_at(m.mc_energy, m.mc_value, o, o, n, xs_upper(m.mc_energy, o, n, ec), ec);
+        henyey_greenstein(g, sp, mc, rng, sc);
+    } else {
+        isotropic(sp, rng, sc);
+    }
+    if (kind == AURORA_ELASTIC && m.neutral_mass > 0.f) {
+        const float mt = m.mass + m.neutral_mass;
+        const float rf = m.neutral_mass / mt;
+        v[0] = (m.mass * v[0] + m.neutral_mass * w[0]) / mt + rf * sc[0];
+        v[1] = (m.mass * v[1] + m.neutral_mass * w[1]) / mt + rf * sc[1];
+        v[2] = (m.mass * v[2] + m.neutral_mass * w[2]) / mt + rf * sc[2];
+    } else {
+        v[0] = w[0] + sc[0];
+        v[1] = w[1] + sc[1];
+        v[2] = w[2] + sc[2];
+    }
+}
+
 // One particle-step of null-collision MCC on velocity v (in place).
-__device__ void mcc_collide(const MccParams& m, const float* E, const float* S, float* v,
-                            float dt, Philox& rng, MccTally& t) {
+template <int NCH>
+__device__ void mcc_collide(const MccParams& m, const XsView& X, const ChannelSel* chan, float* v,
+                            float dt, Philox& rng, MccTally<NCH>& t) {
     t.candidates = 0;
     t.nulls = 0;
-    for (int c = 0; c < m.nchan; ++c) { t.chan[c] = 0; t.energy_change[c] = 0.f; }
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) { t.chan[c] = 0; t.energy_change[c] = 0.f; }
     t.nsec = 0;
     t.err = 0;
     if (!isfinite(v[0]) || !isfinite(v[1]) || !isfinite(v[2])) {
@@ -158,12 +314,17 @@ __device__ void mcc_collide(const MccParams& m, const float* E, const float* S, 
         return;
     }
     float elapsed = 0.f;
-    float rate[kMaxChannels];
+    float rate[NCH];
     for (;;) {
         const float speed = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         if (m.neutral_vstd == 0.f) {
-            if (above_bound(m, mcc_rates(m, E, S, speed, rate))) { t.err = DEV_ERR_RATE; return; }
-        } else if (thermal_bound_exceeded(m, E, S, speed)) {
+            // validate_frequency_bound (Collision.cpp:627-632): below
+            // safe_speed the check cannot fail, so it is evaluated only above
+            if (speed > m.safe_speed && above_bound(m, mcc_rates<NCH>(m, X, speed, rate))) {
+                t.err = DEV_ERR_RATE;
+                return;
+            }
+        } else if (thermal_bound_exceeded<NCH>(m, X, speed)) {
             t.err = DEV_ERR_RATE;
             return;
         }
@@ -177,61 +338,50 @@ __device__ void mcc_collide(const MccParams& m, const float* E, const float* S, 
         }
         float w[3];
         neutral_velocity(m, rng, w);
-        const float g0 = v[0] - w[0], g1 = v[1] - w[1], g2 = v[2] - w[2];
-        const float gs = sqrtf(g0 * g0 + g1 * g1 + g2 * g2);
-        if (m.neutral_vstd != 0.f) {
-            if (above_bound(m, mcc_rates(m, E, S, gs, rate))) { t.err = DEV_ERR_RATE; return; }
+        const float g[3] = {v[0] - w[0], v[1] - w[1], v[2] - w[2]};
+        const float gs = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        // rates at the relative speed (Collision.cpp:943-944); with a cold
+        // neutral gs equals the speed checked above
+        const float total = mcc_rates<NCH>(m, X, gs, rate);
+        if (m.neutral_vstd != 0.f && above_bound(m, total)) {
+            t.err = DEV_ERR_RATE;
+            return;
         }
         const float selection = rng.u01() * m.max_frequency;
         float cumulative = 0.f;
         int accepted = -1;
-        for (int c = 0; c < m.nchan; ++c) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
             cumulative += rate[c];
-            if (selection < cumulative) { accepted = c; break; }
+            if (accepted < 0 && selection < cumulative) accepted = c;
         }
         if (accepted < 0) { ++t.nulls; continue; }
         const int c = accepted;
+        const ChannelSel ch = chan[c];
         const float e_before = m.half_mass * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        if (m.kind[c] == AURORA_ATTACHMENT) {
-            v[0] = v[1] = v[2] = 0.f;
-        } else {
-            float energy = m.half_mass * gs * gs;
-            if (m.kind[c] == AURORA_EXCITATION || m.kind[c] == AURORA_IONIZATION) {
-                energy -= m.threshold[c];
-                if (energy < 0.f) energy = 0.f;
+        apply_channel(m, ch, v, w, g, gs, rng);
+        const float de = m.half_mass * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) - e_before;
+#pragma unroll
+        for (int k = 0; k < NCH; ++k)
+            if (k == c) {
+                t.energy_change[k] += de;
+                ++t.chan[k];
             }
-            if (m.kind[c] == AURORA_IONIZATION) energy *= 0.5f;
-            const float sp = sqrtf(energy / m.half_mass);
-            float sc[3];
-            isotropic(sp, rng, sc);
-            if (m.kind[c] == AURORA_ELASTIC && m.neutral_mass > 0.f) {
-                const float mt = m.mass + m.neutral_mass;
-                const float rf = m.neutral_mass / mt;
-                v[0] = (m.mass * v[0] + m.neutral_mass * w[0]) / mt + rf * sc[0];
-                v[1] = (m.mass * v[1] + m.neutral_mass * w[1]) / mt + rf * sc[1];
-                v[2] = (m.mass * v[2] + m.neutral_mass * w[2]) / mt + rf * sc[2];
-            } else {
-                v[0] = w[0] + sc[0];
-                v[1] = w[1] + sc[1];
-                v[2] = w[2] + sc[2];
-            }
-        }
-        t.energy_change[c] +=
-            m.half_mass * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) - e_before;
-        if (m.kind[c] == AURORA_IONIZATION) {
+        if (ch.kind == AURORA_IONIZATION) {
             const float r0 = v[0] - w[0], r1 = v[1] - w[1], r2 = v[2] - w[2];
             float sec[3];
             isotropic(sqrtf(r0 * r0 + r1 * r1 + r2 * r2), rng, sec);
-            if (t.nsec < 2) {
-                for (int a = 0; a < 3; ++a) {
-                    t.sec[t.nsec][a] = w[a] + sec[a];
-                    t.ion[t.nsec][a] = w[a];
-                }
-            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                if (t.nsec == k)
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        t.sec[k][a] = w[a] + sec[a];
+                        t.ion[k][a] = w[a];
+                    }
             ++t.nsec;
         }
-        ++t.chan[c];
-        if (m.kind[c] == AURORA_ATTACHMENT) break;
+        if (ch.kind == AURORA_ATTACHMENT) break;
     }
 }
 
@@ -241,9 +391,10 @@ struct TallyOut {
     double* energy_change;       // per channel (J, unweighted)
 };
 
-__device__ void flush_tally(const MccParams& m, const MccTally& t, TallyOut out, bool active) {
-    // one atomic per counter per wave (wave-level reduction via ballot-free
-    // shuffles would reorder nothing: integer adds commute)
+template <int NCH>
+__device__ void flush_tally(const MccTally<NCH>& t, TallyOut out, bool active) {
+    // wave-level sums, then one atomic per counter per wave (integer adds
+    // commute, so counts do not depend on the reduction order)
     unsigned long long cand = active ? t.candidates : 0, nul = active ? t.nulls : 0;
     unsigned long long sec = active ? static_cast<unsigned long long>(t.nsec) : 0;
     for (int off = 32; off > 0; off >>= 1) {
@@ -255,9 +406,10 @@ __device__ void flush_tally(const MccParams& m, const MccTally& t, TallyOut out,
     if (lead) {
         if (cand) atomicAdd(out.counts, cand);
         if (nul) atomicAdd(out.counts + 1, nul);
-        if (sec) atomicAdd(out.counts + 2 + m.nchan, sec);
+        if (sec) atomicAdd(out.counts + 2 + NCH, sec);
     }
-    for (int c = 0; c < m.nchan; ++c) {
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
         unsigned long long n = active ? t.chan[c] : 0;
         double e = active ? static_cast<double>(t.energy_change[c]) : 0.0;
         for (int off = 32; off > 0; off >>= 1) {
@@ -275,29 +427,33 @@ __device__ void flush_tally(const MccParams& m, const MccTally& t, TallyOut out,
 // synchronise v from v_half with E(x) at t+dt (Pusher.cpp:74-76), collide,
 // rebuild v_half (Pusher.cpp:66-68).  Up to two ionization product pairs
 // (secondary v, ion v) are staged at prod[0..12); returns their number.
-__device__ __forceinline__ int mcc_1d_particle(const MccParams& m, const float* E, const float* S,
-                                               int c, float o, float* v0, float* v1, float* v2,
+template <int NCH>
+__device__ __forceinline__ int mcc_1d_particle(const MccParams& m, const XsView& X,
+                                               const ChannelSel* chan, int c, float o, float* v0, float* v1, float* v2,
                                                uint32_t uid, const float* E32, int nx,
                                                float half_qm_dt, float dt, uint64_t seed,
-                                               uint32_t step, float* prod, MccTally& t,
+                                               uint32_t step, float* prod, MccTally<NCH>& t,
                                                uint32_t* err) {
     const float ex = gather_1d_f32(E32, nx, c, o);
     const float kick = half_qm_dt * ex;
     float u[3] = {*v0 + kick, *v1, *v2};
     Philox rng;
     rng.init(seed, 0u, uid, step);
-    mcc_collide(m, E, S, u, dt, rng, t);
+    mcc_collide<NCH>(m, X, chan, u, dt, rng, t);
     if (t.err) atomicOr(err, t.err);
     *v0 = u[0] - kick;
     *v1 = u[1];
     *v2 = u[2];
     const int np = t.nsec < 2 ? t.nsec : 2;
     if (t.nsec > 2) atomicOr(err, DEV_ERR_CAPACITY);
-    for (int k = 0; k < np; ++k)
-        for (int q = 0; q < 3; ++q) {
-            prod[6 * k + q] = t.sec[k][q];
-            prod[6 * k + 3 + q] = t.ion[k][q];
-        }
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+        if (k < np)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                prod[6 * k + q] = t.sec[k][q];
+                prod[6 * k + 3 + q] = t.ion[k][q];
+            }
     return np;
 }
 
@@ -337,21 +493,46 @@ __device__ __forceinline__ void append_products_of(int c, float o, int np, int64
     }
 }
 
-MccParams mcc_params(const Context& c);
-
-// Stage the tables in LDS when they fit (kMccLdsRows), else read global.
+// The tables go to LDS when they fit in kMccLdsWords (a shared energy grid
+// is stored once), else the kernels read them from global memory (L2);
+// the per-channel parameter table always goes to LDS.
 struct MccLds {
-    float E[kMccLdsRows];
-    float S[kMccLdsRows];
+    float words[kMccLdsWords];
+    ChannelSel chan[kMaxChannels];
 };
 
-__device__ __forceinline__ void stage_tables(const MccParams& m, MccLds& lds) {
-    if (m.total_rows > kMccLdsRows) return;
-    for (int i = threadIdx.x; i < m.total_rows; i += blockDim.x) {
-        lds.E[i] = m.energy[i];
-        lds.S[i] = m.sigma[i];
+__device__ __forceinline__ XsView stage_tables(const MccParams& m, MccLds& lds) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < kMaxChannels; ++k)
+            if (k < m.nchan)
+                lds.chan[k] = ChannelSel{m.kind[k], m.angular[k], m.threshold[k], m.half_emass[k],
+                                         m.mc_offset[k], m.mc_rows[k]};
+    }
+    XsView X{m.energy, m.sigma, 0};
+    const int ne = m.grid_shared ? m.rows[0] : m.total_rows;
+    if (ne + m.total_rows <= kMccLdsWords) {
+        for (int i = threadIdx.x; i < ne; i += blockDim.x) lds.words[i] = m.energy[i];
+        for (int i = threadIdx.x; i < m.total_rows; i += blockDim.x)
+            lds.words[ne + i] = m.sigma[i];
+        X = XsView{lds.words, lds.words + ne, m.grid_shared};
     }
     __syncthreads();
+    return X;
 }
+
+// ------------------------------------------------------------- host side
+// Validated model tables (NullCollisionModel ctor, Collision.cpp:315-577)
+// into h; particle_mass is the projectile mass.
+void load_mcc_model(MccModelHost& h, int nchan, const int* kind, const double* threshold,
+                    const int64_t* rows, const double* const* energy,
+                    const double* const* sigma, double neutral_density, double max_frequency,
+                    int64_t max_candidates, double particle_mass, double neutral_mass,
+                    double neutral_temperature);
+// Per-channel angular model and energy frame (CollisionChannelConfig).
+void set_channel_options(MccModelHost& h, int channel, int angular, int frame, int64_t rows,
+                         const double* energy, const double* mean_cosine);
+MccParams mcc_params_of(const MccModelHost& h);
+MccParams mcc_params(const Context& c);
 
 }  // namespace aurora

@@ -58,6 +58,7 @@ struct WorkItem {
     int chunk;
 };
 
+template <int NCH>
 __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc* __restrict__ descs,
                                                   const WorkItem* __restrict__ items, int nitems,
                                                   int* head) {
@@ -65,9 +66,7 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
     __shared__ int item_s;
     __shared__ long long red_v[kChunk];
     __shared__ long long red_e[kChunk];
-    stage_tables(m, lds);
-    const float* E = m.total_rows <= kMccLdsRows ? lds.E : m.energy;
-    const float* S = m.total_rows <= kMccLdsRows ? lds.S : m.sigma;
+    const XsView X = stage_tables(m, lds);
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(head, 1);
         __syncthreads();
@@ -80,11 +79,12 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
         const bool active = p < d.particles;
         float v[3] = {0.f, 0.f, 0.f};
         if (active) { v[0] = d.vx[p]; v[1] = d.vy[p]; v[2] = d.vz[p]; }
-        uint32_t cand = 0, nulls = 0, chan[kMaxChannels] = {};
+        uint32_t cand = 0, nulls = 0, chan[NCH] = {};
         uint32_t err = 0;
         float emax = 0.f;
         long long acc_v = 0, acc_e = 0;
-        int group = -1;
+        int group = -1;               // uncertainty block being accumulated
+        int64_t group_end = 0;        // first sample index of the next block
         for (int s = d.step0; s < d.step0 + d.nsteps; ++s) {
             const bool sampling = s >= d.burn_in;
             if (active && !err) {
@@ -94,8 +94,8 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
                 if (e_pre > d.max_energy_ev) err |= DEV_ERR_ENERGY;
                 Philox rng;
                 rng.init(d.seed, 0u, static_cast<uint32_t>(p), static_cast<uint32_t>(s));
-                MccTally t;
-                mcc_collide(m, E, S, v, d.dt, rng, t);
+                MccTally<NCH> t;
+                mcc_collide<NCH>(m, X, lds.chan, v, d.dt, rng, t);
                 err |= t.err;
                 v[0] += d.half_kick;
                 const float e = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -104,15 +104,18 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
                 if (sampling) {
                     cand += t.candidates;
                     nulls += t.nulls;
-                    for (int c = 0; c < m.nchan; ++c) chan[c] += t.chan[c];
+#pragma unroll
+                    for (int c = 0; c < NCH; ++c) chan[c] += t.chan[c];
                 }
             }
             if (sampling) {
-                // uncertainty block of sample k: max b with floor(b S / B) <= k
-                const int k = s - d.burn_in;
-                const int g = static_cast<int>(
-                    ((static_cast<int64_t>(k) + 1) * d.blocks - 1) / d.sampling);
-                if (g != group && group >= 0) {
+                // uncertainty block b covers samples [floor(b S / B),
+                // floor((b + 1) S / B)); blocks are at least one sample long
+                const int64_t k = s - d.burn_in;
+                if (group < 0) {
+                    group = static_cast<int>(((k + 1) * d.blocks - 1) / d.sampling);
+                    group_end = (static_cast<int64_t>(group) + 1) * d.sampling / d.blocks;
+                } else if (k >= group_end) {
                     red_v[threadIdx.x] = acc_v;
                     red_e[threadIdx.x] = acc_e;
                     __syncthreads();
@@ -131,8 +134,9 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
                     __syncthreads();
                     acc_v = 0;
                     acc_e = 0;
+                    group += 1;
+                    group_end = (static_cast<int64_t>(group) + 1) * d.sampling / d.blocks;
                 }
-                group = g;
                 if (active) {
                     const float e = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
                     acc_v += __float2ll_rn(v[0] * d.scale_v);
@@ -164,7 +168,8 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
             d.vz[p] = v[2];
             if (cand) atomicAdd(d.counts, static_cast<unsigned long long>(cand));
             if (nulls) atomicAdd(d.counts + 1, static_cast<unsigned long long>(nulls));
-            for (int c = 0; c < m.nchan; ++c)
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
                 if (chan[c]) atomicAdd(d.counts + 2 + c, static_cast<unsigned long long>(chan[c]));
             atomicMax(d.max_energy_bits, __float_as_uint(emax));
             if (err) atomicOr(d.status, err);
@@ -227,32 +232,7 @@ uint64_t footprint_of(const aurora_hip_swarm_request& r) {
            (kMaxChannels + 3) * 8u + 64u;
 }
 
-MccParams swarm_params(const SwarmState& s) {
-    const MccModelHost& h = s.gas;
-    MccParams m{};
-    m.nchan = h.nchan;
-    int total = 0;
-    for (int k = 0; k < h.nchan; ++k) {
-        m.kind[k] = h.kind[k];
-        m.threshold[k] = static_cast<float>(h.threshold[k]);
-        m.rows[k] = static_cast<int>(h.rows[k]);
-        m.offset[k] = static_cast<int>(h.row_offset[k]);
-        total += m.rows[k];
-    }
-    m.total_rows = total;
-    m.shared_grid = h.shared_grid ? 1 : 0;
-    m.energy = h.energy.ptr;
-    m.sigma = h.sigma.ptr;
-    m.neutral_density = static_cast<float>(h.neutral_density);
-    m.max_frequency = static_cast<float>(h.max_frequency);
-    m.inv_max_frequency = static_cast<float>(1.0 / h.max_frequency);
-    m.max_candidates = static_cast<int>(h.max_candidates);
-    m.mass = static_cast<float>(kElectronMass);
-    m.half_mass = static_cast<float>(0.5 * kElectronMass);
-    m.neutral_mass = static_cast<float>(h.neutral_mass);
-    m.neutral_vstd = 0.f;
-    return m;
-}
+MccParams swarm_params(const SwarmState& s) { return mcc_params_of(s.gas); }
 
 // power-of-two fixed-point scales for the v_x and energy sums of one
 // (chunk, uncertainty block): kChunk x (block length) values each bounded by
@@ -341,6 +321,14 @@ void finalize(Context& c, SwarmState& s, int64_t handle, RequestState& r) {
 
 }  // namespace
 
+MccModelHost& swarm_gas_model(Context& c) {
+    SwarmState& s = state(c);
+    AUR_REQUIRE(s.gas_loaded, AURORA_ERR_INVALID_ARGUMENT, "load the swarm gas first");
+    AUR_REQUIRE(s.active.empty(), AURORA_ERR_INVALID_ARGUMENT,
+                "the swarm gas cannot change while requests are running");
+    return s.gas;
+}
+
 void swarm_destroy(Context& c) {
     if (c.swarm) {
         delete static_cast<SwarmState*>(c.swarm);
@@ -373,54 +361,10 @@ int aurora_hip_swarm_gas(aurora_hip_ctx* h, int nchan, const int* kind, const do
         }
         AUR_REQUIRE(elastic, AURORA_ERR_INVALID_ARGUMENT,
                     "electron swarm benchmark requires an elastic channel");
-        // model checks and tables (shared with the 1D MCC model loader)
+        // model checks and tables (the 1D MCC model loader's)
         MccModelHost& g = s.gas;
-        auto fin_pos = [](double v) { return std::isfinite(v) && v > 0.0; };
-        AUR_REQUIRE(fin_pos(neutral_density), AURORA_ERR_INVALID_ARGUMENT,
-                    "MCC neutral_density must be positive and finite");
-        AUR_REQUIRE(fin_pos(max_frequency), AURORA_ERR_INVALID_ARGUMENT,
-                    "MCC max_frequency must be positive and finite");
-        AUR_REQUIRE(max_candidates > 0, AURORA_ERR_INVALID_ARGUMENT,
-                    "MCC max_candidates_per_particle must be positive");
-        AUR_REQUIRE(nchan > 0 && nchan <= kMaxChannels, AURORA_ERR_INVALID_ARGUMENT,
-                    "MCC requires at least one collision channel");
-        std::vector<float> e32, s32;
-        int64_t off = 0;
-        for (int k = 0; k < nchan; ++k) {
-            AUR_REQUIRE(rows[k] >= 2, AURORA_ERR_INVALID_ARGUMENT,
-                        "collision cross-section table requires at least two rows");
-            for (int64_t i = 0; i < rows[k]; ++i) {
-                AUR_REQUIRE(i == 0 || energy[k][i] > energy[k][i - 1],
-                            AURORA_ERR_INVALID_ARGUMENT,
-                            "collision table energies must be strictly increasing");
-                e32.push_back(static_cast<float>(energy[k][i]));
-                s32.push_back(static_cast<float>(sigma[k][i]));
-            }
-            g.kind[k] = kind[k];
-            g.threshold[k] = threshold[k];
-            g.rows[k] = rows[k];
-            g.row_offset[k] = off;
-            off += rows[k];
-        }
-        g.nchan = nchan;
-        g.shared_grid = true;
-        for (int k = 1; k < nchan; ++k) {
-            if (rows[k] != rows[0]) { g.shared_grid = false; break; }
-            for (int64_t i = 0; i < rows[k]; ++i)
-                if (static_cast<float>(energy[k][i]) != static_cast<float>(energy[0][i])) {
-                    g.shared_grid = false;
-                    break;
-                }
-        }
-        g.neutral_density = neutral_density;
-        g.max_frequency = max_frequency;
-        g.max_candidates = max_candidates;
-        g.neutral_mass = neutral_mass;
-        g.particle_mass = kElectronMass;
-        g.energy.alloc(e32.size());
-        g.sigma.alloc(s32.size());
-        AUR_HIP(hipMemcpy(g.energy.ptr, e32.data(), e32.size() * sizeof(float), hipMemcpyHostToDevice));
-        AUR_HIP(hipMemcpy(g.sigma.ptr, s32.data(), s32.size() * sizeof(float), hipMemcpyHostToDevice));
+        load_mcc_model(g, nchan, kind, threshold, rows, energy, sigma, neutral_density,
+                       max_frequency, max_candidates, kElectronMass, neutral_mass, 0.0);
         // energy ceiling of the tables (Swarm.cpp:517-526), eV
         double emax_ev = 1e300;
         for (int k = 0; k < nchan; ++k)
@@ -551,8 +495,10 @@ int aurora_hip_swarm_advance(aurora_hip_ctx* h, int64_t max_steps, int64_t* acti
             if (grid > static_cast<int64_t>(items.size())) grid = static_cast<int64_t>(items.size());
             {
                 KernelScope ks(c, "swarm");
-                k_swarm<<<static_cast<unsigned>(grid), kChunk, 0, c.stream>>>(
-                    m, s.descs.ptr, s.items.ptr, static_cast<int>(items.size()), s.head.ptr);
+                with_nchan(m.nchan, [&]<int NCH>() {
+                    k_swarm<NCH><<<static_cast<unsigned>(grid), kChunk, 0, c.stream>>>(
+                        m, s.descs.ptr, s.items.ptr, static_cast<int>(items.size()), s.head.ptr);
+                });
                 AUR_HIP(hipGetLastError());
             }
             // the host arrays are reused next call: wait for the copies

@@ -1,0 +1,729 @@
+// aurora_gpu.cpp - C++20 host classes over the C ABI (include/aurora_hip.h).
+#include "aurora_gpu.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <utility>
+
+namespace pic::gpu {
+
+namespace {
+
+
+std::shared_ptr<Device> or_new(std::shared_ptr<Device> d) {
+    return d ? std::move(d) : std::make_shared<Device>(0);
+}
+
+int code(Boundary b) { return b == Boundary::Periodic ? AURORA_PERIODIC : AURORA_DIRICHLET; }
+
+int policy(ParticleBoundary p, Boundary mesh) {
+    switch (p) {
+        case ParticleBoundary::Absorbing: return AURORA_PB_ABSORB;
+        case ParticleBoundary::Reflecting: return AURORA_PB_REFLECT;
+        case ParticleBoundary::Periodic: return AURORA_PB_PERIODIC;
+        case ParticleBoundary::Auto: break;
+    }
+    // Auto: periodic meshes wrap, bounded meshes absorb (Simulation3D.cpp:21-28)
+    return mesh == Boundary::Periodic ? AURORA_PB_PERIODIC : AURORA_PB_ABSORB;
+}
+
+int kind_code(CollisionProcessKind k) {
+    switch (k) {
+        case CollisionProcessKind::Elastic: return AURORA_ELASTIC;
+        case CollisionProcessKind::Excitation: return AURORA_EXCITATION;
+        case CollisionProcessKind::Ionization: return AURORA_IONIZATION;
+        case CollisionProcessKind::Attachment: return AURORA_ATTACHMENT;
+        case CollisionProcessKind::ChargeExchange: return AURORA_CHARGE_EXCHANGE;
+    }
+    return -1;
+}
+
+// Flattened tables of a CollisionConfig for aurora_hip_mcc_load / swarm_gas.
+struct Tables {
+    std::vector<int> kind;
+    std::vector<double> threshold;
+    std::vector<int64_t> rows;
+    std::vector<const double*> energy, sigma;
+    explicit Tables(const CollisionConfig& c) {
+        for (const auto& ch : c.channels) {
+            kind.push_back(kind_code(ch.process));
+            threshold.push_back(ch.threshold_energy);
+            if (ch.cross_section.energies.size() != ch.cross_section.values.size())
+                throw std::invalid_argument("collision table columns differ in length");
+            rows.push_back(static_cast<int64_t>(ch.cross_section.energies.size()));
+            energy.push_back(ch.cross_section.energies.data());
+            sigma.push_back(ch.cross_section.values.data());
+        }
+    }
+};
+
+void apply_options(const Device& d, int model, const CollisionConfig& c) {
+    for (size_t k = 0; k < c.channels.size(); ++k) {
+        const auto& ch = c.channels[k];
+        const int ang = static_cast<int>(ch.angular_scattering);
+        const int frame = ch.energy_frame == CollisionEnergyFrame::CenterOfMass
+                              ? AURORA_FRAME_CENTER_OF_MASS
+                              : AURORA_FRAME_PROJECTILE;
+        if (ang == 0 && frame == AURORA_FRAME_PROJECTILE && ch.mean_cosine.energies.empty())
+            continue;
+        const bool hg = ch.angular_scattering == AngularScatteringKind::HenyeyGreenstein;
+        if (!hg && !ch.mean_cosine.energies.empty())
+            throw std::invalid_argument(
+                "mean-cosine data requires angular_model = henyey_greenstein");
+        d.check(aurora_hip_mcc_channel_options(
+            d.get(), model, static_cast<int>(k), ang, frame,
+            hg ? static_cast<int64_t>(ch.mean_cosine.energies.size()) : 0,
+            hg ? ch.mean_cosine.energies.data() : nullptr,
+            hg ? ch.mean_cosine.values.data() : nullptr));
+    }
+}
+
+void mesh_bind(const Device& d, int dim, const int64_t* n, const double* L, Boundary b,
+               double eps, double depth) {
+    d.check(aurora_hip_mesh_bind(d.get(), dim, n, L, code(b), eps, depth));
+}
+
+// adds the device rho to a host rho vector
+void add_rho(const Device& d, std::vector<double>& rho) {
+    std::vector<double> r(rho.size());
+    d.check(aurora_hip_get_fields(d.get(), r.data(), nullptr, nullptr));
+    for (size_t i = 0; i < rho.size(); ++i) rho[i] += r[i];
+}
+
+template <typename P>
+void upload_and_deposit(const Device& d, int dim, const std::vector<P>& particles, double charge,
+                        double weight, std::vector<double>& rho,
+                        void (*pos_of)(const P&, double*)) {
+    int sid = 0;
+    const int64_t n = static_cast<int64_t>(particles.size());
+    d.check(aurora_hip_species_add(d.get(), charge, 1.0, weight, std::max<int64_t>(n, 1), 64,
+                                   &sid));
+    std::vector<double> pos(static_cast<size_t>(n * dim)), vel(static_cast<size_t>(3 * n), 0.0);
+    std::vector<uint8_t> alive(static_cast<size_t>(n));
+    for (int64_t p = 0; p < n; ++p) {
+        pos_of(particles[static_cast<size_t>(p)], pos.data() + dim * p);
+        alive[static_cast<size_t>(p)] = particles[static_cast<size_t>(p)].alive ? 1 : 0;
+    }
+    d.check(aurora_hip_species_upload(d.get(), sid, n, pos.data(), vel.data(), alive.data(),
+                                      AURORA_VELOCITY_HALF));
+    d.check(aurora_hip_deposit(d.get()));
+    add_rho(d, rho);
+}
+
+void pos1(const Particle& p, double* x) { x[0] = p.x; }
+void pos2(const Particle2D& p, double* x) { x[0] = p.position.x; x[1] = p.position.y; }
+void pos3(const Particle3D& p, double* x) {
+    x[0] = p.position.x; x[1] = p.position.y; x[2] = p.position.z;
+}
+
+struct Downloaded {
+    std::vector<double> pos, vh, v;
+    std::vector<uint8_t> alive;
+    int64_t n = 0;
+};
+
+Downloaded download(const Device& d, int sid, int dim, double dt) {
+    Downloaded out;
+    int64_t slots = 0, live = 0;
+    d.check(aurora_hip_species_count(d.get(), sid, &slots, &live));
+    const int64_t cap = std::max<int64_t>(slots, 1);
+    out.pos.resize(static_cast<size_t>(cap * dim));
+    out.vh.resize(static_cast<size_t>(cap * 3));
+    out.v.resize(static_cast<size_t>(cap * 3));
+    out.alive.resize(static_cast<size_t>(cap));
+    d.check(aurora_hip_species_download(d.get(), sid, cap, out.pos.data(), out.vh.data(),
+                                        out.alive.data(), nullptr, &out.n));
+    d.check(aurora_hip_species_velocity(d.get(), sid, dt, cap, out.v.data()));
+    return out;
+}
+
+DiagnosticSample energies(const Device& d, double dt, size_t nspecies) {
+    std::vector<double> ke(std::max<size_t>(nspecies, 1));
+    double fe = 0.0, l1 = 0.0;
+    int64_t live = 0;
+    d.check(aurora_hip_energy(d.get(), dt, ke.data(), &fe, &l1, &live));
+    DiagnosticSample s;
+    for (size_t k = 0; k < nspecies; ++k) s.kinetic_energy += ke[k];
+    s.field_energy = fe;
+    s.charge_l1 = l1;
+    s.live_particles = static_cast<size_t>(live);
+    return s;
+}
+
+}  // namespace
+
+void raise_status(int status, const std::string& message) {
+    switch (status) {
+        case AURORA_ERR_INVALID_ARGUMENT:
+        case AURORA_ERR_UNSUPPORTED:
+            throw std::invalid_argument(message);
+        case AURORA_ERR_CAPACITY: throw std::overflow_error(message);
+        case AURORA_ERR_B_OUT_OF_RANGE:
+        case AURORA_ERR_NOT_FOUND:
+            throw std::out_of_range(message);
+        default: throw std::runtime_error(message);
+    }
+}
+
+Device::Device(int ordinal) {
+    const int rc = aurora_hip_create(ordinal, &ctx_);
+    if (rc != AURORA_OK)
+        raise_status(rc == AURORA_ERR_INVALID_ARGUMENT ? AURORA_ERR_HIP : rc,
+                     "cannot create a HIP context on device " + std::to_string(ordinal));
+}
+
+Device::~Device() {
+    if (ctx_) aurora_hip_destroy(ctx_);
+}
+
+void Device::check(int status) const {
+    if (status != AURORA_OK) raise_status(status, aurora_hip_last_error(ctx_));
+}
+
+std::string Device::info() const {
+    char buf[256] = {};
+    check(aurora_hip_device_info(ctx_, buf, sizeof buf));
+    return buf;
+}
+
+// -------------------------------------------------------------- meshes
+Grid::Grid(std::size_t nx, double length, Boundary boundary)
+    : nx_(nx), length_(length), boundary_(boundary) {
+    if (nx < 3) throw std::invalid_argument("grid nx must be at least 3");
+    if (!std::isfinite(length) || length <= 0.0)
+        throw std::invalid_argument("domain length must be positive and finite");
+    dx_ = length / static_cast<double>(boundary == Boundary::Periodic ? nx : nx - 1);
+    rho_.assign(nx, 0.0);
+    phi_.assign(nx, 0.0);
+    electric_.assign(nx, 0.0);
+}
+
+void Grid::clear_charge() { std::fill(rho_.begin(), rho_.end(), 0.0); }
+
+double Grid::node_volume(std::size_t i) const {
+    if (i >= nx_) throw std::out_of_range("grid node index out of range");
+    if (boundary_ == Boundary::Periodic) return dx_;
+    return (i == 0 || i + 1 == nx_) ? 0.5 * dx_ : dx_;
+}
+
+Mesh2D::Mesh2D(std::size_t nx, std::size_t ny, double length_x, double length_y,
+               Boundary boundary)
+    : nx_(nx), ny_(ny), lx_(length_x), ly_(length_y), boundary_(boundary) {
+    if (nx < 3 || ny < 3) throw std::invalid_argument("mesh extents must be at least 3");
+    const double cx = static_cast<double>(boundary == Boundary::Periodic ? nx : nx - 1);
+    const double cy = static_cast<double>(boundary == Boundary::Periodic ? ny : ny - 1);
+    dx_ = lx_ / cx;
+    dy_ = ly_ / cy;
+    rho_.assign(nx * ny, 0.0);
+    phi_.assign(nx * ny, 0.0);
+    ex_.assign(nx * ny, 0.0);
+    ey_.assign(nx * ny, 0.0);
+}
+
+void Mesh2D::clear_charge() { std::fill(rho_.begin(), rho_.end(), 0.0); }
+
+Mesh3D::Mesh3D(std::size_t nx, std::size_t ny, std::size_t nz, double length_x, double length_y,
+               double length_z, Boundary boundary)
+    : n_{nx, ny, nz}, l_{length_x, length_y, length_z}, boundary_(boundary) {
+    for (int a = 0; a < 3; ++a) {
+        if (n_[a] < 3) throw std::invalid_argument("mesh extents must be at least 3");
+        d_[a] = l_[a] / static_cast<double>(boundary == Boundary::Periodic ? n_[a] : n_[a] - 1);
+    }
+    const size_t n = nx * ny * nz;
+    rho_.assign(n, 0.0);
+    phi_.assign(n, 0.0);
+    ex_.assign(n, 0.0);
+    ey_.assign(n, 0.0);
+    ez_.assign(n, 0.0);
+}
+
+void Mesh3D::clear_charge() { std::fill(rho_.begin(), rho_.end(), 0.0); }
+
+// --------------------------------------------------------- field solver
+FieldSolver::FieldSolver(double permittivity, std::shared_ptr<Device> device)
+    : permittivity_(permittivity), device_(std::move(device)) {
+    if (!std::isfinite(permittivity) || !(permittivity > 0.0))
+        throw std::invalid_argument("relative_permittivity must be positive and finite");
+}
+
+void FieldSolver::solve(Grid& grid, double phi_left, double phi_right) const {
+    const auto d = or_new(device_);
+    const int64_t n = static_cast<int64_t>(grid.nx());
+    const double L = grid.length();
+    mesh_bind(*d, 1, &n, &L, grid.boundary(), permittivity_, 1.0);
+    d->check(aurora_hip_set_rho(d->get(), grid.rho().data()));
+    d->check(aurora_hip_field_solve(d->get(), phi_left, phi_right, 1));
+    d->check(aurora_hip_get_fields(d->get(), nullptr, grid.phi().data(), grid.electric().data()));
+}
+
+void FieldSolver::solve(Mesh2D& mesh) const {
+    const auto d = or_new(device_);
+    const int64_t n[2] = {static_cast<int64_t>(mesh.nx()), static_cast<int64_t>(mesh.ny())};
+    const double L[2] = {mesh.length_x(), mesh.length_y()};
+    mesh_bind(*d, 2, n, L, mesh.boundary(), permittivity_, 1.0);
+    d->check(aurora_hip_set_rho(d->get(), mesh.rho().data()));
+    d->check(aurora_hip_field_solve(d->get(), 0.0, 0.0, 1));
+    std::vector<double> E(2 * mesh.size());
+    d->check(aurora_hip_get_fields(d->get(), nullptr, mesh.phi().data(), E.data()));
+    std::copy(E.begin(), E.begin() + mesh.size(), mesh.electric_x().begin());
+    std::copy(E.begin() + mesh.size(), E.end(), mesh.electric_y().begin());
+}
+
+void FieldSolver::solve(Mesh3D& mesh) const {
+    const auto d = or_new(device_);
+    const int64_t n[3] = {static_cast<int64_t>(mesh.nx()), static_cast<int64_t>(mesh.ny()),
+                          static_cast<int64_t>(mesh.nz())};
+    const double L[3] = {mesh.length_x(), mesh.length_y(), mesh.length_z()};
+    mesh_bind(*d, 3, n, L, mesh.boundary(), permittivity_, 1.0);
+    d->check(aurora_hip_set_rho(d->get(), mesh.rho().data()));
+    d->check(aurora_hip_field_solve(d->get(), 0.0, 0.0, 1));
+    const size_t s = mesh.size();
+    std::vector<double> E(3 * s);
+    d->check(aurora_hip_get_fields(d->get(), nullptr, mesh.phi().data(), E.data()));
+    std::copy(E.begin(), E.begin() + s, mesh.electric_x().begin());
+    std::copy(E.begin() + s, E.begin() + 2 * s, mesh.electric_y().begin());
+    std::copy(E.begin() + 2 * s, E.end(), mesh.electric_z().begin());
+}
+
+void deposit_charge_cic(Grid& grid, const std::vector<Particle>& particles, double charge,
+                        double weight, const std::shared_ptr<Device>& device) {
+    const auto d = or_new(device);
+    const int64_t n = static_cast<int64_t>(grid.nx());
+    const double L = grid.length();
+    mesh_bind(*d, 1, &n, &L, grid.boundary(), 1.0, 1.0);
+    upload_and_deposit<Particle>(*d, 1, particles, charge, weight, grid.rho(), pos1);
+}
+
+void deposit_charge_cic(Mesh2D& mesh, const std::vector<Particle2D>& particles, double charge,
+                        double weight, const std::shared_ptr<Device>& device) {
+    const auto d = or_new(device);
+    const int64_t n[2] = {static_cast<int64_t>(mesh.nx()), static_cast<int64_t>(mesh.ny())};
+    const double L[2] = {mesh.length_x(), mesh.length_y()};
+    mesh_bind(*d, 2, n, L, mesh.boundary(), 1.0, 1.0);
+    upload_and_deposit<Particle2D>(*d, 2, particles, charge, weight, mesh.rho(), pos2);
+}
+
+void deposit_charge_cic(Mesh3D& mesh, const std::vector<Particle3D>& particles, double charge,
+                        double weight, const std::shared_ptr<Device>& device) {
+    const auto d = or_new(device);
+    const int64_t n[3] = {static_cast<int64_t>(mesh.nx()), static_cast<int64_t>(mesh.ny()),
+                          static_cast<int64_t>(mesh.nz())};
+    const double L[3] = {mesh.length_x(), mesh.length_y(), mesh.length_z()};
+    mesh_bind(*d, 3, n, L, mesh.boundary(), 1.0, 1.0);
+    upload_and_deposit<Particle3D>(*d, 3, particles, charge, weight, mesh.rho(), pos3);
+}
+
+// --------------------------------------------------------------- MCC
+NullCollisionModel::NullCollisionModel(CollisionConfig config, double particle_mass,
+                                       std::uint64_t seed, std::shared_ptr<Device> device)
+    : config_(std::move(config)), particle_mass_(particle_mass), device_(or_new(std::move(device))) {
+    for (const auto& ch : config_.channels) {
+        if (std::find(names_.begin(), names_.end(), ch.name) != names_.end())
+            throw std::invalid_argument("duplicate MCC collision channel name: " + ch.name);
+        names_.push_back(ch.name);
+    }
+    const Device& d = *device_;
+    const int64_t n = 3;
+    const double L = 1.0;
+    mesh_bind(d, 1, &n, &L, Boundary::Periodic, 1.0, 1.0);
+    int sid = 0;
+    d.check(aurora_hip_species_add(d.get(), -1.0, particle_mass, 1.0, 1, 32, &sid));
+    const Tables t(config_);
+    const int none[2] = {-1, -1};
+    d.check(aurora_hip_mcc_load(d.get(), sid, static_cast<int>(t.kind.size()), t.kind.data(),
+                                t.threshold.data(), t.rows.data(), t.energy.data(),
+                                t.sigma.data(), config_.neutral_density, config_.max_frequency,
+                                static_cast<int64_t>(config_.max_candidates_per_particle),
+                                config_.neutral_mass, config_.neutral_temperature, none, seed));
+    apply_options(d, 0, config_);
+}
+
+CollisionStepStatistics NullCollisionModel::collide(std::vector<Vec3>& velocities,
+                                                    double timestep, std::uint64_t step,
+                                                    std::uint32_t stream_id) const {
+    const size_t nch = config_.channels.size();
+    std::vector<double> v(3 * velocities.size());
+    for (size_t p = 0; p < velocities.size(); ++p) {
+        v[3 * p] = velocities[p].x;
+        v[3 * p + 1] = velocities[p].y;
+        v[3 * p + 2] = velocities[p].z;
+    }
+    std::vector<uint64_t> counts(nch + 3, 0);
+    std::vector<double> ech(std::max<size_t>(nch, 1), 0.0);
+    device_->check(aurora_hip_mcc_collide(device_->get(), static_cast<int64_t>(velocities.size()),
+                                          v.data(), timestep, step, stream_id, counts.data(),
+                                          ech.data()));
+    for (size_t p = 0; p < velocities.size(); ++p)
+        velocities[p] = Vec3{v[3 * p], v[3 * p + 1], v[3 * p + 2]};
+    CollisionStepStatistics s;
+    s.candidates = counts[0];
+    s.null_collisions = counts[1];
+    s.channel_collisions.assign(counts.begin() + 2, counts.begin() + 2 + static_cast<long>(nch));
+    s.channel_projectile_energy_change.assign(ech.begin(), ech.begin() + static_cast<long>(nch));
+    s.secondaries = counts[2 + nch];
+    return s;
+}
+
+// ----------------------------------------------------------- Simulation
+Simulation::Simulation(Config cfg, std::shared_ptr<Device> device)
+    : cfg_(std::move(cfg)), dev_(or_new(std::move(device))) {
+    const Device& d = *dev_;
+    const int64_t n = static_cast<int64_t>(cfg_.nx);
+    mesh_bind(d, 1, &n, &cfg_.length, cfg_.boundary, cfg_.permittivity, 1.0);
+    const double left[4] = {cfg_.phi_left, cfg_.phi_left_drive.amplitude,
+                            cfg_.phi_left_drive.frequency, cfg_.phi_left_drive.phase};
+    const double right[4] = {cfg_.phi_right, cfg_.phi_right_drive.amplitude,
+                             cfg_.phi_right_drive.frequency, cfg_.phi_right_drive.phase};
+    d.check(aurora_hip_set_electrodes(d.get(), left, right));
+    for (const auto& s : cfg_.species) {
+        int sid = 0;
+        const int64_t cap = static_cast<int64_t>(
+            std::max(cfg_.max_particles_per_species, s.particles));
+        d.check(aurora_hip_species_add(d.get(), s.charge, s.mass, s.weight, cap, s.precision,
+                                       &sid));
+    }
+    if (cfg_.collision_models.size() > 1)
+        throw std::invalid_argument(
+            "the GPU backend runs one null-collision model per simulation");
+    auto index_of = [&](const std::string& name) {
+        for (size_t k = 0; k < cfg_.species.size(); ++k)
+            if (cfg_.species[k].name == name) return static_cast<int>(k);
+        throw std::invalid_argument("unknown MCC species: " + name);
+    };
+    for (const auto& m : cfg_.collision_models) {
+        const int target = index_of(m.species);
+        int prod[2] = {-1, -1};
+        if (!m.secondary_species.empty()) prod[0] = index_of(m.secondary_species);
+        if (!m.ion_species.empty()) prod[1] = index_of(m.ion_species);
+        const Tables t(m.config);
+        d.check(aurora_hip_mcc_load(
+            d.get(), target, static_cast<int>(t.kind.size()), t.kind.data(), t.threshold.data(),
+            t.rows.data(), t.energy.data(), t.sigma.data(), m.config.neutral_density,
+            m.config.max_frequency, static_cast<int64_t>(m.config.max_candidates_per_particle),
+            m.config.neutral_mass, m.config.neutral_temperature, prod, cfg_.seed));
+        apply_options(d, 0, m.config);
+    }
+}
+
+void Simulation::initialize() {
+    const Device& d = *dev_;
+    for (size_t k = 0; k < cfg_.species.size(); ++k) {
+        const auto& s = cfg_.species[k];
+        const double drift[3] = {s.drift_velocity, 0.0, 0.0};
+        d.check(aurora_hip_species_init_maxwellian(d.get(), static_cast<int>(k),
+                                                   static_cast<int64_t>(s.particles),
+                                                   s.thermal_velocity, drift, cfg_.seed + k));
+    }
+    d.check(aurora_hip_initialize(d.get(), cfg_.dt));
+    time_ = 0.0;
+    step_ = 0;
+    initialized_ = true;
+}
+
+void Simulation::step() {
+    if (!initialized_) initialize();
+    dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, 1));
+    int64_t s = 0;
+    dev_->check(aurora_hip_time(dev_->get(), &time_, &s));
+    step_ = static_cast<size_t>(s);
+}
+
+void Simulation::run() {
+    if (!initialized_) initialize();
+    dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, static_cast<int64_t>(cfg_.steps)));
+    int64_t s = 0;
+    dev_->check(aurora_hip_time(dev_->get(), &time_, &s));
+    step_ = static_cast<size_t>(s);
+}
+
+std::vector<Particle> Simulation::particles(std::size_t species) const {
+    const Downloaded x = download(*dev_, static_cast<int>(species), 1, cfg_.dt);
+    std::vector<Particle> out(static_cast<size_t>(x.n));
+    for (int64_t p = 0; p < x.n; ++p) {
+        Particle& q = out[static_cast<size_t>(p)];
+        q.x = x.pos[p];
+        q.alive = x.alive[p] != 0;
+        q.v_half = x.vh[3 * p];
+        q.v = x.v[3 * p];
+        q.velocity_y = x.v[3 * p + 1];
+        q.velocity_z = x.v[3 * p + 2];
+    }
+    return out;
+}
+
+DiagnosticSample Simulation::sample() const { return energies(*dev_, cfg_.dt, cfg_.species.size()); }
+
+Grid Simulation::fields() const {
+    Grid g(cfg_.nx, cfg_.length, cfg_.boundary);
+    dev_->check(aurora_hip_get_fields(dev_->get(), g.rho().data(), g.phi().data(),
+                                      g.electric().data()));
+    return g;
+}
+
+BoundaryLoss1D Simulation::boundary_losses(std::size_t species) const {
+    int64_t counts[6] = {};
+    double energy[2] = {};
+    dev_->check(aurora_hip_boundary_losses(dev_->get(), static_cast<int>(species), counts, energy));
+    return BoundaryLoss1D{static_cast<uint64_t>(counts[0]), static_cast<uint64_t>(counts[1]),
+                          energy[0], energy[1]};
+}
+
+CollisionStepStatistics Simulation::collision_statistics() const {
+    CollisionStepStatistics s;
+    if (cfg_.collision_models.empty()) return s;
+    const size_t nch = cfg_.collision_models[0].config.channels.size();
+    std::vector<uint64_t> counts(nch + 3);
+    std::vector<double> ech(std::max<size_t>(nch, 1));
+    dev_->check(aurora_hip_mcc_stats(dev_->get(), counts.data(), ech.data()));
+    s.candidates = counts[0];
+    s.null_collisions = counts[1];
+    s.channel_collisions.assign(counts.begin() + 2, counts.begin() + 2 + static_cast<long>(nch));
+    s.channel_projectile_energy_change.assign(ech.begin(), ech.begin() + static_cast<long>(nch));
+    s.secondaries = counts[2 + nch];
+    return s;
+}
+
+// --------------------------------------------------------- Simulation2D
+Simulation2D::Simulation2D(Simulation2DConfig cfg, std::shared_ptr<Device> device)
+    : cfg_(std::move(cfg)), dev_(or_new(std::move(device))) {
+    const Device& d = *dev_;
+    const int64_t n[2] = {static_cast<int64_t>(cfg_.nx), static_cast<int64_t>(cfg_.ny)};
+    const double L[2] = {cfg_.length_x, cfg_.length_y};
+    mesh_bind(d, 2, n, L, cfg_.boundary, cfg_.permittivity, cfg_.out_of_plane_depth);
+    const double b[3] = {cfg_.magnetic_field.x, cfg_.magnetic_field.y, cfg_.magnetic_field.z};
+    d.check(aurora_hip_set_uniform_b(d.get(), b));
+    for (const auto& s : cfg_.species) {
+        int sid = 0;
+        d.check(aurora_hip_species_add(d.get(), s.charge, s.mass, s.weight,
+                                       static_cast<int64_t>(std::max(s.capacity, s.particles)),
+                                       32, &sid));
+    }
+    d.check(aurora_hip_set_sort_interval(d.get(), static_cast<int64_t>(cfg_.sort_interval)));
+}
+
+void Simulation2D::initialize() {
+    const Device& d = *dev_;
+    for (size_t k = 0; k < cfg_.species.size(); ++k) {
+        const auto& s = cfg_.species[k];
+        const double drift[3] = {s.drift_velocity.x, s.drift_velocity.y, s.drift_velocity.z};
+        d.check(aurora_hip_species_init_maxwellian(d.get(), static_cast<int>(k),
+                                                   static_cast<int64_t>(s.particles),
+                                                   s.thermal_velocity, drift, cfg_.seed + k));
+    }
+    d.check(aurora_hip_sort(d.get()));
+    d.check(aurora_hip_initialize(d.get(), cfg_.dt));
+    time_ = 0.0;
+    step_ = 0;
+    initialized_ = true;
+}
+
+void Simulation2D::step() {
+    if (!initialized_) initialize();
+    dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, 1));
+    int64_t s = 0;
+    dev_->check(aurora_hip_time(dev_->get(), &time_, &s));
+    step_ = static_cast<size_t>(s);
+}
+
+void Simulation2D::run() {
+    if (!initialized_) initialize();
+    dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, static_cast<int64_t>(cfg_.steps)));
+    int64_t s = 0;
+    dev_->check(aurora_hip_time(dev_->get(), &time_, &s));
+    step_ = static_cast<size_t>(s);
+}
+
+std::vector<Particle2D> Simulation2D::particles(std::size_t species) const {
+    const Downloaded x = download(*dev_, static_cast<int>(species), 2, cfg_.dt);
+    std::vector<Particle2D> out(static_cast<size_t>(x.n));
+    for (int64_t p = 0; p < x.n; ++p) {
+        Particle2D& q = out[static_cast<size_t>(p)];
+        q.position = Vec2{x.pos[2 * p], x.pos[2 * p + 1]};
+        q.alive = x.alive[p] != 0;
+        q.velocity_half = Vec2{x.vh[3 * p], x.vh[3 * p + 1]};
+        q.velocity_half_z = x.vh[3 * p + 2];
+        q.velocity = Vec2{x.v[3 * p], x.v[3 * p + 1]};
+        q.velocity_z = x.v[3 * p + 2];
+    }
+    return out;
+}
+
+DiagnosticSample Simulation2D::sample() const {
+    return energies(*dev_, cfg_.dt, cfg_.species.size());
+}
+
+Mesh2D Simulation2D::fields() const {
+    Mesh2D m(cfg_.nx, cfg_.ny, cfg_.length_x, cfg_.length_y, cfg_.boundary);
+    std::vector<double> E(2 * m.size());
+    dev_->check(aurora_hip_get_fields(dev_->get(), m.rho().data(), m.phi().data(), E.data()));
+    std::copy(E.begin(), E.begin() + m.size(), m.electric_x().begin());
+    std::copy(E.begin() + m.size(), E.end(), m.electric_y().begin());
+    return m;
+}
+
+// --------------------------------------------------------- Simulation3D
+Simulation3D::Simulation3D(Simulation3DConfig cfg, std::shared_ptr<Device> device)
+    : cfg_(std::move(cfg)), dev_(or_new(std::move(device))) {
+    const Device& d = *dev_;
+    const int64_t n[3] = {static_cast<int64_t>(cfg_.nx), static_cast<int64_t>(cfg_.ny),
+                          static_cast<int64_t>(cfg_.nz)};
+    const double L[3] = {cfg_.length_x, cfg_.length_y, cfg_.length_z};
+    mesh_bind(d, 3, n, L, cfg_.boundary, cfg_.permittivity, 1.0);
+    int pol[6];
+    for (int f = 0; f < 6; ++f) pol[f] = policy(cfg_.particle_boundary[f], cfg_.boundary);
+    d.check(aurora_hip_particle_boundaries(d.get(), pol));
+    if (cfg_.magnetic_field_profile) {
+        const auto& t = *cfg_.magnetic_field_profile;
+        if (t.coordinates.size() != t.values.size())
+            throw std::invalid_argument("tabulated field columns differ in length");
+        std::vector<double> v;
+        for (const Vec3& b : t.values) v.insert(v.end(), {b.x, b.y, b.z});
+        d.check(aurora_hip_set_b_profile(d.get(), t.axis, static_cast<int64_t>(t.values.size()),
+                                         t.coordinates.data(), v.data()));
+    } else {
+        const double b[3] = {cfg_.magnetic_field.x, cfg_.magnetic_field.y, cfg_.magnetic_field.z};
+        d.check(aurora_hip_set_uniform_b(d.get(), b));
+    }
+    for (const auto& s : cfg_.species) {
+        int sid = 0;
+        d.check(aurora_hip_species_add(d.get(), s.charge, s.mass, s.weight,
+                                       static_cast<int64_t>(std::max(s.capacity, s.particles)),
+                                       32, &sid));
+    }
+    loaded_.assign(cfg_.species.size(), false);
+    d.check(aurora_hip_set_sort_interval(d.get(), static_cast<int64_t>(cfg_.sort_interval)));
+}
+
+void Simulation3D::set_particles(std::size_t species, const std::vector<Particle3D>& particles) {
+    if (species >= cfg_.species.size()) throw std::out_of_range("unknown species index");
+    const int64_t n = static_cast<int64_t>(particles.size());
+    std::vector<double> pos(static_cast<size_t>(3 * n)), vel(static_cast<size_t>(3 * n));
+    std::vector<uint8_t> alive(static_cast<size_t>(n));
+    for (int64_t p = 0; p < n; ++p) {
+        const Particle3D& q = particles[static_cast<size_t>(p)];
+        pos3(q, pos.data() + 3 * p);
+        // time-centred velocities; initialize() derives v_half (Species3D.cpp:72)
+        vel[3 * p] = q.velocity.x;
+        vel[3 * p + 1] = q.velocity.y;
+        vel[3 * p + 2] = q.velocity.z;
+        alive[static_cast<size_t>(p)] = q.alive ? 1 : 0;
+    }
+    dev_->check(aurora_hip_species_upload(dev_->get(), static_cast<int>(species), n, pos.data(),
+                                          vel.data(), alive.data(), AURORA_VELOCITY_CENTRED));
+    loaded_[species] = true;
+}
+
+void Simulation3D::initialize() {
+    const Device& d = *dev_;
+    for (size_t k = 0; k < cfg_.species.size(); ++k) {
+        if (loaded_[k]) continue;
+        const auto& s = cfg_.species[k];
+        const double drift[3] = {s.drift_velocity.x, s.drift_velocity.y, s.drift_velocity.z};
+        d.check(aurora_hip_species_init_maxwellian(d.get(), static_cast<int>(k),
+                                                   static_cast<int64_t>(s.particles),
+                                                   s.thermal_velocity, drift, cfg_.seed + k));
+    }
+    if (cfg_.boundary == Boundary::Periodic) d.check(aurora_hip_sort(d.get()));
+    d.check(aurora_hip_initialize(d.get(), cfg_.dt));
+    time_ = 0.0;
+    step_ = 0;
+    initialized_ = true;
+}
+
+void Simulation3D::step() {
+    if (!initialized_) initialize();
+    dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, 1));
+    int64_t s = 0;
+    dev_->check(aurora_hip_time(dev_->get(), &time_, &s));
+    step_ = static_cast<size_t>(s);
+}
+
+void Simulation3D::run() {
+    if (!initialized_) initialize();
+    dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, static_cast<int64_t>(cfg_.steps)));
+    int64_t s = 0;
+    dev_->check(aurora_hip_time(dev_->get(), &time_, &s));
+    step_ = static_cast<size_t>(s);
+}
+
+std::vector<Particle3D> Simulation3D::particles(std::size_t species) const {
+    const Downloaded x = download(*dev_, static_cast<int>(species), 3, cfg_.dt);
+    std::vector<Particle3D> out(static_cast<size_t>(x.n));
+    for (int64_t p = 0; p < x.n; ++p) {
+        Particle3D& q = out[static_cast<size_t>(p)];
+        q.position = Vec3{x.pos[3 * p], x.pos[3 * p + 1], x.pos[3 * p + 2]};
+        q.alive = x.alive[p] != 0;
+        q.velocity_half = Vec3{x.vh[3 * p], x.vh[3 * p + 1], x.vh[3 * p + 2]};
+        q.velocity = Vec3{x.v[3 * p], x.v[3 * p + 1], x.v[3 * p + 2]};
+    }
+    return out;
+}
+
+DiagnosticSample Simulation3D::sample() const {
+    return energies(*dev_, cfg_.dt, cfg_.species.size());
+}
+
+Mesh3D Simulation3D::fields() const {
+    Mesh3D m(cfg_.nx, cfg_.ny, cfg_.nz, cfg_.length_x, cfg_.length_y, cfg_.length_z,
+             cfg_.boundary);
+    const size_t s = m.size();
+    std::vector<double> E(3 * s);
+    dev_->check(aurora_hip_get_fields(dev_->get(), m.rho().data(), m.phi().data(), E.data()));
+    std::copy(E.begin(), E.begin() + s, m.electric_x().begin());
+    std::copy(E.begin() + s, E.begin() + 2 * s, m.electric_y().begin());
+    std::copy(E.begin() + 2 * s, E.end(), m.electric_z().begin());
+    return m;
+}
+
+// -------------------------------------------------------------- swarms
+SwarmService::SwarmService(const CollisionConfig& gas, std::shared_ptr<Device> device)
+    : dev_(or_new(std::move(device))) {
+    const Tables t(gas);
+    dev_->check(aurora_hip_swarm_gas(dev_->get(), static_cast<int>(t.kind.size()), t.kind.data(),
+                                     t.threshold.data(), t.rows.data(), t.energy.data(),
+                                     t.sigma.data(), gas.neutral_density, gas.max_frequency,
+                                     static_cast<int64_t>(gas.max_candidates_per_particle),
+                                     gas.neutral_mass));
+    apply_options(*dev_, 1, gas);
+}
+
+std::int64_t SwarmService::submit(const SwarmRequest& request, std::uint64_t hbm_budget) {
+    int64_t h = 0;
+    dev_->check(aurora_hip_swarm_submit(dev_->get(), &request, hbm_budget, &h));
+    return h;
+}
+
+std::int64_t SwarmService::advance(std::int64_t max_steps) {
+    int64_t active = 0;
+    dev_->check(aurora_hip_swarm_advance(dev_->get(), max_steps, &active));
+    return active;
+}
+
+std::optional<SwarmResult> SwarmService::poll(std::int64_t handle) {
+    int done = 0;
+    SwarmResult r{};
+    dev_->check(aurora_hip_swarm_poll(dev_->get(), handle, &done, &r));
+    if (!done) return std::nullopt;
+    return r;
+}
+
+std::vector<SwarmResult> SwarmService::run_all(const std::vector<SwarmRequest>& requests,
+                                               std::int64_t steps_per_launch) {
+    std::vector<int64_t> handles;
+    for (const auto& r : requests) handles.push_back(submit(r));
+    std::vector<std::optional<SwarmResult>> out(requests.size());
+    size_t left = requests.size();
+    while (left > 0) {
+        advance(steps_per_launch);
+        for (size_t k = 0; k < handles.size(); ++k) {
+            if (out[k]) continue;
+            out[k] = poll(handles[k]);
+            if (out[k]) --left;
+        }
+    }
+    std::vector<SwarmResult> res;
+    for (auto& o : out) res.push_back(*o);
+    return res;
+}
+
+}  // namespace pic::gpu

@@ -1,0 +1,390 @@
+// host_tests.cpp - tests of the C++20 host classes in the style of the
+// reference's tests/test_core.cpp (require / require_near /
+// require_throws_contains), run on the GPU through the C ABI.
+//
+//   aurora_host_tests            run every case
+//   aurora_host_tests <filter>   run the cases whose name contains filter
+//   aurora_host_tests --list     print the case names (no GPU use)
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <iostream>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "aurora_gpu.hpp"
+
+using namespace pic::gpu;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kEV = 1.602176634e-19;
+constexpr double kME = 9.1093837139e-31;
+constexpr double kEps0 = 8.8541878128e-12;
+constexpr double kAmu = 1.66053906660e-27;
+
+void require(bool ok, const std::string& what) {
+    if (!ok) throw std::runtime_error(what);
+}
+
+void require_near(double a, double b, double tol, const std::string& what) {
+    if (!(std::abs(a - b) <= tol))
+        throw std::runtime_error(what + " (" + std::to_string(a) + " vs " + std::to_string(b) +
+                                 ", tolerance " + std::to_string(tol) + ")");
+}
+
+template <typename F>
+void require_throws_contains(F&& f, const std::string& needle, const std::string& what) {
+    try {
+        f();
+    } catch (const std::exception& e) {
+        if (std::string(e.what()).find(needle) != std::string::npos) return;
+        throw std::runtime_error(what + ": wrong message: " + e.what());
+    }
+    throw std::runtime_error(what + ": no exception");
+}
+
+using Dev = std::shared_ptr<Device>;
+
+CollisionConfig unit_elastic(double max_frequency = 2.0) {
+    CollisionConfig c;
+    c.neutral_density = 1.0;
+    c.max_frequency = max_frequency;
+    c.channels.push_back(CollisionChannelConfig{"elastic", CollisionProcessKind::Elastic,
+                                                Table1D{{0.0, 1.0, 2.0}, {0.0, 0.5, 1.0}}});
+    return c;
+}
+
+// Lotz-shaped synthetic helium-like gas on a log grid (J, m^2)
+CollisionConfig synthetic_gas(double density) {
+    const int rows = 400;
+    std::vector<double> e(rows), el(rows), ex(rows), iz(rows);
+    for (int i = 0; i < rows; ++i) {
+        const double ev = 1e-3 * std::pow(1e6, static_cast<double>(i) / (rows - 1));
+        e[i] = ev * kEV;
+        el[i] = 6e-20;
+        ex[i] = ev > 19.8 ? 3e-18 * std::log(ev / 19.8) / (ev * 19.8) : 0.0;
+        iz[i] = ev > 24.6 ? 6e-18 * std::log(ev / 24.6) / (ev * 24.6) : 0.0;
+    }
+    CollisionConfig c;
+    c.neutral_density = density;
+    c.neutral_mass = 4.002602 * kAmu;
+    double numax = 0.0;
+    for (int i = 0; i < rows; ++i)
+        numax = std::max(numax, density * (el[i] + ex[i] + iz[i]) * std::sqrt(2.0 * e[i] / kME));
+    c.max_frequency = 1.05 * numax;
+    c.channels = {
+        {"elastic", CollisionProcessKind::Elastic, Table1D{e, el}},
+        {"excitation", CollisionProcessKind::Excitation, Table1D{e, ex}, 19.8 * kEV},
+        {"ionization", CollisionProcessKind::Ionization, Table1D{e, iz}, 24.6 * kEV}};
+    return c;
+}
+
+// ---------------------------------------------------------------- cases
+void field_solver_dirichlet_parabola(const Dev& d) {
+    // -phi'' = 1 with phi(0) = phi(1) = 0: the central-difference solution
+    // is the parabola x (1 - x) / 2 at the nodes (FieldSolver.cpp:733-759)
+    Grid g(65, 1.0, Boundary::Dirichlet);
+    std::fill(g.rho().begin(), g.rho().end(), 1.0);
+    FieldSolver(1.0, d).solve(g, 0.0, 0.0);
+    for (size_t i = 0; i < g.nx(); ++i) {
+        const double x = g.node_x(i);
+        require_near(g.phi()[i], 0.5 * x * (1.0 - x), 1e-12, "Dirichlet parabola");
+    }
+    require_near(g.electric()[32], 0.0, 1e-12, "Dirichlet field at the centre");
+}
+
+void field_solver_3d_periodic_mode(const Dev& d) {
+    // test_core.cpp:1360-1438: rho = sin(2 pi x) -> phi = rho / k^2,
+    // E_x = -cos(2 pi x) / k with continuous k
+    Mesh3D m(32, 16, 8, 1.0, 2.0, 3.0, Boundary::Periodic);
+    const double k = 2.0 * kPi;
+    for (size_t kk = 0; kk < m.nz(); ++kk)
+        for (size_t j = 0; j < m.ny(); ++j)
+            for (size_t i = 0; i < m.nx(); ++i)
+                m.rho()[m.index(i, j, kk)] = std::sin(k * i * m.dx());
+    FieldSolver(2.0, d).solve(m);
+    for (size_t i = 0; i < m.nx(); ++i) {
+        const size_t n = m.index(i, 3, 5);
+        require_near(m.phi()[n], std::sin(k * i * m.dx()) / (2.0 * k * k), 1e-12, "3D phi mode");
+        require_near(m.electric_x()[n], -std::cos(k * i * m.dx()) / (2.0 * k), 1e-12,
+                     "3D E mode");
+        require_near(m.electric_y()[n], 0.0, 1e-12, "3D E_y");
+    }
+}
+
+void deposit_single_particle_shapes(const Dev& d) {
+    // test_core.cpp:1905-1976: trilinear weights of one particle
+    Mesh3D m(8, 8, 8, 8.0, 8.0, 8.0, Boundary::Periodic);
+    Particle3D p;
+    p.position = Vec3{2.25, 3.5, 4.75};
+    deposit_charge_cic(m, {p}, 2.0, 3.0, d);
+    const double fx[2] = {0.75, 0.25}, fy[2] = {0.5, 0.5}, fz[2] = {0.25, 0.75};
+    double total = 0.0;
+    for (int c = 0; c < 2; ++c)
+        for (int b = 0; b < 2; ++b)
+            for (int a = 0; a < 2; ++a) {
+                const double r = m.rho()[m.index(2 + a, 3 + b, 4 + c)];
+                require_near(r, 6.0 * fx[a] * fy[b] * fz[c], 1e-12, "trilinear corner weight");
+                total += r;
+            }
+    require_near(total, 6.0, 1e-12, "deposited charge");
+    require_near(std::accumulate(m.rho().begin(), m.rho().end(), 0.0), 6.0, 1e-12,
+                 "no charge outside the eight corners");
+}
+
+void deposit_order_independent(const Dev& d) {
+    Mesh2D a(33, 17, 1.0, 1.0, Boundary::Periodic), b = a;
+    std::mt19937_64 rng(11);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    std::vector<Particle2D> ps(20000);
+    for (auto& p : ps) p.position = Vec2{u(rng), u(rng)};
+    deposit_charge_cic(a, ps, -1.0, 0.25, d);
+    std::shuffle(ps.begin(), ps.end(), rng);
+    deposit_charge_cic(b, ps, -1.0, 0.25, d);
+    require(a.rho() == b.rho(), "fixed-point deposit depends on particle order");
+}
+
+void null_collision_elastic_rates(const Dev& d) {
+    // test_core.cpp:353-367
+    NullCollisionModel model(unit_elastic(), 1.0, 8128, d);
+    std::vector<Vec3> v(10000, Vec3{std::sqrt(2.0), 0.0, 0.0});
+    const auto s = model.collide(v, 0.1, 0);
+    require(s.candidates > 1800 && s.candidates < 2200, "candidate rate");
+    require(s.channel_collisions[0] > 620 && s.channel_collisions[0] < 800, "acceptance rate");
+    for (const auto& x : v)
+        require_near(std::sqrt(x.x * x.x + x.y * x.y + x.z * x.z), std::sqrt(2.0), 1e-5,
+                     "elastic speed");
+}
+
+void henyey_greenstein_mean_cosine(const Dev& d) {
+    // test_core.cpp:397-440
+    auto cfg = unit_elastic();
+    cfg.channels[0].angular_scattering = AngularScatteringKind::HenyeyGreenstein;
+    cfg.channels[0].mean_cosine = Table1D{{0.0, 2.0}, {0.8, 0.8}};
+    NullCollisionModel model(cfg, 1.0, 161803, d);
+    const float s2 = static_cast<float>(std::sqrt(2.0));
+    std::vector<Vec3> v(20000, Vec3{s2, 0.0, 0.0});
+    model.collide(v, 0.1, 0);
+    double sum = 0.0;
+    int n = 0;
+    for (const auto& x : v)
+        if (x.x != static_cast<double>(s2)) {
+            sum += x.x / std::sqrt(2.0);
+            ++n;
+        }
+    require(n > 1200, "anisotropic elastic MCC sampled too few collisions");
+    require_near(sum / n, 0.8, 0.04, "Henyey-Greenstein mean cosine");
+}
+
+void collision_config_errors(const Dev& d) {
+    auto cx = unit_elastic();
+    cx.channels[0].process = CollisionProcessKind::ChargeExchange;
+    cx.neutral_mass = 40.0;
+    require_throws_contains([&] { NullCollisionModel(cx, 39.0, 1, d); },
+                            "projectile mass equal to neutral_mass", "mismatched charge exchange");
+    auto backward_excitation = unit_elastic(4.0);
+    backward_excitation.channels[0].process = CollisionProcessKind::Excitation;
+    backward_excitation.channels[0].threshold_energy = 0.5;
+    backward_excitation.channels[0].angular_scattering = AngularScatteringKind::Backward;
+    require_throws_contains([&] { NullCollisionModel(backward_excitation, 1.0, 1, d); },
+                            "supported only for elastic channels", "anisotropic excitation");
+    auto dup = unit_elastic();
+    dup.channels.push_back(dup.channels[0]);
+    require_throws_contains([&] { NullCollisionModel(dup, 1.0, 1, d); },
+                            "duplicate MCC collision channel name", "duplicate channel names");
+}
+
+void simulation3d_tabulated_field_energy(const Dev& d) {
+    // normalised units: n0 = 1, omega_p = 1 for electrons, dx = 1
+    Simulation3DConfig cfg;
+    cfg.nx = cfg.ny = cfg.nz = 16;
+    cfg.length_x = cfg.length_y = cfg.length_z = 16.0;
+    cfg.dt = 0.05;
+    cfg.steps = 40;
+    cfg.permittivity = 1.0;
+    const size_t n = 8 * 16 * 16 * 16;
+    const double w = 16.0 * 16.0 * 16.0 / static_cast<double>(n);
+    cfg.species = {Species3DConfig{"electrons", -1.0, 1.0, w, n, {}, 0.5},
+                   Species3DConfig{"ions", 1.0, 400.0, w, n, {}, 0.02}};
+    TabulatedVectorField1D b;
+    b.axis = 0;
+    for (int r = 0; r < 65; ++r) {
+        const double x = 16.0 * r / 64.0;
+        b.coordinates.push_back(x);
+        b.values.push_back(Vec3{0.0, 0.3 * std::exp(-std::pow((x - 8.0) / 2.0, 2)), 0.0});
+    }
+    cfg.magnetic_field_profile = b;
+    Simulation3D sim(cfg, d);
+    sim.initialize();
+    const auto s0 = sim.sample();
+    sim.run();
+    const auto s1 = sim.sample();
+    require(sim.step_count() == 40, "step count");
+    require_near(sim.time(), 40 * 0.05, 1e-12, "simulation time");
+    require(s1.live_particles == 2 * n, "periodic run lost particles");
+    const double e0 = s0.kinetic_energy + s0.field_energy;
+    const double e1 = s1.kinetic_energy + s1.field_energy;
+    require_near(e1, e0, 0.02 * e0, "total energy drift over 40 steps");
+    const auto ps = sim.particles(0);
+    require(ps.size() == n, "particle download");
+    for (size_t p = 0; p < ps.size(); p += 997)
+        require(ps[p].position.x >= 0.0 && ps[p].position.x <= 16.0, "position inside the box");
+}
+
+void simulation1d_discharge_with_ionization(const Dev& d) {
+    Config cfg;
+    cfg.nx = 65;
+    cfg.length = 0.067;
+    cfg.boundary = Boundary::Dirichlet;
+    cfg.dt = 1.0 / (400.0 * 13.56e6);
+    cfg.steps = 100;
+    cfg.permittivity = kEps0;
+    cfg.phi_right_drive = SinusoidalVoltageConfig{200.0, 13.56e6, 0.0};
+    cfg.max_particles_per_species = 20000;
+    const double w = 2.56e14 * 0.067 / 3000.0;
+    cfg.species = {SpeciesConfig{"electrons", -kEV, kME, w, 3000, 0.0, std::sqrt(8.0 * kEV / kME)},
+                   SpeciesConfig{"ions", kEV, 6.67e-27, w, 3000, 0.0,
+                                 std::sqrt(0.026 * kEV / 6.67e-27)}};
+    cfg.collision_models = {NamedCollisionConfig{"electrons", synthetic_gas(1e22), "electrons",
+                                                 "ions"}};
+    Simulation sim(cfg, d);
+    sim.run();
+    const auto st = sim.collision_statistics();
+    require(st.channel_collisions.size() == 3, "channel statistics");
+    require(st.channel_collisions[2] > 0, "no ionization sampled");
+    require(st.secondaries == st.channel_collisions[2], "one secondary per ionization");
+    const auto e = sim.particles(0);
+    const auto i = sim.particles(1);
+    require(e.size() == 3000 + st.secondaries && i.size() == 3000 + st.secondaries,
+            "products appended to both species");
+    const auto loss = sim.boundary_losses(0);
+    require(loss.left_particles + loss.right_particles > 0, "no electron reached a wall");
+    require(loss.left_energy + loss.right_energy > 0.0, "wall energy ledger");
+    const Grid g = sim.fields();
+    require_near(g.phi()[0], 0.0, 1e-12, "grounded electrode");
+}
+
+void swarm_zero_cross_section(const Dev& d) {
+    // test_core.cpp:6190-6230
+    CollisionConfig gas;
+    gas.neutral_density = 1e20;
+    gas.max_frequency = 1e7;
+    gas.neutral_mass = 4.002602 * kAmu;
+    gas.channels = {{"elastic", CollisionProcessKind::Elastic, Table1D{{0.0, 1e-16}, {0.0, 0.0}}}};
+    SwarmService svc(gas, d);
+    SwarmRequest r{};
+    r.reduced_field_td = 1.0;
+    r.timestep = 1e-9;
+    r.particles = 300;
+    r.steps = 40;
+    r.burn_in = 10;
+    r.blocks = 5;
+    r.initial_mean_energy_ev = 0.0;
+    r.max_energy_ev = 600.0;
+    r.seed = 3;
+    const auto res = svc.run_all({r}, 64);
+    const double a_dt = kEV * 1e-21 * 1e20 / kME * 1e-9;
+    double mean_s = 0.0;
+    for (int s = 11; s <= 40; ++s) mean_s += s;
+    mean_s /= 30.0;
+    require(res[0].status == AURORA_OK, "swarm status");
+    require_near(res[0].drift, a_dt * mean_s, 1e-5 * a_dt * mean_s, "zero-sigma drift");
+    require(res[0].collisions[0] == 0, "collisions with zero cross section");
+}
+
+void swarm_alone_equals_batched(const Dev& d) {
+    const auto gas = synthetic_gas(1e21);
+    SwarmRequest a{};
+    a.reduced_field_td = 150.0;
+    a.timestep = 0.099 / gas.max_frequency;
+    a.particles = 2000;
+    a.steps = 600;
+    a.burn_in = 200;
+    a.blocks = 8;
+    a.initial_mean_energy_ev = 1.0;
+    a.max_energy_ev = 1000.0;
+    a.seed = 77;
+    SwarmRequest b = a;
+    b.reduced_field_td = 20.0;
+    b.particles = 5000;
+    b.seed = 78;
+    SwarmService alone(gas, d);
+    const auto x = alone.run_all({a}, 600);
+    auto d2 = std::make_shared<Device>(0);
+    SwarmService mixed(gas, d2);
+    const auto y = mixed.run_all({b, a}, 97);
+    require(x[0].drift == y[1].drift && x[0].mean_energy_ev == y[1].mean_energy_ev &&
+                x[0].ionization_rate == y[1].ionization_rate,
+            "a request's result depends on its batch");
+}
+
+void error_paths(const Dev& d) {
+    require_throws_contains(
+        [&] {
+            Mesh3D m(8, 8, 8, 1.0, 1.0, 1.0, Boundary::Dirichlet);
+            FieldSolver(1.0, d).solve(m);
+        },
+        "Dirichlet field solves", "3D Dirichlet solve");
+    Simulation3DConfig cfg;
+    cfg.nx = cfg.ny = cfg.nz = 8;
+    cfg.species = {Species3DConfig{}};
+    TabulatedVectorField1D b{0, {0.0, 0.5}, {Vec3{}, Vec3{}}};
+    cfg.magnetic_field_profile = b;
+    require_throws_contains([&] { Simulation3D sim(cfg, d); },
+                            "does not cover the simulation domain", "short B table");
+    require_throws_contains([&] { Grid g(2, 1.0, Boundary::Periodic); }, "at least 3",
+                            "tiny grid");
+}
+
+struct Case {
+    const char* name;
+    void (*body)(const Dev&);
+};
+
+const Case kCases[] = {
+    {"field_solver_dirichlet_parabola", field_solver_dirichlet_parabola},
+    {"field_solver_3d_periodic_mode", field_solver_3d_periodic_mode},
+    {"deposit_single_particle_shapes", deposit_single_particle_shapes},
+    {"deposit_order_independent", deposit_order_independent},
+    {"null_collision_elastic_rates", null_collision_elastic_rates},
+    {"henyey_greenstein_mean_cosine", henyey_greenstein_mean_cosine},
+    {"collision_config_errors", collision_config_errors},
+    {"simulation3d_tabulated_field_energy", simulation3d_tabulated_field_energy},
+    {"simulation1d_discharge_with_ionization", simulation1d_discharge_with_ionization},
+    {"swarm_zero_cross_section", swarm_zero_cross_section},
+    {"swarm_alone_equals_batched", swarm_alone_equals_batched},
+    {"error_paths", error_paths},
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const std::string arg = argc > 1 ? argv[1] : "";
+    if (arg == "--list") {
+        for (const auto& c : kCases) std::printf("%s\n", c.name);
+        return 0;
+    }
+    int passed = 0, failed = 0;
+    for (const auto& c : kCases) {
+        if (!arg.empty() && std::string(c.name).find(arg) == std::string::npos) continue;
+        std::printf("RUN  %s\n", c.name);
+        std::fflush(stdout);
+        try {
+            auto dev = std::make_shared<Device>(0);
+            c.body(dev);
+            std::printf("PASS %s\n", c.name);
+            ++passed;
+        } catch (const std::exception& e) {
+            std::printf("FAIL %s: %s\n", c.name, e.what());
+            ++failed;
+        }
+        std::fflush(stdout);
+    }
+    std::printf("%d passed, %d failed\n", passed, failed);
+    return failed ? 1 : 0;
+}

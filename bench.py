@@ -71,6 +71,77 @@ def cpu_baseline(cfg, ppc_sample: int):
     return None
 
 
+def serving_leg(args):
+    """Batched swarm serving: Poisson arrivals at 200 req/s (E/N log-uniform
+    1-1000 Td, log-normal particle counts, synthetic gas), admitted by HBM
+    footprint and advanced together, mixed with a 1D3V discharge ensemble
+    stepped between swarm launches; plus the sequential single-request
+    baseline on a sample of the same requests."""
+    from aurora_pic import hip, serving
+    from aurora_pic import workloads as wl
+    import numpy as np
+
+    cfg = wl.config_swarm_requests(4096)
+    R = len(cfg["arrival_s"]) if args.serving_full else args.serving_requests
+    steps = cfg["steps"] if args.serving_full else args.serving_steps
+    burn = steps // 2
+    reqs = serving.make_requests(cfg, count=R, steps=steps, burn_in=burn)
+    ens_ctx, ens, ens_dt = serving.discharge_ensemble(args.ensemble_instances,
+                                                      args.ensemble_particles)
+    ens_steps = [0]
+
+    def ensemble_tick():
+        ens.step(ens_dt, args.ensemble_steps_per_launch)
+        ens_steps[0] += args.ensemble_steps_per_launch
+
+    ctx = hip.Context(0)
+    ctx.swarm_gas(cfg["gas"])
+    budget = args.serving_hbm_gib << 30
+    log(f"serving {R} swarm requests x {steps} steps (arrivals 200/s, budget "
+        f"{args.serving_hbm_gib} GiB) + {args.ensemble_instances} discharge instances")
+    t0 = time.perf_counter()
+    rep = serving.serve(ctx, reqs, cfg["arrival_s"][:R], budget,
+                        steps_per_launch=args.serving_launch_steps, on_launch=ensemble_tick)
+    wall = time.perf_counter() - t0
+    summ = rep.summary()
+    ens_updates = ens_steps[0] * args.ensemble_instances * args.ensemble_particles
+    log(f"serving: {json.dumps(summ)}; ensemble {ens_steps[0]} steps")
+    # sequential single-request baseline on an evenly spaced sample
+    idx = list(range(0, R, max(1, R // args.serving_sequential)))[:args.serving_sequential]
+    sample = [reqs[i] for i in idx]
+    seq_results, seq_wall, seq_steps = serving.run_sequential(ctx, sample)
+    seq_rate = seq_steps / seq_wall
+    diffs = []
+    for i, r in zip(idx, seq_results):
+        b = rep.served[i].result
+        for f in ("drift", "mean_energy_ev", "ionization_rate"):
+            x, y = getattr(r, f), getattr(b, f)
+            diffs.append(abs(x - y) / max(abs(x), 1e-300))
+    ctx.close()
+    ens_ctx.close()
+    lat = rep.latencies()
+    return {
+        "workload": f"{R} electron-swarm requests, Poisson arrivals 200/s, E/N 1-1000 Td "
+                    f"log-uniform, particles log-normal median 5e4 [1e4, 1e6], {steps} steps "
+                    f"({burn} burn-in), synthetic elastic/excitation/ionization gas at 1e21 "
+                    f"m^-3; mixed with {args.ensemble_instances} distinct-seed 1D3V discharge "
+                    f"instances of {args.ensemble_particles} particles (HIP-graph ensemble)",
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "requests_per_s": summ["requests_per_s"],
+        "swarm_electron_steps_per_s": summ["electron_steps_per_s"],
+        "wall_s": wall,
+        "max_active": summ["max_active"],
+        "launches": summ["launches"],
+        "budget_refusals": summ["budget_refusals"],
+        "sequential_electron_steps_per_s": seq_rate,
+        "batched_over_sequential": summ["electron_steps_per_s"] / seq_rate,
+        "sequential_sample": len(sample),
+        "max_rel_diff_vs_sequential": max(diffs) if diffs else None,
+        "ensemble_particle_updates_per_s": ens_updates / wall,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +157,17 @@ def main():
                     help="fixed tile shape tx,ty,tz,halo (skips the autotuner)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-full", action="store_true",
+                    help="all 4,096 requests x 20,000 steps (minutes)")
+    ap.add_argument("--serving-requests", type=int, default=256)
+    ap.add_argument("--serving-steps", type=int, default=4000)
+    ap.add_argument("--serving-launch-steps", type=int, default=500)
+    ap.add_argument("--serving-hbm-gib", type=int, default=8)
+    ap.add_argument("--serving-sequential", type=int, default=16)
+    ap.add_argument("--ensemble-instances", type=int, default=64)
+    ap.add_argument("--ensemble-particles", type=int, default=100_000)
+    ap.add_argument("--ensemble-steps-per-launch", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the per-step loop of one 3D3V instance runs on a "
@@ -179,6 +261,14 @@ def main():
                 if line.startswith("bytes_per_launch:"):
                     traffic = int(line.split(":", 1)[1])
 
+    serving_out = None
+    if not args.no_serving:
+        try:
+            serving_out = serving_leg(args)
+        except Exception as e:  # the serving leg must not hide the headline result
+            log("serving leg failed:", repr(e))
+            serving_out = {"error": repr(e)}
+
     base = None
     if not args.no_cpu_baseline:
         log("cpu baseline (reference, bounded sample)...")
@@ -212,6 +302,7 @@ def main():
                      "kernel": "push_deposit (gather+Boris+drift+wrap+int64 deposit)",
                      "bytes_per_update": BYTES_PER_UPDATE},
         "cpu_baseline": base,
+        "serving": serving_out,
     }
     print(json.dumps(out), flush=True)
 

@@ -55,7 +55,11 @@ enum { AURORA_PB_ABSORB = 1, AURORA_PB_REFLECT = 2,
        AURORA_PB_PERIODIC = 3 };                               /* Types.hpp:33 */
 enum { AURORA_VELOCITY_HALF = 0, AURORA_VELOCITY_CENTRED = 1 };
 enum { AURORA_ELASTIC = 0, AURORA_EXCITATION = 1, AURORA_IONIZATION = 2,
-       AURORA_ATTACHMENT = 3 };                                /* Config.hpp:42 */
+       AURORA_ATTACHMENT = 3, AURORA_CHARGE_EXCHANGE = 4 };    /* Config.hpp:42-48 */
+enum { AURORA_SCATTER_ISOTROPIC = 0, AURORA_SCATTER_BACKWARD = 1,
+       AURORA_SCATTER_HENYEY_GREENSTEIN = 2 };                 /* Config.hpp:33-37 */
+enum { AURORA_FRAME_PROJECTILE = 0,
+       AURORA_FRAME_CENTER_OF_MASS = 1 };                      /* Config.hpp:38-41 */
 
 /* ---------------------------------------------------------------- context */
 /* Runtime.cpp:22-76 (RuntimePolicy with backend = gpu).  device: HIP
@@ -220,6 +224,19 @@ int aurora_hip_mcc_load(aurora_hip_ctx* ctx, int target_species, int nchan,
                         double max_frequency, int64_t max_candidates,
                         double neutral_mass, double neutral_temperature,
                         const int* product_species, uint64_t seed);
+/* Per-channel options of CollisionChannelConfig (Config.hpp:82-108) for
+ * the context's MCC model (model = 0) or the swarm gas (model = 1):
+ * angular_scattering (AURORA_SCATTER_*; anisotropic only for elastic
+ * channels, Collision.cpp:455-461), with the Henyey-Greenstein mean-cosine
+ * table (rows, energy J strictly increasing and covering the cross-section
+ * table, mean cosine in (-1, 1); MeanCosineTable, Collision.cpp:239-313),
+ * and energy_frame (AURORA_FRAME_*; centre of mass needs a positive
+ * neutral mass and an elastic or charge-exchange channel,
+ * Collision.cpp:483-500).  Reset to isotropic / projectile on every load. */
+int aurora_hip_mcc_channel_options(aurora_hip_ctx* ctx, int model, int channel,
+                                   int angular, int energy_frame, int64_t rows,
+                                   const double* energy,
+                                   const double* mean_cosine);
 /* Accumulated CollisionStepStatistics (Collision.hpp:50-63) since load:
  * counts = [candidates, nulls, per-channel collisions (nchan)],
  * energy_change per channel (J, weighted by the species weight). */

@@ -89,11 +89,20 @@ static double speed3(const double* v) {
     return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
 }
 
+/* Collision.cpp:579-594: collision energy of channel c at relative speed
+ * g, with the projectile mass or the reduced mass (centre-of-mass frame) */
+static double collision_energy(const or_mcc_model* m, int c, double g) {
+    double mass = m->particle_mass;
+    if (m->frame[c] == OR_FRAME_CENTER_OF_MASS)
+        mass = m->particle_mass * m->neutral_mass / (m->particle_mass + m->neutral_mass);
+    return 0.5 * mass * g * g;
+}
+
 /* Collision.cpp:596-625; returns 0 or 1 (total above max_frequency) */
 static int rates_for_speed(const or_mcc_model* m, double speed, double* rates) {
     double total = 0.0;
     for (int c = 0; c < m->nchan; ++c) {
-        double energy = 0.5 * m->particle_mass * speed * speed;
+        double energy = collision_energy(m, c, speed);
         double rate = 0.0;
         if (energy >= m->threshold[c])
             rate = m->neutral_density *
@@ -114,10 +123,10 @@ static int validate_bound(const or_mcc_model* m, double speed) {
     const double vmax = speed + lim;
     double bound = 0.0;
     for (int c = 0; c < m->nchan; ++c) {
-        double emax = 0.5 * m->particle_mass * vmax * vmax;
+        double emax = collision_energy(m, c, vmax);
         if (emax < m->threshold[c]) continue;
         double vmin = speed - lim > 0.0 ? speed - lim : 0.0;
-        double emin = 0.5 * m->particle_mass * vmin * vmin;
+        double emin = collision_energy(m, c, vmin);
         if (emin < m->threshold[c]) emin = m->threshold[c];
         double smin = or_xs_eval(m->rows[c], m->energy[c], m->sigma[c], emin);
         double smax = or_xs_eval(m->rows[c], m->energy[c], m->sigma[c], emax);
@@ -142,6 +151,38 @@ static void isotropic(double speed, or_stream* rng, double* out) {
     out[0] = speed * sine * cos(azimuth);
     out[1] = speed * sine * sin(azimuth);
     out[2] = speed * cosine;
+}
+
+/* Henyey-Greenstein deflection about the incoming relative velocity
+ * (Collision.cpp:66-134): cosine from the inverse HG CDF with mean cosine
+ * mc, azimuth uniform, frame built from a helper axis. */
+static void angular_velocity(const double* g0, double speed, double mc, or_stream* rng,
+                             double* out) {
+    const double inc = speed3(g0);
+    if (!(inc > 0.0)) { isotropic(speed, rng, out); return; }
+    const double sample = or_stream_u01(rng);
+    double cosine = 2.0 * sample - 1.0;
+    if (mc != 0.0) {
+        const double num = 1.0 - mc * mc;
+        const double den = 1.0 - mc + 2.0 * mc * sample;
+        const double ratio = num / den;
+        cosine = (1.0 + mc * mc - ratio * ratio) / (2.0 * mc);
+        if (cosine < -1.0) cosine = -1.0;
+        if (cosine > 1.0) cosine = 1.0;
+    }
+    const double s2 = 1.0 - cosine * cosine;
+    const double sine = sqrt(s2 > 0.0 ? s2 : 0.0);
+    const double azimuth = 2.0 * PI * or_stream_u01(rng);
+    const double d[3] = {g0[0] / inc, g0[1] / inc, g0[2] / inc};
+    const double h[3] = {0.0, fabs(d[2]) < 0.9 ? 0.0 : 1.0, fabs(d[2]) < 0.9 ? 1.0 : 0.0};
+    double t[3] = {h[1] * d[2] - h[2] * d[1], h[2] * d[0] - h[0] * d[2],
+                   h[0] * d[1] - h[1] * d[0]};
+    const double tn = speed3(t);
+    t[0] /= tn; t[1] /= tn; t[2] /= tn;
+    const double b[3] = {d[1] * t[2] - d[2] * t[1], d[2] * t[0] - d[0] * t[2],
+                         d[0] * t[1] - d[1] * t[0]};
+    const double tf = sine * cos(azimuth), bf = sine * sin(azimuth);
+    for (int a = 0; a < 3; ++a) out[a] = speed * (cosine * d[a] + tf * t[a] + bf * b[a]);
 }
 
 /* Bounded Maxwellian neutral (Collision.cpp:716-731): Box-Muller normals,
@@ -193,9 +234,11 @@ void or_mcc_collide_3v(const or_mcc_model* m, double* v, double dt,
         const int c = accepted;
         const double e_before = 0.5 * m->particle_mass *
                                 (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        /* apply_channel (Collision.cpp:766-829), isotropic scattering */
+        /* apply_channel (Collision.cpp:738-829) */
         if (m->kind[c] == OR_ATTACHMENT) {
             v[0] = v[1] = v[2] = 0.0;
+        } else if (m->kind[c] == OR_CHARGE_EXCHANGE) {
+            for (int a = 0; a < 3; ++a) v[a] = w[a];
         } else {
             const double gs = speed3(g);
             double energy = 0.5 * m->particle_mass * gs * gs;
@@ -206,7 +249,17 @@ void or_mcc_collide_3v(const or_mcc_model* m, double* v, double dt,
             if (m->kind[c] == OR_IONIZATION) energy *= 0.5;
             const double sp = sqrt(2.0 * energy / m->particle_mass);
             double sc[3];
-            isotropic(sp, rng, sc);
+            if (m->angular[c] == OR_BACKWARD) {
+                for (int a = 0; a < 3; ++a) sc[a] = -g[a];
+            } else if (m->angular[c] == OR_HENYEY_GREENSTEIN) {
+                /* MeanCosineTable::evaluate (Collision.cpp:296-313) interpolates
+                 * like the cross-section tables */
+                const double mc = or_xs_eval(m->mc_rows[c], m->mc_energy[c], m->mc_value[c],
+                                             collision_energy(m, c, gs));
+                angular_velocity(g, sp, mc, rng, sc);
+            } else {
+                isotropic(sp, rng, sc);
+            }
             if (m->kind[c] == OR_ELASTIC && m->neutral_mass > 0.0) {
                 const double mt = m->particle_mass + m->neutral_mass;
                 const double rf = m->neutral_mass / mt;

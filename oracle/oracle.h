@@ -117,7 +117,11 @@ double or_u01(uint32_t u);
 
 /* ---- A13 MCC model (Collision.cpp:315-993), cold or thermal neutrals. */
 enum { OR_ELASTIC = 0, OR_EXCITATION = 1, OR_IONIZATION = 2,
-       OR_ATTACHMENT = 3 };
+       OR_ATTACHMENT = 3, OR_CHARGE_EXCHANGE = 4 };
+/* angular scattering (Config.hpp:33-37) and collision energy frame
+ * (Config.hpp:38-41) per channel */
+enum { OR_ISOTROPIC = 0, OR_BACKWARD = 1, OR_HENYEY_GREENSTEIN = 2 };
+enum { OR_FRAME_PROJECTILE = 0, OR_FRAME_CENTER_OF_MASS = 1 };
 typedef struct {
     int nchan;
     int kind[8];
@@ -131,6 +135,11 @@ typedef struct {
     double particle_mass;
     double neutral_mass;          /* 0 = infinitely heavy, cold */
     double neutral_vstd;          /* thermal neutral sigma, 0 = cold */
+    int angular[8];               /* OR_ISOTROPIC / OR_BACKWARD / OR_HENYEY_GREENSTEIN */
+    int frame[8];                 /* OR_FRAME_* */
+    int64_t mc_rows[8];           /* Henyey-Greenstein mean-cosine table */
+    const double* mc_energy[8];   /* J, strictly increasing */
+    const double* mc_value[8];    /* mean cosine in (-1, 1) */
 } or_mcc_model;
 
 typedef struct {

@@ -47,6 +47,11 @@ def i64(a):
     return np.ascontiguousarray(a, dtype=np.int64)
 
 
+ELASTIC, EXCITATION, IONIZATION, ATTACHMENT, CHARGE_EXCHANGE = range(5)
+ISOTROPIC, BACKWARD, HENYEY_GREENSTEIN = range(3)
+FRAME_PROJECTILE, FRAME_CENTER_OF_MASS = range(2)
+
+
 class _MccModel(C.Structure):
     _fields_ = [
         ("nchan", C.c_int),
@@ -61,6 +66,11 @@ class _MccModel(C.Structure):
         ("particle_mass", C.c_double),
         ("neutral_mass", C.c_double),
         ("neutral_vstd", C.c_double),
+        ("angular", C.c_int * 8),
+        ("frame", C.c_int * 8),
+        ("mc_rows", C.c_int64 * 8),
+        ("mc_energy", _dp * 8),
+        ("mc_value", _dp * 8),
     ]
 
 
@@ -124,8 +134,9 @@ class MccModel:
 
     def __init__(self, channels, neutral_density, max_frequency,
                  particle_mass, max_candidates=64, neutral_mass=0.0,
-                 neutral_vstd=0.0):
+                 neutral_vstd=0.0, options=None):
         # channels: list of (kind, threshold_J, energies_J, sigmas_m2)
+        # options: {channel: (angular, frame, mc_energy_J or None, mc_value or None)}
         self.channels = [(int(k), float(t), f64(e), f64(s)) for k, t, e, s in channels]
         m = _MccModel()
         m.nchan = len(self.channels)
@@ -135,6 +146,16 @@ class MccModel:
             m.rows[c] = len(e)
             m.energy[c] = _ptr(e, _dp)
             m.sigma[c] = _ptr(s, _dp)
+        self._mc = []
+        for c, (ang, frame, me, mv) in (options or {}).items():
+            m.angular[c] = int(ang)
+            m.frame[c] = int(frame)
+            if me is not None:
+                me, mv = f64(me), f64(mv)
+                self._mc.append((me, mv))
+                m.mc_rows[c] = len(me)
+                m.mc_energy[c] = _ptr(me, _dp)
+                m.mc_value[c] = _ptr(mv, _dp)
         m.neutral_density = neutral_density
         m.max_frequency = max_frequency
         m.max_candidates = max_candidates

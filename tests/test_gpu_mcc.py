@@ -184,3 +184,115 @@ def test_step_1d_with_mcc_matches_oracle(oracle):
     # product uids continue after the uploaded ones, in parent order
     assert np.array_equal(u[N:slots_e], np.arange(N, slots_e, dtype=np.uint32))
     c.close()
+
+
+def _compare_same_stream(ctx, oracle, model, vin, dt, seed, tol_rel=1e-4, tol_abs=1e-3):
+    vout, counts, _ = ctx.mcc_collide(vin, dt, step=0)
+    nchan = len(model.channels)
+    ocounts = np.zeros(3 + nchan, dtype=np.int64)
+    close = 0
+    for p in range(vin.shape[0]):
+        ov, st = oracle.collide(model, vin[p], dt, seed, 0, p, 0)
+        assert st.status == 0
+        ocounts += [st.candidates, st.nulls] + [st.chan[c] for c in range(nchan)] + [st.nsec]
+        if np.max(np.abs(vout[p] - ov)) <= tol_rel * np.linalg.norm(ov) + tol_abs:
+            close += 1
+    assert close >= 0.99 * vin.shape[0], close
+    for k in range(3 + nchan):
+        assert abs(int(counts[k]) - int(ocounts[k])) <= 5 * math.sqrt(max(ocounts[k], 1)) + 2, k
+    return vout, counts
+
+
+def test_anisotropic_com_electron_model_matches_oracle(ctx, oracle):
+    """Henyey-Greenstein elastic scattering in the centre-of-mass frame with
+    an energy-dependent mean cosine, plus excitation and ionization."""
+    from aurora_pic import hip
+    gas = workloads.synthetic_gas(rows=200)
+    e_mc = np.array([0.0, 10.0 * workloads.EV, 2000.0 * workloads.EV])
+    v_mc = np.array([0.1, 0.6, 0.9])
+    _load(ctx, gas["channels"], gas["neutral_density"], gas["max_frequency"], workloads.ME,
+          neutral_mass=gas["neutral_mass"], seed=99)
+    ctx.mcc_channel_options(0, hip.SCATTER_HENYEY_GREENSTEIN, hip.FRAME_CENTER_OF_MASS, e_mc, v_mc)
+    model = oracle_py.MccModel(gas["channels"], gas["neutral_density"], gas["max_frequency"],
+                               workloads.ME, 64, gas["neutral_mass"],
+                               options={0: (oracle_py.HENYEY_GREENSTEIN,
+                                            oracle_py.FRAME_CENTER_OF_MASS, e_mc, v_mc)})
+    rng = np.random.default_rng(3)
+    vin = rng.normal(0.0, 3.0e6, size=(4000, 3))
+    _compare_same_stream(ctx, oracle, model, vin, 0.099 / gas["max_frequency"] * 5, 99)
+
+
+def test_thermal_ion_charge_exchange_backward_matches_oracle(ctx, oracle):
+    """Ion model: resonant charge exchange and backward elastic scattering,
+    both in the centre-of-mass frame, against 300 K thermal neutrals."""
+    from aurora_pic import hip
+    mi = 40.0 * workloads.AMU
+    e = np.geomspace(1e-3, 1e3, 300) * workloads.EV
+    cx = 5e-19 * np.ones_like(e)
+    el = 3e-19 / np.sqrt(1.0 + e / workloads.EV)
+    chans = [(hip.CHARGE_EXCHANGE, 0.0, e, cx), (hip.ELASTIC, 0.0, e, el)]
+    n0, T = 1e21, 300.0
+    vmax = math.sqrt(2 * 1e3 * workloads.EV / mi)
+    numax = 1.2 * n0 * (5e-19 + 3e-19) * (vmax + 8 * math.sqrt(workloads.KB * T / mi) + 2e4)
+    sid = ctx.add_species(workloads.EV, mi, 1.0, 64)
+    gas = dict(channels=chans, neutral_density=n0, max_frequency=numax, neutral_mass=mi)
+    ctx.mcc_load(sid, gas, seed=4242, neutral_temperature=T)
+    ctx.mcc_channel_options(0, hip.SCATTER_ISOTROPIC, hip.FRAME_CENTER_OF_MASS)
+    ctx.mcc_channel_options(1, hip.SCATTER_BACKWARD, hip.FRAME_CENTER_OF_MASS)
+    model = oracle_py.MccModel(chans, n0, numax, mi, 64, mi, math.sqrt(workloads.KB * T / mi),
+                               options={0: (0, oracle_py.FRAME_CENTER_OF_MASS, None, None),
+                                        1: (oracle_py.BACKWARD, oracle_py.FRAME_CENTER_OF_MASS,
+                                            None, None)})
+    rng = np.random.default_rng(4)
+    vin = rng.normal(0.0, 2.0e3, size=(3000, 3))
+    _, counts = _compare_same_stream(ctx, oracle, model, vin, 5.0 / numax, 4242, tol_abs=1e-1)
+    assert counts[2] > 0 and counts[3] > 0
+
+
+def test_reference_closed_forms_for_channel_options(ctx):
+    """test_core.cpp:397-440 (HG mean cosine 0.8), 530-549 (centre-of-mass
+    cross section), 559-583 (equal-mass backward), 794-817 (charge
+    exchange) on the device."""
+    from aurora_pic import hip
+    e, s = [0.0, 1.0, 2.0], [0.0, 0.5, 1.0]
+    v0 = np.tile([math.sqrt(2.0), 0.0, 0.0], (20000, 1))
+    _load(ctx, [(hip.ELASTIC, 0.0, e, s)], 1.0, 2.0, 1.0, seed=161803)
+    ctx.mcc_channel_options(0, hip.SCATTER_HENYEY_GREENSTEIN, 0, [0.0, 2.0], [0.8, 0.8])
+    vout, counts, _ = ctx.mcc_collide(v0, 0.1, step=0)
+    assert np.max(np.abs(np.linalg.norm(vout, axis=1) - math.sqrt(2.0))) < 1e-5
+    s2 = float(np.float32(math.sqrt(2.0)))    # the uploaded FP32 velocity
+    moved = vout[:, 0] != s2
+    cosines = vout[moved, 0] / math.sqrt(2.0)
+    assert moved.sum() > 1200
+    assert abs(cosines.mean() - 0.8) < 0.04
+    # centre-of-mass frame (equal masses): sigma(0.5) = 0.25 -> ~354 of 10000
+    _load(ctx, [(hip.ELASTIC, 0.0, e, s)], 1.0, 2.0, 1.0, neutral_mass=1.0, seed=141421)
+    ctx.mcc_channel_options(0, 0, hip.FRAME_CENTER_OF_MASS)
+    _, counts, _ = ctx.mcc_collide(v0[:10000], 0.1, step=0)
+    assert 280 < counts[2] < 440
+    # backward, equal masses: the projectile stops
+    ctx.mcc_channel_options(0, hip.SCATTER_BACKWARD, hip.FRAME_CENTER_OF_MASS)
+    vout, counts, _ = ctx.mcc_collide(v0[:10000], 0.1, step=1)
+    hit = vout[:, 0] != s2
+    assert hit.sum() > 0 and np.max(np.linalg.norm(vout[hit], axis=1)) < 1e-6
+    # resonant charge exchange with a cold neutral: the ion stops exactly
+    _load(ctx, [(hip.CHARGE_EXCHANGE, 0.0, e, s)], 1.0, 2.0, 40.0, neutral_mass=40.0, seed=141421)
+    vout, counts, _ = ctx.mcc_collide(np.tile([1.0, 0.0, 0.0], (10000, 1)), 0.01, step=0)
+    hit = counts[2]
+    assert hit > 0 and np.sum(np.linalg.norm(vout, axis=1) == 0.0) == hit
+    assert counts[3] == 0
+
+
+def test_channel_option_errors(ctx):
+    from aurora_pic import hip
+    e, s = [0.0, 1.0, 2.0], [0.0, 0.5, 1.0]
+    _load(ctx, [(hip.EXCITATION, 0.5, e, s)], 1.0, 4.0, 1.0)
+    with pytest.raises(hip.HipError, match="supported only for elastic channels"):
+        ctx.mcc_channel_options(0, hip.SCATTER_BACKWARD, 0)
+    with pytest.raises(hip.HipError, match="requires positive neutral_mass"):
+        ctx.mcc_channel_options(0, 0, hip.FRAME_CENTER_OF_MASS)
+    _load(ctx, [(hip.ELASTIC, 0.0, e, s)], 1.0, 2.0, 1.0)
+    with pytest.raises(hip.HipError, match="must cover the cross-section table"):
+        ctx.mcc_channel_options(0, hip.SCATTER_HENYEY_GREENSTEIN, 0, [0.5, 2.0], [0.2, 0.2])
+    with pytest.raises(hip.HipError, match="projectile mass equal to neutral_mass"):
+        _load(ctx, [(hip.CHARGE_EXCHANGE, 0.0, e, s)], 1.0, 2.0, 39.0, neutral_mass=40.0)

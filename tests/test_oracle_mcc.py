@@ -187,3 +187,87 @@ def test_block_mean_error_known_answer(oracle):
     m, se = oracle.block_mean_error([1.0, 2.0, 3.0, 4.0, 5.0, 6.0], 3)
     assert m == 3.5
     assert abs(se - math.sqrt(((1.5 - 3.5) ** 2 + 0 + (5.5 - 3.5) ** 2) / (3 * 2))) < 1e-15
+
+
+def _elastic_model(max_frequency=2.0, neutral_mass=0.0, options=None, kind=workloads.ELASTIC,
+                   neutral_vstd=0.0, particle_mass=1.0):
+    e, s = [0.0, 1.0, 2.0], [0.0, 0.5, 1.0]
+    return oracle_py.MccModel([(kind, 0.0, e, s)], 1.0, max_frequency, particle_mass, 64,
+                              neutral_mass, neutral_vstd, options=options)
+
+
+def test_henyey_greenstein_mean_cosine(oracle):
+    # test_core.cpp:397-440: mean cosine 0.8 table, single-collision events
+    # keep |v| and scatter with <cos> = 0.8 +- 0.04
+    mc = ([0.0, 2.0], [0.8, 0.8])
+    m = _elastic_model(options={0: (oracle_py.HENYEY_GREENSTEIN, 0, mc[0], mc[1])})
+    n = cos_sum = 0
+    for p in range(20000):
+        v, st = oracle.collide(m, [math.sqrt(2.0), 0.0, 0.0], 0.1, 161803, 0, p, 0)
+        assert abs(np.linalg.norm(v) - math.sqrt(2.0)) < 1e-14
+        if st.chan[0] != 1:
+            continue
+        cos_sum += v[0] / math.sqrt(2.0)
+        n += 1
+    assert n > 1200
+    assert abs(cos_sum / n - 0.8) < 0.04
+
+
+def test_center_of_mass_frame_selects_reduced_energy(oracle):
+    # test_core.cpp:530-549: equal masses, CoM energy 0.5 -> sigma 0.25
+    m = _elastic_model(neutral_mass=1.0, options={0: (0, oracle_py.FRAME_CENTER_OF_MASS,
+                                                         None, None)})
+    acc = 0
+    for p in range(10000):
+        _, st = oracle.collide(m, [math.sqrt(2.0), 0.0, 0.0], 0.1, 141421, 0, p, 0)
+        acc += st.chan[0]
+    assert 280 < acc < 440
+    m2 = _elastic_model(neutral_mass=1.0)
+    acc2 = sum(oracle.collide(m2, [math.sqrt(2.0), 0, 0], 0.1, 141421, 0, p, 0)[1].chan[0]
+               for p in range(10000))
+    assert acc2 > 1.6 * acc       # projectile frame: sigma 0.5
+
+
+def test_backward_equal_mass_exchanges_velocity(oracle):
+    # test_core.cpp:559-583: equal-mass backward scattering stops the projectile
+    m = _elastic_model(neutral_mass=1.0,
+                       options={0: (oracle_py.BACKWARD, oracle_py.FRAME_CENTER_OF_MASS, None,
+                                    None)})
+    seen = False
+    for p in range(10000):
+        v, st = oracle.collide(m, [math.sqrt(2.0), 0.0, 0.0], 0.1, 173205, 0, p, 0)
+        if st.chan[0] != 1:
+            continue
+        assert np.linalg.norm(v) < 1e-14
+        seen = True
+        break
+    assert seen
+
+
+def test_charge_exchange_makes_slow_ion(oracle):
+    # test_core.cpp:794-817 (cold neutral) and 818-852 (thermal neutral)
+    m = _elastic_model(kind=oracle_py.CHARGE_EXCHANGE, neutral_mass=40.0, particle_mass=40.0)
+    seen = False
+    for p in range(10000):
+        v, st = oracle.collide(m, [1.0, 0.0, 0.0], 0.01, 141421, 0, p, 0)
+        if st.chan[0] == 0:
+            continue
+        assert st.chan[0] == 1 and np.linalg.norm(v) == 0.0 and st.nsec == 0
+        seen = True
+        break
+    assert seen
+    mt = _elastic_model(kind=oracle_py.CHARGE_EXCHANGE, neutral_mass=1.0, max_frequency=20.0,
+                        neutral_vstd=1.0)
+    seen = False
+    for p in range(10000):
+        v, st = oracle.collide(mt, [2.0, 0.0, 0.0], 0.01, 173205, 0, p, 0)
+        if st.chan[0] == 0:
+            continue
+        assert 0.0 < np.linalg.norm(v) <= 8.0 and st.nsec == 0
+        seen = True
+        break
+    assert seen
+    unsafe = _elastic_model(kind=oracle_py.CHARGE_EXCHANGE, neutral_mass=1.0, max_frequency=1.0,
+                            neutral_vstd=1.0)
+    _, st = oracle.collide(unsafe, [2.0, 0.0, 0.0], 0.01, 1, 0, 0, 0)
+    assert st.status == 1
