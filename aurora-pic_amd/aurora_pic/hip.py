@@ -123,6 +123,7 @@ def load(path: str = LIB_PATH):
         "aurora_hip_boundary_losses": [ctx, C.c_int, _i64p, _dp],
         "aurora_hip_time": [ctx, _dp, _i64p],
         "aurora_hip_set_sort_interval": [ctx, C.c_int64],
+        "aurora_hip_set_sort_threshold": [ctx, C.c_double],
         "aurora_hip_sort": [ctx],
         "aurora_hip_set_tiling": [ctx, C.c_int, C.c_int, C.c_int, C.c_int],
         "aurora_hip_set_push_variant": [ctx, C.c_int, C.c_int],
@@ -369,6 +370,9 @@ class Context:
 
     def push_variant(self, block, prefetch):
         self._check(self.lib.aurora_hip_set_push_variant(self.h, block, prefetch))
+
+    def sort_threshold(self, fraction):
+        self._check(self.lib.aurora_hip_set_sort_threshold(self.h, fraction))
 
     def sort(self):
         self._check(self.lib.aurora_hip_sort(self.h))

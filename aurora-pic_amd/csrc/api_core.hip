@@ -56,6 +56,22 @@ void check_device_error(Context& ctx) {
     throw Failure{AURORA_ERR_HIP, "device reported error bits " + std::to_string(err)};
 }
 
+// A tile-sorted species is re-sorted only when enough of its pushes since
+// the last sort fell outside their tile's LDS box (those take the global
+// gather/atomic path); results do not depend on the order either way.
+bool sort_needed(Context& c, Species& sp) {
+    if (!sp.tiled || c.sort_threshold <= 0.0 || sp.pushes_since_sort == 0 || sp.slots == 0)
+        return true;
+    unsigned long long out = 0;
+    AUR_HIP(hipMemcpyAsync(&out, sp.absorbed.ptr + 6, sizeof out, hipMemcpyDeviceToHost,
+                           c.stream));
+    AUR_HIP(hipStreamSynchronize(c.stream));
+    const double frac = static_cast<double>(out) /
+                        (static_cast<double>(sp.pushes_since_sort) * static_cast<double>(sp.slots));
+    if (frac >= c.sort_threshold) return true;
+    return false;
+}
+
 }  // namespace aurora
 
 namespace {
@@ -343,8 +359,9 @@ int aurora_hip_species_add(aurora_hip_ctx* h, double charge, double mass, double
         }
         sp.acc.alloc(c.nnodes);
         AUR_HIP(hipMemset(sp.acc.ptr, 0, c.nnodes * sizeof(int64_t)));
-        sp.absorbed.alloc(6);
-        AUR_HIP(hipMemset(sp.absorbed.ptr, 0, 6 * sizeof(unsigned long long)));
+        // 6 face counters + pushes that left their tile's LDS box
+        sp.absorbed.alloc(7);
+        AUR_HIP(hipMemset(sp.absorbed.ptr, 0, 7 * sizeof(unsigned long long)));
         if (species_id) *species_id = static_cast<int>(c.species.size() - 1);
     });
 }
@@ -614,7 +631,8 @@ int aurora_hip_step(aurora_hip_ctx* h, double dt, int64_t nsteps) {
             c.time = t1;
             c.step_count += 1;
             if (c.sort_interval > 0 && c.step_count % c.sort_interval == 0)
-                for (auto& sp : c.species) sort_species(c, sp);
+                for (auto& sp : c.species)
+                    if (sort_needed(c, sp)) sort_species(c, sp);
         }
         check_device_error(c);
         flush_timers(c);
@@ -625,6 +643,14 @@ int aurora_hip_time(aurora_hip_ctx* h, double* time, int64_t* step) {
     return guarded(h, [&](Context& c) {
         if (time) *time = c.time;
         if (step) *step = c.step_count;
+    });
+}
+
+int aurora_hip_set_sort_threshold(aurora_hip_ctx* h, double fraction) {
+    return guarded(h, [&](Context& c) {
+        AUR_REQUIRE(std::isfinite(fraction) && fraction >= 0.0 && fraction <= 1.0,
+                    AURORA_ERR_INVALID_ARGUMENT, "sort threshold must be in [0, 1]");
+        c.sort_threshold = fraction;
     });
 }
 

@@ -104,6 +104,7 @@ struct Species {
     DevBuf<double> absorbed_energy;       // per-block partials (2 per block)
     DevBuf<int32_t> tile_start;           // ntiles + 1 segment starts after a sort
     bool tiled = false;                   // storage is tile-sorted (tile_start valid)
+    int64_t pushes_since_sort = 0;        // tiled pushes counted in absorbed[6]
     int64_t absorbed_host[6] = {0, 0, 0, 0, 0, 0};
     double absorbed_energy_host[2] = {0, 0};
     size_t elem() const { return precision == 64 ? 8 : 4; }
@@ -203,6 +204,7 @@ struct Context {
 
     // sorting
     int64_t sort_interval = 0;
+    double sort_threshold = 0.02;  // re-sort when this fraction of pushes left the tile box
     int tile[3] = {8, 8, 8};
     int halo = 2;
     int push_block = 512;        // tiled push workgroup size
@@ -286,6 +288,7 @@ void deposit_all(Context& ctx);
 void convert_rho(Context& ctx);
 void push_all(Context& ctx, double dt, bool fuse_deposit);
 void sort_species(Context& ctx, Species& sp);
+bool sort_needed(Context& ctx, Species& sp);
 void mcc_step(Context& ctx, double dt);
 void init_half_all(Context& ctx, double dt);
 void set_b_profile_device(Context& ctx);

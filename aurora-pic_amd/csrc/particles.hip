@@ -66,7 +66,7 @@ struct PushArgs {
     R qm_dt;               // qm * dt (leapfrog)
     R dtd0, dtd1, dtd2;    // dt / d per axis
     long long* acc;        // deposit target (fused)
-    unsigned long long* absorbed;  // 6 face counters
+    unsigned long long* absorbed;  // 6 face counters, then pushes outside the tile box
     double* absorbed_energy;       // 2 sides (1D), atomics
     double half_mw;        // 0.5 * m * w (1D loss ledger)
     uint32_t* err;
@@ -634,6 +634,7 @@ __global__ void __launch_bounds__(BLOCK) k_push_tile3d(PushArgs<float> a,
     constexpr int BN = BX * BY * BZ;
     __shared__ float4 sE[BN];
     __shared__ unsigned long long sQ[BN];
+    __shared__ unsigned outside_s;
     const MeshView& m = a.m;
     const SpeciesView<float>& sp = a.sp;
     const int t = blockIdx.x;
@@ -649,7 +650,9 @@ __global__ void __launch_bounds__(BLOCK) k_push_tile3d(PushArgs<float> a,
         sE[q] = E4[(gz * m.ny + gy) * m.nx + gx];
         sQ[q] = 0ull;
     }
+    if (threadIdx.x == 0) outside_s = 0u;
     __syncthreads();
+    unsigned outside = 0;
     struct State {
         int c;
         float o0, o1, o2, v0, v1, v2;
@@ -731,9 +734,13 @@ __global__ void __launch_bounds__(BLOCK) k_push_tile3d(PushArgs<float> a,
                                       quantise(wx[aa] * wy[b] * wz[cc])));
         } else {
             deposit_one<3>(a.acc, m, idx, o);
+            ++outside;
         }
     }
+    if (outside) atomicAdd(&outside_s, outside);
     __syncthreads();
+    if (threadIdx.x == 0 && outside_s)
+        atomicAdd(a.absorbed + 6, static_cast<unsigned long long>(outside_s));
     for (int q = threadIdx.x; q < BN; q += BLOCK) {
         const unsigned long long val = sQ[q];
         if (val == 0ull) continue;
@@ -957,8 +964,10 @@ void push_all(Context& c, double dt, bool fuse_deposit) {
             AUR_HIP(hipMemset(sp.absorbed_energy.ptr, 0, 2 * sizeof(double)));
         }
         if (fuse_deposit && sp.tiled && sp.precision == 32 && c.tiled_push_ok() &&
-            launch_tile3d(c, sp, dt))
+            launch_tile3d(c, sp, dt)) {
+            ++sp.pushes_since_sort;
             continue;
+        }
         dispatch_dim_prec(c, sp, [&]<int DIM, typename R>() {
             launch_push<DIM, R>(c, sp, dt, fuse_deposit);
         });

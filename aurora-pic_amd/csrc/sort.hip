@@ -151,6 +151,8 @@ void sort_species(Context& c, Species& sp) {
     AUR_HIP(hipStreamSynchronize(c.stream));
     sp.slots = live;
     sp.tiled = true;
+    sp.pushes_since_sort = 0;
+    AUR_HIP(hipMemsetAsync(sp.absorbed.ptr + 6, 0, sizeof(unsigned long long), c.stream));
 }
 
 }  // namespace aurora

@@ -208,6 +208,11 @@ int aurora_hip_set_tiling(aurora_hip_ctx* ctx, int tx, int ty, int tz,
 int aurora_hip_set_push_variant(aurora_hip_ctx* ctx, int block, int prefetch);
 /* Cell sort every interval steps (0 = never) and an immediate sort. */
 int aurora_hip_set_sort_interval(aurora_hip_ctx* ctx, int64_t interval);
+/* At a sort step, a tile-sorted species is re-sorted only if at least this
+ * fraction of its particle pushes since its last sort ended outside their
+ * tile's LDS box (default 0.02; 0 = always sort).  Sorting never changes
+ * results (integer deposit), only speed. */
+int aurora_hip_set_sort_threshold(aurora_hip_ctx* ctx, double fraction);
 int aurora_hip_sort(aurora_hip_ctx* ctx);
 
 /* ------------------------------------------------------------------ MCC */

@@ -263,7 +263,9 @@ def test_tiled_push_bit_identical_to_untiled(shape):
     pos = rng.uniform(0, 1, (N, 3))
     vel = rng.normal(0, 0.5, (N, 3))   # fast: many particles leave the halo
     results = []
-    for tiled in (True, False):
+    # tiled re-sorting every 7 steps, tiled never re-sorting (threshold 1:
+    # particles stay where the first sort put them), untiled
+    for tiled, threshold in ((True, 0.0), (True, 1.0), (False, 0.0)):
         with hip.Context(0) as c:
             c.mesh(n, L, 0, 1.0)
             c.b_profile(0, x, bvals)
@@ -272,6 +274,7 @@ def test_tiled_push_bit_identical_to_untiled(shape):
                 c.push_variant(*shape[4:])
             else:
                 c.tiling(0)
+            c.sort_threshold(threshold)
             sid = c.add_species(-1.0, 1.0, 1.0 / N, N)
             c.upload(sid, pos, vel, kind=1)
             c.sort()
@@ -281,9 +284,10 @@ def test_tiled_push_bit_identical_to_untiled(shape):
             p_, v_, a_, u_ = c.download(sid)
             o = np.argsort(u_)
             results.append((c.charge_fixed(sid), p_[o], v_[o]))
-    assert np.array_equal(results[0][0], results[1][0])
-    assert np.array_equal(results[0][1], results[1][1])
-    assert np.array_equal(results[0][2], results[1][2])
+    for other in results[1:]:
+        assert np.array_equal(results[0][0], other[0])
+        assert np.array_equal(results[0][1], other[1])
+        assert np.array_equal(results[0][2], other[2])
 
 
 def test_full_size_3d_invariants(hip_ctx):
