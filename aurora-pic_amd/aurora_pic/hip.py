@@ -136,7 +136,7 @@ def load(path: str = LIB_PATH):
         "aurora_hip_mcc_collide": [ctx, C.c_int64, _dp, C.c_double, C.c_uint64, C.c_uint32,
                                    _u64p, _dp],
         "aurora_hip_swarm_gas": [ctx, C.c_int, _ip, _dp, _i64p, C.POINTER(_dp), C.POINTER(_dp),
-                                 C.c_double, C.c_double, C.c_int64, C.c_double],
+                                 C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double],
         "aurora_hip_swarm_submit": [ctx, C.POINTER(SwarmRequest), C.c_uint64, _i64p],
         "aurora_hip_swarm_advance": [ctx, C.c_int64, _i64p],
         "aurora_hip_swarm_poll": [ctx, C.c_int64, _ip, C.POINTER(SwarmResult)],
@@ -440,7 +440,8 @@ class Context:
         nch, kinds, thr, rows, ep, sp = self._tables(gas["channels"])
         self._check(self.lib.aurora_hip_swarm_gas(
             self.h, nch, kinds, _p(thr, _dp), _p(rows, _i64p), ep, sp, gas["neutral_density"],
-            gas["max_frequency"], gas.get("max_candidates", 64), gas["neutral_mass"]))
+            gas["max_frequency"], gas.get("max_candidates", 64), gas["neutral_mass"],
+            gas.get("neutral_temperature", 0.0)))
         self._nchan = nch
 
     def swarm_submit(self, request: SwarmRequest, hbm_budget=0):

@@ -67,14 +67,6 @@ def synthetic_gas(rows: int = 1000, neutral_density: float = 1.0e21):
                 max_candidates=64)
 
 
-def synthetic_mcc_model(rows: int = 1000, neutral_density: float = 1.0e21):
-    """Oracle-side model object for the synthetic gas (tests only)."""
-    import oracle_py
-    g = synthetic_gas(rows, neutral_density)
-    return oracle_py.MccModel(g["channels"], g["neutral_density"], g["max_frequency"],
-                              g["particle_mass"], g["max_candidates"], g["neutral_mass"])
-
-
 # ------------------------------------------------------------ B profile
 def gaussian_by_profile(length_x: float, rows: int = 512, peak: float = 0.02,
                         width_fraction: float = 0.125):

@@ -345,11 +345,15 @@ extern "C" {
 int aurora_hip_swarm_gas(aurora_hip_ctx* h, int nchan, const int* kind, const double* threshold,
                          const int64_t* rows, const double* const* energy,
                          const double* const* sigma, double neutral_density,
-                         double max_frequency, int64_t max_candidates, double neutral_mass) {
+                         double max_frequency, int64_t max_candidates, double neutral_mass,
+                         double neutral_temperature) {
     return guarded(h, [&](Context& c) {
         SwarmState& s = state(c);
         AUR_REQUIRE(s.active.empty(), AURORA_ERR_INVALID_ARGUMENT,
                     "the swarm gas cannot change while requests are running");
+        AUR_REQUIRE(std::isfinite(neutral_temperature) && neutral_temperature >= 0.0,
+                    AURORA_ERR_INVALID_ARGUMENT,
+                    "swarm neutral_temperature must be finite and non-negative");
         AUR_REQUIRE(std::isfinite(neutral_mass) && neutral_mass > 0.0,
                     AURORA_ERR_INVALID_ARGUMENT, "gas dataset requires positive neutral_mass");
         bool elastic = false;
@@ -364,7 +368,8 @@ int aurora_hip_swarm_gas(aurora_hip_ctx* h, int nchan, const int* kind, const do
         // model checks and tables (the 1D MCC model loader's)
         MccModelHost& g = s.gas;
         load_mcc_model(g, nchan, kind, threshold, rows, energy, sigma, neutral_density,
-                       max_frequency, max_candidates, kElectronMass, neutral_mass, 0.0);
+                       max_frequency, max_candidates, kElectronMass, neutral_mass,
+                       neutral_temperature);
         // energy ceiling of the tables (Swarm.cpp:517-526), eV
         double emax_ev = 1e300;
         for (int k = 0; k < nchan; ++k)

@@ -683,7 +683,7 @@ SwarmService::SwarmService(const CollisionConfig& gas, std::shared_ptr<Device> d
                                      t.threshold.data(), t.rows.data(), t.energy.data(),
                                      t.sigma.data(), gas.neutral_density, gas.max_frequency,
                                      static_cast<int64_t>(gas.max_candidates_per_particle),
-                                     gas.neutral_mass));
+                                     gas.neutral_mass, gas.neutral_temperature));
     apply_options(*dev_, 1, gas);
 }
 

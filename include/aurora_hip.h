@@ -291,13 +291,15 @@ typedef struct aurora_hip_swarm_result {
 } aurora_hip_swarm_result;
 
 /* The gas of every swarm request on this context (GasDataset +
- * collision_config, Swarm.cpp:561-593). */
+ * collision_config, Swarm.cpp:561-593). neutral_temperature (K, >= 0)
+ * is SwarmBenchmarkConfig::neutral_temperature (Swarm.cpp:451-460);
+ * a positive value samples Maxwellian neutral velocities. */
 int aurora_hip_swarm_gas(aurora_hip_ctx* ctx, int nchan, const int* kind,
                          const double* threshold, const int64_t* rows,
                          const double* const* energy,
                          const double* const* sigma, double neutral_density,
                          double max_frequency, int64_t max_candidates,
-                         double neutral_mass);
+                         double neutral_mass, double neutral_temperature);
 /* Admits a request when its device footprint fits the HBM budget
  * (bytes; 0 = no limit); returns AURORA_ERR_CAPACITY otherwise. */
 int aurora_hip_swarm_submit(aurora_hip_ctx* ctx,

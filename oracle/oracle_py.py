@@ -171,6 +171,16 @@ class MccModel:
         self.neutral_vstd = neutral_vstd
 
 
+def model_from_gas(gas: dict, particle_mass: float = None, neutral_vstd: float = 0.0,
+                   options=None) -> MccModel:
+    """The oracle model for a gas description in the device loaders' form
+    (channels, neutral_density, max_frequency, neutral_mass[, particle_mass,
+    max_candidates])."""
+    m = gas.get("particle_mass", 9.1093837139e-31) if particle_mass is None else particle_mass
+    return MccModel(gas["channels"], gas["neutral_density"], gas["max_frequency"], m,
+                    gas.get("max_candidates", 64), gas["neutral_mass"], neutral_vstd, options)
+
+
 class Oracle:
     def __init__(self, path: str = ORACLE_SO):
         if not os.path.exists(path):
@@ -431,6 +441,7 @@ class Reference:
                                 C.c_double, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
                                 C.c_double, C.c_double, C.c_double, C.c_uint64, _dp,
                                 C.c_char_p, C.c_int]
+        L.ref_swarm_file.argtypes = [C.c_char_p, C.c_int, _dp, C.c_char_p, C.c_int]
         L.ref_sim3d_time.argtypes = [_i64p, _dp, C.c_double, C.c_double, C.c_int, C.c_int,
                                      _dp, C.c_int64, _dp, _dp, C.c_int, _dp, _i64p,
                                      C.c_char_p, C.c_int]
@@ -529,6 +540,22 @@ class Reference:
         if rc:
             raise RuntimeError(err.value.decode())
         return out
+
+    SWARM_FILE_COLS = ("reduced_field_td", "drift", "drift_se", "reduced_mobility",
+                       "mean_energy_ev", "mean_energy_se", "ionization_rate",
+                       "ionization_rate_se", "max_energy_ev", "candidates", "nulls",
+                       "neutral_vstd", "collisions0", "collisions1", "collisions2",
+                       "collisions3")
+
+    def swarm_file(self, path, max_fields=64):
+        """A .swarm file through the reference's own loader and driver; one
+        row of SWARM_FILE_COLS per reduced field."""
+        out = np.zeros((max_fields, len(self.SWARM_FILE_COLS)))
+        err = C.create_string_buffer(512)
+        n = self.lib.ref_swarm_file(os.fsencode(path), max_fields, _ptr(out, _dp), err, 512)
+        if n < 0:
+            raise RuntimeError(err.value.decode())
+        return out[:n]
 
     def sim3d_time(self, n, L, dt, eps_rel, si, species, btable, steps):
         n, L = i64(n), f64(L)

@@ -352,6 +352,47 @@ int ref_swarm(int nchan, const int* kind, const double* threshold,
     }
 }
 
+// A .swarm benchmark file through the reference's own loader and driver
+// (load_swarm_benchmark_config, Swarm.cpp:1252; run_swarm_benchmark,
+// Swarm.cpp:1513).  The CSV goes to a temporary directory.  Per field, out
+// receives kSwarmFileCols values: reduced field, drift, drift se, mobility,
+// mean energy, its se, ionization rate, its se, max observed energy,
+// candidates, nulls, neutral velocity stddev, then collisions per channel
+// (up to 4, zero-filled).  Returns the number of fields written.
+int ref_swarm_file(const char* swarm_path, int max_fields, double* out, char* err,
+                   int errlen) {
+    constexpr int kCols = 16;
+    try {
+        auto cfg = pic::load_swarm_benchmark_config(swarm_path);
+        cfg.output_file = temp_dir() / "swarm_file.csv";
+        auto results = pic::run_swarm_benchmark(cfg);
+        const int n = static_cast<int>(std::min<size_t>(results.size(), max_fields));
+        for (int f = 0; f < n; ++f) {
+            const auto& r = results[f];
+            double* o = out + static_cast<size_t>(f) * kCols;
+            std::fill(o, o + kCols, 0.0);
+            o[0] = r.reduced_field_td;
+            o[1] = r.electron_drift_velocity_m_s;
+            o[2] = r.mean_velocity_x_standard_error_m_s;
+            o[3] = r.reduced_mobility_1_v_m_s;
+            o[4] = r.mean_energy_ev;
+            o[5] = r.mean_energy_standard_error_ev;
+            o[6] = r.ionization_rate_s;
+            o[7] = r.ionization_rate_standard_error_s;
+            o[8] = r.maximum_observed_energy_ev;
+            o[9] = static_cast<double>(r.collision_candidates);
+            o[10] = static_cast<double>(r.null_collisions);
+            o[11] = r.neutral_velocity_stddev_m_s;
+            for (size_t c = 0; c < r.channels.size() && c < 4; ++c)
+                o[12 + c] = static_cast<double>(r.channels[c].collisions);
+        }
+        return n;
+    } catch (const std::exception& e) {
+        set_error(err, errlen, e.what());
+        return -1;
+    }
+}
+
 // Times the reference's own Simulation3D::step() on a periodic mesh with
 // species initialised by the reference's initializer.  species: nspecies
 // rows of (charge, mass, weight, particles, thermal_velocity).  B profile

@@ -106,7 +106,7 @@ def main():
         save(f"xs_{c}.npz", e=e, s=s, q=qs, out=ref.xs_eval(e, s, qs))
 
     # MCC statistics on the synthetic tables: the reference's own stream
-    model = workloads.synthetic_mcc_model(rows=200)
+    model = oracle_py.model_from_gas(workloads.synthetic_gas(rows=200))
     n = 6000
     speeds = np.sqrt(2 * np.geomspace(1.0, 400.0, n) * workloads.EV / workloads.ME)
     dirs = rng.normal(size=(n, 3))

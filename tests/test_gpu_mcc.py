@@ -40,7 +40,7 @@ def _load(ctx, channels, density, max_frequency, mass, max_candidates=64, neutra
 def test_collide_matches_oracle_same_stream(ctx, oracle):
     g = golden("mcc_stats.npz")
     gas = workloads.synthetic_gas(rows=200)
-    model = workloads.synthetic_mcc_model(rows=200)
+    model = oracle_py.model_from_gas(workloads.synthetic_gas(rows=200))
     _load(ctx, gas["channels"], gas["neutral_density"], gas["max_frequency"], workloads.ME,
           neutral_mass=gas["neutral_mass"], seed=8128)
     vin = g["vin"].astype(np.float64)
@@ -138,7 +138,7 @@ def test_step_1d_with_mcc_matches_oracle(oracle):
     dt = 1.0 / (400 * 13.56e6)
     N = 3000
     gas = workloads.synthetic_gas(rows=1000, neutral_density=1.0e22)
-    model = workloads.synthetic_mcc_model(rows=1000, neutral_density=1.0e22)
+    model = oracle_py.model_from_gas(workloads.synthetic_gas(rows=1000, neutral_density=1.0e22))
     species = [(-workloads.EV, workloads.ME, 8.0), (workloads.EV, 6.67e-27, 0.026)]
     w = [2.56e14 * L / N] * 2
     state = []

@@ -74,7 +74,7 @@ def _initial_velocities(oracle, n, e0, seed):
 
 def test_swarm_matches_oracle_same_stream(ctx, oracle):
     gas = workloads.synthetic_gas(rows=200)
-    model = workloads.synthetic_mcc_model(rows=200)
+    model = oracle_py.model_from_gas(workloads.synthetic_gas(rows=200))
     ctx.swarm_gas(gas)
     dt = 0.099 / gas["max_frequency"]
     n, steps, burn, blocks, seed = 512, 1200, 400, 8, 4242
@@ -148,3 +148,22 @@ def test_swarm_admission_budget(ctx):
     ctx.swarm_submit(r, hbm_budget=fp + 10)    # memory freed: admitted now
     with pytest.raises(hip.HipError, match="max_frequency \\* timestep"):
         ctx.swarm_submit(_request(10.0, 100, 100, 10, 5, 1.0, 1))
+
+
+def test_reference_swarm_example_through_readers(ctx):
+    """The reference's examples/synthetic_swarm.swarm read by the gas and
+    swarm readers and run on the device (300 K neutrals, Henyey-Greenstein
+    elastic channel): drift, mean energy, elastic collisions and candidates
+    per field within 4 sigma of the reference's own runs, sigma being the
+    reference's spread over 16 seeds."""
+    import os
+    from aurora_pic import gasdata
+    from test_gasdata import SWARM_DIR, check_against_reference
+    cfg = gasdata.load_swarm_config(os.path.join(SWARM_DIR, "synthetic_swarm.swarm"))
+    ds = gasdata.load_gas_dataset(cfg.gas_data_file)
+    ctx.swarm_gas(gasdata.swarm_gas(ds, cfg))
+    gasdata.apply_channel_options(ctx, ds, model=1)
+    results = _run_all(ctx, gasdata.swarm_requests(cfg), 500)
+    assert all(r.status == 0 for r in results)
+    check_against_reference([(r.drift, r.mean_energy_ev, r.collisions[0], r.candidates)
+                             for r in results])

@@ -123,7 +123,7 @@ def test_mcc_statistics_match_reference_golden(oracle):
     """Same inputs as the reference's collide() run on the synthetic gas:
     per-channel collision counts agree within 5 Poisson sigma."""
     g = golden("mcc_stats.npz")
-    model = workloads.synthetic_mcc_model(rows=200)
+    model = oracle_py.model_from_gas(workloads.synthetic_gas(rows=200))
     vin = g["vin"].astype(np.float64)
     dt = float(g["dt"])
     counts = np.zeros(2 + 3 + 1, dtype=np.int64)
@@ -161,7 +161,7 @@ def test_swarm_zero_cross_section_closed_form(oracle):
 
 def test_swarm_matches_reference_golden_statistically(oracle):
     g = golden("swarm_300td.npz")
-    model = workloads.synthetic_mcc_model(rows=200)
+    model = oracle_py.model_from_gas(workloads.synthetic_gas(rows=200))
     cfg = {k: (float(g[k]) if k not in ("steps", "burn_in", "particles", "blocks")
                else int(g[k])) for k in ["neutral_density", "max_frequency", "timestep",
                                           "reduced_field_td", "steps", "burn_in", "particles",
