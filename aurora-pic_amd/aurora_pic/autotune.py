@@ -1,4 +1,5 @@
-"""Autotuner for the LDS-tiled 3D push: tile shape, halo and sort interval.
+"""Autotuner for the LDS-tiled 3D push: tile shape, halo, workgroup variant
+and (under the fixed sort policy) the sort interval.
 
 Every candidate computes bit-identical physics (the tiled and untiled pushes
 perform the same operations and the deposit is integer), so the choice is
@@ -13,23 +14,30 @@ import time
 SHAPES = [(8, 8, 8, 2, 512, 0), (8, 8, 8, 1, 512, 0), (8, 8, 4, 2, 512, 0),
           (16, 8, 4, 2, 512, 0), (4, 4, 4, 2, 512, 0), (8, 8, 4, 2, 512, 1),
           (8, 8, 4, 2, 256, 1), (8, 8, 4, 2, 256, 0), (8, 8, 8, 2, 512, 1),
-          (8, 8, 8, 2, 256, 1)]
+          (8, 8, 8, 2, 256, 1), (8, 8, 4, 2, 512, 3), (8, 8, 4, 1, 512, 3)]
 
 
-def tune(ctx, dt, shapes=SHAPES, sort_intervals=(20,), steps=20, log=None):
-    """Time each (shape, interval) over `steps` steps (plus one warm step
-    after the re-sort) and leave the fastest configured.  Returns
-    (best, table) with table rows (ms_per_step, shape, interval)."""
+def tune(ctx, dt, shapes=SHAPES, sort_intervals=(20,), steps=20, log=None, adaptive=False):
+    """Time each candidate over `steps` steps (plus one warm step after the
+    re-sort) and leave the fastest configured.  Under the adaptive sort
+    policy (adaptive=True) the interval is not tuned: the window should span
+    at least one sort cycle.  Returns (best, table) with table rows
+    (ms_per_step, shape, interval or None)."""
+    from . import hip
     table = []
+    intervals = (None,) if adaptive else sort_intervals
+    if adaptive:
+        ctx.sort_policy(hip.SORT_ADAPTIVE)
     for shape in shapes:
         try:
             ctx.tiling(*shape[:4])
             ctx.push_variant(*(shape[4:] or (512, 0)))
         except Exception:  # shape not applicable to this mesh
             continue
-        for interval in sort_intervals:
+        for interval in intervals:
             ctx.sort()
-            ctx.sort_interval(interval)
+            if interval is not None:
+                ctx.sort_interval(interval)
             ctx.step(dt, 1)
             ctx.sync()
             t0 = time.perf_counter()
@@ -38,10 +46,12 @@ def tune(ctx, dt, shapes=SHAPES, sort_intervals=(20,), steps=20, log=None):
             ms = 1e3 * (time.perf_counter() - t0) / steps
             table.append((ms, shape, interval))
             if log:
-                log(f"autotune {shape} sort every {interval}: {ms:.3f} ms/step")
-    best = min(table)
+                what = "adaptive sort" if interval is None else f"sort every {interval}"
+                log(f"autotune {shape} {what}: {ms:.3f} ms/step")
+    best = min(table, key=lambda r: r[0])
     ctx.tiling(*best[1][:4])
     ctx.push_variant(*(best[1][4:] or (512, 0)))
     ctx.sort()
-    ctx.sort_interval(best[2])
+    if best[2] is not None:
+        ctx.sort_interval(best[2])
     return best, table

@@ -21,6 +21,7 @@ VELOCITY_HALF, VELOCITY_CENTRED = 0, 1
 ELASTIC, EXCITATION, IONIZATION, ATTACHMENT, CHARGE_EXCHANGE = range(5)
 SCATTER_ISOTROPIC, SCATTER_BACKWARD, SCATTER_HENYEY_GREENSTEIN = range(3)
 FRAME_PROJECTILE, FRAME_CENTER_OF_MASS = range(2)
+SORT_FIXED, SORT_ADAPTIVE = range(2)
 (OK, ERR_INVALID_ARGUMENT, ERR_HIP, ERR_RATE_ABOVE_BOUND, ERR_CANDIDATE_LIMIT, ERR_CAPACITY,
  ERR_B_OUT_OF_RANGE, ERR_ENERGY_ABOVE_MAX, ERR_UNSUPPORTED, ERR_NOT_FOUND) = range(10)
 
@@ -124,6 +125,7 @@ def load(path: str = LIB_PATH):
         "aurora_hip_time": [ctx, _dp, _i64p],
         "aurora_hip_set_sort_interval": [ctx, C.c_int64],
         "aurora_hip_set_sort_threshold": [ctx, C.c_double],
+        "aurora_hip_set_sort_policy": [ctx, C.c_int, C.c_int64],
         "aurora_hip_sort": [ctx],
         "aurora_hip_set_tiling": [ctx, C.c_int, C.c_int, C.c_int, C.c_int],
         "aurora_hip_set_push_variant": [ctx, C.c_int, C.c_int],
@@ -373,6 +375,11 @@ class Context:
 
     def sort_threshold(self, fraction):
         self._check(self.lib.aurora_hip_set_sort_threshold(self.h, fraction))
+
+    def sort_policy(self, policy, max_interval=0):
+        """SORT_FIXED (every sort_interval steps) or SORT_ADAPTIVE (per
+        species, when its timed push exceeds its cycle's average cost)."""
+        self._check(self.lib.aurora_hip_set_sort_policy(self.h, policy, max_interval))
 
     def sort(self):
         self._check(self.lib.aurora_hip_sort(self.h))

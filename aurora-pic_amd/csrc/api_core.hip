@@ -72,6 +72,28 @@ bool sort_needed(Context& c, Species& sp) {
     return false;
 }
 
+// The renewal rule: with a cycle cost C(n) = sort + pushes 1..n and pushes
+// that slow down as the cycle ages, the long-run cost per step C(n)/n is
+// least at the first n whose push exceeds C(n)/n.  The push of this step
+// is the mean of the last three (timing noise); its end event is waited
+// for, while the field solve queued behind it keeps the device busy.
+bool adaptive_sort_due(Context& c, Species& sp) {
+    if (sp.precision != 32 || !c.tiled_push_ok()) return false;  // no tiled push to speed up
+    if (!sp.tiled) return sp.slots > 0;
+    if (sp.sort_ev.recorded) sp.cycle_ms += sp.sort_ev.take_ms();
+    if (!sp.push_ev.recorded) return false;
+    const double p = sp.push_ev.take_ms();
+    sp.cycle_ms += p;
+    sp.recent_ms[2] = sp.recent_ms[1];
+    sp.recent_ms[1] = sp.recent_ms[0];
+    sp.recent_ms[0] = p;
+    const int64_t n = sp.pushes_since_sort;
+    if (c.sort_max_interval > 0 && n >= c.sort_max_interval) return true;
+    if (n < 3) return false;
+    const double recent = (sp.recent_ms[0] + sp.recent_ms[1] + sp.recent_ms[2]) / 3.0;
+    return recent * static_cast<double>(n) > sp.cycle_ms;
+}
+
 }  // namespace aurora
 
 namespace {
@@ -630,9 +652,13 @@ int aurora_hip_step(aurora_hip_ctx* h, double dt, int64_t nsteps) {
             if (c.mcc.loaded) mcc_step(c, dt);
             c.time = t1;
             c.step_count += 1;
-            if (c.sort_interval > 0 && c.step_count % c.sort_interval == 0)
+            if (c.sort_policy == AURORA_SORT_ADAPTIVE) {
+                for (auto& sp : c.species)
+                    if (adaptive_sort_due(c, sp)) sort_species(c, sp);
+            } else if (c.sort_interval > 0 && c.step_count % c.sort_interval == 0) {
                 for (auto& sp : c.species)
                     if (sort_needed(c, sp)) sort_species(c, sp);
+            }
         }
         check_device_error(c);
         flush_timers(c);
@@ -654,6 +680,17 @@ int aurora_hip_set_sort_threshold(aurora_hip_ctx* h, double fraction) {
     });
 }
 
+int aurora_hip_set_sort_policy(aurora_hip_ctx* h, int policy, int64_t max_interval) {
+    return guarded(h, [&](Context& c) {
+        AUR_REQUIRE(policy == AURORA_SORT_FIXED || policy == AURORA_SORT_ADAPTIVE,
+                    AURORA_ERR_INVALID_ARGUMENT, "sort policy must be fixed or adaptive");
+        AUR_REQUIRE(max_interval >= 0, AURORA_ERR_INVALID_ARGUMENT,
+                    "sort max_interval must be non-negative");
+        c.sort_policy = policy;
+        c.sort_max_interval = max_interval;
+    });
+}
+
 int aurora_hip_set_sort_interval(aurora_hip_ctx* h, int64_t interval) {
     return guarded(h, [&](Context& c) {
         AUR_REQUIRE(interval >= 0, AURORA_ERR_INVALID_ARGUMENT, "sort interval must be >= 0");
@@ -667,7 +704,7 @@ int aurora_hip_set_tiling(aurora_hip_ctx* h, int tx, int ty, int tz, int halo) {
             c.tiled_enabled = false;
             return;
         }
-        static const int shapes[][4] = {{8, 8, 8, 2}, {8, 8, 8, 1}, {8, 8, 4, 2}, {4, 4, 4, 2},
+        static const int shapes[][4] = {{8, 8, 8, 2}, {8, 8, 8, 1}, {8, 8, 4, 2}, {8, 8, 4, 1}, {4, 4, 4, 2},
                                         {16, 8, 4, 2}};
         bool known = false;
         for (const auto& s : shapes)
@@ -684,7 +721,7 @@ int aurora_hip_set_tiling(aurora_hip_ctx* h, int tx, int ty, int tz, int halo) {
 
 int aurora_hip_set_push_variant(aurora_hip_ctx* h, int block, int prefetch) {
     return guarded(h, [&](Context& c) {
-        AUR_REQUIRE((block == 256 || block == 512) && (prefetch == 0 || prefetch == 1),
+        AUR_REQUIRE((block == 256 || block == 512) && prefetch >= 0 && prefetch <= 3,
                     AURORA_ERR_UNSUPPORTED, "push variant is not built into this library");
         c.push_block = block;
         c.push_prefetch = prefetch;

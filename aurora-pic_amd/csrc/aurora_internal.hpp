@@ -83,6 +83,55 @@ struct DevBuf {
     }
 };
 
+// A start/end event pair for timing one launch on the context stream (the
+// adaptive sort policy's measurements); created on first use.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    bool recorded = false;
+    void record_start(hipStream_t s) {
+        if (!a) {
+            AUR_HIP(hipEventCreate(&a));
+            AUR_HIP(hipEventCreate(&b));
+        }
+        AUR_HIP(hipEventRecord(a, s));
+    }
+    void record_end(hipStream_t s) {
+        AUR_HIP(hipEventRecord(b, s));
+        recorded = true;
+    }
+    // waits for the end event; milliseconds between the two
+    double take_ms() {
+        float ms = 0.f;
+        AUR_HIP(hipEventSynchronize(b));
+        AUR_HIP(hipEventElapsedTime(&ms, a, b));
+        recorded = false;
+        return ms;
+    }
+    void release() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        a = b = nullptr;
+        recorded = false;
+    }
+    EventPair() = default;
+    ~EventPair() { release(); }
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+    EventPair(EventPair&& o) noexcept : a(o.a), b(o.b), recorded(o.recorded) {
+        o.a = o.b = nullptr;
+        o.recorded = false;
+    }
+    EventPair& operator=(EventPair&& o) noexcept {
+        if (this != &o) {
+            release();
+            a = o.a; b = o.b; recorded = o.recorded;
+            o.a = o.b = nullptr;
+            o.recorded = false;
+        }
+        return *this;
+    }
+};
+
 // ------------------------------------------------------------- state
 struct Species {
     double charge = 0, mass = 1, weight = 1;
@@ -105,6 +154,10 @@ struct Species {
     DevBuf<int32_t> tile_start;           // ntiles + 1 segment starts after a sort
     bool tiled = false;                   // storage is tile-sorted (tile_start valid)
     int64_t pushes_since_sort = 0;        // tiled pushes counted in absorbed[6]
+    // adaptive sort policy: measured cost of the current sort cycle
+    EventPair push_ev, sort_ev;
+    double cycle_ms = 0.0;                // last sort + tiled pushes since, ms
+    double recent_ms[3] = {0, 0, 0};      // the last three tiled pushes
     int64_t absorbed_host[6] = {0, 0, 0, 0, 0, 0};
     double absorbed_energy_host[2] = {0, 0};
     size_t elem() const { return precision == 64 ? 8 : 4; }
@@ -204,7 +257,9 @@ struct Context {
 
     // sorting
     int64_t sort_interval = 0;
-    double sort_threshold = 0.02;  // re-sort when this fraction of pushes left the tile box
+    double sort_threshold = 0.0;   // fixed policy: skip a species whose pushes left its tile box less often
+    int sort_policy = AURORA_SORT_FIXED;
+    int64_t sort_max_interval = 0; // adaptive policy: cap on steps between sorts (0 = none)
     int tile[3] = {8, 8, 8};
     int halo = 2;
     int push_block = 512;        // tiled push workgroup size
@@ -289,6 +344,8 @@ void convert_rho(Context& ctx);
 void push_all(Context& ctx, double dt, bool fuse_deposit);
 void sort_species(Context& ctx, Species& sp);
 bool sort_needed(Context& ctx, Species& sp);
+// adaptive policy: the species' push now costs more than its cycle average
+bool adaptive_sort_due(Context& ctx, Species& sp);
 void mcc_step(Context& ctx, double dt);
 void init_half_all(Context& ctx, double dt);
 void set_b_profile_device(Context& ctx);

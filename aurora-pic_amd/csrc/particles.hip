@@ -624,10 +624,12 @@ __device__ __forceinline__ int rel_index(int i, int o, int n) {
     return r;
 }
 
-// BLOCK threads per workgroup; PF = 1 loads the next particle's state
-// while the current one is processed (more memory-level parallelism).
+// BLOCK threads per workgroup; PF bit 0 loads the next particle's state
+// while the current one is processed (more memory-level parallelism); PF
+// bit 1 asks the compiler to fit 8 waves per SIMD (<= 64 VGPRs), so that
+// four 512-thread workgroups share a CU.
 template <int TX, int TY, int TZ, int H, int BMODE, int BLOCK, int PF>
-__global__ void __launch_bounds__(BLOCK) k_push_tile3d(PushArgs<float> a,
+__global__ void __launch_bounds__(BLOCK, (PF & 2) ? 8 : 1) k_push_tile3d(PushArgs<float> a,
                                                        const int32_t* __restrict__ tile_start,
                                                        int ntx, int nty) {
     constexpr int BX = TX + 1 + 2 * H, BY = TY + 1 + 2 * H, BZ = TZ + 1 + 2 * H;
@@ -635,6 +637,12 @@ __global__ void __launch_bounds__(BLOCK) k_push_tile3d(PushArgs<float> a,
     __shared__ float4 sE[BN];
     __shared__ unsigned long long sQ[BN];
     __shared__ unsigned outside_s;
+    // tabulated B along x: the rows a particle of each box x-cell can land
+    // in (those b_table32's walk visits), staged once per tile
+    constexpr int KB = 8;
+    __shared__ float4 sBrow[BMODE == 2 ? (BX - 1) * KB : 1];
+    __shared__ float4 sBslope[BMODE == 2 ? (BX - 1) * KB : 1];
+    __shared__ int sBcnt[BMODE == 2 ? BX - 1 : 1];
     const MeshView& m = a.m;
     const SpeciesView<float>& sp = a.sp;
     const int t = blockIdx.x;
@@ -651,7 +659,34 @@ __global__ void __launch_bounds__(BLOCK) k_push_tile3d(PushArgs<float> a,
         sQ[q] = 0ull;
     }
     if (threadIdx.x == 0) outside_s = 0u;
+    const bool btab_x = BMODE == 2 && a.b.axis == 0 && a.b.hint32 != nullptr;
+    if constexpr (BMODE == 2) {
+        if (btab_x) {
+            for (int i = threadIdx.x; i < BX - 1; i += BLOCK) {
+                // cell ca's walk starts at max(hint[ca], 1) and ends by
+                // hint[ca + 1] (a particle's q never exceeds the next cell
+                // start), so rows s-1 .. e-1 cover it
+                const int ca = wrap_index(ox + i, m.nx);
+                const int s0 = a.b.hint32[ca] > 1 ? a.b.hint32[ca] : 1;
+                const int e0 = a.b.hint32[ca + 1] > s0 ? a.b.hint32[ca + 1] : s0;
+                const int cnt = e0 - s0 + 1;
+                sBcnt[i] = cnt <= KB ? cnt : 0;
+                if (cnt <= KB)
+                    for (int k = 0; k < cnt; ++k) {
+                        sBrow[i * KB + k] = a.b.row32[s0 - 1 + k];
+                        sBslope[i * KB + k] = a.b.slope32[s0 - 1 + k];
+                    }
+            }
+        }
+    }
     __syncthreads();
+    float4 brow0{}, browN{};
+    if constexpr (BMODE == 2) {
+        if (btab_x) {
+            brow0 = a.b.row32[0];
+            browN = a.b.row32[a.b.rows - 1];
+        }
+    }
     unsigned outside = 0;
     struct State {
         int c;
@@ -667,10 +702,11 @@ __global__ void __launch_bounds__(BLOCK) k_push_tile3d(PushArgs<float> a,
     const int p0 = tile_start[t], p1 = tile_start[t + 1];
     int p = p0 + static_cast<int>(threadIdx.x);
     State next{};
-    if (PF && p < p1) next = load(p);
+    constexpr bool kPrefetch = (PF & 1) != 0;
+    if (kPrefetch && p < p1) next = load(p);
     for (; p < p1; p += BLOCK) {
-        const State st = PF ? next : load(p);
-        if (PF && p + BLOCK < p1) next = load(p + BLOCK);
+        const State st = kPrefetch ? next : load(p);
+        if (kPrefetch && p + BLOCK < p1) next = load(p + BLOCK);
         if (st.c < 0) continue;
         int idx[3];
         decompose<3>(st.c, m, idx);
@@ -695,7 +731,34 @@ __global__ void __launch_bounds__(BLOCK) k_push_tile3d(PushArgs<float> a,
             for (int ax = 0; ax < 3; ++ax) v[ax] += a.qm_dt * E[ax];
         } else {
             float Bx, By, Bz;
-            magnetic<3, float>(a.b, idx, off, Bx, By, Bz, a.err);
+            bool done = false;
+            if constexpr (BMODE == 2) {
+                const int bx = rel_index(idx[0], ox, m.nx);
+                if (btab_x && bx <= BX - 2 && sBcnt[bx] > 0) {
+                    // b_table32 with the walk over the staged rows
+                    const float lo = brow0.x, hi = browN.x;
+                    float q = (static_cast<float>(idx[0]) + off[0]) * a.b.d_axis32;
+                    q = fminf(fmaxf(q, lo), hi);
+                    if (q == lo) {
+                        Bx = brow0.y; By = brow0.z; Bz = brow0.w;
+                    } else if (q == hi) {
+                        Bx = browN.y; By = browN.z; Bz = browN.w;
+                    } else {
+                        const int cnt = sBcnt[bx];
+                        const float4* sr = sBrow + bx * KB;
+                        int k = 1;
+                        while (k < cnt && sr[k].x <= q) ++k;
+                        const float4 L = sr[k - 1];
+                        const float4 S = sBslope[bx * KB + k - 1];
+                        const float t = q - L.x;
+                        Bx = L.y + t * S.x;
+                        By = L.z + t * S.y;
+                        Bz = L.w + t * S.z;
+                    }
+                    done = true;
+                }
+            }
+            if (!done) magnetic<3, float>(a.b, idx, off, Bx, By, Bz, a.err);
             boris<float>(v, E, Bx, By, Bz, a.half_qm_dt);
         }
         const float dtd[3] = {a.dtd0, a.dtd1, a.dtd2};
@@ -785,7 +848,9 @@ bool launch_tile3d(Context& c, Species& sp, double dt) {
            launch_tile3d_t<8, 8, 4, 2, 256, 1>(c, sp, a, mode) ||
            launch_tile3d_t<8, 8, 4, 2, 256, 0>(c, sp, a, mode) ||
            launch_tile3d_t<8, 8, 8, 2, 512, 1>(c, sp, a, mode) ||
-           launch_tile3d_t<8, 8, 8, 2, 256, 1>(c, sp, a, mode);
+           launch_tile3d_t<8, 8, 8, 2, 256, 1>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 4, 2, 512, 3>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 4, 1, 512, 3>(c, sp, a, mode);
 }
 
 template <int DIM, typename R, int BMODE>
@@ -963,10 +1028,14 @@ void push_all(Context& c, double dt, bool fuse_deposit) {
             sp.absorbed_energy.alloc(2);
             AUR_HIP(hipMemset(sp.absorbed_energy.ptr, 0, 2 * sizeof(double)));
         }
-        if (fuse_deposit && sp.tiled && sp.precision == 32 && c.tiled_push_ok() &&
-            launch_tile3d(c, sp, dt)) {
-            ++sp.pushes_since_sort;
-            continue;
+        if (fuse_deposit && sp.tiled && sp.precision == 32 && c.tiled_push_ok()) {
+            const bool timed = c.sort_policy == AURORA_SORT_ADAPTIVE;
+            if (timed) sp.push_ev.record_start(c.stream);
+            if (launch_tile3d(c, sp, dt)) {
+                if (timed) sp.push_ev.record_end(c.stream);
+                ++sp.pushes_since_sort;
+                continue;
+            }
         }
         dispatch_dim_prec(c, sp, [&]<int DIM, typename R>() {
             launch_push<DIM, R>(c, sp, dt, fuse_deposit);

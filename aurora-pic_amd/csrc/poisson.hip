@@ -26,6 +26,7 @@ struct AxisPlan {
     int64_t nlines;
     int pow2;
     int log2n;
+    int inner_shift;     // log2(inner_count) when it is a power of two
 };
 
 __device__ __forceinline__ int bitrev(int x, int bits) {
@@ -130,6 +131,36 @@ __device__ __forceinline__ double mode_of(int idx, int n) {
     return idx <= n / 2 ? static_cast<double>(idx) : -static_cast<double>(n - idx);
 }
 
+// The spectral step for one mode (ix, iy, iz) of rho_hat r: phi_hat =
+// rho_hat / (eps k^2) with continuous k, E_a = -i k_a phi (FieldSolver.cpp:486-488,
+// 717-720) with the own-axis Nyquist plane removed (Hermitian part);
+// A = (Ex + i Ey)_hat, Bv = (Ez + i phi)_hat (phi only when wanted).
+__device__ __forceinline__ void green_mode(const GreenArgs& g, int ix, int iy, int iz, double2 r,
+                                           double2& A, double2& Bv) {
+#pragma clang fp contract(off)
+    if (ix == 0 && iy == 0 && iz == 0) {
+        A = make_double2(0.0, 0.0);
+        Bv = make_double2(0.0, 0.0);
+        return;
+    }
+    const double kx = g.twopi_over_L[0] * mode_of(ix, g.nx);
+    const double ky = g.dim > 1 ? g.twopi_over_L[1] * mode_of(iy, g.ny) : 0.0;
+    const double kz = g.dim > 2 ? g.twopi_over_L[2] * mode_of(iz, g.nz) : 0.0;
+    double k2 = kx * kx + ky * ky;
+    if (g.dim > 2) k2 += kz * kz;
+    const double den = g.eps * k2;
+    const double pr = r.x / den, pi = r.y / den;
+    const bool nyx = (g.nx % 2 == 0) && ix == g.nx / 2;
+    const bool nyy = (g.ny % 2 == 0) && iy == g.ny / 2;
+    const bool nyz = (g.nz % 2 == 0) && iz == g.nz / 2;
+    const double exr = nyx ? 0.0 : kx * pi, exi = nyx ? 0.0 : -kx * pr;
+    const double eyr = nyy ? 0.0 : ky * pi, eyi = nyy ? 0.0 : -ky * pr;
+    A = make_double2(exr - eyi, exi + eyr);
+    double ezr = 0.0, ezi = 0.0;
+    if (g.dim > 2 && !nyz) { ezr = kz * pi; ezi = -kz * pr; }
+    Bv = g.want_phi ? make_double2(ezr - pi, ezi + pr) : make_double2(ezr, ezi);
+}
+
 // fa: rho_hat in, (Ex + i Ey)_hat out; fb: (Ez + i phi)_hat out
 __global__ void k_green(GreenArgs g, double2* fa, double2* fb, int64_t total) {
     for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
@@ -137,33 +168,10 @@ __global__ void k_green(GreenArgs g, double2* fa, double2* fb, int64_t total) {
         const int ix = static_cast<int>(i % g.nx);
         const int iy = static_cast<int>((i / g.nx) % g.ny);
         const int iz = static_cast<int>(i / (static_cast<int64_t>(g.nx) * g.ny));
-        if (ix == 0 && iy == 0 && iz == 0) {
-            fa[i] = make_double2(0.0, 0.0);
-            if (fb) fb[i] = make_double2(0.0, 0.0);
-            continue;
-        }
-        const double kx = g.twopi_over_L[0] * mode_of(ix, g.nx);
-        const double ky = g.dim > 1 ? g.twopi_over_L[1] * mode_of(iy, g.ny) : 0.0;
-        const double kz = g.dim > 2 ? g.twopi_over_L[2] * mode_of(iz, g.nz) : 0.0;
-        double k2 = kx * kx + ky * ky;
-        if (g.dim > 2) k2 += kz * kz;
-        const double2 r = fa[i];
-        const double den = g.eps * k2;
-        const double pr = r.x / den, pi = r.y / den;
-        // E_a = -i k_a phi (FieldSolver.cpp:486-488, 717-720), own-axis Nyquist
-        // plane removed (Hermitian part)
-        const bool nyx = (g.nx % 2 == 0) && ix == g.nx / 2;
-        const bool nyy = (g.ny % 2 == 0) && iy == g.ny / 2;
-        const bool nyz = (g.nz % 2 == 0) && iz == g.nz / 2;
-        const double exr = nyx ? 0.0 : kx * pi, exi = nyx ? 0.0 : -kx * pr;
-        const double eyr = nyy ? 0.0 : ky * pi, eyi = nyy ? 0.0 : -ky * pr;
-        fa[i] = make_double2(exr - eyi, exi + eyr);
-        if (fb) {
-            double ezr = 0.0, ezi = 0.0;
-            if (g.dim > 2 && !nyz) { ezr = kz * pi; ezi = -kz * pr; }
-            if (g.want_phi) fb[i] = make_double2(ezr - pi, ezi + pr);
-            else fb[i] = make_double2(ezr, ezi);
-        }
+        double2 A, Bv;
+        green_mode(g, ix, iy, iz, fa[i], A, Bv);
+        fa[i] = A;
+        if (fb) fb[i] = Bv;
     }
 }
 
@@ -222,6 +230,8 @@ AxisPlan plan_axis(const Context& c, int axis) {
     } else {
         p.elem_stride = nx * ny; p.inner_count = nx * ny; p.outer_stride = 0; p.nlines = nx * ny;
     }
+    p.inner_shift = 0;
+    while ((int64_t{1} << p.inner_shift) < p.inner_count) ++p.inner_shift;
     return p;
 }
 
@@ -238,6 +248,318 @@ void run_axis(Context& c, double2* data, int axis, int inverse) {
         k_fft_axis<false><<<static_cast<unsigned>(blocks), 256, 0, c.stream>>>(
             data, p, B, c.tw[axis].ptr, inverse);
     AUR_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------
+// Fused power-of-two path (every axis a power of two, 2D and 3D).  The
+// transforms are the ones k_fft_axis performs (radix-2 DIT over bit-reversed
+// LDS lines, the same twiddle table, 1/n per inverse axis), regrouped so the
+// field solve makes fewer passes over HBM:
+//   fwd_x_real   reads rho, forward transform along x             -> fa
+//   lines        forward (or inverse) transform of a middle axis  fa (fb)
+//   last         forward transform of the last axis, the Green step of
+//                k_green, both inverse transforms of that axis      -> fa, fb
+//   inv_x_unpack inverse transform along x of fa and fb, the k_unpack
+//                outputs (E64, E32, phi) written directly
+// Twiddles are staged in LDS; all index arithmetic is shifts and masks.
+
+template <int LOG2N>
+__device__ __forceinline__ void stage_twiddles(const double2* tw, double2* s_tw) {
+    constexpr int N = 1 << LOG2N;
+    for (int k = threadIdx.x; k < N / 2; k += blockDim.x) s_tw[k] = tw[k];
+}
+
+// radix-2 stages over `lines` lines of N (bit-reversed input) in LDS
+template <int LOG2N>
+__device__ __forceinline__ void fft_lines(double2* lds, int lines, const double2* s_tw,
+                                          bool inverse) {
+    constexpr int N = 1 << LOG2N;
+    const int total = lines * (N / 2);
+#pragma unroll 1
+    for (int st = 0; st < LOG2N; ++st) {
+        const int half = 1 << st;
+        for (int t = threadIdx.x; t < total; t += blockDim.x) {
+            const int b = t >> (LOG2N - 1);
+            const int q = t & (N / 2 - 1);
+            const int grp = q >> st, pos = q & (half - 1);
+            const int i0 = b * N + grp * 2 * half + pos;
+            const int i1 = i0 + half;
+            double2 w = s_tw[pos << (LOG2N - 1 - st)];
+            if (inverse) w.y = -w.y;
+            const double2 u = lds[i0];
+            const double2 x = cmul(lds[i1], w);
+            lds[i0] = make_double2(u.x + x.x, u.y + x.y);
+            lds[i1] = make_double2(u.x - x.x, u.y - x.y);
+        }
+        __syncthreads();
+    }
+}
+
+template <int LOG2N>
+__global__ void __launch_bounds__(256) k_fwd_x_real(const double* __restrict__ rho,
+                                                    double2* __restrict__ out, int64_t nlines,
+                                                    int B, const double2* __restrict__ tw) {
+    constexpr int N = 1 << LOG2N;
+    __shared__ double2 lds[kFftLds];
+    __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
+    stage_twiddles<LOG2N>(tw, s_tw);
+    const int64_t line0 = static_cast<int64_t>(blockIdx.x) * B;
+    const int lines = static_cast<int>(nlines - line0 < B ? nlines - line0 : B);
+    for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
+        const int b = t >> LOG2N, e = t & (N - 1);
+        lds[b * N + bitrev(e, LOG2N)] = make_double2(rho[(line0 + b) * N + e], 0.0);
+    }
+    __syncthreads();
+    fft_lines<LOG2N>(lds, lines, s_tw, false);
+    for (int t = threadIdx.x; t < lines * N; t += blockDim.x) out[line0 * N + t] = lds[t];
+}
+
+// Node index of element e of line `line` of a power-of-two axis plan.
+__device__ __forceinline__ int64_t line_node(const AxisPlan& p, int64_t line, int e) {
+    return (line >> p.inner_shift) * p.outer_stride + (line & (p.inner_count - 1)) +
+           static_cast<int64_t>(e) * p.elem_stride;
+}
+
+// forward or inverse transform of one strided (middle) axis, 1 << LB lines
+// per block (consecutive lines are consecutive nodes: coalesced); blockIdx.y
+// selects the array (a or b2)
+template <int LOG2N>
+__global__ void __launch_bounds__(256) k_axis_lines(double2* a, double2* b2, AxisPlan p, int LB,
+                                                    const double2* __restrict__ tw, int inverse) {
+    constexpr int N = 1 << LOG2N;
+    __shared__ double2 lds[kFftLds];
+    __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
+    double2* data = blockIdx.y == 0 ? a : b2;
+    stage_twiddles<LOG2N>(tw, s_tw);
+    const int B = 1 << LB;
+    const int64_t line0 = static_cast<int64_t>(blockIdx.x) << LB;
+    for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
+        const int bb = t & (B - 1), e = t >> LB;
+        const int64_t line = line0 + bb;
+        lds[bb * N + bitrev(e, LOG2N)] =
+            line < p.nlines ? data[line_node(p, line, e)] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_lines<LOG2N>(lds, B, s_tw, inverse != 0);
+    const double scale = inverse ? 1.0 / static_cast<double>(N) : 1.0;
+    for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
+        const int bb = t & (B - 1), e = t >> LB;
+        const int64_t line = line0 + bb;
+        if (line >= p.nlines) continue;
+        double2 v = lds[bb * N + e];
+        v.x *= scale;
+        v.y *= scale;
+        data[line_node(p, line, e)] = v;
+    }
+}
+
+constexpr int kLastLds = 1024;   // complex elements per buffer in k_last (16 KiB)
+
+// forward transform of the last axis, Green step, inverse transforms of
+// the two spectra along the same axis (scaled 1/N), written to fa and fb
+template <int LOG2N>
+__global__ void __launch_bounds__(256) k_last(double2* fa, double2* fb, AxisPlan p, int LB,
+                                              const double2* __restrict__ tw, GreenArgs g,
+                                              int write_b) {
+    constexpr int N = 1 << LOG2N;
+    __shared__ double2 X[kLastLds];
+    __shared__ double2 A[kLastLds];
+    __shared__ double2 Bb[kLastLds];
+    __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
+    stage_twiddles<LOG2N>(tw, s_tw);
+    const int B = 1 << LB;
+    const int64_t line0 = static_cast<int64_t>(blockIdx.x) << LB;
+    for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
+        const int bb = t & (B - 1), e = t >> LB;
+        const int64_t line = line0 + bb;
+        X[bb * N + bitrev(e, LOG2N)] =
+            line < p.nlines ? fa[line_node(p, line, e)] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_lines<LOG2N>(X, B, s_tw, false);
+    for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
+        const int bb = t >> LOG2N, e = t & (N - 1);
+        const int64_t line = line0 + bb;
+        double2 av = make_double2(0.0, 0.0), bv = make_double2(0.0, 0.0);
+        if (line < p.nlines) {
+            const int64_t node = line_node(p, line, e);
+            const int ix = static_cast<int>(node & (g.nx - 1));
+            const int iy = static_cast<int>((node / g.nx) & (g.ny - 1));
+            const int iz = static_cast<int>(node / (static_cast<int64_t>(g.nx) * g.ny));
+            green_mode(g, ix, iy, iz, X[t], av, bv);
+        }
+        A[bb * N + bitrev(e, LOG2N)] = av;
+        Bb[bb * N + bitrev(e, LOG2N)] = bv;
+    }
+    __syncthreads();
+    fft_lines<LOG2N>(A, B, s_tw, true);
+    if (write_b) fft_lines<LOG2N>(Bb, B, s_tw, true);
+    const double scale = 1.0 / static_cast<double>(N);
+    for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
+        const int bb = t & (B - 1), e = t >> LB;
+        const int64_t line = line0 + bb;
+        if (line >= p.nlines) continue;
+        const int64_t at = line_node(p, line, e);
+        double2 v = A[bb * N + e];
+        v.x *= scale;
+        v.y *= scale;
+        fa[at] = v;
+        if (write_b) {
+            double2 w = Bb[bb * N + e];
+            w.x *= scale;
+            w.y *= scale;
+            fb[at] = w;
+        }
+    }
+}
+
+// inverse transform along x of fa (and fb), scaled 1/N, unpacked as k_unpack
+template <int LOG2N>
+__global__ void __launch_bounds__(256) k_inv_x_unpack(const double2* __restrict__ fa,
+                                                      const double2* __restrict__ fb,
+                                                      int64_t nlines, int B,
+                                                      const double2* __restrict__ tw,
+                                                      double* E64, double* phi, float* E32,
+                                                      int dim, int want_phi, int64_t nn) {
+    constexpr int N = 1 << LOG2N;
+    __shared__ double2 A[kLastLds];
+    __shared__ double2 Bb[kLastLds];
+    __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
+    stage_twiddles<LOG2N>(tw, s_tw);
+    const bool use_b = dim == 3 || want_phi;
+    const int64_t line0 = static_cast<int64_t>(blockIdx.x) * B;
+    const int lines = static_cast<int>(nlines - line0 < B ? nlines - line0 : B);
+    for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
+        const int b = t >> LOG2N, e = t & (N - 1);
+        A[b * N + bitrev(e, LOG2N)] = fa[line0 * N + t];
+        if (use_b) Bb[b * N + bitrev(e, LOG2N)] = fb[line0 * N + t];
+    }
+    __syncthreads();
+    fft_lines<LOG2N>(A, lines, s_tw, true);
+    if (use_b) fft_lines<LOG2N>(Bb, lines, s_tw, true);
+    const double scale = 1.0 / static_cast<double>(N);
+    for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
+        const int64_t i = line0 * N + t;
+        double2 a = A[t];
+        a.x *= scale;
+        a.y *= scale;
+        E64[i] = a.x;
+        E64[nn + i] = a.y;
+        if (dim == 3) {
+            double2 b = Bb[t];
+            b.x *= scale;
+            b.y *= scale;
+            E64[2 * nn + i] = b.x;
+            if (want_phi) phi[i] = b.y;
+            reinterpret_cast<float4*>(E32)[i] = make_float4(static_cast<float>(a.x),
+                                                            static_cast<float>(a.y),
+                                                            static_cast<float>(b.x), 0.0f);
+        } else {
+            if (want_phi) phi[i] = Bb[t].y * scale;
+            reinterpret_cast<float2*>(E32)[i] =
+                make_float2(static_cast<float>(a.x), static_cast<float>(a.y));
+        }
+    }
+}
+
+template <typename F>
+void with_log2(int log2n, F&& f) {
+    switch (log2n) {
+        case 2: f.template operator()<2>(); break;
+        case 3: f.template operator()<3>(); break;
+        case 4: f.template operator()<4>(); break;
+        case 5: f.template operator()<5>(); break;
+        case 6: f.template operator()<6>(); break;
+        case 7: f.template operator()<7>(); break;
+        case 8: f.template operator()<8>(); break;
+        case 9: f.template operator()<9>(); break;
+        default: f.template operator()<10>(); break;
+    }
+}
+
+// every axis a power of two in [4, 1024]: the fused path applies
+bool fused_fft_ok(const Context& c) {
+    if (c.dim < 2) return false;
+    for (int a = 0; a < c.dim; ++a) {
+        const int64_t n = c.n[a];
+        if (n < 4 || n > kLastLds || (n & (n - 1)) != 0) return false;
+    }
+    return true;
+}
+
+int ilog2(int64_t n) {
+    int l = 0;
+    while ((int64_t{1} << l) < n) ++l;
+    return l;
+}
+
+// lines per block as a power of two: the LDS budget over the line length,
+// at most 64
+int lines_shift(int64_t budget, int64_t n) {
+    int l = ilog2(budget / n);
+    return l > 6 ? 6 : l;
+}
+
+void solve_fused(Context& c, const GreenArgs& ga, bool want_phi) {
+    const int last = c.dim - 1;
+    const bool use_b = c.dim == 3 || want_phi;
+    const int64_t nx = c.n[0];
+    const int64_t xlines = c.nnodes / nx;
+    {
+        const int LB = lines_shift(kFftLds, nx);
+        const unsigned grid = static_cast<unsigned>((xlines + (1 << LB) - 1) >> LB);
+        KernelScope ks(c, "fft_fwd_x");
+        with_log2(ilog2(nx), [&]<int L>() {
+            k_fwd_x_real<L><<<grid, 256, 0, c.stream>>>(c.rho.ptr, c.fa.ptr, xlines, 1 << LB,
+                                                        c.tw[0].ptr);
+        });
+        AUR_HIP(hipGetLastError());
+    }
+    for (int a = 1; a < last; ++a) {
+        const AxisPlan p = plan_axis(c, a);
+        const int LB = lines_shift(kFftLds, p.n);
+        const unsigned grid = static_cast<unsigned>((p.nlines + (1 << LB) - 1) >> LB);
+        KernelScope ks(c, "fft_axis");
+        with_log2(p.log2n, [&]<int L>() {
+            k_axis_lines<L><<<dim3(grid, 1), 256, 0, c.stream>>>(c.fa.ptr, c.fb.ptr, p, LB,
+                                                                 c.tw[a].ptr, 0);
+        });
+        AUR_HIP(hipGetLastError());
+    }
+    {
+        const AxisPlan p = plan_axis(c, last);
+        const int LB = lines_shift(kLastLds, p.n);
+        const unsigned grid = static_cast<unsigned>((p.nlines + (1 << LB) - 1) >> LB);
+        KernelScope ks(c, "fft_last");
+        with_log2(p.log2n, [&]<int L>() {
+            k_last<L><<<grid, 256, 0, c.stream>>>(c.fa.ptr, c.fb.ptr, p, LB, c.tw[last].ptr, ga,
+                                                  use_b ? 1 : 0);
+        });
+        AUR_HIP(hipGetLastError());
+    }
+    for (int a = last - 1; a >= 1; --a) {
+        const AxisPlan p = plan_axis(c, a);
+        const int LB = lines_shift(kFftLds, p.n);
+        const unsigned grid = static_cast<unsigned>((p.nlines + (1 << LB) - 1) >> LB);
+        KernelScope ks(c, "fft_axis");
+        with_log2(p.log2n, [&]<int L>() {
+            k_axis_lines<L><<<dim3(grid, use_b ? 2 : 1), 256, 0, c.stream>>>(
+                c.fa.ptr, c.fb.ptr, p, LB, c.tw[a].ptr, 1);
+        });
+        AUR_HIP(hipGetLastError());
+    }
+    {
+        const int LB = lines_shift(kLastLds, nx);
+        const int B = 1 << LB;
+        const unsigned grid = static_cast<unsigned>((xlines + B - 1) / B);
+        KernelScope ks(c, "fft_inv_x");
+        with_log2(ilog2(nx), [&]<int L>() {
+            k_inv_x_unpack<L><<<grid, 256, 0, c.stream>>>(
+                c.fa.ptr, c.fb.ptr, xlines, B, c.tw[0].ptr, c.E64.ptr, c.phi.ptr, c.E32.ptr,
+                c.dim, want_phi ? 1 : 0, c.nnodes);
+        });
+        AUR_HIP(hipGetLastError());
+    }
 }
 
 int blocks_for(const Context& c, int64_t n) {
@@ -322,6 +644,19 @@ void solve_fields(Context& c, double phi_left, double phi_right, bool want_phi) 
     }
     const int64_t nn = c.nnodes;
     const int g = blocks_for(c, nn);
+    if (fused_fft_ok(c)) {
+        GreenArgs ga{};
+        ga.nx = static_cast<int>(c.n[0]);
+        ga.ny = static_cast<int>(c.n[1]);
+        ga.nz = static_cast<int>(c.n[2]);
+        ga.dim = c.dim;
+        for (int a = 0; a < 3; ++a) ga.twopi_over_L[a] = 6.283185307179586476925286766559 / c.L[a];
+        ga.eps = c.eps;
+        ga.want_phi = want_phi ? 1 : 0;
+        solve_fused(c, ga, want_phi);
+        c.phi_valid = want_phi;
+        return;
+    }
     {
         KernelScope ks(c, "fft_load");
         k_load_real<<<g, 256, 0, c.stream>>>(c.rho.ptr, c.fa.ptr, nn);

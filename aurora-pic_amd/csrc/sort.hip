@@ -55,12 +55,36 @@ __global__ void k_tile_start(const unsigned* keys, int64_t n, int32_t* tile_star
     }
 }
 
-template <typename T>
-__global__ void k_gather(const T* __restrict__ src, T* __restrict__ dst,
-                         const unsigned* __restrict__ perm, int64_t n) {
+// every SoA array of a species moved by one permutation in one pass: the
+// permutation is read once and each thread issues all its loads before
+// its stores
+template <typename R>
+struct PermArrays {
+    const R* src[6];
+    R* dst[6];
+    int count;           // offsets (dim) then velocities (3)
+};
+
+template <typename R>
+__global__ void k_permute(const unsigned* __restrict__ perm, int64_t n,
+                          const int32_t* __restrict__ cell, int32_t* __restrict__ cell2,
+                          const uint32_t* __restrict__ uid, uint32_t* __restrict__ uid2,
+                          PermArrays<R> a) {
     for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < n;
-         p += static_cast<int64_t>(gridDim.x) * blockDim.x)
-        dst[p] = src[perm[p]];
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const unsigned q = perm[p];
+        const int32_t c = cell[q];
+        const uint32_t u = uid[q];
+        R v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (k < a.count) v[k] = a.src[k][q];
+        cell2[p] = c;
+        uid2[p] = u;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (k < a.count) a.dst[k][p] = v[k];
+    }
 }
 
 int blocks(const Context& c, int64_t n) {
@@ -69,14 +93,30 @@ int blocks(const Context& c, int64_t n) {
     return static_cast<int>(g < 1 ? 1 : (g > cap ? cap : g));
 }
 
-template <typename T>
-void permute(Context& c, DevBuf<uint8_t>& a, DevBuf<uint8_t>& b, const unsigned* perm,
-             int64_t n, int64_t capacity) {
-    if (!a.ptr) return;
-    if (!b.ptr) b.alloc(capacity * sizeof(T));
-    k_gather<T><<<blocks(c, n), 256, 0, c.stream>>>(reinterpret_cast<const T*>(a.ptr),
-                                                    reinterpret_cast<T*>(b.ptr), perm, n);
-    std::swap(a, b);
+template <typename R>
+void permute_all(Context& c, Species& sp, const unsigned* perm, int64_t n) {
+    PermArrays<R> a{};
+    DevBuf<uint8_t>* from[6];
+    DevBuf<uint8_t>* to[6];
+    int k = 0;
+    for (int ax = 0; ax < 3; ++ax)
+        if (sp.off[ax].ptr) { from[k] = &sp.off[ax]; to[k] = &sp.off2[ax]; ++k; }
+    for (int ax = 0; ax < 3; ++ax)
+        if (sp.vel[ax].ptr) { from[k] = &sp.vel[ax]; to[k] = &sp.vel2[ax]; ++k; }
+    a.count = k;
+    for (int i = 0; i < k; ++i) {
+        if (!to[i]->ptr) to[i]->alloc(sp.capacity * sizeof(R));
+        a.src[i] = reinterpret_cast<const R*>(from[i]->ptr);
+        a.dst[i] = reinterpret_cast<R*>(to[i]->ptr);
+    }
+    if (!sp.cell2.ptr) sp.cell2.alloc(sp.capacity);
+    if (!sp.uid2.ptr) sp.uid2.alloc(sp.capacity);
+    k_permute<R><<<blocks(c, n), 256, 0, c.stream>>>(perm, n, sp.cell.ptr, sp.cell2.ptr,
+                                                     sp.uid.ptr, sp.uid2.ptr, a);
+    AUR_HIP(hipGetLastError());
+    std::swap(sp.cell, sp.cell2);
+    std::swap(sp.uid, sp.uid2);
+    for (int i = 0; i < k; ++i) std::swap(*from[i], *to[i]);
 }
 
 }  // namespace
@@ -85,6 +125,8 @@ void sort_species(Context& c, Species& sp) {
     const int64_t n = sp.slots;
     if (n == 0) return;
     KernelScope ks(c, "sort");
+    const bool timed = c.sort_policy == AURORA_SORT_ADAPTIVE;
+    if (timed) sp.sort_ev.record_start(c.stream);
     KeyArgs k{};
     k.shift = c.dim == 1 ? 0 : pack_shift(c.dim);
     k.mask = c.dim == 1 ? 0x7FFFFFFF : (1 << k.shift) - 1;
@@ -124,21 +166,10 @@ void sort_species(Context& c, Species& sp) {
     AUR_HIP(rocprim::radix_sort_pairs(temp.ptr, temp_bytes, keys.ptr, keys2.ptr, vals.ptr,
                                       vals2.ptr, n, 0, bits, c.stream));
     const unsigned* perm = vals2.ptr;
-    if (!sp.cell2.ptr) sp.cell2.alloc(sp.capacity);
-    if (!sp.uid2.ptr) sp.uid2.alloc(sp.capacity);
-    k_gather<int32_t><<<blocks(c, n), 256, 0, c.stream>>>(sp.cell.ptr, sp.cell2.ptr, perm, n);
-    k_gather<uint32_t><<<blocks(c, n), 256, 0, c.stream>>>(sp.uid.ptr, sp.uid2.ptr, perm, n);
-    std::swap(sp.cell, sp.cell2);
-    std::swap(sp.uid, sp.uid2);
-    for (int a = 0; a < 3; ++a) {
-        if (sp.precision == 64) {
-            permute<double>(c, sp.off[a], sp.off2[a], perm, n, sp.capacity);
-            permute<double>(c, sp.vel[a], sp.vel2[a], perm, n, sp.capacity);
-        } else {
-            permute<float>(c, sp.off[a], sp.off2[a], perm, n, sp.capacity);
-            permute<float>(c, sp.vel[a], sp.vel2[a], perm, n, sp.capacity);
-        }
-    }
+    if (sp.precision == 64)
+        permute_all<double>(c, sp, perm, n);
+    else
+        permute_all<float>(c, sp, perm, n);
     AUR_HIP(hipGetLastError());
     // segment starts; the live count is the start of the sentinel segment
     if (sp.tile_start.count < static_cast<size_t>(keyspace + 1)) sp.tile_start.alloc(keyspace + 1);
@@ -152,6 +183,9 @@ void sort_species(Context& c, Species& sp) {
     sp.slots = live;
     sp.tiled = true;
     sp.pushes_since_sort = 0;
+    sp.cycle_ms = 0.0;
+    sp.push_ev.recorded = false;
+    if (timed) sp.sort_ev.record_end(c.stream);
     AUR_HIP(hipMemsetAsync(sp.absorbed.ptr + 6, 0, sizeof(unsigned long long), c.stream));
 }
 

@@ -1,6 +1,6 @@
 """Benchmark: particle-updates/s of the 3D3V periodic plasma with tabulated B
-(128^3 cells, 16 electrons + 16 ions per cell = 67,108,864 particles, cell
-sort every 20 steps) on one MI355X, with the reference CPU path timed on a
+(128^3 cells, 16 electrons + 16 ions per cell = 67,108,864 particles,
+adaptive cell sort) on one MI355X, with the reference CPU path timed on a
 bounded sample beside it.
 
 One step = gather + Boris push (tabulated B_y(x)) + drift + periodic
@@ -158,6 +158,9 @@ def main():
     ap.add_argument("--cpu-ppc", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--sort", choices=("adaptive", "fixed"), default="adaptive",
+                    help="adaptive: per-species renewal rule on timed pushes; fixed: every "
+                         "20 steps")
     ap.add_argument("--tile", default=None,
                     help="fixed tile shape tx,ty,tz,halo (skips the autotuner)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -207,6 +210,10 @@ def main():
     ctx.sort()
     ctx.initialize(dt)
     ctx.sort_interval(cfg["sort_interval"])
+    adaptive = args.sort == "adaptive"
+    if adaptive:
+        ctx.sort_policy(hip.SORT_ADAPTIVE)
+    sort_desc = "adaptive" if adaptive else cfg["sort_interval"]
     tuned = None
     if args.tile:
         shape = [int(v) for v in args.tile.split(",")]
@@ -214,13 +221,13 @@ def main():
         if len(shape) > 4:
             ctx.push_variant(shape[4], shape[5])
         ctx.sort()
-        tuned = {"tile": shape[:3], "halo": shape[3], "sort_interval": cfg["sort_interval"],
-                 "fixed": True}
+        tuned = {"tile": shape[:3], "halo": shape[3], "sort": sort_desc, "fixed": True}
     elif not args.no_autotune:
         from aurora_pic import autotune
-        best, table = autotune.tune(ctx, dt, sort_intervals=(cfg["sort_interval"],), log=log)
+        best, table = autotune.tune(ctx, dt, sort_intervals=(cfg["sort_interval"],), log=log,
+                                    adaptive=adaptive, steps=40 if adaptive else 20)
         tuned = {"tile": list(best[1][:3]), "halo": best[1][3], "block": best[1][4],
-                 "prefetch": best[1][5], "sort_interval": best[2],
+                 "prefetch": best[1][5], "sort": sort_desc,
                  "candidates_ms_per_step": {
                      f"{s[0]}x{s[1]}x{s[2]}h{s[3]}b{s[4]}p{s[5]}": round(ms, 4)
                      for ms, s, _ in table}}
@@ -243,8 +250,8 @@ def main():
     ctx.step(dt, args.timing_steps)
     ctx.sync()
     kernels = {}
-    for name in ["push_deposit", "rho", "fft_axis", "green", "fft_load", "fft_unpack",
-                 "clear_charge", "sort"]:
+    for name in ["push_deposit", "rho", "fft_fwd_x", "fft_axis", "fft_last", "fft_inv_x",
+                 "green", "fft_load", "fft_unpack", "clear_charge", "sort"]:
         ms, n = ctx.kernel_time(name)
         if n:
             kernels[name] = {"avg_ms": ms / n, "launches": n}
@@ -296,8 +303,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plasma_3d3v: 128^3 cells, 16 e + 16 i per cell "
                                "(67,108,864 particles), Te 5 eV, Ti 0.1 eV, 131 amu ions, "
-                               "tabulated B_y(x) 512 rows, FFT Poisson (FP64), sort every 20 "
-                               "steps" if args.cells == 128 and args.ppc == 16 else
+                               "tabulated B_y(x) 512 rows, FFT Poisson (FP64), "
+                               f"cell sort {'adaptive' if adaptive else 'every 20 steps'}" if args.cells == 128 and args.ppc == 16 else
                                f"plasma_3d3v {args.cells}^3 x {args.ppc}+{args.ppc} ppc",
                    "particles": total, "cells": args.cells ** 3, "dt_s": dt,
                    "autotune": tuned, "kernels_ms": kernels},

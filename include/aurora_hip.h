@@ -197,22 +197,35 @@ int aurora_hip_boundary_losses(aurora_hip_ctx* ctx, int species_id,
 int aurora_hip_time(aurora_hip_ctx* ctx, double* time, int64_t* step);
 /* Tile shape (cells per axis) of the cell sort and of the LDS-tiled 3D
  * push, and its halo in cells; tx = 0 keeps the sort but disables the tiled
- * push.  Built shapes: 8x8x8 (halo 1, 2), 8x8x4, 4x4x4, 16x8x4 (halo 2).
+ * push.  Built shapes: 8x8x8 (halo 1, 2), 8x8x4 (halo 1, 2), 4x4x4,
+ * 16x8x4 (halo 2).
  * No reference counterpart (the reference keeps particles unsorted). */
 int aurora_hip_set_tiling(aurora_hip_ctx* ctx, int tx, int ty, int tz,
                           int halo);
 /* Launch variant of the LDS-tiled push: workgroup size (256 or 512) and
- * prefetch (1: the next particle's state is loaded while the current one is
- * processed).  Built: 512/0 for every shape; 512/1, 256/1, 256/0 for 8x8x4
- * halo 2; 512/1, 256/1 for 8x8x8 halo 2.  Results do not depend on it. */
+ * prefetch flags (bit 0: the next particle's state is loaded while the
+ * current one is processed; bit 1: registers capped at 64 so four 512-thread
+ * workgroups share a CU).  Built: 512/0 for every shape; 512/1, 256/1,
+ * 256/0, 512/3 for 8x8x4 halo 2; 512/3 for 8x8x4 halo 1; 512/1, 256/1 for
+ * 8x8x8 halo 2.  Results do not depend on it. */
 int aurora_hip_set_push_variant(aurora_hip_ctx* ctx, int block, int prefetch);
 /* Cell sort every interval steps (0 = never) and an immediate sort. */
 int aurora_hip_set_sort_interval(aurora_hip_ctx* ctx, int64_t interval);
-/* At a sort step, a tile-sorted species is re-sorted only if at least this
- * fraction of its particle pushes since its last sort ended outside their
- * tile's LDS box (default 0.02; 0 = always sort).  Sorting never changes
- * results (integer deposit), only speed. */
+/* Fixed policy: at a sort step, a tile-sorted species is re-sorted only if
+ * at least this fraction of its particle pushes since its last sort ended
+ * outside their tile's LDS box (default 0 = always sort).  Sorting never
+ * changes results (integer deposit), only speed. */
 int aurora_hip_set_sort_threshold(aurora_hip_ctx* ctx, double fraction);
+/* Sort policy.  AURORA_SORT_FIXED (default): every sort_interval steps.
+ * AURORA_SORT_ADAPTIVE: each tile-sorted species is timed on the device
+ * (events around its push and its sort) and re-sorted at the first step
+ * whose push costs more than the average step cost of its current cycle,
+ * the last sort included -- the renewal rule that minimises the long-run
+ * cost per step when pushes slow down as particles leave their tiles;
+ * max_interval > 0 caps the steps between sorts.  sort_interval is not
+ * used by this policy. */
+enum { AURORA_SORT_FIXED = 0, AURORA_SORT_ADAPTIVE = 1 };
+int aurora_hip_set_sort_policy(aurora_hip_ctx* ctx, int policy, int64_t max_interval);
 int aurora_hip_sort(aurora_hip_ctx* ctx);
 
 /* ------------------------------------------------------------------ MCC */

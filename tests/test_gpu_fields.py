@@ -248,7 +248,8 @@ def test_sort_preserves_particles_and_charge(hip_ctx):
 @pytest.mark.parametrize("shape", [(8, 8, 8, 2, 512, 0), (8, 8, 8, 1, 512, 0),
                                    (4, 4, 4, 2, 512, 0), (16, 8, 4, 2, 512, 0),
                                    (8, 8, 4, 2, 256, 1), (8, 8, 4, 2, 512, 1),
-                                   (8, 8, 8, 2, 256, 1)])
+                                   (8, 8, 8, 2, 256, 1), (8, 8, 4, 2, 512, 3),
+                                   (8, 8, 4, 1, 512, 3)])
 def test_tiled_push_bit_identical_to_untiled(shape):
     """The LDS-tiled 3D push performs the same operations as the untiled
     kernel: particle state and fixed-point charge agree bit for bit, also
@@ -264,8 +265,10 @@ def test_tiled_push_bit_identical_to_untiled(shape):
     vel = rng.normal(0, 0.5, (N, 3))   # fast: many particles leave the halo
     results = []
     # tiled re-sorting every 7 steps, tiled never re-sorting (threshold 1:
-    # particles stay where the first sort put them), untiled
-    for tiled, threshold in ((True, 0.0), (True, 1.0), (False, 0.0)):
+    # particles stay where the first sort put them), tiled under the
+    # adaptive policy (timed, and capped at 5 steps), untiled
+    for tiled, threshold, policy in ((True, 0.0, None), (True, 1.0, None), (True, 0.0, 0),
+                                     (True, 0.0, 5), (False, 0.0, None)):
         with hip.Context(0) as c:
             c.mesh(n, L, 0, 1.0)
             c.b_profile(0, x, bvals)
@@ -275,6 +278,8 @@ def test_tiled_push_bit_identical_to_untiled(shape):
             else:
                 c.tiling(0)
             c.sort_threshold(threshold)
+            if policy is not None:
+                c.sort_policy(hip.SORT_ADAPTIVE, policy)
             sid = c.add_species(-1.0, 1.0, 1.0 / N, N)
             c.upload(sid, pos, vel, kind=1)
             c.sort()
