@@ -361,6 +361,13 @@ __device__ __forceinline__ long long quantise(double s) {
     double t = __fma_rn(s, 0x1p40, 0x1p52);
     return __double_as_longlong(t) - 0x4330000000000000LL;
 }
+// Trilinear weights: round-to-nearest-even of wx * wyz40, with
+// wyz40 = fl(wy * wz) * 2^40 (exact scaling), in one fma -- the product of
+// the three weights is rounded once, to the 2^-40 grid.
+__device__ __forceinline__ long long quantise_prod(double wx, double wyz40) {
+    double t = __fma_rn(wx, wyz40, 0x1p52);
+    return __double_as_longlong(t) - 0x4330000000000000LL;
+}
 
 // Philox4x32-10 (Salmon et al., SC'11)
 struct Philox {

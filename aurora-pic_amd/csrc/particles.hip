@@ -10,6 +10,7 @@
 // Positions are (linear cell index, in-cell offsets in [0,1]); the cell
 // index strides are node counts, so node (i,j,k) of a cell is its corner.
 #include <cmath>
+#include <type_traits>
 
 #include "field_device.hpp"
 #include "push_device.hpp"
@@ -261,7 +262,12 @@ __device__ __forceinline__ void boris(R* v, const R* E, R Bx, R By, R Bz, R h) {
     R vm0 = v[0] + E[0] * h, vm1 = v[1] + E[1] * h, vm2 = v[2] + E[2] * h;
     const R t0 = Bx * h, t1 = By * h, t2 = Bz * h;
     const R tt = t0 * t0 + t1 * t1 + t2 * t2;
-    const R sf = R(2) / (R(1) + tt);
+    // FP32: the hardware reciprocal (1 ulp) in place of a division
+    R sf;
+    if constexpr (sizeof(R) == 4)
+        sf = 2.0f * __builtin_amdgcn_rcpf(1.0f + tt);
+    else
+        sf = R(2) / (R(1) + tt);
     const R s0 = t0 * sf, s1 = t1 * sf, s2 = t2 * sf;
     const R p0 = vm0 + (vm1 * t2 - vm2 * t1);
     const R p1 = vm1 + (vm2 * t0 - vm0 * t2);
@@ -301,18 +307,19 @@ __device__ __forceinline__ void deposit_one(long long* acc, const MeshView& m, c
         } else {
             const int k0 = idx[2];
             const int k1 = (k0 + 1 == m.nz) ? 0 : k0 + 1;
-            const double wz[2] = {1.0 - o[2], o[2]};
+            const double wz40[2] = {(1.0 - o[2]) * 0x1p40, o[2] * 0x1p40};
             const int ii[2] = {i0, i1}, jj[2] = {j0, j1}, kk[2] = {k0, k1};
 #pragma unroll
             for (int c = 0; c < 2; ++c)
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
+                for (int b = 0; b < 2; ++b) {
+                    const double wyz40 = wy[b] * wz40[c];
 #pragma unroll
                     for (int a = 0; a < 2; ++a)
                         atomicAdd(reinterpret_cast<unsigned long long*>(
                                       acc + (kk[c] * m.ny + jj[b]) * m.nx + ii[a]),
-                                  static_cast<unsigned long long>(
-                                      quantise(wx[a] * wy[b] * wz[c])));
+                                  static_cast<unsigned long long>(quantise_prod(wx[a], wyz40)));
+                }
         }
     }
 }
@@ -627,14 +634,19 @@ __device__ __forceinline__ int rel_index(int i, int o, int n) {
 // BLOCK threads per workgroup; PF bit 0 loads the next particle's state
 // while the current one is processed (more memory-level parallelism); PF
 // bit 1 asks the compiler to fit 8 waves per SIMD (<= 64 VGPRs), so that
-// four 512-thread workgroups share a CU.
+// four 512-thread workgroups share a CU.  PF bit 2 lays the field box out
+// by component.
 template <int TX, int TY, int TZ, int H, int BMODE, int BLOCK, int PF>
 __global__ void __launch_bounds__(BLOCK, (PF & 2) ? 8 : 1) k_push_tile3d(PushArgs<float> a,
                                                        const int32_t* __restrict__ tile_start,
                                                        int ntx, int nty) {
     constexpr int BX = TX + 1 + 2 * H, BY = TY + 1 + 2 * H, BZ = TZ + 1 + 2 * H;
     constexpr int BN = BX * BY * BZ;
-    __shared__ float4 sE[BN];
+    // PF bit 2: the field box as three component arrays (4-byte stride:
+    // random corners spread over every LDS bank) instead of float4 nodes
+    constexpr bool kSoaE = (PF & 4) != 0;
+    __shared__ float4 sE[kSoaE ? 1 : BN];
+    __shared__ float sEc[kSoaE ? 3 * BN : 1];
     __shared__ unsigned long long sQ[BN];
     __shared__ unsigned outside_s;
     // tabulated B along x: the rows a particle of each box x-cell can land
@@ -650,12 +662,21 @@ __global__ void __launch_bounds__(BLOCK, (PF & 2) ? 8 : 1) k_push_tile3d(PushArg
     const int oy = ((t / ntx) % nty) * TY - H;
     const int oz = (t / (ntx * nty)) * TZ - H;
     const float4* E4 = reinterpret_cast<const float4*>(a.E32);
+    // box coordinates stay within [-H, n + H + 1) of each axis and n exceeds
+    // the box, so the wraps need no division
     for (int q = threadIdx.x; q < BN; q += BLOCK) {
         const int bx = q % BX, by = (q / BX) % BY, bz = q / (BX * BY);
-        const int gx = wrap_index(ox + bx, m.nx);
-        const int gy = wrap_index(oy + by, m.ny);
-        const int gz = wrap_index(oz + bz, m.nz);
-        sE[q] = E4[(gz * m.ny + gy) * m.nx + gx];
+        const int gx = wrap_near(ox + bx, m.nx);
+        const int gy = wrap_near(oy + by, m.ny);
+        const int gz = wrap_near(oz + bz, m.nz);
+        const float4 e = E4[(gz * m.ny + gy) * m.nx + gx];
+        if constexpr (kSoaE) {
+            sEc[q] = e.x;
+            sEc[BN + q] = e.y;
+            sEc[2 * BN + q] = e.z;
+        } else {
+            sE[q] = e;
+        }
         sQ[q] = 0ull;
     }
     if (threadIdx.x == 0) outside_s = 0u;
@@ -666,7 +687,7 @@ __global__ void __launch_bounds__(BLOCK, (PF & 2) ? 8 : 1) k_push_tile3d(PushArg
                 // cell ca's walk starts at max(hint[ca], 1) and ends by
                 // hint[ca + 1] (a particle's q never exceeds the next cell
                 // start), so rows s-1 .. e-1 cover it
-                const int ca = wrap_index(ox + i, m.nx);
+                const int ca = wrap_near(ox + i, m.nx);
                 const int s0 = a.b.hint32[ca] > 1 ? a.b.hint32[ca] : 1;
                 const int e0 = a.b.hint32[ca + 1] > s0 ? a.b.hint32[ca + 1] : s0;
                 const int cnt = e0 - s0 + 1;
@@ -692,39 +713,59 @@ __global__ void __launch_bounds__(BLOCK, (PF & 2) ? 8 : 1) k_push_tile3d(PushArg
         int c;
         float o0, o1, o2, v0, v1, v2;
     };
-    auto load = [&](int p) {
+    // the tile's slots [p0, p1) addressed from uniform bases with one
+    // 32-bit offset shared by every array
+    const int p0 = tile_start[t], p1 = tile_start[t + 1];
+    const unsigned cnt = static_cast<unsigned>(p1 - p0);
+    int32_t* __restrict__ cellp = sp.cell + p0;
+    float* __restrict__ o0p = sp.off0 + p0;
+    float* __restrict__ o1p = sp.off1 + p0;
+    float* __restrict__ o2p = sp.off2 + p0;
+    float* __restrict__ v0p = sp.v0 + p0;
+    float* __restrict__ v1p = sp.v1 + p0;
+    float* __restrict__ v2p = sp.v2 + p0;
+    // byte offsets below 2^32 (a tile holds fewer than 2^30 slots): the
+    // loads and stores take a scalar base and one 32-bit vector offset
+    auto at = [](auto* base, unsigned boff) -> auto& {
+        using T = std::remove_reference_t<decltype(*base)>;
+        return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff);
+    };
+    auto load = [&](unsigned k) {
+        const unsigned b = k * 4u;
         State st;
-        st.c = sp.cell[p];
-        st.o0 = sp.off0[p]; st.o1 = sp.off1[p]; st.o2 = sp.off2[p];
-        st.v0 = sp.v0[p]; st.v1 = sp.v1[p]; st.v2 = sp.v2[p];
+        st.c = at(cellp, b);
+        st.o0 = at(o0p, b); st.o1 = at(o1p, b); st.o2 = at(o2p, b);
+        st.v0 = at(v0p, b); st.v1 = at(v1p, b); st.v2 = at(v2p, b);
         return st;
     };
-    const int p0 = tile_start[t], p1 = tile_start[t + 1];
-    int p = p0 + static_cast<int>(threadIdx.x);
-    State next{};
     constexpr bool kPrefetch = (PF & 1) != 0;
-    if (kPrefetch && p < p1) next = load(p);
-    for (; p < p1; p += BLOCK) {
-        const State st = kPrefetch ? next : load(p);
-        if (kPrefetch && p + BLOCK < p1) next = load(p + BLOCK);
+    unsigned k = threadIdx.x;
+    State next{};
+    if (kPrefetch && k < cnt) next = load(k);
+    for (; k < cnt; k += BLOCK) {
+        const State st = kPrefetch ? next : load(k);
+        if (kPrefetch && k + BLOCK < cnt) next = load(k + BLOCK);
         if (st.c < 0) continue;
         int idx[3];
         decompose<3>(st.c, m, idx);
         float off[3] = {st.o0, st.o1, st.o2};
         float v[3] = {st.v0, st.v1, st.v2};
+        // position in the box before the move (wrapped into [0, n))
+        const int rx = rel_index(idx[0], ox, m.nx);
+        const int ry = rel_index(idx[1], oy, m.ny);
+        const int rz = rel_index(idx[2], oz, m.nz);
         float E[3];
-        {
-            const int rx = rel_index(idx[0], ox, m.nx);
-            const int ry = rel_index(idx[1], oy, m.ny);
-            const int rz = rel_index(idx[2], oz, m.nz);
-            if (rx <= BX - 2 && ry <= BY - 2 && rz <= BZ - 2) {
-                const int base = (rz * BY + ry) * BX + rx;
-                trilinear_f32(off, [&](int aa, int b, int cc) {
-                    return sE[base + (cc * BY + b) * BX + aa];
-                }, E);
-            } else {
-                gather<3, float>(a, idx, off, E);
-            }
+        if (rx <= BX - 2 && ry <= BY - 2 && rz <= BZ - 2) {
+            const int base = (rz * BY + ry) * BX + rx;
+            trilinear_f32(off, [&](int aa, int b, int cc) {
+                const int nd = base + (cc * BY + b) * BX + aa;
+                if constexpr (kSoaE)
+                    return make_float4(sEc[nd], sEc[BN + nd], sEc[2 * BN + nd], 0.f);
+                else
+                    return sE[nd];
+            }, E);
+        } else {
+            gather<3, float>(a, idx, off, E);
         }
         if constexpr (BMODE == 0) {
 #pragma unroll
@@ -733,8 +774,7 @@ __global__ void __launch_bounds__(BLOCK, (PF & 2) ? 8 : 1) k_push_tile3d(PushArg
             float Bx, By, Bz;
             bool done = false;
             if constexpr (BMODE == 2) {
-                const int bx = rel_index(idx[0], ox, m.nx);
-                if (btab_x && bx <= BX - 2 && sBcnt[bx] > 0) {
+                if (btab_x && rx <= BX - 2 && sBcnt[rx] > 0) {
                     // b_table32 with the walk over the staged rows
                     const float lo = brow0.x, hi = browN.x;
                     float q = (static_cast<float>(idx[0]) + off[0]) * a.b.d_axis32;
@@ -744,16 +784,16 @@ __global__ void __launch_bounds__(BLOCK, (PF & 2) ? 8 : 1) k_push_tile3d(PushArg
                     } else if (q == hi) {
                         Bx = browN.y; By = browN.z; Bz = browN.w;
                     } else {
-                        const int cnt = sBcnt[bx];
-                        const float4* sr = sBrow + bx * KB;
-                        int k = 1;
-                        while (k < cnt && sr[k].x <= q) ++k;
-                        const float4 L = sr[k - 1];
-                        const float4 S = sBslope[bx * KB + k - 1];
-                        const float t = q - L.x;
-                        Bx = L.y + t * S.x;
-                        By = L.z + t * S.y;
-                        Bz = L.w + t * S.z;
+                        const int cntb = sBcnt[rx];
+                        const float4* sr = sBrow + rx * KB;
+                        int kb = 1;
+                        while (kb < cntb && sr[kb].x <= q) ++kb;
+                        const float4 L = sr[kb - 1];
+                        const float4 S = sBslope[rx * KB + kb - 1];
+                        const float tq = q - L.x;
+                        Bx = L.y + tq * S.x;
+                        By = L.z + tq * S.y;
+                        Bz = L.w + tq * S.z;
                     }
                     done = true;
                 }
@@ -761,41 +801,42 @@ __global__ void __launch_bounds__(BLOCK, (PF & 2) ? 8 : 1) k_push_tile3d(PushArg
             if (!done) magnetic<3, float>(a.b, idx, off, Bx, By, Bz, a.err);
             boris<float>(v, E, Bx, By, Bz, a.half_qm_dt);
         }
-        const float dtd[3] = {a.dtd0, a.dtd1, a.dtd2};
-        const int nn[3] = {m.nx, m.ny, m.nz};
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            int hit = 0;
-            drift_axis<float>(idx[ax], off[ax], v[ax], v[ax] * dtd[ax], nn[ax], nn[ax],
-                              AURORA_PB_PERIODIC, AURORA_PB_PERIODIC, hit);
-        }
-        sp.v0[p] = v[0];
-        sp.v1[p] = v[1];
-        sp.v2[p] = v[2];
-        sp.cell[p] = compose<3>(m, idx);
-        sp.off0[p] = off[0];
-        sp.off1[p] = off[1];
-        sp.off2[p] = off[2];
-        const double o[3] = {static_cast<double>(off[0]), static_cast<double>(off[1]),
-                             static_cast<double>(off[2])};
-        const int rx = rel_index(idx[0], ox, m.nx);
-        const int ry = rel_index(idx[1], oy, m.ny);
-        const int rz = rel_index(idx[2], oz, m.nz);
-        if (rx <= BX - 2 && ry <= BY - 2 && rz <= BZ - 2) {
-            const double wx[2] = {1.0 - o[0], o[0]};
-            const double wy[2] = {1.0 - o[1], o[1]};
-            const double wz[2] = {1.0 - o[2], o[2]};
-            const int base = (rz * BY + ry) * BX + rx;
+        int sh[3];
+        drift_periodic(idx[0], off[0], v[0] * a.dtd0, m.nx, sh[0]);
+        drift_periodic(idx[1], off[1], v[1] * a.dtd1, m.ny, sh[1]);
+        drift_periodic(idx[2], off[2], v[2] * a.dtd2, m.nz, sh[2]);
+        const unsigned kb = k * 4u;
+        at(v0p, kb) = v[0];
+        at(v1p, kb) = v[1];
+        at(v2p, kb) = v[2];
+        at(cellp, kb) = compose<3>(m, idx);
+        at(o0p, kb) = off[0];
+        at(o1p, kb) = off[1];
+        at(o2p, kb) = off[2];
+        // position in the box after the move: the shift needs no wrap (a
+        // particle that wrapped out of [0, n) takes the global deposit)
+        const unsigned qx = static_cast<unsigned>(rx + sh[0]);
+        const unsigned qy = static_cast<unsigned>(ry + sh[1]);
+        const unsigned qz = static_cast<unsigned>(rz + sh[2]);
+        if (qx <= BX - 2 && qy <= BY - 2 && qz <= BZ - 2) {
+            const double wx[2] = {1.0 - static_cast<double>(off[0]), static_cast<double>(off[0])};
+            const double wy[2] = {1.0 - static_cast<double>(off[1]), static_cast<double>(off[1])};
+            const double wz40[2] = {(1.0 - static_cast<double>(off[2])) * 0x1p40,
+                                    static_cast<double>(off[2]) * 0x1p40};
+            const int base = (qz * BY + qy) * BX + qx;
 #pragma unroll
             for (int cc = 0; cc < 2; ++cc)
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
+                for (int b = 0; b < 2; ++b) {
+                    const double wyz40 = wy[b] * wz40[cc];
 #pragma unroll
                     for (int aa = 0; aa < 2; ++aa)
                         atomicAdd(&sQ[base + (cc * BY + b) * BX + aa],
-                                  static_cast<unsigned long long>(
-                                      quantise(wx[aa] * wy[b] * wz[cc])));
+                                  static_cast<unsigned long long>(quantise_prod(wx[aa], wyz40)));
+                }
         } else {
+            const double o[3] = {static_cast<double>(off[0]), static_cast<double>(off[1]),
+                                 static_cast<double>(off[2])};
             deposit_one<3>(a.acc, m, idx, o);
             ++outside;
         }
@@ -808,9 +849,9 @@ __global__ void __launch_bounds__(BLOCK, (PF & 2) ? 8 : 1) k_push_tile3d(PushArg
         const unsigned long long val = sQ[q];
         if (val == 0ull) continue;
         const int bx = q % BX, by = (q / BX) % BY, bz = q / (BX * BY);
-        const int gx = wrap_index(ox + bx, m.nx);
-        const int gy = wrap_index(oy + by, m.ny);
-        const int gz = wrap_index(oz + bz, m.nz);
+        const int gx = wrap_near(ox + bx, m.nx);
+        const int gy = wrap_near(oy + by, m.ny);
+        const int gz = wrap_near(oz + bz, m.nz);
         atomicAdd(reinterpret_cast<unsigned long long*>(a.acc) + (gz * m.ny + gy) * m.nx + gx,
                   val);
     }
@@ -850,7 +891,9 @@ bool launch_tile3d(Context& c, Species& sp, double dt) {
            launch_tile3d_t<8, 8, 8, 2, 512, 1>(c, sp, a, mode) ||
            launch_tile3d_t<8, 8, 8, 2, 256, 1>(c, sp, a, mode) ||
            launch_tile3d_t<8, 8, 4, 2, 512, 3>(c, sp, a, mode) ||
-           launch_tile3d_t<8, 8, 4, 1, 512, 3>(c, sp, a, mode);
+           launch_tile3d_t<8, 8, 4, 1, 512, 3>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 4, 2, 512, 5>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 8, 2, 512, 5>(c, sp, a, mode);
 }
 
 template <int DIM, typename R, int BMODE>
@@ -1028,7 +1071,8 @@ void push_all(Context& c, double dt, bool fuse_deposit) {
             sp.absorbed_energy.alloc(2);
             AUR_HIP(hipMemset(sp.absorbed_energy.ptr, 0, 2 * sizeof(double)));
         }
-        if (fuse_deposit && sp.tiled && sp.precision == 32 && c.tiled_push_ok()) {
+        if (fuse_deposit && sp.tiled && sp.precision == 32 && c.tiled_push_ok() &&
+            sp.slots < (int64_t{1} << 30)) {
             const bool timed = c.sort_policy == AURORA_SORT_ADAPTIVE;
             if (timed) sp.push_ev.record_start(c.stream);
             if (launch_tile3d(c, sp, dt)) {

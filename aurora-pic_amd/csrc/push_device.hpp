@@ -13,6 +13,11 @@ __device__ __forceinline__ int wrap_index(int i, int n) {
     return i < 0 ? i + n : i;
 }
 
+// wrap_index for i in [-n, 2n): no division
+__device__ __forceinline__ int wrap_near(int i, int n) {
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
 // ------------------------------------------------------------------ drift
 // Mirror (ni + f) into [0, ncells] (reflect_coordinate, Simulation3D.cpp:30-40)
 // in integer + fraction form.  A particle that would need more than 64
@@ -73,6 +78,27 @@ __device__ __forceinline__ bool drift_axis(int& idx, R& off, R& v, R dx, int nce
     idx = ni;
     off = f;
     return true;
+}
+
+// drift_axis on a periodic axis, also returning the cell shift (the
+// particle's move in cells before the wrap); same operations, same result.
+__device__ __forceinline__ void drift_periodic(int& idx, float& off, float dx, int n,
+                                               int& shift) {
+    const float x = off + dx;
+    if (x >= 0.f && x < 1.f) {
+        off = x;
+        shift = 0;
+        return;
+    }
+    const float fl = floorf(x);
+    int s = static_cast<int>(fl);
+    float f = x - fl;
+    if (f >= 1.f) { f = 0.f; s += 1; }
+    int ni = idx + s;
+    if (static_cast<unsigned>(ni) >= static_cast<unsigned>(n)) ni = wrap_index(ni, n);
+    idx = ni;
+    off = f;
+    shift = s;
 }
 
 // 1D linear gather of the FP32 node field (FieldSolver.cpp:909-924); the

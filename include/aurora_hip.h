@@ -203,11 +203,12 @@ int aurora_hip_time(aurora_hip_ctx* ctx, double* time, int64_t* step);
 int aurora_hip_set_tiling(aurora_hip_ctx* ctx, int tx, int ty, int tz,
                           int halo);
 /* Launch variant of the LDS-tiled push: workgroup size (256 or 512) and
- * prefetch flags (bit 0: the next particle's state is loaded while the
- * current one is processed; bit 1: registers capped at 64 so four 512-thread
- * workgroups share a CU).  Built: 512/0 for every shape; 512/1, 256/1,
- * 256/0, 512/3 for 8x8x4 halo 2; 512/3 for 8x8x4 halo 1; 512/1, 256/1 for
- * 8x8x8 halo 2.  Results do not depend on it. */
+ * flags (bit 0: the next particle's state is loaded while the current one
+ * is processed; bit 1: registers capped at 64 so four 512-thread workgroups
+ * share a CU; bit 2: the field box in LDS by component instead of float4
+ * nodes).  Built: 512/0 for every shape; 512/1, 256/1, 256/0, 512/3, 512/5
+ * for 8x8x4 halo 2; 512/3 for 8x8x4 halo 1; 512/1, 256/1, 512/5 for 8x8x8
+ * halo 2.  Results do not depend on it. */
 int aurora_hip_set_push_variant(aurora_hip_ctx* ctx, int block, int prefetch);
 /* Cell sort every interval steps (0 = never) and an immediate sort. */
 int aurora_hip_set_sort_interval(aurora_hip_ctx* ctx, int64_t interval);

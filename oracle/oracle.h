@@ -46,7 +46,8 @@ void or_deposit_cic(int dim, const int64_t* n, const double* L, int bc,
 /* ---- fixed-point deposit definition the GPU reproduces bit-exactly.
  * cell: linear cell index (-1 = dead); off: np*dim offsets in cell units.
  * Shape weights in FP64 from the offsets, quantised to 2^-40 units with
- * round-to-nearest-even; acc accumulates int64 per node. */
+ * round-to-nearest-even (3D: the product wx * fl(wy * wz) rounded once, by
+ * one fma); acc accumulates int64 per node. */
 void or_deposit_fixed(int dim, const int64_t* n, int bc, int64_t np,
                       const int32_t* cell, const double* off, int64_t* acc);
 /* rho[node] += Sum_s (double)acc_s[node] * (q_s w_s 2^-40) / V(node), in

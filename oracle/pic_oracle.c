@@ -154,6 +154,16 @@ static int64_t quantise(double s) {
     return bits - (int64_t)0x4330000000000000LL;
 }
 
+/* Trilinear weights: round-to-nearest-even of wx * wyz40 with
+ * wyz40 = fl(wy * wz) * 2^40, one correctly rounded fma (the three-weight
+ * product is rounded once, to the 2^-40 grid). */
+static int64_t quantise_prod(double wx, double wyz40) {
+    double t = fma(wx, wyz40, 0x1p52);
+    int64_t bits;
+    memcpy(&bits, &t, sizeof bits);
+    return bits - (int64_t)0x4330000000000000LL;
+}
+
 void or_deposit_fixed(int dim, const int64_t* n, int bc, int64_t np,
                       const int32_t* cell, const double* off, int64_t* acc) {
     for (int64_t p = 0; p < np; ++p) {
@@ -180,12 +190,13 @@ void or_deposit_fixed(int dim, const int64_t* n, int bc, int64_t np,
         }
         int64_t kk[2] = {k, bc == OR_PERIODIC ? (k + 1) % n[2] : k + 1};
         double oz = off[3 * p + 2];
-        double wz[2] = {1.0 - oz, oz};
+        double wz40[2] = {(1.0 - oz) * 0x1p40, oz * 0x1p40};
         for (int cc = 0; cc < 2; ++cc)
-            for (int b = 0; b < 2; ++b)
+            for (int b = 0; b < 2; ++b) {
+                double wyz40 = wy[b] * wz40[cc];
                 for (int a = 0; a < 2; ++a)
-                    acc[(kk[cc] * n[1] + jj[b]) * n[0] + ii[a]] +=
-                        quantise(wx[a] * wy[b] * wz[cc]);
+                    acc[(kk[cc] * n[1] + jj[b]) * n[0] + ii[a]] += quantise_prod(wx[a], wyz40);
+            }
     }
 }
 

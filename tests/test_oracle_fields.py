@@ -297,3 +297,33 @@ def test_live_reference_random_cases(oracle, reference):
         assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
         E = rng.normal(size=(dim, int(np.prod(n))))
         assert np.max(np.abs(oracle.gather(n, L, bc, pos, E) - reference.gather(n, L, bc, pos, E))) < 1e-13
+
+
+def test_fixed_point_3d_weights_exact(oracle):
+    """The 3D fixed-point weight of every corner is wx * fl(wy * wz) * 2^40
+    rounded once, half to even (brute force in exact rationals), including
+    offsets small enough that fl(wy * wz) rounds and products on ties."""
+    from fractions import Fraction
+    rng = np.random.default_rng(11)
+    offs = [rng.random(3).astype(np.float32).astype(np.float64) for _ in range(200)]
+    tiny = np.float32(2.0 ** -20)
+    offs += [np.array([0.5, tiny, np.float32(3.0 * 2.0 ** -21)], dtype=np.float64),
+             np.array([0.25, 0.5, 0.75]), np.array([0.0, 0.0, 0.0]),
+             np.array([np.float32(1.0 - 2.0 ** -24), tiny, tiny], dtype=np.float64)]
+    # float32 offsets where rounding fl(fl(wx * wy) * wz) first and then to
+    # the grid lands one unit away from the single rounding
+    offs += [np.array([0.7479357719421387, 0.7795912623405457, 0.9397904276847839]),
+             np.array([0.22637362778186798, 0.8544207811355591, 0.6317999362945557]),
+             np.array([0.6504586935043335, 0.44892260432243347, 0.829235851764679])]
+    n = [4, 4, 4]
+    for o in offs:
+        acc = oracle.deposit_fixed(n, 0, np.array([0], dtype=np.int32), o.reshape(1, 3))
+        wx = [1.0 - o[0], o[0]]
+        wy = [1.0 - o[1], o[1]]
+        wz = [1.0 - o[2], o[2]]
+        for c in range(2):
+            for b in range(2):
+                wyz = wy[b] * wz[c]                       # one FP64 rounding
+                for a in range(2):
+                    exact = Fraction(wx[a]) * Fraction(wyz) * 2 ** 40
+                    assert acc[(c * 4 + b) * 4 + a] == round(exact), (o, a, b, c)
