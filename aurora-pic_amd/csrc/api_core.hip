@@ -74,24 +74,46 @@ bool sort_needed(Context& c, Species& sp) {
 
 // The renewal rule: with a cycle cost C(n) = sort + pushes 1..n and pushes
 // that slow down as the cycle ages, the long-run cost per step C(n)/n is
-// least at the first n whose push exceeds C(n)/n.  The push of this step
-// is the mean of the last three (timing noise); its end event is waited
-// for, while the field solve queued behind it keeps the device busy.
+// least at the first n whose push exceeds C(n)/n.  The push compared is the
+// mean of the last three timed pushes (timing noise).  Timings are read one
+// step late (the previous step's push has completed by then), so the
+// decision lags by one step and the host never stalls the queue.
 bool adaptive_sort_due(Context& c, Species& sp) {
     if (sp.precision != 32 || !c.tiled_push_ok()) return false;  // no tiled push to speed up
     if (!sp.tiled) return sp.slots > 0;
-    if (sp.sort_ev.recorded) sp.cycle_ms += sp.sort_ev.take_ms();
-    if (!sp.push_ev.recorded) return false;
-    const double p = sp.push_ev.take_ms();
-    sp.cycle_ms += p;
-    sp.recent_ms[2] = sp.recent_ms[1];
-    sp.recent_ms[1] = sp.recent_ms[0];
-    sp.recent_ms[0] = p;
-    const int64_t n = sp.pushes_since_sort;
-    if (c.sort_max_interval > 0 && n >= c.sort_max_interval) return true;
-    if (n < 3) return false;
+    if (sp.sort_ev.recorded && hipEventQuery(sp.sort_ev.b) == hipSuccess)
+        sp.cycle_ms += sp.sort_ev.take_ms();
+    EventPair& ev = sp.push_ev[sp.push_ev_cur];   // recorded by the previous step
+    if (ev.recorded) {
+        const double p = ev.take_ms();
+        sp.cycle_ms += p;
+        sp.recent_ms[2] = sp.recent_ms[1];
+        sp.recent_ms[1] = sp.recent_ms[0];
+        sp.recent_ms[0] = p;
+        ++sp.timed_pushes;
+    }
+    if (c.sort_max_interval > 0 && sp.pushes_since_sort >= c.sort_max_interval) return true;
+    if (sp.sort_ev.recorded || sp.timed_pushes < 3) return false;
     const double recent = (sp.recent_ms[0] + sp.recent_ms[1] + sp.recent_ms[2]) / 3.0;
-    return recent * static_cast<double>(n) > sp.cycle_ms;
+    return recent * static_cast<double>(sp.timed_pushes) > sp.cycle_ms;
+}
+
+double electrode(const double* p, double t) {
+    // Simulation.cpp:678-685
+    return p[0] + p[1] * std::sin(2.0 * 3.14159265358979323846 * p[2] * t + p[3]);
+}
+
+bool any_fp64(const Context& c) {
+    for (const auto& sp : c.species)
+        if (sp.precision == 64) return true;
+    return false;
+}
+
+void ensure_fields(Context& c, bool phi, bool e64) {
+    if ((phi && !c.phi_valid) || (e64 && !c.e64_valid)) {
+        ensure_rho(c);
+        solve_fields(c, electrode(c.left4, c.time), electrode(c.right4, c.time), true, true);
+    }
 }
 
 }  // namespace aurora
@@ -562,6 +584,7 @@ int aurora_hip_set_electric(aurora_hip_ctx* h, const double* E) {
         AUR_REQUIRE(c.dim > 0 && E, AURORA_ERR_INVALID_ARGUMENT, "bind a mesh first");
         AUR_HIP(hipStreamSynchronize(c.stream));
         AUR_HIP(hipMemcpy(c.E64.ptr, E, c.nnodes * c.dim * sizeof(double), hipMemcpyHostToDevice));
+        c.e64_valid = true;
         std::vector<float> packed(c.E32.count, 0.f);
         const int stride = c.dim == 3 ? 4 : c.dim;
         for (int64_t i = 0; i < c.nnodes; ++i)
@@ -574,6 +597,8 @@ int aurora_hip_set_electric(aurora_hip_ctx* h, const double* E) {
 
 int aurora_hip_get_fields(aurora_hip_ctx* h, double* rho, double* phi, double* E) {
     return guarded(h, [&](Context& c) {
+        if (rho || phi || E) ensure_rho(c);
+        if (phi || E) ensure_fields(c, phi != nullptr, E != nullptr);
         AUR_HIP(hipStreamSynchronize(c.stream));
         if (rho) AUR_HIP(hipMemcpy(rho, c.rho.ptr, c.nnodes * sizeof(double), hipMemcpyDeviceToHost));
         if (phi) AUR_HIP(hipMemcpy(phi, c.phi.ptr, c.nnodes * sizeof(double), hipMemcpyDeviceToHost));
@@ -588,6 +613,7 @@ int aurora_hip_set_rho(aurora_hip_ctx* h, const double* rho) {
         AUR_REQUIRE(c.dim > 0 && rho, AURORA_ERR_INVALID_ARGUMENT, "bind a mesh first");
         AUR_HIP(hipStreamSynchronize(c.stream));
         AUR_HIP(hipMemcpy(c.rho.ptr, rho, c.nnodes * sizeof(double), hipMemcpyHostToDevice));
+        c.rho_valid = true;
     });
 }
 
@@ -600,14 +626,10 @@ int aurora_hip_deposit(aurora_hip_ctx* h) {
     });
 }
 
-static double electrode(const double* p, double t) {
-    // Simulation.cpp:678-685
-    return p[0] + p[1] * std::sin(2.0 * 3.14159265358979323846 * p[2] * t + p[3]);
-}
-
 int aurora_hip_field_solve(aurora_hip_ctx* h, double phi_left, double phi_right, int want_phi) {
     return guarded(h, [&](Context& c) {
         AUR_REQUIRE(c.dim > 0, AURORA_ERR_INVALID_ARGUMENT, "bind a mesh first");
+        ensure_rho(c);
         solve_fields(c, phi_left, phi_right, want_phi != 0);
         flush_timers(c);
     });
@@ -646,9 +668,8 @@ int aurora_hip_step(aurora_hip_ctx* h, double dt, int64_t nsteps) {
         for (int64_t s = 0; s < nsteps; ++s) {
             // kick + drift + boundary + deposit at x^{n+1} (fused)
             push_all(c, dt, true);
-            convert_rho(c);
             const double t1 = c.time + dt;
-            solve_fields(c, electrode(c.left4, t1), electrode(c.right4, t1), false);
+            solve_step_fields(c, electrode(c.left4, t1), electrode(c.right4, t1));
             if (c.mcc.loaded) mcc_step(c, dt);
             c.time = t1;
             c.step_count += 1;

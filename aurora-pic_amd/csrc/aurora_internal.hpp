@@ -144,6 +144,8 @@ struct Species {
     DevBuf<uint8_t> vel[3];   // v^{n-1/2} (float or double)
     DevBuf<uint32_t> uid;
     DevBuf<int64_t> acc;      // fixed-point charge, one per node
+    DevBuf<int64_t> acc_next; // zeroed by the step's field solve; the next push swaps it in
+    bool acc_next_clear = false;
     // sort double buffers (allocated on first sort)
     DevBuf<int32_t> cell2;
     DevBuf<uint8_t> off2[3], vel2[3];
@@ -154,10 +156,14 @@ struct Species {
     DevBuf<int32_t> tile_start;           // ntiles + 1 segment starts after a sort
     bool tiled = false;                   // storage is tile-sorted (tile_start valid)
     int64_t pushes_since_sort = 0;        // tiled pushes counted in absorbed[6]
-    // adaptive sort policy: measured cost of the current sort cycle
-    EventPair push_ev, sort_ev;
-    double cycle_ms = 0.0;                // last sort + tiled pushes since, ms
-    double recent_ms[3] = {0, 0, 0};      // the last three tiled pushes
+    // adaptive sort policy: measured cost of the current sort cycle.  Pushes
+    // are timed into two event pairs in turn and read one step later, when
+    // they have completed, so the host does not wait on the device.
+    EventPair push_ev[2], sort_ev;
+    int push_ev_cur = 0;                  // pair the next push records into
+    double cycle_ms = 0.0;                // last sort + timed pushes since, ms
+    int64_t timed_pushes = 0;             // pushes of the cycle in cycle_ms
+    double recent_ms[3] = {0, 0, 0};      // the last three timed pushes
     int64_t absorbed_host[6] = {0, 0, 0, 0, 0, 0};
     double absorbed_energy_host[2] = {0, 0};
     size_t elem() const { return precision == 64 ? 8 : 4; }
@@ -225,6 +231,8 @@ struct Context {
     DevBuf<double> rho, phi, E64;   // E64: dim arrays
     DevBuf<float> E32;              // packed: stride 4 (3D), 2 (2D), 1 (1D)
     bool phi_valid = false;
+    bool e64_valid = false;        // E64 holds the field of the current rho
+    bool rho_valid = false;        // rho holds the last deposit's density
 
     // magnetic field
     double Buni[3] = {0, 0, 0};
@@ -337,7 +345,18 @@ inline dim3 grid_for(int64_t n, int block) {
 }
 
 // host-side API pieces implemented in separate translation units
-void solve_fields(Context& ctx, double phi_left, double phi_right, bool want_phi);
+void solve_fields(Context& ctx, double phi_left, double phi_right, bool want_phi,
+                  bool want_e64 = true);
+// the step's field solve: rho from the accumulators and the Poisson solve,
+// fused on the power-of-two FFT path (rho itself is then written on demand)
+void solve_step_fields(Context& ctx, double phi_left, double phi_right);
+// converts the accumulators when the last step did not write rho
+void ensure_rho(Context& ctx);
+// re-solves from the current rho when phi or E64 was not written by the last
+// solve (steps skip them when no kernel reads them)
+void ensure_fields(Context& ctx, bool phi, bool e64);
+// some species is pushed in FP64 (reads E64)
+bool any_fp64(const Context& ctx);
 void prepare_solver(Context& ctx);
 void deposit_all(Context& ctx);
 void convert_rho(Context& ctx);

@@ -20,6 +20,9 @@ struct RhoArgs {
     int dim;
 };
 
+// The context's species accumulators and weights (particles.hip).
+RhoArgs rho_args(const Context& c);
+
 // Charge density of node i from the per-species fixed-point sums, divided by
 // the node volume (Grid.cpp:26-30, Mesh2D.cpp:77-88, Mesh3D.cpp:52-60), in
 // species order.

@@ -1033,7 +1033,7 @@ void deposit_all(Context& c) {
     }
 }
 
-void convert_rho(Context& c) {
+RhoArgs rho_args(const Context& c) {
     AUR_REQUIRE(c.species.size() <= static_cast<size_t>(kMaxChannels), AURORA_ERR_UNSUPPORTED,
                 "at most 8 species per context");
     RhoArgs a{};
@@ -1052,14 +1052,30 @@ void convert_rho(Context& c) {
     a.dz = c.d[2];
     a.dim = c.dim;
     a.vol = c.dim == 1 ? c.d[0] : (c.dim == 2 ? c.d[0] * c.d[1] : c.d[0] * c.d[1] * c.d[2]);
+    return a;
+}
+
+void convert_rho(Context& c) {
+    const RhoArgs a = rho_args(c);
     KernelScope ks(c, "rho");
     k_rho<<<launch_blocks(c, c.nnodes), kBlock, 0, c.stream>>>(a, c.rho.ptr, c.nnodes);
     AUR_HIP(hipGetLastError());
+    c.rho_valid = true;
+}
+
+void ensure_rho(Context& c) {
+    if (!c.rho_valid) convert_rho(c);
 }
 
 void push_all(Context& c, double dt, bool fuse_deposit) {
     if (fuse_deposit) {
         for (auto& sp : c.species) {
+            if (sp.acc_next_clear && sp.acc_next.ptr) {
+                // zeroed by the last field solve
+                std::swap(sp.acc, sp.acc_next);
+                sp.acc_next_clear = false;
+                continue;
+            }
             KernelScope ks(c, "clear_charge");
             k_fill_i64<<<launch_blocks(c, c.nnodes), kBlock, 0, c.stream>>>(
                 reinterpret_cast<long long*>(sp.acc.ptr), c.nnodes);
@@ -1074,9 +1090,13 @@ void push_all(Context& c, double dt, bool fuse_deposit) {
         if (fuse_deposit && sp.tiled && sp.precision == 32 && c.tiled_push_ok() &&
             sp.slots < (int64_t{1} << 30)) {
             const bool timed = c.sort_policy == AURORA_SORT_ADAPTIVE;
-            if (timed) sp.push_ev.record_start(c.stream);
+            EventPair& ev = sp.push_ev[sp.push_ev_cur];
+            if (timed) ev.record_start(c.stream);
             if (launch_tile3d(c, sp, dt)) {
-                if (timed) sp.push_ev.record_end(c.stream);
+                if (timed) {
+                    ev.record_end(c.stream);
+                    sp.push_ev_cur ^= 1;
+                }
                 ++sp.pushes_since_sort;
                 continue;
             }
@@ -1180,6 +1200,9 @@ int aurora_hip_energy(aurora_hip_ctx* h, double dt, double* kinetic, double* fie
         }
         check_device_error(c);
         if (field_energy || charge_l1) {
+            ensure_rho(c);
+            ensure_fields(c, false, true);
+            AUR_HIP(hipStreamSynchronize(c.stream));
             // FP64 node sums in node order (Simulation3D.cpp:564-576)
             std::vector<double> rho(c.nnodes), E(c.nnodes * c.dim);
             AUR_HIP(hipMemcpy(rho.data(), c.rho.ptr, c.nnodes * sizeof(double),

@@ -314,6 +314,36 @@ __global__ void __launch_bounds__(256) k_fwd_x_real(const double* __restrict__ r
     for (int t = threadIdx.x; t < lines * N; t += blockDim.x) out[line0 * N + t] = lds[t];
 }
 
+// The step's variant of k_fwd_x_real: the charge density of each node from
+// the species accumulators (rho_at, the conversion kernel's operations), the
+// accumulators of the next push zeroed on the way.
+struct ClearArgs {
+    long long* p[kMaxChannels];
+    int n;
+};
+
+template <int LOG2N>
+__global__ void __launch_bounds__(256) k_fwd_x_acc(RhoArgs ra, ClearArgs cl,
+                                                   double2* __restrict__ out, int64_t nlines,
+                                                   int B, const double2* __restrict__ tw) {
+    constexpr int N = 1 << LOG2N;
+    __shared__ double2 lds[kFftLds];
+    __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
+    stage_twiddles<LOG2N>(tw, s_tw);
+    const int64_t line0 = static_cast<int64_t>(blockIdx.x) * B;
+    const int lines = static_cast<int>(nlines - line0 < B ? nlines - line0 : B);
+    for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
+        const int b = t >> LOG2N, e = t & (N - 1);
+        const int64_t i = line0 * N + t;
+        const double r = rho_at(ra, i);
+        for (int s = 0; s < cl.n; ++s) cl.p[s][i] = 0;
+        lds[b * N + bitrev(e, LOG2N)] = make_double2(r, 0.0);
+    }
+    __syncthreads();
+    fft_lines<LOG2N>(lds, lines, s_tw, false);
+    for (int t = threadIdx.x; t < lines * N; t += blockDim.x) out[line0 * N + t] = lds[t];
+}
+
 // Node index of element e of line `line` of a power-of-two axis plan.
 __device__ __forceinline__ int64_t line_node(const AxisPlan& p, int64_t line, int e) {
     return (line >> p.inner_shift) * p.outer_stride + (line & (p.inner_count - 1)) +
@@ -414,6 +444,7 @@ __global__ void __launch_bounds__(256) k_last(double2* fa, double2* fb, AxisPlan
 }
 
 // inverse transform along x of fa (and fb), scaled 1/N, unpacked as k_unpack
+// (E64 may be null: only the FP32 field is written)
 template <int LOG2N>
 __global__ void __launch_bounds__(256) k_inv_x_unpack(const double2* __restrict__ fa,
                                                       const double2* __restrict__ fb,
@@ -443,13 +474,15 @@ __global__ void __launch_bounds__(256) k_inv_x_unpack(const double2* __restrict_
         double2 a = A[t];
         a.x *= scale;
         a.y *= scale;
-        E64[i] = a.x;
-        E64[nn + i] = a.y;
+        if (E64) {
+            E64[i] = a.x;
+            E64[nn + i] = a.y;
+        }
         if (dim == 3) {
             double2 b = Bb[t];
             b.x *= scale;
             b.y *= scale;
-            E64[2 * nn + i] = b.x;
+            if (E64) E64[2 * nn + i] = b.x;
             if (want_phi) phi[i] = b.y;
             reinterpret_cast<float4*>(E32)[i] = make_float4(static_cast<float>(a.x),
                                                             static_cast<float>(a.y),
@@ -500,7 +533,10 @@ int lines_shift(int64_t budget, int64_t n) {
     return l > 6 ? 6 : l;
 }
 
-void solve_fused(Context& c, const GreenArgs& ga, bool want_phi) {
+// from_acc: the first transform reads the species accumulators (the step's
+// solve; rho is not written) instead of rho
+void solve_fused(Context& c, const GreenArgs& ga, bool want_phi, bool want_e64,
+                 bool from_acc = false) {
     const int last = c.dim - 1;
     const bool use_b = c.dim == 3 || want_phi;
     const int64_t nx = c.n[0];
@@ -509,10 +545,26 @@ void solve_fused(Context& c, const GreenArgs& ga, bool want_phi) {
         const int LB = lines_shift(kFftLds, nx);
         const unsigned grid = static_cast<unsigned>((xlines + (1 << LB) - 1) >> LB);
         KernelScope ks(c, "fft_fwd_x");
-        with_log2(ilog2(nx), [&]<int L>() {
-            k_fwd_x_real<L><<<grid, 256, 0, c.stream>>>(c.rho.ptr, c.fa.ptr, xlines, 1 << LB,
-                                                        c.tw[0].ptr);
-        });
+        if (from_acc) {
+            const RhoArgs ra = rho_args(c);
+            ClearArgs cl{};
+            cl.n = static_cast<int>(c.species.size());
+            for (int s = 0; s < cl.n; ++s) {
+                Species& sp = c.species[s];
+                if (!sp.acc_next.ptr) sp.acc_next.alloc(c.nnodes);
+                cl.p[s] = reinterpret_cast<long long*>(sp.acc_next.ptr);
+            }
+            with_log2(ilog2(nx), [&]<int L>() {
+                k_fwd_x_acc<L><<<grid, 256, 0, c.stream>>>(ra, cl, c.fa.ptr, xlines, 1 << LB,
+                                                           c.tw[0].ptr);
+            });
+            for (auto& sp : c.species) sp.acc_next_clear = true;
+        } else {
+            with_log2(ilog2(nx), [&]<int L>() {
+                k_fwd_x_real<L><<<grid, 256, 0, c.stream>>>(c.rho.ptr, c.fa.ptr, xlines, 1 << LB,
+                                                            c.tw[0].ptr);
+            });
+        }
         AUR_HIP(hipGetLastError());
     }
     for (int a = 1; a < last; ++a) {
@@ -555,7 +607,8 @@ void solve_fused(Context& c, const GreenArgs& ga, bool want_phi) {
         KernelScope ks(c, "fft_inv_x");
         with_log2(ilog2(nx), [&]<int L>() {
             k_inv_x_unpack<L><<<grid, 256, 0, c.stream>>>(
-                c.fa.ptr, c.fb.ptr, xlines, B, c.tw[0].ptr, c.E64.ptr, c.phi.ptr, c.E32.ptr,
+                c.fa.ptr, c.fb.ptr, xlines, B, c.tw[0].ptr, want_e64 ? c.E64.ptr : nullptr,
+                c.phi.ptr, c.E32.ptr,
                 c.dim, want_phi ? 1 : 0, c.nnodes);
         });
         AUR_HIP(hipGetLastError());
@@ -621,7 +674,32 @@ void prepare_solver(Context& c) {
     }
 }
 
-void solve_fields(Context& c, double phi_left, double phi_right, bool want_phi) {
+GreenArgs green_args(const Context& c, bool want_phi) {
+    GreenArgs ga{};
+    ga.nx = static_cast<int>(c.n[0]);
+    ga.ny = static_cast<int>(c.n[1]);
+    ga.nz = static_cast<int>(c.n[2]);
+    ga.dim = c.dim;
+    for (int a = 0; a < 3; ++a) ga.twopi_over_L[a] = 6.283185307179586476925286766559 / c.L[a];
+    ga.eps = c.eps;
+    ga.want_phi = want_phi ? 1 : 0;
+    return ga;
+}
+
+void solve_step_fields(Context& c, double phi_left, double phi_right) {
+    const bool e64 = any_fp64(c);
+    if (c.dim >= 2 && fused_fft_ok(c)) {
+        solve_fused(c, green_args(c, false), false, e64, true);
+        c.rho_valid = false;
+        c.phi_valid = false;
+        c.e64_valid = e64;
+        return;
+    }
+    convert_rho(c);
+    solve_fields(c, phi_left, phi_right, false, e64);
+}
+
+void solve_fields(Context& c, double phi_left, double phi_right, bool want_phi, bool want_e64) {
     if (c.dim == 1) {
         const int n = static_cast<int>(c.n[0]);
         if (c.bc == AURORA_PERIODIC) {
@@ -640,21 +718,15 @@ void solve_fields(Context& c, double phi_left, double phi_right, bool want_phi) 
         }
         AUR_HIP(hipGetLastError());
         c.phi_valid = true;
+        c.e64_valid = true;
         return;
     }
     const int64_t nn = c.nnodes;
     const int g = blocks_for(c, nn);
     if (fused_fft_ok(c)) {
-        GreenArgs ga{};
-        ga.nx = static_cast<int>(c.n[0]);
-        ga.ny = static_cast<int>(c.n[1]);
-        ga.nz = static_cast<int>(c.n[2]);
-        ga.dim = c.dim;
-        for (int a = 0; a < 3; ++a) ga.twopi_over_L[a] = 6.283185307179586476925286766559 / c.L[a];
-        ga.eps = c.eps;
-        ga.want_phi = want_phi ? 1 : 0;
-        solve_fused(c, ga, want_phi);
+        solve_fused(c, green_args(c, want_phi), want_phi, want_e64);
         c.phi_valid = want_phi;
+        c.e64_valid = want_e64;
         return;
     }
     {
@@ -685,6 +757,7 @@ void solve_fields(Context& c, double phi_left, double phi_right, bool want_phi) 
     }
     AUR_HIP(hipGetLastError());
     c.phi_valid = want_phi;
+    c.e64_valid = true;
 }
 
 }  // namespace aurora

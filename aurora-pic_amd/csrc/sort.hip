@@ -184,7 +184,9 @@ void sort_species(Context& c, Species& sp) {
     sp.tiled = true;
     sp.pushes_since_sort = 0;
     sp.cycle_ms = 0.0;
-    sp.push_ev.recorded = false;
+    sp.timed_pushes = 0;
+    sp.push_ev[0].recorded = false;   // pushes of the previous cycle
+    sp.push_ev[1].recorded = false;
     if (timed) sp.sort_ev.record_end(c.stream);
     AUR_HIP(hipMemsetAsync(sp.absorbed.ptr + 6, 0, sizeof(unsigned long long), c.stream));
 }

@@ -340,3 +340,42 @@ def test_error_paths(hip_ctx):
         hip_ctx.b_profile(0, [0.0, 0.0, 1.0], [[0, 0, 1]] * 3)
     with pytest.raises(hip.HipError, match="permittivity"):
         hip_ctx.mesh([8], [1.0], 0, 0.0)
+
+
+@pytest.mark.parametrize("n", [[16, 16, 16], [32, 16]])
+def test_fields_after_steps_are_current(oracle, n):
+    """Steps skip rho, phi and the FP64 field when no kernel reads them; the
+    readers recompute them from the last deposit.  The fields read after a
+    step are the solve of that step's charge, and reading them in the middle
+    of a run does not change the run (bit for bit)."""
+    from aurora_pic import hip
+    rng = np.random.default_rng(8)
+    dim = len(n)
+    L = [1.0] * dim
+    N = int(np.prod(n)) * 4
+    pos = rng.uniform(0, 1, (N, dim))
+    vel = rng.normal(0, 0.3, (N, 3))
+    finals = []
+    for peek in (False, True):
+        with hip.Context(0) as c:
+            c.mesh(n, L, 0, 1.0)
+            sid = c.add_species(-1.0, 1.0, 1.0 / N, N)
+            c.upload(sid, pos, vel, kind=1)
+            c.sort()
+            c.initialize(0.01)
+            c.step(0.01, 5)
+            if peek:
+                rho, phi, E = c.fields()
+                phi_o, E_o = oracle.poisson(n, L, 0, 1.0, rho)
+                assert np.max(np.abs(E.reshape(-1) - E_o.reshape(-1))) <= \
+                    1e-10 * np.max(np.abs(E_o))
+                assert np.max(np.abs(phi - phi_o)) <= 1e-10 * np.max(np.abs(phi_o))
+                ke, fe, l1, live = c.energy(0.01)
+                assert live == N and fe > 0.0
+            c.step(0.01, 5)
+            p_, v_, a_, u_ = c.download(sid)
+            o = np.argsort(u_)
+            finals.append((c.charge_fixed(sid), p_[o], v_[o]))
+    assert np.array_equal(finals[0][0], finals[1][0])
+    assert np.array_equal(finals[0][1], finals[1][1])
+    assert np.array_equal(finals[0][2], finals[1][2])
