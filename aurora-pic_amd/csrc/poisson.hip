@@ -269,36 +269,80 @@ __device__ __forceinline__ void stage_twiddles(const double2* tw, double2* s_tw)
     for (int k = threadIdx.x; k < N / 2; k += blockDim.x) s_tw[k] = tw[k];
 }
 
-// radix-2 stages over `lines` lines of N (bit-reversed input) in LDS
+// LDS layout of the fused kernels: element i of line b at b*LS + i + (i>>3)
+// with LS = N + N/8 + 1.  The pad after every 8 elements spreads the
+// 8-element register blocks of the first pass over the banks, and the odd
+// line stride spreads the column accesses of the strided axes.
+template <int LOG2N>
+struct Lines {
+    static constexpr int N = 1 << LOG2N;
+    static constexpr int LS = N + (N >> 3) + 1;
+    __device__ static __forceinline__ int at(int b, int i) { return b * LS + i + (i >> 3); }
+};
+
+// The radix-2 stages over `lines` lines of N (bit-reversed order in LDS),
+// three stages per pass: a thread loads the 8 elements i0 + k*2^S that
+// stages S, S+1 and S+2 combine, runs those butterflies in registers (the
+// same twiddles, products and sums as one stage at a time, so the result is
+// the same to the bit) and writes them back -- ceil(LOG2N / 3) LDS round
+// trips and barriers instead of LOG2N.
+template <int LOG2N, int S, int R>
+__device__ __forceinline__ void radix_pass(double2* lds, int lines, const double2* s_tw,
+                                           bool inverse) {
+    using L = Lines<LOG2N>;
+    constexpr int N = 1 << LOG2N, K = 1 << R, H = 1 << S;
+    constexpr int PER_LINE = N / K;
+    const int total = lines * PER_LINE;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        const int b = t / PER_LINE, q = t & (PER_LINE - 1);
+        const int pos = q & (H - 1), grp = q >> S;
+        const int i0 = grp * (K * H) + pos;
+        double2 v[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = lds[L::at(b, i0 + k * H)];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int hj = 1 << j;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (k & hj) continue;
+                // position of element k in its stage-(S+j) group of 2 * 2^(S+j)
+                const int posj = pos + (k & (hj - 1)) * H;
+                double2 w = s_tw[posj << (LOG2N - 1 - (S + j))];
+                if (inverse) w.y = -w.y;
+                const double2 u = v[k];
+                const double2 x = cmul(v[k + hj], w);
+                v[k] = make_double2(u.x + x.x, u.y + x.y);
+                v[k + hj] = make_double2(u.x - x.x, u.y - x.y);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[L::at(b, i0 + k * H)] = v[k];
+    }
+}
+
+template <int LOG2N, int S>
+__device__ __forceinline__ void radix_passes(double2* lds, int lines, const double2* s_tw,
+                                             bool inverse) {
+    if constexpr (S < LOG2N) {
+        constexpr int R = LOG2N - S >= 3 ? 3 : LOG2N - S;
+        radix_pass<LOG2N, S, R>(lds, lines, s_tw, inverse);
+        __syncthreads();
+        radix_passes<LOG2N, S + R>(lds, lines, s_tw, inverse);
+    }
+}
+
 template <int LOG2N>
 __device__ __forceinline__ void fft_lines(double2* lds, int lines, const double2* s_tw,
                                           bool inverse) {
-    constexpr int N = 1 << LOG2N;
-    const int total = lines * (N / 2);
-#pragma unroll 1
-    for (int st = 0; st < LOG2N; ++st) {
-        const int half = 1 << st;
-        for (int t = threadIdx.x; t < total; t += blockDim.x) {
-            const int b = t >> (LOG2N - 1);
-            const int q = t & (N / 2 - 1);
-            const int grp = q >> st, pos = q & (half - 1);
-            const int i0 = b * N + grp * 2 * half + pos;
-            const int i1 = i0 + half;
-            double2 w = s_tw[pos << (LOG2N - 1 - st)];
-            if (inverse) w.y = -w.y;
-            const double2 u = lds[i0];
-            const double2 x = cmul(lds[i1], w);
-            lds[i0] = make_double2(u.x + x.x, u.y + x.y);
-            lds[i1] = make_double2(u.x - x.x, u.y - x.y);
-        }
-        __syncthreads();
-    }
+    radix_passes<LOG2N, 0>(lds, lines, s_tw, inverse);
 }
 
 template <int LOG2N>
 __global__ void __launch_bounds__(256) k_fwd_x_real(const double* __restrict__ rho,
                                                     double2* __restrict__ out, int64_t nlines,
                                                     int B, const double2* __restrict__ tw) {
+    using L = Lines<LOG2N>;
     constexpr int N = 1 << LOG2N;
     __shared__ double2 lds[kFftLds];
     __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
@@ -307,11 +351,12 @@ __global__ void __launch_bounds__(256) k_fwd_x_real(const double* __restrict__ r
     const int lines = static_cast<int>(nlines - line0 < B ? nlines - line0 : B);
     for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
         const int b = t >> LOG2N, e = t & (N - 1);
-        lds[b * N + bitrev(e, LOG2N)] = make_double2(rho[(line0 + b) * N + e], 0.0);
+        lds[L::at(b, e)] = make_double2(rho[(line0 + b) * N + bitrev(e, LOG2N)], 0.0);
     }
     __syncthreads();
     fft_lines<LOG2N>(lds, lines, s_tw, false);
-    for (int t = threadIdx.x; t < lines * N; t += blockDim.x) out[line0 * N + t] = lds[t];
+    for (int t = threadIdx.x; t < lines * N; t += blockDim.x)
+        out[line0 * N + t] = lds[L::at(t >> LOG2N, t & (N - 1))];
 }
 
 // The step's variant of k_fwd_x_real: the charge density of each node from
@@ -326,6 +371,7 @@ template <int LOG2N>
 __global__ void __launch_bounds__(256) k_fwd_x_acc(RhoArgs ra, ClearArgs cl,
                                                    double2* __restrict__ out, int64_t nlines,
                                                    int B, const double2* __restrict__ tw) {
+    using L = Lines<LOG2N>;
     constexpr int N = 1 << LOG2N;
     __shared__ double2 lds[kFftLds];
     __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
@@ -334,14 +380,15 @@ __global__ void __launch_bounds__(256) k_fwd_x_acc(RhoArgs ra, ClearArgs cl,
     const int lines = static_cast<int>(nlines - line0 < B ? nlines - line0 : B);
     for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
         const int b = t >> LOG2N, e = t & (N - 1);
-        const int64_t i = line0 * N + t;
+        const int64_t i = (line0 + b) * N + bitrev(e, LOG2N);
         const double r = rho_at(ra, i);
         for (int s = 0; s < cl.n; ++s) cl.p[s][i] = 0;
-        lds[b * N + bitrev(e, LOG2N)] = make_double2(r, 0.0);
+        lds[L::at(b, e)] = make_double2(r, 0.0);
     }
     __syncthreads();
     fft_lines<LOG2N>(lds, lines, s_tw, false);
-    for (int t = threadIdx.x; t < lines * N; t += blockDim.x) out[line0 * N + t] = lds[t];
+    for (int t = threadIdx.x; t < lines * N; t += blockDim.x)
+        out[line0 * N + t] = lds[L::at(t >> LOG2N, t & (N - 1))];
 }
 
 // Node index of element e of line `line` of a power-of-two axis plan.
@@ -356,6 +403,7 @@ __device__ __forceinline__ int64_t line_node(const AxisPlan& p, int64_t line, in
 template <int LOG2N>
 __global__ void __launch_bounds__(256) k_axis_lines(double2* a, double2* b2, AxisPlan p, int LB,
                                                     const double2* __restrict__ tw, int inverse) {
+    using L = Lines<LOG2N>;
     constexpr int N = 1 << LOG2N;
     __shared__ double2 lds[kFftLds];
     __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
@@ -366,8 +414,8 @@ __global__ void __launch_bounds__(256) k_axis_lines(double2* a, double2* b2, Axi
     for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
         const int bb = t & (B - 1), e = t >> LB;
         const int64_t line = line0 + bb;
-        lds[bb * N + bitrev(e, LOG2N)] =
-            line < p.nlines ? data[line_node(p, line, e)] : make_double2(0.0, 0.0);
+        lds[L::at(bb, e)] = line < p.nlines ? data[line_node(p, line, bitrev(e, LOG2N))]
+                                            : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_lines<LOG2N>(lds, B, s_tw, inverse != 0);
@@ -376,24 +424,26 @@ __global__ void __launch_bounds__(256) k_axis_lines(double2* a, double2* b2, Axi
         const int bb = t & (B - 1), e = t >> LB;
         const int64_t line = line0 + bb;
         if (line >= p.nlines) continue;
-        double2 v = lds[bb * N + e];
+        double2 v = lds[L::at(bb, e)];
         v.x *= scale;
         v.y *= scale;
         data[line_node(p, line, e)] = v;
     }
 }
 
-constexpr int kLastLds = 1024;   // complex elements per buffer in k_last (16 KiB)
+constexpr int kLastLds = 1280;   // complex elements per buffer in k_last, k_inv_x_unpack (20 KiB)
 
 // forward transform of the last axis, Green step, inverse transforms of
-// the two spectra along the same axis (scaled 1/N), written to fa and fb
+// the two spectra along the same axis (scaled 1/N), written to fa and fb.
+// The first spectrum is written over the transformed input once every
+// thread holds its share of it in registers (at most 4 elements: B*N <= 1024).
 template <int LOG2N>
 __global__ void __launch_bounds__(256) k_last(double2* fa, double2* fb, AxisPlan p, int LB,
                                               const double2* __restrict__ tw, GreenArgs g,
                                               int write_b) {
+    using L = Lines<LOG2N>;
     constexpr int N = 1 << LOG2N;
     __shared__ double2 X[kLastLds];
-    __shared__ double2 A[kLastLds];
     __shared__ double2 Bb[kLastLds];
     __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
     stage_twiddles<LOG2N>(tw, s_tw);
@@ -402,27 +452,39 @@ __global__ void __launch_bounds__(256) k_last(double2* fa, double2* fb, AxisPlan
     for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
         const int bb = t & (B - 1), e = t >> LB;
         const int64_t line = line0 + bb;
-        X[bb * N + bitrev(e, LOG2N)] =
-            line < p.nlines ? fa[line_node(p, line, e)] : make_double2(0.0, 0.0);
+        X[L::at(bb, e)] = line < p.nlines ? fa[line_node(p, line, bitrev(e, LOG2N))]
+                                          : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_lines<LOG2N>(X, B, s_tw, false);
-    for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
+    double2 av[4], bv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int t = threadIdx.x + r * 256;
+        av[r] = make_double2(0.0, 0.0);
+        bv[r] = make_double2(0.0, 0.0);
+        if (t >= B * N) continue;
         const int bb = t >> LOG2N, e = t & (N - 1);
         const int64_t line = line0 + bb;
-        double2 av = make_double2(0.0, 0.0), bv = make_double2(0.0, 0.0);
         if (line < p.nlines) {
             const int64_t node = line_node(p, line, e);
             const int ix = static_cast<int>(node & (g.nx - 1));
             const int iy = static_cast<int>((node / g.nx) & (g.ny - 1));
             const int iz = static_cast<int>(node / (static_cast<int64_t>(g.nx) * g.ny));
-            green_mode(g, ix, iy, iz, X[t], av, bv);
+            green_mode(g, ix, iy, iz, X[L::at(bb, e)], av[r], bv[r]);
         }
-        A[bb * N + bitrev(e, LOG2N)] = av;
-        Bb[bb * N + bitrev(e, LOG2N)] = bv;
     }
     __syncthreads();
-    fft_lines<LOG2N>(A, B, s_tw, true);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int t = threadIdx.x + r * 256;
+        if (t >= B * N) continue;
+        const int bb = t >> LOG2N, e = t & (N - 1);
+        X[L::at(bb, bitrev(e, LOG2N))] = av[r];
+        Bb[L::at(bb, bitrev(e, LOG2N))] = bv[r];
+    }
+    __syncthreads();
+    fft_lines<LOG2N>(X, B, s_tw, true);
     if (write_b) fft_lines<LOG2N>(Bb, B, s_tw, true);
     const double scale = 1.0 / static_cast<double>(N);
     for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
@@ -430,12 +492,12 @@ __global__ void __launch_bounds__(256) k_last(double2* fa, double2* fb, AxisPlan
         const int64_t line = line0 + bb;
         if (line >= p.nlines) continue;
         const int64_t at = line_node(p, line, e);
-        double2 v = A[bb * N + e];
+        double2 v = X[L::at(bb, e)];
         v.x *= scale;
         v.y *= scale;
         fa[at] = v;
         if (write_b) {
-            double2 w = Bb[bb * N + e];
+            double2 w = Bb[L::at(bb, e)];
             w.x *= scale;
             w.y *= scale;
             fb[at] = w;
@@ -452,6 +514,7 @@ __global__ void __launch_bounds__(256) k_inv_x_unpack(const double2* __restrict_
                                                       const double2* __restrict__ tw,
                                                       double* E64, double* phi, float* E32,
                                                       int dim, int want_phi, int64_t nn) {
+    using L = Lines<LOG2N>;
     constexpr int N = 1 << LOG2N;
     __shared__ double2 A[kLastLds];
     __shared__ double2 Bb[kLastLds];
@@ -462,8 +525,9 @@ __global__ void __launch_bounds__(256) k_inv_x_unpack(const double2* __restrict_
     const int lines = static_cast<int>(nlines - line0 < B ? nlines - line0 : B);
     for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
         const int b = t >> LOG2N, e = t & (N - 1);
-        A[b * N + bitrev(e, LOG2N)] = fa[line0 * N + t];
-        if (use_b) Bb[b * N + bitrev(e, LOG2N)] = fb[line0 * N + t];
+        const int64_t src = (line0 + b) * N + bitrev(e, LOG2N);
+        A[L::at(b, e)] = fa[src];
+        if (use_b) Bb[L::at(b, e)] = fb[src];
     }
     __syncthreads();
     fft_lines<LOG2N>(A, lines, s_tw, true);
@@ -471,7 +535,8 @@ __global__ void __launch_bounds__(256) k_inv_x_unpack(const double2* __restrict_
     const double scale = 1.0 / static_cast<double>(N);
     for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
         const int64_t i = line0 * N + t;
-        double2 a = A[t];
+        const int la = L::at(t >> LOG2N, t & (N - 1));
+        double2 a = A[la];
         a.x *= scale;
         a.y *= scale;
         if (E64) {
@@ -479,7 +544,7 @@ __global__ void __launch_bounds__(256) k_inv_x_unpack(const double2* __restrict_
             E64[nn + i] = a.y;
         }
         if (dim == 3) {
-            double2 b = Bb[t];
+            double2 b = Bb[la];
             b.x *= scale;
             b.y *= scale;
             if (E64) E64[2 * nn + i] = b.x;
@@ -488,7 +553,7 @@ __global__ void __launch_bounds__(256) k_inv_x_unpack(const double2* __restrict_
                                                             static_cast<float>(a.y),
                                                             static_cast<float>(b.x), 0.0f);
         } else {
-            if (want_phi) phi[i] = Bb[t].y * scale;
+            if (want_phi) phi[i] = Bb[la].y * scale;
             reinterpret_cast<float2*>(E32)[i] =
                 make_float2(static_cast<float>(a.x), static_cast<float>(a.y));
         }
@@ -526,11 +591,13 @@ int ilog2(int64_t n) {
     return l;
 }
 
-// lines per block as a power of two: the LDS budget over the line length,
-// at most 64
+// lines per block as a power of two: the LDS budget over the padded line
+// length (n + n/8 + 1), at most 64
 int lines_shift(int64_t budget, int64_t n) {
-    int l = ilog2(budget / n);
-    return l > 6 ? 6 : l;
+    const int64_t ls = n + n / 8 + 1;
+    int l = 0;
+    while ((int64_t{2} << l) * ls <= budget && l < 6) ++l;
+    return l;
 }
 
 // from_acc: the first transform reads the species accumulators (the step's
