@@ -1,5 +1,6 @@
 // aurora_gpu.cpp - C++20 host classes over the C ABI (include/aurora_hip.h).
 #include "aurora_gpu.hpp"
+#include "particle_state.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -112,6 +113,18 @@ void upload_and_deposit(const Device& d, int dim, const std::vector<P>& particle
 
 void pos1(const Particle& p, double* x) { x[0] = p.x; }
 void pos2(const Particle2D& p, double* x) { x[0] = p.position.x; x[1] = p.position.y; }
+// the configured species as the expectations of an external state file
+template <typename SpeciesConfigs>
+ExternalParticleState load_external(const std::filesystem::path& path, std::size_t dim,
+                                    UnitSystem units, const SpeciesConfigs& species,
+                                    const std::string& context,
+                                    std::optional<std::uint64_t> signature) {
+    std::vector<ExternalSpeciesExpectation> expected;
+    for (const auto& s : species) expected.push_back({s.name, s.particles});
+    return load_validated_external_particle_state(path, dim, units, expected, context,
+                                                  signature);
+}
+
 void pos3(const Particle3D& p, double* x) {
     x[0] = p.position.x; x[1] = p.position.y; x[2] = p.position.z;
 }
@@ -367,6 +380,12 @@ CollisionStepStatistics NullCollisionModel::collide(std::vector<Vec3>& velocitie
 // ----------------------------------------------------------- Simulation
 Simulation::Simulation(Config cfg, std::shared_ptr<Device> device)
     : cfg_(std::move(cfg)), dev_(or_new(std::move(device))) {
+    // Simulation.cpp:352-362 (the device path is 1D3V)
+    if (cfg_.initial_state_signature && cfg_.initial_state_path.empty())
+        throw std::invalid_argument("initial_state_signature requires initial_state_path");
+    if (!cfg_.initial_state_path.empty())
+        throw std::invalid_argument(
+            "external particle-state initialization is not yet available for 1D3V");
     const Device& d = *dev_;
     const int64_t n = static_cast<int64_t>(cfg_.nx);
     mesh_bind(d, 1, &n, &cfg_.length, cfg_.boundary, cfg_.permittivity, 1.0);
@@ -486,6 +505,8 @@ CollisionStepStatistics Simulation::collision_statistics() const {
 // --------------------------------------------------------- Simulation2D
 Simulation2D::Simulation2D(Simulation2DConfig cfg, std::shared_ptr<Device> device)
     : cfg_(std::move(cfg)), dev_(or_new(std::move(device))) {
+    if (cfg_.initial_state_signature && cfg_.initial_state_path.empty())
+        throw std::invalid_argument("2D initial_state_signature requires initial_state_path");
     const Device& d = *dev_;
     const int64_t n[2] = {static_cast<int64_t>(cfg_.nx), static_cast<int64_t>(cfg_.ny)};
     const double L[2] = {cfg_.length_x, cfg_.length_y};
@@ -503,6 +524,43 @@ Simulation2D::Simulation2D(Simulation2DConfig cfg, std::shared_ptr<Device> devic
 
 void Simulation2D::initialize() {
     const Device& d = *dev_;
+    if (!cfg_.initial_state_path.empty()) {
+        // Simulation2D.cpp:703-771: every configured species from the file,
+        // positions inside the domain (periodic upper ends excluded), then
+        // half-step velocities from the self-consistent field
+        const auto state = load_external(cfg_.initial_state_path, 2, cfg_.units, cfg_.species,
+                                         "2D simulation", cfg_.initial_state_signature);
+        for (size_t k = 0; k < cfg_.species.size(); ++k) {
+            const auto& records = state.species.at(cfg_.species[k].name);
+            const int64_t n = static_cast<int64_t>(records.size());
+            std::vector<double> pos(static_cast<size_t>(2 * n)), vel(static_cast<size_t>(3 * n));
+            for (int64_t p = 0; p < n; ++p) {
+                const auto& r = records[static_cast<size_t>(p)];
+                const bool outside =
+                    r.position.x < 0.0 || r.position.x > cfg_.length_x || r.position.y < 0.0 ||
+                    r.position.y > cfg_.length_y ||
+                    (cfg_.boundary == Boundary::Periodic &&
+                     (r.position.x == cfg_.length_x || r.position.y == cfg_.length_y));
+                if (outside)
+                    throw std::runtime_error("external particle for species '" +
+                                             cfg_.species[k].name +
+                                             "' lies outside the 2D domain");
+                pos[2 * p] = r.position.x;
+                pos[2 * p + 1] = r.position.y;
+                vel[3 * p] = r.velocity.x;
+                vel[3 * p + 1] = r.velocity.y;
+                vel[3 * p + 2] = r.velocity.z;
+            }
+            d.check(aurora_hip_species_upload(d.get(), static_cast<int>(k), n, pos.data(),
+                                              vel.data(), nullptr, AURORA_VELOCITY_CENTRED));
+        }
+        if (cfg_.boundary == Boundary::Periodic) d.check(aurora_hip_sort(d.get()));
+        d.check(aurora_hip_initialize(d.get(), cfg_.dt));
+        time_ = 0.0;
+        step_ = 0;
+        initialized_ = true;
+        return;
+    }
     for (size_t k = 0; k < cfg_.species.size(); ++k) {
         const auto& s = cfg_.species[k];
         const double drift[3] = {s.drift_velocity.x, s.drift_velocity.y, s.drift_velocity.z};
@@ -564,6 +622,8 @@ Mesh2D Simulation2D::fields() const {
 // --------------------------------------------------------- Simulation3D
 Simulation3D::Simulation3D(Simulation3DConfig cfg, std::shared_ptr<Device> device)
     : cfg_(std::move(cfg)), dev_(or_new(std::move(device))) {
+    if (cfg_.initial_state_signature && cfg_.initial_state_path.empty())
+        throw std::invalid_argument("3D initial_state_signature requires initial_state_path");
     const Device& d = *dev_;
     const int64_t n[3] = {static_cast<int64_t>(cfg_.nx), static_cast<int64_t>(cfg_.ny),
                           static_cast<int64_t>(cfg_.nz)};
@@ -615,6 +675,35 @@ void Simulation3D::set_particles(std::size_t species, const std::vector<Particle
 
 void Simulation3D::initialize() {
     const Device& d = *dev_;
+    if (!cfg_.initial_state_path.empty()) {
+        // Simulation3D.cpp:221-290
+        const auto state = load_external(cfg_.initial_state_path, 3, cfg_.units, cfg_.species,
+                                         "3D simulation", cfg_.initial_state_signature);
+        for (size_t k = 0; k < cfg_.species.size(); ++k) {
+            const auto& records = state.species.at(cfg_.species[k].name);
+            std::vector<Particle3D> ps;
+            ps.reserve(records.size());
+            for (const auto& r : records) {
+                const bool outside =
+                    r.position.x < 0.0 || r.position.x > cfg_.length_x || r.position.y < 0.0 ||
+                    r.position.y > cfg_.length_y || r.position.z < 0.0 ||
+                    r.position.z > cfg_.length_z ||
+                    (cfg_.boundary == Boundary::Periodic &&
+                     (r.position.x == cfg_.length_x || r.position.y == cfg_.length_y ||
+                      r.position.z == cfg_.length_z));
+                if (outside)
+                    throw std::runtime_error("external particle for species '" +
+                                             cfg_.species[k].name +
+                                             "' lies outside the 3D domain");
+                Particle3D q;
+                q.position = r.position;
+                q.velocity = r.velocity;
+                q.alive = true;
+                ps.push_back(q);
+            }
+            set_particles(k, ps);
+        }
+    }
     for (size_t k = 0; k < cfg_.species.size(); ++k) {
         if (loaded_[k]) continue;
         const auto& s = cfg_.species[k];

@@ -22,6 +22,7 @@
 
 #include <array>
 #include <cstdint>
+#include <filesystem>
 #include <memory>
 #include <optional>
 #include <stdexcept>
@@ -45,6 +46,9 @@ struct Vec3 {
     double y{0.0};
     double z{0.0};
 };
+
+// Units.hpp: which unit contract a run (and an external state file) uses
+enum class UnitSystem { Normalized, SI };
 
 // Types.hpp:90-113
 struct Particle {
@@ -312,6 +316,10 @@ struct Simulation3DConfig {       // Simulation3D.hpp:26-66, the fields the loop
     double permittivity{1.0};
     std::size_t sort_interval{20};
     std::vector<Species3DConfig> species{};
+    // external time-zero state (.aps version 1) instead of Maxwellian loading
+    UnitSystem units{UnitSystem::Normalized};
+    std::filesystem::path initial_state_path{};
+    std::optional<std::uint64_t> initial_state_signature{};
 };
 
 struct Simulation2DConfig {       // Simulation2D.hpp, structured periodic/Dirichlet
@@ -326,6 +334,9 @@ struct Simulation2DConfig {       // Simulation2D.hpp, structured periodic/Diric
     double permittivity{1.0};
     std::size_t sort_interval{20};
     std::vector<Species3DConfig> species{};
+    UnitSystem units{UnitSystem::Normalized};
+    std::filesystem::path initial_state_path{};
+    std::optional<std::uint64_t> initial_state_signature{};
 };
 
 struct SinusoidalVoltageConfig {  // Config.hpp:139-143
@@ -368,6 +379,9 @@ struct Config {                   // Config.hpp:170-203, the fields the loop use
     std::size_t max_particles_per_species{10000000};
     std::vector<NamedCollisionConfig> collision_models{};
     std::vector<SpeciesConfig> species{};
+    // external state files are rejected for 1D3V runs, as in the reference
+    std::filesystem::path initial_state_path{};
+    std::optional<std::uint64_t> initial_state_signature{};
 };
 
 struct BoundaryLoss1D {           // Simulation.hpp:20-25 (counts and impact energy)

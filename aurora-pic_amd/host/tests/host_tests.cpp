@@ -5,8 +5,12 @@
 //   aurora_host_tests            run every case
 //   aurora_host_tests <filter>   run the cases whose name contains filter
 //   aurora_host_tests --list     print the case names (no GPU use)
+//   aurora_host_tests --cpu <golden dir>   run the host-only cases (no GPU use)
+//   aurora_host_tests --aps-signature <file>   print a state file's signature
 #include <cmath>
 #include <cstdio>
+#include <filesystem>
+#include <fstream>
 #include <functional>
 #include <iostream>
 #include <memory>
@@ -16,6 +20,7 @@
 #include <vector>
 
 #include "aurora_gpu.hpp"
+#include "particle_state.hpp"
 
 using namespace pic::gpu;
 
@@ -341,6 +346,215 @@ void error_paths(const Dev& d) {
                             "tiny grid");
 }
 
+// ---------------------------------------------------------------- .aps
+// tests/test_initialization.cpp:786-930 of the reference, restated
+std::filesystem::path scratch_dir() {
+    auto d = std::filesystem::temp_directory_path() / "aurora_host_tests";
+    std::filesystem::create_directories(d);
+    return d;
+}
+
+void write_state(const std::filesystem::path& p, int dim, int count, const std::string& rows) {
+    std::ofstream out(p);
+    out << "AuroraPIC-particle-state-v1\ndimension " << dim
+        << "\nunits normalized\nweighting species_constant\n"
+           "velocity_staggering time_centered\nparticle_count "
+        << count << "\nrecords\n"
+        << rows << "end\n";
+}
+
+void aps_reference_example_signature(const std::filesystem::path& golden) {
+    // examples/external_state_1d.cfg pins initial_state_signature for this file
+    const auto s = load_external_particle_state(golden / "aps" / "external_state_1d.aps", 4);
+    require(s.signature == 15689689587873937355ULL,
+            "signature of the reference's example state: " + std::to_string(s.signature));
+    require(s.spatial_dimension == 1 && s.particle_count == 4 &&
+                s.species.at("electrons").size() == 2 && s.species.at("ions").size() == 2,
+            "example state lost metadata or records");
+    require(s.species.at("ions")[1].position.x == 0.85 &&
+                s.species.at("electrons")[0].velocity.x == 1.0,
+            "example state records");
+}
+
+void aps_round_trip(const std::filesystem::path&) {
+    const auto dir = scratch_dir();
+    const auto path = dir / "state.aps";
+    write_state(path, 1, 2,
+                "particle electrons 0.25 0 0 1.5 0 0\nparticle electrons 0.75 0 0 -0.5 0 0\n");
+    const auto s = load_external_particle_state(path, 2);
+    require(s.spatial_dimension == 1 && s.particle_count == 2 &&
+                s.species.at("electrons").size() == 2 && s.signature != 0,
+            "loader lost metadata or records");
+    const auto inter = dir / "interleaved.aps";
+    write_state(inter, 1, 4,
+                "particle electrons 0.1 0 0 1 0 0\nparticle ions 0.2 0 0 2 0 0\n"
+                "particle electrons 0.3 0 0 3 0 0\nparticle ions 0.4 0 0 4 0 0\n");
+    const auto si = load_external_particle_state(inter, 4);
+    require(si.species.at("ions")[1].position.x == 0.4 &&
+                si.species.at("electrons")[1].position.x == 0.3,
+            "interleaved species records kept in file order per species");
+    const auto rt = dir / "roundtrip.aps";
+    std::filesystem::remove(rt);
+    write_external_particle_state(rt, si);
+    require(load_external_particle_state(rt, 4).signature == si.signature,
+            "round trip changed the signature");
+    require_throws_contains([&] { write_external_particle_state(rt, si); }, "already exists",
+                            "writer replaced a file without permission");
+    auto mutated = s;
+    mutated.species.at("electrons")[0].velocity.x += 0.25;
+    require(external_particle_state_signature(mutated) != s.signature,
+            "signature ignored a changed record");
+    write_external_particle_state(rt, mutated, true);
+    require(load_external_particle_state(rt, 2).signature ==
+                external_particle_state_signature(mutated),
+            "explicit overwrite did not round trip");
+    require_throws_contains(
+        [&] {
+            (void)load_validated_external_particle_state(path, 1, UnitSystem::Normalized,
+                                                         {{"electrons", 2}}, "test",
+                                                         s.signature + 1);
+        },
+        "signature mismatch", "signature mismatch accepted");
+    validate_external_particle_state(s, 1, UnitSystem::Normalized, {{"electrons", 2}}, "test");
+    require_throws_contains([&] { (void)load_external_particle_state(path, 1); },
+                            "exceeds the configured limit", "count limit ignored");
+    require_throws_contains(
+        [&] {
+            validate_external_particle_state(s, 2, UnitSystem::Normalized, {{"electrons", 2}},
+                                             "test");
+        },
+        "dimension does not match", "dimension mismatch accepted");
+    require_throws_contains(
+        [&] {
+            validate_external_particle_state(s, 1, UnitSystem::SI, {{"electrons", 2}}, "test");
+        },
+        "unit system does not match", "unit mismatch accepted");
+    require_throws_contains(
+        [&] {
+            validate_external_particle_state(s, 1, UnitSystem::Normalized, {{"ions", 2}},
+                                             "test");
+        },
+        "missing species 'ions'", "species mismatch accepted");
+    const auto meta = dir / "meta.txt";
+    write_external_particle_state_metadata(meta, path, {1, 1, UnitSystem::Normalized, 2,
+                                                        s.signature}, std::nullopt);
+    std::ifstream in(meta);
+    const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    require(text.find("signature " + std::to_string(s.signature)) != std::string::npos &&
+                text.find("expected_signature none") != std::string::npos,
+            "metadata file");
+}
+
+void aps_errors(const std::filesystem::path&) {
+    const auto dir = scratch_dir();
+    const auto p = dir / "bad.aps";
+    auto expect = [&](const std::string& text, const std::string& needle) {
+        {
+            std::ofstream out(p);
+            out << text;
+        }
+        require_throws_contains([&] { (void)load_external_particle_state(p, 10); }, needle,
+                                "malformed state file accepted: " + needle);
+    };
+    const std::string head = "AuroraPIC-particle-state-v1\ndimension 2\nunits normalized\n"
+                             "weighting species_constant\nvelocity_staggering time_centered\n";
+    expect("AuroraPIC-particle-state-v2\n", "invalid external particle-state magic");
+    expect("AuroraPIC-particle-state-v1\ndimension 4\n", "invalid dimension");
+    expect("AuroraPIC-particle-state-v1\ndimension 2\nunits cgs\n", "invalid units 'cgs'");
+    expect("AuroraPIC-particle-state-v1\ndimension 2\nunits si\nweighting per_particle\n",
+           "only weighting species_constant");
+    expect(head + "particle_count 1\nrecords\nparticle e 0.1 0.2 0.3 1 2 3\nend\n",
+           "nonzero inactive z position");
+    expect(head + "particle_count 1\nrecords\nparticle e 0.1 0.2 0 1 2 nan\nend\n",
+           "invalid external particle record 0");
+    expect(head + "particle_count 1\nrecords\nparticle e 0.1 0.2 0 1 2 3\nend\nextra\n",
+           "contains trailing data");
+    expect(head + "particle_count 2\nrecords\nparticle e 0.1 0.2 0 1 2 3\nend\n",
+           "expected key 'particle'");
+    expect(head + "particle_count 0\nrecords\nend\n", "count is zero");
+    expect("AuroraPIC-particle-state-v1\ndimension 1\nunits normalized\n"
+           "weighting species_constant\nvelocity_staggering time_centered\nparticle_count 1\n"
+           "records\nparticle e 0.1 0 0 1 2 0\nend\n",
+           "1D1V external particle state has nonzero inactive components");
+}
+
+struct CpuCase {
+    const char* name;
+    void (*body)(const std::filesystem::path&);
+};
+
+const CpuCase kCpuCases[] = {
+    {"aps_reference_example_signature", aps_reference_example_signature},
+    {"aps_round_trip", aps_round_trip},
+    {"aps_errors", aps_errors},
+};
+
+// The 3D simulation initialised from a state file equals the same records
+// set through set_particles; the reference's domain and 1D3V checks.
+void simulation3d_external_state(const Dev& d) {
+    Simulation3DConfig cfg;
+    cfg.nx = cfg.ny = cfg.nz = 16;
+    cfg.dt = 0.02;
+    cfg.species = {Species3DConfig{"electrons", -1.0, 1.0, 1e-3, 300, {}, 0.0},
+                   Species3DConfig{"ions", 1.0, 100.0, 1e-3, 200, {}, 0.0}};
+    ExternalParticleState st;
+    st.spatial_dimension = 3;
+    std::mt19937_64 rng(7);
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    std::normal_distribution<double> g(0.0, 0.3);
+    for (const auto& s : cfg.species)
+        for (size_t i = 0; i < s.particles; ++i)
+            st.species[s.name].push_back({{u(rng), u(rng), u(rng)}, {g(rng), g(rng), g(rng)}});
+    st.particle_count = 500;
+    st.signature = external_particle_state_signature(st);
+    const auto path = scratch_dir() / "state3d.aps";
+    write_external_particle_state(path, st, true);
+    auto a_cfg = cfg;
+    a_cfg.initial_state_path = path;
+    a_cfg.initial_state_signature = st.signature;
+    Simulation3D a(a_cfg, d);
+    a.run();
+    Simulation3D b(cfg, std::make_shared<Device>(0));
+    for (size_t k = 0; k < cfg.species.size(); ++k) {
+        std::vector<Particle3D> ps;
+        for (const auto& r : st.species.at(cfg.species[k].name))
+            ps.push_back({r.position, r.velocity, true, {}});
+        b.set_particles(k, ps);
+    }
+    b.run();
+    for (size_t k = 0; k < cfg.species.size(); ++k) {
+        const auto pa = a.particles(k), pb = b.particles(k);
+        require(pa.size() == pb.size(), "population");
+        for (size_t i = 0; i < pa.size(); ++i)
+            require(pa[i].position.x == pb[i].position.x && pa[i].velocity_half.z ==
+                    pb[i].velocity_half.z, "state-file run differs from set_particles run");
+    }
+    st.species.at("ions")[0].position.x = 1.0;   // periodic upper end
+    st.signature = external_particle_state_signature(st);
+    write_external_particle_state(path, st, true);
+    require_throws_contains(
+        [&] {
+            auto c = cfg;
+            c.initial_state_path = path;
+            Simulation3D s(c, d);
+            s.initialize();
+        },
+        "external particle for species 'ions' lies outside the 3D domain", "domain check");
+    require_throws_contains(
+        [&] {
+            auto c = cfg;
+            c.initial_state_path = path;
+            c.initial_state_signature = st.signature + 1;
+            Simulation3D s(c, d);
+            s.initialize();
+        },
+        "signature mismatch", "pinned signature");
+    Config one;
+    one.initial_state_path = path;
+    require_throws_contains([&] { Simulation s(one, d); },
+                            "not yet available for 1D3V", "1D3V external state");
+}
+
 struct Case {
     const char* name;
     void (*body)(const Dev&);
@@ -359,6 +573,7 @@ const Case kCases[] = {
     {"swarm_zero_cross_section", swarm_zero_cross_section},
     {"swarm_alone_equals_batched", swarm_alone_equals_batched},
     {"error_paths", error_paths},
+    {"simulation3d_external_state", simulation3d_external_state},
 };
 
 }  // namespace
@@ -367,7 +582,33 @@ int main(int argc, char** argv) {
     const std::string arg = argc > 1 ? argv[1] : "";
     if (arg == "--list") {
         for (const auto& c : kCases) std::printf("%s\n", c.name);
+        for (const auto& c : kCpuCases) std::printf("%s\n", c.name);
         return 0;
+    }
+    if (arg == "--aps-signature" && argc > 2) {
+        try {
+            const auto s = load_external_particle_state(argv[2], std::size_t{1} << 40);
+            std::printf("%llu\n", static_cast<unsigned long long>(s.signature));
+        } catch (const std::exception& e) {
+            std::printf("error: %s\n", e.what());
+        }
+        return 0;
+    }
+    if (arg == "--cpu") {
+        const std::filesystem::path golden = argc > 2 ? argv[2] : "tests/golden";
+        int passed = 0, failed = 0;
+        for (const auto& c : kCpuCases) {
+            try {
+                c.body(golden);
+                std::printf("PASS %s\n", c.name);
+                ++passed;
+            } catch (const std::exception& e) {
+                std::printf("FAIL %s: %s\n", c.name, e.what());
+                ++failed;
+            }
+        }
+        std::printf("%d passed, %d failed\n", passed, failed);
+        return failed ? 1 : 0;
     }
     int passed = 0, failed = 0;
     for (const auto& c : kCases) {

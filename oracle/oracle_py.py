@@ -442,6 +442,7 @@ class Reference:
                                 C.c_double, C.c_double, C.c_double, C.c_uint64, _dp,
                                 C.c_char_p, C.c_int]
         L.ref_swarm_file.argtypes = [C.c_char_p, C.c_int, _dp, C.c_char_p, C.c_int]
+        L.ref_aps_signature.argtypes = [C.c_char_p, _u64p, C.c_char_p, C.c_int]
         L.ref_sim3d_time.argtypes = [_i64p, _dp, C.c_double, C.c_double, C.c_int, C.c_int,
                                      _dp, C.c_int64, _dp, _dp, C.c_int, _dp, _i64p,
                                      C.c_char_p, C.c_int]
@@ -556,6 +557,15 @@ class Reference:
         if n < 0:
             raise RuntimeError(err.value.decode())
         return out[:n]
+
+    def aps_signature(self, path):
+        """(signature, None) from the reference's .aps reader, or (None, its
+        exception text)."""
+        sig = C.c_uint64()
+        err = C.create_string_buffer(1024)
+        if self.lib.ref_aps_signature(os.fsencode(path), C.byref(sig), err, 1024):
+            return None, err.value.decode()
+        return sig.value, None
 
     def sim3d_time(self, n, L, dt, eps_rel, si, species, btable, steps):
         n, L = i64(n), f64(L)

@@ -10,6 +10,7 @@
 #include "pic/GasDataset.hpp"
 #include "pic/Mesh2D.hpp"
 #include "pic/Mesh3D.hpp"
+#include "pic/ParticleState.hpp"
 #include "pic/PrescribedField.hpp"
 #include "pic/Pusher.hpp"
 #include "pic/Simulation3D.hpp"
@@ -387,6 +388,19 @@ int ref_swarm_file(const char* swarm_path, int max_fields, double* out, char* er
                 o[12 + c] = static_cast<double>(r.channels[c].collisions);
         }
         return n;
+    } catch (const std::exception& e) {
+        set_error(err, errlen, e.what());
+        return -1;
+    }
+}
+
+// The reference's own .aps reader on a file: its canonical signature, or
+// its exception text (return -1).
+int ref_aps_signature(const char* path, uint64_t* signature, char* err, int errlen) {
+    try {
+        const auto state = pic::load_external_particle_state(path, std::size_t{1} << 40);
+        *signature = state.signature;
+        return 0;
     } catch (const std::exception& e) {
         set_error(err, errlen, e.what());
         return -1;

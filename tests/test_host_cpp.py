@@ -27,7 +27,9 @@ def test_host_library_exports_the_classes():
     for sym in ["pic::gpu::FieldSolver::solve", "pic::gpu::NullCollisionModel::collide",
                 "pic::gpu::Simulation3D::step", "pic::gpu::Simulation::step",
                 "pic::gpu::Simulation2D::step", "pic::gpu::SwarmService::submit",
-                "pic::gpu::deposit_charge_cic"]:
+                "pic::gpu::deposit_charge_cic", "pic::gpu::load_external_particle_state",
+                "pic::gpu::write_external_particle_state",
+                "pic::gpu::external_particle_state_signature"]:
         assert sym in out, sym
 
 
