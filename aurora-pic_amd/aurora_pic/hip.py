@@ -70,6 +70,8 @@ class SwarmResult(C.Structure):
         ("nulls", C.c_uint64),
         ("max_energy_ev", C.c_double),
         ("status", C.c_int),
+        ("longitudinal_diffusion", C.c_double),
+        ("transverse_diffusion", C.c_double),
     ]
 
 

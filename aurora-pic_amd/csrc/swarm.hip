@@ -12,7 +12,10 @@
 // counters;
 // integer sums are associative, so a request's result does not depend on
 // which other requests share its launches or on how its steps are split
-// into launches.  Positions are not tracked (diffusion is not reported).
+// into launches.  Each electron's displacement since the end of burn-in is
+// kept (x += v dt after the first half kick, Swarm.cpp:1026-1031); after
+// the last step every chunk reduces its displacement moments in FP64 in a
+// fixed order, for the diffusion coefficients (Swarm.cpp:1139-1176).
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -34,6 +37,8 @@ struct RequestDesc {
     float* vx;
     float* vy;
     float* vz;
+    float* dx;                    // displacement since burn-in (3 x particles)
+    double* moments;              // [chunk][5]: sums of dx, dy, dz, dx^2, dy^2 + dz^2
     long long* part;              // [chunk][block][2] fixed-point sums of v_x, energy
     unsigned long long* counts;   // [candidates, nulls, chan...]
     unsigned* max_energy_bits;
@@ -66,6 +71,7 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
     __shared__ int item_s;
     __shared__ long long red_v[kChunk];
     __shared__ long long red_e[kChunk];
+    __shared__ double red_m[5][kChunk];
     const XsView X = stage_tables(m, lds);
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(head, 1);
@@ -78,7 +84,13 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
         const int64_t p = static_cast<int64_t>(w.chunk) * kChunk + threadIdx.x;
         const bool active = p < d.particles;
         float v[3] = {0.f, 0.f, 0.f};
+        float x[3] = {0.f, 0.f, 0.f};
         if (active) { v[0] = d.vx[p]; v[1] = d.vy[p]; v[2] = d.vz[p]; }
+        if (active && d.step0 > d.burn_in) {
+            x[0] = d.dx[p];
+            x[1] = d.dx[d.particles + p];
+            x[2] = d.dx[2 * d.particles + p];
+        }
         uint32_t cand = 0, nulls = 0, chan[NCH] = {};
         uint32_t err = 0;
         float emax = 0.f;
@@ -92,6 +104,11 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
                 const float e_pre = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
                 emax = fmaxf(emax, e_pre);
                 if (e_pre > d.max_energy_ev) err |= DEV_ERR_ENERGY;
+                if (sampling) {
+                    x[0] += v[0] * d.dt;
+                    x[1] += v[1] * d.dt;
+                    x[2] += v[2] * d.dt;
+                }
                 Philox rng;
                 rng.init(d.seed, 0u, static_cast<uint32_t>(p), static_cast<uint32_t>(s));
                 MccTally<NCH> t;
@@ -162,6 +179,31 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
             }
             __syncthreads();
         }
+        const bool last = d.step0 + d.nsteps == d.burn_in + d.sampling;
+        if (last) {
+            // displacement moments of the chunk, reduced in a fixed order
+            const double m0 = active ? static_cast<double>(x[0]) : 0.0;
+            const double m1 = active ? static_cast<double>(x[1]) : 0.0;
+            const double m2 = active ? static_cast<double>(x[2]) : 0.0;
+            red_m[0][threadIdx.x] = m0;
+            red_m[1][threadIdx.x] = m1;
+            red_m[2][threadIdx.x] = m2;
+            red_m[3][threadIdx.x] = m0 * m0;
+            red_m[4][threadIdx.x] = m1 * m1 + m2 * m2;
+            __syncthreads();
+            for (int st = kChunk / 2; st > 0; st >>= 1) {
+                if (threadIdx.x < st)
+                    for (int k = 0; k < 5; ++k) red_m[k][threadIdx.x] += red_m[k][threadIdx.x + st];
+                __syncthreads();
+            }
+            if (threadIdx.x < 5) d.moments[static_cast<int64_t>(w.chunk) * 5 + threadIdx.x] =
+                red_m[threadIdx.x][0];
+            __syncthreads();
+        } else if (active && d.step0 + d.nsteps > d.burn_in) {
+            d.dx[p] = x[0];
+            d.dx[d.particles + p] = x[1];
+            d.dx[2 * d.particles + p] = x[2];
+        }
         if (active) {
             d.vx[p] = v[0];
             d.vy[p] = v[1];
@@ -202,6 +244,8 @@ struct RequestState {
     int64_t steps_done = 0;
     uint64_t footprint = 0;
     DevBuf<float> v;                       // 3 x particles (SoA)
+    DevBuf<float> dx;                      // 3 x particles displacement (SoA)
+    DevBuf<double> moments;                // chunks x 5
     DevBuf<long long> part;
     DevBuf<unsigned long long> counts;
     DevBuf<unsigned> maxe;
@@ -228,8 +272,8 @@ SwarmState& state(Context& c) {
 
 uint64_t footprint_of(const aurora_hip_swarm_request& r) {
     const uint64_t chunks = (static_cast<uint64_t>(r.particles) + kChunk - 1) / kChunk;
-    return static_cast<uint64_t>(r.particles) * 12u + chunks * static_cast<uint64_t>(r.blocks) * 16u +
-           (kMaxChannels + 3) * 8u + 64u;
+    return static_cast<uint64_t>(r.particles) * 24u + chunks * static_cast<uint64_t>(r.blocks) * 16u +
+           chunks * 40u + (kMaxChannels + 3) * 8u + 64u;
 }
 
 MccParams swarm_params(const SwarmState& s) { return mcc_params_of(s.gas); }
@@ -277,6 +321,28 @@ void finalize(Context& c, SwarmState& s, int64_t handle, RequestState& r) {
     AUR_HIP(hipMemcpy(counts, r.counts.ptr, sizeof counts, hipMemcpyDeviceToHost));
     AUR_HIP(hipMemcpy(&maxe, r.maxe.ptr, sizeof maxe, hipMemcpyDeviceToHost));
     AUR_HIP(hipMemcpy(&status, r.status.ptr, sizeof status, hipMemcpyDeviceToHost));
+    std::vector<double> mom(static_cast<size_t>(r.chunks * 5));
+    AUR_HIP(hipMemcpy(mom.data(), r.moments.ptr, mom.size() * sizeof(double),
+                      hipMemcpyDeviceToHost));
+    {
+        // Swarm.cpp:1139-1176: variance of the displacement over the
+        // sampling window; D_L = var_x / 2T, D_T = var_yz / 4T
+        double sx = 0, sy = 0, sz = 0, sxx = 0, syz = 0;
+        for (int64_t ch = 0; ch < r.chunks; ++ch) {   // fixed chunk order
+            sx += mom[ch * 5];
+            sy += mom[ch * 5 + 1];
+            sz += mom[ch * 5 + 2];
+            sxx += mom[ch * 5 + 3];
+            syz += mom[ch * 5 + 4];
+        }
+        const double n = static_cast<double>(N);
+        const double mx = sx / n, my = sy / n, mz = sz / n;
+        const double var_x = sxx / n - mx * mx;
+        const double var_yz = syz / n - my * my - mz * mz;
+        const double T = static_cast<double>(S) * q.timestep;
+        out.longitudinal_diffusion = var_x / (2.0 * T);
+        out.transverse_diffusion = var_yz / (4.0 * T);
+    }
     std::vector<double> vm(static_cast<size_t>(B)), em(static_cast<size_t>(B));
     for (int64_t b = 0; b < B; ++b) {
         double sv = 0.0, se = 0.0;
@@ -420,6 +486,8 @@ int aurora_hip_swarm_submit(aurora_hip_ctx* h, const aurora_hip_swarm_request* r
         st->chunks = (r->particles + kChunk - 1) / kChunk;
         st->footprint = fp;
         st->v.alloc(static_cast<size_t>(3 * r->particles));
+        st->dx.alloc(static_cast<size_t>(3 * r->particles));
+        st->moments.alloc(static_cast<size_t>(st->chunks * 5));
         st->part.alloc(static_cast<size_t>(st->chunks * r->blocks * 2));
         st->counts.alloc(kMaxChannels + 3);
         st->maxe.alloc(1);
@@ -461,6 +529,8 @@ int aurora_hip_swarm_advance(aurora_hip_ctx* h, int64_t max_steps, int64_t* acti
             d.vx = r.v.ptr;
             d.vy = r.v.ptr + n;
             d.vz = r.v.ptr + 2 * n;
+            d.dx = r.dx.ptr;
+            d.moments = r.moments.ptr;
             d.part = r.part.ptr;
             d.counts = r.counts.ptr;
             d.max_energy_bits = r.maxe.ptr;

@@ -302,6 +302,8 @@ typedef struct aurora_hip_swarm_result {
     uint64_t nulls;
     double max_energy_ev;
     int status;                   /* aurora_status of the request */
+    double longitudinal_diffusion; /* m^2/s, var(dx) / 2T over the sampling window */
+    double transverse_diffusion;   /* m^2/s, var(dy, dz) / 4T (Swarm.cpp:1139-1176) */
 } aurora_hip_swarm_result;
 
 /* The gas of every swarm request on this context (GasDataset +

@@ -322,10 +322,13 @@ static double kinetic_energy_ev(const double* v) {
 void or_swarm_run(const or_mcc_model* m, const or_swarm_cfg* c,
                   const double* vel0, uint64_t seed, uint32_t instance,
                   or_swarm_result* r) {
-    /* Swarm.cpp:940-1250, fixed population, no position tracking */
+    /* Swarm.cpp:940-1250, fixed population; positions tracked as the
+     * displacement since the start of sampling (Swarm.cpp:999-1003,
+     * 1026-1031), for the diffusion coefficients (1139-1176) */
     memset(r, 0, sizeof *r);
     const int64_t np = c->particles;
     double* v = (double*)malloc((size_t)(3 * np) * sizeof(double));
+    double* x = (double*)calloc((size_t)(3 * np), sizeof(double));
     memcpy(v, vel0, (size_t)(3 * np) * sizeof(double));
     const double efield = c->reduced_field_td * TOWNSEND * c->neutral_density;
     const double accel = -ELEMENTARY_CHARGE * efield / ELECTRON_MASS;
@@ -342,6 +345,8 @@ void or_swarm_run(const or_mcc_model* m, const or_swarm_cfg* c,
             double e_pre = kinetic_energy_ev(u);
             if (e_pre > r->max_energy_ev) r->max_energy_ev = e_pre;
             if (e_pre > c->max_energy_ev) { r->status = 4; goto done; }
+            if (step >= c->burn_in)
+                for (int a = 0; a < 3; ++a) x[3 * p + a] += u[a] * c->timestep;
             or_stream rng;
             or_stream_init(&rng, seed, instance, (uint32_t)p, (uint64_t)step);
             or_mcc_stats st;
@@ -377,6 +382,22 @@ void or_swarm_run(const or_mcc_model* m, const or_swarm_cfg* c,
         r->drift_se = sev;
         r->mean_energy_ev = me;
         r->mean_energy_se = see;
+        /* two-pass variance of the displacement, as Swarm.cpp:1139-1176 */
+        double mean[3] = {0.0, 0.0, 0.0}, var_x = 0.0, var_yz = 0.0;
+        for (int64_t p = 0; p < np; ++p)
+            for (int a = 0; a < 3; ++a) mean[a] += x[3 * p + a];
+        for (int a = 0; a < 3; ++a) mean[a] /= (double)np;
+        for (int64_t p = 0; p < np; ++p) {
+            const double dx = x[3 * p] - mean[0], dy = x[3 * p + 1] - mean[1],
+                         dz = x[3 * p + 2] - mean[2];
+            var_x += dx * dx;
+            var_yz += dy * dy + dz * dz;
+        }
+        var_x /= (double)np;
+        var_yz /= (double)np;
+        const double T = (double)sampling * c->timestep;
+        r->longitudinal_diffusion = var_x / (2.0 * T);
+        r->transverse_diffusion = var_yz / (4.0 * T);
         for (int ch = 0; ch < m->nchan; ++ch) {
             r->rate[ch] = weighted[ch] / exposure;
             r->rate_se[ch] = sqrt(weighted[ch]) / exposure;
@@ -387,5 +408,5 @@ void or_swarm_run(const or_mcc_model* m, const or_swarm_cfg* c,
         }
     }
 done:
-    free(v); free(vs); free(es);
+    free(v); free(x); free(vs); free(es);
 }

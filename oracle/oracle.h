@@ -184,6 +184,7 @@ typedef struct {
     uint64_t collisions[8], candidates, nulls;
     double max_energy_ev;
     int status;                   /* 0 ok; 4 energy above max; else MCC */
+    double longitudinal_diffusion, transverse_diffusion;  /* m^2/s */
 } or_swarm_result;
 /* initial velocities vel0 (particles*3) are inputs; RNG stream per particle
  * step keyed (seed, stream=instance, uid=particle index, step). */

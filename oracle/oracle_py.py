@@ -126,6 +126,8 @@ class _SwarmResult(C.Structure):
         ("nulls", C.c_uint64),
         ("max_energy_ev", C.c_double),
         ("status", C.c_int),
+        ("longitudinal_diffusion", C.c_double),
+        ("transverse_diffusion", C.c_double),
     ]
 
 
@@ -441,7 +443,7 @@ class Reference:
                                 C.c_double, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
                                 C.c_double, C.c_double, C.c_double, C.c_uint64, _dp,
                                 C.c_char_p, C.c_int]
-        L.ref_swarm_file.argtypes = [C.c_char_p, C.c_int, _dp, C.c_char_p, C.c_int]
+        L.ref_swarm_file.argtypes = [C.c_char_p, C.c_int, _dp, C.c_char_p, C.c_char_p, C.c_int]
         L.ref_aps_signature.argtypes = [C.c_char_p, _u64p, C.c_char_p, C.c_int]
         L.ref_sim3d_time.argtypes = [_i64p, _dp, C.c_double, C.c_double, C.c_int, C.c_int,
                                      _dp, C.c_int64, _dp, _dp, C.c_int, _dp, _i64p,
@@ -546,14 +548,16 @@ class Reference:
                        "mean_energy_ev", "mean_energy_se", "ionization_rate",
                        "ionization_rate_se", "max_energy_ev", "candidates", "nulls",
                        "neutral_vstd", "collisions0", "collisions1", "collisions2",
-                       "collisions3")
+                       "collisions3", "longitudinal_diffusion", "transverse_diffusion")
 
-    def swarm_file(self, path, max_fields=64):
+    def swarm_file(self, path, max_fields=64, csv_path=""):
         """A .swarm file through the reference's own loader and driver; one
-        row of SWARM_FILE_COLS per reduced field."""
+        row of SWARM_FILE_COLS per reduced field (csv_path: also write the
+        reference's result CSV there)."""
         out = np.zeros((max_fields, len(self.SWARM_FILE_COLS)))
         err = C.create_string_buffer(512)
-        n = self.lib.ref_swarm_file(os.fsencode(path), max_fields, _ptr(out, _dp), err, 512)
+        n = self.lib.ref_swarm_file(os.fsencode(path), max_fields, _ptr(out, _dp),
+                                    os.fsencode(csv_path), err, 512)
         if n < 0:
             raise RuntimeError(err.value.decode())
         return out[:n]

@@ -355,18 +355,23 @@ int ref_swarm(int nchan, const int* kind, const double* threshold,
 
 // A .swarm benchmark file through the reference's own loader and driver
 // (load_swarm_benchmark_config, Swarm.cpp:1252; run_swarm_benchmark,
-// Swarm.cpp:1513).  The CSV goes to a temporary directory.  Per field, out
+// Swarm.cpp:1513).  csv_path (may be empty) receives the reference's own
+// result CSV (write_swarm_benchmark_csv).  Per field, out
 // receives kSwarmFileCols values: reduced field, drift, drift se, mobility,
 // mean energy, its se, ionization rate, its se, max observed energy,
-// candidates, nulls, neutral velocity stddev, then collisions per channel
-// (up to 4, zero-filled).  Returns the number of fields written.
-int ref_swarm_file(const char* swarm_path, int max_fields, double* out, char* err,
-                   int errlen) {
-    constexpr int kCols = 16;
+// candidates, nulls, neutral velocity stddev, collisions per channel (up to
+// 4, zero-filled), longitudinal and transverse diffusion.  Returns the number
+// of fields written.
+int ref_swarm_file(const char* swarm_path, int max_fields, double* out, const char* csv_path,
+                   char* err, int errlen) {
+    constexpr int kCols = 18;
     try {
         auto cfg = pic::load_swarm_benchmark_config(swarm_path);
         cfg.output_file = temp_dir() / "swarm_file.csv";
         auto results = pic::run_swarm_benchmark(cfg);
+        if (csv_path && *csv_path)
+            pic::write_swarm_benchmark_csv(csv_path, cfg,
+                                           pic::load_gas_dataset(cfg.gas_data_file), results);
         const int n = static_cast<int>(std::min<size_t>(results.size(), max_fields));
         for (int f = 0; f < n; ++f) {
             const auto& r = results[f];
@@ -386,6 +391,8 @@ int ref_swarm_file(const char* swarm_path, int max_fields, double* out, char* er
             o[11] = r.neutral_velocity_stddev_m_s;
             for (size_t c = 0; c < r.channels.size() && c < 4; ++c)
                 o[12 + c] = static_cast<double>(r.channels[c].collisions);
+            o[16] = r.longitudinal_diffusion_m2_s;
+            o[17] = r.transverse_diffusion_m2_s;
         }
         return n;
     } catch (const std::exception& e) {

@@ -5,7 +5,9 @@ Runs the reference's own ``load_swarm_benchmark_config`` and
 oracle/Makefile) on tests/golden/swarm/synthetic_swarm.swarm -- the
 reference's examples/synthetic_swarm.swarm data files, kept here as
 fixtures: 256 electrons, 1/5/10 Td, 300 K neutrals, Henyey-Greenstein
-elastic scattering.  Because the device's random stream differs from
+elastic scattering.  The reference's own result CSV of that run is kept
+beside it (synthetic_swarm_reference.csv) for the column-by-column check of
+the CSV writer.  Because the device's random stream differs from
 std::mt19937_64, the fixture also records the spread of the reference
 itself over independent seeds, which the statistical parity test uses.
 
@@ -43,7 +45,8 @@ def with_seed(src_dir, seed):
 
 def main():
     ref = oracle_py.Reference()
-    example = ref.swarm_file(SWARM)
+    csv = os.path.join(HERE, "swarm", "synthetic_swarm_reference.csv")
+    example = ref.swarm_file(SWARM, csv_path=csv)
     runs = []
     for k in range(SEEDS):
         tmp, path = with_seed(os.path.dirname(SWARM), 1_000_003 * (k + 1))

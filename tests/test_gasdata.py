@@ -177,14 +177,15 @@ def example_model():
 
 def check_against_reference(results, tolerance_sigma=4.0):
     """results: per field (drift, mean energy, elastic collisions,
-    candidates).  The reference's spread over 16 seeds sets the scale; the
+    candidates, longitudinal and transverse diffusion).  The reference's spread over 16 seeds sets the scale; the
     mean of 16 runs carries its own 1/sqrt(16) error."""
     g = golden("swarm/synthetic_swarm_golden.npz")
     cols = list(g["columns"])
     mean, std = g["seed_mean"], g["seed_std"]
     n = int(g["seeds"])
     for f, got in enumerate(results):
-        for name, value in zip(("drift", "mean_energy_ev", "collisions0", "candidates"), got):
+        for name, value in zip(("drift", "mean_energy_ev", "collisions0", "candidates",
+                                "longitudinal_diffusion", "transverse_diffusion"), got):
             c = cols.index(name)
             scale = std[f, c] * math.sqrt(1.0 + 1.0 / n)
             assert abs(value - mean[f, c]) <= tolerance_sigma * scale, \
@@ -206,7 +207,8 @@ def test_oracle_swarm_matches_reference_example(oracle):
                  max_energy_ev=cfg.max_energy_ev)
         r = oracle.swarm(model, c, vel0, cfg.seed + f, 0)
         assert r.status == 0
-        results.append((r.drift, r.mean_energy_ev, r.collisions[0], r.candidates))
+        results.append((r.drift, r.mean_energy_ev, r.collisions[0], r.candidates,
+                        r.longitudinal_diffusion, r.transverse_diffusion))
     check_against_reference(results)
 
 
@@ -215,3 +217,73 @@ def test_neutral_temperature_needs_si_dataset():
     ds = gasdata.load_gas_dataset(os.path.join(GAS_DIR, "synthetic_ionization.gas"))
     with pytest.raises(gasdata.GasDataError, match="requires an SI gas dataset"):
         gasdata.swarm_gas(ds, cfg)
+
+
+# ------------------------------------------------------------------ CSV
+def _reference_csv():
+    import csv
+    with open(os.path.join(SWARM_DIR, "synthetic_swarm_reference.csv")) as f:
+        rows = list(csv.reader(f))
+    return rows[0], rows[1:]
+
+
+def _result_from_reference_row(head, row, nchan):
+    """The statistics of one reference CSV row in the device result's shape."""
+    from types import SimpleNamespace
+    col = {name: row[i] for i, name in enumerate(head)}
+    chans = [n[:-len("_collisions")] for n in head if n.endswith("_collisions")
+             and n != "collision_candidates" and n != "null_collisions"]
+    return SimpleNamespace(
+        drift=float(col["electron_drift_velocity_m_s"]),
+        drift_se=float(col["mean_velocity_x_standard_error_m_s"]),
+        mean_energy_ev=float(col["mean_energy_ev"]),
+        mean_energy_se=float(col["mean_energy_standard_error_ev"]),
+        longitudinal_diffusion=float(col["longitudinal_diffusion_m2_s"]),
+        transverse_diffusion=float(col["transverse_diffusion_m2_s"]),
+        max_energy_ev=float(col["maximum_observed_energy_ev"]),
+        candidates=int(col["collision_candidates"]), nulls=int(col["null_collisions"]),
+        collisions=[int(col[f"{c}_collisions"]) for c in chans],
+        rate=[float(col[f"{c}_rate_per_electron_s"]) for c in chans],
+        rate_se=[float(col[f"{c}_rate_standard_error_s"]) for c in chans])
+
+
+def _strip_paths(cells, head):
+    out = list(cells)
+    for name in ("gas_data_file", "spatial_profile_file"):
+        i = head.index(name)
+        out[i] = os.path.basename(out[i])
+    return out
+
+
+def test_csv_writer_reproduces_reference_csv(tmp_path):
+    """Given the reference run's own statistics, the writer reproduces the
+    reference's CSV cell for cell: header, run description, the collision
+    model signature (7215415709351068229), derived fields (E, mobility,
+    rates, errors) and the fixed-population columns."""
+    from aurora_pic import swarmio
+    head, body = _reference_csv()
+    cfg = gasdata.load_swarm_config(os.path.join(SWARM_DIR, "synthetic_swarm.swarm"))
+    ds = gasdata.load_gas_dataset(cfg.gas_data_file)
+    assert swarmio.header(ds) == head
+    assert swarmio.collision_model_signature(ds, cfg) == 7215415709351068229
+    results = [_result_from_reference_row(head, row, len(ds.channels)) for row in body]
+    out = str(tmp_path / "swarm.csv")
+    swarmio.write_swarm_benchmark_csv(out, cfg, ds, results)
+    import csv
+    with open(out) as f:
+        mine = list(csv.reader(f))
+    assert mine[0] == head
+    for got, want in zip(mine[1:], body):
+        assert _strip_paths(got, head) == _strip_paths(want, head)
+
+
+def test_model_signature_tracks_physics():
+    from aurora_pic import swarmio
+    cfg = gasdata.load_swarm_config(os.path.join(SWARM_DIR, "synthetic_swarm.swarm"))
+    ds = gasdata.load_gas_dataset(cfg.gas_data_file)
+    base = swarmio.collision_model_signature(ds, cfg)
+    cfg.neutral_temperature = 0.0
+    assert swarmio.collision_model_signature(ds, cfg) != base
+    cfg.neutral_temperature = 300.0
+    ds.channels[0].cross_section = ds.channels[0].cross_section * (1 + 1e-15)
+    assert swarmio.collision_model_signature(ds, cfg) != base

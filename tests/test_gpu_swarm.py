@@ -91,6 +91,11 @@ def test_swarm_matches_oracle_same_stream(ctx, oracle):
         assert abs(mine - theirs) <= 4 * math.hypot(se, tse) + 0.01 * abs(theirs)
     assert abs(r.ionization_rate - o.ion_rate) <= 5 * math.hypot(r.ionization_rate_se,
                                                                  o.ion_rate_se) + 1.0
+    # the same displacement histories: diffusion within 5 % (a few FP32/FP64
+    # branch flips change single trajectories)
+    for mine, theirs in [(r.longitudinal_diffusion, o.longitudinal_diffusion),
+                         (r.transverse_diffusion, o.transverse_diffusion)]:
+        assert theirs > 0 and abs(mine - theirs) <= 0.05 * theirs, (mine, theirs)
 
 
 def test_swarm_matches_reference_golden(ctx):
@@ -165,5 +170,44 @@ def test_reference_swarm_example_through_readers(ctx):
     gasdata.apply_channel_options(ctx, ds, model=1)
     results = _run_all(ctx, gasdata.swarm_requests(cfg), 500)
     assert all(r.status == 0 for r in results)
-    check_against_reference([(r.drift, r.mean_energy_ev, r.collisions[0], r.candidates)
+    check_against_reference([(r.drift, r.mean_energy_ev, r.collisions[0], r.candidates,
+                              r.longitudinal_diffusion, r.transverse_diffusion)
+                             for r in results])
+
+
+def test_reference_swarm_example_csv(ctx, tmp_path):
+    """The device run of the reference's example written in the reference's
+    CSV schema: every column that describes the run (dataset, config, seeds,
+    collision model signature, field, fixed-population columns) equals the
+    reference's own CSV; the statistics agree within 4 sigma of the
+    reference's seed spread (checked by the previous test's helper)."""
+    import csv
+    import os
+    from aurora_pic import gasdata, swarmio
+    from test_gasdata import SWARM_DIR, _reference_csv, _strip_paths, check_against_reference
+    cfg = gasdata.load_swarm_config(os.path.join(SWARM_DIR, "synthetic_swarm.swarm"))
+    ds = gasdata.load_gas_dataset(cfg.gas_data_file)
+    ctx.swarm_gas(gasdata.swarm_gas(ds, cfg))
+    gasdata.apply_channel_options(ctx, ds, model=1)
+    results = _run_all(ctx, gasdata.swarm_requests(cfg), 700)
+    out = str(tmp_path / "swarm.csv")
+    swarmio.write_swarm_benchmark_csv(out, cfg, ds, results)
+    head, ref_rows = _reference_csv()
+    with open(out) as f:
+        mine = list(csv.reader(f))
+    assert mine[0] == head
+    statistical = {"mean_velocity_x_m_s", "mean_velocity_x_standard_error_m_s",
+                   "electron_drift_velocity_m_s", "reduced_mobility_1_v_m_s", "mean_energy_ev",
+                   "mean_energy_standard_error_ev", "longitudinal_diffusion_m2_s",
+                   "transverse_diffusion_m2_s", "maximum_observed_energy_ev",
+                   "collision_candidates", "null_collisions"}
+    statistical |= {h for h in head if h.endswith(("_collisions", "_rate_per_electron_s",
+                                                   "_rate_standard_error_s"))}
+    for got, want in zip(mine[1:], ref_rows):
+        g, w = _strip_paths(got, head), _strip_paths(want, head)
+        for name, a, b in zip(head, g, w):
+            if name not in statistical:
+                assert a == b, (name, a, b)
+    check_against_reference([(r.drift, r.mean_energy_ev, r.collisions[0], r.candidates,
+                              r.longitudinal_diffusion, r.transverse_diffusion)
                              for r in results])
