@@ -128,6 +128,9 @@ def load(path: str = LIB_PATH):
         "aurora_hip_set_sort_interval": [ctx, C.c_int64],
         "aurora_hip_set_sort_threshold": [ctx, C.c_double],
         "aurora_hip_set_sort_policy": [ctx, C.c_int, C.c_int64],
+        "aurora_hip_checkpoint_save": [ctx, C.c_char_p],
+        "aurora_hip_checkpoint_load": [ctx, C.c_char_p],
+        "aurora_hip_physics_signature": [ctx, _u64p],
         "aurora_hip_sort": [ctx],
         "aurora_hip_set_tiling": [ctx, C.c_int, C.c_int, C.c_int, C.c_int],
         "aurora_hip_set_push_variant": [ctx, C.c_int, C.c_int],
@@ -377,6 +380,17 @@ class Context:
 
     def sort_threshold(self, fraction):
         self._check(self.lib.aurora_hip_set_sort_threshold(self.h, fraction))
+
+    def checkpoint_save(self, path):
+        self._check(self.lib.aurora_hip_checkpoint_save(self.h, os.fsencode(path)))
+
+    def checkpoint_load(self, path):
+        self._check(self.lib.aurora_hip_checkpoint_load(self.h, os.fsencode(path)))
+
+    def physics_signature(self):
+        v = C.c_uint64()
+        self._check(self.lib.aurora_hip_physics_signature(self.h, C.byref(v)))
+        return v.value
 
     def sort_policy(self, policy, max_interval=0):
         """SORT_FIXED (every sort_interval steps) or SORT_ADAPTIVE (per

@@ -357,6 +357,10 @@ void ensure_rho(Context& ctx);
 void ensure_fields(Context& ctx, bool phi, bool e64);
 // some species is pushed in FP64 (reads E64)
 bool any_fp64(const Context& ctx);
+// electrode potential offset + amplitude sin(2 pi f t + phase) (Simulation.cpp:678-685)
+double electrode(const double* p, double t);
+// hash of the configured physics (checkpoint guard)
+uint64_t physics_signature(const Context& ctx);
 void prepare_solver(Context& ctx);
 void deposit_all(Context& ctx);
 void convert_rho(Context& ctx);

@@ -229,6 +229,20 @@ enum { AURORA_SORT_FIXED = 0, AURORA_SORT_ADAPTIVE = 1 };
 int aurora_hip_set_sort_policy(aurora_hip_ctx* ctx, int policy, int64_t max_interval);
 int aurora_hip_sort(aurora_hip_ctx* ctx);
 
+/* ----------------------------------------------------------- checkpoint */
+/* Checkpoint / restart of the context's device state (the reference's
+ * restart files, Simulation3D.cpp:367-469, Simulation.cpp:1812-2900, guarded
+ * by a physics signature like Collision.cpp:365-577): a versioned binary
+ * dump of time, step, every species' particles as stored, boundary and MCC
+ * counters.  Loading requires a context configured identically (mesh,
+ * species, magnetic field, particle boundaries, electrodes, MCC model and
+ * seed), else the physics-signature mismatch error; it then deposits and
+ * solves, so continuing reproduces the uninterrupted run to the bit.
+ * Batched swarms and ensembles are not included. */
+int aurora_hip_checkpoint_save(aurora_hip_ctx* ctx, const char* path);
+int aurora_hip_checkpoint_load(aurora_hip_ctx* ctx, const char* path);
+int aurora_hip_physics_signature(aurora_hip_ctx* ctx, uint64_t* signature);
+
 /* ------------------------------------------------------------------ MCC */
 /* NullCollisionModel(config, particle_mass) (Collision.cpp:315-577) for
  * the target species.  Tables: per channel rows energies (J, strictly
