@@ -124,6 +124,7 @@ def load(path: str = LIB_PATH):
         "aurora_hip_step": [ctx, C.c_double, C.c_int64],
         "aurora_hip_energy": [ctx, C.c_double, _dp, _dp, _dp, _i64p],
         "aurora_hip_boundary_losses": [ctx, C.c_int, _i64p, _dp],
+        "aurora_hip_power_transfer": [ctx, C.c_int, _dp],
         "aurora_hip_time": [ctx, _dp, _i64p],
         "aurora_hip_set_sort_interval": [ctx, C.c_int64],
         "aurora_hip_set_sort_threshold": [ctx, C.c_double],
@@ -363,6 +364,11 @@ class Context:
         e = np.zeros(2)
         self._check(self.lib.aurora_hip_boundary_losses(self.h, sid, _p(a, _i64p), _p(e, _dp)))
         return a, e
+
+    def power_transfer(self, sid):
+        w = C.c_double()
+        self._check(self.lib.aurora_hip_power_transfer(self.h, sid, C.byref(w)))
+        return w.value
 
     def time(self):
         t, s = C.c_double(), C.c_int64()

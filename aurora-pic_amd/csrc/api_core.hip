@@ -438,16 +438,7 @@ int aurora_hip_species_count(aurora_hip_ctx* h, int sid, int64_t* slots, int64_t
     return guarded(h, [&](Context& c) {
         Species& sp = species_at(c, sid);
         if (slots) *slots = sp.slots;
-        if (live) {
-            std::vector<int32_t> cell(static_cast<size_t>(sp.slots));
-            AUR_HIP(hipStreamSynchronize(c.stream));
-            if (sp.slots)
-                AUR_HIP(hipMemcpy(cell.data(), sp.cell.ptr, sp.slots * sizeof(int32_t),
-                                  hipMemcpyDeviceToHost));
-            int64_t k = 0;
-            for (auto v : cell) k += v >= 0;
-            *live = k;
-        }
+        if (live) *live = live_count(c, sp);
     });
 }
 
@@ -651,6 +642,13 @@ int aurora_hip_initialize(aurora_hip_ctx* h, double dt) {
                     "simulation dt must be positive and finite");
         c.time = 0.0;
         c.step_count = 0;
+        // the wall counters and 1D ledgers start over (Simulation.cpp:698-709)
+        for (auto& sp : c.species) {
+            AUR_HIP(hipMemsetAsync(sp.absorbed.ptr, 0, 6 * sizeof(unsigned long long), c.stream));
+            if (sp.ledger.ptr)
+                AUR_HIP(hipMemsetAsync(sp.ledger.ptr, 0, 3 * sizeof(double), c.stream));
+            sp.ledger_pending = false;
+        }
         deposit_all(c);
         convert_rho(c);
         solve_fields(c, electrode(c.left4, 0.0), electrode(c.right4, 0.0), true);
@@ -667,9 +665,10 @@ int aurora_hip_step(aurora_hip_ctx* h, double dt, int64_t nsteps) {
         AUR_REQUIRE(nsteps >= 0, AURORA_ERR_INVALID_ARGUMENT, "step count must be non-negative");
         for (int64_t s = 0; s < nsteps; ++s) {
             // kick + drift + boundary + deposit at x^{n+1} (fused)
-            push_all(c, dt, true);
+            push_all(c, dt, true, c.dim == 1);
             const double t1 = c.time + dt;
             solve_step_fields(c, electrode(c.left4, t1), electrode(c.right4, t1));
+            if (c.dim == 1) ledger_work_1d(c, dt);
             if (c.mcc.loaded) mcc_step(c, dt);
             c.time = t1;
             c.step_count += 1;
@@ -766,9 +765,23 @@ int aurora_hip_boundary_losses(aurora_hip_ctx* h, int sid, int64_t* absorbed6, d
             for (int f = 0; f < 6; ++f) absorbed6[f] = static_cast<int64_t>(counts[f]);
         if (energy2) {
             energy2[0] = energy2[1] = 0.0;
-            if (sp.absorbed_energy.ptr)
-                AUR_HIP(hipMemcpy(energy2, sp.absorbed_energy.ptr, 2 * sizeof(double),
+            if (sp.ledger.ptr)
+                AUR_HIP(hipMemcpy(energy2, sp.ledger.ptr, 2 * sizeof(double),
                                   hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int aurora_hip_power_transfer(aurora_hip_ctx* h, int sid, double* electric_work) {
+    return guarded(h, [&](Context& c) {
+        Species& sp = species_at(c, sid);
+        AUR_REQUIRE(electric_work, AURORA_ERR_INVALID_ARGUMENT, "null electric_work pointer");
+        *electric_work = 0.0;
+        AUR_HIP(hipStreamSynchronize(c.stream));
+        if (sp.ledger.ptr) {
+            double led[3];
+            AUR_HIP(hipMemcpy(led, sp.ledger.ptr, sizeof led, hipMemcpyDeviceToHost));
+            *electric_work = led[2];
         }
     });
 }

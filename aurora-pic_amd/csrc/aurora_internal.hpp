@@ -150,9 +150,18 @@ struct Species {
     DevBuf<int32_t> cell2;
     DevBuf<uint8_t> off2[3], vel2[3];
     DevBuf<uint32_t> uid2;
-    // boundary ledger on device: [6 counts][2 energies as int64-free doubles]
-    DevBuf<unsigned long long> absorbed;  // 6
-    DevBuf<double> absorbed_energy;       // per-block partials (2 per block)
+    // boundary counters: 6 faces, then tiled pushes that left their tile box
+    DevBuf<unsigned long long> absorbed;  // 7
+    // 1D ledgers (Simulation.cpp:1008-1138): [left wall energy, right wall
+    // energy, electric work], cumulative since initialize.  Each push writes
+    // per-block partials; the work of the survivors is added after the field
+    // solve; one finalize kernel folds the partials in block order, so the
+    // totals do not depend on scheduling.
+    DevBuf<double> ledger;                // 3
+    DevBuf<double> ledger_part;           // 3 per push block, then 1 per work block
+    DevBuf<double> vold;                  // synchronised v_x at the start of the step
+    int ledger_push_blocks = 0;           // grid of the last push
+    bool ledger_pending = false;          // push partials not yet folded
     DevBuf<int32_t> tile_start;           // ntiles + 1 segment starts after a sort
     bool tiled = false;                   // storage is tile-sorted (tile_start valid)
     int64_t pushes_since_sort = 0;        // tiled pushes counted in absorbed[6]
@@ -164,8 +173,6 @@ struct Species {
     double cycle_ms = 0.0;                // last sort + timed pushes since, ms
     int64_t timed_pushes = 0;             // pushes of the cycle in cycle_ms
     double recent_ms[3] = {0, 0, 0};      // the last three timed pushes
-    int64_t absorbed_host[6] = {0, 0, 0, 0, 0, 0};
-    double absorbed_energy_host[2] = {0, 0};
     size_t elem() const { return precision == 64 ? 8 : 4; }
 };
 
@@ -198,7 +205,8 @@ struct MccModelHost {
     std::vector<double> energy_host[kMaxChannels];   // FP64 tables as loaded
     std::vector<double> sigma_host[kMaxChannels];
     DevBuf<unsigned long long> counts;  // [candidates, nulls, chan...]
-    DevBuf<double> energy_change;       // per channel, per-block partials
+    DevBuf<double> energy_change;       // per channel, cumulative
+    DevBuf<double> energy_rows;         // per wave of the last MCC launch, per channel
     DevBuf<int32_t> product_count;      // per target particle
     DevBuf<int64_t> product_offset;
     DevBuf<float> product_vel;          // secondaries: up to 2 per particle x 6 floats
@@ -364,7 +372,13 @@ uint64_t physics_signature(const Context& ctx);
 void prepare_solver(Context& ctx);
 void deposit_all(Context& ctx);
 void convert_rho(Context& ctx);
-void push_all(Context& ctx, double dt, bool fuse_deposit);
+// defer_ledger: 1D push partials stay pending for ledger_work_1d
+void push_all(Context& ctx, double dt, bool fuse_deposit, bool defer_ledger = false);
+// 1D, after the field solve of a step: the survivors' electric work
+// (Simulation.cpp:1094-1131), then every pending ledger folded
+void ledger_work_1d(Context& ctx, double dt);
+// live particles of a species, counted on the device
+int64_t live_count(Context& ctx, Species& sp);
 void sort_species(Context& ctx, Species& sp);
 bool sort_needed(Context& ctx, Species& sp);
 // adaptive policy: the species' push now costs more than its cycle average

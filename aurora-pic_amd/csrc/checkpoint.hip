@@ -6,7 +6,8 @@
 // dump, bit-exact by construction:
 //   magic, version, physics signature, time, step;
 //   per species: slots, next uid, the SoA arrays as stored (packed cell,
-//   offsets, v^{n-1/2}, uid), the boundary counters and energies;
+//   offsets, v^{n-1/2}, uid), the boundary counters and, in 1D, the wall
+//   energy and electric-work ledger;
 //   the MCC counters.
 // The fields are not stored: loading deposits the restored particles and
 // solves, which reproduces the fields of the saved step to the bit (integer
@@ -161,9 +162,9 @@ void checkpoint_save(Context& c, const std::string& path) {
         for (int a = 0; a < 3; ++a) dump(out, sp.vel[a].ptr, n * sp.elem());
         dump(out, sp.uid.ptr, n * sizeof(uint32_t));
         dump(out, sp.absorbed.ptr, 7 * sizeof(unsigned long long));
-        const bool has_e = sp.absorbed_energy.ptr != nullptr;
-        write_pod(out, has_e);
-        if (has_e) dump(out, sp.absorbed_energy.ptr, 2 * sizeof(double));
+        const bool has_ledger = sp.ledger.ptr != nullptr;
+        write_pod(out, has_ledger);
+        if (has_ledger) dump(out, sp.ledger.ptr, 3 * sizeof(double));
     }
     const bool mcc = c.mcc.loaded;
     write_pod(out, mcc);
@@ -209,11 +210,14 @@ void checkpoint_load(Context& c, const std::string& path) {
         for (int a = 0; a < 3; ++a) restore(in, sp.vel[a].ptr, n * sp.elem(), path);
         restore(in, sp.uid.ptr, n * sizeof(uint32_t), path);
         restore(in, sp.absorbed.ptr, 7 * sizeof(unsigned long long), path);
-        const bool has_e = read_pod<bool>(in, path);
-        if (has_e) {
-            if (!sp.absorbed_energy.ptr) sp.absorbed_energy.alloc(2);
-            restore(in, sp.absorbed_energy.ptr, 2 * sizeof(double), path);
+        const bool has_ledger = read_pod<bool>(in, path);
+        if (has_ledger) {
+            if (!sp.ledger.ptr) sp.ledger.alloc(3);
+            restore(in, sp.ledger.ptr, 3 * sizeof(double), path);
+        } else if (sp.ledger.ptr) {
+            AUR_HIP(hipMemset(sp.ledger.ptr, 0, 3 * sizeof(double)));
         }
+        sp.ledger_pending = false;
         sp.slots = slots;
         sp.next_uid = next_uid;
         sp.tiled = false;           // the next sort rebuilds the tile index

@@ -36,6 +36,7 @@ __global__ void __launch_bounds__(kBlk) k_collide(MccParams m, float* v, int64_t
                                                   uint32_t step, TallyOut out, uint32_t* err) {
     __shared__ MccLds lds;
     const XsView X = stage_tables(m, lds);
+    double ewave[NCH] = {};
     for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < count;
          base += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int64_t p = base + threadIdx.x;
@@ -51,8 +52,9 @@ __global__ void __launch_bounds__(kBlk) k_collide(MccParams m, float* v, int64_t
             v[3 * p + 1] = u[1];
             v[3 * p + 2] = u[2];
         }
-        flush_tally<NCH>(t, out, active);
+        flush_tally_rows<NCH>(t, out.counts, active, ewave);
     }
+    store_energy_rows<NCH>(ewave, out.energy_change);
 }
 
 struct Mcc1dArgs {
@@ -79,6 +81,7 @@ template <int NCH>
 __global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, TallyOut out) {
     __shared__ MccLds lds;
     const XsView X = stage_tables(m, lds);
+    double ewave[NCH] = {};
     for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < a.slots;
          base += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int64_t p = base + threadIdx.x;
@@ -94,8 +97,21 @@ __global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, Tally
             a.nprod[p] = mcc_1d_particle<NCH>(m, X, lds.chan, c, a.off[p], a.v0 + p, a.v1 + p, a.v2 + p, a.uid[p],
                                          a.E32, a.nx, a.half_qm_dt, a.dt, a.seed, a.step,
                                          a.prod + 12 * p, t, a.err);
-        flush_tally<NCH>(t, out, active);
+        flush_tally_rows<NCH>(t, out.counts, active, ewave);
     }
+    store_energy_rows<NCH>(ewave, out.energy_change);
+}
+
+// Adds the per-wave energy rows of one launch, in wave order, to out[NCH].
+template <int NCH>
+__global__ void __launch_bounds__(kBlk) k_fold_rows(const double* rows, int64_t nrows,
+                                                    double* out) {
+    double v[NCH] = {};
+    for (int64_t r = threadIdx.x; r < nrows; r += kBlk)
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) v[c] += rows[r * NCH + c];
+    const double s = block_sum<NCH, kBlk>(v);
+    if (threadIdx.x < NCH) out[threadIdx.x] += s;
 }
 
 
@@ -118,9 +134,8 @@ int blocks(const Context& c, int64_t n) {
     return static_cast<int>(g < 1 ? 1 : (g > cap ? cap : g));
 }
 
-TallyOut tally_out(Context& c) {
-    return TallyOut{c.mcc.counts.ptr, c.mcc.energy_change.ptr};
-}
+// energy rows of a launch of g workgroups: one row per wave
+int64_t energy_rows(int g) { return static_cast<int64_t>(g) * (kBlk / 64); }
 
 }  // namespace
 
@@ -262,8 +277,15 @@ void mcc_step(Context& c, double dt) {
     a.err = c.dev_error.ptr;
     {
         KernelScope ks(c, "mcc");
+        const int g = blocks(c, n);
+        const int64_t rows = energy_rows(g);
+        if (h.energy_rows.count < static_cast<size_t>(rows * kMaxChannels))
+            h.energy_rows.alloc(static_cast<size_t>(rows * kMaxChannels));
         with_nchan(m.nchan, [&]<int NCH>() {
-            k_mcc_1d<NCH><<<blocks(c, n), kBlk, 0, c.stream>>>(m, a, tally_out(c));
+            k_mcc_1d<NCH><<<g, kBlk, 0, c.stream>>>(m, a,
+                                                     TallyOut{h.counts.ptr, h.energy_rows.ptr});
+            k_fold_rows<NCH><<<1, kBlk, 0, c.stream>>>(h.energy_rows.ptr, rows,
+                                                       h.energy_change.ptr);
         });
         AUR_HIP(hipGetLastError());
     }
@@ -605,12 +627,17 @@ int aurora_hip_mcc_collide(aurora_hip_ctx* hctx, int64_t count, double* velocity
         const MccParams m = mcc_params(c);
         {
             KernelScope ks(c, "collide");
+            const int g = blocks(c, count);
+            DevBuf<double> rows;
+            rows.alloc(static_cast<size_t>(energy_rows(g) * kMaxChannels));
             with_nchan(m.nchan, [&]<int NCH>() {
-                k_collide<NCH><<<blocks(c, count), kBlk, 0, c.stream>>>(
+                k_collide<NCH><<<g, kBlk, 0, c.stream>>>(
                     m, dv.ptr, count, static_cast<float>(dt), h.seed, stream_id,
-                    static_cast<uint32_t>(step), TallyOut{cnt.ptr, ech.ptr}, c.dev_error.ptr);
+                    static_cast<uint32_t>(step), TallyOut{cnt.ptr, rows.ptr}, c.dev_error.ptr);
+                k_fold_rows<NCH><<<1, kBlk, 0, c.stream>>>(rows.ptr, energy_rows(g), ech.ptr);
             });
             AUR_HIP(hipGetLastError());
+            AUR_HIP(hipStreamSynchronize(c.stream));
         }
         AUR_HIP(hipStreamSynchronize(c.stream));
         check_device_error(c);

@@ -423,6 +423,44 @@ __device__ void flush_tally(const MccTally<NCH>& t, TallyOut out, bool active) {
     }
 }
 
+// flush_tally for kernels that keep their FP64 energy change deterministic:
+// counts go to the global counters (integer atomics), while the wave's energy
+// change per channel is added to ewave (the same value in every lane, summed
+// in a fixed order) for store_energy_rows.
+template <int NCH>
+__device__ void flush_tally_rows(const MccTally<NCH>& t, unsigned long long* counts, bool active,
+                                 double (&ewave)[NCH]) {
+    unsigned long long cand = active ? t.candidates : 0, nul = active ? t.nulls : 0;
+    unsigned long long sec = active ? static_cast<unsigned long long>(t.nsec) : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        cand += __shfl_down(cand, off, 64);
+        nul += __shfl_down(nul, off, 64);
+        sec += __shfl_down(sec, off, 64);
+    }
+    const bool lead = (threadIdx.x & 63) == 0;
+    if (lead) {
+        if (cand) atomicAdd(counts, cand);
+        if (nul) atomicAdd(counts + 1, nul);
+        if (sec) atomicAdd(counts + 2 + NCH, sec);
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        unsigned long long n = active ? t.chan[c] : 0;
+        for (int off = 32; off > 0; off >>= 1) n += __shfl_down(n, off, 64);
+        if (lead && n) atomicAdd(counts + 2 + c, n);
+        ewave[c] += wave_sum(active ? static_cast<double>(t.energy_change[c]) : 0.0);
+    }
+}
+
+// One row of NCH energy sums per wave of the launch, in wave order.
+template <int NCH>
+__device__ __forceinline__ void store_energy_rows(const double (&ewave)[NCH], double* rows) {
+    if ((threadIdx.x & 63) != 0) return;
+    const int64_t w = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) rows[w * NCH + c] = ewave[c];
+}
+
 // One live 1D target particle of the MCC step (Simulation.cpp:900-926):
 // synchronise v from v_half with E(x) at t+dt (Pusher.cpp:74-76), collide,
 // rebuild v_half (Pusher.cpp:66-68).  Up to two ionization product pairs

@@ -68,8 +68,10 @@ struct PushArgs {
     R dtd0, dtd1, dtd2;    // dt / d per axis
     long long* acc;        // deposit target (fused)
     unsigned long long* absorbed;  // 6 face counters, then pushes outside the tile box
-    double* absorbed_energy;       // 2 sides (1D), atomics
-    double half_mw;        // 0.5 * m * w (1D loss ledger)
+    double* ledger_part;   // 1D: per block [left energy, right energy, electric work]
+    double* vold;          // 1D: synchronised v_x at the start of the step, per slot
+    double half_qm, dt64;  // 0.5 * qm and dt in FP64: sync as (0.5 qm E) dt (Pusher.cpp:78-80)
+    double half_mw;        // 0.5 * m * w (1D ledgers)
     uint32_t* err;
 };
 
@@ -370,6 +372,7 @@ __global__ void __launch_bounds__(kBlock) k_deposit(SpeciesView<R> sp, MeshView 
 template <int DIM, typename R, int BMODE, bool DEPOSIT>
 __global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
     const SpeciesView<R>& sp = a.sp;
+    double led[3] = {0.0, 0.0, 0.0};   // 1D: left / right wall energy, electric work
     for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < sp.slots;
          p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int c = sp.cell[p];
@@ -382,6 +385,11 @@ __global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
         R v[3] = {sp.v0[p], sp.v1[p], sp.v2[p]};
         R E[3] = {R(0), R(0), R(0)};
         gather<DIM, R>(a, idx, off, E);
+        // 1D: the time-centred v_x before the kick, the reference's stored
+        // Particle::v (Pusher.cpp:78-80), for the electric-work ledger
+        double vsync = 0.0;
+        if constexpr (DIM == 1)
+            vsync = static_cast<double>(v[0]) + a.half_qm * static_cast<double>(E[0]) * a.dt64;
         if constexpr (BMODE == 0) {
             // leapfrog kick on the mesh axes (Pusher.cpp:70-72, 126-129, 162-166)
 #pragma unroll
@@ -414,14 +422,15 @@ __global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
             sp.cell[p] = -1;
             atomicAdd(a.absorbed + face, 1ull);
             if constexpr (DIM == 1) {
-                // Simulation.cpp:1059-1076: represented energy at the wall
-                const double e = a.half_mw * (static_cast<double>(v[0]) * v[0] +
-                                              static_cast<double>(v[1]) * v[1] +
-                                              static_cast<double>(v[2]) * v[2]);
-                atomicAdd(a.absorbed_energy + face, e);
+                // Simulation.cpp:1046-1076: the longitudinal work up to the
+                // crossing, and the represented energy at the wall
+                const double vx = v[0], vy = v[1], vz = v[2];
+                led[2] += a.half_mw * vx * vx - a.half_mw * vsync * vsync;
+                led[face] += a.half_mw * (vx * vx + vy * vy + vz * vz);
             }
             continue;
         }
+        if constexpr (DIM == 1) a.vold[p] = vsync;
         sp.cell[p] = compose<DIM>(a.m, idx);
         sp.off0[p] = off[0];
         if constexpr (DIM > 1) sp.off1[p] = off[1];
@@ -432,6 +441,61 @@ __global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
             deposit_one<DIM>(a.acc, a.m, idx, o);
         }
     }
+    if constexpr (DIM == 1) {
+        const double s = block_sum<3, kBlock>(led);
+        if (threadIdx.x < 3) a.ledger_part[3 * blockIdx.x + threadIdx.x] = s;
+    }
+}
+
+// 1D survivors after the field solve at t+dt (Simulation.cpp:1094-1131):
+// v_x synchronised with E^{n+1}, and the work 0.5 m w (v^2 - v_old^2)
+// since the start of the step.  Per-block sums into part[blockIdx.x].
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_work_1d(PushArgs<R> a, double* part) {
+    const SpeciesView<R>& sp = a.sp;
+    double w[1] = {0.0};
+    for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < sp.slots;
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int c = sp.cell[p];
+        if (c < 0) continue;
+        int idx[3] = {c, 0, 0};
+        R off[3] = {sp.off0[p], R(0), R(0)};
+        R E[3] = {R(0), R(0), R(0)};
+        gather<1, R>(a, idx, off, E);
+        const double vn =
+            static_cast<double>(sp.v0[p]) + a.half_qm * static_cast<double>(E[0]) * a.dt64;
+        const double vo = a.vold[p];
+        w[0] += a.half_mw * (vn * vn - vo * vo);
+    }
+    const double s = block_sum<1, kBlock>(w);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Folds a species' ledger partials in block order into its totals: push
+// partials (3 per block) and, when present, work partials (1 per block).
+__global__ void __launch_bounds__(kBlock) k_ledger_1d(const double* push_part, int push_blocks,
+                                                      const double* work_part, int work_blocks,
+                                                      double* ledger) {
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < push_blocks; b += kBlock) {
+        v[0] += push_part[3 * b];
+        v[1] += push_part[3 * b + 1];
+        v[2] += push_part[3 * b + 2];
+    }
+    if (work_part)
+        for (int b = threadIdx.x; b < work_blocks; b += kBlock) v[2] += work_part[b];
+    const double s = block_sum<3, kBlock>(v);
+    if (threadIdx.x < 3) ledger[threadIdx.x] += s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_count_live(const int32_t* cell, int64_t slots,
+                                                       unsigned long long* out) {
+    unsigned long long n = 0;
+    for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < slots;
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        n += cell[p] >= 0;
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_down(n, off, 64);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(out, n);
 }
 
 // sign = +1: synchronize (v from v_half); -1: half-step init (v_half from v)
@@ -605,7 +669,10 @@ PushArgs<R> push_args(Context& c, Species& sp, double dt) {
     a.dtd2 = static_cast<R>(dt / c.d[2]);
     a.acc = reinterpret_cast<long long*>(sp.acc.ptr);
     a.absorbed = sp.absorbed.ptr;
-    a.absorbed_energy = sp.absorbed_energy.ptr;
+    a.ledger_part = sp.ledger_part.ptr;
+    a.vold = sp.vold.ptr;
+    a.half_qm = 0.5 * qm;
+    a.dt64 = dt;
     a.half_mw = 0.5 * sp.mass * sp.weight;
     a.err = c.dev_error.ptr;
     return a;
@@ -1067,7 +1134,60 @@ void ensure_rho(Context& c) {
     if (!c.rho_valid) convert_rho(c);
 }
 
-void push_all(Context& c, double dt, bool fuse_deposit) {
+namespace {
+
+void ensure_ledger_1d(Context& c, Species& sp) {
+    if (!sp.ledger.ptr) {
+        sp.ledger.alloc(3);
+        AUR_HIP(hipMemset(sp.ledger.ptr, 0, 3 * sizeof(double)));
+    }
+    if (!sp.ledger_part.ptr)
+        sp.ledger_part.alloc(static_cast<size_t>(4 * static_cast<int64_t>(c.cu_count) * 32));
+    if (!sp.vold.ptr) sp.vold.alloc(static_cast<size_t>(sp.capacity > 0 ? sp.capacity : 1));
+}
+
+void fold_ledger_1d(Context& c, Species& sp, const double* work_part, int work_blocks) {
+    KernelScope ks(c, "ledger");
+    k_ledger_1d<<<1, kBlock, 0, c.stream>>>(sp.ledger_part.ptr, sp.ledger_push_blocks, work_part,
+                                            work_blocks, sp.ledger.ptr);
+    AUR_HIP(hipGetLastError());
+    sp.ledger_pending = false;
+}
+
+}  // namespace
+
+void ledger_work_1d(Context& c, double dt) {
+    if (c.dim != 1) return;
+    for (auto& sp : c.species) {
+        if (!sp.ledger_pending) continue;
+        const int g = launch_blocks(c, sp.slots);
+        double* part = sp.ledger_part.ptr + 3 * static_cast<int64_t>(c.cu_count) * 32;
+        dispatch_dim_prec(c, sp, [&]<int DIM, typename R>() {
+            if constexpr (DIM == 1) {
+                auto a = push_args<R>(c, sp, dt);
+                KernelScope ks(c, "work");
+                k_work_1d<R><<<g, kBlock, 0, c.stream>>>(a, part);
+                AUR_HIP(hipGetLastError());
+            }
+        });
+        fold_ledger_1d(c, sp, part, g);
+    }
+}
+
+int64_t live_count(Context& c, Species& sp) {
+    if (sp.slots == 0) return 0;
+    DevBuf<unsigned long long> n;
+    n.alloc(1);
+    AUR_HIP(hipMemsetAsync(n.ptr, 0, sizeof(unsigned long long), c.stream));
+    k_count_live<<<launch_blocks(c, sp.slots), kBlock, 0, c.stream>>>(sp.cell.ptr, sp.slots, n.ptr);
+    AUR_HIP(hipGetLastError());
+    unsigned long long h = 0;
+    AUR_HIP(hipMemcpyAsync(&h, n.ptr, sizeof h, hipMemcpyDeviceToHost, c.stream));
+    AUR_HIP(hipStreamSynchronize(c.stream));
+    return static_cast<int64_t>(h);
+}
+
+void push_all(Context& c, double dt, bool fuse_deposit, bool defer_ledger) {
     if (fuse_deposit) {
         for (auto& sp : c.species) {
             if (sp.acc_next_clear && sp.acc_next.ptr) {
@@ -1083,9 +1203,10 @@ void push_all(Context& c, double dt, bool fuse_deposit) {
     }
     for (auto& sp : c.species) {
         if (sp.slots == 0) continue;
-        if (c.dim == 1 && !sp.absorbed_energy.ptr) {
-            sp.absorbed_energy.alloc(2);
-            AUR_HIP(hipMemset(sp.absorbed_energy.ptr, 0, 2 * sizeof(double)));
+        if (c.dim == 1) {
+            ensure_ledger_1d(c, sp);
+            // a push whose partials were never folded (push without a step)
+            if (sp.ledger_pending) fold_ledger_1d(c, sp, nullptr, 0);
         }
         if (fuse_deposit && sp.tiled && sp.precision == 32 && c.tiled_push_ok() &&
             sp.slots < (int64_t{1} << 30)) {
@@ -1104,6 +1225,11 @@ void push_all(Context& c, double dt, bool fuse_deposit) {
         dispatch_dim_prec(c, sp, [&]<int DIM, typename R>() {
             launch_push<DIM, R>(c, sp, dt, fuse_deposit);
         });
+        if (c.dim == 1) {
+            sp.ledger_push_blocks = launch_blocks(c, sp.slots);
+            sp.ledger_pending = true;
+            if (!defer_ledger) fold_ledger_1d(c, sp, nullptr, 0);
+        }
     }
 }
 
@@ -1191,10 +1317,7 @@ int aurora_hip_energy(aurora_hip_ctx* h, double dt, double* kinetic, double* fie
                 AUR_HIP(hipStreamSynchronize(c.stream));
                 AUR_HIP(hipMemcpy(hp.data(), part.ptr, g * sizeof(double), hipMemcpyDeviceToHost));
                 for (double v : hp) ke += v;
-                std::vector<int32_t> cell(static_cast<size_t>(sp.slots));
-                AUR_HIP(hipMemcpy(cell.data(), sp.cell.ptr, sp.slots * sizeof(int32_t),
-                                  hipMemcpyDeviceToHost));
-                for (auto v : cell) nlive += v >= 0;
+                nlive += live_count(c, sp);
             }
             if (kinetic) kinetic[s] = ke;
         }

@@ -13,6 +13,35 @@ __device__ __forceinline__ int wrap_index(int i, int n) {
     return i < 0 ? i + n : i;
 }
 
+// ------------------------------------------------- deterministic FP64 sums
+// Sum over the 64 lanes of a wave in a fixed butterfly order: every lane
+// gets the same value, which depends only on the lanes' inputs, never on
+// scheduling (unlike FP64 atomics, whose order varies from run to run).
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block sums of K values per thread, waves added in wave order; thread k < K
+// returns sum k (others return 0).  Every thread of the block must call it.
+template <int K, int BLOCK>
+__device__ __forceinline__ double block_sum(const double (&v)[K]) {
+    static_assert(BLOCK % 64 == 0 && BLOCK / 64 <= 16, "block of whole waves");
+    __shared__ double red[K][BLOCK / 64];
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double s = wave_sum(v[k]);
+        if ((threadIdx.x & 63) == 0) red[k][wave] = s;
+    }
+    __syncthreads();
+    double out = 0.0;
+    if (threadIdx.x < K)
+        for (int w = 0; w < BLOCK / 64; ++w) out += red[threadIdx.x][w];
+    __syncthreads();
+    return out;
+}
+
 // wrap_index for i in [-n, 2n): no division
 __device__ __forceinline__ int wrap_near(int i, int n) {
     return i < 0 ? i + n : (i >= n ? i - n : i);

@@ -188,11 +188,19 @@ int aurora_hip_step(aurora_hip_ctx* ctx, double dt, int64_t nsteps);
 int aurora_hip_energy(aurora_hip_ctx* ctx, double dt, double* kinetic,
                       double* field_energy, double* charge_l1,
                       int64_t* live);
-/* Boundary-loss ledger of a species since the last reset: absorbed
- * counts per face (6) and, in 1D, absorbed kinetic energy per side (2)
- * (BoundaryLoss1D, Simulation.hpp:20-25). */
+/* Boundary-loss ledger of a species since aurora_hip_initialize: absorbed
+ * counts per face (6) and, in 1D, the represented kinetic energy at each
+ * wall (2): species_boundary_losses() (BoundaryLoss1D, Simulation.hpp:20-25,
+ * accumulated as Simulation.cpp:1046-1091). */
 int aurora_hip_boundary_losses(aurora_hip_ctx* ctx, int species_id,
                                int64_t* absorbed6, double* energy2);
+/* 1D electric work on a species since aurora_hip_initialize:
+ * species_power_transfer() (SpeciesPower1D, Simulation.hpp:27-29): the
+ * longitudinal kinetic-energy change of the kick up to a wall crossing, and
+ * of every survivor from the start of the step to its synchronised velocity
+ * after the field solve (Simulation.cpp:1046-1131).  Summed in a fixed
+ * order on the device: identical from run to run.  0 outside 1D. */
+int aurora_hip_power_transfer(aurora_hip_ctx* ctx, int species_id, double* electric_work);
 /* Simulation time and step count kept by the context. */
 int aurora_hip_time(aurora_hip_ctx* ctx, double* time, int64_t* step);
 /* Tile shape (cells per axis) of the cell sort and of the LDS-tiled 3D
