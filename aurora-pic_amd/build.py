@@ -49,12 +49,13 @@ CXX = os.environ.get("CXX", "g++")
 
 
 def build_host() -> str:
-    """C++20 host classes (host/aurora_gpu.cpp, host/particle_state.cpp) over the C ABI, and their
+    """C++20 host classes (host/aurora_gpu.cpp, host/particle_state.cpp, host/diagnostics.cpp) over the C ABI, and their
     test program; plain g++, linked against libaurora_hip.so."""
     inc = [f"-I{os.path.join(REPO, 'include')}", f"-I{HOST}"]
     rpath = "-Wl,-rpath,$ORIGIN"
     cmd = [CXX, "-std=c++20", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", *inc,
            os.path.join(HOST, "aurora_gpu.cpp"), os.path.join(HOST, "particle_state.cpp"),
+           os.path.join(HOST, "diagnostics.cpp"),
            "-o", HOST_SO + ".tmp", f"-L{LIB}",
            "-laurora_hip", rpath]
     r = subprocess.run(cmd, capture_output=True, text=True)

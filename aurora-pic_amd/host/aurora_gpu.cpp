@@ -1,9 +1,11 @@
 // aurora_gpu.cpp - C++20 host classes over the C ABI (include/aurora_hip.h).
 #include "aurora_gpu.hpp"
+#include "diagnostics.hpp"
 #include "particle_state.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <random>
 #include <utility>
 
 namespace pic::gpu {
@@ -150,17 +152,61 @@ Downloaded download(const Device& d, int sid, int dim, double dt) {
     return out;
 }
 
-DiagnosticSample energies(const Device& d, double dt, size_t nspecies) {
+// One diagnostic sample of the context (Diagnostics.cpp:37-59, 114-140,
+// 193-229): energies on the device, species kinetic energies added in
+// species order, wall counters summed over species, and the electrode
+// potentials of a 1D Dirichlet mesh.
+DiagnosticSample energies(const Device& d, double dt, size_t nspecies, int dim = 0,
+                          Boundary boundary = Boundary::Periodic, std::size_t nx = 0) {
     std::vector<double> ke(std::max<size_t>(nspecies, 1));
     double fe = 0.0, l1 = 0.0;
     int64_t live = 0;
     d.check(aurora_hip_energy(d.get(), dt, ke.data(), &fe, &l1, &live));
     DiagnosticSample s;
-    for (size_t k = 0; k < nspecies; ++k) s.kinetic_energy += ke[k];
+    int64_t step = 0;
+    d.check(aurora_hip_time(d.get(), &s.time, &step));
+    s.step = static_cast<size_t>(step);
+    for (size_t k = 0; k < nspecies; ++k) {
+        s.kinetic_energy += ke[k];
+        int64_t slots = 0, lk = 0;
+        d.check(aurora_hip_species_count(d.get(), static_cast<int>(k), &slots, &lk));
+        s.live_particles_by_species.push_back(static_cast<size_t>(lk));
+        int64_t faces[6] = {};
+        d.check(aurora_hip_boundary_losses(d.get(), static_cast<int>(k), faces, nullptr));
+        for (int f = 0; f < 6; ++f) s.absorbed[f] += static_cast<uint64_t>(faces[f]);
+    }
     s.field_energy = fe;
+    s.total_energy = s.kinetic_energy + s.field_energy;
     s.charge_l1 = l1;
     s.live_particles = static_cast<size_t>(live);
+    if (dim == 1 && boundary == Boundary::Dirichlet && nx > 0) {
+        std::vector<double> phi(nx);
+        d.check(aurora_hip_get_fields(d.get(), nullptr, phi.data(), nullptr));
+        s.phi_left = phi.front();
+        s.phi_right = phi.back();
+    }
     return s;
+}
+
+// run() of the reference (Simulation*.cpp): samples at step 0, every
+// output_interval steps and at the last step.  step_to(k) advances the
+// device to step k; write(s) writes one output row set.
+template <typename StepTo, typename Write>
+RunSummary run_loop(std::size_t start, std::size_t limit, std::size_t interval,
+                    StepTo&& step_to, Write&& write) {
+    RunSummary summary;
+    summary.final_sample = write(true);
+    std::size_t step = start;
+    while (step < limit) {
+        std::size_t next = limit;
+        if (interval > 0) next = std::min(limit, (step / interval + 1) * interval);
+        step_to(next);
+        step = next;
+        summary.final_sample = write(false);
+    }
+    summary.steps_completed = step;
+    summary.final_time = summary.final_sample.time;
+    return summary;
 }
 
 }  // namespace
@@ -380,12 +426,25 @@ CollisionStepStatistics NullCollisionModel::collide(std::vector<Vec3>& velocitie
 // ----------------------------------------------------------- Simulation
 Simulation::Simulation(Config cfg, std::shared_ptr<Device> device)
     : cfg_(std::move(cfg)), dev_(or_new(std::move(device))) {
-    // Simulation.cpp:352-362 (the device path is 1D3V)
+    // Simulation.cpp:341-380
     if (cfg_.initial_state_signature && cfg_.initial_state_path.empty())
         throw std::invalid_argument("initial_state_signature requires initial_state_path");
-    if (!cfg_.initial_state_path.empty())
+    if (cfg_.velocity_dimensions != 1 && cfg_.velocity_dimensions != 3)
+        throw std::invalid_argument("simulation velocity_dimensions must be 1 or 3");
+    if (cfg_.velocity_dimensions == 3 && !cfg_.initial_state_path.empty())
         throw std::invalid_argument(
             "external particle-state initialization is not yet available for 1D3V");
+    if (cfg_.max_particles_per_species == 0)
+        throw std::invalid_argument("max_particles_per_species must be positive");
+    if (cfg_.output_interval == 0) throw std::invalid_argument("output_interval must be positive");
+    for (const auto& m : cfg_.collision_models) {
+        for (const auto& ch : m.config.channels)
+            if (ch.process == CollisionProcessKind::Ionization && cfg_.velocity_dimensions != 3)
+                throw std::invalid_argument("1D ionization requires velocity_dimensions = 3");
+        if (cfg_.velocity_dimensions != 3)
+            throw std::invalid_argument(
+                "the GPU backend's 1D collisions are 1D3V (the 1D1V collide is not built)");
+    }
     const Device& d = *dev_;
     const int64_t n = static_cast<int64_t>(cfg_.nx);
     mesh_bind(d, 1, &n, &cfg_.length, cfg_.boundary, cfg_.permittivity, 1.0);
@@ -424,35 +483,154 @@ Simulation::Simulation(Config cfg, std::shared_ptr<Device> device)
     }
 }
 
+// Simulation::initialize (Simulation.cpp:698-770).  Generated states follow
+// Species::initialize for uniform random loading (Species.cpp:40-78): one
+// std::mt19937_64 seeded with the config seed, species in config order,
+// per particle x, then v_x, then (1D3V) v_y and v_z, each from its own
+// distribution object, so the particles equal the reference's.  External
+// states (1D1V) take x and v_x from the file.  The device then deposits,
+// solves at t = 0 and sets v_half = v - qE dt / 2m.
 void Simulation::initialize() {
     const Device& d = *dev_;
-    for (size_t k = 0; k < cfg_.species.size(); ++k) {
-        const auto& s = cfg_.species[k];
-        const double drift[3] = {s.drift_velocity, 0.0, 0.0};
-        d.check(aurora_hip_species_init_maxwellian(d.get(), static_cast<int>(k),
-                                                   static_cast<int64_t>(s.particles),
-                                                   s.thermal_velocity, drift, cfg_.seed + k));
+    const size_t ns = cfg_.species.size();
+    std::vector<std::vector<double>> pos(ns), vel(ns);
+    if (!cfg_.initial_state_path.empty()) {
+        const auto state = load_external(cfg_.initial_state_path, 1, cfg_.units, cfg_.species,
+                                         "1D simulation", cfg_.initial_state_signature);
+        for (size_t k = 0; k < ns; ++k) {
+            const auto& sc = cfg_.species[k];
+            const double lo = sc.init_x_min;
+            const double hi = sc.init_x_max < 0.0 ? cfg_.length : sc.init_x_max;
+            for (const auto& r : state.species.at(sc.name)) {
+                if (r.position.x < lo || r.position.x > hi ||
+                    (cfg_.boundary == Boundary::Periodic && r.position.x == cfg_.length))
+                    throw std::runtime_error("external particle for species '" + sc.name +
+                                             "' lies outside its 1D initialization interval");
+                pos[k].push_back(r.position.x);
+                vel[k].insert(vel[k].end(), {r.velocity.x, 0.0, 0.0});
+            }
+        }
+    } else {
+        std::mt19937_64 rng(cfg_.seed);
+        for (size_t k = 0; k < ns; ++k) {
+            const auto& sc = cfg_.species[k];
+            const double lo = sc.init_x_min;
+            const double hi = sc.init_x_max < 0.0 ? cfg_.length : sc.init_x_max;
+            std::uniform_real_distribution<double> ux(lo, hi);
+            std::normal_distribution<double> nv(sc.drift_velocity, sc.thermal_velocity);
+            std::normal_distribution<double> nvy(sc.drift_velocity_y, sc.thermal_velocity);
+            std::normal_distribution<double> nvz(sc.drift_velocity_z, sc.thermal_velocity);
+            pos[k].reserve(sc.particles);
+            vel[k].reserve(3 * sc.particles);
+            for (size_t p = 0; p < sc.particles; ++p) {
+                double x = ux(rng);
+                if (cfg_.boundary == Boundary::Periodic)
+                    x = std::fmod(std::fmod(x, cfg_.length) + cfg_.length, cfg_.length);
+                else
+                    x = std::clamp(x, 0.0, cfg_.length);
+                const double vx = nv(rng);
+                double vy = 0.0, vz = 0.0;
+                if (cfg_.velocity_dimensions == 3) {
+                    vy = nvy(rng);
+                    vz = nvz(rng);
+                }
+                pos[k].push_back(x);
+                vel[k].insert(vel[k].end(), {vx, vy, vz});
+            }
+        }
     }
+    for (size_t k = 0; k < ns; ++k)
+        d.check(aurora_hip_species_upload(d.get(), static_cast<int>(k),
+                                          static_cast<int64_t>(pos[k].size()), pos[k].data(),
+                                          vel[k].data(), nullptr, AURORA_VELOCITY_CENTRED));
     d.check(aurora_hip_initialize(d.get(), cfg_.dt));
     time_ = 0.0;
     step_ = 0;
     initialized_ = true;
 }
 
-void Simulation::step() {
-    if (!initialized_) initialize();
-    dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, 1));
+void Simulation::sync_clock() {
     int64_t s = 0;
     dev_->check(aurora_hip_time(dev_->get(), &time_, &s));
     step_ = static_cast<size_t>(s);
 }
 
-void Simulation::run() {
+void Simulation::step() {
     if (!initialized_) initialize();
-    dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, static_cast<int64_t>(cfg_.steps)));
-    int64_t s = 0;
-    dev_->check(aurora_hip_time(dev_->get(), &time_, &s));
-    step_ = static_cast<size_t>(s);
+    dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, 1));
+    sync_clock();
+}
+
+RunSummary Simulation::run() {
+    if (!initialized_) initialize();
+    auto step_to = [&](size_t k) {
+        dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, static_cast<int64_t>(k - step_)));
+        sync_clock();
+    };
+    std::vector<std::string> names;
+    std::vector<double> charge_per_macro;
+    for (const auto& sc : cfg_.species) {
+        names.push_back(sc.name);
+        charge_per_macro.push_back(sc.charge * sc.weight);
+    }
+    if (cfg_.output_dir.empty())
+        return run_loop(step_, cfg_.steps, cfg_.output_interval, step_to,
+                        [&](bool) { return sample(); });
+    // Simulation.cpp:2942-3050
+    const bool si = cfg_.units == UnitSystem::SI;
+    Diagnostics diag(cfg_.output_dir, names, cfg_.permittivity);
+    diag.write_header();
+    const bool collisions = !cfg_.collision_models.empty();
+    std::ofstream coll, losses, power;
+    CollisionDiagnostics last;
+    if (collisions) {
+        coll.open(cfg_.output_dir / "collisions.csv");
+        if (!coll) throw std::runtime_error("cannot open collision diagnostics output");
+        last = collision_diagnostics();
+        write_collision_header(coll, last.channel_names, si);
+    }
+    losses.open(cfg_.output_dir / "boundary_losses.csv");
+    if (!losses) throw std::runtime_error("cannot open 1D boundary-loss diagnostics output");
+    write_boundary_loss_header(losses, names, si);
+    power.open(cfg_.output_dir / "power_transfer.csv");
+    if (!power) throw std::runtime_error("cannot open 1D power-transfer diagnostics output");
+    write_power_transfer_header(power, names, si);
+    auto write = [&](bool first) {
+        const DiagnosticSample smp = diag.sample(*this);
+        diag.write_sample(smp);
+        diag.write_fields(step_, fields());
+        if (collisions) {
+            // the interval counters start empty at the first row (also after
+            // a restart) and hold the change since the previous row after it
+            const CollisionDiagnostics now = collision_diagnostics();
+            const CollisionDiagnostics& since = first ? now : last;
+            CollisionDiagnostics interval = now;
+            interval.candidates -= since.candidates;
+            interval.null_collisions -= since.null_collisions;
+            for (size_t c = 0; c < now.channel_collisions.size(); ++c) {
+                interval.channel_collisions[c] -= since.channel_collisions[c];
+                interval.channel_energy_change[c] -= since.channel_energy_change[c];
+            }
+            write_collision_sample(coll, step_, time_, interval, now);
+            last = now;
+        }
+        write_boundary_loss_sample(losses, step_, time_, boundary_loss_origin_step(),
+                                   charge_per_macro, species_boundary_losses());
+        write_power_transfer_sample(power, step_, time_, power_transfer_origin_step(),
+                                    species_power_transfer());
+        return smp;
+    };
+    return run_loop(step_, cfg_.steps, cfg_.output_interval, step_to, write);
+}
+
+void Simulation::save_checkpoint(const std::filesystem::path& path) const {
+    dev_->check(aurora_hip_checkpoint_save(dev_->get(), path.string().c_str()));
+}
+
+void Simulation::load_checkpoint(const std::filesystem::path& path) {
+    dev_->check(aurora_hip_checkpoint_load(dev_->get(), path.string().c_str()));
+    sync_clock();
+    initialized_ = true;
 }
 
 std::vector<Particle> Simulation::particles(std::size_t species) const {
@@ -470,7 +648,9 @@ std::vector<Particle> Simulation::particles(std::size_t species) const {
     return out;
 }
 
-DiagnosticSample Simulation::sample() const { return energies(*dev_, cfg_.dt, cfg_.species.size()); }
+DiagnosticSample Simulation::sample() const {
+    return energies(*dev_, cfg_.dt, cfg_.species.size(), 1, cfg_.boundary, cfg_.nx);
+}
 
 Grid Simulation::fields() const {
     Grid g(cfg_.nx, cfg_.length, cfg_.boundary);
@@ -483,8 +663,36 @@ BoundaryLoss1D Simulation::boundary_losses(std::size_t species) const {
     int64_t counts[6] = {};
     double energy[2] = {};
     dev_->check(aurora_hip_boundary_losses(dev_->get(), static_cast<int>(species), counts, energy));
-    return BoundaryLoss1D{static_cast<uint64_t>(counts[0]), static_cast<uint64_t>(counts[1]),
+    return BoundaryLoss1D{static_cast<size_t>(counts[0]), static_cast<size_t>(counts[1]),
                           energy[0], energy[1]};
+}
+
+std::vector<BoundaryLoss1D> Simulation::species_boundary_losses() const {
+    std::vector<BoundaryLoss1D> out;
+    for (size_t k = 0; k < cfg_.species.size(); ++k) out.push_back(boundary_losses(k));
+    return out;
+}
+
+std::vector<SpeciesPower1D> Simulation::species_power_transfer() const {
+    std::vector<SpeciesPower1D> out(cfg_.species.size());
+    for (size_t k = 0; k < out.size(); ++k)
+        dev_->check(aurora_hip_power_transfer(dev_->get(), static_cast<int>(k),
+                                              &out[k].electric_work));
+    return out;
+}
+
+CollisionDiagnostics Simulation::collision_diagnostics() const {
+    CollisionDiagnostics d;
+    if (cfg_.collision_models.empty()) return d;
+    const NamedCollisionConfig& m = cfg_.collision_models[0];
+    for (const auto& ch : m.config.channels)
+        d.channel_names.push_back(m.name.empty() ? ch.name : m.name + "." + ch.name);
+    const CollisionStepStatistics s = collision_statistics();
+    d.candidates = s.candidates;
+    d.null_collisions = s.null_collisions;
+    d.channel_collisions = s.channel_collisions;
+    d.channel_energy_change = s.channel_projectile_energy_change;
+    return d;
 }
 
 CollisionStepStatistics Simulation::collision_statistics() const {
@@ -583,12 +791,38 @@ void Simulation2D::step() {
     step_ = static_cast<size_t>(s);
 }
 
-void Simulation2D::run() {
+RunSummary Simulation2D::run() {
     if (!initialized_) initialize();
-    dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, static_cast<int64_t>(cfg_.steps)));
+    auto step_to = [&](size_t k) {
+        dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, static_cast<int64_t>(k - step_)));
+        int64_t s = 0;
+        dev_->check(aurora_hip_time(dev_->get(), &time_, &s));
+        step_ = static_cast<size_t>(s);
+    };
+    if (cfg_.output_dir.empty())
+        return run_loop(step_, cfg_.steps, cfg_.output_interval, step_to,
+                        [&](bool) { return sample(); });
+    std::vector<std::string> names;
+    for (const auto& sc : cfg_.species) names.push_back(sc.name);
+    Diagnostics2D diag(cfg_.output_dir, names, cfg_.permittivity, cfg_.out_of_plane_depth);
+    diag.write_header();
+    return run_loop(step_, cfg_.steps, cfg_.output_interval, step_to, [&](bool) {
+        const DiagnosticSample smp = diag.sample(*this);
+        diag.write_sample(smp);
+        return smp;
+    });
+}
+
+void Simulation2D::save_checkpoint(const std::filesystem::path& path) const {
+    dev_->check(aurora_hip_checkpoint_save(dev_->get(), path.string().c_str()));
+}
+
+void Simulation2D::load_checkpoint(const std::filesystem::path& path) {
+    dev_->check(aurora_hip_checkpoint_load(dev_->get(), path.string().c_str()));
     int64_t s = 0;
     dev_->check(aurora_hip_time(dev_->get(), &time_, &s));
     step_ = static_cast<size_t>(s);
+    initialized_ = true;
 }
 
 std::vector<Particle2D> Simulation2D::particles(std::size_t species) const {
@@ -607,7 +841,7 @@ std::vector<Particle2D> Simulation2D::particles(std::size_t species) const {
 }
 
 DiagnosticSample Simulation2D::sample() const {
-    return energies(*dev_, cfg_.dt, cfg_.species.size());
+    return energies(*dev_, cfg_.dt, cfg_.species.size(), 2);
 }
 
 Mesh2D Simulation2D::fields() const {
@@ -727,12 +961,38 @@ void Simulation3D::step() {
     step_ = static_cast<size_t>(s);
 }
 
-void Simulation3D::run() {
+RunSummary Simulation3D::run() {
     if (!initialized_) initialize();
-    dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, static_cast<int64_t>(cfg_.steps)));
+    auto step_to = [&](size_t k) {
+        dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, static_cast<int64_t>(k - step_)));
+        int64_t s = 0;
+        dev_->check(aurora_hip_time(dev_->get(), &time_, &s));
+        step_ = static_cast<size_t>(s);
+    };
+    if (cfg_.output_dir.empty())
+        return run_loop(step_, cfg_.steps, cfg_.output_interval, step_to,
+                        [&](bool) { return sample(); });
+    std::vector<std::string> names;
+    for (const auto& sc : cfg_.species) names.push_back(sc.name);
+    Diagnostics3D diag(cfg_.output_dir, names, cfg_.permittivity);
+    diag.write_header();
+    return run_loop(step_, cfg_.steps, cfg_.output_interval, step_to, [&](bool) {
+        const DiagnosticSample smp = diag.sample(*this);
+        diag.write_sample(smp);
+        return smp;
+    });
+}
+
+void Simulation3D::save_checkpoint(const std::filesystem::path& path) const {
+    dev_->check(aurora_hip_checkpoint_save(dev_->get(), path.string().c_str()));
+}
+
+void Simulation3D::load_checkpoint(const std::filesystem::path& path) {
+    dev_->check(aurora_hip_checkpoint_load(dev_->get(), path.string().c_str()));
     int64_t s = 0;
     dev_->check(aurora_hip_time(dev_->get(), &time_, &s));
     step_ = static_cast<size_t>(s);
+    initialized_ = true;
 }
 
 std::vector<Particle3D> Simulation3D::particles(std::size_t species) const {
@@ -749,7 +1009,7 @@ std::vector<Particle3D> Simulation3D::particles(std::size_t species) const {
 }
 
 DiagnosticSample Simulation3D::sample() const {
-    return energies(*dev_, cfg_.dt, cfg_.species.size());
+    return energies(*dev_, cfg_.dt, cfg_.species.size(), 3);
 }
 
 Mesh3D Simulation3D::fields() const {

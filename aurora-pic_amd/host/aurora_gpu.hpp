@@ -16,8 +16,11 @@
 //   - NullCollisionModel::collide works on a batch of velocities with one
 //     counter-based (Philox4x32-10) stream per particle and step instead of
 //     one std::mt19937_64 passed by reference;
-//   - species initialisation draws from Philox as well, so initial states
-//     equal the reference's in distribution, not sample by sample.
+//   - 2D/3D species initialisation draws from Philox as well, so those
+//     initial states equal the reference's in distribution, not sample by
+//     sample; the 1D loading (uniform positions, Maxwellian velocities)
+//     draws from std::mt19937_64 in the reference's order and so gives the
+//     reference's particles.
 #pragma once
 
 #include <array>
@@ -276,11 +279,32 @@ private:
 };
 
 // ---------------------------------------------------------- simulations
+// DiagnosticSample / DiagnosticSample2D / DiagnosticSample3D
+// (Diagnostics.hpp): FP64 energies from the device (kinetic energy of the
+// time-centred velocities, field energy and charge L1 over node volumes),
+// the electrode potentials of a 1D Dirichlet run, and the wall counters of
+// a 2D/3D run summed over species (faces left, right, bottom, top, back,
+// front).
 struct DiagnosticSample {
-    double kinetic_energy{0.0};   // sum over species, J (time-centred velocities)
+    std::size_t step{0};
+    double time{0.0};
+    double kinetic_energy{0.0};
     double field_energy{0.0};
+    double total_energy{0.0};
     double charge_l1{0.0};
     std::size_t live_particles{0};
+    double phi_left{0.0};
+    double phi_right{0.0};
+    std::vector<std::size_t> live_particles_by_species{};
+    std::array<std::uint64_t, 6> absorbed{};
+};
+
+// RunSummary (Simulation.hpp): what run() returns.
+struct RunSummary {
+    std::size_t steps_completed{0};
+    double final_time{0.0};
+    DiagnosticSample final_sample{};
+    bool steady_state_reached{false};
 };
 
 // TabulatedVectorField1D (PrescribedField.hpp): rows along one axis.
@@ -320,6 +344,9 @@ struct Simulation3DConfig {       // Simulation3D.hpp:26-66, the fields the loop
     UnitSystem units{UnitSystem::Normalized};
     std::filesystem::path initial_state_path{};
     std::optional<std::uint64_t> initial_state_signature{};
+    // run() writes scalars.csv there every output_interval steps (empty: none)
+    std::filesystem::path output_dir{};
+    std::size_t output_interval{10};
 };
 
 struct Simulation2DConfig {       // Simulation2D.hpp, structured periodic/Dirichlet
@@ -337,6 +364,8 @@ struct Simulation2DConfig {       // Simulation2D.hpp, structured periodic/Diric
     UnitSystem units{UnitSystem::Normalized};
     std::filesystem::path initial_state_path{};
     std::optional<std::uint64_t> initial_state_signature{};
+    std::filesystem::path output_dir{};
+    std::size_t output_interval{10};
 };
 
 struct SinusoidalVoltageConfig {  // Config.hpp:139-143
@@ -351,6 +380,8 @@ struct NamedCollisionConfig {
     CollisionConfig config{};
     std::string secondary_species{};
     std::string ion_species{};
+    // diagnostics name: channel columns are "<name>.<channel>" when set
+    std::string name{};
 };
 
 struct SpeciesConfig {            // Config.hpp:16-30
@@ -362,6 +393,10 @@ struct SpeciesConfig {            // Config.hpp:16-30
     double drift_velocity{0.0};
     double thermal_velocity{0.1};
     int precision{32};            // 64 keeps FP64 offsets and velocities
+    double init_x_min{0.0};
+    double init_x_max{-1.0};      // negative means the full domain
+    double drift_velocity_y{0.0};
+    double drift_velocity_z{0.0};
 };
 
 struct Config {                   // Config.hpp:170-203, the fields the loop uses
@@ -379,14 +414,36 @@ struct Config {                   // Config.hpp:170-203, the fields the loop use
     std::size_t max_particles_per_species{10000000};
     std::vector<NamedCollisionConfig> collision_models{};
     std::vector<SpeciesConfig> species{};
-    // external state files are rejected for 1D3V runs, as in the reference
+    // 1 (x only) or 3; external state files are 1D1V only, as in the reference
+    std::size_t velocity_dimensions{1};
+    UnitSystem units{UnitSystem::Normalized};
     std::filesystem::path initial_state_path{};
     std::optional<std::uint64_t> initial_state_signature{};
+    // run() writes scalars.csv, fields_<step>.csv, collisions.csv,
+    // boundary_losses.csv and power_transfer.csv there (empty: no output)
+    std::filesystem::path output_dir{};
+    std::size_t output_interval{10};
 };
 
 struct BoundaryLoss1D {           // Simulation.hpp:20-25 (counts and impact energy)
-    std::uint64_t left_particles{0}, right_particles{0};
-    double left_energy{0.0}, right_energy{0.0};
+    std::size_t absorbed_left{0};
+    std::size_t absorbed_right{0};
+    double kinetic_energy_left{0.0};
+    double kinetic_energy_right{0.0};
+};
+
+struct SpeciesPower1D {           // Simulation.hpp:27-29
+    double electric_work{0.0};
+};
+
+// CollisionDiagnostics (Collision.hpp:65-71): cumulative since initialize,
+// one column per channel ("<model>.<channel>" for named models).
+struct CollisionDiagnostics {
+    std::uint64_t candidates{0};
+    std::uint64_t null_collisions{0};
+    std::vector<std::string> channel_names{};
+    std::vector<std::uint64_t> channel_collisions{};
+    std::vector<double> channel_energy_change{};
 };
 
 // Simulation (1D, Simulation.hpp): electrodes, named MCC models with
@@ -396,17 +453,29 @@ public:
     explicit Simulation(Config cfg, std::shared_ptr<Device> device = nullptr);
     void initialize();
     void step();
-    void run();
+    // steps until step_count() == steps, writing the outputs of the
+    // reference's run() (Simulation.cpp:2902-3057) when output_dir is set
+    RunSummary run();
     std::vector<Particle> particles(std::size_t species) const;
     DiagnosticSample sample() const;
     Grid fields() const;
     BoundaryLoss1D boundary_losses(std::size_t species) const;
+    std::vector<BoundaryLoss1D> species_boundary_losses() const;
+    std::vector<SpeciesPower1D> species_power_transfer() const;
+    // the ledgers count from initialize() and survive a checkpoint
+    std::size_t boundary_loss_origin_step() const { return 0; }
+    std::size_t power_transfer_origin_step() const { return 0; }
     CollisionStepStatistics collision_statistics() const;
+    CollisionDiagnostics collision_diagnostics() const;
+    // binary checkpoint of the device state behind a physics signature
+    void save_checkpoint(const std::filesystem::path& path) const;
+    void load_checkpoint(const std::filesystem::path& path);
     double time() const { return time_; }
     std::size_t step_count() const { return step_; }
     const Config& config() const { return cfg_; }
 
 private:
+    void sync_clock();
     Config cfg_;
     std::shared_ptr<Device> dev_;
     double time_{0.0};
@@ -419,7 +488,9 @@ public:
     explicit Simulation2D(Simulation2DConfig cfg, std::shared_ptr<Device> device = nullptr);
     void initialize();
     void step();
-    void run();
+    RunSummary run();                 // scalars.csv when output_dir is set
+    void save_checkpoint(const std::filesystem::path& path) const;
+    void load_checkpoint(const std::filesystem::path& path);
     std::vector<Particle2D> particles(std::size_t species) const;
     DiagnosticSample sample() const;
     Mesh2D fields() const;
@@ -439,7 +510,9 @@ public:
     explicit Simulation3D(Simulation3DConfig cfg, std::shared_ptr<Device> device = nullptr);
     void initialize();
     void step();
-    void run();
+    RunSummary run();                 // scalars.csv when output_dir is set
+    void save_checkpoint(const std::filesystem::path& path) const;
+    void load_checkpoint(const std::filesystem::path& path);
     // replaces a species' particles (external state), before initialize()
     void set_particles(std::size_t species, const std::vector<Particle3D>& particles);
     std::vector<Particle3D> particles(std::size_t species) const;

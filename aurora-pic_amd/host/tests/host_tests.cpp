@@ -7,6 +7,9 @@
 //   aurora_host_tests --list     print the case names (no GPU use)
 //   aurora_host_tests --cpu <golden dir>   run the host-only cases (no GPU use)
 //   aurora_host_tests --aps-signature <file>   print a state file's signature
+//   aurora_host_tests --run1d <case.txt> <output dir>   run a 1D case file
+//                    through Simulation::run(), writing the reference's outputs
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <filesystem>
@@ -16,10 +19,12 @@
 #include <memory>
 #include <numeric>
 #include <random>
+#include <sstream>
 #include <string>
 #include <vector>
 
 #include "aurora_gpu.hpp"
+#include "diagnostics.hpp"
 #include "particle_state.hpp"
 
 using namespace pic::gpu;
@@ -255,10 +260,12 @@ void simulation1d_discharge_with_ionization(const Dev& d) {
     cfg.species = {SpeciesConfig{"electrons", -kEV, kME, w, 3000, 0.0, std::sqrt(8.0 * kEV / kME)},
                    SpeciesConfig{"ions", kEV, 6.67e-27, w, 3000, 0.0,
                                  std::sqrt(0.026 * kEV / 6.67e-27)}};
+    cfg.velocity_dimensions = 3;
     cfg.collision_models = {NamedCollisionConfig{"electrons", synthetic_gas(1e22), "electrons",
                                                  "ions"}};
     Simulation sim(cfg, d);
-    sim.run();
+    const RunSummary summary = sim.run();
+    require(summary.steps_completed == 100 && summary.final_sample.step == 100, "run summary");
     const auto st = sim.collision_statistics();
     require(st.channel_collisions.size() == 3, "channel statistics");
     require(st.channel_collisions[2] > 0, "no ionization sampled");
@@ -268,8 +275,8 @@ void simulation1d_discharge_with_ionization(const Dev& d) {
     require(e.size() == 3000 + st.secondaries && i.size() == 3000 + st.secondaries,
             "products appended to both species");
     const auto loss = sim.boundary_losses(0);
-    require(loss.left_particles + loss.right_particles > 0, "no electron reached a wall");
-    require(loss.left_energy + loss.right_energy > 0.0, "wall energy ledger");
+    require(loss.absorbed_left + loss.absorbed_right > 0, "no electron reached a wall");
+    require(loss.kinetic_energy_left + loss.kinetic_energy_right > 0.0, "wall energy ledger");
     const Grid g = sim.fields();
     require_near(g.phi()[0], 0.0, 1e-12, "grounded electrode");
 }
@@ -445,6 +452,47 @@ void aps_round_trip(const std::filesystem::path&) {
             "metadata file");
 }
 
+// The CSV writers of Simulation.cpp:154-310 on fixed values.
+void ledger_csv_writers(const std::filesystem::path&) {
+    require(csv_cell("plain") == "plain" && csv_cell("a,b") == "\"a,b\"" &&
+                csv_cell("q\"x") == "\"q\"\"x\"",
+            "csv_cell quoting");
+    std::ostringstream h;
+    write_collision_header(h, {"argon.elastic"}, true);
+    require(h.str() ==
+                "step,time,candidates,null_collisions,collisions_argon.elastic,"
+                "tracked_kinetic_energy_change_argon.elastic_J_m-2,cumulative_candidates,"
+                "cumulative_null_collisions,cumulative_collisions_argon.elastic,"
+                "cumulative_tracked_kinetic_energy_change_argon.elastic_J_m-2\n",
+            "collision header");
+    CollisionDiagnostics iv{3, 1, {"argon.elastic"}, {2}, {0.1}};
+    CollisionDiagnostics tot{7, 2, {"argon.elastic"}, {5}, {1.0 / 3.0}};
+    std::ostringstream c;
+    write_collision_sample(c, 10, 0.25, iv, tot);
+    require(c.str() == "10,0.25,3,1,2,0.10000000000000001,7,2,5,0.33333333333333331\n",
+            "collision row: " + c.str());
+    std::ostringstream b;
+    write_boundary_loss_header(b, {"e,1"}, false);
+    require(b.str().find(",\"absorbed_left_count_e,1\",") != std::string::npos,
+            "quoted species column");
+    std::ostringstream r;
+    write_boundary_loss_sample(r, 0, 0.0, 0, {-1.5}, {BoundaryLoss1D{}});
+    require(r.str() == "0,0,0,0,0,-0,-0,0,0\n", "boundary-loss row: " + r.str());
+    bool threw = false;
+    try {
+        write_boundary_loss_sample(r, 0, 0.0, 0, {}, {BoundaryLoss1D{}});
+    } catch (const std::logic_error&) {
+        threw = true;
+    }
+    require(threw, "species mismatch must throw");
+    std::ostringstream pw;
+    write_power_transfer_header(pw, {"electrons"}, true);
+    write_power_transfer_sample(pw, 5, 1e-9, 0, {SpeciesPower1D{-2.5e-20}});
+    require(pw.str() == "step,time,counter_origin_step,electric_work_electrons_J_m-2\n"
+                        "5,1.0000000000000001e-09,0,-2.4999999999999999e-20\n",
+            "power rows: " + pw.str());
+}
+
 void aps_errors(const std::filesystem::path&) {
     const auto dir = scratch_dir();
     const auto p = dir / "bad.aps";
@@ -487,6 +535,7 @@ const CpuCase kCpuCases[] = {
     {"aps_reference_example_signature", aps_reference_example_signature},
     {"aps_round_trip", aps_round_trip},
     {"aps_errors", aps_errors},
+    {"ledger_csv_writers", ledger_csv_writers},
 };
 
 // The 3D simulation initialised from a state file equals the same records
@@ -550,9 +599,116 @@ void simulation3d_external_state(const Dev& d) {
         },
         "signature mismatch", "pinned signature");
     Config one;
+    one.velocity_dimensions = 3;
     one.initial_state_path = path;
     require_throws_contains([&] { Simulation s(one, d); },
                             "not yet available for 1D3V", "1D3V external state");
+    one.velocity_dimensions = 2;
+    one.initial_state_path.clear();
+    require_throws_contains([&] { Simulation s(one, d); },
+                            "velocity_dimensions must be 1 or 3", "velocity dimensions");
+}
+
+// test_core.cpp:3282-3380: one neutral particle per species leaves through
+// each wall in the first step; the ledgers, their CSV contracts and a
+// restart from a checkpoint.
+void simulation1d_wall_ledgers(const Dev& d) {
+    const auto dir = std::filesystem::temp_directory_path() / "aurora_test_1d_boundary_losses";
+    std::filesystem::remove_all(dir);
+    Config cfg;
+    cfg.nx = 3;
+    cfg.length = 1.0;
+    cfg.dt = 0.2;
+    cfg.steps = 1;
+    cfg.output_interval = 1;
+    cfg.output_dir = dir;
+    cfg.boundary = Boundary::Dirichlet;
+    cfg.max_particles_per_species = 1;
+    cfg.species = {SpeciesConfig{"left", 0.0, 2.0, 3.0, 1, -10.0, 0.0, 64},
+                   SpeciesConfig{"right", 0.0, 2.0, 3.0, 1, 10.0, 0.0, 64}};
+    Simulation sim(cfg, d);
+    const auto summary = sim.run();
+    const auto losses = sim.species_boundary_losses();
+    require(summary.final_sample.live_particles == 0 && losses.size() == 2 &&
+                losses[0].absorbed_left == 1 && losses[0].absorbed_right == 0 &&
+                losses[1].absorbed_left == 0 && losses[1].absorbed_right == 1,
+            "1D species-resolved wall counts are wrong");
+    require_near(losses[0].kinetic_energy_left, 300.0, 1e-12, "left-wall impact energy");
+    require_near(losses[1].kinetic_energy_right, 300.0, 1e-12, "right-wall impact energy");
+    const auto power = sim.species_power_transfer();
+    require_near(power[0].electric_work, 0.0, 1e-14, "neutral particle acquired work");
+    require_near(power[1].electric_work, 0.0, 1e-14, "neutral particle acquired work");
+    auto text = [](const std::filesystem::path& p) {
+        std::ifstream in(p);
+        return std::string((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    };
+    auto lines = [](const std::string& t) { return std::count(t.begin(), t.end(), '\n'); };
+    const std::string bl = text(dir / "boundary_losses.csv");
+    require(bl.find("absorbed_left_count_left,absorbed_right_count_left,"
+                    "absorbed_left_charge_left_normalized,absorbed_right_charge_left_normalized,"
+                    "absorbed_left_kinetic_energy_left_normalized,"
+                    "absorbed_right_kinetic_energy_left_normalized") != std::string::npos &&
+                lines(bl) == 3,
+            "1D wall-loss CSV contract is wrong");
+    const std::string pt = text(dir / "power_transfer.csv");
+    require(pt.find("step,time,counter_origin_step,electric_work_left_normalized,"
+                    "electric_work_right_normalized\n") == 0 &&
+                lines(pt) == 3,
+            "1D power-transfer CSV contract is wrong");
+    const std::string sc = text(dir / "scalars.csv");
+    require(sc.find("step,time,kinetic_energy,field_energy,total_energy,charge_l1,"
+                    "live_particles,phi_left,phi_right,live_particles_left,"
+                    "live_particles_right\n") == 0 && lines(sc) == 3,
+            "1D scalar CSV contract is wrong");
+    require(std::filesystem::exists(dir / "fields_1.csv"), "final field diagnostics");
+    sim.save_checkpoint(dir / "checkpoint_1.ckpt");
+    Simulation restarted(cfg, d);
+    restarted.load_checkpoint(dir / "checkpoint_1.ckpt");
+    const auto rl = restarted.species_boundary_losses();
+    require(rl[0].absorbed_left == 1 && rl[1].absorbed_right == 1 &&
+                restarted.boundary_loss_origin_step() == 0 && restarted.step_count() == 1,
+            "1D checkpoint lost species-resolved wall counts");
+    require_near(rl[0].kinetic_energy_left, 300.0, 1e-12, "checkpoint lost wall energy");
+    std::filesystem::remove_all(dir);
+}
+
+// test_core.cpp:3470-3540: the electric work on collisionless survivors
+// equals their kinetic-energy change, and a restart continues it.
+void simulation1d_power_transfer(const Dev& d) {
+    const auto dir = std::filesystem::temp_directory_path() / "aurora_test_1d_power";
+    std::filesystem::remove_all(dir);
+    std::filesystem::create_directories(dir);
+    Config cfg;
+    cfg.nx = 17;
+    cfg.length = 1.0;
+    cfg.dt = 1e-4;
+    cfg.steps = 1;
+    cfg.boundary = Boundary::Dirichlet;
+    cfg.max_particles_per_species = 32;
+    SpeciesConfig e{"electrons", -1.0, 1.0, 1.0, 32, 0.0, 0.0, 64};
+    e.init_x_min = 0.1;
+    e.init_x_max = 0.9;
+    cfg.species = {e};
+    Simulation continuous(cfg, d);
+    continuous.initialize();
+    const double e0 = continuous.sample().kinetic_energy;
+    continuous.step();
+    const double change = continuous.sample().kinetic_energy - e0;
+    const double work = continuous.species_power_transfer()[0].electric_work;
+    require(std::abs(work) > 1e-12, "zero-work case");
+    require_near(work, change, 1e-12, "survivor work does not equal its energy change");
+    continuous.save_checkpoint(dir / "power.ckpt");
+    Simulation restarted(cfg, d);
+    restarted.load_checkpoint(dir / "power.ckpt");
+    require(restarted.power_transfer_origin_step() == 0, "power-transfer origin");
+    require_near(restarted.species_power_transfer()[0].electric_work, work, 1e-14,
+                 "checkpoint lost cumulative electric work");
+    continuous.step();
+    restarted.step();
+    require_near(restarted.species_power_transfer()[0].electric_work,
+                 continuous.species_power_transfer()[0].electric_work, 1e-14,
+                 "power transfer is not restart deterministic");
+    std::filesystem::remove_all(dir);
 }
 
 struct Case {
@@ -574,9 +730,62 @@ const Case kCases[] = {
     {"swarm_alone_equals_batched", swarm_alone_equals_batched},
     {"error_paths", error_paths},
     {"simulation3d_external_state", simulation3d_external_state},
+    {"simulation1d_wall_ledgers", simulation1d_wall_ledgers},
+    {"simulation1d_power_transfer", simulation1d_power_transfer},
 };
 
 }  // namespace
+
+// A 1D case file (written by tests/golden/make_run1d_golden.py): one
+// "key values..." line per setting, one "species name charge mass weight
+// particles drift thermal x_min x_max" line per species.  FP64 particles.
+void run1d_case(const std::filesystem::path& case_file, const std::filesystem::path& out) {
+    std::ifstream in(case_file);
+    if (!in) throw std::runtime_error("cannot open " + case_file.string());
+    Config cfg;
+    cfg.permittivity = 1.0;
+    std::string line;
+    size_t most = 1;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::string key;
+        if (!(ls >> key)) continue;
+        if (key == "nx") ls >> cfg.nx;
+        else if (key == "length") ls >> cfg.length;
+        else if (key == "dt") ls >> cfg.dt;
+        else if (key == "steps") ls >> cfg.steps;
+        else if (key == "output_interval") ls >> cfg.output_interval;
+        else if (key == "velocity_dimensions") ls >> cfg.velocity_dimensions;
+        else if (key == "seed") ls >> cfg.seed;
+        else if (key == "dirichlet") {
+            int v = 0;
+            ls >> v;
+            cfg.boundary = v ? Boundary::Dirichlet : Boundary::Periodic;
+        } else if (key == "electrodes") {
+            ls >> cfg.phi_left >> cfg.phi_left_drive.amplitude >> cfg.phi_left_drive.frequency >>
+                cfg.phi_left_drive.phase >> cfg.phi_right >> cfg.phi_right_drive.amplitude >>
+                cfg.phi_right_drive.frequency >> cfg.phi_right_drive.phase;
+        } else if (key == "initial_state") {
+            std::string f;
+            ls >> f;
+            cfg.initial_state_path = case_file.parent_path() / f;
+        } else if (key == "species") {
+            SpeciesConfig sc;
+            sc.precision = 64;
+            ls >> sc.name >> sc.charge >> sc.mass >> sc.weight >> sc.particles >>
+                sc.drift_velocity >> sc.thermal_velocity >> sc.init_x_min >> sc.init_x_max;
+            most = std::max(most, sc.particles);
+            cfg.species.push_back(sc);
+        } else {
+            throw std::runtime_error("unknown case key " + key);
+        }
+        if (ls.fail()) throw std::runtime_error("malformed case line: " + line);
+    }
+    cfg.max_particles_per_species = most;
+    cfg.output_dir = out;
+    Simulation sim(cfg, std::make_shared<Device>(0));
+    sim.run();
+}
 
 int main(int argc, char** argv) {
     const std::string arg = argc > 1 ? argv[1] : "";
@@ -591,6 +800,15 @@ int main(int argc, char** argv) {
             std::printf("%llu\n", static_cast<unsigned long long>(s.signature));
         } catch (const std::exception& e) {
             std::printf("error: %s\n", e.what());
+        }
+        return 0;
+    }
+    if (arg == "--run1d" && argc > 3) {
+        try {
+            run1d_case(argv[2], argv[3]);
+        } catch (const std::exception& e) {
+            std::printf("error: %s\n", e.what());
+            return 1;
         }
         return 0;
     }

@@ -448,6 +448,9 @@ class Reference:
         L.ref_sim3d_time.argtypes = [_i64p, _dp, C.c_double, C.c_double, C.c_int, C.c_int,
                                      _dp, C.c_int64, _dp, _dp, C.c_int, _dp, _i64p,
                                      C.c_char_p, C.c_int]
+        L.ref_run1d.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, C.c_double, C.c_double,
+                                C.c_int64, C.c_int64, C.c_int, _dp, C.c_int, C.c_uint64, C.c_int,
+                                C.POINTER(C.c_char_p), _dp, C.c_char_p, C.c_int]
 
     def deposit(self, n, L, bc, pos, charge, weight, alive=None):
         n, L = i64(n), f64(L)
@@ -590,6 +593,26 @@ class Reference:
         if rc:
             raise RuntimeError(err.value.decode())
         return sec.value, live.value
+
+
+    RUN1D_SPECIES_KEYS = ("charge", "mass", "weight", "particles", "drift_velocity",
+                          "thermal_velocity", "init_x_min", "init_x_max")
+
+    def run1d(self, case: dict, output_dir: str, aps_path: str = ""):
+        """The reference's 1D Simulation::run() on `case` (the keys of a
+        tests/golden/run1d case file), writing its outputs into output_dir."""
+        sp = case["species"]
+        names = (C.c_char_p * len(sp))(*[s["name"].encode() for s in sp])
+        rows = f64([[s[k] for k in self.RUN1D_SPECIES_KEYS] for s in sp]).reshape(-1)
+        el = f64(case["electrodes"])
+        err = C.create_string_buffer(512)
+        rc = self.lib.ref_run1d(output_dir.encode(), aps_path.encode(), case["nx"],
+                                case["length"], case["dt"], case["steps"],
+                                case["output_interval"], int(case["dirichlet"]), _ptr(el, _dp),
+                                case["velocity_dimensions"], case["seed"], len(sp), names,
+                                _ptr(rows, _dp), err, 512)
+        if rc:
+            raise RuntimeError(err.value.decode())
 
 
 def reference_available() -> bool:

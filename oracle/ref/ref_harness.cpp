@@ -13,6 +13,7 @@
 #include "pic/ParticleState.hpp"
 #include "pic/PrescribedField.hpp"
 #include "pic/Pusher.hpp"
+#include "pic/Simulation.hpp"
 #include "pic/Simulation3D.hpp"
 #include "pic/Species.hpp"
 #include "pic/Swarm.hpp"
@@ -464,6 +465,56 @@ int ref_sim3d_time(const int64_t* n, const double* L, double dt,
         int64_t live = 0;
         for (const auto& sp : sim.species()) live += static_cast<int64_t>(sp.live_count());
         *live_particles = live;
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(err, errlen, e.what());
+        return -1;
+    }
+}
+
+// Runs the reference's own 1D Simulation::run() (collisionless) and lets it
+// write its outputs (scalars.csv, fields_<step>.csv, boundary_losses.csv,
+// power_transfer.csv, ...) into output_dir.  electrodes: left then right
+// {offset, amplitude, frequency, phase}.  species: nspecies rows of
+// (charge, mass, weight, particles, drift_velocity, thermal_velocity,
+// init_x_min, init_x_max); names: nspecies NUL-terminated strings.  An
+// empty aps_path loads the species the reference's way from seed.
+int ref_run1d(const char* output_dir, const char* aps_path, int64_t nx, double length, double dt,
+              int64_t steps, int64_t output_interval, int dirichlet, const double* electrodes,
+              int velocity_dimensions, uint64_t seed, int nspecies, const char* const* names,
+              const double* species, char* err, int errlen) {
+    try {
+        pic::Config cfg;
+        cfg.velocity_dimensions = static_cast<std::size_t>(velocity_dimensions);
+        cfg.nx = static_cast<std::size_t>(nx);
+        cfg.length = length;
+        cfg.dt = dt;
+        cfg.steps = static_cast<std::size_t>(steps);
+        cfg.output_interval = static_cast<std::size_t>(output_interval);
+        cfg.output_dir = output_dir;
+        cfg.boundary = dirichlet ? pic::Boundary::Dirichlet : pic::Boundary::Periodic;
+        cfg.phi_left = electrodes[0];
+        cfg.phi_left_drive = {electrodes[1], electrodes[2], electrodes[3]};
+        cfg.phi_right = electrodes[4];
+        cfg.phi_right_drive = {electrodes[5], electrodes[6], electrodes[7]};
+        cfg.seed = static_cast<unsigned>(seed);
+        if (aps_path && *aps_path) cfg.initial_state_path = aps_path;
+        for (int k = 0; k < nspecies; ++k) {
+            const double* r = species + 8 * k;
+            pic::SpeciesConfig sc;
+            sc.name = names[k];
+            sc.charge = r[0];
+            sc.mass = r[1];
+            sc.weight = r[2];
+            sc.particles = static_cast<std::size_t>(r[3]);
+            sc.drift_velocity = r[4];
+            sc.thermal_velocity = r[5];
+            sc.init_x_min = r[6];
+            sc.init_x_max = r[7];
+            cfg.species.push_back(sc);
+        }
+        pic::Simulation sim(cfg);
+        sim.run();
         return 0;
     } catch (const std::exception& e) {
         set_error(err, errlen, e.what());
