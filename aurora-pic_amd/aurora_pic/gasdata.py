@@ -496,14 +496,30 @@ def apply_channel_options(ctx, ds: GasDataset, model: int) -> None:
                                 model=model)
 
 
+SWARM_ALLOCATION_LIMIT = 10_000_000      # Swarm.cpp:28
+
+
+def effective_population_limit(cfg: SwarmConfig) -> int:
+    """The branching population limit (Swarm.cpp:240-262): the configured
+    one, else the larger of twice the particles and particles + 1024, within
+    the allocation safety limit."""
+    if cfg.population_limit != 0:
+        return cfg.population_limit
+    if cfg.particles >= SWARM_ALLOCATION_LIMIT:
+        return cfg.particles
+    capacity = SWARM_ALLOCATION_LIMIT - cfg.particles
+    doubled = capacity if cfg.particles > capacity // 2 else 2 * cfg.particles
+    buffer = min(capacity, 1024)
+    buffered = capacity if cfg.particles > capacity - buffer else cfg.particles + 1024
+    return min(capacity, max(doubled, buffered))
+
+
 def swarm_requests(cfg: SwarmConfig):
     """One device request per reduced field, seeded as
     run_swarm_benchmark seeds its fields (config seed + field index,
-    Swarm.cpp:1513-1515)."""
+    Swarm.cpp:1513-1515).  Fixed populations go to swarm_submit, branching
+    ones to swarm_branching with effective_population_limit(cfg)."""
     from .hip import SwarmRequest
-    if cfg.population_model != "fixed_population_no_avalanche":
-        raise GasDataError("the device swarm supports population_model = "
-                           "fixed_population_no_avalanche")
     out = []
     for index, td in enumerate(cfg.reduced_fields_td):
         out.append(SwarmRequest(reduced_field_td=td, timestep=cfg.timestep,

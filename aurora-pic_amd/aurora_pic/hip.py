@@ -23,7 +23,8 @@ SCATTER_ISOTROPIC, SCATTER_BACKWARD, SCATTER_HENYEY_GREENSTEIN = range(3)
 FRAME_PROJECTILE, FRAME_CENTER_OF_MASS = range(2)
 SORT_FIXED, SORT_ADAPTIVE = range(2)
 (OK, ERR_INVALID_ARGUMENT, ERR_HIP, ERR_RATE_ABOVE_BOUND, ERR_CANDIDATE_LIMIT, ERR_CAPACITY,
- ERR_B_OUT_OF_RANGE, ERR_ENERGY_ABOVE_MAX, ERR_UNSUPPORTED, ERR_NOT_FOUND) = range(10)
+ ERR_B_OUT_OF_RANGE, ERR_ENERGY_ABOVE_MAX, ERR_UNSUPPORTED, ERR_NOT_FOUND,
+ ERR_POPULATION) = range(11)
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
@@ -72,6 +73,25 @@ class SwarmResult(C.Structure):
         ("status", C.c_int),
         ("longitudinal_diffusion", C.c_double),
         ("transverse_diffusion", C.c_double),
+    ]
+
+
+class SwarmGrowth(C.Structure):
+    _fields_ = [
+        ("initial_weight", C.c_double),
+        ("final_weight", C.c_double),
+        ("final_particles", C.c_int64),
+        ("growth_rate", C.c_double),
+        ("growth_rate_se", C.c_double),
+        ("attachment_rate", C.c_double),
+        ("attachment_rate_se", C.c_double),
+        ("net_creation_rate", C.c_double),
+        ("net_creation_rate_se", C.c_double),
+        ("townsend_available", C.c_int),
+        ("townsend_growth", C.c_double),
+        ("townsend_growth_se", C.c_double),
+        ("townsend_rate_balance", C.c_double),
+        ("townsend_rate_balance_se", C.c_double),
     ]
 
 
@@ -148,6 +168,8 @@ def load(path: str = LIB_PATH):
         "aurora_hip_swarm_submit": [ctx, C.POINTER(SwarmRequest), C.c_uint64, _i64p],
         "aurora_hip_swarm_advance": [ctx, C.c_int64, _i64p],
         "aurora_hip_swarm_poll": [ctx, C.c_int64, _ip, C.POINTER(SwarmResult)],
+        "aurora_hip_swarm_branching": [ctx, C.POINTER(SwarmRequest), C.c_int64,
+                                       C.POINTER(SwarmResult), C.POINTER(SwarmGrowth)],
         "aurora_hip_ensemble_create": [ctx, C.c_int64, _i64p, _i64p],
         "aurora_hip_ensemble_load": [ctx, C.c_int64, C.c_int64, C.c_uint64],
         "aurora_hip_ensemble_graph": [ctx, C.c_int64, C.c_int],
@@ -489,6 +511,15 @@ class Context:
         r = SwarmResult()
         self._check(self.lib.aurora_hip_swarm_poll(self.h, handle, C.byref(done), C.byref(r)))
         return bool(done.value), r
+
+    def swarm_branching(self, request: SwarmRequest, population_limit):
+        """One request with the branching / resampled population model, run
+        to completion: (SwarmResult, SwarmGrowth)."""
+        r, g = SwarmResult(), SwarmGrowth()
+        self._check(self.lib.aurora_hip_swarm_branching(self.h, C.byref(request),
+                                                        int(population_limit), C.byref(r),
+                                                        C.byref(g)))
+        return r, g
 
     def swarm_footprint(self, request: SwarmRequest):
         return int(self.lib.aurora_hip_swarm_footprint(C.byref(request)))

@@ -7,9 +7,11 @@ output: the run description (dataset, config, seeds), the collision model
 signature (NullCollisionModel's FNV-1a physics fingerprint,
 Collision.cpp:365-577), and the statistics -- drift and mean energy with
 block standard errors, diffusion, per-channel counts, rates and Poisson
-errors.  The fixed population has no growth, attachment, Townsend or
-spatial-flux estimates; those columns carry the reference's values for that
-mode (zeros, "no" and empty cells).
+errors.  A fixed population has no growth, attachment or Townsend
+estimates and a branching one no diffusion; those columns carry the
+reference's values for the mode (zeros, "no" and empty cells).  The
+spatial-flux experiment is not built: its columns are always the
+reference's "not run" cells.
 """
 from __future__ import annotations
 
@@ -163,16 +165,19 @@ def header(ds: gasdata.GasDataset) -> List[str]:
     return cols
 
 
-def rows(cfg: gasdata.SwarmConfig, ds: gasdata.GasDataset, results: Sequence) -> List[List[str]]:
+def rows(cfg: gasdata.SwarmConfig, ds: gasdata.GasDataset, results: Sequence,
+         growth: Sequence = None) -> List[List[str]]:
     """One row of cells per reduced field; results[i] is the device result
-    (hip.SwarmResult) of cfg.reduced_fields_td[i], seeded seed + i."""
+    (hip.SwarmResult) of cfg.reduced_fields_td[i], seeded seed + i, and for
+    population_model = branching_resampled growth[i] its hip.SwarmGrowth."""
     if len(results) == 0:
         raise ValueError("cannot write an empty swarm result set")
-    if cfg.population_model != "fixed_population_no_avalanche":
-        raise ValueError("the device swarm runs population_model = "
-                         "fixed_population_no_avalanche")
+    branching = cfg.population_model == "branching_resampled"
+    if branching and (growth is None or len(growth) != len(results)):
+        raise ValueError("a branching swarm result needs its growth statistics")
     sig = collision_model_signature(ds, cfg)
     vstd = neutral_velocity_stddev(ds, cfg)
+    limit = gasdata.effective_population_limit(cfg) if branching else 0
     out = []
     for i, (td, r) in enumerate(zip(cfg.reduced_fields_td, results)):
         efield = td * TOWNSEND * cfg.neutral_density
@@ -186,22 +191,36 @@ def rows(cfg: gasdata.SwarmConfig, ds: gasdata.GasDataset, results: Sequence) ->
             elif c.process == gasdata.ATTACHMENT:
                 att += r.rate[k]
                 att_se = math.hypot(att_se, r.rate_se[k])
+        if branching:
+            g = growth[i]
+            diffusion = ["no", "", ""]
+            weights = [_num(g.initial_weight), _num(g.final_weight),
+                       _num(int(g.final_particles)), _num(g.growth_rate),
+                       _num(g.growth_rate_se)]
+            townsend = (["yes", _num(g.townsend_growth), _num(g.townsend_growth_se),
+                         _num(g.townsend_rate_balance), _num(g.townsend_rate_balance_se)]
+                        if g.townsend_available else ["no", "", "", "", ""])
+        else:
+            diffusion = ["yes", _num(r.longitudinal_diffusion), _num(r.transverse_diffusion)]
+            weights = [_num(cfg.particles), _num(float(cfg.particles)), _num(cfg.particles),
+                       "0", "0"]
+            townsend = ["no", "", "", "", ""]
         cells = [_cell(ds.dataset_id), _cell(ds.dataset_version), _cell(ds.gas_name),
                  _cell(ds.retrieved), _cell(ds.data_provenance), _cell(ds.citation),
                  _cell(ds.license), _cell(cfg.gas_data_file), _cell(cfg.population_model),
                  _num(cfg.neutral_density), _num(cfg.neutral_temperature), _num(vstd),
                  _num(NEUTRAL_SPEED_LIMIT_SIGMA), _num(cfg.timestep), _num(cfg.steps),
-                 _num(cfg.burn_in_steps), _num(cfg.particles), "0", _num(cfg.work_item_limit),
+                 _num(cfg.burn_in_steps), _num(cfg.particles), _num(limit),
+                 _num(cfg.work_item_limit),
                  _num(cfg.seed + i), _num(cfg.spatial_histories), _num(cfg.spatial_length_m),
                  _num(cfg.spatial_bins), _num(cfg.spatial_fit_begin_bin), "0",
                  _num(cfg.spatial_max_steps), _num(cfg.spatial_work_item_limit),
                  _num(cfg.spatial_min_r_squared), _cell(cfg.spatial_profile_file),
                  _num(td), str(sig), _num(efield), _num(-r.drift), _num(r.drift_se),
                  _num(r.drift), _num(mobility), _num(r.mean_energy_ev), _num(r.mean_energy_se),
-                 "yes", _num(r.longitudinal_diffusion), _num(r.transverse_diffusion),
-                 _num(cfg.particles), _num(float(cfg.particles)), _num(cfg.particles), "0", "0",
+                 *diffusion, *weights,
                  _num(ion), _num(ion_se), _num(att), _num(att_se), _num(ion - att),
-                 _num(math.hypot(ion_se, att_se)), "no", "", "", "", "", "no", "", "", "",
+                 _num(math.hypot(ion_se, att_se)), *townsend, "no", "", "", "",
                  "0", "0", "0", _num(r.max_energy_ev), _num(int(r.candidates)),
                  _num(int(r.nulls))]
         for k in range(len(ds.channels)):
@@ -211,9 +230,9 @@ def rows(cfg: gasdata.SwarmConfig, ds: gasdata.GasDataset, results: Sequence) ->
 
 
 def write_swarm_benchmark_csv(path: str, cfg: gasdata.SwarmConfig, ds: gasdata.GasDataset,
-                              results: Sequence) -> None:
+                              results: Sequence, growth: Sequence = None) -> None:
     """write_swarm_benchmark_csv (Swarm.cpp:1526-1716) for device results."""
-    body = rows(cfg, ds, results)
+    body = rows(cfg, ds, results, growth)
     with open(path, "w") as f:
         f.write(",".join(header(ds)) + "\n")
         for cells in body:

@@ -42,6 +42,9 @@ enum : uint32_t {
     DEV_ERR_CAPACITY = 8u,
     DEV_ERR_ENERGY = 16u,
     DEV_ERR_NONFINITE = 32u,
+    DEV_ERR_POPULATION = 64u,     // branching swarm: population limit
+    DEV_ERR_EXTINCT = 128u,       // branching swarm: no electron left
+    DEV_ERR_PRODUCTS = 256u,      // more secondaries in one step than staged
 };
 
 constexpr int kMaxChannels = 8;

@@ -19,8 +19,12 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
+
+#include <rocprim/rocprim.hpp>
 
 #include "mcc_device.hpp"
 
@@ -385,6 +389,246 @@ void finalize(Context& c, SwarmState& s, int64_t handle, RequestState& r) {
     s.bytes_in_use -= r.footprint;
 }
 
+// validate_config (Swarm.cpp:277-559), the checks common to both
+// population models
+void validate_request(const SwarmState& s, const aurora_hip_swarm_request* r) {
+    AUR_REQUIRE(std::isfinite(r->timestep) && r->timestep > 0.0, AURORA_ERR_INVALID_ARGUMENT,
+                "swarm timestep must be positive and finite");
+    AUR_REQUIRE(s.gas.max_frequency * r->timestep <= 0.1, AURORA_ERR_INVALID_ARGUMENT,
+                "swarm max_frequency * timestep must not exceed 0.1");
+    AUR_REQUIRE(r->steps > 0 && r->burn_in < r->steps, AURORA_ERR_INVALID_ARGUMENT,
+                "swarm steps must exceed burn_in_steps");
+    AUR_REQUIRE(r->steps < 2147483647, AURORA_ERR_INVALID_ARGUMENT, "swarm steps too large");
+    AUR_REQUIRE(r->particles > 0, AURORA_ERR_INVALID_ARGUMENT,
+                "swarm particles must be positive");
+    AUR_REQUIRE(r->blocks > 0 && r->blocks <= r->steps - r->burn_in,
+                AURORA_ERR_INVALID_ARGUMENT,
+                "swarm block statistics require at least one sample per block");
+    AUR_REQUIRE(std::isfinite(r->reduced_field_td) && r->reduced_field_td >= 0.0,
+                AURORA_ERR_INVALID_ARGUMENT,
+                "swarm reduced fields must be finite and non-negative");
+    AUR_REQUIRE(std::isfinite(r->initial_mean_energy_ev) && r->initial_mean_energy_ev >= 0.0,
+                AURORA_ERR_INVALID_ARGUMENT,
+                "swarm initial_mean_energy_ev must be finite and non-negative");
+    AUR_REQUIRE(std::isfinite(r->max_energy_ev) && r->max_energy_ev > 0.0,
+                AURORA_ERR_INVALID_ARGUMENT, "swarm max_energy_ev must be positive and finite");
+    AUR_REQUIRE(r->max_energy_ev <= s.table_max_ev, AURORA_ERR_INVALID_ARGUMENT,
+                "swarm max_energy_ev exceeds channel table coverage");
+}
+
+// ------------------------------------------- branching / resampled population
+// Swarm.cpp:1063-1104 and systematic_resample (616-643).  Every electron of
+// a step carries the same weight w: secondaries inherit their parent's and
+// resampling gives all outputs W / N.  So the population is N velocities
+// plus one weight, and the systematic resample of the M survivors and
+// secondaries (equal weights) takes for output j the source
+// ceil((u + j) M / N) - 1, where the cumulative weight first reaches the
+// threshold (u + j) W / N.  One step is five launches: collide, scan of the
+// packed (secondaries, alive) counts, gather (survivors in order, then the
+// secondaries in parent order), resample, and a one-workgroup fold of the
+// step's tallies and samples in a fixed order.
+constexpr int kBr = 256;
+
+struct BranchArgs {
+    float* v;                    // the population, 3 x N (SoA)
+    float* pre;                  // survivors then secondaries, 3 x cap (SoA)
+    float* sec;                  // per slot: 2 secondaries x 3
+    long long* key;              // per slot: (secondaries << 32) | alive
+    long long* off;              // exclusive scan of key
+    unsigned long long* step_counts;  // [candidates, nulls, chan...] of the step
+    unsigned long long* totals;       // the same, summed over sampled steps
+    unsigned* max_energy_bits;
+    uint32_t* status;
+    double* part;                // per resample block: sum v_x, sum energy (eV)
+    double* state;               // [w, exposure, weighted[8], squared[8]]
+    double* samples;             // [3][sampling]: mean v_x, mean energy, log W
+    int64_t N, cap, limit;
+    float half_kick, dt, max_energy_ev, ev_per_v2;
+    uint64_t seed;
+    int step, burn_in, sampling;
+    uint32_t attach_mask;        // channels whose collision removes the electron
+};
+
+template <int NCH>
+__global__ void __launch_bounds__(kBr) k_br_collide(MccParams m, BranchArgs a) {
+    __shared__ MccLds lds;
+    const XsView X = stage_tables(m, lds);
+    const bool sampling = a.step >= a.burn_in;
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBr; base < a.N;
+         base += static_cast<int64_t>(gridDim.x) * kBr) {
+        const int64_t p = base + threadIdx.x;
+        const bool active = p < a.N;
+        MccTally<NCH> t{};
+        uint32_t err = 0;
+        if (active) {
+            float v[3] = {a.v[p], a.v[a.N + p], a.v[2 * a.N + p]};
+            float emax = 0.f;
+            v[0] += a.half_kick;
+            const float e_pre = a.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            emax = fmaxf(emax, e_pre);
+            if (e_pre > a.max_energy_ev) err |= DEV_ERR_ENERGY;
+            Philox rng;
+            rng.init(a.seed, 0u, static_cast<uint32_t>(p), static_cast<uint32_t>(a.step));
+            mcc_collide<NCH>(m, X, lds.chan, v, a.dt, rng, t);
+            err |= t.err;
+            v[0] += a.half_kick;
+            const float e = a.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            emax = fmaxf(emax, e);
+            if (e > a.max_energy_ev) err |= DEV_ERR_ENERGY;
+            long long alive = 1;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c)
+                if (((a.attach_mask >> c) & 1u) && t.chan[c]) alive = 0;
+            int ns = t.nsec;
+            if (ns > 2) {
+                err |= DEV_ERR_PRODUCTS;
+                ns = 2;
+            }
+            for (int k = 0; k < ns; ++k)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) a.sec[6 * p + 3 * k + d] = t.sec[k][d];
+            a.v[p] = v[0];
+            a.v[a.N + p] = v[1];
+            a.v[2 * a.N + p] = v[2];
+            a.key[p] = (static_cast<long long>(ns) << 32) | alive;
+            atomicMax(a.max_energy_bits, __float_as_uint(emax));
+            if (err) atomicOr(a.status, err);
+        }
+        if (sampling) {
+            unsigned long long cnt[2 + NCH];
+            cnt[0] = active ? t.candidates : 0;
+            cnt[1] = active ? t.nulls : 0;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) cnt[2 + c] = active ? t.chan[c] : 0;
+#pragma unroll
+            for (int k = 0; k < 2 + NCH; ++k) {
+                unsigned long long x = cnt[k];
+                for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+                if ((threadIdx.x & 63) == 0 && x) atomicAdd(a.step_counts + k, x);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void br_totals(const BranchArgs& a, int64_t& A, int64_t& S) {
+    const long long tot = a.off[a.N - 1] + a.key[a.N - 1];
+    A = tot & 0xffffffffll;
+    S = tot >> 32;
+}
+
+__global__ void __launch_bounds__(kBr) k_br_gather(BranchArgs a) {
+    int64_t A, S;
+    br_totals(a, A, S);
+    for (int64_t p = blockIdx.x * static_cast<int64_t>(kBr) + threadIdx.x; p < a.N;
+         p += static_cast<int64_t>(gridDim.x) * kBr) {
+        const long long k = a.key[p], o = a.off[p];
+        const int ns = static_cast<int>(k >> 32);
+        if (k & 1) {
+            const int64_t d = o & 0xffffffffll;
+            a.pre[d] = a.v[p];
+            a.pre[a.cap + d] = a.v[a.N + p];
+            a.pre[2 * a.cap + d] = a.v[2 * a.N + p];
+        }
+        const int64_t d0 = A + (o >> 32);
+        for (int i = 0; i < ns; ++i) {
+            a.pre[d0 + i] = a.sec[6 * p + 3 * i];
+            a.pre[a.cap + d0 + i] = a.sec[6 * p + 3 * i + 1];
+            a.pre[2 * a.cap + d0 + i] = a.sec[6 * p + 3 * i + 2];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBr) k_br_resample(BranchArgs a) {
+    int64_t A, S;
+    br_totals(a, A, S);
+    const int64_t M = A + S;
+    Philox rng;
+    rng.init(a.seed, 1u, 0xFFFFFFFFu, static_cast<uint32_t>(a.step));
+    const double u = static_cast<double>(rng.u01());
+    double sums[2] = {0.0, 0.0};
+    for (int64_t j = blockIdx.x * static_cast<int64_t>(kBr) + threadIdx.x; j < a.N;
+         j += static_cast<int64_t>(gridDim.x) * kBr) {
+        if (M == 0) break;                     // extinct: the fold reports it
+        int64_t src = j;
+        if (M != a.N) {
+            src = static_cast<int64_t>(ceil((u + static_cast<double>(j)) *
+                                            static_cast<double>(M) / static_cast<double>(a.N))) - 1;
+            src = src < 0 ? 0 : (src > M - 1 ? M - 1 : src);
+        }
+        const float vx = a.pre[src], vy = a.pre[a.cap + src], vz = a.pre[2 * a.cap + src];
+        a.v[j] = vx;
+        a.v[a.N + j] = vy;
+        a.v[2 * a.N + j] = vz;
+        sums[0] += vx;
+        sums[1] += static_cast<double>(a.ev_per_v2) *
+                   (static_cast<double>(vx) * vx + static_cast<double>(vy) * vy +
+                    static_cast<double>(vz) * vz);
+    }
+    const double r = block_sum<2, kBr>(sums);
+    if (threadIdx.x < 2) a.part[2 * blockIdx.x + threadIdx.x] = r;
+}
+
+// one workgroup: the step's checks, weights, tallies and samples
+__global__ void __launch_bounds__(kBr) k_br_fold(BranchArgs a, int blocks, int nch) {
+    int64_t A, S;
+    br_totals(a, A, S);
+    const int64_t M = A + S;
+    double v[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < blocks; b += kBr) {
+        v[0] += a.part[2 * b];
+        v[1] += a.part[2 * b + 1];
+    }
+    const double sum = block_sum<2, kBr>(v);
+    __shared__ double sums[2];
+    if (threadIdx.x < 2) sums[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    uint32_t err = 0;
+    if (a.N + S > a.limit) err |= DEV_ERR_POPULATION;   // Swarm.cpp:1069-1075
+    if (M == 0) err |= DEV_ERR_EXTINCT;                 // Swarm.cpp:1095-1100
+    if (err) atomicOr(a.status, err);
+    const double w = a.state[0];
+    const double n = static_cast<double>(a.N);
+    const double w_new = (M == a.N || M == 0) ? w : w * static_cast<double>(M) / n;
+    a.state[0] = w_new;
+    if (a.step >= a.burn_in) {
+        a.state[1] += n * w * static_cast<double>(a.dt);
+        for (int c = 0; c < nch; ++c) {
+            const double k = static_cast<double>(a.step_counts[2 + c]);
+            a.state[2 + c] += w * k;
+            a.state[10 + c] += w * w * k;
+        }
+        for (int c = 0; c < 2 + nch; ++c) a.totals[c] += a.step_counts[c];
+        const int64_t k = a.step - a.burn_in;
+        a.samples[k] = sums[0] / n;
+        a.samples[a.sampling + k] = sums[1] / n;
+        a.samples[2 * a.sampling + k] = log(n * w_new);
+    }
+    for (int c = 0; c < 2 + nch; ++c) a.step_counts[c] = 0;
+}
+
+// block_slope_and_error (Swarm.cpp:136-197)
+void block_slope_error(const double* y, int64_t n, double dt, int64_t blocks, double& slope,
+                       double& se) {
+    const int64_t size = n / blocks;
+    std::vector<double> slopes(static_cast<size_t>(blocks));
+    for (int64_t b = 0; b < blocks; ++b) {
+        const double* x = y + b * size;
+        const double mid = 0.5 * static_cast<double>(size - 1);
+        double mean = 0.0;
+        for (int64_t i = 0; i < size; ++i) mean += x[i];
+        mean /= static_cast<double>(size);
+        double cov = 0.0, var = 0.0;
+        for (int64_t i = 0; i < size; ++i) {
+            const double t = static_cast<double>(i) - mid;
+            cov += t * (x[i] - mean);
+            var += t * t;
+        }
+        slopes[static_cast<size_t>(b)] = cov / (var * dt);
+    }
+    block_mean_error(slopes, slope, se);
+}
+
 }  // namespace
 
 MccModelHost& swarm_gas_model(Context& c) {
@@ -423,12 +667,7 @@ int aurora_hip_swarm_gas(aurora_hip_ctx* h, int nchan, const int* kind, const do
         AUR_REQUIRE(std::isfinite(neutral_mass) && neutral_mass > 0.0,
                     AURORA_ERR_INVALID_ARGUMENT, "gas dataset requires positive neutral_mass");
         bool elastic = false;
-        for (int k = 0; k < nchan; ++k) {
-            elastic = elastic || kind[k] == AURORA_ELASTIC;
-            AUR_REQUIRE(kind[k] != AURORA_ATTACHMENT, AURORA_ERR_UNSUPPORTED,
-                        "fixed-population swarm does not support attachment; use "
-                        "population_model = branching_resampled");
-        }
+        for (int k = 0; k < nchan; ++k) elastic = elastic || kind[k] == AURORA_ELASTIC;
         AUR_REQUIRE(elastic, AURORA_ERR_INVALID_ARGUMENT,
                     "electron swarm benchmark requires an elastic channel");
         // model checks and tables (the 1D MCC model loader's)
@@ -455,29 +694,11 @@ int aurora_hip_swarm_submit(aurora_hip_ctx* h, const aurora_hip_swarm_request* r
         SwarmState& s = state(c);
         AUR_REQUIRE(s.gas_loaded, AURORA_ERR_INVALID_ARGUMENT, "load the swarm gas first");
         AUR_REQUIRE(r, AURORA_ERR_INVALID_ARGUMENT, "null request");
-        // validate_config (Swarm.cpp:277-559)
-        AUR_REQUIRE(std::isfinite(r->timestep) && r->timestep > 0.0, AURORA_ERR_INVALID_ARGUMENT,
-                    "swarm timestep must be positive and finite");
-        AUR_REQUIRE(s.gas.max_frequency * r->timestep <= 0.1, AURORA_ERR_INVALID_ARGUMENT,
-                    "swarm max_frequency * timestep must not exceed 0.1");
-        AUR_REQUIRE(r->steps > 0 && r->burn_in < r->steps, AURORA_ERR_INVALID_ARGUMENT,
-                    "swarm steps must exceed burn_in_steps");
-        AUR_REQUIRE(r->steps < 2147483647, AURORA_ERR_INVALID_ARGUMENT, "swarm steps too large");
-        AUR_REQUIRE(r->particles > 0, AURORA_ERR_INVALID_ARGUMENT,
-                    "swarm particles must be positive");
-        AUR_REQUIRE(r->blocks > 0 && r->blocks <= r->steps - r->burn_in,
-                    AURORA_ERR_INVALID_ARGUMENT,
-                    "swarm block statistics require at least one sample per block");
-        AUR_REQUIRE(std::isfinite(r->reduced_field_td) && r->reduced_field_td >= 0.0,
-                    AURORA_ERR_INVALID_ARGUMENT,
-                    "swarm reduced fields must be finite and non-negative");
-        AUR_REQUIRE(std::isfinite(r->initial_mean_energy_ev) && r->initial_mean_energy_ev >= 0.0,
-                    AURORA_ERR_INVALID_ARGUMENT,
-                    "swarm initial_mean_energy_ev must be finite and non-negative");
-        AUR_REQUIRE(std::isfinite(r->max_energy_ev) && r->max_energy_ev > 0.0,
-                    AURORA_ERR_INVALID_ARGUMENT, "swarm max_energy_ev must be positive and finite");
-        AUR_REQUIRE(r->max_energy_ev <= s.table_max_ev, AURORA_ERR_INVALID_ARGUMENT,
-                    "swarm max_energy_ev exceeds channel table coverage");
+        validate_request(s, r);
+        for (int k = 0; k < s.gas.nchan; ++k)
+            AUR_REQUIRE(s.gas.kind[k] != AURORA_ATTACHMENT, AURORA_ERR_UNSUPPORTED,
+                        "fixed-population swarm does not support attachment; use "
+                        "population_model = branching_resampled");
         const uint64_t fp = footprint_of(*r);
         if (hbm_budget && s.bytes_in_use + fp > hbm_budget)
             throw Failure{AURORA_ERR_CAPACITY, "swarm request does not fit the HBM budget now"};
@@ -605,6 +826,220 @@ int aurora_hip_swarm_poll(aurora_hip_ctx* h, int64_t handle, int* done,
         }
         AUR_REQUIRE(s.active.count(handle), AURORA_ERR_NOT_FOUND, "unknown swarm request handle");
         if (done) *done = 0;
+    });
+}
+
+int aurora_hip_swarm_branching(aurora_hip_ctx* h, const aurora_hip_swarm_request* r,
+                               int64_t population_limit, aurora_hip_swarm_result* result,
+                               aurora_hip_swarm_growth* growth) {
+    return guarded(h, [&](Context& c) {
+        SwarmState& s = state(c);
+        AUR_REQUIRE(s.gas_loaded, AURORA_ERR_INVALID_ARGUMENT, "load the swarm gas first");
+        AUR_REQUIRE(r && result && growth, AURORA_ERR_INVALID_ARGUMENT, "null argument");
+        validate_request(s, r);
+        const int64_t N = r->particles, S = r->steps - r->burn_in;
+        // Swarm.cpp:327-335, 373-379, 553-557
+        AUR_REQUIRE(population_limit > N && population_limit <= 10000000 - N,
+                    AURORA_ERR_INVALID_ARGUMENT,
+                    "branching swarm population_limit must exceed particles and "
+                    "population_limit + particles must not exceed the allocation safety "
+                    "limit of 10000000");
+        AUR_REQUIRE(S % r->blocks == 0, AURORA_ERR_INVALID_ARGUMENT,
+                    "swarm sampling steps must be divisible by uncertainty_blocks");
+        AUR_REQUIRE(S / r->blocks >= 2, AURORA_ERR_INVALID_ARGUMENT,
+                    "branching swarm requires at least two sampling steps per uncertainty "
+                    "block");
+        uint32_t attach = 0;
+        bool creates = false;
+        for (int k = 0; k < s.gas.nchan; ++k) {
+            if (s.gas.kind[k] == AURORA_ATTACHMENT) attach |= 1u << k;
+            creates = creates || s.gas.kind[k] == AURORA_ATTACHMENT ||
+                      s.gas.kind[k] == AURORA_IONIZATION;
+        }
+        AUR_REQUIRE(creates, AURORA_ERR_INVALID_ARGUMENT,
+                    "branching swarm requires an ionization or attachment channel");
+        const int64_t cap = 3 * N;
+        const int nch = s.gas.nchan;
+        DevBuf<float> v, pre, sec;
+        DevBuf<long long> key, off;
+        DevBuf<unsigned long long> counts;     // step counters, then totals
+        DevBuf<unsigned> maxe;
+        DevBuf<uint32_t> status;
+        DevBuf<double> part, st, samples;
+        v.alloc(static_cast<size_t>(3 * N));
+        pre.alloc(static_cast<size_t>(3 * cap));
+        sec.alloc(static_cast<size_t>(6 * N));
+        key.alloc(static_cast<size_t>(N));
+        off.alloc(static_cast<size_t>(N));
+        counts.alloc(2 * (kMaxChannels + 2));
+        maxe.alloc(1);
+        status.alloc(1);
+        int64_t g = (N + kBr - 1) / kBr;
+        if (g > static_cast<int64_t>(c.cu_count) * 8) g = static_cast<int64_t>(c.cu_count) * 8;
+        part.alloc(static_cast<size_t>(2 * g));
+        st.alloc(18);
+        samples.alloc(static_cast<size_t>(3 * S));
+        AUR_HIP(hipMemsetAsync(counts.ptr, 0, counts.count * 8, c.stream));
+        AUR_HIP(hipMemsetAsync(maxe.ptr, 0, 4, c.stream));
+        AUR_HIP(hipMemsetAsync(status.ptr, 0, 4, c.stream));
+        double st0[18] = {1.0};
+        AUR_HIP(hipMemcpyAsync(st.ptr, st0, sizeof st0, hipMemcpyHostToDevice, c.stream));
+        const double sigma =
+            std::sqrt(2.0 * r->initial_mean_energy_ev * kElementaryCharge / (3.0 * kElectronMass));
+        k_swarm_init<<<static_cast<unsigned>(std::min<int64_t>(g, 4096)), 256, 0, c.stream>>>(
+            v.ptr, v.ptr + N, v.ptr + 2 * N, N, static_cast<float>(sigma), r->seed);
+        AUR_HIP(hipGetLastError());
+        size_t temp = 0;
+        AUR_HIP(rocprim::exclusive_scan(nullptr, temp, key.ptr, off.ptr, 0LL,
+                                        static_cast<size_t>(N), rocprim::plus<long long>(),
+                                        c.stream));
+        DevBuf<unsigned char> scratch;
+        scratch.alloc(temp > 0 ? temp : 1);
+        BranchArgs a{};
+        a.v = v.ptr;
+        a.pre = pre.ptr;
+        a.sec = sec.ptr;
+        a.key = key.ptr;
+        a.off = off.ptr;
+        a.step_counts = counts.ptr;
+        a.totals = counts.ptr + kMaxChannels + 2;
+        a.max_energy_bits = maxe.ptr;
+        a.status = status.ptr;
+        a.part = part.ptr;
+        a.state = st.ptr;
+        a.samples = samples.ptr;
+        a.N = N;
+        a.cap = cap;
+        a.limit = population_limit;
+        const double efield = r->reduced_field_td * kTownsend * s.gas.neutral_density;
+        a.half_kick = static_cast<float>(0.5 * (-kElementaryCharge * efield / kElectronMass) *
+                                         r->timestep);
+        a.dt = static_cast<float>(r->timestep);
+        a.max_energy_ev = static_cast<float>(r->max_energy_ev);
+        a.ev_per_v2 = static_cast<float>(0.5 * kElectronMass / kElementaryCharge);
+        a.seed = r->seed;
+        a.burn_in = static_cast<int>(r->burn_in);
+        a.sampling = static_cast<int>(S);
+        a.attach_mask = attach;
+        const MccParams m = swarm_params(s);
+        auto check = [&]() {
+            uint32_t err = 0;
+            AUR_HIP(hipMemcpyAsync(&err, status.ptr, 4, hipMemcpyDeviceToHost, c.stream));
+            AUR_HIP(hipStreamSynchronize(c.stream));
+            if (!err) return;
+            const std::string td = std::to_string(r->reduced_field_td);
+            if (err & DEV_ERR_POPULATION)
+                throw Failure{AURORA_ERR_POPULATION,
+                              "branching swarm exceeded population_limit before resampling; "
+                              "reduce timestep or raise the limit on an appropriate compute "
+                              "host"};
+            if (err & DEV_ERR_EXTINCT)
+                throw Failure{AURORA_ERR_POPULATION,
+                              "branching swarm electron ensemble became extinct; increase "
+                              "particles or reduce timestep"};
+            if (err & DEV_ERR_ENERGY)
+                throw Failure{AURORA_ERR_ENERGY_ABOVE_MAX,
+                              "swarm particle energy exceeded max_energy_ev at E/N=" + td +
+                                  " Td"};
+            if (err & DEV_ERR_RATE)
+                throw Failure{AURORA_ERR_RATE_ABOVE_BOUND,
+                              "MCC total collision frequency exceeds configured max_frequency"};
+            if (err & DEV_ERR_CANDIDATES)
+                throw Failure{AURORA_ERR_CANDIDATE_LIMIT,
+                              "MCC candidate limit exceeded; reduce dt or increase "
+                              "max_candidates_per_particle"};
+            if (err & DEV_ERR_PRODUCTS)
+                throw Failure{AURORA_ERR_UNSUPPORTED,
+                              "more than two ionizations of one electron in one step; reduce "
+                              "timestep"};
+            throw Failure{AURORA_ERR_INVALID_ARGUMENT, "MCC particle velocity must be finite"};
+        };
+        {
+            KernelScope ks(c, "swarm_branching");
+            for (int64_t step = 0; step < r->steps; ++step) {
+                a.step = static_cast<int>(step);
+                with_nchan(m.nchan, [&]<int NCH>() {
+                    k_br_collide<NCH><<<static_cast<unsigned>(g), kBr, 0, c.stream>>>(m, a);
+                });
+                AUR_HIP(rocprim::exclusive_scan(scratch.ptr, temp, key.ptr, off.ptr, 0LL,
+                                                static_cast<size_t>(N),
+                                                rocprim::plus<long long>(), c.stream));
+                k_br_gather<<<static_cast<unsigned>(g), kBr, 0, c.stream>>>(a);
+                k_br_resample<<<static_cast<unsigned>(g), kBr, 0, c.stream>>>(a);
+                k_br_fold<<<1, kBr, 0, c.stream>>>(a, static_cast<int>(g), nch);
+                AUR_HIP(hipGetLastError());
+                if ((step & 511) == 511) check();
+            }
+        }
+        check();
+        std::vector<double> smp(static_cast<size_t>(3 * S));
+        double sv[18];
+        unsigned long long tot[kMaxChannels + 2];
+        unsigned mx = 0;
+        AUR_HIP(hipMemcpy(smp.data(), samples.ptr, smp.size() * 8, hipMemcpyDeviceToHost));
+        AUR_HIP(hipMemcpy(sv, st.ptr, sizeof sv, hipMemcpyDeviceToHost));
+        AUR_HIP(hipMemcpy(tot, a.totals, sizeof tot, hipMemcpyDeviceToHost));
+        AUR_HIP(hipMemcpy(&mx, maxe.ptr, 4, hipMemcpyDeviceToHost));
+        // Swarm.cpp:1106-1245 on the samples and tallies
+        aurora_hip_swarm_result out{};
+        aurora_hip_swarm_growth gr{};
+        const int64_t B = r->blocks, len = S / B;
+        std::vector<double> vm(static_cast<size_t>(B)), em(static_cast<size_t>(B));
+        for (int64_t b = 0; b < B; ++b) {
+            double a0 = 0.0, a1 = 0.0;
+            for (int64_t i = b * len; i < (b + 1) * len; ++i) {
+                a0 += smp[static_cast<size_t>(i)];
+                a1 += smp[static_cast<size_t>(S + i)];
+            }
+            vm[static_cast<size_t>(b)] = a0 / static_cast<double>(len);
+            em[static_cast<size_t>(b)] = a1 / static_cast<double>(len);
+        }
+        double mv, sev, me, see;
+        block_mean_error(vm, mv, sev);
+        block_mean_error(em, me, see);
+        out.drift = -mv;
+        out.drift_se = sev;
+        out.mean_energy_ev = me;
+        out.mean_energy_se = see;
+        const double exposure = sv[1];
+        out.candidates = tot[0];
+        out.nulls = tot[1];
+        for (int k = 0; k < nch; ++k) {
+            out.collisions[k] = tot[2 + k];
+            out.rate[k] = sv[2 + k] / exposure;
+            out.rate_se[k] = std::sqrt(sv[10 + k]) / exposure;
+            if (s.gas.kind[k] == AURORA_IONIZATION) {
+                out.ionization_rate += out.rate[k];
+                out.ionization_rate_se = std::hypot(out.ionization_rate_se, out.rate_se[k]);
+            } else if (s.gas.kind[k] == AURORA_ATTACHMENT) {
+                gr.attachment_rate += out.rate[k];
+                gr.attachment_rate_se = std::hypot(gr.attachment_rate_se, out.rate_se[k]);
+            }
+        }
+        float mxf;
+        std::memcpy(&mxf, &mx, sizeof mxf);
+        out.max_energy_ev = mxf;
+        out.status = AURORA_OK;
+        gr.initial_weight = static_cast<double>(N);
+        gr.final_weight = static_cast<double>(N) * sv[0];
+        gr.final_particles = N;
+        gr.net_creation_rate = out.ionization_rate - gr.attachment_rate;
+        gr.net_creation_rate_se = std::hypot(out.ionization_rate_se, gr.attachment_rate_se);
+        block_slope_error(smp.data() + 2 * S, S, r->timestep, B, gr.growth_rate,
+                          gr.growth_rate_se);
+        if (out.drift > 0.0) {
+            const double d = out.drift, de = out.drift_se;
+            gr.townsend_available = 1;
+            gr.townsend_growth = gr.growth_rate / d;
+            gr.townsend_growth_se =
+                std::hypot(gr.growth_rate_se / d, gr.growth_rate * de / (d * d));
+            gr.townsend_rate_balance = gr.net_creation_rate / d;
+            gr.townsend_rate_balance_se =
+                std::hypot(gr.net_creation_rate_se / d, gr.net_creation_rate * de / (d * d));
+        }
+        *result = out;
+        *growth = gr;
+        flush_timers(c);
     });
 }
 

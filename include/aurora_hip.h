@@ -47,7 +47,8 @@ typedef enum aurora_status {
     AURORA_ERR_B_OUT_OF_RANGE = 6,   /* PrescribedField.cpp:95-100 */
     AURORA_ERR_ENERGY_ABOVE_MAX = 7, /* Swarm.cpp:1021-1044 */
     AURORA_ERR_UNSUPPORTED = 8,      /* configuration outside this build */
-    AURORA_ERR_NOT_FOUND = 9         /* unknown species / request handle */
+    AURORA_ERR_NOT_FOUND = 9,        /* unknown species / request handle */
+    AURORA_ERR_POPULATION = 10       /* branching swarm limit / extinction, Swarm.cpp:1069-1100 */
 } aurora_status;
 
 enum { AURORA_PERIODIC = 0, AURORA_DIRICHLET = 1 };            /* Types.hpp:31 */
@@ -353,6 +354,31 @@ int aurora_hip_swarm_poll(aurora_hip_ctx* ctx, int64_t handle, int* done,
                           aurora_hip_swarm_result* result);
 /* Device bytes a request of this size holds while admitted. */
 uint64_t aurora_hip_swarm_footprint(const aurora_hip_swarm_request* request);
+
+/* Branching / resampled population (population_model = branching_resampled:
+ * run_field, Swarm.cpp:940-1250 with 1063-1104, systematic_resample 616-643,
+ * block_slope_and_error 136-197).  Ionization appends the secondary at the
+ * parent's weight, attachment removes the electron, and every step
+ * resamples systematically back to `particles` electrons of equal weight,
+ * so the total weight W tracks the physical growth.  Runs one request to
+ * completion on the context's stream with the swarm gas (which may hold
+ * attachment channels), seeded like swarm_submit.  result carries the drift,
+ * energy and per-channel rates (weighted by the electron weight; no
+ * diffusion in this model); growth the rest. */
+typedef struct aurora_hip_swarm_growth {
+    double initial_weight;            /* initial_total_electron_weight */
+    double final_weight;              /* final_total_electron_weight */
+    int64_t final_particles;
+    double growth_rate, growth_rate_se;          /* 1/s, slope of log W */
+    double attachment_rate, attachment_rate_se;  /* 1/s per electron */
+    double net_creation_rate, net_creation_rate_se;
+    int townsend_available;                      /* drift > 0 */
+    double townsend_growth, townsend_growth_se;  /* growth / drift, 1/m */
+    double townsend_rate_balance, townsend_rate_balance_se;  /* net / drift */
+} aurora_hip_swarm_growth;
+int aurora_hip_swarm_branching(aurora_hip_ctx* ctx, const aurora_hip_swarm_request* request,
+                               int64_t population_limit, aurora_hip_swarm_result* result,
+                               aurora_hip_swarm_growth* growth);
 
 /* ---------------------------------------------------------- ensembles */
 /* Batched 1D ensembles: many independent instances of one 1D discharge

@@ -410,3 +410,179 @@ void or_swarm_run(const or_mcc_model* m, const or_swarm_cfg* c,
 done:
     free(v); free(x); free(vs); free(es);
 }
+
+int or_block_slope_error(int64_t n, const double* samples, double dt, int64_t blocks,
+                         double* slope, double* se) {
+    /* Swarm.cpp:136-197: least-squares slope against the sample index in
+     * each block, divided by the spacing; mean and standard error of the
+     * block slopes */
+    if (blocks < 1 || n % blocks != 0 || n / blocks < 2) return -1;
+    const int64_t size = n / blocks;
+    double* slopes = (double*)malloc((size_t)blocks * sizeof(double));
+    for (int64_t b = 0; b < blocks; ++b) {
+        const double* y = samples + b * size;
+        const double mid = 0.5 * (double)(size - 1);
+        double mean = 0.0;
+        for (int64_t i = 0; i < size; ++i) mean += y[i];
+        mean /= (double)size;
+        double cov = 0.0, var = 0.0;
+        for (int64_t i = 0; i < size; ++i) {
+            const double t = (double)i - mid;
+            cov += t * (y[i] - mean);
+            var += t * t;
+        }
+        slopes[b] = cov / (var * dt);
+    }
+    double m = 0.0;
+    for (int64_t b = 0; b < blocks; ++b) m += slopes[b];
+    m /= (double)blocks;
+    double sq = 0.0;
+    for (int64_t b = 0; b < blocks; ++b) sq += (slopes[b] - m) * (slopes[b] - m);
+    *slope = m;
+    *se = blocks == 1 ? 0.0 : sqrt(sq / ((double)blocks * (double)(blocks - 1)));
+    free(slopes);
+    return 0;
+}
+
+int64_t or_systematic_source(int64_t j, int64_t M, int64_t N, double u) {
+    int64_t s = (int64_t)ceil((u + (double)j) * (double)M / (double)N) - 1;
+    if (s < 0) s = 0;
+    if (s > M - 1) s = M - 1;
+    return s;
+}
+
+void or_swarm_branching_run(const or_mcc_model* m, const or_swarm_cfg* c,
+                            int64_t population_limit, const double* vel0, uint64_t seed,
+                            uint32_t instance, or_swarm_result* r, or_swarm_growth* g) {
+    /* Swarm.cpp:940-1250 with population_model = branching_resampled.  Every
+     * electron of a step carries the same weight w (secondaries inherit the
+     * parent's, resampling gives all outputs W / N), so the population is a
+     * velocity list plus one weight. */
+    memset(r, 0, sizeof *r);
+    memset(g, 0, sizeof *g);
+    const int64_t np = c->particles;
+    const int64_t cap = 3 * np;
+    double* v = (double*)malloc((size_t)(3 * np) * sizeof(double));
+    double* pre = (double*)malloc((size_t)(3 * cap) * sizeof(double));
+    double* sec = (double*)malloc((size_t)(3 * cap) * sizeof(double));
+    unsigned char* alive = (unsigned char*)malloc((size_t)np);
+    memcpy(v, vel0, (size_t)(3 * np) * sizeof(double));
+    const double efield = c->reduced_field_td * TOWNSEND * c->neutral_density;
+    const double accel = -ELEMENTARY_CHARGE * efield / ELECTRON_MASS;
+    const double half_kick = 0.5 * accel * c->timestep;
+    const int64_t sampling = c->steps - c->burn_in;
+    const int64_t ns = sampling > 0 ? sampling : 1;
+    double* vs = (double*)malloc((size_t)ns * sizeof(double));
+    double* es = (double*)malloc((size_t)ns * sizeof(double));
+    double* lw = (double*)malloc((size_t)ns * sizeof(double));
+    double weighted[8] = {0}, squared[8] = {0}, exposure = 0.0;
+    double w = 1.0;
+    g->initial_weight = (double)np;
+    for (int64_t step = 0; step < c->steps; ++step) {
+        if (step >= c->burn_in) exposure += (double)np * w * c->timestep;
+        int64_t nsec = 0;
+        for (int64_t p = 0; p < np; ++p) {
+            double* u = v + 3 * p;
+            u[0] += half_kick;
+            double e_pre = kinetic_energy_ev(u);
+            if (e_pre > r->max_energy_ev) r->max_energy_ev = e_pre;
+            if (e_pre > c->max_energy_ev) { r->status = 4; goto done; }
+            or_stream rng;
+            or_stream_init(&rng, seed, instance, (uint32_t)p, (uint64_t)step);
+            or_mcc_stats st;
+            or_mcc_collide_3v(m, u, c->timestep, &rng, &st);
+            if (st.status) { r->status = st.status; goto done; }
+            u[0] += half_kick;
+            double e = kinetic_energy_ev(u);
+            if (e > r->max_energy_ev) r->max_energy_ev = e;
+            if (e > c->max_energy_ev) { r->status = 4; goto done; }
+            alive[p] = 1;
+            for (int ch = 0; ch < m->nchan; ++ch) {
+                if (step >= c->burn_in) {
+                    r->collisions[ch] += st.chan[ch];
+                    weighted[ch] += w * (double)st.chan[ch];
+                    squared[ch] += w * w * (double)st.chan[ch];
+                }
+                if (m->kind[ch] == OR_ATTACHMENT && st.chan[ch]) alive[p] = 0;
+            }
+            if (step >= c->burn_in) {
+                r->candidates += st.candidates;
+                r->nulls += st.nulls;
+            }
+            /* Swarm.cpp:1063-1087: a secondary is appended only while the
+             * list (parents included) is below the limit */
+            for (int k = 0; k < st.nsec; ++k) {
+                if (np + nsec >= population_limit) { r->status = 5; goto done; }
+                for (int a = 0; a < 3; ++a) sec[3 * nsec + a] = st.sec_v[k][a];
+                ++nsec;
+            }
+        }
+        /* surviving parents in order, then the secondaries */
+        int64_t M = 0;
+        for (int64_t p = 0; p < np; ++p)
+            if (alive[p]) { memcpy(pre + 3 * M, v + 3 * p, 3 * sizeof(double)); ++M; }
+        memcpy(pre + 3 * M, sec, (size_t)(3 * nsec) * sizeof(double));
+        M += nsec;
+        if (M == 0) { r->status = 6; goto done; }
+        if (M == np) {
+            memcpy(v, pre, (size_t)(3 * np) * sizeof(double));
+        } else {
+            or_stream rr;
+            or_stream_init(&rr, seed, instance + 1u, 0xFFFFFFFFu, (uint64_t)step);
+            const double u = or_stream_u01(&rr);
+            for (int64_t j = 0; j < np; ++j)
+                memcpy(v + 3 * j, pre + 3 * or_systematic_source(j, M, np, u),
+                       3 * sizeof(double));
+            w = w * (double)M / (double)np;
+        }
+        if (step < c->burn_in) continue;
+        double vsum = 0.0, esum = 0.0;
+        for (int64_t p = 0; p < np; ++p) {
+            vsum += v[3 * p];
+            esum += kinetic_energy_ev(v + 3 * p);
+        }
+        vs[step - c->burn_in] = vsum / (double)np;
+        es[step - c->burn_in] = esum / (double)np;
+        lw[step - c->burn_in] = log((double)np * w);
+    }
+    {
+        double mv, sev, me, see;
+        or_block_mean_error(sampling, vs, c->blocks, &mv, &sev);
+        or_block_mean_error(sampling, es, c->blocks, &me, &see);
+        r->drift = -mv;
+        r->drift_se = sev;
+        r->mean_energy_ev = me;
+        r->mean_energy_se = see;
+        for (int ch = 0; ch < m->nchan; ++ch) {
+            r->rate[ch] = weighted[ch] / exposure;
+            r->rate_se[ch] = sqrt(squared[ch]) / exposure;
+            if (m->kind[ch] == OR_IONIZATION) {
+                r->ion_rate += r->rate[ch];
+                r->ion_rate_se = hypot(r->ion_rate_se, r->rate_se[ch]);
+            } else if (m->kind[ch] == OR_ATTACHMENT) {
+                g->attachment_rate += r->rate[ch];
+                g->attachment_rate_se = hypot(g->attachment_rate_se, r->rate_se[ch]);
+            }
+        }
+        g->net_creation_rate = r->ion_rate - g->attachment_rate;
+        g->net_creation_rate_se = hypot(r->ion_rate_se, g->attachment_rate_se);
+        g->final_weight = (double)np * w;
+        g->final_particles = np;
+        if (or_block_slope_error(sampling, lw, c->timestep, c->blocks, &g->growth_rate,
+                                 &g->growth_rate_se) != 0) {
+            r->status = 7;
+            goto done;
+        }
+        if (r->drift > 0.0) {
+            const double d = r->drift, de = r->drift_se;
+            g->townsend_available = 1;
+            g->townsend_growth = g->growth_rate / d;
+            g->townsend_growth_se = hypot(g->growth_rate_se / d, g->growth_rate * de / (d * d));
+            g->townsend_rate_balance = g->net_creation_rate / d;
+            g->townsend_rate_balance_se =
+                hypot(g->net_creation_rate_se / d, g->net_creation_rate * de / (d * d));
+        }
+    }
+done:
+    free(v); free(pre); free(sec); free(alive); free(vs); free(es); free(lw);
+}

@@ -194,6 +194,34 @@ void or_swarm_run(const or_mcc_model* m, const or_swarm_cfg* c,
 void or_block_mean_error(int64_t n, const double* samples, int64_t blocks,
                          double* mean, double* se);
 
+/* ---- swarm run_field, branching / resampled population (Swarm.cpp:1063-1104,
+ * 1214-1245; systematic_resample 616-643; block_slope_and_error 136-197). */
+typedef struct {
+    double initial_weight, final_weight;
+    int64_t final_particles;
+    double growth_rate, growth_rate_se;             /* 1/s */
+    double attachment_rate, attachment_rate_se;
+    double net_creation_rate, net_creation_rate_se;
+    int townsend_available;
+    double townsend_growth, townsend_growth_se;     /* 1/m */
+    double townsend_rate_balance, townsend_rate_balance_se;
+} or_swarm_growth;
+/* Slope of samples against time (spacing dt) per equal block, mean and
+ * standard error over the blocks.  Returns 0, or -1 when a block would hold
+ * fewer than two samples or n is not a multiple of blocks. */
+int or_block_slope_error(int64_t n, const double* samples, double dt, int64_t blocks,
+                         double* slope, double* se);
+/* Systematic resampling of M equal-weight electrons to N: output j takes
+ * source ceil((u + j) M / N) - 1 (u in [0, 1) the run's draw for the step),
+ * the index where the cumulative weight first reaches the j-th threshold. */
+int64_t or_systematic_source(int64_t j, int64_t M, int64_t N, double u);
+/* status: 0 ok, 4 energy above max, 5 population limit, 6 extinct, else MCC.
+ * Streams: collisions of slot p at step s (seed, instance, p, s); the
+ * resampling draw of step s (seed, instance + 1, 0xFFFFFFFF, s). */
+void or_swarm_branching_run(const or_mcc_model* m, const or_swarm_cfg* c,
+                            int64_t population_limit, const double* vel0, uint64_t seed,
+                            uint32_t instance, or_swarm_result* r, or_swarm_growth* g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,6 +111,25 @@ class _SwarmCfg(C.Structure):
     ]
 
 
+class _SwarmGrowth(C.Structure):
+    _fields_ = [
+        ("initial_weight", C.c_double),
+        ("final_weight", C.c_double),
+        ("final_particles", C.c_int64),
+        ("growth_rate", C.c_double),
+        ("growth_rate_se", C.c_double),
+        ("attachment_rate", C.c_double),
+        ("attachment_rate_se", C.c_double),
+        ("net_creation_rate", C.c_double),
+        ("net_creation_rate_se", C.c_double),
+        ("townsend_available", C.c_int),
+        ("townsend_growth", C.c_double),
+        ("townsend_growth_se", C.c_double),
+        ("townsend_rate_balance", C.c_double),
+        ("townsend_rate_balance_se", C.c_double),
+    ]
+
+
 class _SwarmResult(C.Structure):
     _fields_ = [
         ("drift", C.c_double),
@@ -228,6 +247,12 @@ class Oracle:
         L.or_swarm_run.argtypes = [C.POINTER(_MccModel), C.POINTER(_SwarmCfg), _dp,
                                    C.c_uint64, C.c_uint32, C.POINTER(_SwarmResult)]
         L.or_block_mean_error.argtypes = [C.c_int64, _dp, C.c_int64, _dp, _dp]
+        L.or_block_slope_error.argtypes = [C.c_int64, _dp, C.c_double, C.c_int64, _dp, _dp]
+        L.or_systematic_source.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_double]
+        L.or_systematic_source.restype = C.c_int64
+        L.or_swarm_branching_run.argtypes = [C.POINTER(_MccModel), C.POINTER(_SwarmCfg),
+                                             C.c_int64, _dp, C.c_uint64, C.c_uint32,
+                                             C.POINTER(_SwarmResult), C.POINTER(_SwarmGrowth)]
 
     # ---- geometry
     @staticmethod
@@ -408,6 +433,28 @@ class Oracle:
                               instance, C.byref(r))
         return r
 
+    def swarm_branching(self, model: MccModel, cfg: dict, vel0, seed, population_limit,
+                        instance=0):
+        c = _SwarmCfg(**cfg)
+        vel0 = f64(vel0).reshape(-1)
+        r, g = _SwarmResult(), _SwarmGrowth()
+        self.lib.or_swarm_branching_run(C.byref(model.struct), C.byref(c), population_limit,
+                                        _ptr(vel0, _dp), seed, instance, C.byref(r),
+                                        C.byref(g))
+        return r, g
+
+    def block_slope_error(self, samples, dt, blocks):
+        samples = f64(samples)
+        m, se = C.c_double(), C.c_double()
+        rc = self.lib.or_block_slope_error(samples.size, _ptr(samples, _dp), dt, blocks,
+                                           C.byref(m), C.byref(se))
+        if rc:
+            raise ValueError("growth statistics need two samples per equal block")
+        return m.value, se.value
+
+    def systematic_source(self, j, M, N, u):
+        return self.lib.or_systematic_source(j, M, N, u)
+
     def block_mean_error(self, samples, blocks):
         samples = f64(samples)
         m = C.c_double()
@@ -551,7 +598,12 @@ class Reference:
                        "mean_energy_ev", "mean_energy_se", "ionization_rate",
                        "ionization_rate_se", "max_energy_ev", "candidates", "nulls",
                        "neutral_vstd", "collisions0", "collisions1", "collisions2",
-                       "collisions3", "longitudinal_diffusion", "transverse_diffusion")
+                       "collisions3", "longitudinal_diffusion", "transverse_diffusion",
+                       "final_weight", "final_particles", "growth_rate", "growth_rate_se",
+                       "attachment_rate", "attachment_rate_se", "net_creation_rate",
+                       "net_creation_rate_se", "townsend_available", "townsend_growth",
+                       "townsend_growth_se", "townsend_rate_balance",
+                       "townsend_rate_balance_se")
 
     def swarm_file(self, path, max_fields=64, csv_path=""):
         """A .swarm file through the reference's own loader and driver; one

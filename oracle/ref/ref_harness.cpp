@@ -365,7 +365,7 @@ int ref_swarm(int nchan, const int* kind, const double* threshold,
 // of fields written.
 int ref_swarm_file(const char* swarm_path, int max_fields, double* out, const char* csv_path,
                    char* err, int errlen) {
-    constexpr int kCols = 18;
+    constexpr int kCols = 31;
     try {
         auto cfg = pic::load_swarm_benchmark_config(swarm_path);
         cfg.output_file = temp_dir() / "swarm_file.csv";
@@ -394,6 +394,19 @@ int ref_swarm_file(const char* swarm_path, int max_fields, double* out, const ch
                 o[12 + c] = static_cast<double>(r.channels[c].collisions);
             o[16] = r.longitudinal_diffusion_m2_s;
             o[17] = r.transverse_diffusion_m2_s;
+            o[18] = r.final_total_electron_weight;
+            o[19] = static_cast<double>(r.final_computational_particles);
+            o[20] = r.temporal_growth_rate_s;
+            o[21] = r.temporal_growth_rate_standard_error_s;
+            o[22] = r.attachment_rate_s;
+            o[23] = r.attachment_rate_standard_error_s;
+            o[24] = r.net_creation_rate_s;
+            o[25] = r.net_creation_rate_standard_error_s;
+            o[26] = r.townsend_available ? 1.0 : 0.0;
+            o[27] = r.growth_over_flux_drift_townsend_1_m;
+            o[28] = r.growth_over_flux_drift_townsend_standard_error_1_m;
+            o[29] = r.rate_balance_effective_townsend_1_m;
+            o[30] = r.rate_balance_effective_townsend_standard_error_1_m;
         }
         return n;
     } catch (const std::exception& e) {
