@@ -725,7 +725,7 @@ Simulation2D::Simulation2D(Simulation2DConfig cfg, std::shared_ptr<Device> devic
         int sid = 0;
         d.check(aurora_hip_species_add(d.get(), s.charge, s.mass, s.weight,
                                        static_cast<int64_t>(std::max(s.capacity, s.particles)),
-                                       32, &sid));
+                                       s.precision, &sid));
     }
     d.check(aurora_hip_set_sort_interval(d.get(), static_cast<int64_t>(cfg_.sort_interval)));
 }
@@ -769,12 +769,33 @@ void Simulation2D::initialize() {
         initialized_ = true;
         return;
     }
+    // Species2D::initialize (Species2D.cpp:29-75): one generator for all
+    // species in config order; per particle x, y, then v_x, v_y, v_z
+    std::mt19937_64 rng(cfg_.seed);
+    const double L[2] = {cfg_.length_x, cfg_.length_y};
     for (size_t k = 0; k < cfg_.species.size(); ++k) {
         const auto& s = cfg_.species[k];
-        const double drift[3] = {s.drift_velocity.x, s.drift_velocity.y, s.drift_velocity.z};
-        d.check(aurora_hip_species_init_maxwellian(d.get(), static_cast<int>(k),
-                                                   static_cast<int64_t>(s.particles),
-                                                   s.thermal_velocity, drift, cfg_.seed + k));
+        const double hx = s.init_x_max < 0.0 ? L[0] : s.init_x_max;
+        const double hy = s.init_y_max < 0.0 ? L[1] : s.init_y_max;
+        std::uniform_real_distribution<double> ux(s.init_x_min, hx), uy(s.init_y_min, hy);
+        std::normal_distribution<double> vx(s.drift_velocity.x, s.thermal_velocity),
+            vy(s.drift_velocity.y, s.thermal_velocity), vz(s.drift_velocity.z, s.thermal_velocity);
+        const int64_t n = static_cast<int64_t>(s.particles);
+        std::vector<double> pos(static_cast<size_t>(2 * n)), vel(static_cast<size_t>(3 * n));
+        for (int64_t p = 0; p < n; ++p) {
+            double x[2] = {ux(rng), uy(rng)};
+            for (int a = 0; a < 2; ++a)
+                x[a] = cfg_.boundary == Boundary::Periodic
+                           ? std::fmod(std::fmod(x[a], L[a]) + L[a], L[a])
+                           : std::clamp(x[a], 0.0, L[a]);
+            pos[2 * p] = x[0];
+            pos[2 * p + 1] = x[1];
+            vel[3 * p] = vx(rng);
+            vel[3 * p + 1] = vy(rng);
+            vel[3 * p + 2] = vz(rng);
+        }
+        d.check(aurora_hip_species_upload(d.get(), static_cast<int>(k), n, pos.data(), vel.data(),
+                                          nullptr, AURORA_VELOCITY_CENTRED));
     }
     d.check(aurora_hip_sort(d.get()));
     d.check(aurora_hip_initialize(d.get(), cfg_.dt));
@@ -882,7 +903,7 @@ Simulation3D::Simulation3D(Simulation3DConfig cfg, std::shared_ptr<Device> devic
         int sid = 0;
         d.check(aurora_hip_species_add(d.get(), s.charge, s.mass, s.weight,
                                        static_cast<int64_t>(std::max(s.capacity, s.particles)),
-                                       32, &sid));
+                                       s.precision, &sid));
     }
     loaded_.assign(cfg_.species.size(), false);
     d.check(aurora_hip_set_sort_interval(d.get(), static_cast<int64_t>(cfg_.sort_interval)));
@@ -938,13 +959,36 @@ void Simulation3D::initialize() {
             set_particles(k, ps);
         }
     }
+    // Species3D::initialize (Species3D.cpp:28-76): one generator for all
+    // species in config order; per particle x, y, z, then v_x, v_y, v_z,
+    // each from its own distribution object
+    std::mt19937_64 rng(cfg_.seed);
     for (size_t k = 0; k < cfg_.species.size(); ++k) {
         if (loaded_[k]) continue;
         const auto& s = cfg_.species[k];
-        const double drift[3] = {s.drift_velocity.x, s.drift_velocity.y, s.drift_velocity.z};
-        d.check(aurora_hip_species_init_maxwellian(d.get(), static_cast<int>(k),
-                                                   static_cast<int64_t>(s.particles),
-                                                   s.thermal_velocity, drift, cfg_.seed + k));
+        const double L[3] = {cfg_.length_x, cfg_.length_y, cfg_.length_z};
+        const double lo[3] = {s.init_x_min, s.init_y_min, s.init_z_min};
+        const double hi[3] = {s.init_x_max < 0.0 ? L[0] : s.init_x_max,
+                              s.init_y_max < 0.0 ? L[1] : s.init_y_max,
+                              s.init_z_max < 0.0 ? L[2] : s.init_z_max};
+        std::uniform_real_distribution<double> ux(lo[0], hi[0]), uy(lo[1], hi[1]),
+            uz(lo[2], hi[2]);
+        std::normal_distribution<double> vx(s.drift_velocity.x, s.thermal_velocity),
+            vy(s.drift_velocity.y, s.thermal_velocity), vz(s.drift_velocity.z, s.thermal_velocity);
+        std::vector<Particle3D> ps(s.particles);
+        for (auto& q : ps) {
+            double x[3] = {ux(rng), uy(rng), uz(rng)};
+            for (int a = 0; a < 3; ++a)
+                x[a] = cfg_.boundary == Boundary::Periodic
+                           ? std::fmod(std::fmod(x[a], L[a]) + L[a], L[a])
+                           : std::clamp(x[a], 0.0, L[a]);
+            q.position = Vec3{x[0], x[1], x[2]};
+            q.velocity.x = vx(rng);
+            q.velocity.y = vy(rng);
+            q.velocity.z = vz(rng);
+            q.alive = true;
+        }
+        set_particles(k, ps);
     }
     if (cfg_.boundary == Boundary::Periodic) d.check(aurora_hip_sort(d.get()));
     d.check(aurora_hip_initialize(d.get(), cfg_.dt));

@@ -16,10 +16,9 @@
 //   - NullCollisionModel::collide works on a batch of velocities with one
 //     counter-based (Philox4x32-10) stream per particle and step instead of
 //     one std::mt19937_64 passed by reference;
-//   - 2D/3D species initialisation draws from Philox as well, so those
-//     initial states equal the reference's in distribution, not sample by
-//     sample; the 1D loading (uniform positions, Maxwellian velocities)
-//     draws from std::mt19937_64 in the reference's order and so gives the
+//   - species loading (uniform positions, Maxwellian velocities) draws
+//     from one std::mt19937_64 in the reference's order (Species.cpp:40-78,
+//     Species2D.cpp:29-75, Species3D.cpp:28-76) and so starts from the
 //     reference's particles.
 #pragma once
 
@@ -323,6 +322,10 @@ struct Species3DConfig {          // Species3D.hpp:9-26 (uniform Maxwellian load
     Vec3 drift_velocity{};
     double thermal_velocity{0.1};
     std::size_t capacity{0};      // slots (0 = particles)
+    int precision{32};            // 64 keeps FP64 offsets and velocities (untiled push)
+    double init_x_min{0.0}, init_x_max{-1.0};   // negative maxima mean the full axis
+    double init_y_min{0.0}, init_y_max{-1.0};
+    double init_z_min{0.0}, init_z_max{-1.0};
 };
 
 struct Simulation3DConfig {       // Simulation3D.hpp:26-66, the fields the loop uses

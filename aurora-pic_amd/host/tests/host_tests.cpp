@@ -9,9 +9,12 @@
 //   aurora_host_tests --aps-signature <file>   print a state file's signature
 //   aurora_host_tests --run1d <case.txt> <output dir>   run a 1D case file
 //                    through Simulation::run(), writing the reference's outputs
+//   aurora_host_tests --run3d <case.txt> <output dir> <32|64>   the same for a
+//                    periodic 3D case through Simulation3D::run()
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <filesystem>
 #include <fstream>
 #include <functional>
@@ -787,6 +790,53 @@ void run1d_case(const std::filesystem::path& case_file, const std::filesystem::p
     sim.run();
 }
 
+// A periodic 3D case file (tests/golden/make_run3d_golden.py): "n", "length",
+// "dt", "steps", "output_interval", "seed", "btable <axis> <file>" (rows x bx
+// by bz) and "species name charge mass weight particles thermal" lines.
+void run3d_case(const std::filesystem::path& case_file, const std::filesystem::path& out,
+                int precision) {
+    std::ifstream in(case_file);
+    if (!in) throw std::runtime_error("cannot open " + case_file.string());
+    Simulation3DConfig cfg;
+    cfg.permittivity = 1.0;
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::string key;
+        if (!(ls >> key)) continue;
+        if (key == "n") ls >> cfg.nx >> cfg.ny >> cfg.nz;
+        else if (key == "length") ls >> cfg.length_x >> cfg.length_y >> cfg.length_z;
+        else if (key == "dt") ls >> cfg.dt;
+        else if (key == "steps") ls >> cfg.steps;
+        else if (key == "output_interval") ls >> cfg.output_interval;
+        else if (key == "seed") ls >> cfg.seed;
+        else if (key == "btable") {
+            TabulatedVectorField1D t;
+            std::string f;
+            ls >> t.axis >> f;
+            std::ifstream rows(case_file.parent_path() / f);
+            double x, bx, by, bz;
+            while (rows >> x >> bx >> by >> bz) {
+                t.coordinates.push_back(x);
+                t.values.push_back(Vec3{bx, by, bz});
+            }
+            cfg.magnetic_field_profile = t;
+        } else if (key == "species") {
+            Species3DConfig sc;
+            ls >> sc.name >> sc.charge >> sc.mass >> sc.weight >> sc.particles >>
+                sc.thermal_velocity;
+            sc.precision = precision;
+            cfg.species.push_back(sc);
+        } else {
+            throw std::runtime_error("unknown case key " + key);
+        }
+        if (ls.fail()) throw std::runtime_error("malformed case line: " + line);
+    }
+    cfg.output_dir = out;
+    Simulation3D sim(cfg, std::make_shared<Device>(0));
+    sim.run();
+}
+
 int main(int argc, char** argv) {
     const std::string arg = argc > 1 ? argv[1] : "";
     if (arg == "--list") {
@@ -806,6 +856,15 @@ int main(int argc, char** argv) {
     if (arg == "--run1d" && argc > 3) {
         try {
             run1d_case(argv[2], argv[3]);
+        } catch (const std::exception& e) {
+            std::printf("error: %s\n", e.what());
+            return 1;
+        }
+        return 0;
+    }
+    if (arg == "--run3d" && argc > 4) {
+        try {
+            run3d_case(argv[2], argv[3], std::atoi(argv[4]));
         } catch (const std::exception& e) {
             std::printf("error: %s\n", e.what());
             return 1;

@@ -495,6 +495,9 @@ class Reference:
         L.ref_sim3d_time.argtypes = [_i64p, _dp, C.c_double, C.c_double, C.c_int, C.c_int,
                                      _dp, C.c_int64, _dp, _dp, C.c_int, _dp, _i64p,
                                      C.c_char_p, C.c_int]
+        L.ref_run3d.argtypes = [C.c_char_p, _i64p, _dp, C.c_double, C.c_int64, C.c_int64,
+                                C.c_uint64, C.c_int, C.c_int64, _dp, _dp, C.c_int,
+                                C.POINTER(C.c_char_p), _dp, C.c_char_p, C.c_int]
         L.ref_run1d.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, C.c_double, C.c_double,
                                 C.c_int64, C.c_int64, C.c_int, _dp, C.c_int, C.c_uint64, C.c_int,
                                 C.POINTER(C.c_char_p), _dp, C.c_char_p, C.c_int]
@@ -663,6 +666,23 @@ class Reference:
                                 case["output_interval"], int(case["dirichlet"]), _ptr(el, _dp),
                                 case["velocity_dimensions"], case["seed"], len(sp), names,
                                 _ptr(rows, _dp), err, 512)
+        if rc:
+            raise RuntimeError(err.value.decode())
+
+
+    def run3d(self, case: dict, output_dir: str):
+        """The reference's Simulation3D::run() on a tests/golden/run3d case."""
+        sp = case["species"]
+        names = (C.c_char_p * len(sp))(*[s["name"].encode() for s in sp])
+        rows = f64([[s["charge"], s["mass"], s["weight"], s["particles"],
+                     s["thermal_velocity"]] for s in sp]).reshape(-1)
+        n, L = i64(case["n"]), f64(case["length"])
+        bx, bv = f64(case["bcoords"]), f64(case["bvalues"]).reshape(-1)
+        err = C.create_string_buffer(512)
+        rc = self.lib.ref_run3d(output_dir.encode(), _ptr(n, _i64p), _ptr(L, _dp), case["dt"],
+                                case["steps"], case["output_interval"], case["seed"],
+                                case["b_axis"], bx.size, _ptr(bx, _dp), _ptr(bv, _dp), len(sp),
+                                names, _ptr(rows, _dp), err, 512)
         if rc:
             raise RuntimeError(err.value.decode())
 

@@ -535,4 +535,55 @@ int ref_run1d(const char* output_dir, const char* aps_path, int64_t nx, double l
     }
 }
 
+// The reference's own Simulation3D::run() on a periodic mesh (species
+// loaded the reference's way from seed; rows of (charge, mass, weight,
+// particles, thermal_velocity)), with an optional tabulated B along
+// b_axis, writing scalars.csv into output_dir.
+int ref_run3d(const char* output_dir, const int64_t* n, const double* L, double dt,
+              int64_t steps, int64_t output_interval, uint64_t seed, int b_axis, int64_t brows,
+              const double* bcoords, const double* bvalues, int nspecies,
+              const char* const* names, const double* species, char* err, int errlen) {
+    try {
+        pic::Simulation3DConfig cfg;
+        cfg.nx = static_cast<std::size_t>(n[0]);
+        cfg.ny = static_cast<std::size_t>(n[1]);
+        cfg.nz = static_cast<std::size_t>(n[2]);
+        cfg.length_x = L[0];
+        cfg.length_y = L[1];
+        cfg.length_z = L[2];
+        cfg.dt = dt;
+        cfg.steps = static_cast<std::size_t>(steps);
+        cfg.output_interval = static_cast<std::size_t>(output_interval);
+        cfg.output_dir = output_dir;
+        cfg.seed = static_cast<unsigned>(seed);
+        cfg.boundary = pic::Boundary::Periodic;
+        if (brows > 0) {
+            std::vector<double> c(bcoords, bcoords + brows);
+            std::vector<pic::Vec3> v(static_cast<std::size_t>(brows));
+            for (int64_t i = 0; i < brows; ++i)
+                v[i] = {bvalues[3 * i], bvalues[3 * i + 1], bvalues[3 * i + 2]};
+            const auto axis = b_axis == 0 ? pic::CoordinateAxis::X
+                              : b_axis == 1 ? pic::CoordinateAxis::Y : pic::CoordinateAxis::Z;
+            cfg.magnetic_field_profile = pic::TabulatedVectorField1D(axis, c, v);
+        }
+        for (int k = 0; k < nspecies; ++k) {
+            const double* r = species + 5 * k;
+            pic::Species3DConfig sc;
+            sc.name = names[k];
+            sc.charge = r[0];
+            sc.mass = r[1];
+            sc.weight = r[2];
+            sc.particles = static_cast<std::size_t>(r[3]);
+            sc.thermal_velocity = r[4];
+            cfg.species.push_back(sc);
+        }
+        pic::Simulation3D sim(cfg);
+        sim.run();
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(err, errlen, e.what());
+        return -1;
+    }
+}
+
 }  // extern "C"
