@@ -1326,7 +1326,9 @@ int aurora_hip_energy(aurora_hip_ctx* h, double dt, double* kinetic, double* fie
             ensure_rho(c);
             ensure_fields(c, false, true);
             AUR_HIP(hipStreamSynchronize(c.stream));
-            // FP64 node sums in node order (Simulation3D.cpp:564-576)
+            // FP64 node sums in node order (Simulation3D.cpp:564-576); a 2D
+            // node's volume is its area times the out-of-plane depth
+            // (Diagnostics.cpp:127-135)
             std::vector<double> rho(c.nnodes), E(c.nnodes * c.dim);
             AUR_HIP(hipMemcpy(rho.data(), c.rho.ptr, c.nnodes * sizeof(double),
                               hipMemcpyDeviceToHost));
@@ -1336,7 +1338,7 @@ int aurora_hip_energy(aurora_hip_ctx* h, double dt, double* kinetic, double* fie
             for (int64_t i = 0; i < c.nnodes; ++i) {
                 double e2 = 0.0;
                 for (int a = 0; a < c.dim; ++a) e2 += E[a * c.nnodes + i] * E[a * c.nnodes + i];
-                double vol = 1.0;
+                double vol = c.dim == 2 ? c.depth : 1.0;
                 int64_t rem = i;
                 for (int a = 0; a < c.dim; ++a) {
                     const int64_t k = rem % c.n[a];

@@ -11,6 +11,7 @@
 //                    through Simulation::run(), writing the reference's outputs
 //   aurora_host_tests --run3d <case.txt> <output dir> <32|64>   the same for a
 //                    periodic 3D case through Simulation3D::run()
+//   aurora_host_tests --run2d <case.txt> <output dir> <32|64>   and for 2D
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -837,6 +838,44 @@ void run3d_case(const std::filesystem::path& case_file, const std::filesystem::p
     sim.run();
 }
 
+// A periodic 2D case file (tests/golden/make_run2d_golden.py): "n", "length",
+// "B" (uniform), "depth", "dt", "steps", "output_interval", "seed" and
+// "species" lines as in the 3D case.
+void run2d_case(const std::filesystem::path& case_file, const std::filesystem::path& out,
+                int precision) {
+    std::ifstream in(case_file);
+    if (!in) throw std::runtime_error("cannot open " + case_file.string());
+    Simulation2DConfig cfg;
+    cfg.permittivity = 1.0;
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::string key;
+        if (!(ls >> key)) continue;
+        if (key == "n") ls >> cfg.nx >> cfg.ny;
+        else if (key == "length") ls >> cfg.length_x >> cfg.length_y;
+        else if (key == "B") ls >> cfg.magnetic_field.x >> cfg.magnetic_field.y >> cfg.magnetic_field.z;
+        else if (key == "depth") ls >> cfg.out_of_plane_depth;
+        else if (key == "dt") ls >> cfg.dt;
+        else if (key == "steps") ls >> cfg.steps;
+        else if (key == "output_interval") ls >> cfg.output_interval;
+        else if (key == "seed") ls >> cfg.seed;
+        else if (key == "species") {
+            Species3DConfig sc;
+            ls >> sc.name >> sc.charge >> sc.mass >> sc.weight >> sc.particles >>
+                sc.thermal_velocity;
+            sc.precision = precision;
+            cfg.species.push_back(sc);
+        } else {
+            throw std::runtime_error("unknown case key " + key);
+        }
+        if (ls.fail()) throw std::runtime_error("malformed case line: " + line);
+    }
+    cfg.output_dir = out;
+    Simulation2D sim(cfg, std::make_shared<Device>(0));
+    sim.run();
+}
+
 int main(int argc, char** argv) {
     const std::string arg = argc > 1 ? argv[1] : "";
     if (arg == "--list") {
@@ -856,6 +895,15 @@ int main(int argc, char** argv) {
     if (arg == "--run1d" && argc > 3) {
         try {
             run1d_case(argv[2], argv[3]);
+        } catch (const std::exception& e) {
+            std::printf("error: %s\n", e.what());
+            return 1;
+        }
+        return 0;
+    }
+    if (arg == "--run2d" && argc > 4) {
+        try {
+            run2d_case(argv[2], argv[3], std::atoi(argv[4]));
         } catch (const std::exception& e) {
             std::printf("error: %s\n", e.what());
             return 1;

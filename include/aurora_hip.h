@@ -184,8 +184,10 @@ int aurora_hip_initialize(aurora_hip_ctx* ctx, double dt);
  * diagnostic synchronisation is computed on demand instead of every step
  * (aurora_hip_species_velocity, aurora_hip_energy). */
 int aurora_hip_step(aurora_hip_ctx* ctx, double dt, int64_t nsteps);
-/* Simulation3D::sample (Simulation3D.cpp:551-579), FP64: kinetic energy
- * per species (time-centred v), field energy, charge L1, live count. */
+/* Simulation3D::sample (Simulation3D.cpp:551-579) and its 1D/2D
+ * counterparts (Diagnostics.cpp:37-59, 114-140), FP64: kinetic energy per
+ * species (time-centred v), field energy and charge L1 over node volumes
+ * (a 2D node's area times the out-of-plane depth), live count. */
 int aurora_hip_energy(aurora_hip_ctx* ctx, double dt, double* kinetic,
                       double* field_energy, double* charge_l1,
                       int64_t* live);

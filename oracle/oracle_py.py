@@ -498,6 +498,9 @@ class Reference:
         L.ref_run3d.argtypes = [C.c_char_p, _i64p, _dp, C.c_double, C.c_int64, C.c_int64,
                                 C.c_uint64, C.c_int, C.c_int64, _dp, _dp, C.c_int,
                                 C.POINTER(C.c_char_p), _dp, C.c_char_p, C.c_int]
+        L.ref_run2d.argtypes = [C.c_char_p, _i64p, _dp, C.c_double, C.c_double, C.c_int64,
+                                C.c_int64, C.c_uint64, _dp, C.c_int, C.POINTER(C.c_char_p), _dp,
+                                C.c_char_p, C.c_int]
         L.ref_run1d.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, C.c_double, C.c_double,
                                 C.c_int64, C.c_int64, C.c_int, _dp, C.c_int, C.c_uint64, C.c_int,
                                 C.POINTER(C.c_char_p), _dp, C.c_char_p, C.c_int]
@@ -682,6 +685,22 @@ class Reference:
         rc = self.lib.ref_run3d(output_dir.encode(), _ptr(n, _i64p), _ptr(L, _dp), case["dt"],
                                 case["steps"], case["output_interval"], case["seed"],
                                 case["b_axis"], bx.size, _ptr(bx, _dp), _ptr(bv, _dp), len(sp),
+                                names, _ptr(rows, _dp), err, 512)
+        if rc:
+            raise RuntimeError(err.value.decode())
+
+
+    def run2d(self, case: dict, output_dir: str):
+        """The reference's Simulation2D::run() on a tests/golden/run2d case."""
+        sp = case["species"]
+        names = (C.c_char_p * len(sp))(*[s["name"].encode() for s in sp])
+        rows = f64([[s["charge"], s["mass"], s["weight"], s["particles"],
+                     s["thermal_velocity"]] for s in sp]).reshape(-1)
+        n, L, B = i64(case["n"]), f64(case["length"]), f64(case["B"])
+        err = C.create_string_buffer(512)
+        rc = self.lib.ref_run2d(output_dir.encode(), _ptr(n, _i64p), _ptr(L, _dp),
+                                case["depth"], case["dt"], case["steps"],
+                                case["output_interval"], case["seed"], _ptr(B, _dp), len(sp),
                                 names, _ptr(rows, _dp), err, 512)
         if rc:
             raise RuntimeError(err.value.decode())

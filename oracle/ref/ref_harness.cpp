@@ -14,6 +14,7 @@
 #include "pic/PrescribedField.hpp"
 #include "pic/Pusher.hpp"
 #include "pic/Simulation.hpp"
+#include "pic/Simulation2D.hpp"
 #include "pic/Simulation3D.hpp"
 #include "pic/Species.hpp"
 #include "pic/Swarm.hpp"
@@ -578,6 +579,50 @@ int ref_run3d(const char* output_dir, const int64_t* n, const double* L, double 
             cfg.species.push_back(sc);
         }
         pic::Simulation3D sim(cfg);
+        sim.run();
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(err, errlen, e.what());
+        return -1;
+    }
+}
+
+// The reference's own Simulation2D::run() on a periodic mesh with a
+// uniform B (any nonzero component selects its 2D3V Boris push), species
+// loaded from seed; rows of (charge, mass, weight, particles,
+// thermal_velocity); writes scalars.csv into output_dir.
+int ref_run2d(const char* output_dir, const int64_t* n, const double* L, double depth, double dt,
+              int64_t steps, int64_t output_interval, uint64_t seed, const double* B,
+              int nspecies, const char* const* names, const double* species, char* err,
+              int errlen) {
+    try {
+        pic::Simulation2DConfig cfg;
+        cfg.nx = static_cast<std::size_t>(n[0]);
+        cfg.ny = static_cast<std::size_t>(n[1]);
+        cfg.length_x = L[0];
+        cfg.length_y = L[1];
+        cfg.out_of_plane_depth = depth;
+        cfg.dt = dt;
+        cfg.steps = static_cast<std::size_t>(steps);
+        cfg.output_interval = static_cast<std::size_t>(output_interval);
+        cfg.output_dir = output_dir;
+        cfg.seed = static_cast<unsigned>(seed);
+        cfg.boundary = pic::Boundary::Periodic;
+        cfg.magnetic_field_x = B[0];
+        cfg.magnetic_field_y = B[1];
+        cfg.magnetic_field_z = B[2];
+        for (int k = 0; k < nspecies; ++k) {
+            const double* r = species + 5 * k;
+            pic::Species2DConfig sc;
+            sc.name = names[k];
+            sc.charge = r[0];
+            sc.mass = r[1];
+            sc.weight = r[2];
+            sc.particles = static_cast<std::size_t>(r[3]);
+            sc.thermal_velocity = r[4];
+            cfg.species.push_back(sc);
+        }
+        pic::Simulation2D sim(cfg);
         sim.run();
         return 0;
     } catch (const std::exception& e) {
