@@ -56,8 +56,10 @@ __global__ void k_tile_start(const unsigned* keys, int64_t n, int32_t* tile_star
 }
 
 // every SoA array of a species moved by one permutation in one pass: the
-// permutation is read once and each thread issues all its loads before
-// its stores
+// permutation is read once, and each thread takes kPermUnroll elements a
+// grid stride apart (coalesced across the wave for each of them), issuing
+// all their permutation loads, then all their gathers, then all stores, so
+// several dependent load chains are in flight per thread
 template <typename R>
 struct PermArrays {
     const R* src[6];
@@ -65,25 +67,45 @@ struct PermArrays {
     int count;           // offsets (dim) then velocities (3)
 };
 
+constexpr int kPermUnroll = 4;
+
 template <typename R>
 __global__ void k_permute(const unsigned* __restrict__ perm, int64_t n,
                           const int32_t* __restrict__ cell, int32_t* __restrict__ cell2,
                           const uint32_t* __restrict__ uid, uint32_t* __restrict__ uid2,
                           PermArrays<R> a) {
-    for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < n;
-         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const unsigned q = perm[p];
-        const int32_t c = cell[q];
-        const uint32_t u = uid[q];
-        R v[6];
+    constexpr int U = kPermUnroll;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t base = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; base < n;
+         base += U * stride) {
+        unsigned q[U];
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
-            if (k < a.count) v[k] = a.src[k][q];
-        cell2[p] = c;
-        uid2[p] = u;
+        for (int u = 0; u < U; ++u) {
+            const int64_t p = base + u * stride;
+            q[u] = p < n ? perm[p] : 0u;
+        }
+        int32_t c[U];
+        uint32_t id[U];
+        R v[U][6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
-            if (k < a.count) a.dst[k][p] = v[k];
+        for (int u = 0; u < U; ++u) {
+            if (base + u * stride >= n) continue;
+            c[u] = cell[q[u]];
+            id[u] = uid[q[u]];
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if (k < a.count) v[u][k] = a.src[k][q[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t p = base + u * stride;
+            if (p >= n) continue;
+            cell2[p] = c[u];
+            uid2[p] = id[u];
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if (k < a.count) a.dst[k][p] = v[u][k];
+        }
     }
 }
 
