@@ -263,6 +263,15 @@ void run_axis(Context& c, double2* data, int axis, int inverse) {
 //                outputs (E64, E32, phi) written directly
 // Twiddles are staged in LDS; all index arithmetic is shifts and masks.
 
+// The fused kernels run 256 threads; a block stages at most kFftLds
+// (kLastLds) elements, so a thread loads at most kPer (kPerLast) of them.
+// The forward kernels and the middle axes issue all of a thread's global
+// loads into registers before the first LDS store, so the loads' latencies
+// overlap instead of adding up (the x inverse, which reads two arrays,
+// keeps its loop: holding both in registers cost it occupancy).
+constexpr int kFftThreads = 256;
+constexpr int kPer = kFftLds / kFftThreads;
+
 template <int LOG2N>
 __device__ __forceinline__ void stage_twiddles(const double2* tw, double2* s_tw) {
     constexpr int N = 1 << LOG2N;
@@ -349,9 +358,17 @@ __global__ void __launch_bounds__(256) k_fwd_x_real(const double* __restrict__ r
     stage_twiddles<LOG2N>(tw, s_tw);
     const int64_t line0 = static_cast<int64_t>(blockIdx.x) * B;
     const int lines = static_cast<int>(nlines - line0 < B ? nlines - line0 : B);
-    for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
-        const int b = t >> LOG2N, e = t & (N - 1);
-        lds[L::at(b, e)] = make_double2(rho[(line0 + b) * N + bitrev(e, LOG2N)], 0.0);
+    double r[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int t = threadIdx.x + u * kFftThreads;
+        if (t < lines * N)
+            r[u] = rho[(line0 + (t >> LOG2N)) * N + bitrev(t & (N - 1), LOG2N)];
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int t = threadIdx.x + u * kFftThreads;
+        if (t < lines * N) lds[L::at(t >> LOG2N, t & (N - 1))] = make_double2(r[u], 0.0);
     }
     __syncthreads();
     fft_lines<LOG2N>(lds, lines, s_tw, false);
@@ -378,12 +395,21 @@ __global__ void __launch_bounds__(256) k_fwd_x_acc(RhoArgs ra, ClearArgs cl,
     stage_twiddles<LOG2N>(tw, s_tw);
     const int64_t line0 = static_cast<int64_t>(blockIdx.x) * B;
     const int lines = static_cast<int>(nlines - line0 < B ? nlines - line0 : B);
-    for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
+    double r[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int t = threadIdx.x + u * kFftThreads;
+        if (t < lines * N)
+            r[u] = rho_at(ra, (line0 + (t >> LOG2N)) * N + bitrev(t & (N - 1), LOG2N));
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int t = threadIdx.x + u * kFftThreads;
+        if (t >= lines * N) continue;
         const int b = t >> LOG2N, e = t & (N - 1);
         const int64_t i = (line0 + b) * N + bitrev(e, LOG2N);
-        const double r = rho_at(ra, i);
         for (int s = 0; s < cl.n; ++s) cl.p[s][i] = 0;
-        lds[L::at(b, e)] = make_double2(r, 0.0);
+        lds[L::at(b, e)] = make_double2(r[u], 0.0);
     }
     __syncthreads();
     fft_lines<LOG2N>(lds, lines, s_tw, false);
@@ -411,11 +437,19 @@ __global__ void __launch_bounds__(256) k_axis_lines(double2* a, double2* b2, Axi
     stage_twiddles<LOG2N>(tw, s_tw);
     const int B = 1 << LB;
     const int64_t line0 = static_cast<int64_t>(blockIdx.x) << LB;
-    for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
+    double2 r[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int t = threadIdx.x + u * kFftThreads;
         const int bb = t & (B - 1), e = t >> LB;
         const int64_t line = line0 + bb;
-        lds[L::at(bb, e)] = line < p.nlines ? data[line_node(p, line, bitrev(e, LOG2N))]
+        r[u] = t < B * N && line < p.nlines ? data[line_node(p, line, bitrev(e, LOG2N))]
                                             : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int t = threadIdx.x + u * kFftThreads;
+        if (t < B * N) lds[L::at(t & (B - 1), t >> LB)] = r[u];
     }
     __syncthreads();
     fft_lines<LOG2N>(lds, B, s_tw, inverse != 0);
@@ -432,6 +466,7 @@ __global__ void __launch_bounds__(256) k_axis_lines(double2* a, double2* b2, Axi
 }
 
 constexpr int kLastLds = 1280;   // complex elements per buffer in k_last, k_inv_x_unpack (20 KiB)
+constexpr int kPerLast = (kLastLds + kFftThreads - 1) / kFftThreads;
 
 // forward transform of the last axis, Green step, inverse transforms of
 // the two spectra along the same axis (scaled 1/N), written to fa and fb.
@@ -449,11 +484,19 @@ __global__ void __launch_bounds__(256) k_last(double2* fa, double2* fb, AxisPlan
     stage_twiddles<LOG2N>(tw, s_tw);
     const int B = 1 << LB;
     const int64_t line0 = static_cast<int64_t>(blockIdx.x) << LB;
-    for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
+    double2 r[kPerLast];
+#pragma unroll
+    for (int u = 0; u < kPerLast; ++u) {
+        const int t = threadIdx.x + u * kFftThreads;
         const int bb = t & (B - 1), e = t >> LB;
         const int64_t line = line0 + bb;
-        X[L::at(bb, e)] = line < p.nlines ? fa[line_node(p, line, bitrev(e, LOG2N))]
-                                          : make_double2(0.0, 0.0);
+        r[u] = t < B * N && line < p.nlines ? fa[line_node(p, line, bitrev(e, LOG2N))]
+                                            : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < kPerLast; ++u) {
+        const int t = threadIdx.x + u * kFftThreads;
+        if (t < B * N) X[L::at(t & (B - 1), t >> LB)] = r[u];
     }
     __syncthreads();
     fft_lines<LOG2N>(X, B, s_tw, false);
