@@ -15,7 +15,8 @@ SHAPES = [(8, 8, 8, 2, 512, 0), (8, 8, 8, 1, 512, 0), (8, 8, 4, 2, 512, 0),
           (16, 8, 4, 2, 512, 0), (4, 4, 4, 2, 512, 0), (8, 8, 4, 2, 512, 1),
           (8, 8, 4, 2, 256, 1), (8, 8, 4, 2, 256, 0), (8, 8, 8, 2, 512, 1),
           (8, 8, 8, 2, 256, 1), (8, 8, 4, 2, 512, 3), (8, 8, 4, 1, 512, 3),
-          (8, 8, 4, 2, 512, 5), (8, 8, 8, 2, 512, 5)]
+          (8, 8, 4, 2, 512, 5), (8, 8, 8, 2, 512, 5), (8, 8, 8, 1, 512, 1),
+          (8, 8, 8, 1, 512, 5)]
 
 
 def tune(ctx, dt, shapes=SHAPES, sort_intervals=(20,), steps=20, log=None, adaptive=False):

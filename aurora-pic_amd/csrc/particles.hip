@@ -960,7 +960,9 @@ bool launch_tile3d(Context& c, Species& sp, double dt) {
            launch_tile3d_t<8, 8, 4, 2, 512, 3>(c, sp, a, mode) ||
            launch_tile3d_t<8, 8, 4, 1, 512, 3>(c, sp, a, mode) ||
            launch_tile3d_t<8, 8, 4, 2, 512, 5>(c, sp, a, mode) ||
-           launch_tile3d_t<8, 8, 8, 2, 512, 5>(c, sp, a, mode);
+           launch_tile3d_t<8, 8, 8, 2, 512, 5>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 8, 1, 512, 1>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 8, 1, 512, 5>(c, sp, a, mode);
 }
 
 template <int DIM, typename R, int BMODE>

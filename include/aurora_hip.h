@@ -219,7 +219,7 @@ int aurora_hip_set_tiling(aurora_hip_ctx* ctx, int tx, int ty, int tz,
  * share a CU; bit 2: the field box in LDS by component instead of float4
  * nodes).  Built: 512/0 for every shape; 512/1, 256/1, 256/0, 512/3, 512/5
  * for 8x8x4 halo 2; 512/3 for 8x8x4 halo 1; 512/1, 256/1, 512/5 for 8x8x8
- * halo 2.  Results do not depend on it. */
+ * halo 2; 512/1, 512/5 for 8x8x8 halo 1.  Results do not depend on it. */
 int aurora_hip_set_push_variant(aurora_hip_ctx* ctx, int block, int prefetch);
 /* Cell sort every interval steps (0 = never) and an immediate sort. */
 int aurora_hip_set_sort_interval(aurora_hip_ctx* ctx, int64_t interval);

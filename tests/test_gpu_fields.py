@@ -250,7 +250,8 @@ def test_sort_preserves_particles_and_charge(hip_ctx):
                                    (8, 8, 4, 2, 256, 1), (8, 8, 4, 2, 512, 1),
                                    (8, 8, 8, 2, 256, 1), (8, 8, 4, 2, 512, 3),
                                    (8, 8, 4, 1, 512, 3), (8, 8, 4, 2, 512, 5),
-                                   (8, 8, 8, 2, 512, 5)])
+                                   (8, 8, 8, 2, 512, 5), (8, 8, 8, 1, 512, 1),
+                                   (8, 8, 8, 1, 512, 5)])
 def test_tiled_push_bit_identical_to_untiled(shape):
     """The LDS-tiled 3D push performs the same operations as the untiled
     kernel: particle state and fixed-point charge agree bit for bit, also
