@@ -488,14 +488,23 @@ __global__ void __launch_bounds__(kBlock) k_ledger_1d(const double* push_part, i
     if (threadIdx.x < 3) ledger[threadIdx.x] += s;
 }
 
+// one atomic per workgroup (a few per CU): per-wave atomics on one address
+// serialise
 __global__ void __launch_bounds__(kBlock) k_count_live(const int32_t* cell, int64_t slots,
                                                        unsigned long long* out) {
+    __shared__ unsigned long long wsum[kBlock / 64];
     unsigned long long n = 0;
     for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < slots;
          p += static_cast<int64_t>(gridDim.x) * blockDim.x)
         n += cell[p] >= 0;
     for (int off = 32; off > 0; off >>= 1) n += __shfl_down(n, off, 64);
-    if ((threadIdx.x & 63) == 0 && n) atomicAdd(out, n);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += wsum[w];
+        if (t) atomicAdd(out, t);
+    }
 }
 
 // sign = +1: synchronize (v from v_half); -1: half-step init (v_half from v)
@@ -1181,7 +1190,8 @@ int64_t live_count(Context& c, Species& sp) {
     DevBuf<unsigned long long> n;
     n.alloc(1);
     AUR_HIP(hipMemsetAsync(n.ptr, 0, sizeof(unsigned long long), c.stream));
-    k_count_live<<<launch_blocks(c, sp.slots), kBlock, 0, c.stream>>>(sp.cell.ptr, sp.slots, n.ptr);
+    const int g = std::min<int>(launch_blocks(c, sp.slots), c.cu_count * 4);
+    k_count_live<<<g, kBlock, 0, c.stream>>>(sp.cell.ptr, sp.slots, n.ptr);
     AUR_HIP(hipGetLastError());
     unsigned long long h = 0;
     AUR_HIP(hipMemcpyAsync(&h, n.ptr, sizeof h, hipMemcpyDeviceToHost, c.stream));
