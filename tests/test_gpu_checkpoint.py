@@ -7,7 +7,10 @@ continuing from a restart file equals the uninterrupted run).
 - 1D3V RF discharge with null-collision MCC and ionization products
   (absorbing walls, appended products with new uids, the RNG keyed by step):
   the same, plus the MCC counters and the wall ledger.
-- A context configured differently refuses the file (physics signature)."""
+- A context configured differently refuses the file (physics signature).
+- 2D3V with uniform B_z and an FP64 species beside an FP32 one: the same.
+- A species with no particles goes through steps, sorts, energies, ledgers
+  and a checkpoint round trip."""
 import math
 
 import numpy as np
@@ -143,3 +146,73 @@ def test_checkpoint_refuses_other_physics(tmp_path):
         with pytest.raises(hip.HipError, match="physics signature mismatch"):
             d.checkpoint_load(path)
         d.close()
+
+
+def test_checkpoint_restart_2d_fp64_is_bit_exact(tmp_path):
+    """2D3V with a uniform B_z, an FP64 species beside an FP32 one: the
+    restart continues to the bit, FP64 arrays included."""
+    from aurora_pic import hip
+    path = str(tmp_path / "run2d.ckpt")
+    dt = 0.02
+
+    def ctx():
+        rng = np.random.default_rng(11)
+        c = hip.Context(0)
+        c.mesh([64, 32], [2.0, 1.0], 0, 1.0)
+        c.uniform_b([0.0, 0.0, 0.4])
+        sids = []
+        for q, m, prec in ((-1.0, 1.0, 64), (1.0, 25.0, 32)):
+            N = 5000
+            sid = c.add_species(q, m, 2.0 / N, N, precision=prec)
+            c.upload(sid, rng.uniform(0, 1, (N, 2)) * [2.0, 1.0],
+                     rng.normal(0, 0.2 / math.sqrt(m), (N, 3)), kind=1)
+            sids.append(sid)
+        return c, sids
+
+    c, sids = ctx()
+    c.initialize(dt)
+    c.step(dt, 15)
+    c.checkpoint_save(path)
+    c.step(dt, 10)
+    a = _state(c, sids)
+    c.close()
+    d, dsids = ctx()
+    d.checkpoint_load(path)
+    d.step(dt, 10)
+    _assert_same(a, _state(d, dsids))
+    d.close()
+
+
+def test_empty_species_through_step_sort_checkpoint(tmp_path):
+    """A species with no particles (before and after an upload of zero)
+    takes part in every call: steps, sorts, energies, ledgers, checkpoints."""
+    from aurora_pic import hip
+    path = str(tmp_path / "empty.ckpt")
+    rng = np.random.default_rng(2)
+    c = hip.Context(0)
+    c.mesh([16, 16, 16], [1.0, 1.0, 1.0], 0, 1.0)
+    full = c.add_species(-1.0, 1.0, 1.0 / 4096, 4096)
+    empty = c.add_species(1.0, 100.0, 1.0, 16)
+    c.upload(full, rng.uniform(0, 1, (4096, 3)), rng.normal(0, 0.1, (4096, 3)))
+    c.upload(empty, np.zeros((0, 3)), np.zeros((0, 3)))
+    c.sort()
+    c.initialize(0.01)
+    c.step(0.01, 5)
+    c.sort()
+    ke, fe, l1, live = c.energy(0.01)
+    assert live == 4096 and ke[1] == 0.0 and ke[0] > 0.0
+    assert c.count(empty) == (0, 0)
+    counts, e2 = c.losses(empty)
+    assert counts.sum() == 0
+    c.checkpoint_save(path)
+    c.step(0.01, 3)
+    s = _state(c, [full, empty])
+    c.close()
+    d = hip.Context(0)
+    d.mesh([16, 16, 16], [1.0, 1.0, 1.0], 0, 1.0)
+    full2 = d.add_species(-1.0, 1.0, 1.0 / 4096, 4096)
+    empty2 = d.add_species(1.0, 100.0, 1.0, 16)
+    d.checkpoint_load(path)
+    d.step(0.01, 3)
+    _assert_same(s, _state(d, [full2, empty2]))
+    d.close()
