@@ -2,11 +2,15 @@
 //   1D periodic   direct spectral sum with continuous k (407-440)
 //   1D Dirichlet  Thomas algorithm, same recurrence (733-759)
 //   2D/3D periodic FFT Poisson with spectral E (442-499, 659-731)
-// The 2D/3D solve runs one forward complex transform of rho and two
-// inverse transforms: (Ex + i Ey) and (Ez + i phi).  Each spectral field is
-// first projected onto its Hermitian part (zeroing the own-axis Nyquist
-// plane of E_a), which leaves its real inverse - the quantity the reference
-// keeps - unchanged and lets two real fields share one complex inverse.
+// The 2D/3D solve transforms rho forward and the spectral fields back.
+// Each spectral field is first projected onto its Hermitian part (zeroing
+// the own-axis Nyquist plane of E_a), which leaves its real inverse - the
+// quantity the reference keeps - unchanged.  The general path packs two real
+// fields into one complex inverse, (Ex + i Ey) and (Ez + i phi); the
+// power-of-two path keeps only the non-negative x frequencies of every
+// spectrum (the others are their conjugates) and rebuilds each x line's full
+// spectrum for the last inverse.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -161,6 +165,37 @@ __device__ __forceinline__ void green_mode(const GreenArgs& g, int ix, int iy, i
     Bv = g.want_phi ? make_double2(ezr - pi, ezi + pr) : make_double2(ezr, ezi);
 }
 
+// green_mode's spectral step with the fields kept apart: E_x, E_y, E_z and
+// phi of the mode (E_z zero outside 3D); ex + i ey and ez + i ph are
+// green_mode's A and Bv.  Returned by value: no local's address is taken,
+// so every value stays in registers.
+struct ModeFields {
+    double2 ex, ey, ez, ph;
+};
+
+__device__ __forceinline__ ModeFields green_split(const GreenArgs& g, int ix, int iy, int iz,
+                                                  double2 r) {
+#pragma clang fp contract(off)
+    ModeFields f;
+    f.ex = f.ey = f.ez = f.ph = make_double2(0.0, 0.0);
+    if (ix == 0 && iy == 0 && iz == 0) return f;
+    const double kx = g.twopi_over_L[0] * mode_of(ix, g.nx);
+    const double ky = g.dim > 1 ? g.twopi_over_L[1] * mode_of(iy, g.ny) : 0.0;
+    const double kz = g.dim > 2 ? g.twopi_over_L[2] * mode_of(iz, g.nz) : 0.0;
+    double k2 = kx * kx + ky * ky;
+    if (g.dim > 2) k2 += kz * kz;
+    const double den = g.eps * k2;
+    const double pr = r.x / den, pi = r.y / den;
+    const bool nyx = (g.nx % 2 == 0) && ix == g.nx / 2;
+    const bool nyy = (g.ny % 2 == 0) && iy == g.ny / 2;
+    const bool nyz = (g.nz % 2 == 0) && iz == g.nz / 2;
+    if (!nyx) f.ex = make_double2(kx * pi, -kx * pr);
+    if (!nyy) f.ey = make_double2(ky * pi, -ky * pr);
+    if (g.dim > 2 && !nyz) f.ez = make_double2(kz * pi, -kz * pr);
+    f.ph = make_double2(pr, pi);
+    return f;
+}
+
 // fa: rho_hat in, (Ex + i Ey)_hat out; fb: (Ez + i phi)_hat out
 __global__ void k_green(GreenArgs g, double2* fa, double2* fb, int64_t total) {
     for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
@@ -254,23 +289,44 @@ void run_axis(Context& c, double2* data, int axis, int inverse) {
 // Fused power-of-two path (every axis a power of two, 2D and 3D).  The
 // transforms are the ones k_fft_axis performs (radix-2 DIT over bit-reversed
 // LDS lines, the same twiddle table, 1/n per inverse axis), regrouped so the
-// field solve makes fewer passes over HBM:
-//   fwd_x_real   reads rho, forward transform along x             -> fa
-//   lines        forward (or inverse) transform of a middle axis  fa (fb)
-//   last         forward transform of the last axis, the Green step of
-//                k_green, both inverse transforms of that axis      -> fa, fb
-//   inv_x_unpack inverse transform along x of fa and fb, the k_unpack
-//                outputs (E64, E32, phi) written directly
+// field solve makes fewer passes over HBM, on half spectra: rho is real, so
+// only x frequencies 0 .. nx/2 are kept (NH = nx/2 + 1 of them), laid out
+// [kx][z][y] (node (kx * nz + z) * ny + y; P = ny * nz nodes per kx plane):
+//   fwd_x        reads rho (or the species accumulators), forward
+//                transform along x, frequencies 0 .. nx/2 written     -> H0
+//   rows         forward or inverse transform along y (3D): the lines
+//                are contiguous in this layout                   H0 (H1..H3)
+//   last         forward transform of the last axis (z in 3D, y in
+//                2D), the Green step of k_green with the fields kept
+//                apart, their inverse transforms of that axis
+//                                              -> H0 Ex, H1 Ey, H2 Ez, H3 phi
+//   inv_x_c2r    per x line, the full spectrum of (Ex + i Ey) and
+//                (Ez + i phi) rebuilt from the halves (frequency n - k
+//                is the conjugate of k), inverse transform along x, the
+//                k_unpack outputs (E64, E32, phi) written directly
 // Twiddles are staged in LDS; all index arithmetic is shifts and masks.
 
-// The fused kernels run 256 threads; a block stages at most kFftLds
-// (kLastLds) elements, so a thread loads at most kPer (kPerLast) of them.
-// The forward kernels and the middle axes issue all of a thread's global
-// loads into registers before the first LDS store, so the loads' latencies
-// overlap instead of adding up (the x inverse, which reads two arrays,
-// keeps its loop: holding both in registers cost it occupancy).
+// The fused kernels run 256 threads; a block stages at most kLastLds
+// elements per buffer, so a thread loads at most kPerLast of them.  The
+// forward kernels, the y lines and the last axis issue all of a thread's
+// global loads into registers before the first LDS store, so the loads'
+// latencies overlap instead of adding up (the x inverse, which reads two or
+// four arrays, keeps its loop).
 constexpr int kFftThreads = 256;
-constexpr int kPer = kFftLds / kFftThreads;
+// complex elements per LDS buffer of the fused kernels (20 KiB): lines of
+// N take N + N/8 + 1 elements, so a block holds B lines with B * N <= 1024
+// (powers of two), and small buffers let more blocks share a CU
+constexpr int kLastLds = 1280;
+constexpr int kPerLast = (kLastLds + kFftThreads - 1) / kFftThreads;
+
+// lines per block as a power of two: the LDS budget over the padded line
+// length (n + n/8 + 1), at most 64
+constexpr int lines_shift(int64_t budget, int64_t n) {
+    const int64_t ls = n + n / 8 + 1;
+    int l = 0;
+    while ((int64_t{2} << l) * ls <= budget && l < 6) ++l;
+    return l;
+}
 
 template <int LOG2N>
 __device__ __forceinline__ void stage_twiddles(const double2* tw, double2* s_tw) {
@@ -288,6 +344,12 @@ struct Lines {
     static constexpr int LS = N + (N >> 3) + 1;
     __device__ static __forceinline__ int at(int b, int i) { return b * LS + i + (i >> 3); }
 };
+
+// LDS elements of one buffer: the lines a block of the fused path holds
+// (lines_shift(kLastLds, N), which the host launches with) times LS; the
+// exact size lets more blocks share a CU
+template <int LOG2N>
+constexpr int kLds = (1 << lines_shift(kLastLds, 1 << LOG2N)) * Lines<LOG2N>::LS;
 
 // The radix-2 stages over `lines` lines of N (bit-reversed order in LDS),
 // three stages per pass: a thread loads the 8 elements i0 + k*2^S that
@@ -347,33 +409,47 @@ __device__ __forceinline__ void fft_lines(double2* lds, int lines, const double2
     radix_passes<LOG2N, 0>(lds, lines, s_tw, inverse);
 }
 
+// frequencies 0 .. N/2 of the B lines in LDS to the half layout: element
+// kx of x line l at kx * P + l (consecutive threads, consecutive lines)
+template <int LOG2N>
+__device__ __forceinline__ void store_half_x(const double2* lds, double2* __restrict__ out,
+                                             int64_t line0, int lines, int LB, int64_t P) {
+    using L = Lines<LOG2N>;
+    constexpr int N = 1 << LOG2N, NH = N / 2 + 1;
+    const int B = 1 << LB;
+    for (int t = threadIdx.x; t < B * NH; t += blockDim.x) {
+        const int b = t & (B - 1), kx = t >> LB;
+        if (b < lines) out[kx * P + line0 + b] = lds[L::at(b, kx)];
+    }
+}
+
 template <int LOG2N>
 __global__ void __launch_bounds__(256) k_fwd_x_real(const double* __restrict__ rho,
                                                     double2* __restrict__ out, int64_t nlines,
-                                                    int B, const double2* __restrict__ tw) {
+                                                    int LB, const double2* __restrict__ tw) {
     using L = Lines<LOG2N>;
     constexpr int N = 1 << LOG2N;
-    __shared__ double2 lds[kFftLds];
+    __shared__ double2 lds[kLds<LOG2N>];
     __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
     stage_twiddles<LOG2N>(tw, s_tw);
+    const int B = 1 << LB;
     const int64_t line0 = static_cast<int64_t>(blockIdx.x) * B;
     const int lines = static_cast<int>(nlines - line0 < B ? nlines - line0 : B);
-    double r[kPer];
+    double r[kPerLast];
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
+    for (int u = 0; u < kPerLast; ++u) {
         const int t = threadIdx.x + u * kFftThreads;
-        if (t < lines * N)
-            r[u] = rho[(line0 + (t >> LOG2N)) * N + bitrev(t & (N - 1), LOG2N)];
+        if (t < lines * N) r[u] = rho[line0 * N + t];
     }
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
+    for (int u = 0; u < kPerLast; ++u) {
         const int t = threadIdx.x + u * kFftThreads;
-        if (t < lines * N) lds[L::at(t >> LOG2N, t & (N - 1))] = make_double2(r[u], 0.0);
+        if (t < lines * N)
+            lds[L::at(t >> LOG2N, bitrev(t & (N - 1), LOG2N))] = make_double2(r[u], 0.0);
     }
     __syncthreads();
     fft_lines<LOG2N>(lds, lines, s_tw, false);
-    for (int t = threadIdx.x; t < lines * N; t += blockDim.x)
-        out[line0 * N + t] = lds[L::at(t >> LOG2N, t & (N - 1))];
+    store_half_x<LOG2N>(lds, out, line0, lines, LB, nlines);
 }
 
 // The step's variant of k_fwd_x_real: the charge density of each node from
@@ -387,34 +463,31 @@ struct ClearArgs {
 template <int LOG2N>
 __global__ void __launch_bounds__(256) k_fwd_x_acc(RhoArgs ra, ClearArgs cl,
                                                    double2* __restrict__ out, int64_t nlines,
-                                                   int B, const double2* __restrict__ tw) {
+                                                   int LB, const double2* __restrict__ tw) {
     using L = Lines<LOG2N>;
     constexpr int N = 1 << LOG2N;
-    __shared__ double2 lds[kFftLds];
+    __shared__ double2 lds[kLds<LOG2N>];
     __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
     stage_twiddles<LOG2N>(tw, s_tw);
+    const int B = 1 << LB;
     const int64_t line0 = static_cast<int64_t>(blockIdx.x) * B;
     const int lines = static_cast<int>(nlines - line0 < B ? nlines - line0 : B);
-    double r[kPer];
+    double r[kPerLast];
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
+    for (int u = 0; u < kPerLast; ++u) {
         const int t = threadIdx.x + u * kFftThreads;
-        if (t < lines * N)
-            r[u] = rho_at(ra, (line0 + (t >> LOG2N)) * N + bitrev(t & (N - 1), LOG2N));
+        if (t < lines * N) r[u] = rho_at(ra, line0 * N + t);
     }
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
+    for (int u = 0; u < kPerLast; ++u) {
         const int t = threadIdx.x + u * kFftThreads;
         if (t >= lines * N) continue;
-        const int b = t >> LOG2N, e = t & (N - 1);
-        const int64_t i = (line0 + b) * N + bitrev(e, LOG2N);
-        for (int s = 0; s < cl.n; ++s) cl.p[s][i] = 0;
-        lds[L::at(b, e)] = make_double2(r[u], 0.0);
+        for (int s = 0; s < cl.n; ++s) cl.p[s][line0 * N + t] = 0;
+        lds[L::at(t >> LOG2N, bitrev(t & (N - 1), LOG2N))] = make_double2(r[u], 0.0);
     }
     __syncthreads();
     fft_lines<LOG2N>(lds, lines, s_tw, false);
-    for (int t = threadIdx.x; t < lines * N; t += blockDim.x)
-        out[line0 * N + t] = lds[L::at(t >> LOG2N, t & (N - 1))];
+    store_half_x<LOG2N>(lds, out, line0, lines, LB, nlines);
 }
 
 // Node index of element e of line `line` of a power-of-two axis plan.
@@ -423,63 +496,68 @@ __device__ __forceinline__ int64_t line_node(const AxisPlan& p, int64_t line, in
            static_cast<int64_t>(e) * p.elem_stride;
 }
 
-// forward or inverse transform of one strided (middle) axis, 1 << LB lines
-// per block (consecutive lines are consecutive nodes: coalesced); blockIdx.y
-// selects the array (a or b2)
+// the half spectra H0 .. H3 of the power-of-two path
+struct Halves {
+    double2* h[4];
+};
+
+// forward or inverse transform of contiguous lines of N (the y lines of the
+// half layout), 1 << LB lines per block; blockIdx.y selects the array
 template <int LOG2N>
-__global__ void __launch_bounds__(256) k_axis_lines(double2* a, double2* b2, AxisPlan p, int LB,
-                                                    const double2* __restrict__ tw, int inverse) {
+__global__ void __launch_bounds__(256) k_rows(Halves arr, int64_t nlines, int LB,
+                                              const double2* __restrict__ tw, int inverse) {
     using L = Lines<LOG2N>;
     constexpr int N = 1 << LOG2N;
-    __shared__ double2 lds[kFftLds];
+    __shared__ double2 lds[kLds<LOG2N>];
     __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
-    double2* data = blockIdx.y == 0 ? a : b2;
+    // a select, not arr.h[blockIdx.y]: a dynamic index into the argument
+    // array would copy it to private (scratch) memory
+    const unsigned w = blockIdx.y;
+    double2* data = w == 0 ? arr.h[0] : w == 1 ? arr.h[1] : w == 2 ? arr.h[2] : arr.h[3];
     stage_twiddles<LOG2N>(tw, s_tw);
     const int B = 1 << LB;
-    const int64_t line0 = static_cast<int64_t>(blockIdx.x) << LB;
-    double2 r[kPer];
+    const int64_t line0 = static_cast<int64_t>(blockIdx.x) * B;
+    const int lines = static_cast<int>(nlines - line0 < B ? nlines - line0 : B);
+    double2 r[kPerLast];
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
+    for (int u = 0; u < kPerLast; ++u) {
         const int t = threadIdx.x + u * kFftThreads;
-        const int bb = t & (B - 1), e = t >> LB;
-        const int64_t line = line0 + bb;
-        r[u] = t < B * N && line < p.nlines ? data[line_node(p, line, bitrev(e, LOG2N))]
-                                            : make_double2(0.0, 0.0);
+        r[u] = t < lines * N ? data[line0 * N + t] : make_double2(0.0, 0.0);
     }
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
+    for (int u = 0; u < kPerLast; ++u) {
         const int t = threadIdx.x + u * kFftThreads;
-        if (t < B * N) lds[L::at(t & (B - 1), t >> LB)] = r[u];
+        if (t < lines * N) lds[L::at(t >> LOG2N, bitrev(t & (N - 1), LOG2N))] = r[u];
     }
     __syncthreads();
-    fft_lines<LOG2N>(lds, B, s_tw, inverse != 0);
+    fft_lines<LOG2N>(lds, lines, s_tw, inverse != 0);
     const double scale = inverse ? 1.0 / static_cast<double>(N) : 1.0;
-    for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
-        const int bb = t & (B - 1), e = t >> LB;
-        const int64_t line = line0 + bb;
-        if (line >= p.nlines) continue;
-        double2 v = lds[L::at(bb, e)];
+    for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
+        double2 v = lds[L::at(t >> LOG2N, t & (N - 1))];
         v.x *= scale;
         v.y *= scale;
-        data[line_node(p, line, e)] = v;
+        data[line0 * N + t] = v;
     }
 }
 
-constexpr int kLastLds = 1280;   // complex elements per buffer in k_last, k_inv_x_unpack (20 KiB)
-constexpr int kPerLast = (kLastLds + kFftThreads - 1) / kFftThreads;
-
-// forward transform of the last axis, Green step, inverse transforms of
-// the two spectra along the same axis (scaled 1/N), written to fa and fb.
-// The first spectrum is written over the transformed input once every
-// thread holds its share of it in registers (at most 4 elements: B*N <= 1024).
+// forward transform of the last axis of H0, Green step (green_split),
+// inverse transforms along the same axis (scaled 1/N) of Ex, Ey, Ez (3D)
+// and phi (when wanted) to H0 .. H3.  The fields go two at a time through
+// one LDS buffer of 2B lines (field a in lines 0 .. B-1, field b in B ..
+// 2B-1), so one pass of radix stages transforms both and keeps every thread
+// busy.  A thread keeps only its (at most 4: B*N <= 1024) elements of
+// rho_hat in registers and evaluates the Green step again for each pair
+// (the same operations, so the same values).  The fields are written over
+// the transformed input once every thread holds its share of rho_hat.
+// sy, sp: log2 of ny and of P = ny * nz (the half-layout node is
+// (kx << sp) + (z << sy) + y).
 template <int LOG2N>
-__global__ void __launch_bounds__(256) k_last(double2* fa, double2* fb, AxisPlan p, int LB,
+__global__ void __launch_bounds__(256) k_last(Halves arr, AxisPlan p, int LB,
                                               const double2* __restrict__ tw, GreenArgs g,
-                                              int write_b) {
+                                              int sy, int sp, int want_ez) {
     using L = Lines<LOG2N>;
     constexpr int N = 1 << LOG2N;
-    __shared__ double2 X[kLastLds];
-    __shared__ double2 Bb[kLastLds];
+    __shared__ double2 X[2 * kLds<LOG2N>];
     __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
     stage_twiddles<LOG2N>(tw, s_tw);
     const int B = 1 << LB;
@@ -490,8 +568,11 @@ __global__ void __launch_bounds__(256) k_last(double2* fa, double2* fb, AxisPlan
         const int t = threadIdx.x + u * kFftThreads;
         const int bb = t & (B - 1), e = t >> LB;
         const int64_t line = line0 + bb;
-        r[u] = t < B * N && line < p.nlines ? fa[line_node(p, line, bitrev(e, LOG2N))]
-                                            : make_double2(0.0, 0.0);
+        // an unconditional load and selects of the components: a select
+        // between a loaded element and a local would go through scratch
+        const bool ok = t < B * N && line < p.nlines;
+        const double2 v = arr.h[0][ok ? line_node(p, line, bitrev(e, LOG2N)) : 0];
+        r[u] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
     }
 #pragma unroll
     for (int u = 0; u < kPerLast; ++u) {
@@ -500,77 +581,118 @@ __global__ void __launch_bounds__(256) k_last(double2* fa, double2* fb, AxisPlan
     }
     __syncthreads();
     fft_lines<LOG2N>(X, B, s_tw, false);
-    double2 av[4], bv[4];
+    double2 rh[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int t = threadIdx.x + r * 256;
-        av[r] = make_double2(0.0, 0.0);
-        bv[r] = make_double2(0.0, 0.0);
-        if (t >= B * N) continue;
-        const int bb = t >> LOG2N, e = t & (N - 1);
-        const int64_t line = line0 + bb;
-        if (line < p.nlines) {
-            const int64_t node = line_node(p, line, e);
-            const int ix = static_cast<int>(node & (g.nx - 1));
-            const int iy = static_cast<int>((node / g.nx) & (g.ny - 1));
-            const int iz = static_cast<int>(node / (static_cast<int64_t>(g.nx) * g.ny));
-            green_mode(g, ix, iy, iz, X[L::at(bb, e)], av[r], bv[r]);
-        }
+    for (int q = 0; q < 4; ++q) {
+        const int t = threadIdx.x + q * 256;
+        const int tc = t < B * N ? t : 0;
+        const double2 v = X[L::at(tc >> LOG2N, tc & (N - 1))];
+        const bool ok = t < B * N && line0 + (t >> LOG2N) < p.nlines;
+        rh[q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
     }
-    __syncthreads();
+    // fields FA (and FB >= 0) of every element (0 Ex, 1 Ey, 2 Ez, 3 phi)
+    // into X in bit-reversed line order, inverse transforms written to fa
+    // (and fb)
+    auto pair = [&]<int FA, int FB>(double2* fa, double2* fb) {
+        __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int t = threadIdx.x + r * 256;
-        if (t >= B * N) continue;
-        const int bb = t >> LOG2N, e = t & (N - 1);
-        X[L::at(bb, bitrev(e, LOG2N))] = av[r];
-        Bb[L::at(bb, bitrev(e, LOG2N))] = bv[r];
-    }
-    __syncthreads();
-    fft_lines<LOG2N>(X, B, s_tw, true);
-    if (write_b) fft_lines<LOG2N>(Bb, B, s_tw, true);
-    const double scale = 1.0 / static_cast<double>(N);
-    for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
-        const int bb = t & (B - 1), e = t >> LB;
-        const int64_t line = line0 + bb;
-        if (line >= p.nlines) continue;
-        const int64_t at = line_node(p, line, e);
-        double2 v = X[L::at(bb, e)];
-        v.x *= scale;
-        v.y *= scale;
-        fa[at] = v;
-        if (write_b) {
-            double2 w = Bb[L::at(bb, e)];
-            w.x *= scale;
-            w.y *= scale;
-            fb[at] = w;
+        for (int q = 0; q < 4; ++q) {
+            const int t = threadIdx.x + q * 256;
+            if (t >= B * N) continue;
+            const int bb = t >> LOG2N, e = t & (N - 1);
+            // lines past the end (rh zero) give zero fields
+            const int64_t node = line_node(p, line0 + bb, e);
+            const int ix = static_cast<int>(node >> sp);
+            const int iy = static_cast<int>(node & ((int64_t{1} << sy) - 1));
+            const int iz = static_cast<int>((node & ((int64_t{1} << sp) - 1)) >> sy);
+            const ModeFields mf = green_split(g, ix, iy, iz, rh[q]);
+            const int at = L::at(bb, bitrev(e, LOG2N));
+            X[at] = FA == 0 ? mf.ex : FA == 1 ? mf.ey : FA == 2 ? mf.ez : mf.ph;
+            if constexpr (FB >= 0)
+                X[B * L::LS + at] = FB == 0 ? mf.ex : FB == 1 ? mf.ey : FB == 2 ? mf.ez : mf.ph;
         }
-    }
+        __syncthreads();
+        constexpr int NF = FB >= 0 ? 2 : 1;
+        fft_lines<LOG2N>(X, NF * B, s_tw, true);
+        const double scale = 1.0 / static_cast<double>(N);
+        for (int t = threadIdx.x; t < NF * B * N; t += blockDim.x) {
+            const int f = t >= B * N ? 1 : 0;
+            const int tt = t - f * B * N;
+            const int bb = tt & (B - 1), e = tt >> LB;
+            const int64_t line = line0 + bb;
+            if (line >= p.nlines) continue;
+            double2 v = X[L::at(f * B + bb, e)];
+            v.x *= scale;
+            v.y *= scale;
+            (f ? fb : fa)[line_node(p, line, e)] = v;
+        }
+    };
+    pair.template operator()<0, 1>(arr.h[0], arr.h[1]);
+    if (want_ez && g.want_phi)
+        pair.template operator()<2, 3>(arr.h[2], arr.h[3]);
+    else if (want_ez)
+        pair.template operator()<2, -1>(arr.h[2], nullptr);
+    else if (g.want_phi)
+        pair.template operator()<3, -1>(arr.h[3], nullptr);
 }
 
-// inverse transform along x of fa (and fb), scaled 1/N, unpacked as k_unpack
-// (E64 may be null: only the FP32 field is written)
+// One x line's full spectrum of (a + i b) from the half spectra a_k, b_k
+// (k = 0 .. N/2) of two real fields: C_k = a_k + i b_k and C_{N-k} =
+// conj(a_k) + i conj(b_k); at k = 0 and N/2 (their own conjugates) the real
+// parts.  Stored at the bit-reversed positions of line b.
 template <int LOG2N>
-__global__ void __launch_bounds__(256) k_inv_x_unpack(const double2* __restrict__ fa,
-                                                      const double2* __restrict__ fb,
-                                                      int64_t nlines, int B,
-                                                      const double2* __restrict__ tw,
-                                                      double* E64, double* phi, float* E32,
-                                                      int dim, int want_phi, int64_t nn) {
+__device__ __forceinline__ void put_pair(double2* lds, int b, int k, double2 a, double2 v) {
     using L = Lines<LOG2N>;
     constexpr int N = 1 << LOG2N;
-    __shared__ double2 A[kLastLds];
-    __shared__ double2 Bb[kLastLds];
+    if (k == 0 || k == N / 2) {
+        lds[L::at(b, bitrev(k, LOG2N))] = make_double2(a.x, v.x);
+        return;
+    }
+    lds[L::at(b, bitrev(k, LOG2N))] = make_double2(a.x - v.y, a.y + v.x);
+    lds[L::at(b, bitrev(N - k, LOG2N))] = make_double2(a.x + v.y, v.x - a.y);
+}
+
+// inverse transform along x: (Ex + i Ey) from H0, H1 and (Ez + i phi) from
+// H2, H3 (an absent half reads as zero), scaled 1/N, unpacked as k_unpack
+// (E64 may be null: only the FP32 field is written)
+template <int LOG2N>
+__global__ void __launch_bounds__(256) k_inv_x_c2r(Halves arr, int64_t nlines, int LB,
+                                                   const double2* __restrict__ tw,
+                                                   double* E64, double* phi, float* E32,
+                                                   int dim, int want_phi, int64_t nn) {
+    using L = Lines<LOG2N>;
+    constexpr int N = 1 << LOG2N, NH = N / 2 + 1;
+    __shared__ double2 A[kLds<LOG2N>];
+    __shared__ double2 Bb[kLds<LOG2N>];
     __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
     stage_twiddles<LOG2N>(tw, s_tw);
     const bool use_b = dim == 3 || want_phi;
+    const int B = 1 << LB;
     const int64_t line0 = static_cast<int64_t>(blockIdx.x) * B;
     const int lines = static_cast<int>(nlines - line0 < B ? nlines - line0 : B);
-    for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
-        const int b = t >> LOG2N, e = t & (N - 1);
-        const int64_t src = (line0 + b) * N + bitrev(e, LOG2N);
-        A[L::at(b, e)] = fa[src];
-        if (use_b) Bb[L::at(b, e)] = fb[src];
+    // every load issued before the first LDS store (B * NH <= 576: at most
+    // 3 elements per thread and half)
+    constexpr int kPer = 3;
+    const double2 zero = make_double2(0.0, 0.0);
+    double2 va[kPer], vb[kPer], vc[kPer], vd[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int t = threadIdx.x + u * kFftThreads;
+        const int b = t & (B - 1), k = t >> LB;
+        const bool ok = t < B * NH && b < lines;
+        const int64_t src = k * nlines + line0 + b;
+        va[u] = ok ? arr.h[0][src] : zero;
+        vb[u] = ok ? arr.h[1][src] : zero;
+        vc[u] = ok && dim == 3 ? arr.h[2][src] : zero;
+        vd[u] = ok && want_phi ? arr.h[3][src] : zero;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int t = threadIdx.x + u * kFftThreads;
+        const int b = t & (B - 1), k = t >> LB;
+        if (t >= B * NH || b >= lines) continue;
+        put_pair<LOG2N>(A, b, k, va[u], vb[u]);
+        if (use_b) put_pair<LOG2N>(Bb, b, k, vc[u], vd[u]);
     }
     __syncthreads();
     fft_lines<LOG2N>(A, lines, s_tw, true);
@@ -634,26 +756,23 @@ int ilog2(int64_t n) {
     return l;
 }
 
-// lines per block as a power of two: the LDS budget over the padded line
-// length (n + n/8 + 1), at most 64
-int lines_shift(int64_t budget, int64_t n) {
-    const int64_t ls = n + n / 8 + 1;
-    int l = 0;
-    while ((int64_t{2} << l) * ls <= budget && l < 6) ++l;
-    return l;
+
+// the half layout's x frequencies (nx/2 + 1) times P = ny * nz nodes
+int64_t half_nodes(const Context& c) {
+    const int64_t nz = c.dim > 2 ? c.n[2] : 1;
+    return (c.n[0] / 2 + 1) * c.n[1] * nz;
 }
 
 // from_acc: the first transform reads the species accumulators (the step's
 // solve; rho is not written) instead of rho
 void solve_fused(Context& c, const GreenArgs& ga, bool want_phi, bool want_e64,
                  bool from_acc = false) {
-    const int last = c.dim - 1;
-    const bool use_b = c.dim == 3 || want_phi;
-    const int64_t nx = c.n[0];
-    const int64_t xlines = c.nnodes / nx;
+    const int64_t nx = c.n[0], ny = c.n[1], nz = c.dim > 2 ? c.n[2] : 1;
+    const int64_t P = ny * nz, NH = nx / 2 + 1, half = half_nodes(c);
+    Halves H{{c.fa.ptr, c.fa.ptr + half, c.fb.ptr, c.fb.ptr + half}};
     {
-        const int LB = lines_shift(kFftLds, nx);
-        const unsigned grid = static_cast<unsigned>((xlines + (1 << LB) - 1) >> LB);
+        const int LB = lines_shift(kLastLds, nx);
+        const unsigned grid = static_cast<unsigned>((P + (1 << LB) - 1) >> LB);
         KernelScope ks(c, "fft_fwd_x");
         if (from_acc) {
             const RhoArgs ra = rho_args(c);
@@ -665,60 +784,74 @@ void solve_fused(Context& c, const GreenArgs& ga, bool want_phi, bool want_e64,
                 cl.p[s] = reinterpret_cast<long long*>(sp.acc_next.ptr);
             }
             with_log2(ilog2(nx), [&]<int L>() {
-                k_fwd_x_acc<L><<<grid, 256, 0, c.stream>>>(ra, cl, c.fa.ptr, xlines, 1 << LB,
-                                                           c.tw[0].ptr);
+                k_fwd_x_acc<L><<<grid, 256, 0, c.stream>>>(ra, cl, H.h[0], P, LB, c.tw[0].ptr);
             });
             for (auto& sp : c.species) sp.acc_next_clear = true;
         } else {
             with_log2(ilog2(nx), [&]<int L>() {
-                k_fwd_x_real<L><<<grid, 256, 0, c.stream>>>(c.rho.ptr, c.fa.ptr, xlines, 1 << LB,
+                k_fwd_x_real<L><<<grid, 256, 0, c.stream>>>(c.rho.ptr, H.h[0], P, LB,
                                                             c.tw[0].ptr);
             });
         }
         AUR_HIP(hipGetLastError());
     }
-    for (int a = 1; a < last; ++a) {
-        const AxisPlan p = plan_axis(c, a);
-        const int LB = lines_shift(kFftLds, p.n);
-        const unsigned grid = static_cast<unsigned>((p.nlines + (1 << LB) - 1) >> LB);
+    // y lines of the half layout (3D): NH * nz contiguous lines of ny
+    const int64_t rows = NH * nz;
+    const int LBy = lines_shift(kLastLds, ny);
+    const unsigned grid_y = static_cast<unsigned>((rows + (1 << LBy) - 1) >> LBy);
+    if (c.dim == 3) {
         KernelScope ks(c, "fft_axis");
-        with_log2(p.log2n, [&]<int L>() {
-            k_axis_lines<L><<<dim3(grid, 1), 256, 0, c.stream>>>(c.fa.ptr, c.fb.ptr, p, LB,
-                                                                 c.tw[a].ptr, 0);
+        with_log2(ilog2(ny), [&]<int L>() {
+            k_rows<L><<<dim3(grid_y, 1), 256, 0, c.stream>>>(H, rows, LBy, c.tw[1].ptr, 0);
         });
         AUR_HIP(hipGetLastError());
     }
     {
-        const AxisPlan p = plan_axis(c, last);
+        // last axis: z lines (kx, y) at stride ny in 3D, y lines (kx) in 2D
+        const int last = c.dim - 1;
+        AxisPlan p{};
+        p.n = static_cast<int>(c.n[last]);
+        p.log2n = ilog2(p.n);
+        p.pow2 = 1;
+        if (c.dim == 3) {
+            p.elem_stride = ny;
+            p.inner_count = ny;
+            p.outer_stride = P;
+            p.nlines = NH * ny;
+            p.inner_shift = ilog2(ny);
+        } else {
+            p.elem_stride = 1;
+            p.inner_count = 1;
+            p.outer_stride = ny;
+            p.nlines = NH;
+            p.inner_shift = 0;
+        }
         const int LB = lines_shift(kLastLds, p.n);
         const unsigned grid = static_cast<unsigned>((p.nlines + (1 << LB) - 1) >> LB);
         KernelScope ks(c, "fft_last");
         with_log2(p.log2n, [&]<int L>() {
-            k_last<L><<<grid, 256, 0, c.stream>>>(c.fa.ptr, c.fb.ptr, p, LB, c.tw[last].ptr, ga,
-                                                  use_b ? 1 : 0);
+            k_last<L><<<grid, 256, 0, c.stream>>>(H, p, LB, c.tw[last].ptr, ga, ilog2(ny),
+                                                  ilog2(P), c.dim == 3 ? 1 : 0);
         });
         AUR_HIP(hipGetLastError());
     }
-    for (int a = last - 1; a >= 1; --a) {
-        const AxisPlan p = plan_axis(c, a);
-        const int LB = lines_shift(kFftLds, p.n);
-        const unsigned grid = static_cast<unsigned>((p.nlines + (1 << LB) - 1) >> LB);
+    if (c.dim == 3) {
+        // inverse y of Ex, Ey, Ez (and phi): the wanted halves, packed first
+        Halves inv{{H.h[0], H.h[1], H.h[2], H.h[3]}};
+        const unsigned narr = want_phi ? 4u : 3u;
         KernelScope ks(c, "fft_axis");
-        with_log2(p.log2n, [&]<int L>() {
-            k_axis_lines<L><<<dim3(grid, use_b ? 2 : 1), 256, 0, c.stream>>>(
-                c.fa.ptr, c.fb.ptr, p, LB, c.tw[a].ptr, 1);
+        with_log2(ilog2(ny), [&]<int L>() {
+            k_rows<L><<<dim3(grid_y, narr), 256, 0, c.stream>>>(inv, rows, LBy, c.tw[1].ptr, 1);
         });
         AUR_HIP(hipGetLastError());
     }
     {
         const int LB = lines_shift(kLastLds, nx);
-        const int B = 1 << LB;
-        const unsigned grid = static_cast<unsigned>((xlines + B - 1) / B);
+        const unsigned grid = static_cast<unsigned>((P + (1 << LB) - 1) >> LB);
         KernelScope ks(c, "fft_inv_x");
         with_log2(ilog2(nx), [&]<int L>() {
-            k_inv_x_unpack<L><<<grid, 256, 0, c.stream>>>(
-                c.fa.ptr, c.fb.ptr, xlines, B, c.tw[0].ptr, want_e64 ? c.E64.ptr : nullptr,
-                c.phi.ptr, c.E32.ptr,
+            k_inv_x_c2r<L><<<grid, 256, 0, c.stream>>>(
+                H, P, LB, c.tw[0].ptr, want_e64 ? c.E64.ptr : nullptr, c.phi.ptr, c.E32.ptr,
                 c.dim, want_phi ? 1 : 0, c.nnodes);
         });
         AUR_HIP(hipGetLastError());
@@ -779,8 +912,10 @@ void prepare_solver(Context& c) {
             c.tw[a].alloc(n);
             AUR_HIP(hipMemcpy(c.tw[a].ptr, tw.data(), n * sizeof(double2), hipMemcpyHostToDevice));
         }
-        c.fa.alloc(c.nnodes);
-        c.fb.alloc(c.nnodes);
+        // the power-of-two path keeps two half spectra in each buffer
+        const int64_t nbuf = fused_fft_ok(c) ? std::max(c.nnodes, 2 * half_nodes(c)) : c.nnodes;
+        c.fa.alloc(nbuf);
+        c.fb.alloc(nbuf);
     }
 }
 
