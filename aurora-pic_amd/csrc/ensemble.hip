@@ -198,11 +198,11 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_
     __shared__ double mean_s;
     const int inst = blockIdx.x;
     const int64_t o = static_cast<int64_t>(inst) * a.nx;
-    RhoArgs ra = a.rho_args;
+    // the instance's sums read in place (a modified copy of the argument's
+    // pointer array would live in scratch memory)
     long long* acc = a.acc + static_cast<int64_t>(inst) * a.S * a.nx;
-    for (int s = 0; s < a.S; ++s) ra.acc[s] = acc + static_cast<int64_t>(s) * a.nx;
     for (int i = threadIdx.x; i < a.nx; i += blockDim.x) {
-        a.rho[o + i] = rho_at(ra, i);
+        a.rho[o + i] = rho_at_strided(a.rho_args, acc, a.nx, i);
         for (int s = 0; s < a.S; ++s) acc[static_cast<int64_t>(s) * a.nx + i] = 0;
     }
     __syncthreads();

@@ -26,8 +26,7 @@ RhoArgs rho_args(const Context& c);
 // Charge density of node i from the per-species fixed-point sums, divided by
 // the node volume (Grid.cpp:26-30, Mesh2D.cpp:77-88, Mesh3D.cpp:52-60), in
 // species order.
-__device__ __forceinline__ double rho_at(const RhoArgs& a, int64_t i) {
-#pragma clang fp contract(off)
+__device__ __forceinline__ double node_volume(const RhoArgs& a, int64_t i) {
     double vol = a.vol;
     if (!a.periodic) {
         // Grid.cpp:26-30, Mesh2D.cpp:77-88, Mesh3D.cpp:52-60
@@ -42,9 +41,27 @@ __device__ __forceinline__ double rho_at(const RhoArgs& a, int64_t i) {
             else vol = wx * wy * ((iz == 0 || iz + 1 == a.nz) ? 0.5 * a.dz : a.dz);
         }
     }
+    return vol;
+}
+
+__device__ __forceinline__ double rho_at(const RhoArgs& a, int64_t i) {
+#pragma clang fp contract(off)
+    const double vol = node_volume(a, i);
     double r = 0.0;
     for (int s = 0; s < a.nspecies; ++s)
         r += (static_cast<double>(a.acc[s][i]) * a.cw[s]) / vol;
+    return r;
+}
+
+// rho_at with species s's sums at acc + s * stride (the batched 1D
+// instances) instead of a.acc[s]: the same operations in the same order
+__device__ __forceinline__ double rho_at_strided(const RhoArgs& a, const long long* acc,
+                                                 int64_t stride, int64_t i) {
+#pragma clang fp contract(off)
+    const double vol = node_volume(a, i);
+    double r = 0.0;
+    for (int s = 0; s < a.nspecies; ++s)
+        r += (static_cast<double>(acc[s * stride + i]) * a.cw[s]) / vol;
     return r;
 }
 

@@ -69,8 +69,10 @@ struct PermArrays {
 
 constexpr int kPermUnroll = 4;
 
+// 256 threads (the launch below): the FP64 variant's unrolled gathers need
+// more registers than the default 1024-thread bound allows
 template <typename R>
-__global__ void k_permute(const unsigned* __restrict__ perm, int64_t n,
+__global__ void __launch_bounds__(256) k_permute(const unsigned* __restrict__ perm, int64_t n,
                           const int32_t* __restrict__ cell, int32_t* __restrict__ cell2,
                           const uint32_t* __restrict__ uid, uint32_t* __restrict__ uid2,
                           PermArrays<R> a) {

@@ -484,9 +484,13 @@ __global__ void __launch_bounds__(kBr) k_br_collide(MccParams m, BranchArgs a) {
                 err |= DEV_ERR_PRODUCTS;
                 ns = 2;
             }
-            for (int k = 0; k < ns; ++k)
+            // both slots unrolled: a runtime index into t.sec would put the
+            // tally in scratch memory
 #pragma unroll
-                for (int d = 0; d < 3; ++d) a.sec[6 * p + 3 * k + d] = t.sec[k][d];
+            for (int k = 0; k < 2; ++k)
+                if (k < ns)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) a.sec[6 * p + 3 * k + d] = t.sec[k][d];
             a.v[p] = v[0];
             a.v[a.N + p] = v[1];
             a.v[2 * a.N + p] = v[2];
