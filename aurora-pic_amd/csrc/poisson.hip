@@ -307,11 +307,11 @@ void run_axis(Context& c, double2* data, int axis, int inverse) {
 // Twiddles are staged in LDS; all index arithmetic is shifts and masks.
 
 // The fused kernels run 256 threads; a block stages at most kLastLds
-// elements per buffer, so a thread loads at most kPerLast of them.  The
-// forward kernels, the y lines and the last axis issue all of a thread's
-// global loads into registers before the first LDS store, so the loads'
-// latencies overlap instead of adding up (the x inverse, which reads two or
-// four arrays, keeps its loop).
+// elements per buffer, so a thread loads at most kPerLast of them.  Every
+// fused kernel issues all of a thread's global loads into registers before
+// the first LDS store, so the loads' latencies overlap instead of adding up;
+// the x and y lines are read in order (coalesced) and bit-reversed on the
+// way into LDS.
 constexpr int kFftThreads = 256;
 // complex elements per LDS buffer of the fused kernels (20 KiB): lines of
 // N take N + N/8 + 1 elements, so a block holds B lines with B * N <= 1024
