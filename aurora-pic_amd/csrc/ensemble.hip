@@ -193,7 +193,8 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_push(EnsArgs a) {
 
 // rho from the fixed-point sums (clearing them), then the field solve at
 // solve index (*step + solve_offset): one workgroup per instance
-__global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_offset) {
+__global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_offset,
+                                                           int thomas_lds) {
     extern __shared__ double sh[];
     __shared__ double mean_s;
     const int inst = blockIdx.x;
@@ -213,7 +214,7 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_
                           sh, &mean_s);
     else
         thomas_solve_1d(a.rho + o, a.phi + o, a.E64 + o, a.E32 + o, a.cp, a.denom, a.dp + o, a.nx,
-                        a.dx, a.eps, pv[0], pv[1]);
+                        a.dx, a.eps, pv[0], pv[1], thomas_lds ? sh : nullptr);
 }
 
 // MCC on the target species: one workgroup per 256 target slots of one
@@ -630,9 +631,10 @@ CompactArgs compact_args(Ensemble& e, int s) {
 
 // Queue one step (or the compaction) of every instance on the stream.
 void enqueue_step(Context& c, Ensemble& e, const EnsArgs& a, const MccParams& m) {
-    const size_t shmem = a.periodic ? 3 * static_cast<size_t>(a.nx) * sizeof(double) : 0;
+    const size_t thomas = a.periodic ? 0 : thomas_lds_doubles(a.nx);
+    const size_t shmem = (a.periodic ? 3 * static_cast<size_t>(a.nx) : thomas) * sizeof(double);
     k_ens_push<true><<<push_grid(a), kEnsThreads, 0, c.stream>>>(a);
-    k_ens_field<<<a.M, kEnsThreads, shmem, c.stream>>>(a, 1);
+    k_ens_field<<<a.M, kEnsThreads, shmem, c.stream>>>(a, 1, thomas > 0 ? 1 : 0);
     if (a.target >= 0) {
         const int g = a.M * a.mchunks;
         with_nchan(m.nchan, [&]<int NCH>() {

@@ -130,11 +130,50 @@ __device__ inline void spectral_solve_1d(const double* rho, double* phi, double*
 // lane, then the field is differenced in parallel.
 __device__ inline void thomas_solve_1d(const double* rho, double* phi, double* E, float* E32,
                                 const double* cp, const double* denom, double* dp, int n,
-                                double dx, double eps, double phi_left, double phi_right) {
+                                double dx, double eps, double phi_left, double phi_right,
+                                double* sh = nullptr) {
 #pragma clang fp contract(off)
-    if (threadIdx.x == 0) {
-        const int m = n - 2;
-        const double dx2 = dx * dx;
+    const int m = n - 2;
+    const double dx2 = dx * dx;
+    double* ph = phi;
+    if (sh) {
+        // sh (6 n doubles of LDS): the right-hand sides d_i (all lanes, the
+        // same operations as the serial form), cp, the denominators, dp and
+        // phi, so that thread 0's two sweeps wait on LDS only and their
+        // loads run ahead of the division chain
+        double* ds = sh;
+        double* cs = sh + n;
+        double* dn = sh + 2 * n;
+        double* dpv = sh + 3 * n;
+        ph = sh + 4 * n;
+        for (int i = threadIdx.x; i < m; i += blockDim.x) {
+            double di = rho[i + 1] * dx2 / eps;
+            if (i == 0) di += phi_left;
+            if (i == m - 1) di += phi_right;
+            ds[i] = di;
+            cs[i] = cp[i];
+            dn[i] = denom[i];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            ph[0] = phi_left;
+            ph[n - 1] = phi_right;
+            double prev = ds[0] / 2.0;
+            dpv[0] = prev;
+#pragma unroll 8
+            for (int i = 1; i < m; ++i) {
+                prev = (ds[i] + prev) / dn[i];
+                dpv[i] = prev;
+            }
+            double nxt = prev;
+            ph[n - 2] = nxt;
+#pragma unroll 8
+            for (int ii = m - 2; ii >= 0; --ii) {
+                nxt = dpv[ii] - cs[ii] * nxt;
+                ph[ii + 1] = nxt;
+            }
+        }
+    } else if (threadIdx.x == 0) {
         phi[0] = phi_left;
         phi[n - 1] = phi_right;
         double prev = 0.0;
@@ -155,12 +194,19 @@ __device__ inline void thomas_solve_1d(const double* rho, double* phi, double* E
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         double e;
-        if (i == 0) e = -(phi[1] - phi[0]) / dx;
-        else if (i == n - 1) e = -(phi[n - 1] - phi[n - 2]) / dx;
-        else e = -(phi[i + 1] - phi[i - 1]) / (2.0 * dx);
+        if (i == 0) e = -(ph[1] - ph[0]) / dx;
+        else if (i == n - 1) e = -(ph[n - 1] - ph[n - 2]) / dx;
+        else e = -(ph[i + 1] - ph[i - 1]) / (2.0 * dx);
         E[i] = e;
         E32[i] = static_cast<float>(e);
+        if (sh) phi[i] = ph[i];
     }
+}
+
+// LDS doubles thomas_solve_1d stages into (0: too large, global memory)
+inline size_t thomas_lds_doubles(int n) {
+    const size_t d = 6 * static_cast<size_t>(n);
+    return d * sizeof(double) <= 48 * 1024 ? d : 0;
 }
 
 }  // namespace aurora

@@ -247,8 +247,10 @@ __global__ void __launch_bounds__(256) k_thomas_1d(const double* rho, double* ph
                                                    float* E32, const double* cp,
                                                    const double* denom, double* dp, int n,
                                                    double dx, double eps, double phi_left,
-                                                   double phi_right) {
-    thomas_solve_1d(rho, phi, E, E32, cp, denom, dp, n, dx, eps, phi_left, phi_right);
+                                                   double phi_right, int use_lds) {
+    extern __shared__ double sh[];
+    thomas_solve_1d(rho, phi, E, E32, cp, denom, dp, n, dx, eps, phi_left, phi_right,
+                    use_lds ? sh : nullptr);
 }
 
 AxisPlan plan_axis(const Context& c, int axis) {
@@ -956,10 +958,10 @@ void solve_fields(Context& c, double phi_left, double phi_right, bool want_phi, 
         } else {
             KernelScope ks(c, "solve_thomas_1d");
             const int m = n - 2;
-            k_thomas_1d<<<1, 256, 0, c.stream>>>(c.rho.ptr, c.phi.ptr, c.E64.ptr, c.E32.ptr,
-                                                 c.thomas_cp.ptr, c.thomas_inv.ptr,
-                                                 c.thomas_inv.ptr + m, n, c.d[0], c.eps,
-                                                 phi_left, phi_right);
+            const size_t lds = thomas_lds_doubles(n);
+            k_thomas_1d<<<1, 256, lds * sizeof(double), c.stream>>>(
+                c.rho.ptr, c.phi.ptr, c.E64.ptr, c.E32.ptr, c.thomas_cp.ptr, c.thomas_inv.ptr,
+                c.thomas_inv.ptr + m, n, c.d[0], c.eps, phi_left, phi_right, lds > 0 ? 1 : 0);
         }
         AUR_HIP(hipGetLastError());
         c.phi_valid = true;
