@@ -68,15 +68,39 @@ struct WorkItem {
 };
 
 template <int NCH>
-__global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc* __restrict__ descs,
+__global__ void __launch_bounds__(kChunk, 5) k_swarm(MccParams m, const RequestDesc* __restrict__ descs,
                                                   const WorkItem* __restrict__ items, int nitems,
                                                   int* head) {
     __shared__ MccLds lds;
     __shared__ int item_s;
-    __shared__ long long red_v[kChunk];
-    __shared__ long long red_e[kChunk];
-    __shared__ double red_m[5][kChunk];
+    // small reduction buffers (LDS limits how many blocks share a CU): one
+    // slot per wave for the integer sums (exact in any order), half a chunk
+    // for the cross-wave steps of the FP64 moment tree
+    __shared__ long long red_w[2][kChunk / 64];
+    __shared__ double red_m[5][kChunk / 2];
     const XsView X = stage_tables(m, lds);
+    // the chunk's integer sums of acc_v and acc_e added to q[0], q[1]
+    auto add_chunk_sums = [&](long long av, long long ae, long long* q) {
+        for (int o = 32; o > 0; o >>= 1) {
+            av += __shfl_down(av, o, 64);
+            ae += __shfl_down(ae, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            red_w[0][threadIdx.x >> 6] = av;
+            red_w[1][threadIdx.x >> 6] = ae;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long sv = 0, se = 0;
+            for (int wv = 0; wv < kChunk / 64; ++wv) {
+                sv += red_w[0][wv];
+                se += red_w[1][wv];
+            }
+            q[0] += sv;
+            q[1] += se;
+        }
+        __syncthreads();
+    };
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(head, 1);
         __syncthreads();
@@ -137,22 +161,8 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
                     group = static_cast<int>(((k + 1) * d.blocks - 1) / d.sampling);
                     group_end = (static_cast<int64_t>(group) + 1) * d.sampling / d.blocks;
                 } else if (k >= group_end) {
-                    red_v[threadIdx.x] = acc_v;
-                    red_e[threadIdx.x] = acc_e;
-                    __syncthreads();
-                    for (int st = kChunk / 2; st > 0; st >>= 1) {
-                        if (threadIdx.x < st) {
-                            red_v[threadIdx.x] += red_v[threadIdx.x + st];
-                            red_e[threadIdx.x] += red_e[threadIdx.x + st];
-                        }
-                        __syncthreads();
-                    }
-                    if (threadIdx.x == 0) {
-                        long long* q = d.part + (static_cast<int64_t>(w.chunk) * d.blocks + group) * 2;
-                        q[0] += red_v[0];
-                        q[1] += red_e[0];
-                    }
-                    __syncthreads();
+                    add_chunk_sums(acc_v, acc_e,
+                                   d.part + (static_cast<int64_t>(w.chunk) * d.blocks + group) * 2);
                     acc_v = 0;
                     acc_e = 0;
                     group += 1;
@@ -165,43 +175,35 @@ __global__ void __launch_bounds__(kChunk) k_swarm(MccParams m, const RequestDesc
                 }
             }
         }
-        if (group >= 0) {
-            red_v[threadIdx.x] = acc_v;
-            red_e[threadIdx.x] = acc_e;
-            __syncthreads();
-            for (int st = kChunk / 2; st > 0; st >>= 1) {
-                if (threadIdx.x < st) {
-                    red_v[threadIdx.x] += red_v[threadIdx.x + st];
-                    red_e[threadIdx.x] += red_e[threadIdx.x + st];
-                }
-                __syncthreads();
-            }
-            if (threadIdx.x == 0) {
-                long long* q = d.part + (static_cast<int64_t>(w.chunk) * d.blocks + group) * 2;
-                q[0] += red_v[0];
-                q[1] += red_e[0];
-            }
-            __syncthreads();
-        }
+        if (group >= 0)
+            add_chunk_sums(acc_v, acc_e,
+                           d.part + (static_cast<int64_t>(w.chunk) * d.blocks + group) * 2);
         const bool last = d.step0 + d.nsteps == d.burn_in + d.sampling;
         if (last) {
             // displacement moments of the chunk, reduced in a fixed order
             const double m0 = active ? static_cast<double>(x[0]) : 0.0;
             const double m1 = active ? static_cast<double>(x[1]) : 0.0;
             const double m2 = active ? static_cast<double>(x[2]) : 0.0;
-            red_m[0][threadIdx.x] = m0;
-            red_m[1][threadIdx.x] = m1;
-            red_m[2][threadIdx.x] = m2;
-            red_m[3][threadIdx.x] = m0 * m0;
-            red_m[4][threadIdx.x] = m1 * m1 + m2 * m2;
-            __syncthreads();
-            for (int st = kChunk / 2; st > 0; st >>= 1) {
+            double mv[5] = {m0, m1, m2, m0 * m0, m1 * m1 + m2 * m2};
+            // the pairwise tree: at stride st, element i < st adds element
+            // i + st; strides of a wave or more through LDS, the rest by
+            // shuffles within the first wave (the same pairs and sums)
+            for (int st = kChunk / 2; st >= 64; st >>= 1) {
+                if (threadIdx.x >= st && threadIdx.x < 2 * st)
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) red_m[k][threadIdx.x - st] = mv[k];
+                __syncthreads();
                 if (threadIdx.x < st)
-                    for (int k = 0; k < 5; ++k) red_m[k][threadIdx.x] += red_m[k][threadIdx.x + st];
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) mv[k] += red_m[k][threadIdx.x];
                 __syncthreads();
             }
-            if (threadIdx.x < 5) d.moments[static_cast<int64_t>(w.chunk) * 5 + threadIdx.x] =
-                red_m[threadIdx.x][0];
+            for (int st = 32; st > 0; st >>= 1)
+#pragma unroll
+                for (int k = 0; k < 5; ++k) mv[k] += __shfl_down(mv[k], st, 64);
+            if (threadIdx.x == 0)
+#pragma unroll
+                for (int k = 0; k < 5; ++k) d.moments[static_cast<int64_t>(w.chunk) * 5 + k] = mv[k];
             __syncthreads();
         } else if (active && d.step0 + d.nsteps > d.burn_in) {
             d.dx[p] = x[0];
