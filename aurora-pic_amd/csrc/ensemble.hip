@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_
 // MCC on the target species: one workgroup per 256 target slots of one
 // instance; product pairs are staged per parent with in-chunk offsets.
 template <int NCH>
-__global__ void __launch_bounds__(kEnsThreads) k_ens_mcc(EnsArgs a, MccParams m) {
+__global__ void __launch_bounds__(kEnsThreads, 5) k_ens_mcc(EnsArgs a, MccParams m) {
     __shared__ MccLds lds;
     __shared__ int scan_sh[kEnsThreads];
     const XsView X = stage_tables(m, lds);

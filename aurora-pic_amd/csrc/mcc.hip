@@ -31,7 +31,7 @@ constexpr int kBlk = 256;
 
 // standalone collide of count velocities (parity checks): uid = index
 template <int NCH>
-__global__ void __launch_bounds__(kBlk) k_collide(MccParams m, float* v, int64_t count,
+__global__ void __launch_bounds__(kBlk, 5) k_collide(MccParams m, float* v, int64_t count,
                                                   float dt, uint64_t seed, uint32_t stream,
                                                   uint32_t step, TallyOut out, uint32_t* err) {
     __shared__ MccLds lds;
