@@ -81,6 +81,7 @@ struct EnsArgs {
     int64_t* step;                  // device step counter (all instances advance together)
     unsigned long long* absorbed;   // [M][S][2]
     double* absorbed_energy;        // [M][S][2]
+    double* wall_rows;              // [push workgroup][2]: the workgroup's wall energies
     uint32_t* err;                  // [M]
     // MCC
     int target, sec, ion;
@@ -91,6 +92,8 @@ struct EnsArgs {
     int32_t* chunk_tot;             // [M][mchunks]
     int32_t* chunk_off;
     int mchunks;
+    int mgroups;                    // MCC workgroups per instance (each walks chunks mgroups apart)
+    double* energy_rows;            // [M][mgroups * waves][kMaxChannels]: per-wave energy change
     int64_t* base;                  // [M][kBaseFields]
     unsigned long long* counts;     // [M][kMaxChannels + 3]
     double* ech;                    // [M][kMaxChannels]
@@ -127,7 +130,10 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_push(EnsArgs a) {
     const int chunk = rel - inst * sp.push_chunks;
     const int64_t n = a.slots[s * a.M + inst];
     const int64_t lo = static_cast<int64_t>(chunk) * kEnsPushChunk;
-    if (lo >= n) return;
+    if (lo >= n) {
+        if (PUSH && threadIdx.x < 2) a.wall_rows[2 * wg + threadIdx.x] = 0.0;
+        return;
+    }
     const int64_t hi = lo + kEnsPushChunk < n ? lo + kEnsPushChunk : n;
     const bool lds = a.nx <= kEnsLdsNodes;
     long long* gacc = a.acc + (static_cast<int64_t>(inst) * a.S + s) * a.nx;
@@ -137,6 +143,7 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_push(EnsArgs a) {
     }
     const float* E32 = a.E32 + static_cast<int64_t>(inst) * a.nx;
     const int64_t first = static_cast<int64_t>(inst) * sp.cap;
+    double wall[2] = {0.0, 0.0};   // this thread's represented energy at each wall
     for (int64_t q = lo + threadIdx.x; q < hi; q += blockDim.x) {
         const int64_t p = first + q;
         int c = sp.cell[p];
@@ -161,7 +168,7 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_push(EnsArgs a) {
                 const double e = sp.half_mw * (static_cast<double>(v[0]) * v[0] +
                                                static_cast<double>(v[1]) * v[1] +
                                                static_cast<double>(v[2]) * v[2]);
-                atomicAdd(a.absorbed_energy + k, e);
+                if (hit) wall[1] += e; else wall[0] += e;
                 continue;
             }
             sp.cell[p] = c;
@@ -189,6 +196,16 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_push(EnsArgs a) {
                 atomicAdd(reinterpret_cast<unsigned long long*>(gacc + i),
                           static_cast<unsigned long long>(lacc[i]));
     }
+    if constexpr (PUSH) {
+        // the workgroup's wall energies in a fixed order (threads' sums,
+        // waves in wave order), folded per instance by k_ens_field
+        if (__syncthreads_or(wall[0] != 0.0 || wall[1] != 0.0)) {
+            const double t = block_sum<2, kEnsThreads>(wall);
+            if (threadIdx.x < 2) a.wall_rows[2 * wg + threadIdx.x] = t;
+        } else if (threadIdx.x < 2) {
+            a.wall_rows[2 * wg + threadIdx.x] = 0.0;
+        }
+    }
 }
 
 // rho from the fixed-point sums (clearing them), then the field solve at
@@ -199,6 +216,19 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_
     __shared__ double mean_s;
     const int inst = blockIdx.x;
     const int64_t o = static_cast<int64_t>(inst) * a.nx;
+    // wall energies of this step's push, workgroup rows added in chunk order
+    for (int s = 0; s < a.S; ++s) {
+        const int ch = a.sp[s].push_chunks;
+        const double* rows = a.wall_rows + 2 * (a.sp[s].wg_begin + static_cast<int64_t>(inst) * ch);
+        double w[2] = {0.0, 0.0};
+        for (int k = threadIdx.x; k < ch; k += blockDim.x) {
+            w[0] += rows[2 * k];
+            w[1] += rows[2 * k + 1];
+        }
+        const double t = block_sum<2, kEnsThreads>(w);
+        if (threadIdx.x < 2 && t != 0.0)
+            a.absorbed_energy[(static_cast<int64_t>(inst) * a.S + s) * 2 + threadIdx.x] += t;
+    }
     // the instance's sums read in place (a modified copy of the argument's
     // pointer array would live in scratch memory)
     long long* acc = a.acc + static_cast<int64_t>(inst) * a.S * a.nx;
@@ -217,52 +247,77 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_
                         a.dx, a.eps, pv[0], pv[1], thomas_lds ? sh : nullptr);
 }
 
-// MCC on the target species: one workgroup per 256 target slots of one
-// instance; product pairs are staged per parent with in-chunk offsets.
+// MCC on the target species.  mgroups workgroups per instance walk its
+// 256-slot chunks mgroups apart, so the cross-section tables are staged in
+// LDS once per workgroup; product pairs are staged per parent with in-chunk
+// offsets, and the energy change goes to one row per wave (summed in a
+// fixed order by k_ens_products).
 template <int NCH>
 __global__ void __launch_bounds__(kEnsThreads, 5) k_ens_mcc(EnsArgs a, MccParams m) {
     __shared__ MccLds lds;
     __shared__ int scan_sh[kEnsThreads];
     const XsView X = stage_tables(m, lds);
-    const int inst = blockIdx.x / a.mchunks;
-    const int chunk = blockIdx.x - inst * a.mchunks;
+    const int inst = blockIdx.x / a.mgroups;
+    const int group = blockIdx.x - inst * a.mgroups;
     const EnsSpecies& sp = a.sp[a.target];
     const int64_t n = a.slots[a.target * a.M + inst];
-    const int64_t lo = static_cast<int64_t>(chunk) * kEnsMccChunk;
-    if (lo >= n) {
-        if (threadIdx.x == 0) a.chunk_tot[blockIdx.x] = 0;
-        return;
+    const float* E32 = a.E32 + static_cast<int64_t>(inst) * a.nx;
+    unsigned long long* counts = a.counts + static_cast<int64_t>(inst) * (kMaxChannels + 3);
+    double ewave[NCH] = {};
+    for (int chunk = group; chunk < a.mchunks; chunk += a.mgroups) {
+        const int64_t lo = static_cast<int64_t>(chunk) * kEnsMccChunk;
+        const int64_t ci = static_cast<int64_t>(inst) * a.mchunks + chunk;
+        if (lo >= n) {
+            if (threadIdx.x == 0) a.chunk_tot[ci] = 0;
+            continue;
+        }
+        const int64_t q = lo + threadIdx.x;
+        const int64_t p = static_cast<int64_t>(inst) * sp.cap + q;
+        const bool in = q < n;
+        const int c = in ? sp.cell[p] : -1;
+        const bool active = c >= 0;
+        MccTally<NCH> t{};
+        int np = 0;
+        if (active)
+            np = mcc_1d_particle<NCH>(m, X, lds.chan, c, sp.off[p], sp.v0 + p, sp.v1 + p,
+                                      sp.v2 + p, sp.uid[p], E32, a.nx, sp.half_qm_dt, a.dt,
+                                      a.seeds[inst], static_cast<uint32_t>(*a.step),
+                                      a.prod + 12 * p, t, a.err + inst);
+        flush_tally_rows<NCH>(t, counts, active, ewave);
+        int total = 0;
+        const int ex = block_exclusive_scan(np, scan_sh, total);
+        if (in) {
+            a.nprod[p] = np;
+            a.loff[p] = ex;
+        }
+        if (threadIdx.x == 0) a.chunk_tot[ci] = total;
     }
-    const int64_t q = lo + threadIdx.x;
-    const int64_t p = static_cast<int64_t>(inst) * sp.cap + q;
-    const bool in = q < n;
-    const int c = in ? sp.cell[p] : -1;
-    const bool active = c >= 0;
-    MccTally<NCH> t{};
-    int np = 0;
-    if (active)
-        np = mcc_1d_particle<NCH>(m, X, lds.chan, c, sp.off[p], sp.v0 + p, sp.v1 + p, sp.v2 + p, sp.uid[p],
-                             a.E32 + static_cast<int64_t>(inst) * a.nx, a.nx, sp.half_qm_dt,
-                             a.dt, a.seeds[inst], static_cast<uint32_t>(*a.step),
-                             a.prod + 12 * p, t, a.err + inst);
-    flush_tally<NCH>(t,
-                TallyOut{a.counts + static_cast<int64_t>(inst) * (kMaxChannels + 3),
-                         a.ech + static_cast<int64_t>(inst) * kMaxChannels},
-                active);
-    int total = 0;
-    const int ex = block_exclusive_scan(np, scan_sh, total);
-    if (in) {
-        a.nprod[p] = np;
-        a.loff[p] = ex;
+    if ((threadIdx.x & 63) == 0) {
+        const int64_t w = (static_cast<int64_t>(inst) * a.mgroups + group) * (kEnsThreads / 64) +
+                          (threadIdx.x >> 6);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) a.energy_rows[w * kMaxChannels + c] = ewave[c];
     }
-    if (threadIdx.x == 0) a.chunk_tot[blockIdx.x] = total;
 }
 
 // Per instance: chunk offsets, all-or-nothing capacity check
 // (Simulation.cpp:929-964), slot and uid reservation.
+template <int NCH>
 __global__ void __launch_bounds__(kEnsThreads) k_ens_products(EnsArgs a) {
     __shared__ int scan_sh[kEnsThreads];
     const int inst = blockIdx.x;
+    {
+        // this step's MCC energy change: the instance's wave rows, in order
+        const int64_t nrows = static_cast<int64_t>(a.mgroups) * (kEnsThreads / 64);
+        const double* rows = a.energy_rows + static_cast<int64_t>(inst) * nrows * kMaxChannels;
+        double v[NCH] = {};
+        for (int64_t r = threadIdx.x; r < nrows; r += kEnsThreads)
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) v[c] += rows[r * kMaxChannels + c];
+        const double t = block_sum<NCH, kEnsThreads>(v);
+        if (threadIdx.x < NCH) a.ech[static_cast<int64_t>(inst) * kMaxChannels + threadIdx.x] += t;
+    }
+    if (a.sec < 0) return;
     const int per = (a.mchunks + kEnsThreads - 1) / kEnsThreads;
     const int c0 = threadIdx.x * per;
     const int32_t* tot = a.chunk_tot + static_cast<int64_t>(inst) * a.mchunks;
@@ -472,11 +527,14 @@ struct Ensemble {
     DevBuf<int64_t> step;
     DevBuf<unsigned long long> absorbed;
     DevBuf<double> absorbed_energy;
+    DevBuf<double> wall_rows;
     DevBuf<uint32_t> err;
     DevBuf<uint64_t> seeds;
     DevBuf<int32_t> nprod, loff;
     DevBuf<float> prod;
     DevBuf<int32_t> chunk_tot, chunk_off;
+    int mgroups = 0;
+    DevBuf<double> energy_rows;
     DevBuf<int64_t> base;
     DevBuf<unsigned long long> counts;
     DevBuf<double> ech;
@@ -577,6 +635,7 @@ EnsArgs make_args(Context& c, Ensemble& e, double dt) {
     a.step = e.step.ptr;
     a.absorbed = e.absorbed.ptr;
     a.absorbed_energy = e.absorbed_energy.ptr;
+    a.wall_rows = e.wall_rows.ptr;
     a.err = e.err.ptr;
     a.target = c.mcc.loaded ? c.mcc.target : -1;
     a.sec = c.mcc.secondary_species;
@@ -590,6 +649,8 @@ EnsArgs make_args(Context& c, Ensemble& e, double dt) {
     a.mchunks = a.target >= 0
                     ? static_cast<int>((e.sp[a.target].cap + kEnsMccChunk - 1) / kEnsMccChunk)
                     : 0;
+    a.mgroups = e.mgroups;
+    a.energy_rows = e.energy_rows.ptr;
     a.base = e.base.ptr;
     a.counts = e.counts.ptr;
     a.ech = e.ech.ptr;
@@ -638,12 +699,10 @@ void enqueue_step(Context& c, Ensemble& e, const EnsArgs& a, const MccParams& m)
     if (a.target >= 0) {
         const int g = a.M * a.mchunks;
         with_nchan(m.nchan, [&]<int NCH>() {
-            k_ens_mcc<NCH><<<g, kEnsThreads, 0, c.stream>>>(a, m);
+            k_ens_mcc<NCH><<<a.M * a.mgroups, kEnsThreads, 0, c.stream>>>(a, m);
+            k_ens_products<NCH><<<a.M, kEnsThreads, 0, c.stream>>>(a);
         });
-        if (a.sec >= 0) {
-            k_ens_products<<<a.M, kEnsThreads, 0, c.stream>>>(a);
-            k_ens_append<<<g, kEnsThreads, 0, c.stream>>>(a);
-        }
+        if (a.sec >= 0) k_ens_append<<<g, kEnsThreads, 0, c.stream>>>(a);
     }
     k_ens_tick<<<1, 64, 0, c.stream>>>(e.step.ptr);
 }
@@ -795,6 +854,10 @@ int aurora_hip_ensemble_create(aurora_hip_ctx* h, int64_t instances, const int64
         e->step.alloc(1);
         e->absorbed.alloc(static_cast<size_t>(M * S * 2));
         e->absorbed_energy.alloc(static_cast<size_t>(M * S * 2));
+        int64_t push_wgs = 0;
+        for (int s = 0; s < S; ++s)
+            push_wgs += M * ((e->sp[s].cap + kEnsPushChunk - 1) / kEnsPushChunk);
+        e->wall_rows.alloc(static_cast<size_t>(2 * push_wgs));
         e->err.alloc(static_cast<size_t>(M));
         e->seeds.alloc(static_cast<size_t>(M));
         AUR_HIP(hipMemsetAsync(e->absorbed.ptr, 0, M * S * 2 * 8, c.stream));
@@ -809,6 +872,11 @@ int aurora_hip_ensemble_create(aurora_hip_ctx* h, int64_t instances, const int64
             e->prod.alloc(static_cast<size_t>(12 * n));
             e->chunk_tot.alloc(static_cast<size_t>(M * mch));
             e->chunk_off.alloc(static_cast<size_t>(M * mch));
+            // about two rounds of resident MCC workgroups (five per CU)
+            const int64_t want = static_cast<int64_t>(c.cu_count) * 10;
+            e->mgroups = static_cast<int>(std::clamp<int64_t>((want + M - 1) / M, 1, mch));
+            e->energy_rows.alloc(static_cast<size_t>(M * e->mgroups * (kEnsThreads / 64) *
+                                                     kMaxChannels));
             e->base.alloc(static_cast<size_t>(M * kBaseFields));
             AUR_HIP(hipMemsetAsync(e->base.ptr, 0, M * kBaseFields * 8, c.stream));
         }
@@ -1021,9 +1089,12 @@ int aurora_hip_ensemble_stats(aurora_hip_ctx* h, int64_t handle, int64_t instanc
         if (counts)
             AUR_HIP(hipMemcpy(counts, e.counts.ptr + instance * (kMaxChannels + 3),
                               (nch + 3) * 8, hipMemcpyDeviceToHost));
-        if (energy_change && nch)
+        if (energy_change && nch) {
             AUR_HIP(hipMemcpy(energy_change, e.ech.ptr + instance * kMaxChannels, nch * 8,
                               hipMemcpyDeviceToHost));
+            const double w = e.sp[c.mcc.target].weight;
+            for (int k = 0; k < nch; ++k) energy_change[k] *= w;
+        }
         if (absorbed)
             AUR_HIP(hipMemcpy(absorbed, e.absorbed.ptr + instance * e.S * 2, e.S * 2 * 8,
                               hipMemcpyDeviceToHost));

@@ -39,9 +39,10 @@ def _alone(hip, gas, k, mcc_seed, steps):
     c.step(DT, steps)
     out = [c.download(s) for s in sids]
     fields = c.fields()
-    counts, _ = c.mcc_stats()
+    counts, ech = c.mcc_stats()
+    walls = [c.losses(s) for s in sids]
     c.close()
-    return out, fields, counts
+    return out, fields, counts, ech, walls
 
 
 def _by_uid(pos, vel, alive, uid):
@@ -69,7 +70,7 @@ def test_instances_equal_single_context_runs():
     t, s = ens.time()
     assert s == steps
     for k in (0, 3):
-        ref, ref_fields, ref_counts = _alone(hip, gas, k, 500 + k, steps)
+        ref, ref_fields, ref_counts, ref_ech, ref_walls = _alone(hip, gas, k, 500 + k, steps)
         for sid in sids:
             a = _by_uid(*ens.download(k, sid))
             b = _by_uid(*ref[sid])
@@ -79,11 +80,39 @@ def test_instances_equal_single_context_runs():
             assert np.array_equal(a[2], b[2])
         rho, phi, E = ens.fields(k)
         assert np.array_equal(rho, ref_fields[0])
+        assert np.array_equal(phi, ref_fields[1])
         assert np.array_equal(E, ref_fields[2][0])
-        counts, _, _, _ = ens.stats(k, 3)
+        counts, ech, absorbed, absorbed_e = ens.stats(k, 3)
         assert np.array_equal(counts, ref_counts)
         assert counts[4] > 0          # ionizations happened: products were placed
+        # FP64 ledgers: the same terms summed in another fixed order (the lone
+        # context keeps dead slots in place, the ensemble compacts them), so
+        # they agree to rounding: 1e-12 relative
+        assert np.allclose(ech, ref_ech, rtol=1e-12, atol=0.0), (ech, ref_ech)
+        assert ech[1] < 0 and ech[2] < 0   # excitation and ionization take energy
+        for sid in sids:
+            ref_n, ref_e = ref_walls[sid]
+            assert np.array_equal(absorbed[sid], ref_n[:2].astype(np.uint64))
+            assert np.allclose(absorbed_e[sid], ref_e, rtol=1e-12, atol=0.0), (absorbed_e, ref_e)
+        assert absorbed[0].sum() > 0 and absorbed_e[0].sum() > 0   # electrons reached the walls
     c.close()
+
+
+def test_ensemble_ledgers_repeat_to_the_bit():
+    # every FP64 tally of an instance is summed in a fixed order: two
+    # ensembles of the same instances give identical ledgers
+    from aurora_pic import hip
+    gas = workloads.synthetic_gas(rows=1000, neutral_density=1.0e22)
+    runs = []
+    for _ in range(2):
+        c, sids, ens = _build_ensemble(hip, gas, 3, 25)
+        ens.step(DT, 60)
+        runs.append([ens.stats(k, 3) for k in range(3)])
+        c.close()
+    for a, b in zip(*runs):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+        assert a[3][0].sum() > 0
 
 
 def test_graph_replays_equal_direct_launches():
