@@ -514,6 +514,115 @@ def effective_population_limit(cfg: SwarmConfig) -> int:
     return min(capacity, max(doubled, buffered))
 
 
+_TOWNSEND = 1.0e-21                       # V m^2 per Td
+_EV = 1.602176634e-19
+
+
+def _exceeds_product_limit(first: int, second: int, limit: int) -> bool:
+    return first != 0 and second > limit // first
+
+
+def validate_swarm_config(cfg: SwarmConfig, ds: GasDataset) -> None:
+    """validate_config (Swarm.cpp:277-559): the reference's checks of a
+    swarm scan against its gas dataset, in its order and with its messages,
+    including the work_item_limit product check (fields x particles x steps,
+    Swarm.cpp:336-359) and the population_limit rules (Swarm.cpp:317-335).
+    Raises GasDataError."""
+    def finite_pos(v):
+        return math.isfinite(v) and v > 0.0
+
+    if ds.format_version != 2 or ds.units != "si":
+        raise GasDataError("swarm benchmarks require a version-2 SI gas dataset")
+    if not finite_pos(cfg.neutral_density):
+        raise GasDataError("swarm neutral_density must be positive and finite")
+    if not finite_pos(cfg.max_frequency):
+        raise GasDataError("swarm max_frequency must be positive and finite")
+    if not finite_pos(cfg.timestep):
+        raise GasDataError("swarm timestep must be positive and finite")
+    if cfg.max_frequency * cfg.timestep > 0.1:
+        raise GasDataError("swarm max_frequency * timestep must not exceed 0.1")
+    if cfg.steps == 0 or cfg.burn_in_steps >= cfg.steps:
+        raise GasDataError("swarm steps must exceed burn_in_steps")
+    if cfg.particles == 0:
+        raise GasDataError("swarm particles must be positive")
+    if cfg.particles > SWARM_ALLOCATION_LIMIT:
+        raise GasDataError("swarm particles exceeds the safety limit of 10000000")
+    limit = effective_population_limit(cfg)
+    if cfg.population_model == "fixed_population_no_avalanche":
+        if cfg.population_limit != 0:
+            raise GasDataError("swarm population_limit is valid only for "
+                               "population_model = branching_resampled")
+    elif limit <= cfg.particles or limit > SWARM_ALLOCATION_LIMIT - cfg.particles:
+        raise GasDataError("branching swarm population_limit must exceed particles and "
+                           "population_limit + particles must not exceed the allocation "
+                           "safety limit of 10000000")
+    if cfg.work_item_limit == 0:
+        raise GasDataError("swarm work_item_limit must be positive")
+    nf = len(cfg.reduced_fields_td)
+    if (_exceeds_product_limit(nf, cfg.particles, cfg.work_item_limit)
+            or _exceeds_product_limit(nf * cfg.particles, cfg.steps, cfg.work_item_limit)):
+        raise GasDataError("swarm scan exceeds work_item_limit; raise it explicitly only on "
+                           "an appropriate compute host")
+    sampling = cfg.steps - cfg.burn_in_steps
+    if cfg.uncertainty_blocks == 0 or cfg.uncertainty_blocks > sampling:
+        raise GasDataError("swarm uncertainty_blocks must be between 1 and the number of "
+                           "sampling steps")
+    if sampling % cfg.uncertainty_blocks != 0:
+        raise GasDataError("swarm sampling steps must be divisible by uncertainty_blocks")
+    branching = cfg.population_model == "branching_resampled"
+    if branching and sampling // cfg.uncertainty_blocks < 2:
+        raise GasDataError("branching swarm requires at least two sampling steps per "
+                           "uncertainty block")
+    if cfg.spatial_histories == 0:
+        if (cfg.spatial_length_m != 0.0 or cfg.spatial_bins != 0
+                or cfg.spatial_fit_begin_bin != 0 or cfg.spatial_fit_end_bin != 0
+                or cfg.spatial_max_steps != 0 or cfg.spatial_min_r_squared != 0.0):
+            raise GasDataError("spatial Townsend controls require positive spatial_histories")
+    else:
+        # the spatial Townsend experiment is not built (Swarm.cpp:645-938)
+        raise GasDataError("spatial Townsend estimation (spatial_histories > 0) is not "
+                           "supported by this build")
+    if not math.isfinite(cfg.initial_mean_energy_ev) or cfg.initial_mean_energy_ev < 0.0:
+        raise GasDataError("swarm initial_mean_energy_ev must be finite and non-negative")
+    if not finite_pos(cfg.max_energy_ev):
+        raise GasDataError("swarm max_energy_ev must be positive and finite")
+    if not math.isfinite(cfg.neutral_temperature) or cfg.neutral_temperature < 0.0:
+        raise GasDataError("swarm neutral_temperature must be finite and non-negative")
+    if not cfg.reduced_fields_td:
+        raise GasDataError("swarm requires at least one reduced field")
+    for f in cfg.reduced_fields_td:
+        if not math.isfinite(f) or f < 0.0:
+            raise GasDataError("swarm reduced fields must be finite and non-negative")
+        if not math.isfinite(f * _TOWNSEND * cfg.neutral_density):
+            raise GasDataError("swarm reduced field and neutral density overflow the "
+                               "electric field")
+    if len(set(cfg.reduced_fields_td)) != nf:
+        raise GasDataError("swarm reduced fields must not contain duplicates")
+    if not ds.channels:
+        raise GasDataError("swarm gas dataset has no collision channels")
+    kinds = set()
+    for ch in ds.channels:
+        kinds.add(ch.process)
+        if ch.process == CHARGE_EXCHANGE:
+            raise GasDataError("electron swarm benchmark does not support charge-exchange "
+                               "channels")
+        if ch.energy_frame != FRAME_PROJECTILE:
+            raise GasDataError("electron swarm benchmark requires projectile-frame "
+                               "cross-section energy")
+        if cfg.max_energy_ev > ch.energy[-1] / _EV:
+            raise GasDataError(f"swarm max_energy_ev exceeds channel '{ch.name}' table coverage")
+        if ch.angular == HENYEY_GREENSTEIN and cfg.max_energy_ev > ch.mean_cosine_energy[-1] / _EV:
+            raise GasDataError(f"swarm max_energy_ev exceeds channel '{ch.name}' angular "
+                               "table coverage")
+    if ELASTIC not in kinds:
+        raise GasDataError("electron swarm benchmark requires an elastic channel")
+    if not branching and ATTACHMENT in kinds:
+        raise GasDataError("fixed-population swarm does not support attachment; use "
+                           "population_model = branching_resampled")
+    if branching and IONIZATION not in kinds and ATTACHMENT not in kinds:
+        raise GasDataError("branching swarm requires an ionization or attachment channel")
+
+
 def swarm_requests(cfg: SwarmConfig):
     """One device request per reduced field, seeded as
     run_swarm_benchmark seeds its fields (config seed + field index,

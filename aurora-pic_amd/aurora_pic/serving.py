@@ -36,6 +36,8 @@ class Served:
     admit_s: float = float("nan")
     done_s: float = float("nan")
     result: object = None
+    refusals: int = 0          # admission attempts refused for lack of HBM budget
+    rejected: str = ""         # why the request was never admitted (empty when served)
 
     @property
     def latency_s(self):
@@ -53,7 +55,10 @@ class ServingReport:
     timeline: list = field(default_factory=list)
 
     def latencies(self):
-        return np.array([s.latency_s for s in self.served])
+        return np.array([s.latency_s for s in self.served if not s.rejected])
+
+    def rejected(self):
+        return [s for s in self.served if s.rejected]
 
     def summary(self):
         lat = self.latencies()
@@ -68,6 +73,7 @@ class ServingReport:
             "launches": self.launches,
             "max_active": self.max_active,
             "budget_refusals": self.refused_for_budget,
+            "rejected": len(self.rejected()),
         }
 
 
@@ -88,12 +94,34 @@ def make_requests(cfg, count=None, steps=None, burn_in=None, particle_scale=1.0)
     return out
 
 
+WORK_ITEM_LIMIT = 100_000_000      # SwarmBenchmarkConfig::work_item_limit default (Swarm.hpp:42)
+
+
+def work_items_exceeded(request, work_item_limit=WORK_ITEM_LIMIT):
+    """The reference's work-item gate for one field point (Swarm.cpp:340-359:
+    fields x particles x steps, here 1 x particles x steps): the message it
+    raises when the request exceeds work_item_limit, else ""."""
+    if work_item_limit <= 0:
+        return "swarm work_item_limit must be positive"
+    if request.particles > 0 and request.steps > work_item_limit // request.particles:
+        return ("swarm scan exceeds work_item_limit; raise it explicitly only on an "
+                "appropriate compute host")
+    return ""
+
+
 def serve(ctx, requests, arrivals_s, hbm_budget, steps_per_launch=1000, clock=time.perf_counter,
-          on_launch=None):
+          on_launch=None, work_item_limit=WORK_ITEM_LIMIT):
     """Run the request loop until every request has a result.
 
     ctx: hip.Context with the swarm gas loaded.  arrivals_s: arrival time of
     each request relative to the start of the loop (non-decreasing).
+    A request whose own footprint exceeds hbm_budget can never be admitted:
+    it is rejected on arrival (Served.rejected says why) instead of waiting.
+    Admission is first come, first served: a request waiting for memory
+    holds back the requests that arrived after it.  A request above
+    work_item_limit (particles x steps) is rejected with the reference's
+    message; the default is the reference's 1e8, and a GPU deployment raises
+    it explicitly.
     on_launch: optional callable run after every swarm launch (e.g. to
     interleave other device work).  Returns a ServingReport."""
     order = np.argsort(np.asarray(arrivals_s), kind="stable")
@@ -107,7 +135,19 @@ def serve(ctx, requests, arrivals_s, hbm_budget, steps_per_launch=1000, clock=ti
     while pending or queued or active:
         now = clock() - t0
         while pending and pending[0].arrival_s <= now:
-            queued.append(pending.popleft())
+            s = pending.popleft()
+            fp = ctx.swarm_footprint(s.request)
+            s.rejected = work_items_exceeded(s.request, work_item_limit)
+            if s.rejected:
+                s.done_s = now
+                served.append(s)
+                continue
+            if hbm_budget and fp > hbm_budget:
+                s.rejected = f"footprint {fp} B exceeds the HBM budget of {hbm_budget} B"
+                s.done_s = now
+                served.append(s)
+                continue
+            queued.append(s)
         while queued:
             s = queued[0]
             try:
@@ -115,7 +155,11 @@ def serve(ctx, requests, arrivals_s, hbm_budget, steps_per_launch=1000, clock=ti
             except hip.HipError as e:
                 if e.code != hip.ERR_CAPACITY:
                     raise
+                if not active:
+                    # nothing holds memory, yet the request does not fit
+                    raise RuntimeError(f"request {s.index} refused with no request running: {e}")
                 refused += 1
+                s.refusals += 1
                 break
             s.admit_s = clock() - t0
             active[h] = s

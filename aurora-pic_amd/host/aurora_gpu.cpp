@@ -5,13 +5,30 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <random>
 #include <utility>
 
 namespace pic::gpu {
 
-namespace {
+bool exceeds_local_run_limit(const std::vector<std::size_t>& particles, std::size_t steps) {
+    std::size_t total = 0;
+    for (std::size_t n : particles) {
+        if (n > std::numeric_limits<std::size_t>::max() - total) return true;
+        total += n;
+    }
+    return steps != 0 && total > local_particle_update_limit / steps;
+}
 
+void require_large_run_acknowledgement(bool exceeds_limit, bool acknowledged) {
+    if (exceeds_limit && !acknowledged)
+        throw std::runtime_error(
+            "estimated initial particle updates exceed the conservative 100,000,000-update CLI "
+            "limit; inspect with --validate-only, or deliberately launch with --allow-large-run "
+            "I_UNDERSTAND_THIS_IS_A_LARGE_RUN");
+}
+
+namespace {
 
 std::shared_ptr<Device> or_new(std::shared_ptr<Device> d) {
     return d ? std::move(d) : std::make_shared<Device>(0);
@@ -562,6 +579,10 @@ void Simulation::step() {
 }
 
 RunSummary Simulation::run() {
+    std::vector<std::size_t> counts;
+    for (const auto& sc : cfg_.species) counts.push_back(sc.particles);
+    require_large_run_acknowledgement(exceeds_local_run_limit(counts, cfg_.steps),
+                                      cfg_.allow_large_run == large_run_acknowledgement);
     if (!initialized_) initialize();
     auto step_to = [&](size_t k) {
         dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, static_cast<int64_t>(k - step_)));
@@ -813,6 +834,10 @@ void Simulation2D::step() {
 }
 
 RunSummary Simulation2D::run() {
+    std::vector<std::size_t> counts;
+    for (const auto& sc : cfg_.species) counts.push_back(sc.particles);
+    require_large_run_acknowledgement(exceeds_local_run_limit(counts, cfg_.steps),
+                                      cfg_.allow_large_run == large_run_acknowledgement);
     if (!initialized_) initialize();
     auto step_to = [&](size_t k) {
         dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, static_cast<int64_t>(k - step_)));
@@ -1006,6 +1031,10 @@ void Simulation3D::step() {
 }
 
 RunSummary Simulation3D::run() {
+    std::vector<std::size_t> counts;
+    for (const auto& sc : cfg_.species) counts.push_back(sc.particles);
+    require_large_run_acknowledgement(exceeds_local_run_limit(counts, cfg_.steps),
+                                      cfg_.allow_large_run == large_run_acknowledgement);
     if (!initialized_) initialize();
     auto step_to = [&](size_t k) {
         dev_->check(aurora_hip_step(dev_->get(), cfg_.dt, static_cast<int64_t>(k - step_)));

@@ -29,11 +29,22 @@
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <string_view>
 #include <vector>
 
 #include "aurora_hip.h"
 
 namespace pic::gpu {
+
+// The reference CLI's local-run guard (main.cpp:16-46): a run whose initial
+// particles x steps exceed 1e8 updates is refused unless it is launched
+// with this acknowledgement.  run() applies it to its configuration; the
+// configs' allow_large_run field carries the acknowledgement, which is how
+// a GPU deployment lifts the limit explicitly.
+inline constexpr std::string_view large_run_acknowledgement = "I_UNDERSTAND_THIS_IS_A_LARGE_RUN";
+inline constexpr std::size_t local_particle_update_limit = 100'000'000;
+bool exceeds_local_run_limit(const std::vector<std::size_t>& particles, std::size_t steps);
+void require_large_run_acknowledgement(bool exceeds_limit, bool acknowledged);
 
 enum class Boundary { Periodic, Dirichlet };                      // Types.hpp:31
 enum class ParticleBoundary { Auto, Absorbing, Reflecting, Periodic };  // Types.hpp:33
@@ -350,6 +361,7 @@ struct Simulation3DConfig {       // Simulation3D.hpp:26-66, the fields the loop
     // run() writes scalars.csv there every output_interval steps (empty: none)
     std::filesystem::path output_dir{};
     std::size_t output_interval{10};
+    std::string allow_large_run{};    // large_run_acknowledgement lifts the 1e8-update guard
 };
 
 struct Simulation2DConfig {       // Simulation2D.hpp, structured periodic/Dirichlet
@@ -369,6 +381,7 @@ struct Simulation2DConfig {       // Simulation2D.hpp, structured periodic/Diric
     std::optional<std::uint64_t> initial_state_signature{};
     std::filesystem::path output_dir{};
     std::size_t output_interval{10};
+    std::string allow_large_run{};    // large_run_acknowledgement lifts the 1e8-update guard
 };
 
 struct SinusoidalVoltageConfig {  // Config.hpp:139-143
@@ -426,6 +439,7 @@ struct Config {                   // Config.hpp:170-203, the fields the loop use
     // boundary_losses.csv and power_transfer.csv there (empty: no output)
     std::filesystem::path output_dir{};
     std::size_t output_interval{10};
+    std::string allow_large_run{};    // large_run_acknowledgement lifts the 1e8-update guard
 };
 
 struct BoundaryLoss1D {           // Simulation.hpp:20-25 (counts and impact energy)

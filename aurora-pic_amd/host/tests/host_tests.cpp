@@ -14,6 +14,7 @@
 //   aurora_host_tests --run2d <case.txt> <output dir> <32|64>   and for 2D
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstdlib>
 #include <filesystem>
@@ -530,6 +531,21 @@ void aps_errors(const std::filesystem::path&) {
            "1D1V external particle state has nonzero inactive components");
 }
 
+// main.cpp:16-46: the 1e8-update local-run guard and its message
+void large_run_guard(const std::filesystem::path&) {
+    require(!exceeds_local_run_limit({50'000'000, 50'000'000}, 1), "exactly 1e8 updates");
+    require(exceeds_local_run_limit({50'000'000, 50'000'001}, 1), "1e8 + 1 updates");
+    require(exceeds_local_run_limit({1'000'000}, 101), "1e6 particles x 101 steps");
+    require(!exceeds_local_run_limit({1'000'000}, 0), "zero steps");
+    require(exceeds_local_run_limit({std::numeric_limits<std::size_t>::max(), 1}, 1),
+            "particle count overflow");
+    require_throws_contains([] { require_large_run_acknowledgement(true, false); },
+                            "exceed the conservative 100,000,000-update CLI limit",
+                            "unacknowledged large run");
+    require_large_run_acknowledgement(true, true);
+    require_large_run_acknowledgement(false, false);
+}
+
 struct CpuCase {
     const char* name;
     void (*body)(const std::filesystem::path&);
@@ -540,6 +556,7 @@ const CpuCase kCpuCases[] = {
     {"aps_round_trip", aps_round_trip},
     {"aps_errors", aps_errors},
     {"ledger_csv_writers", ledger_csv_writers},
+    {"large_run_guard", large_run_guard},
 };
 
 // The 3D simulation initialised from a state file equals the same records
@@ -715,6 +732,34 @@ void simulation1d_power_transfer(const Dev& d) {
     std::filesystem::remove_all(dir);
 }
 
+// run() applies the CLI guard to its configuration: 2 x 600,000 particles
+// x 100 steps is 1.2e8 updates
+void simulation3d_large_run_guard(const Dev& d) {
+    Simulation3DConfig cfg;
+    cfg.nx = cfg.ny = cfg.nz = 8;
+    cfg.steps = 100;
+    cfg.dt = 0.01;
+    for (int k = 0; k < 2; ++k) {
+        Species3DConfig sc;
+        sc.name = k ? "ions" : "electrons";
+        sc.charge = k ? 1.0 : -1.0;
+        sc.mass = k ? 100.0 : 1.0;
+        sc.particles = 600'000;
+        sc.weight = 1.0 / 600'000;
+        cfg.species.push_back(sc);
+    }
+    Simulation3D sim(cfg, d);
+    require_throws_contains([&] { sim.run(); }, "--allow-large-run I_UNDERSTAND_THIS_IS_A_LARGE_RUN",
+                            "large run launched without acknowledgement");
+    require(sim.step_count() == 0, "a refused run stepped");
+    cfg.allow_large_run = std::string(large_run_acknowledgement);
+    Simulation3D lifted(cfg, d);
+    lifted.initialize();
+    for (int k = 0; k < 98; ++k) lifted.step();   // the run continues from step 98
+    const auto r = lifted.run();
+    require(r.steps_completed == 100, "acknowledged large run did not complete");
+}
+
 struct Case {
     const char* name;
     void (*body)(const Dev&);
@@ -736,6 +781,7 @@ const Case kCases[] = {
     {"simulation3d_external_state", simulation3d_external_state},
     {"simulation1d_wall_ledgers", simulation1d_wall_ledgers},
     {"simulation1d_power_transfer", simulation1d_power_transfer},
+    {"simulation3d_large_run_guard", simulation3d_large_run_guard},
 };
 
 }  // namespace

@@ -44,31 +44,59 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def host_threads():
+    """Threads the CPU baseline may use: the process's CPU share
+    (OMP_NUM_THREADS when set, else the affinity mask)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+# Time of the restatement's step over the reference's own Simulation3D::step,
+# measured in the build container (8 x Intel Xeon @ 2.70 GHz, one thread,
+# 128^3 mesh, 2+2 particles per cell, same tabulated B): 0.895.
+PORT_OVER_REFERENCE_TIME = 0.895
+
+
 def cpu_baseline(cfg, ppc_sample: int):
-    """Reference Simulation3D::step() (compiled from the reference sources into
-    oracle/_ref) on the same mesh with fewer particles per cell, one step."""
+    """The oracle's FP64 restatement of Simulation3D::step (or_step3d: push,
+    deposit and FFT solve serial as in the reference, the synchronisation
+    loop on OpenMP threads as the reference's runtime_parallel_for) on the
+    same mesh and field with fewer particles per cell: one step on one
+    thread, then one on every thread of the host share."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_py
     from aurora_pic import workloads as wl
 
+    orc = oracle_py.Oracle()
     cells = cfg["n"][0]
     sample = wl.config_plasma_3d(cells=cells, ppc=ppc_sample)
-    sp = []
-    for s in sample["species"]:
-        sp += [s["charge"], s["mass"], s["weight"], s["particles"],
-               wl.maxwellian_sigma(s["temperature_ev"], s["mass"])]
+    sp = [(s["charge"], s["mass"], s["weight"], s["particles"],
+           wl.maxwellian_sigma(s["temperature_ev"], s["mass"])) for s in sample["species"]]
     ax, bx, bv = sample["btable"]
-    desc = (f"reference Simulation3D::step() (oracle/_ref, -O3, 1 thread) on the "
-            f"{cells}^3 mesh with {ppc_sample}+{ppc_sample} particles per cell "
-            f"({2 * sample['species'][0]['particles']} particles), tabulated B_y(x), 1 step "
-            f"after initialize(); host {cpu_model()}")
-    if oracle_py.reference_available():
-        ref = oracle_py.Reference()
-        sec, live = ref.sim3d_time(sample["n"], sample["L"], sample["dt"], 1.0, 1, sp,
-                                   (bx, bv), 1)
-        return {"value": live / sec, "unit": "particle-updates/s", "cores": 1,
-                "kind": "reference", "sample": desc}
-    return None
+    st = orc.plasma3d_init(sample["n"], sample["L"], sample["eps"], sample["dt"], sp,
+                           btable=(ax, bx, bv), seed=sample["seed"])
+    threads = host_threads()
+    rates = {}
+    for th in (1, threads):
+        t0 = time.perf_counter()
+        live = orc.step3d(st, th)
+        rates[th] = live / (time.perf_counter() - t0)
+        log(f"cpu baseline: {th} thread(s) {rates[th]:.4g} particle-updates/s")
+    total = sum(s[3] for s in sp)
+    return {"value": rates[threads], "unit": "particle-updates/s", "cores": threads,
+            "kind": "port", "single_thread_value": rates[1],
+            "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "threads": threads},
+            "port_over_reference_time": PORT_OVER_REFERENCE_TIME,
+            "sample": f"FP64 restatement of Simulation3D::step (oracle/pic_oracle.c or_step3d, "
+                      f"-O3) on the {cells}^3 mesh with {ppc_sample}+{ppc_sample} particles per "
+                      f"cell ({total} particles), tabulated B_y(x), one step on 1 thread and one "
+                      f"on {threads} threads (OpenMP on the synchronisation loop, where the "
+                      f"reference parallelises); host {cpu_model()}, nproc {os.cpu_count()}"}
 
 
 def serving_leg(args):
@@ -98,8 +126,12 @@ def serving_leg(args):
     ctx = hip.Context(0)
     ctx.swarm_gas(cfg["gas"])
     budget = args.serving_hbm_gib << 30
+    # the reference's 1e8 work-item gate (Swarm.cpp:340-359), raised
+    # explicitly for the GPU: a 1e6-electron request of 20,000 steps is 2e10
+    work_items = 10 ** 12
     # capacity: every request present at t = 0, advanced together
-    sat = serving.serve(ctx, reqs, np.zeros(R), budget, steps_per_launch=args.serving_launch_steps)
+    sat = serving.serve(ctx, reqs, np.zeros(R), budget, steps_per_launch=args.serving_launch_steps,
+                        work_item_limit=work_items)
     # the same requests one at a time, each alone on the GPU
     seq_results, seq_wall, seq_steps = serving.run_sequential(ctx, reqs)
     log(f"capacity: {R} requests in {sat.wall_s:.3f} s batched, {seq_wall:.3f} s sequential")
@@ -107,7 +139,8 @@ def serving_leg(args):
         f"{args.serving_hbm_gib} GiB) + {args.ensemble_instances} discharge instances")
     t0 = time.perf_counter()
     rep = serving.serve(ctx, reqs, cfg["arrival_s"][:R], budget,
-                        steps_per_launch=args.serving_launch_steps, on_launch=ensemble_tick)
+                        steps_per_launch=args.serving_launch_steps, on_launch=ensemble_tick,
+                        work_item_limit=work_items)
     wall = time.perf_counter() - t0
     summ = rep.summary()
     ens_updates = ens_steps[0] * args.ensemble_instances * args.ensemble_particles
@@ -155,7 +188,7 @@ def main():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--ppc", type=int, default=16)
     ap.add_argument("--timing-steps", type=int, default=20)
-    ap.add_argument("--cpu-ppc", type=int, default=4)
+    ap.add_argument("--cpu-ppc", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--sort", choices=("adaptive", "fixed"), default="adaptive",
@@ -282,7 +315,7 @@ def main():
 
     base = None
     if not args.no_cpu_baseline:
-        log("cpu baseline (reference, bounded sample)...")
+        log("cpu baseline (FP64 restatement, bounded sample)...")
         try:
             base = cpu_baseline(cfg, args.cpu_ppc)
         except Exception as e:  # the baseline must not hide the GPU result

@@ -287,3 +287,67 @@ def test_model_signature_tracks_physics():
     cfg.neutral_temperature = 300.0
     ds.channels[0].cross_section = ds.channels[0].cross_section * (1 + 1e-15)
     assert swarmio.collision_model_signature(ds, cfg) != base
+
+
+def _example_config():
+    cfg = gasdata.load_swarm_config(os.path.join(SWARM_DIR, "synthetic_swarm.swarm"))
+    return cfg, gasdata.load_gas_dataset(cfg.gas_data_file)
+
+
+def test_swarm_validator_accepts_the_reference_examples():
+    cfg, ds = _example_config()
+    gasdata.validate_swarm_config(cfg, ds)
+    b = gasdata.load_swarm_config(os.path.join(GOLDEN, "swarm_branching", "branching.swarm"))
+    gasdata.validate_swarm_config(b, gasdata.load_gas_dataset(b.gas_data_file))
+
+
+@pytest.mark.parametrize("change,message", [
+    # Swarm.cpp:336-359: the work-item gates
+    (dict(work_item_limit=0), "swarm work_item_limit must be positive"),
+    (dict(work_item_limit=3 * 256 * 2000 - 1), "swarm scan exceeds work_item_limit"),
+    (dict(work_item_limit=3 * 256 - 1), "swarm scan exceeds work_item_limit"),
+    # Swarm.cpp:317-335: population limits
+    (dict(population_limit=1000), "population_limit is valid only for population_model = "
+                                  "branching_resampled"),
+    (dict(population_model="branching_resampled", population_limit=256),
+     "branching swarm population_limit must exceed particles"),
+    (dict(population_model="branching_resampled", population_limit=10_000_000 - 255),
+     "branching swarm population_limit must exceed particles"),
+    (dict(particles=10_000_001), "swarm particles exceeds the safety limit of 10000000"),
+    (dict(particles=0), "swarm particles must be positive"),
+    (dict(timestep=1.0), "max_frequency \\* timestep must not exceed 0.1"),
+    (dict(burn_in_steps=2000), "swarm steps must exceed burn_in_steps"),
+    (dict(uncertainty_blocks=0), "uncertainty_blocks must be between 1 and the number"),
+    (dict(uncertainty_blocks=7), "sampling steps must be divisible by uncertainty_blocks"),
+    (dict(reduced_fields_td=[1.0, 1.0]), "must not contain duplicates"),
+    (dict(reduced_fields_td=[]), "at least one reduced field"),
+    (dict(max_energy_ev=1e9), "swarm max_energy_ev exceeds channel"),
+    (dict(spatial_bins=8), "spatial Townsend controls require positive spatial_histories"),
+])
+def test_swarm_validator_messages(change, message):
+    cfg, ds = _example_config()
+    for k, v in change.items():
+        setattr(cfg, k, v)
+    with pytest.raises(gasdata.GasDataError, match=message):
+        gasdata.validate_swarm_config(cfg, ds)
+
+
+def test_swarm_validator_work_items_boundary():
+    # fields x particles x steps equal to the limit passes, one more fails
+    cfg, ds = _example_config()
+    cfg.work_item_limit = 3 * 256 * 2000
+    gasdata.validate_swarm_config(cfg, ds)
+    cfg.steps += 10
+    with pytest.raises(gasdata.GasDataError, match="work_item_limit"):
+        gasdata.validate_swarm_config(cfg, ds)
+
+
+def test_serving_work_item_gate():
+    # the reference's per-scan gate applied to one served field point
+    from aurora_pic import hip, serving
+    r = hip.SwarmRequest(particles=50_000, steps=2000)
+    assert serving.work_items_exceeded(r) == ""                  # 1e8 exactly
+    r.steps = 2001
+    assert "swarm scan exceeds work_item_limit" in serving.work_items_exceeded(r)
+    assert serving.work_items_exceeded(r, 10 ** 12) == ""
+    assert "must be positive" in serving.work_items_exceeded(r, 0)
