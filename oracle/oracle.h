@@ -103,6 +103,12 @@ int or_sync_species(int dim, const int64_t* n, const double* L, int bc,
                     double dt, const double* Buni, int baxis, int64_t brows,
                     const double* bcoords, const double* bvalues, double* v);
 /* half-step init: v_half from v (Pusher.cpp:66-68, 158-160). */
+int or_step3d(const int64_t* n, const double* L, int nspecies, const int64_t* np,
+              double* const* pos, double* const* vh, uint8_t* const* alive,
+              double* const* v, const double* qm, const double* charge,
+              const double* weight, double eps, double dt, const double* Buni,
+              int baxis, int64_t brows, const double* bcoords, const double* bvalues,
+              int threads, double* rho, double* phi, double* E);
 int or_init_half_species(int dim, const int64_t* n, const double* L, int bc,
                          int64_t np, const double* pos, const double* v,
                          const uint8_t* alive, const double* E, double qm,

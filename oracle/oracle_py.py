@@ -233,6 +233,10 @@ class Oracle:
                      C.c_double, C.c_double, _dp, C.c_int, C.c_int64, _dp, _dp, _dp]
         L.or_sync_species.argtypes = half_args
         L.or_init_half_species.argtypes = half_args
+        _pp = C.POINTER(_dp)
+        L.or_step3d.argtypes = [_i64p, _dp, C.c_int, _i64p, _pp, _pp, C.POINTER(_u8p), _pp,
+                                _dp, _dp, _dp, C.c_double, C.c_double, _dp, C.c_int,
+                                C.c_int64, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.or_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
         L.or_u01.argtypes = [C.c_uint32]
         L.or_u01.restype = C.c_double
@@ -393,6 +397,59 @@ class Oracle:
     def init_half(self, n, L, bc, pos, v, alive, E, qm, dt, B=None, btable=None):
         return self._half(self.lib.or_init_half_species, n, L, bc, pos, v, alive, E, qm, dt,
                           B, btable)
+
+    # ---- whole 3D steps (the CPU timing baseline)
+    def plasma3d_init(self, n, L, eps, dt, species, btable=None, seed=1):
+        """A periodic 3D plasma state for step3d: species = [(charge, mass,
+        weight, count, thermal_velocity)], positions uniform and velocities
+        Maxwellian from numpy (seeded), then Simulation3D::initialize
+        (Simulation3D.cpp:288-299): deposit, solve at t = 0 and v_half from
+        v with a backward half step."""
+        n, L, _ = self._mesh(n, L)
+        rng = np.random.default_rng(seed)
+        st = dict(n=n, L=L, eps=float(eps), dt=float(dt), btable=btable, species=species,
+                  pos=[], vh=[], v=[], alive=[])
+        rho = np.zeros(int(np.prod(n)))
+        for q, m, w, count, vth in species:
+            pos = rng.uniform(0.0, 1.0, size=(count, 3)) * L
+            st["pos"].append(np.ascontiguousarray(pos.reshape(-1)))
+            st["v"].append(np.ascontiguousarray(rng.normal(0.0, vth, size=3 * count)))
+            st["alive"].append(np.ones(count, dtype=np.uint8))
+            rho += self.deposit_cic(n, L, 0, pos, q, w)
+        _, E = self.poisson(n, L, 0, eps, rho)
+        st["E"] = np.ascontiguousarray(E.reshape(-1))
+        for k, (q, m, w, count, vth) in enumerate(species):
+            vh = self.init_half(n, L, 0, st["pos"][k].reshape(-1, 3), st["v"][k].reshape(-1, 3),
+                                st["alive"][k], st["E"].reshape(3, -1), q / m, dt,
+                                btable=btable)
+            st["vh"].append(np.ascontiguousarray(vh.reshape(-1)))
+        st["rho"], st["phi"] = rho, np.zeros_like(rho)
+        return st
+
+    def step3d(self, st, threads=1):
+        """One Simulation3D::step of the state in place (or_step3d: serial
+        push, deposit and FFT solve, the synchronisation loop on `threads`
+        OpenMP threads).  Returns the live particle count."""
+        sp = st["species"]
+        S = len(sp)
+        pp = (_dp * S)(*[_ptr(x, _dp) for x in st["pos"]])
+        vv = (_dp * S)(*[_ptr(x, _dp) for x in st["vh"]])
+        vc = (_dp * S)(*[_ptr(x, _dp) for x in st["v"]])
+        al = (_u8p * S)(*[_ptr(x, _u8p) for x in st["alive"]])
+        np_ = i64([len(a) for a in st["alive"]])
+        qm = f64([q / m for q, m, _, _, _ in sp])
+        ch = f64([q for q, _, _, _, _ in sp])
+        wt = f64([w for _, _, w, _, _ in sp])
+        Buni, axis, rows, bc_, bv = self._bargs(None, st["btable"])
+        rc = self.lib.or_step3d(_ptr(st["n"], _i64p), _ptr(st["L"], _dp), S, _ptr(np_, _i64p),
+                                pp, vv, al, vc, _ptr(qm, _dp), _ptr(ch, _dp), _ptr(wt, _dp),
+                                st["eps"], st["dt"], _ptr(Buni, _dp), axis, rows,
+                                None if bc_ is None else _ptr(bc_, _dp),
+                                None if bv is None else _ptr(bv, _dp), int(threads),
+                                _ptr(st["rho"], _dp), _ptr(st["phi"], _dp), _ptr(st["E"], _dp))
+        if rc:
+            raise ValueError("tabulated vector field query lies outside the table")
+        return int(sum(int(a.sum()) for a in st["alive"]))
 
     # ---- RNG / MCC / swarm
     def philox(self, ctr, key):

@@ -669,3 +669,65 @@ int or_init_half_species(int dim, const int64_t* n, const double* L, int bc,
     return half_kick_species(dim, n, L, bc, np, pos, v, alive, E, qm, dt, -1.0,
                              Buni, baxis, brows, bcoords, bvalues, vh);
 }
+
+/* ------------------------------------------------ one 3D step (timing) */
+/* Simulation3D::step (Simulation3D.cpp:342-365) over every species: kick +
+ * drift + particle boundaries (serial, as the reference), deposit of all
+ * species and the periodic FFT solve (serial, Simulation3D.cpp:303-307),
+ * then the diagnostic synchronisation of every live particle, which the
+ * reference runs through runtime_parallel_for (Simulation3D.cpp:355-361):
+ * that loop alone uses `threads` OpenMP threads here, as the reference's
+ * OpenMP backend does.  Periodic mesh and particle boundaries.  E: 3 node
+ * arrays, updated to the new field; v receives the time-centred velocity.
+ * Returns -1 when a tabulated-B query leaves its table. */
+int or_step3d(const int64_t* n, const double* L, int nspecies, const int64_t* np,
+              double* const* pos, double* const* vh, uint8_t* const* alive,
+              double* const* v, const double* qm, const double* charge,
+              const double* weight, double eps, double dt, const double* Buni,
+              int baxis, int64_t brows, const double* bcoords, const double* bvalues,
+              int threads, double* rho, double* phi, double* E) {
+    const int pb[6] = {OR_PB_PERIODIC, OR_PB_PERIODIC, OR_PB_PERIODIC,
+                       OR_PB_PERIODIC, OR_PB_PERIODIC, OR_PB_PERIODIC};
+    int64_t absorbed[6] = {0, 0, 0, 0, 0, 0};
+    for (int s = 0; s < nspecies; ++s)
+        if (or_push_species(3, n, L, OR_PERIODIC, np[s], pos[s], vh[s], alive[s], E, qm[s],
+                            dt, Buni, baxis, brows, bcoords, bvalues, pb, absorbed))
+            return -1;
+    const int64_t nn = n[0] * n[1] * n[2];
+    for (int64_t i = 0; i < nn; ++i) rho[i] = 0.0;
+    for (int s = 0; s < nspecies; ++s)
+        or_deposit_cic(3, n, L, OR_PERIODIC, np[s], pos[s], alive[s], charge[s], weight[s],
+                       rho);
+    or_poisson_periodic_3d(n[0], n[1], n[2], L[0], L[1], L[2], eps, rho, phi, E, E + nn,
+                           E + 2 * nn);
+    const int boris = has_magnetic(Buni, brows);
+    int err = 0;
+    for (int s = 0; s < nspecies; ++s) {
+        const double* ps = pos[s];
+        const double* vs = vh[s];
+        const uint8_t* as = alive[s];
+        double* out = v[s];
+        const double q = qm[s];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(threads) reduction(| : err)
+#endif
+        for (int64_t p = 0; p < np[s]; ++p) {
+            if (!as[p]) continue;
+            const double* x = ps + 3 * p;
+            double e3[3];
+            gather_one(3, n, L, OR_PERIODIC, x, E, e3);
+            if (boris) {
+                double B[3];
+                if (magnetic_at(3, x, Buni, baxis, brows, bcoords, bvalues, B)) {
+                    err |= 1;
+                    continue;
+                }
+                or_boris_advance(vs + 3 * p, e3, B, q, 0.5 * dt, out + 3 * p);
+            } else {
+                for (int a = 0; a < 3; ++a) out[3 * p + a] = vs[3 * p + a] + 0.5 * q * e3[a] * dt;
+            }
+        }
+    }
+    (void)threads;
+    return err ? -1 : 0;
+}
