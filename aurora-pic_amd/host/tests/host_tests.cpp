@@ -789,6 +789,25 @@ const Case kCases[] = {
 // A 1D case file (written by tests/golden/make_run1d_golden.py): one
 // "key values..." line per setting, one "species name charge mass weight
 // particles drift thermal x_min x_max" line per species.  FP64 particles.
+// A two-column table (energy J, cross section m^2), '#'/';' comments.
+Table1D read_table(const std::filesystem::path& path) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("cannot open collision table " + path.string());
+    Table1D t;
+    std::string line;
+    while (std::getline(in, line)) {
+        const auto c = line.find_first_of("#;");
+        if (c != std::string::npos) line.resize(c);
+        std::istringstream ls(line);
+        double e = 0.0, s = 0.0;
+        if (!(ls >> e)) continue;
+        if (!(ls >> s)) throw std::runtime_error("malformed table row in " + path.string());
+        t.energies.push_back(e);
+        t.values.push_back(s);
+    }
+    return t;
+}
+
 void run1d_case(const std::filesystem::path& case_file, const std::filesystem::path& out) {
     std::ifstream in(case_file);
     if (!in) throw std::runtime_error("cannot open " + case_file.string());
@@ -796,11 +815,44 @@ void run1d_case(const std::filesystem::path& case_file, const std::filesystem::p
     cfg.permittivity = 1.0;
     std::string line;
     size_t most = 1;
+    size_t max_particles = 0;
+    int precision = 64;
     while (std::getline(in, line)) {
         std::istringstream ls(line);
         std::string key;
         if (!(ls >> key)) continue;
         if (key == "nx") ls >> cfg.nx;
+        else if (key == "precision") ls >> precision;
+        else if (key == "max_particles") ls >> max_particles;
+        else if (key == "units") {
+            std::string u;
+            ls >> u;
+            cfg.units = u == "si" ? UnitSystem::SI : UnitSystem::Normalized;
+            cfg.permittivity = u == "si" ? kEps0 : 1.0;
+        } else if (key == "model") {
+            // model <name> <target> <density> <max_frequency> <neutral_mass> <max_candidates>
+            NamedCollisionConfig m;
+            ls >> m.name >> m.species >> m.config.neutral_density >> m.config.max_frequency >>
+                m.config.neutral_mass >> m.config.max_candidates_per_particle;
+            cfg.collision_models.push_back(m);
+        } else if (key == "channel") {
+            // channel <name> <kind> <threshold J> <table file> <secondary|-> <ion|->
+            if (cfg.collision_models.empty()) throw std::runtime_error("channel before model");
+            auto& m = cfg.collision_models.back();
+            CollisionChannelConfig ch;
+            std::string kind, table, sec, ion;
+            ls >> ch.name >> kind >> ch.threshold_energy >> table >> sec >> ion;
+            ch.process = kind == "elastic"      ? CollisionProcessKind::Elastic
+                         : kind == "excitation" ? CollisionProcessKind::Excitation
+                         : kind == "ionization" ? CollisionProcessKind::Ionization
+                         : kind == "charge_exchange"
+                             ? CollisionProcessKind::ChargeExchange
+                             : throw std::runtime_error("unknown channel kind " + kind);
+            ch.cross_section = read_table(case_file.parent_path() / table);
+            if (sec != "-") m.secondary_species = sec;
+            if (ion != "-") m.ion_species = ion;
+            m.config.channels.push_back(ch);
+        }
         else if (key == "length") ls >> cfg.length;
         else if (key == "dt") ls >> cfg.dt;
         else if (key == "steps") ls >> cfg.steps;
@@ -821,7 +873,6 @@ void run1d_case(const std::filesystem::path& case_file, const std::filesystem::p
             cfg.initial_state_path = case_file.parent_path() / f;
         } else if (key == "species") {
             SpeciesConfig sc;
-            sc.precision = 64;
             ls >> sc.name >> sc.charge >> sc.mass >> sc.weight >> sc.particles >>
                 sc.drift_velocity >> sc.thermal_velocity >> sc.init_x_min >> sc.init_x_max;
             most = std::max(most, sc.particles);
@@ -831,10 +882,21 @@ void run1d_case(const std::filesystem::path& case_file, const std::filesystem::p
         }
         if (ls.fail()) throw std::runtime_error("malformed case line: " + line);
     }
-    cfg.max_particles_per_species = most;
+    for (auto& sc : cfg.species) sc.precision = precision;
+    cfg.max_particles_per_species = std::max(most, max_particles);
     cfg.output_dir = out;
+    if (const char* seed = std::getenv("AURORA_CASE_SEED")) cfg.seed = std::stoul(seed);
     Simulation sim(cfg, std::make_shared<Device>(0));
     sim.run();
+    // the final live particles, (x, v, v_y, v_z) doubles per species
+    for (size_t k = 0; k < cfg.species.size(); ++k) {
+        std::ofstream f(out / ("final_" + cfg.species[k].name + ".f64"), std::ios::binary);
+        for (const auto& p : sim.particles(k)) {
+            if (!p.alive) continue;
+            const double row[4] = {p.x, p.v, p.velocity_y, p.velocity_z};
+            f.write(reinterpret_cast<const char*>(row), sizeof row);
+        }
+    }
 }
 
 // A periodic 3D case file (tests/golden/make_run3d_golden.py): "n", "length",

@@ -730,6 +730,18 @@ class Reference:
             raise RuntimeError(err.value.decode())
 
 
+    def run1d_case(self, case_path: str, output_dir: str, seed: int = -1):
+        """The reference's 1D Simulation::run() on a case file (named MCC
+        models included), writing its outputs and the final live particles
+        (final_<species>.f64) into output_dir; seed >= 0 replaces the case's."""
+        err = C.create_string_buffer(512)
+        self.lib.ref_run1d_case.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, C.c_char_p,
+                                            C.c_int]
+        rc = self.lib.ref_run1d_case(case_path.encode(), output_dir.encode(), int(seed), err,
+                                     512)
+        if rc:
+            raise RuntimeError(err.value.decode())
+
     def run3d(self, case: dict, output_dir: str):
         """The reference's Simulation3D::run() on a tests/golden/run3d case."""
         sp = case["species"]

@@ -27,6 +27,7 @@
 #include <fstream>
 #include <iomanip>
 #include <random>
+#include <sstream>
 #include <string>
 #include <unistd.h>
 #include <vector>
@@ -624,6 +625,113 @@ int ref_run2d(const char* output_dir, const int64_t* n, const double* L, double 
         }
         pic::Simulation2D sim(cfg);
         sim.run();
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(err, errlen, e.what());
+        return -1;
+    }
+}
+
+
+// The reference's own 1D Simulation::run() on a case file (the format of
+// tests/golden/run1d/<case>/case.txt, read the same way by the host test
+// program's --run1d mode): mesh, drive, species lines, and optionally
+// "units si", "max_particles N" and named null-collision models
+//   model <name> <target species> <neutral_density> <max_frequency>
+//         <neutral_mass> <max_candidates>
+//   channel <name> <elastic|excitation|ionization> <threshold J>
+//           <table file, energies J and m^2> <secondary|-> <ion|->
+// (channels belong to the model above them).  seed >= 0 replaces the case's
+// seed.  After the run, final_<species>.f64 in output_dir holds the live
+// particles as (x, v, velocity_y, velocity_z) doubles.
+int ref_run1d_case(const char* case_path, const char* output_dir, int64_t seed, char* err,
+                   int errlen) {
+    try {
+        const std::filesystem::path path(case_path);
+        std::ifstream in(path);
+        if (!in) throw std::runtime_error(std::string("cannot open ") + case_path);
+        pic::Config cfg;
+        cfg.output_dir = output_dir;
+        std::string line;
+        while (std::getline(in, line)) {
+            std::istringstream ls(line);
+            std::string key;
+            if (!(ls >> key)) continue;
+            if (key == "nx") ls >> cfg.nx;
+            else if (key == "length") ls >> cfg.length;
+            else if (key == "dt") ls >> cfg.dt;
+            else if (key == "steps") ls >> cfg.steps;
+            else if (key == "output_interval") ls >> cfg.output_interval;
+            else if (key == "velocity_dimensions") ls >> cfg.velocity_dimensions;
+            else if (key == "seed") ls >> cfg.seed;
+            else if (key == "precision") { int p; ls >> p; }
+            else if (key == "max_particles") ls >> cfg.max_particles_per_species;
+            else if (key == "units") {
+                std::string u;
+                ls >> u;
+                cfg.units.system = u == "si" ? pic::UnitSystem::SI : pic::UnitSystem::Normalized;
+            } else if (key == "dirichlet") {
+                int v = 0;
+                ls >> v;
+                cfg.boundary = v ? pic::Boundary::Dirichlet : pic::Boundary::Periodic;
+            } else if (key == "electrodes") {
+                ls >> cfg.phi_left >> cfg.phi_left_drive.amplitude >>
+                    cfg.phi_left_drive.frequency >> cfg.phi_left_drive.phase >> cfg.phi_right >>
+                    cfg.phi_right_drive.amplitude >> cfg.phi_right_drive.frequency >>
+                    cfg.phi_right_drive.phase;
+            } else if (key == "initial_state") {
+                std::string f;
+                ls >> f;
+                cfg.initial_state_path = path.parent_path() / f;
+            } else if (key == "species") {
+                pic::SpeciesConfig sc;
+                ls >> sc.name >> sc.charge >> sc.mass >> sc.weight >> sc.particles >>
+                    sc.drift_velocity >> sc.thermal_velocity >> sc.init_x_min >> sc.init_x_max;
+                cfg.species.push_back(sc);
+            } else if (key == "model") {
+                pic::NamedCollisionConfig m;
+                std::size_t cand = 64;
+                ls >> m.name >> m.config.species >> m.config.neutral_density >>
+                    m.config.max_frequency >> m.config.neutral_mass >> cand;
+                m.config.enabled = true;
+                m.config.model = pic::CollisionModelKind::NullCollision;
+                m.config.max_candidates_per_particle = cand;
+                m.config.gas_data_units = pic::UnitSystem::SI;
+                cfg.collision_models.push_back(m);
+            } else if (key == "channel") {
+                if (cfg.collision_models.empty())
+                    throw std::runtime_error("channel line before any model line");
+                pic::CollisionChannelConfig ch;
+                std::string kind, table, sec, ion;
+                ls >> ch.name >> kind >> ch.threshold_energy >> table >> sec >> ion;
+                ch.process = kind == "elastic"      ? pic::CollisionProcessKind::Elastic
+                             : kind == "excitation" ? pic::CollisionProcessKind::Excitation
+                             : kind == "ionization" ? pic::CollisionProcessKind::Ionization
+                             : kind == "charge_exchange"
+                                 ? pic::CollisionProcessKind::ChargeExchange
+                                 : throw std::runtime_error("unknown channel kind " + kind);
+                ch.cross_section_file = path.parent_path() / table;
+                if (sec != "-") ch.secondary_species = sec;
+                if (ion != "-") ch.ion_species = ion;
+                cfg.collision_models.back().config.channels.push_back(ch);
+            } else {
+                throw std::runtime_error("unknown case key " + key);
+            }
+            if (ls.fail()) throw std::runtime_error("malformed case line: " + line);
+        }
+        if (seed >= 0) cfg.seed = static_cast<unsigned>(seed);
+        pic::Simulation sim(cfg);
+        sim.run();
+        for (const auto& sp : sim.species()) {
+            std::ofstream out(std::filesystem::path(output_dir) /
+                                  ("final_" + sp.config().name + ".f64"),
+                              std::ios::binary);
+            for (const auto& p : sp.particles()) {
+                if (!p.alive) continue;
+                const double row[4] = {p.x, p.v, p.velocity_y, p.velocity_z};
+                out.write(reinterpret_cast<const char*>(row), sizeof row);
+            }
+        }
         return 0;
     } catch (const std::exception& e) {
         set_error(err, errlen, e.what());
