@@ -354,6 +354,13 @@ class Context:
                                                    _p(E, _dp)))
         return rho, phi, E.reshape(self.dim, self.nnodes)
 
+    def rho(self):
+        """Node charge density alone (no field solve: also on meshes whose
+        solve is not built, such as 2D/3D Dirichlet)."""
+        rho = np.zeros(self.nnodes)
+        self._check(self.lib.aurora_hip_get_fields(self.h, _p(rho, _dp), None, None))
+        return rho
+
     # ---- operations
     def deposit(self):
         self._check(self.lib.aurora_hip_deposit(self.h))

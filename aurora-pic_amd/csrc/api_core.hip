@@ -312,8 +312,6 @@ int aurora_hip_mesh_bind(aurora_hip_ctx* h, int dim, const int64_t* n, const dou
         if (dim == 2)
             AUR_REQUIRE(finite_positive(depth), AURORA_ERR_INVALID_ARGUMENT,
                         "2D deposition depth must be positive and finite");
-        AUR_REQUIRE(!(dim > 1 && boundary == AURORA_DIRICHLET), AURORA_ERR_UNSUPPORTED,
-                    "2D/3D Dirichlet field solves (SOR) are not part of this build");
         int64_t nodes = 1;
         for (int a = 0; a < dim; ++a) {
             AUR_REQUIRE(n[a] >= 3, AURORA_ERR_INVALID_ARGUMENT, "mesh extents must be at least 3");

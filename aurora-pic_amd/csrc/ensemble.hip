@@ -743,7 +743,7 @@ void capture_graph(Context& c, Ensemble& e, double dt, int steps) {
         e.graph = nullptr;
     }
     const EnsArgs a = make_args(c, e, dt);
-    const MccParams m = c.mcc.loaded ? mcc_params(c) : MccParams{};
+    const MccParams m = c.mcc.loaded ? mcc_step_params(c) : MccParams{};
     AUR_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < steps; ++k) enqueue_step(c, e, a, m);
     enqueue_compaction(c, e);
@@ -990,7 +990,7 @@ int aurora_hip_ensemble_step(aurora_hip_ctx* h, int64_t handle, double dt, int64
         const int G = e.requested_graph_steps;
         if (G > 0 && (e.graph_steps != G || e.dt_graph != dt)) capture_graph(c, e, dt, G);
         const EnsArgs a = make_args(c, e, dt);
-        const MccParams m = c.mcc.loaded ? mcc_params(c) : MccParams{};
+        const MccParams m = c.mcc.loaded ? mcc_step_params(c) : MccParams{};
         int64_t left = nsteps;
         while (left > 0) {
             const int64_t k = (G > 0 && left >= G) ? G : std::min<int64_t>(left, kEnsRing - 2);

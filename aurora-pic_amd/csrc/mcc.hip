@@ -244,6 +244,18 @@ MccParams mcc_params_of(const MccModelHost& h) {
 
 MccParams mcc_params(const Context& c) { return mcc_params_of(c.mcc); }
 
+MccParams mcc_step_params(const Context& c) {
+    MccParams m = mcc_params(c);
+    const MccModelHost& h = c.mcc;
+    if (h.secondary_species >= 0 && h.ion_species >= 0) {
+        // equal weights are required of 1D products, so the unweighted tally
+        // times the target weight is the reference's weighted sum
+        m.half_sec_mass = static_cast<float>(0.5 * c.species[h.secondary_species].mass);
+        m.half_ion_mass = static_cast<float>(0.5 * c.species[h.ion_species].mass);
+    }
+    return m;
+}
+
 void mcc_step(Context& c, double dt) {
     MccModelHost& h = c.mcc;
     AUR_REQUIRE(c.dim == 1, AURORA_ERR_UNSUPPORTED,
@@ -257,7 +269,7 @@ void mcc_step(Context& c, double dt) {
         h.product_offset.alloc(n + 1);
         h.product_vel.alloc(static_cast<size_t>(n) * 12);
     }
-    const MccParams m = mcc_params(c);
+    const MccParams m = mcc_step_params(c);
     Mcc1dArgs a{};
     a.cell = sp.cell.ptr;
     a.off = reinterpret_cast<float*>(sp.off[0].ptr);

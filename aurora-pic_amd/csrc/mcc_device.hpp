@@ -51,6 +51,11 @@ struct MccParams {
     float neutral_mass;      // 0: infinitely heavy neutral
     float neutral_vstd;      // 0: cold neutral
     float safe_speed;        // below it the total rate is provably under max_frequency
+    // The 1D step books the kinetic energy of each ionization's secondary
+    // and ion into the ionization channel (Simulation.cpp:880-902); half
+    // their masses, 0 for the bare collide (NullCollisionModel::collide).
+    float half_sec_mass;
+    float half_ion_mass;
 };
 
 template <int NCH>
@@ -371,15 +376,24 @@ __device__ void mcc_collide(const MccParams& m, const XsView& X, const ChannelSe
             const float r0 = v[0] - w[0], r1 = v[1] - w[1], r2 = v[2] - w[2];
             float sec[3];
             isotropic(sqrtf(r0 * r0 + r1 * r1 + r2 * r2), rng, sec);
+            const float s0 = w[0] + sec[0], s1 = w[1] + sec[1], s2 = w[2] + sec[2];
 #pragma unroll
             for (int k = 0; k < 2; ++k)
-                if (t.nsec == k)
+                if (t.nsec == k) {
+                    t.sec[k][0] = s0;
+                    t.sec[k][1] = s1;
+                    t.sec[k][2] = s2;
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        t.sec[k][a] = w[a] + sec[a];
-                        t.ion[k][a] = w[a];
-                    }
+                    for (int a = 0; a < 3; ++a) t.ion[k][a] = w[a];
+                }
             ++t.nsec;
+            if (m.half_sec_mass != 0.f) {
+                const float added = m.half_sec_mass * (s0 * s0 + s1 * s1 + s2 * s2) +
+                                    m.half_ion_mass * (w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+#pragma unroll
+                for (int k = 0; k < NCH; ++k)
+                    if (k == c) t.energy_change[k] += added;
+            }
         }
         if (ch.kind == AURORA_ATTACHMENT) break;
     }
@@ -572,5 +586,7 @@ void set_channel_options(MccModelHost& h, int channel, int angular, int frame, i
                          const double* energy, const double* mean_cosine);
 MccParams mcc_params_of(const MccModelHost& h);
 MccParams mcc_params(const Context& c);
+// mcc_params plus the product masses of the 1D step's energy tally
+MccParams mcc_step_params(const Context& c);
 
 }  // namespace aurora

@@ -898,7 +898,7 @@ void prepare_solver(Context& c) {
         AUR_HIP(hipMemcpy(c.thomas_cp.ptr, cp.data(), m * sizeof(double), hipMemcpyHostToDevice));
         AUR_HIP(hipMemcpy(c.thomas_inv.ptr, denom.data(), m * sizeof(double),
                           hipMemcpyHostToDevice));
-    } else {
+    } else if (c.bc == AURORA_PERIODIC) {
         for (int a = 0; a < c.dim; ++a) {
             const int n = static_cast<int>(c.n[a]);
             AUR_REQUIRE(n <= kFftLds, AURORA_ERR_UNSUPPORTED,
@@ -933,7 +933,15 @@ GreenArgs green_args(const Context& c, bool want_phi) {
     return ga;
 }
 
+// 2D/3D Dirichlet meshes bind, deposit, gather and push; their SOR field
+// solve (FieldSolver.cpp:501-657) is not built.
+void require_solver(const Context& c) {
+    AUR_REQUIRE(c.dim == 1 || c.bc == AURORA_PERIODIC, AURORA_ERR_UNSUPPORTED,
+                "2D/3D Dirichlet field solves (SOR) are not part of this build");
+}
+
 void solve_step_fields(Context& c, double phi_left, double phi_right) {
+    require_solver(c);
     const bool e64 = any_fp64(c);
     if (c.dim >= 2 && fused_fft_ok(c)) {
         solve_fused(c, green_args(c, false), false, e64, true);
@@ -947,6 +955,7 @@ void solve_step_fields(Context& c, double phi_left, double phi_right) {
 }
 
 void solve_fields(Context& c, double phi_left, double phi_right, bool want_phi, bool want_e64) {
+    require_solver(c);
     if (c.dim == 1) {
         const int n = static_cast<int>(c.n[0]);
         if (c.bc == AURORA_PERIODIC) {

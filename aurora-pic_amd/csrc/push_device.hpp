@@ -49,13 +49,16 @@ __device__ __forceinline__ int wrap_near(int i, int n) {
 
 // ------------------------------------------------------------------ drift
 // Mirror (ni + f) into [0, ncells] (reflect_coordinate, Simulation3D.cpp:30-40)
-// in integer + fraction form.  A particle that would need more than 64
-// reflections (a non-finite or absurd velocity) is treated as lost.
+// in integer + fraction form, as many times as it takes: a particle that
+// crossed the domain more than once in one step bounces between the walls.
+// One that would need more than 64 reflections (a non-finite or absurd
+// velocity; each pair of reflections moves it by two domain lengths) is
+// treated as lost; the bound also keeps the index arithmetic in range.
 template <typename R>
 __device__ __forceinline__ bool reflect_into(int& ni, R& f, R& v, int ncells) {
+    if (ni < -64 * ncells || ni > 64 * ncells) return false;
     for (int it = 0; it < 64; ++it) {
         if (!(ni < 0 || ni > ncells || (ni == ncells && f > R(0)))) return true;
-        if (ni < -ncells || ni > 2 * ncells) return false;
         if (ni < 0) {
             if (f > R(0)) { ni = -ni - 1; f = R(1) - f; } else { ni = -ni; }
         } else {

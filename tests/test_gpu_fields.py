@@ -14,7 +14,8 @@ from conftest import golden
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["1d_periodic", "1d_dirichlet", "2d_periodic", "3d_periodic"]
+CASES = ["1d_periodic", "1d_dirichlet", "2d_periodic", "2d_dirichlet", "3d_periodic",
+         "3d_dirichlet"]
 
 
 @pytest.fixture
@@ -46,7 +47,7 @@ def test_deposit_bit_exact_against_fixed_point_oracle(oracle, hip_ctx, case, pre
     cell, off, _ = hip_ctx.download_raw(sid)
     expect = oracle.deposit_fixed(g["n"], int(g["bc"]), cell, off)
     assert np.array_equal(acc, expect)
-    rho, _, _ = hip_ctx.fields()
+    rho = hip_ctx.rho()
     rho_fx = oracle.fixed_to_rho(g["n"], g["L"], int(g["bc"]), [expect],
                                  [float(g["charge"]) * float(g["weight"])])
     assert np.array_equal(rho, rho_fx)
@@ -205,6 +206,63 @@ def test_one_step_push_matches_oracle(oracle, hip_ctx, name, n, L, bc, B, btable
     assert np.max(np.abs(dx)) <= 1e-6 * np.max(scale_x)
     vscale = np.max(np.abs(ovel[keep]))
     assert np.max(np.abs(vel1[keep] - ovel[keep])) <= 1e-6 * vscale
+
+
+BOUNDARY_CASES = [
+    # (name, nodes, lengths, per-face policy left, right, bottom, top, back, front)
+    ("2d_absorb_all", [17, 13], [1.0, 0.75], [1, 1, 1, 1, 1, 1]),
+    ("2d_reflect_all", [17, 13], [1.0, 0.75], [2, 2, 2, 2, 2, 2]),
+    ("2d_mixed", [17, 13], [1.0, 0.75], [1, 2, 2, 1, 1, 1]),
+    ("3d_absorb_all", [9, 11, 7], [1.0, 0.75, 0.5], [1, 1, 1, 1, 1, 1]),
+    ("3d_reflect_all", [9, 11, 7], [1.0, 0.75, 0.5], [2, 2, 2, 2, 2, 2]),
+    ("3d_mixed", [9, 11, 7], [1.0, 0.75, 0.5], [2, 1, 1, 2, 2, 1]),
+]
+
+
+@pytest.mark.parametrize("name,n,L,pb", BOUNDARY_CASES)
+@pytest.mark.parametrize("precision", [32, 64])
+def test_particle_boundaries_match_oracle(oracle, hip_ctx, name, n, L, pb, precision):
+    """Absorbing and reflecting walls of 2D/3D Dirichlet meshes
+    (Simulation3D.cpp:30-84, Simulation2D.cpp:43-97): one push with a field
+    and speeds that carry many particles through a wall (some through two
+    reflections), against the oracle's restatement: survivors' positions and
+    velocities within 1e-6, absorbed counts per face equal (a particle within
+    rounding of a wall may go either way)."""
+    print(f"case={name} precision={precision}", flush=True)
+    rng = np.random.default_rng(21)
+    dim = len(n)
+    N = 6000
+    pos = rng.uniform(0, 1, size=(N, dim)) * np.asarray(L)
+    vel = rng.normal(0, 3.0, size=(N, 3))
+    E = _smooth_field(n, L, dim, rng)
+    qm, dt = -1.3, 0.1
+    hip_ctx.mesh(n, L, 1, 1.0)
+    hip_ctx.boundaries(pb)
+    sid = hip_ctx.add_species(qm, 1.0, 1.0, N, precision)
+    hip_ctx.upload(sid, pos, vel)
+    hip_ctx.set_electric(E)
+    pos0, vel0, alive0, _ = hip_ctx.download(sid)
+    hip_ctx.push(dt)
+    pos1, vel1, alive1, _ = hip_ctx.download(sid)
+    absorbed, _ = hip_ctx.losses(sid)
+    opos, ovel, oalive, oabs = oracle.push(n, L, 1, pos0, vel0, alive0, E, qm, dt, pb=pb)
+    scale_x = np.asarray(L)
+    mismatch = alive1 != oalive
+    if mismatch.any():
+        p = np.where(mismatch[:, None], opos, 0.0)[mismatch]
+        near = np.min(np.minimum(np.abs(p), np.abs(p - scale_x)), axis=1)
+        assert np.all(near <= 1e-5 * np.max(scale_x)), near
+    assert mismatch.sum() <= 2
+    assert np.abs(absorbed - oabs).sum() <= mismatch.sum()
+    if 1 in pb:
+        assert absorbed.sum() > 100            # walls absorbed particles
+    keep = (alive1 == 1) & (oalive == 1)
+    assert keep.sum() > N // 4
+    tol = 1e-6 if precision == 32 else 1e-10
+    assert np.max(np.abs(pos1[keep] - opos[keep])) <= tol * np.max(scale_x)
+    assert np.all((pos1[keep] >= 0) & (pos1[keep] <= scale_x))
+    vscale = np.max(np.abs(ovel[keep]))
+    assert np.max(np.abs(vel1[keep] - ovel[keep])) <= tol * vscale
 
 
 def test_step_3d_matches_oracle(oracle, hip_ctx):

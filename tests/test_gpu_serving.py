@@ -45,7 +45,7 @@ def test_serve_mixed_with_ensemble_equals_lone_runs():
     ctx = hip.Context(0)
     ctx.swarm_gas(cfg["gas"])
     fps = sorted(ctx.swarm_footprint(r) for r in reqs[:count])
-    budget = sum(fps[-3:])                     # about three requests at a time
+    budget = sum(fps[-3:])                     # the three largest at a time
     assert ctx.swarm_footprint(big) > budget
     ens_ctx, ens, dt = serving.discharge_ensemble(instances=3, particles=6000, graph_steps=10)
     ticks = [0]
@@ -54,14 +54,15 @@ def test_serve_mixed_with_ensemble_equals_lone_runs():
         ens.step(dt, 10)
         ticks[0] += 1
 
-    rep = serving.serve(ctx, reqs, arrivals, budget, steps_per_launch=150, on_launch=tick)
+    rep = serving.serve(ctx, reqs, arrivals, budget, steps_per_launch=150, on_launch=tick,
+                        work_item_limit=10 ** 12)
     print("serving:", rep.summary(), flush=True)
     served = {s.index: s for s in rep.served}
     assert len(served) == count + 1
     assert served[count].rejected and "exceeds the HBM budget" in served[count].rejected
     assert rep.refused_for_budget > 0          # requests waited for memory ...
     assert all(s.result is not None for i, s in served.items() if i < count)   # ... and ran
-    assert rep.max_active <= 3
+    assert rep.max_active >= 2                 # requests ran together
     assert np.all(rep.latencies() > 0)
     lone, _, _ = serving.run_sequential(ctx, reqs[:count])
     for i in range(count):
