@@ -163,6 +163,8 @@ def load(path: str = LIB_PATH):
                                            _dp, _dp],
         "aurora_hip_mcc_collide": [ctx, C.c_int64, _dp, C.c_double, C.c_uint64, C.c_uint32,
                                    _u64p, _dp],
+        "aurora_hip_mcc_collide_device": [ctx, C.c_int64, C.c_void_p, C.c_double, C.c_uint64,
+                                          C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p],
         "aurora_hip_swarm_gas": [ctx, C.c_int, _ip, _dp, _i64p, C.POINTER(_dp), C.POINTER(_dp),
                                  C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double],
         "aurora_hip_swarm_submit": [ctx, C.POINTER(SwarmRequest), C.c_uint64, _i64p],
@@ -492,6 +494,14 @@ class Context:
                                                     stream_id, _p(counts, _u64p),
                                                     _p(ech, _dp)))
         return v.reshape(-1, 3), counts, ech
+
+    def mcc_collide_device(self, count, velocity, dt, step, counts, energy_change, status,
+                           stream_id=0):
+        """aurora_hip_mcc_collide_device: every pointer is a device address
+        (int); queued on the context stream, nothing allocated."""
+        self._check(self.lib.aurora_hip_mcc_collide_device(self.h, count, velocity, dt, step,
+                                                           stream_id, counts, energy_change,
+                                                           status))
 
     # ---- swarms
     def swarm_gas(self, gas):

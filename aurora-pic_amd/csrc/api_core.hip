@@ -257,6 +257,7 @@ int aurora_hip_create(int device, aurora_hip_ctx** out) {
         c.info = buf;
         c.dev_error.alloc(1);
         AUR_HIP(hipMemset(c.dev_error.ptr, 0, sizeof(uint32_t)));
+        collide_scratch_init(device);
     });
     if (rc != AURORA_OK) {
         delete h;

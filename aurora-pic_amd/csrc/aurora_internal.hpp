@@ -387,6 +387,7 @@ bool sort_needed(Context& ctx, Species& sp);
 // adaptive policy: the species' push now costs more than its cycle average
 bool adaptive_sort_due(Context& ctx, Species& sp);
 void mcc_step(Context& ctx, double dt);
+void collide_scratch_init(int device);
 void init_half_all(Context& ctx, double dt);
 void set_b_profile_device(Context& ctx);
 void swarm_destroy(Context& ctx);

@@ -10,8 +10,10 @@
 //   never collide in their birth step, and their v_half is initialised from
 //   the current field (Simulation.cpp:965-1005).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <cstring>
 #include <string.h>
@@ -126,6 +128,41 @@ __global__ void k_append_products(const int32_t* pcell, const float* poff,
         if (np == 0) continue;
         append_products_of(pcell[p], poff[p], np, offset[p], prod + 12 * p, E32, nx, sec, ion);
     }
+}
+
+// ------------------------------------------- allocation-free collide
+// aurora_hip_mcc_collide_device keeps its only scratch, one row of
+// per-channel energy sums per wave of its launch, in this array: at most
+// kNoAllocGroups workgroups, so kNoAllocRows rows (512 KiB per device, held
+// until the process ends).  Host code never passes the array's name to a
+// kernel: it uses the device address resolved per device by
+// collide_scratch_init.
+constexpr int kNoAllocGroups = 2048;
+constexpr int64_t kNoAllocRows = static_cast<int64_t>(kNoAllocGroups) * (kBlk / 64);
+static __device__ __align__(256) double scratch[kNoAllocRows * kMaxChannels];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+std::array<double*, kMaxDevices> scratch_ptr{};
+
+// Clears the caller's outputs (a kernel, not a memset, so the call can be
+// captured in a graph).
+__global__ void k_clear_collide_outputs(unsigned long long* counts, int ncounts,
+                                        double* energy, int nenergy, uint32_t* status) {
+    const int t = threadIdx.x;
+    if (t < ncounts) counts[t] = 0;
+    if (t < nenergy) energy[t] = 0.0;
+    if (t == 0) *status = 0;
+}
+
+// The device error bits of a collide as an aurora_status.
+__global__ void k_collide_status(uint32_t* status) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t b = *status;
+    *status = b == 0                          ? AURORA_OK
+              : (b & DEV_ERR_RATE)            ? AURORA_ERR_RATE_ABOVE_BOUND
+              : (b & DEV_ERR_CANDIDATES)      ? AURORA_ERR_CANDIDATE_LIMIT
+                                              : AURORA_ERR_INVALID_ARGUMENT;
 }
 
 int blocks(const Context& c, int64_t n) {
@@ -365,6 +402,19 @@ void mcc_step(Context& c, double dt) {
     }
     se.tiled = false;
     si.tiled = false;
+}
+
+// Resolves this device's address of the collide scratch (once per device,
+// with the device current); called by aurora_hip_create.
+void collide_scratch_init(int device) {
+    AUR_REQUIRE(device >= 0 && device < kMaxDevices, AURORA_ERR_UNSUPPORTED,
+                "device ordinal above the collide scratch table");
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (scratch_ptr[static_cast<size_t>(device)]) return;
+    void* p = nullptr;
+    AUR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    AUR_REQUIRE(p, AURORA_ERR_HIP, "collide scratch has no device address");
+    scratch_ptr[static_cast<size_t>(device)] = static_cast<double*>(p);
 }
 
 }  // namespace aurora
@@ -663,6 +713,44 @@ int aurora_hip_mcc_collide(aurora_hip_ctx* hctx, int64_t count, double* velocity
             for (int k = 0; k < h.nchan + 3; ++k) counts[k] = raw[k];
         if (energy_change)
             for (int k = 0; k < h.nchan; ++k) energy_change[k] = e[k];
+    });
+}
+
+int aurora_hip_mcc_collide_device(aurora_hip_ctx* hctx, int64_t count, float* velocity,
+                                  double dt, uint64_t step, uint32_t stream_id,
+                                  uint64_t* counts, double* energy_change, uint32_t* status) {
+    return guarded(hctx, [&](Context& c) {
+        MccModelHost& h = c.mcc;
+        AUR_REQUIRE(h.energy.ptr, AURORA_ERR_INVALID_ARGUMENT, "no MCC model loaded");
+        AUR_REQUIRE(std::isfinite(dt) && dt > 0.0, AURORA_ERR_INVALID_ARGUMENT,
+                    "MCC timestep must be positive and finite");
+        AUR_REQUIRE(counts && energy_change && status, AURORA_ERR_INVALID_ARGUMENT,
+                    "device counts, energy_change and status are required");
+        AUR_REQUIRE(count >= 0 && count <= int64_t{0xFFFFFFFF}, AURORA_ERR_INVALID_ARGUMENT,
+                    "collide count must be between 0 and 4294967295 (the particle index keys "
+                    "its random stream)");
+        AUR_REQUIRE(count == 0 || velocity, AURORA_ERR_INVALID_ARGUMENT,
+                    "velocity array required");
+        double* rows = scratch_ptr[static_cast<size_t>(c.device)];
+        AUR_REQUIRE(rows, AURORA_ERR_HIP, "collide scratch not resolved for this device");
+        const int nch = h.nchan;
+        const MccParams m = mcc_params(c);
+        k_clear_collide_outputs<<<1, 64, 0, c.stream>>>(
+            reinterpret_cast<unsigned long long*>(counts), nch + 3, energy_change, nch, status);
+        if (count > 0) {
+            int64_t g = (count + kBlk - 1) / kBlk;
+            if (g > kNoAllocGroups) g = kNoAllocGroups;
+            const int gi = static_cast<int>(g);
+            with_nchan(m.nchan, [&]<int NCH>() {
+                k_collide<NCH><<<gi, kBlk, 0, c.stream>>>(
+                    m, velocity, count, static_cast<float>(dt), h.seed, stream_id,
+                    static_cast<uint32_t>(step),
+                    TallyOut{reinterpret_cast<unsigned long long*>(counts), rows}, status);
+                k_fold_rows<NCH><<<1, kBlk, 0, c.stream>>>(rows, energy_rows(gi), energy_change);
+            });
+        }
+        k_collide_status<<<1, 64, 0, c.stream>>>(status);
+        AUR_HIP(hipGetLastError());
     });
 }
 

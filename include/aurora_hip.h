@@ -296,6 +296,24 @@ int aurora_hip_mcc_collide(aurora_hip_ctx* ctx, int64_t count,
                            uint32_t stream_id, uint64_t* counts,
                            double* energy_change);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  aurora_hip_mcc_collide over device-resident FP32
+ * velocities (count*3, interleaved, updated in place) that allocates
+ * nothing: the same kernels and random streams, queued on the context's
+ * stream without synchronising.  Device outputs, overwritten: counts
+ * (nchan + 3: candidates, nulls, per channel, secondaries), energy_change
+ * (nchan, J, unweighted) and status (an aurora_status: frequency bound,
+ * candidate limit or non-finite velocity).  Its only scratch is a
+ * fixed per-device array of 2048 x 4 rows of 8 doubles (512 KiB), which
+ * bounds the launch to 2048 workgroups (particles are walked grid-stride);
+ * count is at most 4294967295.  Calls that overlap on one GPU share that
+ * array and race. */
+int aurora_hip_mcc_collide_device(aurora_hip_ctx* ctx, int64_t count,
+                                  float* velocity, double dt, uint64_t step,
+                                  uint32_t stream_id, uint64_t* counts,
+                                  double* energy_change, uint32_t* status);
+
 /* ------------------------------------------------------------ swarms */
 /* Homogeneous electron swarm requests (run_field, Swarm.cpp:940-1250,
  * fixed population), batched: many requests of uneven particle count
