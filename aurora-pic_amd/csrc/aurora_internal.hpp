@@ -267,7 +267,7 @@ struct Context {
     DevBuf<double2> fa, fb;
     DevBuf<double2> tw[3];          // per-axis twiddles e^{-2 pi i k / n}
     // 1D Dirichlet precomputed Thomas coefficients
-    DevBuf<double> thomas_cp, thomas_inv;
+    DevBuf<double> thomas_inv;       // 1D Dirichlet solve: first-difference scratch
     DevBuf<double> dft_cos, dft_sin;
 
     // reductions

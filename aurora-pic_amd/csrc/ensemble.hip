@@ -70,10 +70,8 @@ struct EnsArgs {
     double* phi;
     double* E64;
     float* E32;
-    double* dp;                     // Thomas scratch [M][nx]
+    double* dp;                     // Dirichlet solve scratch [M][nx]
     RhoArgs rho_args;               // accumulator pointers of instance 0
-    const double* cp;
-    const double* denom;
     const double* cs;
     const double* sn;
     double dx, eps, L;
@@ -93,7 +91,7 @@ struct EnsArgs {
     int32_t* chunk_off;
     int mchunks;
     int mgroups;                    // MCC workgroups per instance (each walks chunks mgroups apart)
-    double* energy_rows;            // [M][mgroups * waves][kMaxChannels]: per-wave energy change
+    double* energy_rows;            // [M][mgroups * waves][kTallyStride]: per-wave tallies
     int64_t* base;                  // [M][kBaseFields]
     unsigned long long* counts;     // [M][kMaxChannels + 3]
     double* ech;                    // [M][kMaxChannels]
@@ -210,8 +208,7 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_push(EnsArgs a) {
 
 // rho from the fixed-point sums (clearing them), then the field solve at
 // solve index (*step + solve_offset): one workgroup per instance
-__global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_offset,
-                                                           int thomas_lds) {
+__global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_offset) {
     extern __shared__ double sh[];
     __shared__ double mean_s;
     const int inst = blockIdx.x;
@@ -243,8 +240,8 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_
         spectral_solve_1d(a.rho + o, a.phi + o, a.E64 + o, a.E32 + o, a.cs, a.sn, a.nx, a.L, a.eps,
                           sh, &mean_s);
     else
-        thomas_solve_1d(a.rho + o, a.phi + o, a.E64 + o, a.E32 + o, a.cp, a.denom, a.dp + o, a.nx,
-                        a.dx, a.eps, pv[0], pv[1], thomas_lds ? sh : nullptr);
+        dirichlet_solve_1d(a.rho + o, a.phi + o, a.E64 + o, a.E32 + o, a.dp + o, a.nx, a.dx,
+                           a.eps, pv[0], pv[1]);
 }
 
 // MCC on the target species.  mgroups workgroups per instance walk its
@@ -262,8 +259,7 @@ __global__ void __launch_bounds__(kEnsThreads, 5) k_ens_mcc(EnsArgs a, MccParams
     const EnsSpecies& sp = a.sp[a.target];
     const int64_t n = a.slots[a.target * a.M + inst];
     const float* E32 = a.E32 + static_cast<int64_t>(inst) * a.nx;
-    unsigned long long* counts = a.counts + static_cast<int64_t>(inst) * (kMaxChannels + 3);
-    double ewave[NCH] = {};
+    WaveTally<NCH> w{};
     for (int chunk = group; chunk < a.mchunks; chunk += a.mgroups) {
         const int64_t lo = static_cast<int64_t>(chunk) * kEnsMccChunk;
         const int64_t ci = static_cast<int64_t>(inst) * a.mchunks + chunk;
@@ -283,7 +279,7 @@ __global__ void __launch_bounds__(kEnsThreads, 5) k_ens_mcc(EnsArgs a, MccParams
                                       sp.v2 + p, sp.uid[p], E32, a.nx, sp.half_qm_dt, a.dt,
                                       a.seeds[inst], static_cast<uint32_t>(*a.step),
                                       a.prod + 12 * p, t, a.err + inst);
-        flush_tally_rows<NCH>(t, counts, active, ewave);
+        flush_tally_rows<NCH>(t, active, w);
         int total = 0;
         const int ex = block_exclusive_scan(np, scan_sh, total);
         if (in) {
@@ -292,12 +288,9 @@ __global__ void __launch_bounds__(kEnsThreads, 5) k_ens_mcc(EnsArgs a, MccParams
         }
         if (threadIdx.x == 0) a.chunk_tot[ci] = total;
     }
-    if ((threadIdx.x & 63) == 0) {
-        const int64_t w = (static_cast<int64_t>(inst) * a.mgroups + group) * (kEnsThreads / 64) +
-                          (threadIdx.x >> 6);
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) a.energy_rows[w * kMaxChannels + c] = ewave[c];
-    }
+    store_tally_row<NCH>(w, a.energy_rows,
+                         (static_cast<int64_t>(inst) * a.mgroups + group) * (kEnsThreads / 64) +
+                             (threadIdx.x >> 6));
 }
 
 // Per instance: chunk offsets, all-or-nothing capacity check
@@ -307,15 +300,12 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_products(EnsArgs a) {
     __shared__ int scan_sh[kEnsThreads];
     const int inst = blockIdx.x;
     {
-        // this step's MCC energy change: the instance's wave rows, in order
+        // this step's MCC tallies: the instance's wave rows, in order
         const int64_t nrows = static_cast<int64_t>(a.mgroups) * (kEnsThreads / 64);
-        const double* rows = a.energy_rows + static_cast<int64_t>(inst) * nrows * kMaxChannels;
-        double v[NCH] = {};
-        for (int64_t r = threadIdx.x; r < nrows; r += kEnsThreads)
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) v[c] += rows[r * kMaxChannels + c];
-        const double t = block_sum<NCH, kEnsThreads>(v);
-        if (threadIdx.x < NCH) a.ech[static_cast<int64_t>(inst) * kMaxChannels + threadIdx.x] += t;
+        fold_tally_rows<NCH, kEnsThreads>(
+            a.energy_rows + static_cast<int64_t>(inst) * nrows * kTallyStride, nrows,
+            a.ech + static_cast<int64_t>(inst) * kMaxChannels,
+            a.counts + static_cast<int64_t>(inst) * (kMaxChannels + 3));
     }
     if (a.sec < 0) return;
     const int per = (a.mchunks + kEnsThreads - 1) / kEnsThreads;
@@ -624,8 +614,6 @@ EnsArgs make_args(Context& c, Ensemble& e, double dt) {
     r.dz = c.d[2];
     r.dim = 1;
     r.vol = c.d[0];
-    a.cp = c.thomas_cp.ptr;
-    a.denom = c.thomas_inv.ptr;
     a.cs = c.dft_cos.ptr;
     a.sn = c.dft_sin.ptr;
     a.dx = c.d[0];
@@ -692,10 +680,9 @@ CompactArgs compact_args(Ensemble& e, int s) {
 
 // Queue one step (or the compaction) of every instance on the stream.
 void enqueue_step(Context& c, Ensemble& e, const EnsArgs& a, const MccParams& m) {
-    const size_t thomas = a.periodic ? 0 : thomas_lds_doubles(a.nx);
-    const size_t shmem = (a.periodic ? 3 * static_cast<size_t>(a.nx) : thomas) * sizeof(double);
+    const size_t shmem = (a.periodic ? 3 * static_cast<size_t>(a.nx) : 0) * sizeof(double);
     k_ens_push<true><<<push_grid(a), kEnsThreads, 0, c.stream>>>(a);
-    k_ens_field<<<a.M, kEnsThreads, shmem, c.stream>>>(a, 1, thomas > 0 ? 1 : 0);
+    k_ens_field<<<a.M, kEnsThreads, shmem, c.stream>>>(a, 1);
     if (a.target >= 0) {
         const int g = a.M * a.mchunks;
         with_nchan(m.nchan, [&]<int NCH>() {
@@ -876,7 +863,7 @@ int aurora_hip_ensemble_create(aurora_hip_ctx* h, int64_t instances, const int64
             const int64_t want = static_cast<int64_t>(c.cu_count) * 10;
             e->mgroups = static_cast<int>(std::clamp<int64_t>((want + M - 1) / M, 1, mch));
             e->energy_rows.alloc(static_cast<size_t>(M * e->mgroups * (kEnsThreads / 64) *
-                                                     kMaxChannels));
+                                                     kTallyStride));
             e->base.alloc(static_cast<size_t>(M * kBaseFields));
             AUR_HIP(hipMemsetAsync(e->base.ptr, 0, M * kBaseFields * 8, c.stream));
         }

@@ -124,89 +124,84 @@ __device__ inline void spectral_solve_1d(const double* rho, double* phi, double*
 }
 
 
-// 1D Dirichlet Thomas solve (FieldSolver.cpp:733-759): the coefficients
-// cp and denominators depend only on the index and are precomputed on the
-// host with the reference's operations; the data recurrence runs in one
-// lane, then the field is differenced in parallel.
-__device__ inline void thomas_solve_1d(const double* rho, double* phi, double* E, float* E32,
-                                const double* cp, const double* denom, double* dp, int n,
-                                double dx, double eps, double phi_left, double phi_right,
-                                double* sh = nullptr) {
-#pragma clang fp contract(off)
-    const int m = n - 2;
-    const double dx2 = dx * dx;
-    double* ph = phi;
-    if (sh) {
-        // sh (6 n doubles of LDS): the right-hand sides d_i (all lanes, the
-        // same operations as the serial form), cp, the denominators, dp and
-        // phi, so that thread 0's two sweeps wait on LDS only and their
-        // loads run ahead of the division chain
-        double* ds = sh;
-        double* cs = sh + n;
-        double* dn = sh + 2 * n;
-        double* dpv = sh + 3 * n;
-        ph = sh + 4 * n;
-        for (int i = threadIdx.x; i < m; i += blockDim.x) {
-            double di = rho[i + 1] * dx2 / eps;
-            if (i == 0) di += phi_left;
-            if (i == m - 1) di += phi_right;
-            ds[i] = di;
-            cs[i] = cp[i];
-            dn[i] = denom[i];
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            ph[0] = phi_left;
-            ph[n - 1] = phi_right;
-            double prev = ds[0] / 2.0;
-            dpv[0] = prev;
-#pragma unroll 8
-            for (int i = 1; i < m; ++i) {
-                prev = (ds[i] + prev) / dn[i];
-                dpv[i] = prev;
-            }
-            double nxt = prev;
-            ph[n - 2] = nxt;
-#pragma unroll 8
-            for (int ii = m - 2; ii >= 0; --ii) {
-                nxt = dpv[ii] - cs[ii] * nxt;
-                ph[ii + 1] = nxt;
-            }
-        }
-    } else if (threadIdx.x == 0) {
-        phi[0] = phi_left;
-        phi[n - 1] = phi_right;
-        double prev = 0.0;
-        for (int i = 0; i < m; ++i) {
-            double di = rho[i + 1] * dx2 / eps;
-            if (i == 0) di += phi_left;
-            if (i == m - 1) di += phi_right;
-            prev = (i == 0) ? di / 2.0 : (di + prev) / denom[i];
-            dp[i] = prev;
-        }
-        double nxt = dp[m - 1];
-        phi[n - 2] = nxt;
-        for (int ii = m - 1; ii-- > 0;) {
-            nxt = dp[ii] - cp[ii] * nxt;
-            phi[ii + 1] = nxt;
-        }
-    }
+// Exclusive prefix sum over the block of one double per thread (inclusive
+// Hillis-Steele in LDS, a fixed order of additions, then shifted by one);
+// total receives the block's sum.  Every thread of the block calls it.
+__device__ inline double block_exclusive_scan_f64(double x, double* sh, double& total) {
+    const int t = threadIdx.x, T = blockDim.x;
+    sh[t] = x;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double e;
-        if (i == 0) e = -(ph[1] - ph[0]) / dx;
-        else if (i == n - 1) e = -(ph[n - 1] - ph[n - 2]) / dx;
-        else e = -(ph[i + 1] - ph[i - 1]) / (2.0 * dx);
-        E[i] = e;
-        E32[i] = static_cast<float>(e);
-        if (sh) phi[i] = ph[i];
+    for (int d = 1; d < T; d <<= 1) {
+        const double y = t >= d ? sh[t - d] : 0.0;
+        __syncthreads();
+        sh[t] = sh[t] + y;
+        __syncthreads();
     }
+    total = sh[T - 1];
+    const double ex = t > 0 ? sh[t - 1] : 0.0;
+    __syncthreads();
+    return ex;
 }
 
-// LDS doubles thomas_solve_1d stages into (0: too large, global memory)
-inline size_t thomas_lds_doubles(int n) {
-    const size_t d = 6 * static_cast<size_t>(n);
-    return d * sizeof(double) <= 48 * 1024 ? d : 0;
+// 1D Dirichlet solve of -phi'' = rho / eps with phi(0) = phi_left and
+// phi(L) = phi_right (FieldSolver.cpp:733-759, the same system as the
+// reference's Thomas recurrence), by two prefix sums instead of a serial
+// recurrence.  With b_i = rho_i dx^2 / eps and the first differences
+// s_k = phi_{k+1} - phi_k, the interior equations 2 phi_i - phi_{i-1} -
+// phi_{i+1} = b_i give s_k = s_0 - B_k (B_k = b_1 + ... + b_k), the end
+// values fix s_0 = (phi_right - phi_left + sum_k B_k) / (n - 1), and
+// phi_{k+1} = phi_left + (k + 1) s_0 - (B_0 + ... + B_k).  E is the
+// reference's difference of phi (central inside, one-sided at the
+// electrodes), formed from s: -(s_i + s_{i-1}) / 2dx inside.  Each thread
+// owns a contiguous run of k; the runs are combined by two block scans, so
+// the sums are formed in one fixed order.  s: n - 1 doubles of global
+// scratch.  Any block size up to 1024 threads.
+__device__ inline void dirichlet_solve_1d(const double* rho, double* phi, double* E, float* E32,
+                                          double* s, int n, double dx, double eps,
+                                          double phi_left, double phi_right) {
+#pragma clang fp contract(off)
+    __shared__ double scan_sh[1024];
+    __shared__ double s0_sh;
+    const int T = blockDim.x, t = threadIdx.x;
+    const int N = n - 1;
+    const int seg = (N + T - 1) / T;
+    const int k0 = t * seg < N ? t * seg : N;
+    const int k1 = k0 + seg < N ? k0 + seg : N;
+    const double dx2 = dx * dx;
+    double loc = 0.0;
+    for (int k = k0; k < k1; ++k) loc += k < n - 2 ? rho[k + 1] * dx2 / eps : 0.0;
+    double btot = 0.0;
+    double B = block_exclusive_scan_f64(loc, scan_sh, btot);
+    double locc = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        s[k] = B;                          // B_k, replaced by s_k below
+        locc += B;
+        B += k < n - 2 ? rho[k + 1] * dx2 / eps : 0.0;
+    }
+    double ctot = 0.0;
+    double C = block_exclusive_scan_f64(locc, scan_sh, ctot);
+    if (t == 0) s0_sh = (phi_right - phi_left + ctot) / static_cast<double>(n - 1);
+    __syncthreads();
+    const double s0 = s0_sh;
+    for (int k = k0; k < k1; ++k) {
+        const double Bk = s[k];
+        C += Bk;
+        s[k] = s0 - Bk;
+        if (k + 1 < n - 1) phi[k + 1] = phi_left + static_cast<double>(k + 1) * s0 - C;
+    }
+    if (t == 0) {
+        phi[0] = phi_left;
+        phi[n - 1] = phi_right;
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += T) {
+        double e;
+        if (i == 0) e = -s[0] / dx;
+        else if (i == n - 1) e = -s[n - 2] / dx;
+        else e = -(s[i] + s[i - 1]) / (2.0 * dx);
+        E[i] = e;
+        E32[i] = static_cast<float>(e);
+    }
 }
 
 }  // namespace aurora

@@ -35,10 +35,10 @@ constexpr int kBlk = 256;
 template <int NCH>
 __global__ void __launch_bounds__(kBlk, 5) k_collide(MccParams m, float* v, int64_t count,
                                                   float dt, uint64_t seed, uint32_t stream,
-                                                  uint32_t step, TallyOut out, uint32_t* err) {
+                                                  uint32_t step, double* rows, uint32_t* err) {
     __shared__ MccLds lds;
     const XsView X = stage_tables(m, lds);
-    double ewave[NCH] = {};
+    WaveTally<NCH> w{};
     for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < count;
          base += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int64_t p = base + threadIdx.x;
@@ -54,9 +54,9 @@ __global__ void __launch_bounds__(kBlk, 5) k_collide(MccParams m, float* v, int6
             v[3 * p + 1] = u[1];
             v[3 * p + 2] = u[2];
         }
-        flush_tally_rows<NCH>(t, out.counts, active, ewave);
+        flush_tally_rows<NCH>(t, active, w);
     }
-    store_energy_rows<NCH>(ewave, out.energy_change);
+    store_wave_row<NCH>(w, rows);
 }
 
 struct Mcc1dArgs {
@@ -80,10 +80,10 @@ struct Mcc1dArgs {
 
 // sync -> collide -> rebuild v_half for every live target particle
 template <int NCH>
-__global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, TallyOut out) {
+__global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, double* rows) {
     __shared__ MccLds lds;
     const XsView X = stage_tables(m, lds);
-    double ewave[NCH] = {};
+    WaveTally<NCH> w{};
     for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < a.slots;
          base += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int64_t p = base + threadIdx.x;
@@ -99,21 +99,17 @@ __global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, Tally
             a.nprod[p] = mcc_1d_particle<NCH>(m, X, lds.chan, c, a.off[p], a.v0 + p, a.v1 + p, a.v2 + p, a.uid[p],
                                          a.E32, a.nx, a.half_qm_dt, a.dt, a.seed, a.step,
                                          a.prod + 12 * p, t, a.err);
-        flush_tally_rows<NCH>(t, out.counts, active, ewave);
+        flush_tally_rows<NCH>(t, active, w);
     }
-    store_energy_rows<NCH>(ewave, out.energy_change);
+    store_wave_row<NCH>(w, rows);
 }
 
-// Adds the per-wave energy rows of one launch, in wave order, to out[NCH].
+// Adds the tally rows of one launch, in row order, to the energy change
+// [NCH] and the counters [NCH + 3].
 template <int NCH>
 __global__ void __launch_bounds__(kBlk) k_fold_rows(const double* rows, int64_t nrows,
-                                                    double* out) {
-    double v[NCH] = {};
-    for (int64_t r = threadIdx.x; r < nrows; r += kBlk)
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) v[c] += rows[r * NCH + c];
-    const double s = block_sum<NCH, kBlk>(v);
-    if (threadIdx.x < NCH) out[threadIdx.x] += s;
+                                                    double* energy, unsigned long long* counts) {
+    fold_tally_rows<NCH, kBlk>(rows, nrows, energy, counts);
 }
 
 
@@ -131,15 +127,15 @@ __global__ void k_append_products(const int32_t* pcell, const float* poff,
 }
 
 // ------------------------------------------- allocation-free collide
-// aurora_hip_mcc_collide_device keeps its only scratch, one row of
-// per-channel energy sums per wave of its launch, in this array: at most
-// kNoAllocGroups workgroups, so kNoAllocRows rows (512 KiB per device, held
-// until the process ends).  Host code never passes the array's name to a
+// aurora_hip_mcc_collide_device keeps its only scratch, one tally row
+// (per-channel energy sums and counters) per wave of its launch, in this
+// array: at most kNoAllocGroups workgroups, so kNoAllocRows rows of
+// kTallyStride words (1.19 MiB per device, held until the process ends).  Host code never passes the array's name to a
 // kernel: it uses the device address resolved per device by
 // collide_scratch_init.
 constexpr int kNoAllocGroups = 2048;
 constexpr int64_t kNoAllocRows = static_cast<int64_t>(kNoAllocGroups) * (kBlk / 64);
-static __device__ __align__(256) double scratch[kNoAllocRows * kMaxChannels];
+static __device__ __align__(256) double scratch[kNoAllocRows * kTallyStride];
 
 constexpr int kMaxDevices = 64;
 std::mutex scratch_mutex;
@@ -328,13 +324,12 @@ void mcc_step(Context& c, double dt) {
         KernelScope ks(c, "mcc");
         const int g = blocks(c, n);
         const int64_t rows = energy_rows(g);
-        if (h.energy_rows.count < static_cast<size_t>(rows * kMaxChannels))
-            h.energy_rows.alloc(static_cast<size_t>(rows * kMaxChannels));
+        if (h.energy_rows.count < static_cast<size_t>(rows * kTallyStride))
+            h.energy_rows.alloc(static_cast<size_t>(rows * kTallyStride));
         with_nchan(m.nchan, [&]<int NCH>() {
-            k_mcc_1d<NCH><<<g, kBlk, 0, c.stream>>>(m, a,
-                                                     TallyOut{h.counts.ptr, h.energy_rows.ptr});
+            k_mcc_1d<NCH><<<g, kBlk, 0, c.stream>>>(m, a, h.energy_rows.ptr);
             k_fold_rows<NCH><<<1, kBlk, 0, c.stream>>>(h.energy_rows.ptr, rows,
-                                                       h.energy_change.ptr);
+                                                       h.energy_change.ptr, h.counts.ptr);
         });
         AUR_HIP(hipGetLastError());
     }
@@ -691,12 +686,13 @@ int aurora_hip_mcc_collide(aurora_hip_ctx* hctx, int64_t count, double* velocity
             KernelScope ks(c, "collide");
             const int g = blocks(c, count);
             DevBuf<double> rows;
-            rows.alloc(static_cast<size_t>(energy_rows(g) * kMaxChannels));
+            rows.alloc(static_cast<size_t>(energy_rows(g) * kTallyStride));
             with_nchan(m.nchan, [&]<int NCH>() {
                 k_collide<NCH><<<g, kBlk, 0, c.stream>>>(
                     m, dv.ptr, count, static_cast<float>(dt), h.seed, stream_id,
-                    static_cast<uint32_t>(step), TallyOut{cnt.ptr, rows.ptr}, c.dev_error.ptr);
-                k_fold_rows<NCH><<<1, kBlk, 0, c.stream>>>(rows.ptr, energy_rows(g), ech.ptr);
+                    static_cast<uint32_t>(step), rows.ptr, c.dev_error.ptr);
+                k_fold_rows<NCH><<<1, kBlk, 0, c.stream>>>(rows.ptr, energy_rows(g), ech.ptr,
+                                                           cnt.ptr);
             });
             AUR_HIP(hipGetLastError());
             AUR_HIP(hipStreamSynchronize(c.stream));
@@ -744,9 +740,10 @@ int aurora_hip_mcc_collide_device(aurora_hip_ctx* hctx, int64_t count, float* ve
             with_nchan(m.nchan, [&]<int NCH>() {
                 k_collide<NCH><<<gi, kBlk, 0, c.stream>>>(
                     m, velocity, count, static_cast<float>(dt), h.seed, stream_id,
-                    static_cast<uint32_t>(step),
-                    TallyOut{reinterpret_cast<unsigned long long*>(counts), rows}, status);
-                k_fold_rows<NCH><<<1, kBlk, 0, c.stream>>>(rows, energy_rows(gi), energy_change);
+                    static_cast<uint32_t>(step), rows, status);
+                k_fold_rows<NCH><<<1, kBlk, 0, c.stream>>>(
+                    rows, energy_rows(gi), energy_change,
+                    reinterpret_cast<unsigned long long*>(counts));
             });
         }
         k_collide_status<<<1, 64, 0, c.stream>>>(status);

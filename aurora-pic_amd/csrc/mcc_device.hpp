@@ -437,42 +437,88 @@ __device__ void flush_tally(const MccTally<NCH>& t, TallyOut out, bool active) {
     }
 }
 
-// flush_tally for kernels that keep their FP64 energy change deterministic:
-// counts go to the global counters (integer atomics), while the wave's energy
-// change per channel is added to ewave (the same value in every lane, summed
-// in a fixed order) for store_energy_rows.
+// The tallies of a launch without atomics: each wave sums its lanes'
+// counts and energy changes in a fixed order and, at the end, stores one
+// row: NCH energy sums (J, unweighted) at [0, NCH), then candidates, nulls,
+// per-channel collisions and secondaries as integers at [kMaxChannels,
+// kMaxChannels + NCH + 3).  k_fold_rows adds the rows in row order, so
+// counts are exact and energies repeat to the bit; and many waves never
+// contend for one counter.
+constexpr int kTallyStride = 2 * kMaxChannels + 3;
+
 template <int NCH>
-__device__ void flush_tally_rows(const MccTally<NCH>& t, unsigned long long* counts, bool active,
-                                 double (&ewave)[NCH]) {
-    unsigned long long cand = active ? t.candidates : 0, nul = active ? t.nulls : 0;
-    unsigned long long sec = active ? static_cast<unsigned long long>(t.nsec) : 0;
-    for (int off = 32; off > 0; off >>= 1) {
-        cand += __shfl_down(cand, off, 64);
-        nul += __shfl_down(nul, off, 64);
-        sec += __shfl_down(sec, off, 64);
-    }
-    const bool lead = (threadIdx.x & 63) == 0;
-    if (lead) {
-        if (cand) atomicAdd(counts, cand);
-        if (nul) atomicAdd(counts + 1, nul);
-        if (sec) atomicAdd(counts + 2 + NCH, sec);
-    }
+struct WaveTally {
+    double energy[NCH];
+    unsigned long long count[NCH + 3];
+};
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int NCH>
+__device__ void flush_tally_rows(const MccTally<NCH>& t, bool active, WaveTally<NCH>& w) {
+    w.count[0] += wave_sum_u64(active ? t.candidates : 0);
+    w.count[1] += wave_sum_u64(active ? t.nulls : 0);
+    w.count[2 + NCH] += wave_sum_u64(active ? static_cast<unsigned long long>(t.nsec) : 0);
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        unsigned long long n = active ? t.chan[c] : 0;
-        for (int off = 32; off > 0; off >>= 1) n += __shfl_down(n, off, 64);
-        if (lead && n) atomicAdd(counts + 2 + c, n);
-        ewave[c] += wave_sum(active ? static_cast<double>(t.energy_change[c]) : 0.0);
+        w.count[2 + c] += wave_sum_u64(active ? t.chan[c] : 0);
+        w.energy[c] += wave_sum(active ? static_cast<double>(t.energy_change[c]) : 0.0);
     }
 }
 
-// One row of NCH energy sums per wave of the launch, in wave order.
+// Stores the wave's row at rows + row * kTallyStride (lane 0).
 template <int NCH>
-__device__ __forceinline__ void store_energy_rows(const double (&ewave)[NCH], double* rows) {
+__device__ __forceinline__ void store_tally_row(const WaveTally<NCH>& w, double* rows,
+                                                int64_t row) {
     if ((threadIdx.x & 63) != 0) return;
-    const int64_t w = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    double* r = rows + row * kTallyStride;
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) rows[w * NCH + c] = ewave[c];
+    for (int c = 0; c < NCH; ++c) r[c] = w.energy[c];
+    unsigned long long* n = reinterpret_cast<unsigned long long*>(r + kMaxChannels);
+#pragma unroll
+    for (int k = 0; k < NCH + 3; ++k) n[k] = w.count[k];
+}
+
+// One row per wave of the launch, in wave order.
+template <int NCH>
+__device__ __forceinline__ void store_wave_row(const WaveTally<NCH>& w, double* rows) {
+    store_tally_row<NCH>(w, rows,
+                         static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6));
+}
+
+// Adds rows [0, nrows) in row order: energies into energy[NCH], counts into
+// counts[NCH + 3].  Block of BLOCK threads; every thread calls it.
+template <int NCH, int BLOCK>
+__device__ void fold_tally_rows(const double* rows, int64_t nrows, double* energy,
+                                unsigned long long* counts) {
+    double v[NCH] = {};
+    for (int64_t r = threadIdx.x; r < nrows; r += BLOCK)
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) v[c] += rows[r * kTallyStride + c];
+    const double s = block_sum<NCH, BLOCK>(v);
+    if (threadIdx.x < NCH) energy[threadIdx.x] += s;
+    // counts: integer sums, any order gives the same value
+    __shared__ unsigned long long cs[NCH + 3];
+    if (threadIdx.x < NCH + 3) cs[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long u[NCH + 3] = {};
+    for (int64_t r = threadIdx.x; r < nrows; r += BLOCK) {
+        const unsigned long long* n =
+            reinterpret_cast<const unsigned long long*>(rows + r * kTallyStride + kMaxChannels);
+#pragma unroll
+        for (int k = 0; k < NCH + 3; ++k) u[k] += n[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NCH + 3; ++k) {
+        const unsigned long long x = wave_sum_u64(u[k]);
+        if ((threadIdx.x & 63) == 0 && x) atomicAdd(cs + k, x);
+    }
+    __syncthreads();
+    if (threadIdx.x < NCH + 3) counts[threadIdx.x] += cs[threadIdx.x];
+    __syncthreads();
 }
 
 // One live 1D target particle of the MCC step (Simulation.cpp:900-926):

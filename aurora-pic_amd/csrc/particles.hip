@@ -369,10 +369,24 @@ __global__ void __launch_bounds__(kBlock) k_deposit(SpeciesView<R> sp, MeshView 
     }
 }
 
+// 1D meshes up to this many nodes deposit into LDS, flushed once per block
+// (a 1D mesh has few nodes, so global atomics would all land on them)
+constexpr int kLds1dNodes = 4096;
+
 template <int DIM, typename R, int BMODE, bool DEPOSIT>
 __global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
     const SpeciesView<R>& sp = a.sp;
     double led[3] = {0.0, 0.0, 0.0};   // 1D: left / right wall energy, electric work
+    constexpr bool kLds = DIM == 1 && DEPOSIT;
+    long long* acc = a.acc;
+    if constexpr (kLds) {
+        __shared__ long long lacc[kLds1dNodes];
+        if (a.m.nx <= kLds1dNodes) {
+            for (int i = threadIdx.x; i < a.m.nx; i += blockDim.x) lacc[i] = 0;
+            __syncthreads();
+            acc = lacc;
+        }
+    }
     for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < sp.slots;
          p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int c = sp.cell[p];
@@ -438,7 +452,17 @@ __global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
         if constexpr (DEPOSIT) {
             double o[3] = {static_cast<double>(off[0]), static_cast<double>(off[1]),
                            static_cast<double>(off[2])};
-            deposit_one<DIM>(a.acc, a.m, idx, o);
+            deposit_one<DIM>(acc, a.m, idx, o);
+        }
+    }
+    if constexpr (kLds) {
+        if (acc != a.acc) {
+            // the block's integer sums into the species accumulator
+            __syncthreads();
+            for (int i = threadIdx.x; i < a.m.nx; i += blockDim.x)
+                if (acc[i] != 0)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(a.acc + i),
+                              static_cast<unsigned long long>(acc[i]));
         }
     }
     if constexpr (DIM == 1) {
@@ -974,10 +998,18 @@ bool launch_tile3d(Context& c, Species& sp, double dt) {
            launch_tile3d_t<8, 8, 8, 1, 512, 5>(c, sp, a, mode);
 }
 
+// Workgroups of a push launch: 1D deposits flush one LDS copy of the mesh
+// per workgroup, so their launches are capped at two workgroups per CU.
+int push_blocks(const Context& c, int64_t slots, bool deposit) {
+    const int g = launch_blocks(c, slots);
+    if (c.dim == 1 && deposit && c.n[0] <= kLds1dNodes) return std::min(g, 2 * c.cu_count);
+    return g;
+}
+
 template <int DIM, typename R, int BMODE>
 void launch_push_b(Context& c, Species& sp, double dt, bool deposit) {
     auto a = push_args<R>(c, sp, dt);
-    const int g = launch_blocks(c, sp.slots);
+    const int g = push_blocks(c, sp.slots, deposit);
     if (deposit) {
         KernelScope ks(c, "push_deposit");
         k_push<DIM, R, BMODE, true><<<g, kBlock, 0, c.stream>>>(a);
@@ -1238,7 +1270,7 @@ void push_all(Context& c, double dt, bool fuse_deposit, bool defer_ledger) {
             launch_push<DIM, R>(c, sp, dt, fuse_deposit);
         });
         if (c.dim == 1) {
-            sp.ledger_push_blocks = launch_blocks(c, sp.slots);
+            sp.ledger_push_blocks = push_blocks(c, sp.slots, fuse_deposit);
             sp.ledger_pending = true;
             if (!defer_ledger) fold_ledger_1d(c, sp, nullptr, 0);
         }

@@ -244,13 +244,9 @@ __global__ void __launch_bounds__(1024) k_spectral_1d(const double* rho, double*
 
 // 1D Dirichlet Thomas solve (FieldSolver.cpp:733-759)
 __global__ void __launch_bounds__(256) k_thomas_1d(const double* rho, double* phi, double* E,
-                                                   float* E32, const double* cp,
-                                                   const double* denom, double* dp, int n,
-                                                   double dx, double eps, double phi_left,
-                                                   double phi_right, int use_lds) {
-    extern __shared__ double sh[];
-    thomas_solve_1d(rho, phi, E, E32, cp, denom, dp, n, dx, eps, phi_left, phi_right,
-                    use_lds ? sh : nullptr);
+                                                   float* E32, double* s, int n, double dx,
+                                                   double eps, double phi_left, double phi_right) {
+    dirichlet_solve_1d(rho, phi, E, E32, s, n, dx, eps, phi_left, phi_right);
 }
 
 AxisPlan plan_axis(const Context& c, int axis) {
@@ -885,19 +881,8 @@ void prepare_solver(Context& c) {
         AUR_HIP(hipMemcpy(c.dft_cos.ptr, cs.data(), n * sizeof(double), hipMemcpyHostToDevice));
         AUR_HIP(hipMemcpy(c.dft_sin.ptr, sn.data(), n * sizeof(double), hipMemcpyHostToDevice));
     } else if (c.dim == 1) {
-        const int n = static_cast<int>(c.n[0]);
-        const int m = n - 2;
-        std::vector<double> cp(m), denom(m, 2.0);
-        cp[0] = -1.0 / 2.0;
-        for (int i = 1; i < m; ++i) {
-            denom[i] = 2.0 + cp[i - 1];
-            cp[i] = (i == m - 1) ? 0.0 : -1.0 / denom[i];
-        }
-        c.thomas_cp.alloc(m);
-        c.thomas_inv.alloc(2 * m);  // denominators + dp scratch
-        AUR_HIP(hipMemcpy(c.thomas_cp.ptr, cp.data(), m * sizeof(double), hipMemcpyHostToDevice));
-        AUR_HIP(hipMemcpy(c.thomas_inv.ptr, denom.data(), m * sizeof(double),
-                          hipMemcpyHostToDevice));
+        // first-difference scratch of the prefix-sum solve
+        c.thomas_inv.alloc(static_cast<size_t>(c.n[0]));
     } else if (c.bc == AURORA_PERIODIC) {
         for (int a = 0; a < c.dim; ++a) {
             const int n = static_cast<int>(c.n[a]);
@@ -966,11 +951,9 @@ void solve_fields(Context& c, double phi_left, double phi_right, bool want_phi, 
                                                         c.eps);
         } else {
             KernelScope ks(c, "solve_thomas_1d");
-            const int m = n - 2;
-            const size_t lds = thomas_lds_doubles(n);
-            k_thomas_1d<<<1, 256, lds * sizeof(double), c.stream>>>(
-                c.rho.ptr, c.phi.ptr, c.E64.ptr, c.E32.ptr, c.thomas_cp.ptr, c.thomas_inv.ptr,
-                c.thomas_inv.ptr + m, n, c.d[0], c.eps, phi_left, phi_right, lds > 0 ? 1 : 0);
+            k_thomas_1d<<<1, 256, 0, c.stream>>>(c.rho.ptr, c.phi.ptr, c.E64.ptr, c.E32.ptr,
+                                                 c.thomas_inv.ptr, n, c.d[0], c.eps, phi_left,
+                                                 phi_right);
         }
         AUR_HIP(hipGetLastError());
         c.phi_valid = true;

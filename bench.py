@@ -99,6 +99,57 @@ def cpu_baseline(cfg, ppc_sample: int):
                       f"reference parallelises); host {cpu_model()}, nproc {os.cpu_count()}"}
 
 
+def discharge_leg(args):
+    """The 1D3V bounded RF discharge with MCC at its full size (513 nodes,
+    5e5 electrons + 5e5 ions, FP32, synthetic gas at 1e21 m^-3, ionization
+    products): particle-updates/s of the whole step (push, deposit, field
+    solve at t + dt, MCC and product placement), after a warm-up."""
+    from aurora_pic import hip
+    from aurora_pic import workloads as wl
+
+    cfg = wl.config_discharge()
+    dt = cfg["dt"]
+    ctx = hip.Context(0)
+    ctx.mesh(cfg["n"], cfg["L"], hip.DIRICHLET, cfg["eps"])
+    ctx.electrodes([0.0, 0.0, 0.0, 0.0], [0.0, cfg["phi_right_amplitude"], cfg["frequency"], 0.0])
+    sids = []
+    for k, s in enumerate(cfg["species"]):
+        sid = ctx.add_species(s["charge"], s["mass"], s["weight"], 4 * s["particles"])
+        ctx.init_maxwellian(sid, s["particles"],
+                            wl.maxwellian_sigma(s["temperature_ev"], s["mass"]), cfg["seed"] + k)
+        sids.append(sid)
+    ctx.mcc_load(sids[0], cfg["gas"], product_species=(sids[0], sids[1]), seed=cfg["seed"])
+    ctx.initialize(dt)
+    ctx.step(dt, args.discharge_warmup)
+    ctx.sync()
+    live0 = sum(ctx.count(sid)[1] for sid in sids)
+    t0 = time.perf_counter()
+    ctx.step(dt, args.discharge_steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    live1 = sum(ctx.count(sid)[1] for sid in sids)
+    ctx.timing(True)
+    ctx.step(dt, 20)
+    ctx.sync()
+    kernels = {}
+    for name in ["push_deposit", "push", "rho", "solve_thomas_1d", "ledger", "work", "mcc",
+                 "append_products", "clear_charge"]:
+        ms, n = ctx.kernel_time(name)
+        if n:
+            kernels[name] = {"avg_ms": ms / n, "launches": n}
+    ctx.timing(False)
+    counts, _ = ctx.mcc_stats()
+    ctx.close()
+    updates = 0.5 * (live0 + live1) * args.discharge_steps
+    return {"workload": "1D3V RF discharge: 513 nodes, 5e5 electrons + 5e5 ions (FP32), 200 V "
+                        "at 13.56 MHz, synthetic gas at 1e21 m^-3 with ionization products",
+            "particle_updates_per_s": updates / wall,
+            "ms_per_step": 1e3 * wall / args.discharge_steps,
+            "steps": args.discharge_steps, "warmup": args.discharge_warmup,
+            "live_particles": [live0, live1], "ionizations": int(counts[4]),
+            "kernels_ms": kernels}
+
+
 def serving_leg(args):
     """Batched swarm serving: Poisson arrivals at 200 req/s (E/N log-uniform
     1-1000 Td, log-normal particle counts, synthetic gas), admitted by HBM
@@ -198,6 +249,9 @@ def main():
                     help="fixed tile shape tx,ty,tz,halo (skips the autotuner)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--no-discharge", action="store_true")
+    ap.add_argument("--discharge-steps", type=int, default=1000)
+    ap.add_argument("--discharge-warmup", type=int, default=100)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-full", action="store_true",
                     help="all 4,096 requests x 20,000 steps (minutes)")
@@ -305,6 +359,15 @@ def main():
                 if line.startswith("bytes_per_launch:"):
                     traffic = int(line.split(":", 1)[1])
 
+    discharge_out = None
+    if not args.no_discharge:
+        try:
+            discharge_out = discharge_leg(args)
+            log("discharge:", json.dumps(discharge_out))
+        except Exception as e:  # a secondary leg must not hide the headline result
+            log("discharge leg failed:", repr(e))
+            discharge_out = {"error": repr(e)}
+
     serving_out = None
     if not args.no_serving:
         try:
@@ -346,6 +409,7 @@ def main():
                      "kernel": "push_deposit (gather+Boris+drift+wrap+int64 deposit)",
                      "bytes_per_update": BYTES_PER_UPDATE},
         "cpu_baseline": base,
+        "discharge_1d3v": discharge_out,
         "serving": serving_out,
     }
     print(json.dumps(out), flush=True)

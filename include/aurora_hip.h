@@ -305,8 +305,9 @@ int aurora_hip_mcc_collide(aurora_hip_ctx* ctx, int64_t count,
  * (nchan + 3: candidates, nulls, per channel, secondaries), energy_change
  * (nchan, J, unweighted) and status (an aurora_status: frequency bound,
  * candidate limit or non-finite velocity).  Its only scratch is a
- * fixed per-device array of 2048 x 4 rows of 8 doubles (512 KiB), which
- * bounds the launch to 2048 workgroups (particles are walked grid-stride);
+ * fixed per-device array of 2048 x 4 tally rows of 19 words (1.19 MiB),
+ * which bounds the launch to 2048 workgroups (particles are walked
+ * grid-stride);
  * count is at most 4294967295.  Calls that overlap on one GPU share that
  * array and race. */
 int aurora_hip_mcc_collide_device(aurora_hip_ctx* ctx, int64_t count,

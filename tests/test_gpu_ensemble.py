@@ -18,9 +18,9 @@ DT = 1.0 / (400 * 13.56e6)
 SPECIES = [(-workloads.EV, workloads.ME, 8.0), (workloads.EV, 6.67e-27, 0.026)]
 
 
-def _template(hip, gas, capacity=4 * N, mcc_seed=1):
+def _template(hip, gas, capacity=4 * N, mcc_seed=1, nx=NX):
     c = hip.Context(0)
-    c.mesh([NX], [L], hip.DIRICHLET, workloads.EPS0)
+    c.mesh([nx], [L], hip.DIRICHLET, workloads.EPS0)
     c.electrodes([0, 0, 0, 0], [0.0, 200.0, 13.56e6, 0.0])
     sids = [c.add_species(q, m, 2.56e14 * L / N, capacity) for q, m, _ in SPECIES]
     c.mcc_load(sids[0], gas, product_species=(sids[0], sids[1]), seed=mcc_seed)
@@ -33,8 +33,8 @@ def _init_instance(c, sids, k):
     c.initialize(DT)
 
 
-def _alone(hip, gas, k, mcc_seed, steps):
-    c, sids = _template(hip, gas, mcc_seed=mcc_seed)
+def _alone(hip, gas, k, mcc_seed, steps, nx=NX):
+    c, sids = _template(hip, gas, mcc_seed=mcc_seed, nx=nx)
     _init_instance(c, sids, k)
     c.step(DT, steps)
     out = [c.download(s) for s in sids]
@@ -51,8 +51,8 @@ def _by_uid(pos, vel, alive, uid):
     return uid[m][order], pos[m][order], vel[m][order]
 
 
-def _build_ensemble(hip, gas, instances, graph_steps, capacity=4 * N):
-    c, sids = _template(hip, gas, capacity)
+def _build_ensemble(hip, gas, instances, graph_steps, capacity=4 * N, nx=NX):
+    c, sids = _template(hip, gas, capacity, nx=nx)
     ens = c.ensemble_create(instances, [capacity, capacity])
     ens.graph(graph_steps)
     for k in range(instances):
@@ -95,6 +95,27 @@ def test_instances_equal_single_context_runs():
             assert np.array_equal(absorbed[sid], ref_n[:2].astype(np.uint64))
             assert np.allclose(absorbed_e[sid], ref_e, rtol=1e-12, atol=0.0), (absorbed_e, ref_e)
         assert absorbed[0].sum() > 0 and absorbed_e[0].sum() > 0   # electrons reached the walls
+    c.close()
+
+
+def test_instances_equal_lone_runs_on_a_long_mesh():
+    # 1500 nodes: the field step's Dirichlet solve on a mesh beyond 1024
+    # nodes, in the ensemble and on a context
+    from aurora_pic import hip
+    gas = workloads.synthetic_gas(rows=1000, neutral_density=1.0e22)
+    c, sids, ens = _build_ensemble(hip, gas, 2, 20, nx=1500)
+    ens.step(DT, 40)
+    for k in (0, 1):
+        ref, ref_fields, ref_counts, _, _ = _alone(hip, gas, k, 500 + k, 40, nx=1500)
+        for sid in sids:
+            a = _by_uid(*ens.download(k, sid))
+            b = _by_uid(*ref[sid])
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
+        rho, phi, E = ens.fields(k)
+        assert np.array_equal(phi, ref_fields[1])
+        assert np.array_equal(E, ref_fields[2][0])
+        assert np.array_equal(ens.stats(k, 3)[0], ref_counts)
     c.close()
 
 

@@ -87,6 +87,56 @@ def test_field_solve_matches_reference_golden(hip_ctx, case):
     assert np.max(np.abs(phi - g["phi"])) <= 1e-10 * np.max(np.abs(g["phi"]))
 
 
+def _dirichlet_long_double(rho, dx, eps, left, right):
+    """The reference's Thomas recurrence (FieldSolver.cpp:733-759) in numpy
+    long double (the x87 80-bit format on this host): the reference's
+    algorithm with a 64-bit mantissa, a yardstick for an FP64 solve."""
+    ld = np.longdouble
+    n = len(rho)
+    m = n - 2
+    d = rho[1:n - 1].astype(ld) * (ld(dx) * ld(dx)) / ld(eps)
+    d[0] += ld(left)
+    d[m - 1] += ld(right)
+    cp = np.zeros(m, dtype=ld)
+    dp = np.zeros(m, dtype=ld)
+    cp[0], dp[0] = ld(-0.5), d[0] / 2
+    for i in range(1, m):
+        den = ld(2) + cp[i - 1]
+        cp[i] = ld(0) if i == m - 1 else ld(-1) / den
+        dp[i] = (d[i] + dp[i - 1]) / den
+    phi = np.zeros(n, dtype=ld)
+    phi[0], phi[n - 1], phi[n - 2] = left, right, dp[m - 1]
+    for i in range(m - 2, -1, -1):
+        phi[i + 1] = dp[i] - cp[i] * phi[i + 2]
+    E = np.zeros(n, dtype=ld)
+    E[1:n - 1] = -(phi[2:] - phi[:n - 2]) / (2 * ld(dx))
+    E[0] = -(phi[1] - phi[0]) / ld(dx)
+    E[n - 1] = -(phi[n - 1] - phi[n - 2]) / ld(dx)
+    return phi, E
+
+
+@pytest.mark.parametrize("n", [513, 1500, 4097])
+def test_dirichlet_solve_large_meshes(oracle, hip_ctx, n):
+    """The 1D Dirichlet solve (prefix sums in place of the reference's
+    serial recurrence) on meshes up to 4097 nodes: within 1e-12 of the
+    reference's recurrence in 80-bit arithmetic, and within 1e-10 of the
+    oracle's FP64 recurrence (whose own rounding grows with n)."""
+    print(f"n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    L, eps = 0.067, 8.8541878128e-12
+    rho = rng.normal(0.0, 1e-6, n) + 2e-7 * np.sin(np.linspace(0, 7, n))
+    hip_ctx.mesh([n], [L], 1, eps)
+    hip_ctx.set_rho(rho)
+    hip_ctx.solve(0.0, 173.0, True)
+    _, phi, E = hip_ctx.fields()
+    ref_phi, ref_E = _dirichlet_long_double(rho, L / (n - 1), eps, 0.0, 173.0)
+    assert np.max(np.abs(phi - ref_phi)) <= 1e-12 * float(np.max(np.abs(ref_phi)))
+    assert np.max(np.abs(E[0] - ref_E)) <= 1e-12 * float(np.max(np.abs(ref_E)))
+    o_phi, o_E = oracle.poisson([n], [L], 1, eps, rho, 0.0, 173.0)
+    assert np.max(np.abs(phi - o_phi)) <= 1e-10 * np.max(np.abs(o_phi))
+    assert np.max(np.abs(E[0] - o_E)) <= 1e-10 * np.max(np.abs(o_E))
+
+
 def test_field_solve_3d_analytic_mode(hip_ctx):
     # test_core.cpp:1360-1438 on a 32^3 power-of-two mesh
     n, L = [32, 32, 32], [1.0, 1.25, 1.5]
