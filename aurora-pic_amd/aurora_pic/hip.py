@@ -22,6 +22,7 @@ ELASTIC, EXCITATION, IONIZATION, ATTACHMENT, CHARGE_EXCHANGE = range(5)
 SCATTER_ISOTROPIC, SCATTER_BACKWARD, SCATTER_HENYEY_GREENSTEIN = range(3)
 FRAME_PROJECTILE, FRAME_CENTER_OF_MASS = range(2)
 SORT_FIXED, SORT_ADAPTIVE = range(2)
+SORT_LOCAL, SORT_RADIX = range(2)
 (OK, ERR_INVALID_ARGUMENT, ERR_HIP, ERR_RATE_ABOVE_BOUND, ERR_CANDIDATE_LIMIT, ERR_CAPACITY,
  ERR_B_OUT_OF_RANGE, ERR_ENERGY_ABOVE_MAX, ERR_UNSUPPORTED, ERR_NOT_FOUND,
  ERR_POPULATION) = range(11)
@@ -149,6 +150,7 @@ def load(path: str = LIB_PATH):
         "aurora_hip_set_sort_interval": [ctx, C.c_int64],
         "aurora_hip_set_sort_threshold": [ctx, C.c_double],
         "aurora_hip_set_sort_policy": [ctx, C.c_int, C.c_int64],
+        "aurora_hip_set_sort_method": [ctx, C.c_int],
         "aurora_hip_checkpoint_save": [ctx, C.c_char_p],
         "aurora_hip_checkpoint_load": [ctx, C.c_char_p],
         "aurora_hip_physics_signature": [ctx, _u64p],
@@ -159,6 +161,13 @@ def load(path: str = LIB_PATH):
                                 C.POINTER(_dp), C.c_double, C.c_double, C.c_int64, C.c_double,
                                 C.c_double, _ip, C.c_uint64],
         "aurora_hip_mcc_stats": [ctx, _u64p, _dp],
+        "aurora_hip_mcc_add": [ctx, C.c_int, C.c_int, _ip, _dp, _i64p, C.POINTER(_dp),
+                               C.POINTER(_dp), C.c_double, C.c_double, C.c_int64, C.c_double,
+                               C.c_double, _ip, C.c_uint64, _ip],
+        "aurora_hip_mcc_model_options": [ctx, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64,
+                                         _dp, _dp],
+        "aurora_hip_mcc_model_stats": [ctx, C.c_int, _u64p, _dp],
+        "aurora_hip_set_velocity_dimensions": [ctx, C.c_int],
         "aurora_hip_mcc_channel_options": [ctx, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64,
                                            _dp, _dp],
         "aurora_hip_mcc_collide": [ctx, C.c_int64, _dp, C.c_double, C.c_uint64, C.c_uint32,
@@ -169,6 +178,8 @@ def load(path: str = LIB_PATH):
                                  C.c_double, C.c_double, C.c_int64, C.c_double, C.c_double],
         "aurora_hip_swarm_submit": [ctx, C.POINTER(SwarmRequest), C.c_uint64, _i64p],
         "aurora_hip_swarm_advance": [ctx, C.c_int64, _i64p],
+        "aurora_hip_swarm_launch": [ctx, C.c_int64, _i64p],
+        "aurora_hip_swarm_collect": [ctx, C.c_int, _ip],
         "aurora_hip_swarm_poll": [ctx, C.c_int64, _ip, C.POINTER(SwarmResult)],
         "aurora_hip_swarm_branching": [ctx, C.POINTER(SwarmRequest), C.c_int64,
                                        C.POINTER(SwarmResult), C.POINTER(SwarmGrowth)],
@@ -184,6 +195,7 @@ def load(path: str = LIB_PATH):
         "aurora_hip_ensemble_time": [ctx, C.c_int64, _dp, _i64p],
         "aurora_hip_ensemble_destroy": [ctx, C.c_int64],
         "aurora_hip_kernel_timing": [ctx, C.c_int],
+        "aurora_hip_stream_bandwidth": [ctx, C.c_int, C.c_int64, C.c_int, _dp],
         "aurora_hip_kernel_time": [ctx, C.c_char_p, _dp, _i64p],
     }
     for name, args in sig.items():
@@ -434,10 +446,23 @@ class Context:
         species, when its timed push exceeds its cycle's average cost)."""
         self._check(self.lib.aurora_hip_set_sort_policy(self.h, policy, max_interval))
 
+    def sort_method(self, method):
+        """SORT_LOCAL (counting sort of a tile-sorted species, the default)
+        or SORT_RADIX (always the full radix sort)."""
+        self._check(self.lib.aurora_hip_set_sort_method(self.h, method))
+
     def sort(self):
         self._check(self.lib.aurora_hip_sort(self.h))
 
     # ---- timing
+    def stream_bandwidth(self, pattern, nbytes=2 << 30, repeats=5):
+        """Attainable GB/s of a hand-written streaming kernel (0: copy, 1:
+        seven arrays read and rewritten in place)."""
+        g = C.c_double()
+        self._check(self.lib.aurora_hip_stream_bandwidth(self.h, pattern, nbytes, repeats,
+                                                         C.byref(g)))
+        return g.value
+
     def timing(self, enable=True):
         self._check(self.lib.aurora_hip_kernel_timing(self.h, int(enable)))
 
@@ -479,6 +504,31 @@ class Context:
         self._check(self.lib.aurora_hip_mcc_channel_options(self.h, model, channel, angular,
                                                             frame, len(e), _p(e, _dp),
                                                             _p(v, _dp)))
+
+    def mcc_add(self, target, gas, product_species=(-1, -1), seed=1, neutral_temperature=0.0):
+        """A further named null-collision model (after mcc_load); returns its
+        model index."""
+        nch, kinds, thr, rows, ep, sp = self._tables(gas["channels"])
+        prod = (C.c_int * 2)(*product_species)
+        model = C.c_int()
+        self._check(self.lib.aurora_hip_mcc_add(
+            self.h, target, nch, kinds, _p(thr, _dp), _p(rows, _i64p), ep, sp,
+            gas["neutral_density"], gas["max_frequency"], gas.get("max_candidates", 64),
+            gas["neutral_mass"], neutral_temperature, prod, seed, C.byref(model)))
+        self._model_nchan = getattr(self, "_model_nchan", {})
+        self._model_nchan[model.value] = nch
+        return model.value
+
+    def mcc_model_stats(self, model):
+        nch = self._nchan if model == 0 else self._model_nchan[model]
+        counts = np.zeros(3 + nch, dtype=np.uint64)
+        ech = np.zeros(nch)
+        self._check(self.lib.aurora_hip_mcc_model_stats(self.h, model, _p(counts, _u64p),
+                                                        _p(ech, _dp)))
+        return counts, ech
+
+    def velocity_dimensions(self, dims):
+        self._check(self.lib.aurora_hip_set_velocity_dimensions(self.h, dims))
 
     def mcc_stats(self):
         counts = np.zeros(3 + self._nchan, dtype=np.uint64)
@@ -522,6 +572,19 @@ class Context:
         a = C.c_int64()
         self._check(self.lib.aurora_hip_swarm_advance(self.h, max_steps, C.byref(a)))
         return a.value
+
+    def swarm_launch(self, max_steps):
+        """Queues one swarm launch and returns at once (the number of
+        requests still admitted)."""
+        a = C.c_int64()
+        self._check(self.lib.aurora_hip_swarm_launch(self.h, max_steps, C.byref(a)))
+        return a.value
+
+    def swarm_collect(self, wait=True):
+        """Completes (wait) or checks the launch in flight; True while it runs."""
+        busy = C.c_int()
+        self._check(self.lib.aurora_hip_swarm_collect(self.h, 1 if wait else 0, C.byref(busy)))
+        return bool(busy.value)
 
     def swarm_poll(self, handle):
         done = C.c_int()

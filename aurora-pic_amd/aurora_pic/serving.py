@@ -4,8 +4,9 @@ Requests arrive over time (each with an arrival timestamp).  The loop admits
 an arrived request as soon as its HBM footprint fits the budget
 (``aurora_hip_swarm_submit`` refuses it with AURORA_ERR_CAPACITY otherwise,
 and it stays queued), advances every admitted request together by a bounded
-number of steps per launch (``aurora_hip_swarm_advance``: one persistent
-kernel over the chunks of all active requests), and collects results as
+number of steps per launch (``aurora_hip_swarm_launch``: one persistent
+kernel over the chunks of all active requests, queued without waiting, so
+other device work and admissions overlap it), and collects results as
 requests finish.  State stays on the device from admission to the result.
 
 A request's result does not depend on what it is batched with (integer
@@ -122,8 +123,12 @@ def serve(ctx, requests, arrivals_s, hbm_budget, steps_per_launch=1000, clock=ti
     work_item_limit (particles x steps) is rejected with the reference's
     message; the default is the reference's 1e8, and a GPU deployment raises
     it explicitly.
-    on_launch: optional callable run after every swarm launch (e.g. to
-    interleave other device work).  Returns a ServingReport."""
+
+    Swarm launches are queued without waiting (aurora_hip_swarm_launch), so
+    while one runs the loop keeps admitting arrivals (they join the next
+    launch) and calls on_launch, e.g. to step a 1D ensemble on its own
+    context and stream beside the swarms; it then collects the launch and
+    the requests it finished.  Returns a ServingReport."""
     order = np.argsort(np.asarray(arrivals_s), kind="stable")
     pending = collections.deque(Served(int(i), float(arrivals_s[i]), requests[i]) for i in order)
     queued = collections.deque()
@@ -131,19 +136,19 @@ def serve(ctx, requests, arrivals_s, hbm_budget, steps_per_launch=1000, clock=ti
     served = []
     launches = max_active = refused = 0
     electron_steps = 0
+    in_flight = False
     t0 = clock()
-    while pending or queued or active:
+
+    def admit():
+        nonlocal refused
         now = clock() - t0
         while pending and pending[0].arrival_s <= now:
             s = pending.popleft()
             fp = ctx.swarm_footprint(s.request)
             s.rejected = work_items_exceeded(s.request, work_item_limit)
-            if s.rejected:
-                s.done_s = now
-                served.append(s)
-                continue
-            if hbm_budget and fp > hbm_budget:
+            if not s.rejected and hbm_budget and fp > hbm_budget:
                 s.rejected = f"footprint {fp} B exceeds the HBM budget of {hbm_budget} B"
+            if s.rejected:
                 s.done_s = now
                 served.append(s)
                 continue
@@ -164,22 +169,31 @@ def serve(ctx, requests, arrivals_s, hbm_budget, steps_per_launch=1000, clock=ti
             s.admit_s = clock() - t0
             active[h] = s
             queued.popleft()
-        if not active:
-            if pending:
-                wait = pending[0].arrival_s - (clock() - t0)
-                if wait > 0:
-                    time.sleep(min(wait, 0.05))
-            continue
-        max_active = max(max_active, len(active))
-        for s in active.values():
-            done_steps = getattr(s, "_steps", 0)
-            k = min(steps_per_launch, s.request.steps - done_steps)
-            s._steps = done_steps + k
-            electron_steps += k * s.request.particles
-        ctx.swarm_advance(steps_per_launch)
-        launches += 1
+
+    while pending or queued or active:
+        admit()
+        if not in_flight:
+            runnable = [s for s in active.values() if getattr(s, "_steps", 0) < s.request.steps]
+            if not runnable:
+                if pending and not active:
+                    wait = pending[0].arrival_s - (clock() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.05))
+                continue
+            max_active = max(max_active, len(runnable))
+            for s in runnable:
+                done_steps = getattr(s, "_steps", 0)
+                k = min(steps_per_launch, s.request.steps - done_steps)
+                s._steps = done_steps + k
+                electron_steps += k * s.request.particles
+            ctx.swarm_launch(steps_per_launch)
+            launches += 1
+            in_flight = True
         if on_launch is not None:
             on_launch()
+        in_flight = ctx.swarm_collect(wait=on_launch is None)
+        if in_flight:
+            continue
         t_done = clock() - t0
         for h in list(active):
             done, r = ctx.swarm_poll(h)

@@ -329,6 +329,7 @@ int aurora_hip_mesh_bind(aurora_hip_ctx* h, int dim, const int64_t* n, const dou
         AUR_HIP(hipStreamSynchronize(c.stream));
         c.species.clear();
         c.mcc.loaded = false;
+        c.mcc_more.clear();
         c.dim = dim;
         for (int a = 0; a < 3; ++a) {
             c.n[a] = a < dim ? n[a] : 1;
@@ -707,6 +708,14 @@ int aurora_hip_set_sort_policy(aurora_hip_ctx* h, int policy, int64_t max_interv
                     "sort max_interval must be non-negative");
         c.sort_policy = policy;
         c.sort_max_interval = max_interval;
+    });
+}
+
+int aurora_hip_set_sort_method(aurora_hip_ctx* h, int method) {
+    return guarded(h, [&](Context& c) {
+        AUR_REQUIRE(method == AURORA_SORT_LOCAL || method == AURORA_SORT_RADIX,
+                    AURORA_ERR_INVALID_ARGUMENT, "sort method must be local or radix");
+        c.sort_method = method;
     });
 }
 

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -167,6 +168,8 @@ struct Species {
     bool ledger_pending = false;          // push partials not yet folded
     DevBuf<int32_t> tile_start;           // ntiles + 1 segment starts after a sort
     bool tiled = false;                   // storage is tile-sorted (tile_start valid)
+    int64_t sorted_slots = -1;            // slots right after the last sort
+    int64_t tile_keyspace = -1;           // tiles tile_start describes
     int64_t pushes_since_sort = 0;        // tiled pushes counted in absorbed[6]
     // adaptive sort policy: measured cost of the current sort cycle.  Pushes
     // are timed into two event pairs in turn and read one step later, when
@@ -278,6 +281,7 @@ struct Context {
     int64_t sort_interval = 0;
     double sort_threshold = 0.0;   // fixed policy: skip a species whose pushes left its tile box less often
     int sort_policy = AURORA_SORT_FIXED;
+    int sort_method = 0;           // 0: local counting sort when tile-sorted, 1: always radix
     int64_t sort_max_interval = 0; // adaptive policy: cap on steps between sorts (0 = none)
     int tile[3] = {8, 8, 8};
     int halo = 2;
@@ -287,9 +291,27 @@ struct Context {
     DevBuf<int32_t> tile_count, tile_offset;
     DevBuf<unsigned> sort_keys, sort_keys2, sort_vals, sort_vals2;
     DevBuf<uint8_t> sort_temp;
+    // local counting sort (tile-sorted species): per old tile segment the
+    // counts by neighbour direction, their placement, the particles that
+    // moved further (staged, then placed by destination) and a summary
+    DevBuf<uint32_t> ls_count, ls_offset, ls_far_base, ls_far_count, ls_new_count, ls_summary;
+    DevBuf<int32_t> ls_new_start;
+    DevBuf<unsigned> ls_far_key, ls_far_key2, ls_far_val, ls_far_val2;
+    DevBuf<uint8_t> ls_stage;
 
     std::vector<Species> species;
-    MccModelHost mcc;
+    MccModelHost mcc;                          // null-collision model 0 of the step
+    // further named models (Simulation.cpp:405-520), applied after model 0
+    // in order, each on its own target species and random stream
+    std::vector<std::unique_ptr<MccModelHost>> mcc_more;
+    // 1: 1D1V particles (the scalar collide), 3: 1D3V (Config::velocity_dimensions)
+    int velocity_dims = 3;
+
+    int mcc_models() const { return mcc.loaded ? 1 + static_cast<int>(mcc_more.size()) : 0; }
+    MccModelHost& mcc_model(int k) { return k == 0 ? mcc : *mcc_more[static_cast<size_t>(k - 1)]; }
+    const MccModelHost& mcc_model(int k) const {
+        return k == 0 ? mcc : *mcc_more[static_cast<size_t>(k - 1)];
+    }
 
     // swarm batch state lives in swarm.hip, batched 1D ensembles in ensemble.hip
     void* swarm = nullptr;

@@ -27,7 +27,7 @@ namespace aurora {
 namespace {
 
 constexpr char kMagic[16] = "AURORA-HIP-CKPT";
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;   // 2: every null-collision model, velocity dimensions
 
 struct Fnv {
     uint64_t h = 14695981039346656037ULL;
@@ -114,9 +114,10 @@ uint64_t physics_signature(const Context& c) {
         f.pod(sp.weight);
         f.pod(sp.precision);
     }
-    const MccModelHost& m = c.mcc;
-    f.pod(m.loaded);
-    if (m.loaded) {
+    f.pod(c.velocity_dims);
+    f.pod(c.mcc_models());
+    for (int model = 0; model < c.mcc_models(); ++model) {
+        const MccModelHost& m = c.mcc_model(model);
         f.pod(m.target);
         f.pod(m.nchan);
         for (int k = 0; k < m.nchan; ++k) {
@@ -166,11 +167,10 @@ void checkpoint_save(Context& c, const std::string& path) {
         write_pod(out, has_ledger);
         if (has_ledger) dump(out, sp.ledger.ptr, 3 * sizeof(double));
     }
-    const bool mcc = c.mcc.loaded;
-    write_pod(out, mcc);
-    if (mcc) {
-        dump(out, c.mcc.counts.ptr, (kMaxChannels + 3) * sizeof(unsigned long long));
-        dump(out, c.mcc.energy_change.ptr, kMaxChannels * sizeof(double));
+    write_pod(out, c.mcc_models());
+    for (int model = 0; model < c.mcc_models(); ++model) {
+        dump(out, c.mcc_model(model).counts.ptr, (kMaxChannels + 3) * sizeof(unsigned long long));
+        dump(out, c.mcc_model(model).energy_change.ptr, kMaxChannels * sizeof(double));
     }
     out.flush();
     AUR_REQUIRE(out.good(), AURORA_ERR_INVALID_ARGUMENT,
@@ -224,12 +224,13 @@ void checkpoint_load(Context& c, const std::string& path) {
         sp.pushes_since_sort = 0;
         sp.acc_next_clear = false;
     }
-    const bool mcc = read_pod<bool>(in, path);
-    AUR_REQUIRE(mcc == c.mcc.loaded, AURORA_ERR_INVALID_ARGUMENT,
-                "checkpoint MCC state does not match the context");
-    if (mcc) {
-        restore(in, c.mcc.counts.ptr, (kMaxChannels + 3) * sizeof(unsigned long long), path);
-        restore(in, c.mcc.energy_change.ptr, kMaxChannels * sizeof(double), path);
+    const int models = read_pod<int>(in, path);
+    AUR_REQUIRE(models == c.mcc_models(), AURORA_ERR_INVALID_ARGUMENT,
+                "checkpoint collision models do not match this context");
+    for (int model = 0; model < models; ++model) {
+        restore(in, c.mcc_model(model).counts.ptr, (kMaxChannels + 3) * sizeof(unsigned long long),
+                path);
+        restore(in, c.mcc_model(model).energy_change.ptr, kMaxChannels * sizeof(double), path);
     }
     c.time = time;
     c.step_count = step;

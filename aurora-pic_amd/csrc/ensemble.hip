@@ -785,6 +785,8 @@ int aurora_hip_ensemble_create(aurora_hip_ctx* h, int64_t instances, const int64
                                int64_t* handle) {
     return guarded(h, [&](Context& c) {
         AUR_REQUIRE(c.dim == 1, AURORA_ERR_UNSUPPORTED, "ensembles are built for 1D meshes");
+        AUR_REQUIRE(c.mcc_models() <= 1 && c.velocity_dims == 3, AURORA_ERR_UNSUPPORTED,
+                    "ensembles run 1D3V instances with at most one null-collision model");
         AUR_REQUIRE(instances > 0 && instances <= 65536, AURORA_ERR_INVALID_ARGUMENT,
                     "ensemble instance count must be in [1, 65536]");
         const int S = static_cast<int>(c.species.size());

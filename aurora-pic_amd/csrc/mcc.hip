@@ -35,7 +35,8 @@ constexpr int kBlk = 256;
 template <int NCH>
 __global__ void __launch_bounds__(kBlk, 5) k_collide(MccParams m, float* v, int64_t count,
                                                   float dt, uint64_t seed, uint32_t stream,
-                                                  uint32_t step, double* rows, uint32_t* err) {
+                                                  uint32_t step, double* rows, uint32_t* err,
+                                                  int one_v) {
     __shared__ MccLds lds;
     const XsView X = stage_tables(m, lds);
     WaveTally<NCH> w{};
@@ -48,7 +49,10 @@ __global__ void __launch_bounds__(kBlk, 5) k_collide(MccParams m, float* v, int6
             float u[3] = {v[3 * p], v[3 * p + 1], v[3 * p + 2]};
             Philox rng;
             rng.init(seed, stream, static_cast<uint32_t>(p), step);
-            mcc_collide<NCH>(m, X, lds.chan, u, dt, rng, t);
+            if (one_v)
+                mcc_collide_1v<NCH>(m, X, lds.chan, u[0], dt, rng, t);
+            else
+                mcc_collide<NCH>(m, X, lds.chan, u, dt, rng, t);
             if (t.err) atomicOr(err, t.err);
             v[3 * p] = u[0];
             v[3 * p + 1] = u[1];
@@ -72,6 +76,8 @@ struct Mcc1dArgs {
     float half_qm_dt;       // 0.5 * qm * dt of the target species
     float dt;
     uint64_t seed;
+    uint32_t stream;        // the model's index: its own random stream
+    int one_v;              // 1D1V: the scalar collide on v_x
     uint32_t step;
     int32_t* nprod;         // per particle ionization events (0..2)
     float* prod;            // per particle: 2 x (secondary v[3], ion v[3])
@@ -96,9 +102,10 @@ __global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, doubl
         }
         MccTally<NCH> t{};
         if (active)
-            a.nprod[p] = mcc_1d_particle<NCH>(m, X, lds.chan, c, a.off[p], a.v0 + p, a.v1 + p, a.v2 + p, a.uid[p],
-                                         a.E32, a.nx, a.half_qm_dt, a.dt, a.seed, a.step,
-                                         a.prod + 12 * p, t, a.err);
+            a.nprod[p] = mcc_1d_particle<NCH>(m, X, lds.chan, c, a.off[p], a.v0 + p, a.v1 + p,
+                                              a.v2 + p, a.uid[p], a.E32, a.nx, a.half_qm_dt,
+                                              a.dt, a.seed, a.step, a.prod + 12 * p, t, a.err,
+                                              a.stream, a.one_v != 0);
         flush_tally_rows<NCH>(t, active, w);
     }
     store_wave_row<NCH>(w, rows);
@@ -277,9 +284,9 @@ MccParams mcc_params_of(const MccModelHost& h) {
 
 MccParams mcc_params(const Context& c) { return mcc_params_of(c.mcc); }
 
-MccParams mcc_step_params(const Context& c) {
-    MccParams m = mcc_params(c);
-    const MccModelHost& h = c.mcc;
+MccParams mcc_step_params(const Context& c, int model) {
+    const MccModelHost& h = c.mcc_model(model);
+    MccParams m = mcc_params_of(h);
     if (h.secondary_species >= 0 && h.ion_species >= 0) {
         // equal weights are required of 1D products, so the unweighted tally
         // times the target weight is the reference's weighted sum
@@ -289,20 +296,23 @@ MccParams mcc_step_params(const Context& c) {
     return m;
 }
 
-void mcc_step(Context& c, double dt) {
-    MccModelHost& h = c.mcc;
-    AUR_REQUIRE(c.dim == 1, AURORA_ERR_UNSUPPORTED,
-                "MCC inside the particle step is built for 1D meshes");
+namespace {
+
+// One model's collide pass over its target species (Simulation.cpp:835-927):
+// synchronise, collide, rebuild v_half, stage the products of each parent;
+// returns the number of ionization events (0 without products).
+int64_t mcc_collide_pass(Context& c, int model, double dt) {
+    MccModelHost& h = c.mcc_model(model);
     Species& sp = c.species[h.target];
     AUR_REQUIRE(sp.precision == 32, AURORA_ERR_UNSUPPORTED, "MCC targets FP32 species");
     const int64_t n = sp.slots;
-    if (n == 0) return;
+    if (n == 0) return 0;
     if (h.product_count.count < static_cast<size_t>(n)) {
         h.product_count.alloc(n);
         h.product_offset.alloc(n + 1);
         h.product_vel.alloc(static_cast<size_t>(n) * 12);
     }
-    const MccParams m = mcc_step_params(c);
+    const MccParams m = mcc_step_params(c, model);
     Mcc1dArgs a{};
     a.cell = sp.cell.ptr;
     a.off = reinterpret_cast<float*>(sp.off[0].ptr);
@@ -316,6 +326,8 @@ void mcc_step(Context& c, double dt) {
     a.half_qm_dt = static_cast<float>(0.5 * sp.charge / sp.mass * dt);
     a.dt = static_cast<float>(dt);
     a.seed = h.seed;
+    a.stream = static_cast<uint32_t>(model);
+    a.one_v = c.velocity_dims == 1 ? 1 : 0;
     a.step = static_cast<uint32_t>(c.step_count);
     a.nprod = h.product_count.ptr;
     a.prod = h.product_vel.ptr;
@@ -333,7 +345,7 @@ void mcc_step(Context& c, double dt) {
         });
         AUR_HIP(hipGetLastError());
     }
-    if (h.secondary_species < 0) return;
+    if (h.secondary_species < 0) return 0;
     // exclusive prefix sum of event counts -> product slots
     int32_t* offs = reinterpret_cast<int32_t*>(h.product_offset.ptr);
     size_t temp = 0;
@@ -348,16 +360,18 @@ void mcc_step(Context& c, double dt) {
     AUR_HIP(hipMemcpyAsync(&last_cnt, h.product_count.ptr + (n - 1), sizeof last_cnt,
                            hipMemcpyDeviceToHost, c.stream));
     AUR_HIP(hipStreamSynchronize(c.stream));
-    const int64_t total = static_cast<int64_t>(last_off) + last_cnt;
-    if (total == 0) return;
+    return static_cast<int64_t>(last_off) + last_cnt;
+}
+
+// Places the `total` product pairs staged by one model's pass after the
+// live particles of the secondary and ion species (Simulation.cpp:965-1005).
+void mcc_append(Context& c, int model, int64_t total, double dt) {
+    MccModelHost& h = c.mcc_model(model);
+    Species& sp = c.species[h.target];
+    const int64_t n = sp.slots;
+    const int32_t* offs = reinterpret_cast<const int32_t*>(h.product_offset.ptr);
     Species& se = c.species[h.secondary_species];
     Species& si = c.species[h.ion_species];
-    // all-or-nothing capacity preflight (Simulation.cpp:929-964)
-    const int64_t need_e = total * (h.secondary_species == h.ion_species ? 2 : 1);
-    if (se.slots + need_e > se.capacity || si.slots + total > si.capacity)
-        throw Failure{AURORA_ERR_CAPACITY,
-                      "reactive collisions exceeded max_particles_per_species for species"};
-    AppendSide sec{}, ion{};
     auto side = [&](Species& s, int64_t base_slot) {
         AppendSide d{};
         d.cell = s.cell.ptr;
@@ -370,7 +384,7 @@ void mcc_step(Context& c, double dt) {
         d.half_qm_dt = static_cast<float>(0.5 * s.charge / s.mass * dt);
         return d;
     };
-    sec = side(se, se.slots);
+    AppendSide sec = side(se, se.slots), ion{};
     sec.base_uid = se.next_uid;
     if (h.secondary_species == h.ion_species) {
         ion = side(si, si.slots + total);
@@ -397,6 +411,42 @@ void mcc_step(Context& c, double dt) {
     }
     se.tiled = false;
     si.tiled = false;
+}
+
+}  // namespace
+
+// The 1D collision step (Simulation::apply_collisions, Simulation.cpp:771-1006):
+// every model's pass in order over the particles present at the start of
+// the step (products are placed only after all passes, so none collides in
+// its birth step), then the all-or-nothing capacity check of all products
+// together (Simulation.cpp:929-964), then placement in model order.
+void mcc_step(Context& c, double dt) {
+    AUR_REQUIRE(c.dim == 1, AURORA_ERR_UNSUPPORTED,
+                "MCC inside the particle step is built for 1D meshes");
+    const int models = c.mcc_models();
+    if (c.velocity_dims == 1)
+        for (int k = 0; k < models; ++k) {
+            const MccModelHost& h = c.mcc_model(k);
+            for (int ch = 0; ch < h.nchan; ++ch)
+                AUR_REQUIRE(h.kind[ch] != AURORA_IONIZATION && h.angular[ch] == SCATTER_ISOTROPIC,
+                            AURORA_ERR_INVALID_ARGUMENT,
+                            "ionization and anisotropic MCC require the 3V collision interface");
+        }
+    std::vector<int64_t> total(static_cast<size_t>(models), 0);
+    for (int k = 0; k < models; ++k) total[static_cast<size_t>(k)] = mcc_collide_pass(c, k, dt);
+    std::vector<int64_t> need(c.species.size(), 0);
+    for (int k = 0; k < models; ++k) {
+        const MccModelHost& h = c.mcc_model(k);
+        if (total[static_cast<size_t>(k)] == 0) continue;
+        need[static_cast<size_t>(h.secondary_species)] += total[static_cast<size_t>(k)];
+        need[static_cast<size_t>(h.ion_species)] += total[static_cast<size_t>(k)];
+    }
+    for (size_t s = 0; s < c.species.size(); ++s)
+        if (need[s] && c.species[s].slots + need[s] > c.species[s].capacity)
+            throw Failure{AURORA_ERR_CAPACITY,
+                          "reactive collisions exceeded max_particles_per_species for species"};
+    for (int k = 0; k < models; ++k)
+        if (total[static_cast<size_t>(k)] > 0) mcc_append(c, k, total[static_cast<size_t>(k)], dt);
 }
 
 // Resolves this device's address of the collide scratch (once per device,
@@ -608,6 +658,7 @@ int aurora_hip_mcc_load(aurora_hip_ctx* hctx, int target, int nchan, const int* 
         Species& sp = species_at(c, target);
         MccModelHost& h = c.mcc;
         h.loaded = false;
+        c.mcc_more.clear();
         load_mcc_model(h, nchan, kind, threshold, rows, energy, sigma, neutral_density,
                    max_frequency, max_candidates, sp.mass, neutral_mass, neutral_temperature);
         h.target = target;
@@ -626,6 +677,81 @@ int aurora_hip_mcc_load(aurora_hip_ctx* hctx, int target, int nchan, const int* 
             h.ion_species = product_species[1];
         }
         h.loaded = true;
+    });
+}
+
+int aurora_hip_mcc_add(aurora_hip_ctx* hctx, int target, int nchan, const int* kind,
+                       const double* threshold, const int64_t* rows,
+                       const double* const* energy, const double* const* sigma,
+                       double neutral_density, double max_frequency, int64_t max_candidates,
+                       double neutral_mass, double neutral_temperature,
+                       const int* product_species, uint64_t seed, int* model) {
+    return guarded(hctx, [&](Context& c) {
+        AUR_REQUIRE(c.mcc.loaded, AURORA_ERR_INVALID_ARGUMENT,
+                    "load the first null-collision model with aurora_hip_mcc_load");
+        Species& sp = species_at(c, target);
+        // Simulation.cpp:508-514
+        for (int k = 0; k < c.mcc_models(); ++k)
+            AUR_REQUIRE(c.mcc_model(k).target != target, AURORA_ERR_INVALID_ARGUMENT,
+                        "multiple collision models target species " + std::to_string(target));
+        auto h = std::make_unique<MccModelHost>();
+        load_mcc_model(*h, nchan, kind, threshold, rows, energy, sigma, neutral_density,
+                       max_frequency, max_candidates, sp.mass, neutral_mass, neutral_temperature);
+        h->target = target;
+        h->seed = seed;
+        bool has_iz = false;
+        for (int k = 0; k < nchan; ++k) has_iz = has_iz || kind[k] == AURORA_IONIZATION;
+        if (has_iz) {
+            AUR_REQUIRE(product_species && product_species[0] >= 0 && product_species[1] >= 0,
+                        AURORA_ERR_INVALID_ARGUMENT,
+                        "ionization MCC channel requires secondary and ion species");
+            species_at(c, product_species[0]);
+            species_at(c, product_species[1]);
+            h->secondary_species = product_species[0];
+            h->ion_species = product_species[1];
+        }
+        h->loaded = true;
+        c.mcc_more.push_back(std::move(h));
+        if (model) *model = c.mcc_models() - 1;
+    });
+}
+
+int aurora_hip_mcc_model_options(aurora_hip_ctx* hctx, int model, int channel, int angular,
+                                 int energy_frame, int64_t rows, const double* energy,
+                                 const double* mean_cosine) {
+    return guarded(hctx, [&](Context& c) {
+        AUR_REQUIRE(model >= 0 && model < c.mcc_models(), AURORA_ERR_NOT_FOUND,
+                    "unknown null-collision model");
+        AUR_HIP(hipStreamSynchronize(c.stream));
+        set_channel_options(c.mcc_model(model), channel, angular, energy_frame, rows, energy,
+                            mean_cosine);
+    });
+}
+
+int aurora_hip_mcc_model_stats(aurora_hip_ctx* hctx, int model, uint64_t* counts,
+                               double* energy_change) {
+    return guarded(hctx, [&](Context& c) {
+        AUR_REQUIRE(model >= 0 && model < c.mcc_models(), AURORA_ERR_NOT_FOUND,
+                    "unknown null-collision model");
+        const MccModelHost& h = c.mcc_model(model);
+        unsigned long long raw[kMaxChannels + 3];
+        double e[kMaxChannels];
+        AUR_HIP(hipStreamSynchronize(c.stream));
+        AUR_HIP(hipMemcpy(raw, h.counts.ptr, sizeof raw, hipMemcpyDeviceToHost));
+        AUR_HIP(hipMemcpy(e, h.energy_change.ptr, sizeof e, hipMemcpyDeviceToHost));
+        const double w = c.species[h.target].weight;
+        if (counts)
+            for (int k = 0; k < h.nchan + 3; ++k) counts[k] = raw[k];
+        if (energy_change)
+            for (int k = 0; k < h.nchan; ++k) energy_change[k] = e[k] * w;
+    });
+}
+
+int aurora_hip_set_velocity_dimensions(aurora_hip_ctx* hctx, int dims) {
+    return guarded(hctx, [&](Context& c) {
+        AUR_REQUIRE(dims == 1 || dims == 3, AURORA_ERR_INVALID_ARGUMENT,
+                    "simulation velocity_dimensions must be 1 or 3");
+        c.velocity_dims = dims;
     });
 }
 
@@ -668,6 +794,11 @@ int aurora_hip_mcc_collide(aurora_hip_ctx* hctx, int64_t count, double* velocity
                     "MCC timestep must be positive and finite");
         AUR_REQUIRE(count >= 0 && (count == 0 || velocity), AURORA_ERR_INVALID_ARGUMENT,
                     "velocity array required");
+        if (c.velocity_dims == 1)
+            for (int ch = 0; ch < h.nchan; ++ch)
+                AUR_REQUIRE(h.kind[ch] != AURORA_IONIZATION && h.angular[ch] == SCATTER_ISOTROPIC,
+                            AURORA_ERR_INVALID_ARGUMENT,
+                            "ionization and anisotropic MCC require the 3V collision interface");
         if (count == 0) return;
         std::vector<float> v(static_cast<size_t>(3 * count));
         for (int64_t i = 0; i < 3 * count; ++i) v[i] = static_cast<float>(velocity[i]);
@@ -690,7 +821,8 @@ int aurora_hip_mcc_collide(aurora_hip_ctx* hctx, int64_t count, double* velocity
             with_nchan(m.nchan, [&]<int NCH>() {
                 k_collide<NCH><<<g, kBlk, 0, c.stream>>>(
                     m, dv.ptr, count, static_cast<float>(dt), h.seed, stream_id,
-                    static_cast<uint32_t>(step), rows.ptr, c.dev_error.ptr);
+                    static_cast<uint32_t>(step), rows.ptr, c.dev_error.ptr,
+                    c.velocity_dims == 1 ? 1 : 0);
                 k_fold_rows<NCH><<<1, kBlk, 0, c.stream>>>(rows.ptr, energy_rows(g), ech.ptr,
                                                            cnt.ptr);
             });
@@ -740,7 +872,7 @@ int aurora_hip_mcc_collide_device(aurora_hip_ctx* hctx, int64_t count, float* ve
             with_nchan(m.nchan, [&]<int NCH>() {
                 k_collide<NCH><<<gi, kBlk, 0, c.stream>>>(
                     m, velocity, count, static_cast<float>(dt), h.seed, stream_id,
-                    static_cast<uint32_t>(step), rows, status);
+                    static_cast<uint32_t>(step), rows, status, c.velocity_dims == 1 ? 1 : 0);
                 k_fold_rows<NCH><<<1, kBlk, 0, c.stream>>>(
                     rows, energy_rows(gi), energy_change,
                     reinterpret_cast<unsigned long long*>(counts));

@@ -399,6 +399,107 @@ __device__ void mcc_collide(const MccParams& m, const XsView& X, const ChannelSe
     }
 }
 
+// bounded Maxwellian neutral velocity of the 1V collide: one normal from
+// Box-Muller (u1, u2), rejected beyond 8 sigma (Collision.cpp:686-699)
+__device__ __forceinline__ float neutral_velocity_1v(const MccParams& m, Philox& rng) {
+    if (m.neutral_vstd == 0.f) return 0.f;
+    const float lim = 8.0f * m.neutral_vstd;
+    float w = 0.f;
+    for (int attempt = 0; attempt < 1024; ++attempt) {
+        const float u1 = rng.u01(), u2 = rng.u01();
+        w = m.neutral_vstd * sqrtf(-2.0f * logf(u1)) * cosf(6.28318530717958647692f * u2);
+        if (fabsf(w) <= lim) return w;
+    }
+    return w;
+}
+
+// One particle-step of the 1V null-collision MCC (Collision.cpp:831-905) on
+// the scalar velocity v: the candidate loop of the 3V form on |v - w|, and
+// apply_channel's 1V branch (Collision.cpp:717-760): attachment stops the
+// particle, charge exchange takes the neutral's velocity, otherwise the
+// speed after the energy loss is scattered forward or backward with equal
+// probability (in the centre-of-mass frame for elastic channels with a
+// neutral mass).  Ionization and anisotropic channels are 3V only; the host
+// rejects them.
+template <int NCH>
+__device__ void mcc_collide_1v(const MccParams& m, const XsView& X, const ChannelSel* chan,
+                               float& v, float dt, Philox& rng, MccTally<NCH>& t) {
+    t.candidates = 0;
+    t.nulls = 0;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) { t.chan[c] = 0; t.energy_change[c] = 0.f; }
+    t.nsec = 0;
+    t.err = 0;
+    if (!isfinite(v)) {
+        t.err = DEV_ERR_NONFINITE;
+        return;
+    }
+    float elapsed = 0.f;
+    float rate[NCH];
+    for (;;) {
+        const float speed = fabsf(v);
+        if (m.neutral_vstd == 0.f) {
+            if (speed > m.safe_speed && above_bound(m, mcc_rates<NCH>(m, X, speed, rate))) {
+                t.err = DEV_ERR_RATE;
+                return;
+            }
+        } else if (thermal_bound_exceeded<NCH>(m, X, speed)) {
+            t.err = DEV_ERR_RATE;
+            return;
+        }
+        const float waiting = -logf(rng.u01()) * m.inv_max_frequency;
+        if (!(waiting < dt - elapsed)) break;
+        elapsed += waiting;
+        ++t.candidates;
+        if (t.candidates > static_cast<uint32_t>(m.max_candidates)) {
+            t.err = DEV_ERR_CANDIDATES;
+            return;
+        }
+        const float w = neutral_velocity_1v(m, rng);
+        const float g = v - w;
+        mcc_rates<NCH>(m, X, fabsf(g), rate);
+        const float selection = rng.u01() * m.max_frequency;
+        float cumulative = 0.f;
+        int accepted = -1;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            cumulative += rate[c];
+            if (accepted < 0 && selection < cumulative) accepted = c;
+        }
+        if (accepted < 0) { ++t.nulls; continue; }
+        const int c = accepted;
+        const ChannelSel ch = chan[c];
+        const float e_before = m.half_mass * v * v;
+        if (ch.kind == AURORA_ATTACHMENT) {
+            v = 0.f;
+        } else if (ch.kind == AURORA_CHARGE_EXCHANGE) {
+            v = w;
+        } else {
+            float energy = m.half_mass * g * g;
+            if (ch.kind == AURORA_EXCITATION) {
+                energy -= ch.threshold;
+                if (energy < 0.f) energy = 0.f;
+            }
+            const float sp = sqrtf(energy / m.half_mass);
+            const float scattered = rng.u01() < 0.5f ? -sp : sp;
+            if (ch.kind == AURORA_ELASTIC && m.neutral_mass > 0.f) {
+                const float mt = m.mass + m.neutral_mass;
+                v = (m.mass * v + m.neutral_mass * w) / mt + (m.neutral_mass / mt) * scattered;
+            } else {
+                v = w + scattered;
+            }
+        }
+        const float de = m.half_mass * v * v - e_before;
+#pragma unroll
+        for (int k = 0; k < NCH; ++k)
+            if (k == c) {
+                t.energy_change[k] += de;
+                ++t.chan[k];
+            }
+        if (ch.kind == AURORA_ATTACHMENT) break;
+    }
+}
+
 // block-level tally reduction into global counters
 struct TallyOut {
     unsigned long long* counts;  // [candidates, nulls, chan..., secondaries]
@@ -531,13 +632,17 @@ __device__ __forceinline__ int mcc_1d_particle(const MccParams& m, const XsView&
                                                uint32_t uid, const float* E32, int nx,
                                                float half_qm_dt, float dt, uint64_t seed,
                                                uint32_t step, float* prod, MccTally<NCH>& t,
-                                               uint32_t* err) {
+                                               uint32_t* err, uint32_t stream = 0u,
+                                               bool one_v = false) {
     const float ex = gather_1d_f32(E32, nx, c, o);
     const float kick = half_qm_dt * ex;
     float u[3] = {*v0 + kick, *v1, *v2};
     Philox rng;
-    rng.init(seed, 0u, uid, step);
-    mcc_collide<NCH>(m, X, chan, u, dt, rng, t);
+    rng.init(seed, stream, uid, step);
+    if (one_v)
+        mcc_collide_1v<NCH>(m, X, chan, u[0], dt, rng, t);   // 1D1V: v_y, v_z stay as they are
+    else
+        mcc_collide<NCH>(m, X, chan, u, dt, rng, t);
     if (t.err) atomicOr(err, t.err);
     *v0 = u[0] - kick;
     *v1 = u[1];
@@ -632,7 +737,8 @@ void set_channel_options(MccModelHost& h, int channel, int angular, int frame, i
                          const double* energy, const double* mean_cosine);
 MccParams mcc_params_of(const MccModelHost& h);
 MccParams mcc_params(const Context& c);
-// mcc_params plus the product masses of the 1D step's energy tally
-MccParams mcc_step_params(const Context& c);
+// the parameters of the context's model `model` plus the product masses of
+// the 1D step's energy tally
+MccParams mcc_step_params(const Context& c, int model = 0);
 
 }  // namespace aurora

@@ -143,7 +143,417 @@ void permute_all(Context& c, Species& sp, const unsigned* perm, int64_t n) {
     for (int i = 0; i < k; ++i) std::swap(*from[i], *to[i]);
 }
 
+// --------------------------------------------------- local counting sort
+// Between sorts a particle moves a few cells, so a particle of old tile s
+// now sits in s or in one of its 26 neighbours.  One workgroup per old
+// tile segment counts its particles by direction (27 codes), "moved
+// further" (far) and dead; one thread per new tile orders the (source,
+// direction) contributions of that tile by source tile and gives each its
+// offset; a scan gives the new tile starts.  A second pass over each old
+// segment writes every near particle to
+//   new_start[t] + offset(s -> t) + its rank among the segment's particles
+//   of the same direction (slot order, from wave ballots),
+// with coalesced reads and runs of writes.  Far particles are staged in slot
+// order and placed after the near particles of their tile through a stable
+// radix sort of that short list.  The result is a function of the input
+// (no atomics decide a position); dead particles are dropped.
+
+constexpr int kLsBlock = 256;
+constexpr int kLsWaves = kLsBlock / 64;
+constexpr int kNear = 27, kFar = 27, kDead = 28, kCodes = 29;
+
+struct LsGeom {
+    KeyArgs k;
+    int dim, ntz, ntiles;
+};
+
+__device__ __forceinline__ void cell_tile(const LsGeom& g, int c, int& tx, int& ty, int& tz) {
+    int i = c, j = 0, kk = 0;
+    if (g.dim > 1) {
+        i = c & g.k.mask;
+        j = (c >> g.k.shift) & g.k.mask;
+        if (g.dim > 2) kk = c >> (2 * g.k.shift);
+    }
+    tx = i / g.k.tx;
+    ty = j / g.k.ty;
+    tz = kk / g.k.tz;
+}
+
+// step from source tile s to tile d along an axis of nt tiles (periodic
+// numbering): -1, 0, +1, or 2 when d is no neighbour.  With two tiles the
+// other one is +1 either way round; with one tile the step is 0.
+__device__ __host__ __forceinline__ int axis_step(int d, int s, int nt) {
+    const int dd = d - s;
+    if (dd == 0) return 0;
+    if (nt == 2) return 1;
+    if (dd == 1 || dd == 1 - nt) return 1;
+    if (dd == -1 || dd == nt - 1) return -1;
+    return 2;
+}
+
+__device__ __forceinline__ bool step_valid(int a, int nt) {
+    return nt >= 3 || (nt == 2 ? a != -1 : a == 0);
+}
+
+// direction code of a particle of source tile (sx, sy, sz) now in cell c;
+// t = its tile (or -1 when dead)
+__device__ __forceinline__ int ls_code(const LsGeom& g, int c, int sx, int sy, int sz, int& t) {
+    if (c < 0) {
+        t = -1;
+        return kDead;
+    }
+    int tx, ty, tz;
+    cell_tile(g, c, tx, ty, tz);
+    t = (tz * g.k.nty + ty) * g.k.ntx + tx;
+    const int ax = axis_step(tx, sx, g.k.ntx);
+    const int ay = axis_step(ty, sy, g.k.nty);
+    const int az = axis_step(tz, sz, g.ntz);
+    if (ax == 2 || ay == 2 || az == 2) return kFar;
+    return (az + 1) * 9 + (ay + 1) * 3 + (ax + 1);
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+__global__ void k_fill_u32(uint32_t* p, int64_t n) {
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[i] = 0u;
+}
+
+__global__ void __launch_bounds__(kLsBlock) k_ls_count(LsGeom g, const int32_t* __restrict__ old_start,
+                                                       const int32_t* __restrict__ cell,
+                                                       uint32_t* __restrict__ count,
+                                                       uint32_t* __restrict__ far_count) {
+    __shared__ uint32_t s_cnt[kCodes];
+    const int s = blockIdx.x;
+    const int sx = s % g.k.ntx, sy = (s / g.k.ntx) % g.k.nty, sz = s / (g.k.ntx * g.k.nty);
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x < kCodes) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const int p0 = old_start[s], p1 = old_start[s + 1];
+    for (int base = p0; base < p1; base += kLsBlock) {
+        const int p = base + static_cast<int>(threadIdx.x);
+        int t = -1, code = -1;
+        if (p < p1) code = ls_code(g, cell[p], sx, sy, sz, t);
+        if (code == kFar) atomicAdd(far_count + t, 1u);
+        // one LDS add per distinct code of the wave
+        bool pending = code >= 0;
+        for (;;) {
+            const uint64_t mp = __ballot(pending);
+            if (mp == 0) break;
+            const int leader = __builtin_ctzll(mp);
+            const int lc = __shfl(code, leader);
+            const bool mine = pending && code == lc;
+            const uint64_t m = __ballot(mine);
+            if (lane == leader) atomicAdd(&s_cnt[lc], static_cast<uint32_t>(__popcll(m)));
+            if (mine) pending = false;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kCodes) count[static_cast<int64_t>(s) * kCodes + threadIdx.x] = s_cnt[threadIdx.x];
+}
+
+// one thread per new tile t: its sources in increasing tile order
+__global__ void k_ls_offsets(LsGeom g, const uint32_t* __restrict__ count,
+                             const uint32_t* __restrict__ far_count, uint32_t* __restrict__ offset,
+                             uint32_t* __restrict__ new_count) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.ntiles) return;
+    const int ntx = g.k.ntx, nty = g.k.nty, ntz = g.ntz;
+    const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+    int src[kNear], code[kNear], n = 0;
+    for (int cd = 0; cd < kNear; ++cd) {
+        const int ax = cd % 3 - 1, ay = (cd / 3) % 3 - 1, az = cd / 9 - 1;
+        if (!step_valid(ax, ntx) || !step_valid(ay, nty) || !step_valid(az, ntz)) continue;
+        const int sx = (tx - ax + ntx) % ntx, sy = (ty - ay + nty) % nty,
+                  sz = (tz - az + ntz) % ntz;
+        const int s = (sz * nty + sy) * ntx + sx;
+        int k = n++;
+        while (k > 0 && src[k - 1] > s) {
+            src[k] = src[k - 1];
+            code[k] = code[k - 1];
+            --k;
+        }
+        src[k] = s;
+        code[k] = cd;
+    }
+    uint32_t run = 0u;
+    for (int k = 0; k < n; ++k) {
+        offset[static_cast<int64_t>(src[k]) * kNear + code[k]] = run;
+        run += count[static_cast<int64_t>(src[k]) * kCodes + code[k]];
+    }
+    new_count[t] = run + far_count[t];
+}
+
+// exclusive scan of one value per thread over a 1024-thread block
+__device__ uint32_t block_scan_u32(uint32_t v, uint32_t* lds, uint32_t& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) lds[w] = x;
+    __syncthreads();
+    uint32_t before = 0u;
+    total = 0u;
+    for (int k = 0; k < 16; ++k) {
+        if (k < w) before += lds[k];
+        total += lds[k];
+    }
+    __syncthreads();
+    return before + x - v;
+}
+
+// new tile starts, far bases per source, totals (one 1024-thread block)
+__global__ void __launch_bounds__(1024) k_ls_scan(int ntiles, const uint32_t* __restrict__ new_count,
+                                                  const uint32_t* __restrict__ count,
+                                                  int32_t* __restrict__ new_start,
+                                                  uint32_t* __restrict__ far_base,
+                                                  uint32_t* __restrict__ summary) {
+    __shared__ uint32_t lds[16];
+    const int per = (ntiles + 1023) / 1024;
+    const int i0 = threadIdx.x * per, i1 = min(ntiles, i0 + per);
+    uint32_t a = 0u, f = 0u, d = 0u;
+    for (int i = i0; i < i1; ++i) {
+        a += new_count[i];
+        f += count[static_cast<int64_t>(i) * kCodes + kFar];
+        d += count[static_cast<int64_t>(i) * kCodes + kDead];
+    }
+    uint32_t ta, tf, td;
+    uint32_t ea = block_scan_u32(a, lds, ta);
+    uint32_t ef = block_scan_u32(f, lds, tf);
+    (void)block_scan_u32(d, lds, td);
+    for (int i = i0; i < i1; ++i) {
+        new_start[i] = static_cast<int32_t>(ea);
+        ea += new_count[i];
+        far_base[i] = ef;
+        ef += count[static_cast<int64_t>(i) * kCodes + kFar];
+    }
+    if (threadIdx.x == 0) {
+        new_start[ntiles] = static_cast<int32_t>(ta);
+        summary[0] = tf;
+        summary[1] = td;
+    }
+}
+
+template <typename R>
+struct LsPayload {
+    const int32_t* cell;
+    int32_t* cell2;
+    const uint32_t* uid;
+    uint32_t* uid2;
+    const R* src[6];
+    R* dst[6];
+    int count;
+    // far particles: staged state, destination tile, staging index
+    int32_t* st_cell;
+    uint32_t* st_uid;
+    R* st[6];
+    unsigned* far_key;
+    unsigned* far_val;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(kLsBlock) k_ls_scatter(LsGeom g, const int32_t* __restrict__ old_start,
+                                                         const int32_t* __restrict__ new_start,
+                                                         const uint32_t* __restrict__ offset,
+                                                         const uint32_t* __restrict__ far_base,
+                                                         LsPayload<R> a) {
+    __shared__ uint32_t s_run[kCodes];
+    __shared__ uint32_t s_wave[kLsWaves][kCodes];
+    const int s = blockIdx.x;
+    const int sx = s % g.k.ntx, sy = (s / g.k.ntx) % g.k.nty, sz = s / (g.k.ntx * g.k.nty);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (threadIdx.x < kCodes) s_run[threadIdx.x] = 0u;
+    for (int q = threadIdx.x; q < kLsWaves * kCodes; q += kLsBlock) (&s_wave[0][0])[q] = 0u;
+    __syncthreads();
+    const int p0 = old_start[s], p1 = old_start[s + 1];
+    const uint32_t fb = far_base[s];
+    const uint32_t* off_s = offset + static_cast<int64_t>(s) * kNear;
+    for (int base = p0; base < p1; base += kLsBlock) {
+        const int p = base + static_cast<int>(threadIdx.x);
+        int t = -1, code = -1;
+        int32_t c = -1;
+        if (p < p1) {
+            c = a.cell[p];
+            code = ls_code(g, c, sx, sy, sz, t);
+        }
+        // rank among the wave's particles of the same code (slot order)
+        uint32_t rank = 0u;
+        bool pending = code >= 0;
+        for (;;) {
+            const uint64_t mp = __ballot(pending);
+            if (mp == 0) break;
+            const int leader = __builtin_ctzll(mp);
+            const int lc = __shfl(code, leader);
+            const bool mine = pending && code == lc;
+            const uint64_t m = __ballot(mine);
+            if (mine) {
+                rank = lanes_below(m);
+                pending = false;
+            }
+            if (lane == leader) s_wave[w][lc] = static_cast<uint32_t>(__popcll(m));
+        }
+        __syncthreads();
+        if (code >= 0 && code != kDead) {
+            uint32_t r = s_run[code] + rank;
+            for (int v = 0; v < w; ++v) r += s_wave[v][code];
+            if (code == kFar) {
+                const uint32_t f = fb + r;
+                a.st_cell[f] = c;
+                a.st_uid[f] = a.uid[p];
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                    if (k < a.count) a.st[k][f] = a.src[k][p];
+                a.far_key[f] = static_cast<unsigned>(t);
+                a.far_val[f] = f;
+            } else {
+                const int64_t q = static_cast<int64_t>(new_start[t]) + off_s[code] + r;
+                a.cell2[q] = c;
+                a.uid2[q] = a.uid[p];
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                    if (k < a.count) a.dst[k][q] = a.src[k][p];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < kCodes) {
+            uint32_t sum = 0u;
+            for (int v = 0; v < kLsWaves; ++v) {
+                sum += s_wave[v][threadIdx.x];
+                s_wave[v][threadIdx.x] = 0u;
+            }
+            s_run[threadIdx.x] += sum;
+        }
+        __syncthreads();
+    }
+}
+
+// far particles after the near ones of their tile, in staging (slot) order
+template <typename R>
+__global__ void k_ls_far_place(const unsigned* __restrict__ key, const unsigned* __restrict__ val,
+                               int64_t nfar, const int32_t* __restrict__ new_start,
+                               const uint32_t* __restrict__ far_count, LsPayload<R> a) {
+    for (int64_t r = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; r < nfar;
+         r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const unsigned t = key[r];
+        const unsigned f = val[r];
+        int64_t lo = 0, hi = r;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) / 2;
+            if (key[mid] < t) lo = mid + 1;
+            else hi = mid;
+        }
+        const int64_t q = static_cast<int64_t>(new_start[t + 1]) - far_count[t] + (r - lo);
+        a.cell2[q] = a.st_cell[f];
+        a.uid2[q] = a.st_uid[f];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (k < a.count) a.dst[k][q] = a.st[k][f];
+    }
+}
+
+// The local sort of a tile-sorted species; false (and nothing changed) when
+// too many particles moved beyond a neighbouring tile, so the full radix
+// sort is the better choice.
+template <typename R>
+bool local_sort(Context& c, Species& sp, const KeyArgs& k, int ntz, unsigned bits) {
+    const int64_t n = sp.slots;
+    const int T = k.ntx * k.nty * ntz;
+    LsGeom g{k, c.dim, ntz, T};
+    auto grow = [](auto& b, size_t want) {
+        if (b.count < want || !b.ptr) b.alloc(want);
+    };
+    grow(c.ls_count, static_cast<size_t>(T) * kCodes);
+    grow(c.ls_offset, static_cast<size_t>(T) * kNear);
+    grow(c.ls_far_base, T);
+    grow(c.ls_far_count, T);
+    grow(c.ls_new_count, T);
+    grow(c.ls_new_start, static_cast<size_t>(T) + 1);
+    grow(c.ls_summary, 4);
+    k_fill_u32<<<(T + 255) / 256, 256, 0, c.stream>>>(c.ls_far_count.ptr, T);
+    k_ls_count<<<T, kLsBlock, 0, c.stream>>>(g, sp.tile_start.ptr, sp.cell.ptr, c.ls_count.ptr,
+                                            c.ls_far_count.ptr);
+    k_ls_offsets<<<(T + 255) / 256, 256, 0, c.stream>>>(g, c.ls_count.ptr, c.ls_far_count.ptr,
+                                                        c.ls_offset.ptr, c.ls_new_count.ptr);
+    k_ls_scan<<<1, 1024, 0, c.stream>>>(T, c.ls_new_count.ptr, c.ls_count.ptr,
+                                        c.ls_new_start.ptr, c.ls_far_base.ptr, c.ls_summary.ptr);
+    AUR_HIP(hipGetLastError());
+    uint32_t summary[2] = {0u, 0u};
+    AUR_HIP(hipMemcpyAsync(summary, c.ls_summary.ptr, sizeof summary, hipMemcpyDeviceToHost,
+                           c.stream));
+    AUR_HIP(hipStreamSynchronize(c.stream));
+    const int64_t nfar = summary[0], ndead = summary[1];
+    if (nfar > n / 8) return false;
+
+    LsPayload<R> a{};
+    DevBuf<uint8_t>* from[6];
+    DevBuf<uint8_t>* to[6];
+    int cnt = 0;
+    for (int ax = 0; ax < 3; ++ax)
+        if (sp.off[ax].ptr) { from[cnt] = &sp.off[ax]; to[cnt] = &sp.off2[ax]; ++cnt; }
+    for (int ax = 0; ax < 3; ++ax)
+        if (sp.vel[ax].ptr) { from[cnt] = &sp.vel[ax]; to[cnt] = &sp.vel2[ax]; ++cnt; }
+    a.count = cnt;
+    for (int i = 0; i < cnt; ++i) {
+        if (!to[i]->ptr) to[i]->alloc(sp.capacity * sizeof(R));
+        a.src[i] = reinterpret_cast<const R*>(from[i]->ptr);
+        a.dst[i] = reinterpret_cast<R*>(to[i]->ptr);
+    }
+    if (!sp.cell2.ptr) sp.cell2.alloc(sp.capacity);
+    if (!sp.uid2.ptr) sp.uid2.alloc(sp.capacity);
+    a.cell = sp.cell.ptr;
+    a.cell2 = sp.cell2.ptr;
+    a.uid = sp.uid.ptr;
+    a.uid2 = sp.uid2.ptr;
+    if (nfar > 0) {
+        const size_t nf = (static_cast<size_t>(nfar) + 63) & ~size_t{63};
+        grow(c.ls_stage, nf * (8 + 6 * sizeof(R)));
+        grow(c.ls_far_key, nf);
+        grow(c.ls_far_key2, nf);
+        grow(c.ls_far_val, nf);
+        grow(c.ls_far_val2, nf);
+        uint8_t* st = c.ls_stage.ptr;
+        a.st_cell = reinterpret_cast<int32_t*>(st);
+        a.st_uid = reinterpret_cast<uint32_t*>(st + nf * 4);
+        for (int i = 0; i < 6; ++i) a.st[i] = reinterpret_cast<R*>(st + nf * (8 + i * sizeof(R)));
+        a.far_key = c.ls_far_key.ptr;
+        a.far_val = c.ls_far_val.ptr;
+    }
+    k_ls_scatter<R><<<T, kLsBlock, 0, c.stream>>>(g, sp.tile_start.ptr, c.ls_new_start.ptr,
+                                                 c.ls_offset.ptr, c.ls_far_base.ptr, a);
+    AUR_HIP(hipGetLastError());
+    if (nfar > 0) {
+        size_t temp_bytes = 0;
+        AUR_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, c.ls_far_key.ptr,
+                                          c.ls_far_key2.ptr, c.ls_far_val.ptr,
+                                          c.ls_far_val2.ptr, nfar, 0, bits, c.stream));
+        DevBuf<uint8_t>& temp = c.sort_temp;
+        grow(temp, temp_bytes > 0 ? temp_bytes : 1);
+        AUR_HIP(rocprim::radix_sort_pairs(temp.ptr, temp_bytes, c.ls_far_key.ptr,
+                                          c.ls_far_key2.ptr, c.ls_far_val.ptr,
+                                          c.ls_far_val2.ptr, nfar, 0, bits, c.stream));
+        k_ls_far_place<R><<<blocks(c, nfar), 256, 0, c.stream>>>(
+            c.ls_far_key2.ptr, c.ls_far_val2.ptr, nfar, c.ls_new_start.ptr,
+            c.ls_far_count.ptr, a);
+        AUR_HIP(hipGetLastError());
+    }
+    std::swap(sp.cell, sp.cell2);
+    std::swap(sp.uid, sp.uid2);
+    for (int i = 0; i < cnt; ++i) std::swap(*from[i], *to[i]);
+    std::swap(sp.tile_start, c.ls_new_start);
+    sp.slots = n - ndead;
+    return true;
+}
+
 }  // namespace
+
+void finish_sort(Context& c, Species& sp, bool timed);
 
 void sort_species(Context& c, Species& sp) {
     const int64_t n = sp.slots;
@@ -169,6 +579,16 @@ void sort_species(Context& c, Species& sp) {
     k.sentinel = static_cast<unsigned>(keyspace);
     unsigned bits = 1;
     while ((1ull << bits) <= keyspace) ++bits;
+
+    // a tile-sorted species whose tiling is unchanged: the local counting sort
+    const bool local_ok = c.sort_method == 0 && sp.tiled && sp.sorted_slots == n &&
+                          sp.tile_start.count >= static_cast<size_t>(keyspace + 1) &&
+                          sp.tile_keyspace == static_cast<int64_t>(keyspace);
+    if (local_ok && (sp.precision == 64 ? local_sort<double>(c, sp, k, ntz, bits)
+                                        : local_sort<float>(c, sp, k, ntz, bits))) {
+        finish_sort(c, sp, timed);
+        return;
+    }
 
     DevBuf<unsigned>& keys = c.sort_keys;
     DevBuf<unsigned>& keys2 = c.sort_keys2;
@@ -205,7 +625,13 @@ void sort_species(Context& c, Species& sp) {
                            hipMemcpyDeviceToHost, c.stream));
     AUR_HIP(hipStreamSynchronize(c.stream));
     sp.slots = live;
+    sp.tile_keyspace = static_cast<int64_t>(keyspace);
+    finish_sort(c, sp, timed);
+}
+
+void finish_sort(Context& c, Species& sp, bool timed) {
     sp.tiled = true;
+    sp.sorted_slots = sp.slots;
     sp.pushes_since_sort = 0;
     sp.cycle_ms = 0.0;
     sp.timed_pushes = 0;

@@ -258,6 +258,23 @@ struct RequestState {
     DevBuf<uint32_t> status;
 };
 
+// Descriptor and work-item arrays of one launch: pinned host staging and
+// device copies, reused once the event recorded after the launch completes.
+struct LaunchBuffers {
+    RequestDesc* h_descs = nullptr;
+    WorkItem* h_items = nullptr;
+    size_t cap_descs = 0, cap_items = 0;
+    DevBuf<RequestDesc> descs;
+    DevBuf<WorkItem> items;
+    DevBuf<int> head;
+    hipEvent_t done = nullptr;
+    ~LaunchBuffers() {
+        if (h_descs) (void)hipHostFree(h_descs);
+        if (h_items) (void)hipHostFree(h_items);
+        if (done) (void)hipEventDestroy(done);
+    }
+};
+
 struct SwarmState {
     MccModelHost gas;
     bool gas_loaded = false;
@@ -266,9 +283,11 @@ struct SwarmState {
     int64_t next_handle = 1;
     uint64_t bytes_in_use = 0;
     double table_max_ev = 0.0;
-    DevBuf<RequestDesc> descs;
-    DevBuf<WorkItem> items;
-    DevBuf<int> head;
+    LaunchBuffers buf[2];           // alternate between launches
+    int cur = 0;
+    bool in_flight = false;         // a launch has been queued and not collected
+    std::vector<int64_t> finishing; // requests whose last steps the launch in flight runs
+    int blocks_per_cu = 0;          // resident k_swarm workgroups per CU (occupancy)
 };
 
 SwarmState& state(Context& c) {
@@ -635,6 +654,126 @@ void block_slope_error(const double* y, int64_t n, double dt, int64_t blocks, do
     block_mean_error(slopes, slope, se);
 }
 
+// Queues one k_swarm launch of every admitted, unfinished request (up to
+// max_steps steps each) and records the launch's event; steps are booked at
+// once, and the requests this launch completes are finalised by
+// collect_launch.
+void queue_launch(Context& c, SwarmState& s, int64_t max_steps) {
+    LaunchBuffers& b = s.buf[s.cur];
+    if (b.done) AUR_HIP(hipEventSynchronize(b.done));   // the buffers' previous launch
+    std::vector<RequestDesc> descs;
+    std::vector<WorkItem> items;
+    s.finishing.clear();
+    for (auto& kv : s.active) {
+        RequestState& r = *kv.second;
+        const auto& q = r.req;
+        const int64_t remaining = q.steps - r.steps_done;
+        if (remaining <= 0) continue;
+        const int64_t ns = std::min(max_steps, remaining);
+        RequestDesc d{};
+        const int64_t n = q.particles;
+        d.vx = r.v.ptr;
+        d.vy = r.v.ptr + n;
+        d.vz = r.v.ptr + 2 * n;
+        d.dx = r.dx.ptr;
+        d.moments = r.moments.ptr;
+        d.part = r.part.ptr;
+        d.counts = r.counts.ptr;
+        d.max_energy_bits = r.maxe.ptr;
+        d.status = r.status.ptr;
+        d.particles = n;
+        const double efield = q.reduced_field_td * kTownsend * s.gas.neutral_density;
+        const double accel = -kElementaryCharge * efield / kElectronMass;
+        d.half_kick = static_cast<float>(0.5 * accel * q.timestep);
+        d.dt = static_cast<float>(q.timestep);
+        d.max_energy_ev = static_cast<float>(q.max_energy_ev);
+        d.ev_per_v2 = static_cast<float>(0.5 * kElectronMass / kElementaryCharge);
+        d.scale_v = static_cast<float>(scales(q).first);
+        d.scale_e = static_cast<float>(scales(q).second);
+        d.step0 = static_cast<int>(r.steps_done);
+        d.nsteps = static_cast<int>(ns);
+        d.burn_in = static_cast<int>(q.burn_in);
+        d.sampling = static_cast<int>(q.steps - q.burn_in);
+        d.blocks = static_cast<int>(q.blocks);
+        d.seed = q.seed;
+        const int ridx = static_cast<int>(descs.size());
+        descs.push_back(d);
+        for (int64_t ch = 0; ch < r.chunks; ++ch) items.push_back(WorkItem{ridx, static_cast<int>(ch)});
+        r.steps_done += ns;
+        if (r.steps_done >= q.steps) s.finishing.push_back(kv.first);
+    }
+    if (items.empty()) return;
+    if (b.cap_descs < descs.size()) {
+        if (b.h_descs) AUR_HIP(hipHostFree(b.h_descs));
+        b.h_descs = nullptr;
+        b.cap_descs = descs.size() * 2;
+        AUR_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_descs),
+                              b.cap_descs * sizeof(RequestDesc), hipHostMallocDefault));
+        b.descs.alloc(b.cap_descs);
+    }
+    if (b.cap_items < items.size()) {
+        if (b.h_items) AUR_HIP(hipHostFree(b.h_items));
+        b.h_items = nullptr;
+        b.cap_items = items.size() * 2;
+        AUR_HIP(hipHostMalloc(reinterpret_cast<void**>(&b.h_items), b.cap_items * sizeof(WorkItem),
+                              hipHostMallocDefault));
+        b.items.alloc(b.cap_items);
+    }
+    if (!b.head.ptr) b.head.alloc(1);
+    if (!b.done) AUR_HIP(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
+    std::memcpy(b.h_descs, descs.data(), descs.size() * sizeof(RequestDesc));
+    std::memcpy(b.h_items, items.data(), items.size() * sizeof(WorkItem));
+    AUR_HIP(hipMemcpyAsync(b.descs.ptr, b.h_descs, descs.size() * sizeof(RequestDesc),
+                           hipMemcpyHostToDevice, c.stream));
+    AUR_HIP(hipMemcpyAsync(b.items.ptr, b.h_items, items.size() * sizeof(WorkItem),
+                           hipMemcpyHostToDevice, c.stream));
+    AUR_HIP(hipMemsetAsync(b.head.ptr, 0, sizeof(int), c.stream));
+    const MccParams m = swarm_params(s);
+    if (s.blocks_per_cu == 0) {
+        // as many workgroups as the CUs hold at once: the kernel pulls
+        // chunks from the queue, so more would only wait
+        int per_cu = 0;
+        with_nchan(m.nchan, [&]<int NCH>() {
+            AUR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_swarm<NCH>, kChunk, 0));
+        });
+        s.blocks_per_cu = per_cu > 0 ? per_cu : 4;
+    }
+    int64_t grid = static_cast<int64_t>(c.cu_count) * s.blocks_per_cu;
+    if (grid > static_cast<int64_t>(items.size())) grid = static_cast<int64_t>(items.size());
+    {
+        KernelScope ks(c, "swarm");
+        with_nchan(m.nchan, [&]<int NCH>() {
+            k_swarm<NCH><<<static_cast<unsigned>(grid), kChunk, 0, c.stream>>>(
+                m, b.descs.ptr, b.items.ptr, static_cast<int>(items.size()), b.head.ptr);
+        });
+        AUR_HIP(hipGetLastError());
+    }
+    AUR_HIP(hipEventRecord(b.done, c.stream));
+    s.in_flight = true;
+    s.cur ^= 1;
+}
+
+// Completes the launch in flight: waits for it (wait) or returns while it
+// still runs; once done, finalises the requests it completed.
+void collect_launch(Context& c, SwarmState& s, bool wait) {
+    LaunchBuffers& b = s.buf[s.cur ^ 1];
+    if (wait) {
+        AUR_HIP(hipEventSynchronize(b.done));
+    } else {
+        const hipError_t q = hipEventQuery(b.done);
+        if (q == hipErrorNotReady) return;
+        AUR_HIP(q);
+    }
+    s.in_flight = false;
+    for (int64_t handle : s.finishing) {
+        auto it = s.active.find(handle);
+        if (it == s.active.end()) continue;
+        finalize(c, s, handle, *it->second);
+        s.active.erase(it);
+    }
+    s.finishing.clear();
+}
+
 }  // namespace
 
 MccModelHost& swarm_gas_model(Context& c) {
@@ -647,6 +786,7 @@ MccModelHost& swarm_gas_model(Context& c) {
 
 void swarm_destroy(Context& c) {
     if (c.swarm) {
+        (void)hipStreamSynchronize(c.stream);   // a launch may still read the buffers
         delete static_cast<SwarmState*>(c.swarm);
         c.swarm = nullptr;
     }
@@ -738,82 +878,32 @@ int aurora_hip_swarm_submit(aurora_hip_ctx* h, const aurora_hip_swarm_request* r
     });
 }
 
+int aurora_hip_swarm_launch(aurora_hip_ctx* h, int64_t max_steps, int64_t* active_out) {
+    return guarded(h, [&](Context& c) {
+        SwarmState& s = state(c);
+        AUR_REQUIRE(max_steps > 0, AURORA_ERR_INVALID_ARGUMENT, "max_steps must be positive");
+        if (s.in_flight) collect_launch(c, s, true);
+        queue_launch(c, s, max_steps);
+        if (active_out) *active_out = static_cast<int64_t>(s.active.size());
+    });
+}
+
+int aurora_hip_swarm_collect(aurora_hip_ctx* h, int wait, int* busy) {
+    return guarded(h, [&](Context& c) {
+        SwarmState& s = state(c);
+        if (s.in_flight) collect_launch(c, s, wait != 0);
+        if (busy) *busy = s.in_flight ? 1 : 0;
+        flush_timers(c);
+    });
+}
+
 int aurora_hip_swarm_advance(aurora_hip_ctx* h, int64_t max_steps, int64_t* active_out) {
     return guarded(h, [&](Context& c) {
         SwarmState& s = state(c);
         AUR_REQUIRE(max_steps > 0, AURORA_ERR_INVALID_ARGUMENT, "max_steps must be positive");
-        std::vector<RequestDesc> descs;
-        std::vector<WorkItem> items;
-        std::vector<std::pair<int64_t, RequestState*>> launched;
-        for (auto& kv : s.active) {
-            RequestState& r = *kv.second;
-            const auto& q = r.req;
-            const int64_t remaining = q.steps - r.steps_done;
-            if (remaining <= 0) continue;
-            const int64_t ns = std::min(max_steps, remaining);
-            RequestDesc d{};
-            const int64_t n = q.particles;
-            d.vx = r.v.ptr;
-            d.vy = r.v.ptr + n;
-            d.vz = r.v.ptr + 2 * n;
-            d.dx = r.dx.ptr;
-            d.moments = r.moments.ptr;
-            d.part = r.part.ptr;
-            d.counts = r.counts.ptr;
-            d.max_energy_bits = r.maxe.ptr;
-            d.status = r.status.ptr;
-            d.particles = n;
-            const double efield = q.reduced_field_td * kTownsend * s.gas.neutral_density;
-            const double accel = -kElementaryCharge * efield / kElectronMass;
-            d.half_kick = static_cast<float>(0.5 * accel * q.timestep);
-            d.dt = static_cast<float>(q.timestep);
-            d.max_energy_ev = static_cast<float>(q.max_energy_ev);
-            d.ev_per_v2 = static_cast<float>(0.5 * kElectronMass / kElementaryCharge);
-            d.scale_v = static_cast<float>(scales(q).first);
-            d.scale_e = static_cast<float>(scales(q).second);
-            d.step0 = static_cast<int>(r.steps_done);
-            d.nsteps = static_cast<int>(ns);
-            d.burn_in = static_cast<int>(q.burn_in);
-            d.sampling = static_cast<int>(q.steps - q.burn_in);
-            d.blocks = static_cast<int>(q.blocks);
-            d.seed = q.seed;
-            const int ridx = static_cast<int>(descs.size());
-            descs.push_back(d);
-            for (int64_t ch = 0; ch < r.chunks; ++ch)
-                items.push_back(WorkItem{ridx, static_cast<int>(ch)});
-            launched.push_back({kv.first, &r});
-        }
-        if (!items.empty()) {
-            if (s.descs.count < descs.size()) s.descs.alloc(descs.size() * 2);
-            if (s.items.count < items.size()) s.items.alloc(items.size() * 2);
-            if (!s.head.ptr) s.head.alloc(1);
-            AUR_HIP(hipMemcpyAsync(s.descs.ptr, descs.data(), descs.size() * sizeof(RequestDesc),
-                                   hipMemcpyHostToDevice, c.stream));
-            AUR_HIP(hipMemcpyAsync(s.items.ptr, items.data(), items.size() * sizeof(WorkItem),
-                                   hipMemcpyHostToDevice, c.stream));
-            AUR_HIP(hipMemsetAsync(s.head.ptr, 0, sizeof(int), c.stream));
-            const MccParams m = swarm_params(s);
-            int64_t grid = static_cast<int64_t>(c.cu_count) * 4;
-            if (grid > static_cast<int64_t>(items.size())) grid = static_cast<int64_t>(items.size());
-            {
-                KernelScope ks(c, "swarm");
-                with_nchan(m.nchan, [&]<int NCH>() {
-                    k_swarm<NCH><<<static_cast<unsigned>(grid), kChunk, 0, c.stream>>>(
-                        m, s.descs.ptr, s.items.ptr, static_cast<int>(items.size()), s.head.ptr);
-                });
-                AUR_HIP(hipGetLastError());
-            }
-            // the host arrays are reused next call: wait for the copies
-            AUR_HIP(hipStreamSynchronize(c.stream));
-        }
-        for (auto& lr : launched) {
-            RequestState& r = *lr.second;
-            r.steps_done += std::min(max_steps, r.req.steps - r.steps_done);
-            if (r.steps_done >= r.req.steps) {
-                finalize(c, s, lr.first, r);
-                s.active.erase(lr.first);
-            }
-        }
+        if (s.in_flight) collect_launch(c, s, true);
+        queue_launch(c, s, max_steps);
+        if (s.in_flight) collect_launch(c, s, true);
         if (active_out) *active_out = static_cast<int64_t>(s.active.size());
         flush_timers(c);
     });

@@ -77,7 +77,9 @@ struct Tables {
     }
 };
 
-void apply_options(const Device& d, int model, const CollisionConfig& c) {
+// Channel options of a context model (context = true: the model of that
+// index; false: model 0 = the context MCC, 1 = the swarm gas).
+void apply_channel_options(const Device& d, int model, const CollisionConfig& c, bool context) {
     for (size_t k = 0; k < c.channels.size(); ++k) {
         const auto& ch = c.channels[k];
         const int ang = static_cast<int>(ch.angular_scattering);
@@ -90,12 +92,24 @@ void apply_options(const Device& d, int model, const CollisionConfig& c) {
         if (!hg && !ch.mean_cosine.energies.empty())
             throw std::invalid_argument(
                 "mean-cosine data requires angular_model = henyey_greenstein");
-        d.check(aurora_hip_mcc_channel_options(
-            d.get(), model, static_cast<int>(k), ang, frame,
-            hg ? static_cast<int64_t>(ch.mean_cosine.energies.size()) : 0,
-            hg ? ch.mean_cosine.energies.data() : nullptr,
-            hg ? ch.mean_cosine.values.data() : nullptr));
+        const int64_t rows = hg ? static_cast<int64_t>(ch.mean_cosine.energies.size()) : 0;
+        const double* e = hg ? ch.mean_cosine.energies.data() : nullptr;
+        const double* v = hg ? ch.mean_cosine.values.data() : nullptr;
+        if (context)
+            d.check(aurora_hip_mcc_model_options(d.get(), model, static_cast<int>(k), ang, frame,
+                                                 rows, e, v));
+        else
+            d.check(aurora_hip_mcc_channel_options(d.get(), model, static_cast<int>(k), ang,
+                                                   frame, rows, e, v));
     }
+}
+
+void apply_options(const Device& d, int model, const CollisionConfig& c) {
+    apply_channel_options(d, model, c, false);
+}
+
+void apply_model_options(const Device& d, int model, const CollisionConfig& c) {
+    apply_channel_options(d, model, c, true);
 }
 
 void mesh_bind(const Device& d, int dim, const int64_t* n, const double* L, Boundary b,
@@ -454,14 +468,13 @@ Simulation::Simulation(Config cfg, std::shared_ptr<Device> device)
     if (cfg_.max_particles_per_species == 0)
         throw std::invalid_argument("max_particles_per_species must be positive");
     if (cfg_.output_interval == 0) throw std::invalid_argument("output_interval must be positive");
-    for (const auto& m : cfg_.collision_models) {
-        for (const auto& ch : m.config.channels)
+    for (const auto& m : cfg_.collision_models)
+        for (const auto& ch : m.config.channels) {
             if (ch.process == CollisionProcessKind::Ionization && cfg_.velocity_dimensions != 3)
                 throw std::invalid_argument("1D ionization requires velocity_dimensions = 3");
-        if (cfg_.velocity_dimensions != 3)
-            throw std::invalid_argument(
-                "the GPU backend's 1D collisions are 1D3V (the 1D1V collide is not built)");
-    }
+            if (ch.process == CollisionProcessKind::Attachment)
+                throw std::invalid_argument("1D MCC does not support attachment");
+        }
     const Device& d = *dev_;
     const int64_t n = static_cast<int64_t>(cfg_.nx);
     mesh_bind(d, 1, &n, &cfg_.length, cfg_.boundary, cfg_.permittivity, 1.0);
@@ -477,9 +490,8 @@ Simulation::Simulation(Config cfg, std::shared_ptr<Device> device)
         d.check(aurora_hip_species_add(d.get(), s.charge, s.mass, s.weight, cap, s.precision,
                                        &sid));
     }
-    if (cfg_.collision_models.size() > 1)
-        throw std::invalid_argument(
-            "the GPU backend runs one null-collision model per simulation");
+    d.check(aurora_hip_set_velocity_dimensions(d.get(),
+                                               static_cast<int>(cfg_.velocity_dimensions)));
     auto index_of = [&](const std::string& name) {
         for (size_t k = 0; k < cfg_.species.size(); ++k)
             if (cfg_.species[k].name == name) return static_cast<int>(k);
@@ -491,12 +503,24 @@ Simulation::Simulation(Config cfg, std::shared_ptr<Device> device)
         if (!m.secondary_species.empty()) prod[0] = index_of(m.secondary_species);
         if (!m.ion_species.empty()) prod[1] = index_of(m.ion_species);
         const Tables t(m.config);
-        d.check(aurora_hip_mcc_load(
-            d.get(), target, static_cast<int>(t.kind.size()), t.kind.data(), t.threshold.data(),
-            t.rows.data(), t.energy.data(), t.sigma.data(), m.config.neutral_density,
-            m.config.max_frequency, static_cast<int64_t>(m.config.max_candidates_per_particle),
-            m.config.neutral_mass, m.config.neutral_temperature, prod, cfg_.seed));
-        apply_options(d, 0, m.config);
+        // models in config order (Simulation.cpp:490-518): the first through
+        // aurora_hip_mcc_load, the others added after it
+        int model = 0;
+        if (&m == &cfg_.collision_models.front())
+            d.check(aurora_hip_mcc_load(
+                d.get(), target, static_cast<int>(t.kind.size()), t.kind.data(),
+                t.threshold.data(), t.rows.data(), t.energy.data(), t.sigma.data(),
+                m.config.neutral_density, m.config.max_frequency,
+                static_cast<int64_t>(m.config.max_candidates_per_particle), m.config.neutral_mass,
+                m.config.neutral_temperature, prod, cfg_.seed));
+        else
+            d.check(aurora_hip_mcc_add(
+                d.get(), target, static_cast<int>(t.kind.size()), t.kind.data(),
+                t.threshold.data(), t.rows.data(), t.energy.data(), t.sigma.data(),
+                m.config.neutral_density, m.config.max_frequency,
+                static_cast<int64_t>(m.config.max_candidates_per_particle), m.config.neutral_mass,
+                m.config.neutral_temperature, prod, cfg_.seed, &model));
+        apply_model_options(d, model, m.config);
     }
 }
 
@@ -702,32 +726,44 @@ std::vector<SpeciesPower1D> Simulation::species_power_transfer() const {
     return out;
 }
 
+// One column per channel of every model, in model order (the reference's
+// diagnostic offsets, Simulation.cpp:420-480); candidates and nulls summed
+// over the models.
 CollisionDiagnostics Simulation::collision_diagnostics() const {
     CollisionDiagnostics d;
-    if (cfg_.collision_models.empty()) return d;
-    const NamedCollisionConfig& m = cfg_.collision_models[0];
-    for (const auto& ch : m.config.channels)
-        d.channel_names.push_back(m.name.empty() ? ch.name : m.name + "." + ch.name);
-    const CollisionStepStatistics s = collision_statistics();
-    d.candidates = s.candidates;
-    d.null_collisions = s.null_collisions;
-    d.channel_collisions = s.channel_collisions;
-    d.channel_energy_change = s.channel_projectile_energy_change;
+    for (size_t k = 0; k < cfg_.collision_models.size(); ++k) {
+        const NamedCollisionConfig& m = cfg_.collision_models[k];
+        const size_t nch = m.config.channels.size();
+        for (const auto& ch : m.config.channels)
+            d.channel_names.push_back(m.name.empty() ? ch.name : m.name + "." + ch.name);
+        std::vector<uint64_t> counts(nch + 3);
+        std::vector<double> ech(std::max<size_t>(nch, 1));
+        dev_->check(aurora_hip_mcc_model_stats(dev_->get(), static_cast<int>(k), counts.data(),
+                                               ech.data()));
+        d.candidates += counts[0];
+        d.null_collisions += counts[1];
+        d.channel_collisions.insert(d.channel_collisions.end(), counts.begin() + 2,
+                                    counts.begin() + 2 + static_cast<long>(nch));
+        d.channel_energy_change.insert(d.channel_energy_change.end(), ech.begin(),
+                                       ech.begin() + static_cast<long>(nch));
+    }
     return d;
 }
 
 CollisionStepStatistics Simulation::collision_statistics() const {
     CollisionStepStatistics s;
-    if (cfg_.collision_models.empty()) return s;
-    const size_t nch = cfg_.collision_models[0].config.channels.size();
-    std::vector<uint64_t> counts(nch + 3);
-    std::vector<double> ech(std::max<size_t>(nch, 1));
-    dev_->check(aurora_hip_mcc_stats(dev_->get(), counts.data(), ech.data()));
-    s.candidates = counts[0];
-    s.null_collisions = counts[1];
-    s.channel_collisions.assign(counts.begin() + 2, counts.begin() + 2 + static_cast<long>(nch));
-    s.channel_projectile_energy_change.assign(ech.begin(), ech.begin() + static_cast<long>(nch));
-    s.secondaries = counts[2 + nch];
+    const CollisionDiagnostics d = collision_diagnostics();
+    s.candidates = d.candidates;
+    s.null_collisions = d.null_collisions;
+    s.channel_collisions = d.channel_collisions;
+    s.channel_projectile_energy_change = d.channel_energy_change;
+    for (size_t k = 0; k < cfg_.collision_models.size(); ++k) {
+        const size_t nch = cfg_.collision_models[k].config.channels.size();
+        std::vector<uint64_t> counts(nch + 3);
+        dev_->check(aurora_hip_mcc_model_stats(dev_->get(), static_cast<int>(k), counts.data(),
+                                               nullptr));
+        s.secondaries += counts[2 + nch];
+    }
     return s;
 }
 

@@ -830,10 +830,12 @@ void run1d_case(const std::filesystem::path& case_file, const std::filesystem::p
             cfg.units = u == "si" ? UnitSystem::SI : UnitSystem::Normalized;
             cfg.permittivity = u == "si" ? kEps0 : 1.0;
         } else if (key == "model") {
-            // model <name> <target> <density> <max_frequency> <neutral_mass> <max_candidates>
+            // model <name> <target> <density> <max_frequency> <neutral_mass>
+            //       <max_candidates> <neutral_temperature K>
             NamedCollisionConfig m;
             ls >> m.name >> m.species >> m.config.neutral_density >> m.config.max_frequency >>
-                m.config.neutral_mass >> m.config.max_candidates_per_particle;
+                m.config.neutral_mass >> m.config.max_candidates_per_particle >>
+                m.config.neutral_temperature;
             cfg.collision_models.push_back(m);
         } else if (key == "channel") {
             // channel <name> <kind> <threshold J> <table file> <secondary|-> <ion|->

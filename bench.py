@@ -150,13 +150,39 @@ def discharge_leg(args):
             "kernels_ms": kernels}
 
 
+def sequential_estimate(ctx, reqs, strata, log_each=False):
+    """The sequential single-instance baseline of a request stream, from a
+    stratified sample: requests ordered by particle count, cut into `strata`
+    bins of equal count; each bin's median request runs alone on the GPU
+    (through the same kernel) and stands for every request of its bin.
+    Returns (estimated seconds for all requests, sample results by index,
+    per-bin detail)."""
+    from aurora_pic import serving
+    order = sorted(range(len(reqs)), key=lambda i: reqs[i].particles)
+    bins = [order[k * len(order) // strata:(k + 1) * len(order) // strata]
+            for k in range(strata)]
+    total, sample, detail = 0.0, {}, []
+    for b in bins:
+        if not b:
+            continue
+        i = b[len(b) // 2]
+        res, wall, _ = serving.run_sequential(ctx, [reqs[i]])
+        sample[i] = res[0]
+        total += wall * len(b)
+        detail.append({"particles": int(reqs[i].particles), "requests": len(b), "alone_s": wall})
+        if log_each:
+            log(f"alone: request {i} ({reqs[i].particles} electrons) {wall:.4f} s x {len(b)}")
+    return total, sample, detail
+
+
 def serving_leg(args):
     """Batched swarm serving: Poisson arrivals at 200 req/s (E/N log-uniform
     1-1000 Td, log-normal particle counts, synthetic gas), admitted by HBM
     footprint and advanced together, mixed with a 1D3V discharge ensemble
-    stepped between swarm launches (latency); plus the capacity of the
-    batched path (every request present at t = 0) against the same requests
-    run one at a time, each alone on the GPU."""
+    that steps on its own context and stream while the swarm launches run
+    (latency); plus the capacity of the batched path (every request present
+    at t = 0) against the same requests run one at a time, each alone on
+    the GPU (every request, or a stratified sample for the full stream)."""
     from aurora_pic import hip, serving
     from aurora_pic import workloads as wl
     import numpy as np
@@ -183,9 +209,19 @@ def serving_leg(args):
     # capacity: every request present at t = 0, advanced together
     sat = serving.serve(ctx, reqs, np.zeros(R), budget, steps_per_launch=args.serving_launch_steps,
                         work_item_limit=work_items)
+    log(f"capacity: {R} requests in {sat.wall_s:.3f} s batched")
     # the same requests one at a time, each alone on the GPU
-    seq_results, seq_wall, seq_steps = serving.run_sequential(ctx, reqs)
-    log(f"capacity: {R} requests in {sat.wall_s:.3f} s batched, {seq_wall:.3f} s sequential")
+    if args.serving_full:
+        seq_wall, seq_sample, strata = sequential_estimate(ctx, reqs, 16, log_each=True)
+        seq_results = {i: r for i, r in seq_sample.items()}
+        seq_kind = "stratified sample of 16 requests (median of each particle-count sixteenth)"
+    else:
+        res, seq_wall, _ = serving.run_sequential(ctx, reqs)
+        seq_results = dict(enumerate(res))
+        strata = None
+        seq_kind = "every request"
+    electron_steps = sum(r.steps * r.particles for r in reqs)
+    log(f"sequential ({seq_kind}): {seq_wall:.3f} s")
     log(f"serving {R} swarm requests x {steps} steps (arrivals 200/s, budget "
         f"{args.serving_hbm_gib} GiB) + {args.ensemble_instances} discharge instances")
     t0 = time.perf_counter()
@@ -198,12 +234,11 @@ def serving_leg(args):
     log(f"serving: {json.dumps(summ)}; ensemble {ens_steps[0]} steps")
     # every served result against its sequential run (bit-identical by design)
     diffs = []
-    for i, r in enumerate(seq_results):
+    for i, r in seq_results.items():
         for b in (rep.served[i].result, sat.served[i].result):
             for f in ("drift", "mean_energy_ev", "ionization_rate"):
                 x, y = getattr(r, f), getattr(b, f)
                 diffs.append(abs(x - y) / max(abs(x), 1e-300))
-    small = [i for i, r in enumerate(reqs) if r.particles < 30000]
     ctx.close()
     ens_ctx.close()
     lat = rep.latencies()
@@ -212,7 +247,8 @@ def serving_leg(args):
                     f"log-uniform, particles log-normal median 5e4 [1e4, 1e6], {steps} steps "
                     f"({burn} burn-in), synthetic elastic/excitation/ionization gas at 1e21 "
                     f"m^-3; mixed with {args.ensemble_instances} distinct-seed 1D3V discharge "
-                    f"instances of {args.ensemble_particles} particles (HIP-graph ensemble)",
+                    f"instances of {args.ensemble_particles} particles (HIP-graph ensemble on "
+                    f"its own stream, stepped while swarm launches run)",
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "requests_per_s_offered_load": summ["requests_per_s"],
@@ -221,12 +257,14 @@ def serving_leg(args):
         "launches": summ["launches"],
         "budget_refusals": summ["budget_refusals"],
         "capacity_requests_per_s": R / sat.wall_s,
-        "capacity_electron_steps_per_s": seq_steps / sat.wall_s,
+        "capacity_electron_steps_per_s": electron_steps / sat.wall_s,
+        "sequential": seq_kind,
+        "sequential_strata": strata,
         "sequential_requests_per_s": R / seq_wall,
-        "sequential_electron_steps_per_s": seq_steps / seq_wall,
+        "sequential_electron_steps_per_s": electron_steps / seq_wall,
         "batched_over_sequential": seq_wall / sat.wall_s,
-        "small_requests": len(small),
         "max_rel_diff_vs_sequential": max(diffs) if diffs else None,
+        "ensemble_steps": ens_steps[0],
         "ensemble_particle_updates_per_s": ens_updates / wall,
     }
 
@@ -349,6 +387,10 @@ def main():
     log("kernels:", json.dumps(kernels))
     ke, fe, l1, live = ctx.energy(dt)
     log(f"live {live}, kinetic {ke.sum():.6e} J, field {fe:.6e} J")
+    # the attainable HBM bandwidth of hand-written streaming kernels
+    stream_peak = {"copy_gbs": ctx.stream_bandwidth(0),
+                   "soa7_in_place_gbs": ctx.stream_bandwidth(1)}
+    log("stream peak:", json.dumps(stream_peak))
     ctx.close()
 
     traffic = None
@@ -407,7 +449,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "push_deposit (gather+Boris+drift+wrap+int64 deposit)",
-                     "bytes_per_update": BYTES_PER_UPDATE},
+                     "bytes_per_update": BYTES_PER_UPDATE,
+                     "attainable": stream_peak},
         "cpu_baseline": base,
         "discharge_1d3v": discharge_out,
         "serving": serving_out,

@@ -238,6 +238,17 @@ int aurora_hip_set_sort_threshold(aurora_hip_ctx* ctx, double fraction);
  * used by this policy. */
 enum { AURORA_SORT_FIXED = 0, AURORA_SORT_ADAPTIVE = 1 };
 int aurora_hip_set_sort_policy(aurora_hip_ctx* ctx, int policy, int64_t max_interval);
+/* How a sort orders particles.  AURORA_SORT_LOCAL (default): a species
+ * still tile-sorted from its last sort is re-sorted by a counting sort over
+ * its old tile segments (particles move at most into a neighbouring tile
+ * between sorts; those that moved further are placed through a short radix
+ * sort), and by the full radix sort otherwise or when over 1/8 of the
+ * particles moved further.  AURORA_SORT_RADIX: always the full stable radix
+ * sort.  Both are deterministic and give the same set of particles per tile
+ * (the order inside a tile can differ); results do not depend on it.  No
+ * reference counterpart (the reference keeps particles in creation order). */
+enum { AURORA_SORT_LOCAL = 0, AURORA_SORT_RADIX = 1 };
+int aurora_hip_set_sort_method(aurora_hip_ctx* ctx, int method);
 int aurora_hip_sort(aurora_hip_ctx* ctx);
 
 /* ----------------------------------------------------------- checkpoint */
@@ -268,6 +279,35 @@ int aurora_hip_mcc_load(aurora_hip_ctx* ctx, int target_species, int nchan,
                         double max_frequency, int64_t max_candidates,
                         double neutral_mass, double neutral_temperature,
                         const int* product_species, uint64_t seed);
+/* Further named null-collision models (Config::collision_models,
+ * Simulation.cpp:405-520): each on its own target species, with the
+ * arguments of aurora_hip_mcc_load; *model receives its index (model 0 is
+ * the one aurora_hip_mcc_load loads, which clears the others).  The step
+ * applies the models in index order to the particles present at the start
+ * of the step (Simulation.cpp:835-927), each on its own random stream, then
+ * places all their ionization products after one all-or-nothing capacity
+ * check (Simulation.cpp:929-1005).  Two models may not target one species. */
+int aurora_hip_mcc_add(aurora_hip_ctx* ctx, int target_species, int nchan,
+                       const int* kind, const double* threshold,
+                       const int64_t* rows, const double* const* energy,
+                       const double* const* sigma, double neutral_density,
+                       double max_frequency, int64_t max_candidates,
+                       double neutral_mass, double neutral_temperature,
+                       const int* product_species, uint64_t seed, int* model);
+/* aurora_hip_mcc_channel_options and aurora_hip_mcc_stats for the
+ * context's model of index `model`. */
+int aurora_hip_mcc_model_options(aurora_hip_ctx* ctx, int model, int channel,
+                                 int angular, int energy_frame, int64_t rows,
+                                 const double* energy,
+                                 const double* mean_cosine);
+int aurora_hip_mcc_model_stats(aurora_hip_ctx* ctx, int model, uint64_t* counts,
+                               double* energy_change);
+/* Config::velocity_dimensions (Config.hpp:172) of a 1D run: 3 (default)
+ * collides 1D3V velocities; 1 applies the scalar 1V collide
+ * (NullCollisionModel::collide(double&), Collision.cpp:831-905) to v_x, in
+ * the step and in aurora_hip_mcc_collide, and rejects ionization and
+ * anisotropic channels as the reference does. */
+int aurora_hip_set_velocity_dimensions(aurora_hip_ctx* ctx, int dims);
 /* Per-channel options of CollisionChannelConfig (Config.hpp:82-108) for
  * the context's MCC model (model = 0) or the swarm gas (model = 1):
  * angular_scattering (AURORA_SCATTER_*; anisotropic only for elastic
@@ -369,6 +409,17 @@ int aurora_hip_swarm_submit(aurora_hip_ctx* ctx,
  * one launch; *active receives the number still running. */
 int aurora_hip_swarm_advance(aurora_hip_ctx* ctx, int64_t max_steps,
                              int64_t* active);
+/* aurora_hip_swarm_advance without waiting: queues the launch on the
+ * context's stream and returns at once, so other device work (another
+ * context's ensemble, say) runs beside it and requests can be submitted
+ * meanwhile (they join the next launch).  One launch is in flight per
+ * context: a further launch first completes the previous one.
+ * aurora_hip_swarm_collect completes the launch in flight (wait = 1) or
+ * checks it (wait = 0, *busy = 1 while it runs); once it has completed, the
+ * requests it finished report done in aurora_hip_swarm_poll. */
+int aurora_hip_swarm_launch(aurora_hip_ctx* ctx, int64_t max_steps,
+                            int64_t* active);
+int aurora_hip_swarm_collect(aurora_hip_ctx* ctx, int wait, int* busy);
 /* done = 1 once the request finished; result then holds its outputs and
  * the request's device memory is released. */
 int aurora_hip_swarm_poll(aurora_hip_ctx* ctx, int64_t handle, int* done,
@@ -455,6 +506,14 @@ int aurora_hip_ensemble_time(aurora_hip_ctx* ctx, int64_t handle,
 int aurora_hip_ensemble_destroy(aurora_hip_ctx* ctx, int64_t handle);
 
 /* -------------------------------------------------------------- timing */
+/* The attainable HBM bandwidth (GB/s, bytes read + written per second) of
+ * hand-written streaming kernels on this device, best of `repeats` timed
+ * launches over `bytes` of traffic: pattern 0 copies one array with 16-byte
+ * accesses; pattern 1 reads and rewrites seven 4-byte arrays in place (the
+ * 3D push's particle streams).  No reference counterpart: it calibrates
+ * the HBM roofline the kernels are measured against. */
+int aurora_hip_stream_bandwidth(aurora_hip_ctx* ctx, int pattern, int64_t bytes,
+                                int repeats, double* gbs);
 /* Per-kernel HIP-event timing on the context stream (off by default). */
 int aurora_hip_kernel_timing(aurora_hip_ctx* ctx, int enable);
 /* Total milliseconds and launch count of kernel `name` since enabled. */

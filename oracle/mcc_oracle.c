@@ -187,6 +187,31 @@ static void angular_velocity(const double* g0, double speed, double mc, or_strea
 
 /* Bounded Maxwellian neutral (Collision.cpp:716-731): Box-Muller normals,
  * rejected above 8 sigma. */
+/* decision margins: relative distance of a compared value to its boundary */
+static void note_margin(or_mcc_stats* st, double a, double b, double scale) {
+    const double d = fabs(a - b) / (scale > 0.0 ? scale : 1.0);
+    if (d < st->margin) st->margin = d;
+}
+
+/* the rate table decisions at relative speed g: each channel's energy
+ * against its threshold (below it the rate is zero) */
+static void note_threshold_margins(const or_mcc_model* m, double g, or_mcc_stats* st) {
+    for (int c = 0; c < m->nchan; ++c)
+        if (m->threshold[c] > 0.0)
+            note_margin(st, collision_energy(m, c, g), m->threshold[c], m->threshold[c]);
+}
+
+/* the selection against the cumulative rates (each boundary it was compared with) */
+static void note_selection_margins(const or_mcc_model* m, const double* rates, double selection,
+                                   or_mcc_stats* st) {
+    double cumulative = 0.0;
+    for (int c = 0; c < m->nchan; ++c) {
+        cumulative += rates[c];
+        note_margin(st, selection, cumulative, m->max_frequency);
+        if (selection < cumulative) break;
+    }
+}
+
 static void neutral_velocity(const or_mcc_model* m, or_stream* rng, double* w) {
     w[0] = w[1] = w[2] = 0.0;
     if (m->neutral_vstd == 0.0) return;
@@ -206,6 +231,7 @@ void or_mcc_collide_3v(const or_mcc_model* m, double* v, double dt,
                        or_stream* rng, or_mcc_stats* st) {
     /* Collision.cpp:907-993 */
     memset(st, 0, sizeof *st);
+    st->margin = 1.0;
     if (!isfinite(v[0]) || !isfinite(v[1]) || !isfinite(v[2])) {
         st->status = 3;
         return;
@@ -214,6 +240,7 @@ void or_mcc_collide_3v(const or_mcc_model* m, double* v, double dt,
     for (;;) {
         if (validate_bound(m, speed3(v))) { st->status = 1; return; }
         const double waiting = -log(or_stream_u01(rng)) / m->max_frequency;
+        note_margin(st, waiting, dt - elapsed, dt);
         if (!isfinite(waiting) || waiting >= dt - elapsed) break;
         elapsed += waiting;
         ++st->candidates;
@@ -223,7 +250,9 @@ void or_mcc_collide_3v(const or_mcc_model* m, double* v, double dt,
         double g[3] = {v[0] - w[0], v[1] - w[1], v[2] - w[2]};
         double rates[8];
         if (rates_for_speed(m, speed3(g), rates)) { st->status = 1; return; }
+        note_threshold_margins(m, speed3(g), st);
         const double selection = or_stream_u01(rng) * m->max_frequency;
+        note_selection_margins(m, rates, selection, st);
         double cumulative = 0.0;
         int accepted = -1;
         for (int c = 0; c < m->nchan; ++c) {
@@ -287,6 +316,83 @@ void or_mcc_collide_3v(const or_mcc_model* m, double* v, double dt,
             }
             ++st->nsec;
         }
+        ++st->chan[c];
+        if (m->kind[c] == OR_ATTACHMENT) break;
+    }
+}
+
+/* bounded Maxwellian scalar neutral velocity (Collision.cpp:686-699), one
+ * Box-Muller normal from (u1, u2) per attempt */
+static double neutral_velocity_1v(const or_mcc_model* m, or_stream* rng, or_mcc_stats* st) {
+    if (m->neutral_vstd == 0.0) return 0.0;
+    const double lim = 8.0 * m->neutral_vstd;
+    double w = 0.0;
+    for (int attempt = 0; attempt < 1024; ++attempt) {
+        const double u1 = or_stream_u01(rng), u2 = or_stream_u01(rng);
+        w = m->neutral_vstd * sqrt(-2.0 * log(u1)) * cos(2.0 * PI * u2);
+        note_margin(st, fabs(w), lim, lim);
+        if (fabs(w) <= lim) return w;
+    }
+    return w;
+}
+
+void or_mcc_collide_1v(const or_mcc_model* m, double* v, double dt,
+                       or_stream* rng, or_mcc_stats* st) {
+    /* Collision.cpp:831-905 with apply_channel's scalar branch (717-760) */
+    memset(st, 0, sizeof *st);
+    st->margin = 1.0;
+    if (!isfinite(*v)) {
+        st->status = 3;
+        return;
+    }
+    double elapsed = 0.0;
+    for (;;) {
+        if (validate_bound(m, fabs(*v))) { st->status = 1; return; }
+        const double waiting = -log(or_stream_u01(rng)) / m->max_frequency;
+        note_margin(st, waiting, dt - elapsed, dt);
+        if (!isfinite(waiting) || waiting >= dt - elapsed) break;
+        elapsed += waiting;
+        ++st->candidates;
+        if (st->candidates > (uint64_t)m->max_candidates) { st->status = 2; return; }
+        const double w = neutral_velocity_1v(m, rng, st);
+        const double g = *v - w;
+        double rates[8];
+        if (rates_for_speed(m, fabs(g), rates)) { st->status = 1; return; }
+        note_threshold_margins(m, fabs(g), st);
+        const double selection = or_stream_u01(rng) * m->max_frequency;
+        note_selection_margins(m, rates, selection, st);
+        double cumulative = 0.0;
+        int accepted = -1;
+        for (int c = 0; c < m->nchan; ++c) {
+            cumulative += rates[c];
+            if (selection < cumulative) { accepted = c; break; }
+        }
+        if (accepted < 0) { ++st->nulls; continue; }
+        const int c = accepted;
+        const double e_before = 0.5 * m->particle_mass * *v * *v;
+        if (m->kind[c] == OR_ATTACHMENT) {
+            *v = 0.0;
+        } else if (m->kind[c] == OR_CHARGE_EXCHANGE) {
+            *v = w;
+        } else {
+            double energy = 0.5 * m->particle_mass * g * g;
+            if (m->kind[c] == OR_EXCITATION) {
+                energy = energy - m->threshold[c];
+                if (energy < 0.0) energy = 0.0;
+            }
+            const double sp = sqrt(2.0 * energy / m->particle_mass);
+            const double u = or_stream_u01(rng);
+            note_margin(st, u, 0.5, 1.0);
+            const double scattered = u < 0.5 ? -sp : sp;
+            if (m->kind[c] == OR_ELASTIC && m->neutral_mass > 0.0) {
+                const double mt = m->particle_mass + m->neutral_mass;
+                *v = (m->particle_mass * *v + m->neutral_mass * w) / mt +
+                     (m->neutral_mass / mt) * scattered;
+            } else {
+                *v = w + scattered;
+            }
+        }
+        st->energy_change[c] += 0.5 * m->particle_mass * *v * *v - e_before;
         ++st->chan[c];
         if (m->kind[c] == OR_ATTACHMENT) break;
     }

@@ -158,6 +158,12 @@ typedef struct {
     double ion_v[8][3];
     int status;                   /* 0 ok; 1 rate above bound; 2 candidate
                                      limit; 3 non-finite velocity */
+    double margin;                /* smallest relative distance of any branch
+                                     decision of the call to its boundary
+                                     (waiting time vs the time left, channel
+                                     selection vs the cumulative rates, energy
+                                     vs a threshold, scatter sign, neutral
+                                     rejection); 1 when none was taken */
 } or_mcc_stats;
 
 /* RNG stream for one particle-step: key=(seed lo, seed hi),
@@ -175,6 +181,8 @@ double or_stream_u01(or_stream* s);
 
 double or_xs_eval(int64_t rows, const double* e, const double* s, double x);
 void or_mcc_collide_3v(const or_mcc_model* m, double* v, double dt,
+                       or_stream* rng, or_mcc_stats* st);
+void or_mcc_collide_1v(const or_mcc_model* m, double* v, double dt,
                        or_stream* rng, or_mcc_stats* st);
 
 /* ---- A15 swarm run_field (Swarm.cpp:940-1250), fixed population. */

@@ -84,6 +84,7 @@ class _MccStats(C.Structure):
         ("sec_v", (C.c_double * 3) * 8),
         ("ion_v", (C.c_double * 3) * 8),
         ("status", C.c_int),
+        ("margin", C.c_double),
     ]
 
 
@@ -247,6 +248,8 @@ class Oracle:
         L.or_xs_eval.argtypes = [C.c_int64, _dp, _dp, C.c_double]
         L.or_xs_eval.restype = C.c_double
         L.or_mcc_collide_3v.argtypes = [C.POINTER(_MccModel), _dp, C.c_double,
+                                        C.POINTER(_Stream), C.POINTER(_MccStats)]
+        L.or_mcc_collide_1v.argtypes = [C.POINTER(_MccModel), _dp, C.c_double,
                                         C.POINTER(_Stream), C.POINTER(_MccStats)]
         L.or_swarm_run.argtypes = [C.POINTER(_MccModel), C.POINTER(_SwarmCfg), _dp,
                                    C.c_uint64, C.c_uint32, C.POINTER(_SwarmResult)]
@@ -481,6 +484,16 @@ class Oracle:
         self.lib.or_mcc_collide_3v(C.byref(model.struct), _ptr(v, _dp), dt, C.byref(s),
                                    C.byref(st))
         return v, st
+
+    def collide_1v(self, model: MccModel, v, dt, seed, stream, uid, step):
+        """The scalar collide (Collision.cpp:831-905) on v (a float) with the
+        stream keyed (seed, stream, uid, step)."""
+        x = np.array([float(v)])
+        st = _MccStats()
+        s = self.stream(seed, stream, uid, step)
+        self.lib.or_mcc_collide_1v(C.byref(model.struct), _ptr(x, _dp), dt, C.byref(s),
+                                   C.byref(st))
+        return float(x[0]), st
 
     def swarm(self, model: MccModel, cfg: dict, vel0, seed, instance=0):
         c = _SwarmCfg(**cfg)

@@ -638,7 +638,7 @@ int ref_run2d(const char* output_dir, const int64_t* n, const double* L, double 
 // program's --run1d mode): mesh, drive, species lines, and optionally
 // "units si", "max_particles N" and named null-collision models
 //   model <name> <target species> <neutral_density> <max_frequency>
-//         <neutral_mass> <max_candidates>
+//         <neutral_mass> <max_candidates> <neutral_temperature K>
 //   channel <name> <elastic|excitation|ionization> <threshold J>
 //           <table file, energies J and m^2> <secondary|-> <ion|->
 // (channels belong to the model above them).  seed >= 0 replaces the case's
@@ -692,7 +692,8 @@ int ref_run1d_case(const char* case_path, const char* output_dir, int64_t seed, 
                 pic::NamedCollisionConfig m;
                 std::size_t cand = 64;
                 ls >> m.name >> m.config.species >> m.config.neutral_density >>
-                    m.config.max_frequency >> m.config.neutral_mass >> cand;
+                    m.config.max_frequency >> m.config.neutral_mass >> cand >>
+                    m.config.neutral_temperature;
                 m.config.enabled = true;
                 m.config.model = pic::CollisionModelKind::NullCollision;
                 m.config.max_candidates_per_particle = cand;

@@ -1,16 +1,22 @@
-"""Write tests/golden/run1d/rf_1d3v_mcc/ from the REAL reference: a 1D3V RF
-discharge with electron-impact MCC and ionization products (the physics of
-the 1D3V discharge configuration at a size the reference runs in seconds).
+"""Write tests/golden/run1d/<case>/ from the REAL reference: 1D RF discharges
+with null-collision MCC (the physics of the 1D3V discharge configuration at
+a size the reference runs in seconds).
 
-  mesh       129 nodes over 0.067 m, grounded left electrode, 200 V at
-             13.56 MHz on the right, absorbing walls, SI units
-  species    2e4 electrons (2 eV) and 2e4 singly charged ions (0.026 eV,
-             6.67e-27 kg), uniform over the gap, 2.56e14 m^-3
-  gas        the synthetic helium-like gas: 1,000 log-spaced energies
-             1e-3..1e3 eV, elastic 6e-20 m^2 flat, Lotz excitation (19.8 eV)
-             and ionization (24.6 eV), 1e21 m^-3, cold neutrals of 4 amu
-  time       dt = 1 / (400 x 13.56 MHz), 2,000 steps (5 RF periods),
-             outputs every 200 steps
+Common: 129 nodes over 0.067 m, grounded left electrode, 200 V at 13.56 MHz
+on the right, absorbing walls, SI units; 2e4 electrons (2 eV) and 2e4 singly
+charged ions (0.026 eV, 6.67e-27 kg) uniform over the gap at 2.56e14 m^-3;
+dt = 1 / (400 x 13.56 MHz), 2,000 steps (5 RF periods), outputs every 200
+steps; the synthetic helium-like gas at 1e21 m^-3 (1,000 log-spaced
+energies 1e-3..1e3 eV: elastic 6e-20 m^2 flat, Lotz excitation at 19.8 eV and
+ionization at 24.6 eV; cold neutrals of 4 amu).
+
+  rf_1d3v_mcc        1D3V, one model: electron impact with ionization
+                     products (e_he)
+  rf_1d3v_mcc_ions   1D3V, two named models: e_he, then ion-neutral elastic
+                     and resonant charge exchange (i_he, flat 5e-19 m^2 and
+                     3e-19 m^2) on 300 K neutrals
+  rf_1d1v_mcc        1D1V, the scalar collide: electron elastic and
+                     excitation (e_he1v, no ionization: 1V has no products)
 
 The reference's MCC draws from std::mt19937_64 and the device from Philox,
 so the comparison is statistical: the reference runs once per seed (the
@@ -21,7 +27,7 @@ ledgers (boundary_losses.csv, power_transfer.csv) and the final electron
 energy distribution (fraction per log-energy bin) and mean energy.  The
 outputs of the case's own seed are kept whole for the column contracts.
 
-    python tests/golden/make_run1d_mcc_golden.py [seeds]
+    python tests/golden/make_run1d_mcc_golden.py [seeds] [case ...]
 """
 import csv
 import os
@@ -38,37 +44,68 @@ sys.path[:0] = [os.path.join(REPO, "oracle"), os.path.join(REPO, "aurora-pic_amd
 import oracle_py  # noqa: E402
 from aurora_pic import workloads as wl  # noqa: E402
 
-CASE = "rf_1d3v_mcc"
-OUT = os.path.join(HERE, "run1d", CASE)
+ROOT = os.path.join(HERE, "run1d")
 SEED = 31
 NX, L, N, STEPS, EVERY = 129, 0.067, 20000, 2000, 200
+WEIGHT = 2.56e14 * L / N
 ION_MASS = 6.67e-27
 EEDF_EDGES = np.geomspace(1.0e-2, 1.0e3, 26)        # eV; 25 log bins
-CHANNELS = ("elastic", "excitation", "ionization")
+ELECTRON = ("elastic", "excitation", "ionization")
+CASES = ("rf_1d3v_mcc", "rf_1d3v_mcc_ions", "rf_1d1v_mcc")
 
 
-def write_case(d):
+def case_dir(case):
+    return os.path.join(ROOT, case)
+
+
+def _table(path, name, e, s):
+    with open(path, "w") as f:
+        f.write(f"# {name}: energy (J), cross section (m^2)\n")
+        for a, b in zip(e, s):
+            f.write(f"{float(a)!r} {float(b)!r}\n")
+
+
+def write_case(case):
+    d = case_dir(case)
+    os.makedirs(d, exist_ok=True)
     gas = wl.synthetic_gas(1000, 1.0e21)
-    for (_, _, e, s), name in zip(gas["channels"], CHANNELS):
-        with open(os.path.join(d, name + ".dat"), "w") as f:
-            f.write(f"# {name}: energy (J), cross section (m^2)\n")
-            for a, b in zip(e, s):
-                f.write(f"{float(a)!r} {float(b)!r}\n")
-    w = 2.56e14 * L / N
+    for (_, _, e, s), name in zip(gas["channels"], ELECTRON):
+        _table(os.path.join(d, name + ".dat"), name, e, s)
+    vdim = 1 if case == "rf_1d1v_mcc" else 3
     vte = (2.0 * wl.EV / wl.ME) ** 0.5
     vti = (0.026 * wl.EV / ION_MASS) ** 0.5
     dt = 1.0 / (400 * 13.56e6)
     lines = [f"nx {NX}", f"length {L!r}", f"dt {dt!r}", f"steps {STEPS}",
-             f"output_interval {EVERY}", "dirichlet 1", "velocity_dimensions 3", f"seed {SEED}",
-             "units si", "precision 32", f"max_particles {4 * N}",
+             f"output_interval {EVERY}", "dirichlet 1", f"velocity_dimensions {vdim}",
+             f"seed {SEED}", "units si", "precision 32", f"max_particles {4 * N}",
              "electrodes 0.0 0.0 0.0 0.0 0.0 200.0 13560000.0 0.0",
-             f"species electrons {-wl.EV!r} {wl.ME!r} {w!r} {N} 0.0 {vte!r} 0.0 {L!r}",
-             f"species ions {wl.EV!r} {ION_MASS!r} {w!r} {N} 0.0 {vti!r} 0.0 {L!r}",
-             f"model e_he electrons {gas['neutral_density']!r} {gas['max_frequency']!r} "
-             f"{wl.HELIUM_MASS!r} 64",
-             "channel elastic elastic 0.0 elastic.dat - -",
-             f"channel excitation excitation {19.8 * wl.EV!r} excitation.dat - -",
-             f"channel ionization ionization {24.6 * wl.EV!r} ionization.dat electrons ions"]
+             f"species electrons {-wl.EV!r} {wl.ME!r} {WEIGHT!r} {N} 0.0 {vte!r} 0.0 {L!r}",
+             f"species ions {wl.EV!r} {ION_MASS!r} {WEIGHT!r} {N} 0.0 {vti!r} 0.0 {L!r}"]
+    if vdim == 1:
+        two = gas["channels"][:2]
+        nu = wl.max_collision_frequency(two, gas["neutral_density"])
+        lines += [f"model e_he1v electrons {gas['neutral_density']!r} {nu!r} "
+                  f"{wl.HELIUM_MASS!r} 64 0.0",
+                  "channel elastic elastic 0.0 elastic.dat - -",
+                  f"channel excitation excitation {19.8 * wl.EV!r} excitation.dat - -"]
+    else:
+        lines += [f"model e_he electrons {gas['neutral_density']!r} {gas['max_frequency']!r} "
+                  f"{wl.HELIUM_MASS!r} 64 0.0",
+                  "channel elastic elastic 0.0 elastic.dat - -",
+                  f"channel excitation excitation {19.8 * wl.EV!r} excitation.dat - -",
+                  f"channel ionization ionization {24.6 * wl.EV!r} ionization.dat electrons ions"]
+    if case == "rf_1d3v_mcc_ions":
+        e = np.geomspace(1e-3, 1e3, 200) * wl.EV
+        ion = [(wl.ELASTIC, 0.0, e, np.full(200, 5e-19)),
+               (oracle_py.CHARGE_EXCHANGE, 0.0, e, np.full(200, 3e-19))]
+        _table(os.path.join(d, "ion_elastic.dat"), "ion_elastic", ion[0][2], ion[0][3])
+        _table(os.path.join(d, "ion_cx.dat"), "ion_cx", ion[1][2], ion[1][3])
+        # the bound: 8 sigma thermal neutrals on top of an ion of 650 eV
+        nu = 1.0e21 * 8e-19 * 2.5e5
+        # resonant charge exchange: the neutral has the ion's mass
+        lines += [f"model i_he ions 1e+21 {nu!r} {ION_MASS!r} 64 300.0",
+                  "channel elastic elastic 0.0 ion_elastic.dat - -",
+                  "channel cx charge_exchange 0.0 ion_cx.dat - -"]
     with open(os.path.join(d, "case.txt"), "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -79,6 +116,11 @@ def read_csv(path):
     return rows[0], np.array([[float(x) for x in r] for r in rows[1:]])
 
 
+def channel_columns(header):
+    """The channel names of a collisions.csv, in column order."""
+    return [h[len("collisions_"):] for h in header if h.startswith("collisions_")]
+
+
 def summarise(out_dir):
     """The per-run statistics both sides are compared on (also used by
     tests/test_gpu_run1d_mcc.py on the device's outputs)."""
@@ -86,9 +128,10 @@ def summarise(out_dir):
     hs, s = read_csv(os.path.join(out_dir, "scalars.csv"))
     hb, b = read_csv(os.path.join(out_dir, "boundary_losses.csv"))
     hp, p = read_csv(os.path.join(out_dir, "power_transfer.csv"))
+    names = channel_columns(hc)
     win = [hc.index(k) for k in ["candidates", "null_collisions"] +
-           [f"collisions_e_he.{ch}" for ch in CHANNELS]]
-    ewin = [hc.index(f"tracked_kinetic_energy_change_e_he.{ch}_J_m-2") for ch in CHANNELS]
+           [f"collisions_{ch}" for ch in names]]
+    ewin = [hc.index(f"tracked_kinetic_energy_change_{ch}_J_m-2") for ch in names]
     sc = [hs.index(k) for k in ("kinetic_energy", "field_energy", "live_particles_electrons",
                                 "live_particles_ions")]
     bl = [j for j, h in enumerate(hb) if "_count_" in h or "kinetic_energy" in h]
@@ -101,29 +144,63 @@ def summarise(out_dir):
                 mean_energy_ev=np.array([ev.mean()]))
 
 
-def main():
-    seeds = int(sys.argv[1]) if len(sys.argv) > 1 else 12
-    os.makedirs(OUT, exist_ok=True)
-    write_case(OUT)
-    ref = oracle_py.Reference()
+def ledger_residual(out_dir):
+    """Kinetic-energy change minus (electric work - wall energy + every
+    channel's tracked energy change, ionization products included), at
+    every output, and the largest electric work: the reference closes it to
+    rounding.  Also the product check: live particles = initial +
+    ionizations - absorbed, per species."""
+    hc, c = read_csv(os.path.join(out_dir, "collisions.csv"))
+    hs, s = read_csv(os.path.join(out_dir, "scalars.csv"))
+    hb, b = read_csv(os.path.join(out_dir, "boundary_losses.csv"))
+    hp, p = read_csv(os.path.join(out_dir, "power_transfer.csv"))
+    names = channel_columns(hc)
+    nion = sum(c[:, hc.index(f"cumulative_collisions_{ch}")] for ch in names
+               if ch.endswith(".ionization"))
+    products_ok = True
+    for sp in ("electrons", "ions"):
+        absorbed = b[:, hb.index(f"absorbed_left_count_{sp}")] + \
+            b[:, hb.index(f"absorbed_right_count_{sp}")]
+        products_ok &= bool(np.array_equal(s[:, hs.index(f"live_particles_{sp}")],
+                                           N + nion - absorbed))
+    ke = s[:, hs.index("kinetic_energy")]
+    work = p[:, [j for j, h in enumerate(hp) if h.startswith("electric_work")]].sum(axis=1)
+    wall = b[:, [j for j, h in enumerate(hb) if "kinetic_energy" in h]].sum(axis=1)
+    tracked = sum(c[:, hc.index(f"cumulative_tracked_kinetic_energy_change_{ch}_J_m-2")]
+                  for ch in names)
+    resid = (ke - ke[0]) - (work - wall + tracked)
+    return resid, float(np.max(np.abs(work))), products_ok, nion
+
+
+def run_case(ref, case, seeds):
+    write_case(case)
+    d = case_dir(case)
     runs = []
     for k in range(seeds):
         tmp = tempfile.mkdtemp(prefix="run1d_mcc_")
         try:
-            ref.run1d_case(os.path.join(OUT, "case.txt"), tmp, SEED + k)
+            ref.run1d_case(os.path.join(d, "case.txt"), tmp, SEED + k)
             runs.append(summarise(tmp))
             if k == 0:
                 for f in ("scalars.csv", "collisions.csv", "boundary_losses.csv",
                           "power_transfer.csv"):
-                    shutil.copy(os.path.join(tmp, f), os.path.join(OUT, f))
+                    shutil.copy(os.path.join(tmp, f), os.path.join(d, f))
         finally:
             shutil.rmtree(tmp)
-        print(f"seed {SEED + k}: ionizations {runs[-1]['windows'][:, 4].sum():.0f}, "
+        print(f"{case} seed {SEED + k}: collisions {runs[-1]['windows'][:, 2:].sum(axis=0)}, "
               f"mean energy {runs[-1]['mean_energy_ev'][0]:.3f} eV", flush=True)
-    np.savez_compressed(os.path.join(OUT, "seeds.npz"), seeds=np.arange(SEED, SEED + seeds),
+    np.savez_compressed(os.path.join(d, "seeds.npz"), seeds=np.arange(SEED, SEED + seeds),
                         eedf_edges_ev=EEDF_EDGES,
                         **{k: np.stack([r[k] for r in runs]) for k in runs[0]})
-    print("wrote", OUT)
+    print("wrote", d)
+
+
+def main():
+    seeds = int(sys.argv[1]) if len(sys.argv) > 1 else 12
+    cases = sys.argv[2:] or CASES
+    ref = oracle_py.Reference()
+    for case in cases:
+        run_case(ref, case, seeds)
 
 
 if __name__ == "__main__":

@@ -393,6 +393,72 @@ def test_sort_preserves_particles_and_charge(hip_ctx):
     assert len(cell) == N
 
 
+def _tile_keys(cell, cells, tile):
+    """Tile-major sort key of linear 3D cell indices (x fastest)."""
+    i, j, k = cell % cells[0], (cell // cells[0]) % cells[1], cell // (cells[0] * cells[1])
+    tx, ty, tz = [min(t, m) for t, m in zip(tile, cells)]
+    ntx, nty = -(-cells[0] // tx), -(-cells[1] // ty)
+    return ((k // tz) * nty + j // ty) * ntx + i // tx
+
+
+@pytest.mark.parametrize("name,n,tile,spread,walls", [
+    ("near", (32, 32, 32), (8, 8, 8, 2), 0.5, False),
+    ("far", (32, 32, 32), (4, 4, 4, 2), 1.5, False),
+    ("two_tiles", (16, 8, 8), (8, 8, 4, 2), 1.0, False),
+    ("absorbing", (17, 17, 17), (4, 4, 4, 2), 1.5, True)])
+def test_local_sort_matches_radix_sort(name, n, tile, spread, walls):
+    """The counting sort of a tile-sorted species (particles that moved into
+    a neighbouring tile counted by direction, those that moved further
+    placed through a short radix sort, dead ones dropped) against the full
+    radix sort, over three sort cycles: the same particles in every tile,
+    tile keys non-decreasing in slot order, no dead slot kept, the same
+    states and charge by uid; a second run gives the same slot order."""
+    from aurora_pic import hip
+    print(f"case={name}", flush=True)
+    rng = np.random.default_rng(5)
+    N = 40000
+    L = [1.0, 1.0, 1.0]
+    cells = list(n)          # cell indices run over the mesh extents (nodes when walled)
+    pos = rng.uniform(0, 1, (N, 3))
+    vel = rng.normal(0, spread, (N, 3))
+    E = _smooth_field(list(n), L, 3, rng)
+    runs = []
+    for method in (hip.SORT_RADIX, hip.SORT_LOCAL, hip.SORT_LOCAL):
+        with hip.Context(0) as c:
+            c.mesh(list(n), L, 1 if walls else 0, 1.0)
+            if walls:
+                c.boundaries([1] * 6)
+            c.tiling(*tile)
+            c.sort_method(method)
+            sid = c.add_species(-1.0, 1.0, 1.0 / N, N)
+            c.upload(sid, pos, vel)
+            c.set_electric(E)
+            c.sort()
+            for _ in range(3):
+                for _ in range(4):
+                    c.push(0.01)
+                c.sort()
+            cell, _, _ = c.download_raw(sid)
+            p_, v_, a_, u_ = c.download(sid)
+            c.deposit()
+            runs.append(dict(cell=cell, uid=u_, pos=p_, vel=v_, alive=a_,
+                             charge=c.charge_fixed(sid)))
+    radix, local, again = runs
+    for r in runs:
+        assert np.all(r["cell"] >= 0) and np.all(r["alive"] == 1)
+        assert np.all(np.diff(_tile_keys(r["cell"], cells, tile)) >= 0)
+    assert len(local["uid"]) == len(radix["uid"])
+    if walls:
+        assert len(local["uid"]) < N            # walls absorbed particles
+    keyed = [np.lexsort((r["uid"], _tile_keys(r["cell"], cells, tile))) for r in (radix, local)]
+    assert np.array_equal(radix["uid"][keyed[0]], local["uid"][keyed[1]])
+    o_r, o_l = np.argsort(radix["uid"]), np.argsort(local["uid"])
+    assert np.array_equal(radix["pos"][o_r], local["pos"][o_l])
+    assert np.array_equal(radix["vel"][o_r], local["vel"][o_l])
+    assert np.array_equal(radix["charge"], local["charge"])
+    assert np.array_equal(local["cell"], again["cell"]) and np.array_equal(local["uid"], again["uid"])
+
+
 @pytest.mark.parametrize("shape", [(8, 8, 8, 2, 512, 0), (8, 8, 8, 1, 512, 0),
                                    (4, 4, 4, 2, 512, 0), (16, 8, 4, 2, 512, 0),
                                    (8, 8, 4, 2, 256, 1), (8, 8, 4, 2, 512, 1),

@@ -1,10 +1,13 @@
-"""The running 1D3V discharge with MCC and ionization products against the
-reference's own runs (tests/golden/run1d/rf_1d3v_mcc, written by
-tests/golden/make_run1d_mcc_golden.py from the reference's Simulation::run()
-over 12 seeds).
+"""Running 1D discharges with MCC against the reference's own runs
+(tests/golden/run1d/<case>, written by tests/golden/make_run1d_mcc_golden.py
+from the reference's Simulation::run() over 12 seeds):
+
+  rf_1d3v_mcc        electron impact with ionization products
+  rf_1d3v_mcc_ions   two named models: electrons, then ions on 300 K neutrals
+  rf_1d1v_mcc        1D1V, the scalar collide
 
 The device (pic::gpu::Simulation::run() through the host test program,
-FP32 particles, Philox streams) runs the same case over 6 seeds; both sides
+FP32 particles, Philox streams) runs each case over 6 seeds; both sides
 load their initial particles from the seed the reference's way, but their
 collision random numbers differ, so the bar is statistical: every compared
 quantity's device mean lies within 5 standard errors of the reference's
@@ -19,9 +22,8 @@ electron energy distribution (fraction per log-energy bin, each bin above
 Exact on every device run: the column contracts of the four CSV files, the
 product count (live particles = initial + ionizations - absorbed, per
 species) and the energy ledger, which the reference closes to rounding:
-kinetic-energy change = electric work - wall energy + elastic and
-excitation energy changes - ionizations x threshold x weight (within 1e-6 of
-the work: FP32 velocities)."""
+kinetic-energy change = electric work - wall energy + every channel's
+tracked energy change (within 1e-6 of the work: FP32 velocities)."""
 import csv
 import os
 import subprocess
@@ -37,34 +39,31 @@ import make_run1d_mcc_golden as golden_case  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 TESTS = os.path.join(REPO, "aurora-pic_amd", "lib", "aurora_host_tests")
-CASE_DIR = golden_case.OUT
 DEVICE_SEEDS = 6
+_RUNS = {}
 
 
-def _run(tmp_path, seed):
-    out = str(tmp_path / f"seed{seed}")
-    env = dict(os.environ, AURORA_CASE_SEED=str(seed))
-    print(f"device run seed {seed}", flush=True)
-    r = subprocess.run([TESTS, "--run1d", os.path.join(CASE_DIR, "case.txt"), out],
-                       capture_output=True, text=True, timeout=300, env=env)
-    assert r.returncode == 0, r.stdout + r.stderr
-    return out
-
-
-@pytest.fixture(scope="module")
-def device_runs(tmp_path_factory):
-    tmp = tmp_path_factory.mktemp("run1d_mcc")
-    return [_run(tmp, golden_case.SEED + k) for k in range(DEVICE_SEEDS)]
+def _device_runs(case, tmp_path_factory):
+    if case not in _RUNS:
+        tmp = tmp_path_factory.mktemp(case)
+        outs = []
+        for k in range(DEVICE_SEEDS):
+            seed = golden_case.SEED + k
+            out = str(tmp / f"seed{seed}")
+            print(f"device run {case} seed {seed}", flush=True)
+            r = subprocess.run([TESTS, "--run1d",
+                                os.path.join(golden_case.case_dir(case), "case.txt"), out],
+                               capture_output=True, text=True, timeout=300,
+                               env=dict(os.environ, AURORA_CASE_SEED=str(seed)))
+            assert r.returncode == 0, r.stdout + r.stderr
+            outs.append(out)
+        _RUNS[case] = outs
+    return _RUNS[case]
 
 
 def _header(path):
     with open(path) as f:
         return next(csv.reader(f))
-
-
-def test_column_contracts(device_runs):
-    for f in ("scalars.csv", "collisions.csv", "boundary_losses.csv", "power_transfer.csv"):
-        assert _header(os.path.join(device_runs[0], f)) == _header(os.path.join(CASE_DIR, f)), f
 
 
 def _within(name, dev, ref, counts=False, sigmas=5.0):
@@ -80,9 +79,19 @@ def _within(name, dev, ref, counts=False, sigmas=5.0):
     return z
 
 
-def test_statistics_match_the_reference(device_runs):
-    ref = np.load(os.path.join(CASE_DIR, "seeds.npz"), allow_pickle=False)
-    dev = [golden_case.summarise(d) for d in device_runs]
+@pytest.mark.parametrize("case", golden_case.CASES)
+def test_column_contracts(case, tmp_path_factory):
+    runs = _device_runs(case, tmp_path_factory)
+    for f in ("scalars.csv", "collisions.csv", "boundary_losses.csv", "power_transfer.csv"):
+        assert _header(os.path.join(runs[0], f)) == \
+            _header(os.path.join(golden_case.case_dir(case), f)), f
+
+
+@pytest.mark.parametrize("case", golden_case.CASES)
+def test_statistics_match_the_reference(case, tmp_path_factory):
+    runs = _device_runs(case, tmp_path_factory)
+    ref = np.load(os.path.join(golden_case.case_dir(case), "seeds.npz"), allow_pickle=False)
+    dev = [golden_case.summarise(d) for d in runs]
     stack = {k: np.stack([r[k] for r in dev]) for k in dev[0]}
     assert stack["windows"].shape[1:] == ref["windows"].shape[1:]
     zs = {}
@@ -93,38 +102,22 @@ def test_statistics_match_the_reference(device_runs):
     zs["work"] = _within("electric work", stack["work"], ref["work"])
     zs["mean_energy"] = _within("mean electron energy", stack["mean_energy_ev"],
                                 ref["mean_energy_ev"])
-    # EEDF: populated bins one by one, and a chi-square over all of them
     f_ref, f_dev = ref["eedf"], stack["eedf"]
     keep = f_ref.mean(axis=0) > 0.01
     z = _within("EEDF bins", f_dev[:, keep], f_ref[:, keep])
     chi2 = float(np.mean(z ** 2))
-    print({k: float(np.max(v)) for k, v in zs.items()}, "EEDF chi2/dof", chi2, flush=True)
+    print(case, {k: round(float(np.max(v)), 3) for k, v in zs.items()},
+          "EEDF bins", np.round(z, 2).tolist(), "chi2/dof", round(chi2, 3), flush=True)
     assert chi2 < 4.0
-    # ionization happened and products were placed on both sides
-    assert stack["windows"][:, :, 4].sum(axis=1).min() > 1500
+    assert stack["windows"][:, :, 2:].sum(axis=1).min() > 50     # every channel fired
 
 
-def test_products_and_energy_ledger_close(device_runs):
-    w = 2.56e14 * golden_case.L / golden_case.N
-    thr = 24.6 * golden_case.wl.EV
-    for d in device_runs:
-        hc, c = golden_case.read_csv(os.path.join(d, "collisions.csv"))
-        hs, s = golden_case.read_csv(os.path.join(d, "scalars.csv"))
-        hb, b = golden_case.read_csv(os.path.join(d, "boundary_losses.csv"))
-        hp, p = golden_case.read_csv(os.path.join(d, "power_transfer.csv"))
-        nion = c[:, hc.index("cumulative_collisions_e_he.ionization")]
-        for sp in ("electrons", "ions"):
-            absorbed = b[:, hb.index(f"absorbed_left_count_{sp}")] + \
-                b[:, hb.index(f"absorbed_right_count_{sp}")]
-            live = s[:, hs.index(f"live_particles_{sp}")]
-            assert np.array_equal(live, golden_case.N + nion - absorbed), sp
-        ke = s[:, hs.index("kinetic_energy")]
-        work = p[:, [j for j, h in enumerate(hp) if h.startswith("electric_work")]].sum(axis=1)
-        wall = b[:, [j for j, h in enumerate(hb) if "kinetic_energy" in h]].sum(axis=1)
-        el = c[:, hc.index("cumulative_tracked_kinetic_energy_change_e_he.elastic_J_m-2")]
-        ex = c[:, hc.index("cumulative_tracked_kinetic_energy_change_e_he.excitation_J_m-2")]
-        resid = (ke - ke[0]) - (work - wall + el + ex - thr * w * nion)
-        assert np.max(np.abs(resid)) <= 1e-6 * np.max(np.abs(work)), (resid, work)
+@pytest.mark.parametrize("case", golden_case.CASES)
+def test_products_and_energy_ledger_close(case, tmp_path_factory):
+    for d in _device_runs(case, tmp_path_factory):
+        resid, work, products_ok, _ = golden_case.ledger_residual(d)
+        assert products_ok
+        assert np.max(np.abs(resid)) <= 1e-6 * work, (resid, work)
 
 
 def test_full_size_discharge_invariants():
@@ -165,8 +158,7 @@ def test_full_size_discharge_invariants():
         w, q = cfg["species"][0]["weight"], wl.EV
         charge = q * w * ((live1 - live0) - 2 * nion + absorbed[0] + absorbed[1])
         assert charge == 0.0
-        thr = 24.6 * wl.EV
-        resid = (ke1.sum() - ke0.sum()) - (work - wall + ech[0] + ech[1] - thr * w * nion)
+        resid = (ke1.sum() - ke0.sum()) - (work - wall + ech.sum())
         print(f"ionizations {nion}, work {work:.4e}, wall {wall:.4e}, residual {resid:.3e}",
               flush=True)
         assert abs(resid) <= 1e-6 * abs(work)

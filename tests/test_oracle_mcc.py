@@ -271,3 +271,76 @@ def test_charge_exchange_makes_slow_ion(oracle):
                             neutral_vstd=1.0)
     _, st = oracle.collide(unsafe, [2.0, 0.0, 0.0], 0.01, 1, 0, 0, 0)
     assert st.status == 1
+
+
+# ------------------------------------------------------------ 1V collide
+def test_1v_elastic_rates_and_speed(oracle):
+    # test_core.cpp:336-367 (the scalar collide): candidates ~ nu_max dt N =
+    # 2000, accepted ~ n sigma v dt N = 707, |v| kept to 1e-14; the sign
+    # flips in about half of the accepted events (forward or backward)
+    m = _unit_table_model(workloads.ELASTIC)
+    cand = acc = flips = 0
+    for p in range(10000):
+        v, st = oracle.collide_1v(m, math.sqrt(2.0), 0.1, 8128, 0, p, 0)
+        assert st.status == 0
+        assert abs(abs(v) - math.sqrt(2.0)) < 1e-14
+        cand += st.candidates
+        acc += st.chan[0]
+        flips += v < 0
+    assert 1800 < cand < 2200
+    assert 620 < acc < 800
+    # a particle reversed by an odd number of its accepted events
+    assert 0.3 * acc < flips < 0.7 * acc
+
+
+def test_1v_excitation_removes_threshold(oracle):
+    m = _unit_table_model(workloads.EXCITATION, threshold=0.5, sigma_scale=2.0,
+                          max_frequency=4.0)
+    accepted = 0
+    for p in range(50):
+        v, st = oracle.collide_1v(m, 2.0, 2.0, 77, 0, p, 0)
+        assert st.status == 0
+        accepted += st.chan[0]
+        e0, e1 = 0.5 * 4.0, 0.5 * v * v
+        assert abs((e0 - e1) - 0.5 * st.chan[0]) < 1e-13
+        assert abs(st.energy_change[0] - (e1 - e0)) < 1e-13
+    assert accepted > 0
+
+
+def test_1v_heavy_neutral_elastic_and_charge_exchange(oracle):
+    # elastic on a neutral at rest of mass M: v' = m v / (m + M) +/- M |v| / (m + M)
+    e, s = [0.0, 10.0], [1.0, 1.0]
+    m = oracle_py.MccModel([(workloads.ELASTIC, 0.0, e, s)], 1.0, 4.0, 1.0, 64, 3.0)
+    for p in range(40):
+        v, st = oracle.collide_1v(m, 1.5, 0.2, 5, 0, p, 0)
+        if st.chan[0] == 1:
+            assert min(abs(v - (1.5 + 4.5) / 4.0), abs(v - (1.5 - 4.5) / 4.0)) < 1e-14
+    # charge exchange on cold neutrals stops the ion
+    m = oracle_py.MccModel([(oracle_py.CHARGE_EXCHANGE, 0.0, e, s)], 1.0, 4.0, 1.0, 64, 1.0)
+    stopped = 0
+    for p in range(40):
+        v, st = oracle.collide_1v(m, 1.5, 0.5, 6, 0, p, 0)
+        if st.chan[0]:
+            assert v == 0.0
+            stopped += 1
+    assert stopped > 0
+
+
+def test_decision_margins(oracle):
+    # the margin is the smallest relative distance to a decision boundary:
+    # with a zero cross section the decisions are the waiting times against
+    # the time left (|t_wait - left| / dt) and each candidate's selection
+    # against the zero cumulative rate (selection / nu_max = its uniform)
+    m = _unit_table_model(workloads.ELASTIC, sigma_scale=0.0)
+    v, st = oracle.collide(m, [1.0, 0.0, 0.0], 0.1, 3, 0, 0, 0)
+    s = oracle.stream(3, 0, 0, 0)
+    left, margin = 0.1, 1.0
+    while True:
+        t = -math.log(oracle.stream_u01(s)) / 2.0
+        margin = min(margin, abs(t - left) / 0.1)
+        if t >= left:
+            break
+        left -= t
+        margin = min(margin, oracle.stream_u01(s))   # the selection of a candidate
+    assert st.margin == pytest.approx(margin, rel=1e-12)
+    assert 0.0 < st.margin <= 1.0
