@@ -16,7 +16,10 @@ SHAPES = [(8, 8, 8, 2, 512, 0), (8, 8, 8, 1, 512, 0), (8, 8, 4, 2, 512, 0),
           (8, 8, 4, 2, 256, 1), (8, 8, 4, 2, 256, 0), (8, 8, 8, 2, 512, 1),
           (8, 8, 8, 2, 256, 1), (8, 8, 4, 2, 512, 3), (8, 8, 4, 1, 512, 3),
           (8, 8, 4, 2, 512, 5), (8, 8, 8, 2, 512, 5), (8, 8, 8, 1, 512, 1),
-          (8, 8, 8, 1, 512, 5)]
+          (8, 8, 8, 1, 512, 5), (8, 8, 8, 1, 512, 3), (8, 8, 8, 1, 512, 7),
+          (8, 8, 8, 2, 512, 13), (8, 8, 8, 1, 512, 13), (8, 8, 8, 1, 512, 15),
+          (8, 8, 4, 2, 512, 13), (8, 8, 4, 2, 512, 11), (8, 8, 4, 2, 512, 7),
+          (8, 8, 4, 2, 512, 15)]
 
 
 def tune(ctx, dt, shapes=SHAPES, sort_intervals=(20,), steps=20, log=None, adaptive=False):

@@ -749,7 +749,7 @@ int aurora_hip_set_tiling(aurora_hip_ctx* h, int tx, int ty, int tz, int halo) {
 
 int aurora_hip_set_push_variant(aurora_hip_ctx* h, int block, int prefetch) {
     return guarded(h, [&](Context& c) {
-        AUR_REQUIRE((block == 256 || block == 512) && prefetch >= 0 && prefetch <= 7,
+        AUR_REQUIRE((block == 256 || block == 512) && prefetch >= 0 && prefetch <= 15,
                     AURORA_ERR_UNSUPPORTED, "push variant is not built into this library");
         c.push_block = block;
         c.push_prefetch = prefetch;

@@ -735,7 +735,8 @@ __device__ __forceinline__ int rel_index(int i, int o, int n) {
 // while the current one is processed (more memory-level parallelism); PF
 // bit 1 asks the compiler to fit 8 waves per SIMD (<= 64 VGPRs), so that
 // four 512-thread workgroups share a CU.  PF bit 2 lays the field box out
-// by component.
+// by component.  PF bit 3 writes the particle state back with non-temporal
+// stores (it is not read again within the push).
 template <int TX, int TY, int TZ, int H, int BMODE, int BLOCK, int PF>
 __global__ void __launch_bounds__(BLOCK, (PF & 2) ? 8 : 1) k_push_tile3d(PushArgs<float> a,
                                                        const int32_t* __restrict__ tile_start,
@@ -830,12 +831,19 @@ __global__ void __launch_bounds__(BLOCK, (PF & 2) ? 8 : 1) k_push_tile3d(PushArg
         using T = std::remove_reference_t<decltype(*base)>;
         return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff);
     };
+    constexpr bool kNT = (PF & 8) != 0;
+    auto ld = [&](auto* base, unsigned boff) { return at(base, boff); };
+    auto put = [&](auto* base, unsigned boff, auto v) {
+        auto* ptr = &at(base, boff);
+        if constexpr (kNT) __builtin_nontemporal_store(v, ptr);
+        else *ptr = v;
+    };
     auto load = [&](unsigned k) {
         const unsigned b = k * 4u;
         State st;
-        st.c = at(cellp, b);
-        st.o0 = at(o0p, b); st.o1 = at(o1p, b); st.o2 = at(o2p, b);
-        st.v0 = at(v0p, b); st.v1 = at(v1p, b); st.v2 = at(v2p, b);
+        st.c = ld(cellp, b);
+        st.o0 = ld(o0p, b); st.o1 = ld(o1p, b); st.o2 = ld(o2p, b);
+        st.v0 = ld(v0p, b); st.v1 = ld(v1p, b); st.v2 = ld(v2p, b);
         return st;
     };
     constexpr bool kPrefetch = (PF & 1) != 0;
@@ -906,13 +914,13 @@ __global__ void __launch_bounds__(BLOCK, (PF & 2) ? 8 : 1) k_push_tile3d(PushArg
         drift_periodic(idx[1], off[1], v[1] * a.dtd1, m.ny, sh[1]);
         drift_periodic(idx[2], off[2], v[2] * a.dtd2, m.nz, sh[2]);
         const unsigned kb = k * 4u;
-        at(v0p, kb) = v[0];
-        at(v1p, kb) = v[1];
-        at(v2p, kb) = v[2];
-        at(cellp, kb) = compose<3>(m, idx);
-        at(o0p, kb) = off[0];
-        at(o1p, kb) = off[1];
-        at(o2p, kb) = off[2];
+        put(v0p, kb, v[0]);
+        put(v1p, kb, v[1]);
+        put(v2p, kb, v[2]);
+        put(cellp, kb, static_cast<int32_t>(compose<3>(m, idx)));
+        put(o0p, kb, off[0]);
+        put(o1p, kb, off[1]);
+        put(o2p, kb, off[2]);
         // position in the box after the move: the shift needs no wrap (a
         // particle that wrapped out of [0, n) takes the global deposit)
         const unsigned qx = static_cast<unsigned>(rx + sh[0]);
@@ -995,7 +1003,16 @@ bool launch_tile3d(Context& c, Species& sp, double dt) {
            launch_tile3d_t<8, 8, 4, 2, 512, 5>(c, sp, a, mode) ||
            launch_tile3d_t<8, 8, 8, 2, 512, 5>(c, sp, a, mode) ||
            launch_tile3d_t<8, 8, 8, 1, 512, 1>(c, sp, a, mode) ||
-           launch_tile3d_t<8, 8, 8, 1, 512, 5>(c, sp, a, mode);
+           launch_tile3d_t<8, 8, 8, 1, 512, 5>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 8, 1, 512, 3>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 8, 1, 512, 7>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 8, 2, 512, 13>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 8, 1, 512, 13>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 8, 1, 512, 15>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 4, 2, 512, 13>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 4, 2, 512, 11>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 4, 2, 512, 7>(c, sp, a, mode) ||
+           launch_tile3d_t<8, 8, 4, 2, 512, 15>(c, sp, a, mode);
 }
 
 // Workgroups of a push launch: 1D deposits flush one LDS copy of the mesh

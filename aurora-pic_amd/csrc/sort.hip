@@ -223,6 +223,26 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n) {
         p[i] = 0u;
 }
 
+// distinct codes of a wave, one at a time (a wave-uniform loop): calls
+// f(code, mask of the lanes holding it) with every lane present
+template <typename F>
+__device__ __forceinline__ void for_each_code(int code, F&& f) {
+    bool pending = code >= 0;
+    for (;;) {
+        const uint64_t mp = __ballot(pending);
+        if (mp == 0) break;
+        const int lc = __shfl(code, __builtin_ctzll(mp));
+        const bool mine = pending && code == lc;
+        const uint64_t m = __ballot(mine);
+        f(lc, m, mine, __builtin_ctzll(mp));
+        if (mine) pending = false;
+    }
+}
+
+// each thread takes kLsUnroll particles per pass (loads issued together)
+constexpr int kLsUnroll = 4;
+constexpr int kLsChunk = kLsUnroll * kLsBlock;
+
 __global__ void __launch_bounds__(kLsBlock) k_ls_count(LsGeom g, const int32_t* __restrict__ old_start,
                                                        const int32_t* __restrict__ cell,
                                                        uint32_t* __restrict__ count,
@@ -234,58 +254,62 @@ __global__ void __launch_bounds__(kLsBlock) k_ls_count(LsGeom g, const int32_t* 
     if (threadIdx.x < kCodes) s_cnt[threadIdx.x] = 0u;
     __syncthreads();
     const int p0 = old_start[s], p1 = old_start[s + 1];
-    for (int base = p0; base < p1; base += kLsBlock) {
-        const int p = base + static_cast<int>(threadIdx.x);
-        int t = -1, code = -1;
-        if (p < p1) code = ls_code(g, cell[p], sx, sy, sz, t);
-        if (code == kFar) atomicAdd(far_count + t, 1u);
-        // one LDS add per distinct code of the wave
-        bool pending = code >= 0;
-        for (;;) {
-            const uint64_t mp = __ballot(pending);
-            if (mp == 0) break;
-            const int leader = __builtin_ctzll(mp);
-            const int lc = __shfl(code, leader);
-            const bool mine = pending && code == lc;
-            const uint64_t m = __ballot(mine);
-            if (lane == leader) atomicAdd(&s_cnt[lc], static_cast<uint32_t>(__popcll(m)));
-            if (mine) pending = false;
+    for (int base = p0; base < p1; base += kLsChunk) {
+        int32_t cv[kLsUnroll];
+#pragma unroll
+        for (int u = 0; u < kLsUnroll; ++u) {
+            const int p = base + u * kLsBlock + static_cast<int>(threadIdx.x);
+            cv[u] = p < p1 ? cell[p] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kLsUnroll; ++u) {
+            const int p = base + u * kLsBlock + static_cast<int>(threadIdx.x);
+            int t = -1, code = -1;
+            if (p < p1) code = ls_code(g, cv[u], sx, sy, sz, t);
+            if (code == kFar) atomicAdd(far_count + t, 1u);
+            // one LDS add per distinct code of the wave
+            for_each_code(code, [&](int lc, uint64_t m, bool, int leader) {
+                if (lane == leader) atomicAdd(&s_cnt[lc], static_cast<uint32_t>(__popcll(m)));
+            });
         }
     }
     __syncthreads();
     if (threadIdx.x < kCodes) count[static_cast<int64_t>(s) * kCodes + threadIdx.x] = s_cnt[threadIdx.x];
 }
 
-// one thread per new tile t: its sources in increasing tile order
-__global__ void k_ls_offsets(LsGeom g, const uint32_t* __restrict__ count,
-                             const uint32_t* __restrict__ far_count, uint32_t* __restrict__ offset,
-                             uint32_t* __restrict__ new_count) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.ntiles) return;
+// one wave per new tile t, lane = direction code (27 lanes): the
+// contributions (source s -> t) are placed in increasing source order, so
+// each lane's offset is the sum of the counts of the lanes whose source
+// tile is smaller (sources of one tile are distinct)
+__global__ void __launch_bounds__(256) k_ls_offsets(LsGeom g, const uint32_t* __restrict__ count,
+                                                    const uint32_t* __restrict__ far_count,
+                                                    uint32_t* __restrict__ offset,
+                                                    uint32_t* __restrict__ new_count) {
+    const int t = static_cast<int>((blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) >> 6);
+    const int cd = threadIdx.x & 63;
+    if (t >= g.ntiles) return;             // whole waves leave together
     const int ntx = g.k.ntx, nty = g.k.nty, ntz = g.ntz;
     const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
-    int src[kNear], code[kNear], n = 0;
-    for (int cd = 0; cd < kNear; ++cd) {
+    int s = -1;
+    uint32_t n = 0u;
+    if (cd < kNear) {
         const int ax = cd % 3 - 1, ay = (cd / 3) % 3 - 1, az = cd / 9 - 1;
-        if (!step_valid(ax, ntx) || !step_valid(ay, nty) || !step_valid(az, ntz)) continue;
-        const int sx = (tx - ax + ntx) % ntx, sy = (ty - ay + nty) % nty,
-                  sz = (tz - az + ntz) % ntz;
-        const int s = (sz * nty + sy) * ntx + sx;
-        int k = n++;
-        while (k > 0 && src[k - 1] > s) {
-            src[k] = src[k - 1];
-            code[k] = code[k - 1];
-            --k;
+        if (step_valid(ax, ntx) && step_valid(ay, nty) && step_valid(az, ntz)) {
+            const int sx = (tx - ax + ntx) % ntx, sy = (ty - ay + nty) % nty,
+                      sz = (tz - az + ntz) % ntz;
+            s = (sz * nty + sy) * ntx + sx;
+            n = count[static_cast<int64_t>(s) * kCodes + cd];
         }
-        src[k] = s;
-        code[k] = cd;
     }
-    uint32_t run = 0u;
-    for (int k = 0; k < n; ++k) {
-        offset[static_cast<int64_t>(src[k]) * kNear + code[k]] = run;
-        run += count[static_cast<int64_t>(src[k]) * kCodes + code[k]];
+    uint32_t before = 0u, total = 0u;
+    for (int j = 0; j < kNear; ++j) {
+        const int sj = __shfl(s, j);
+        const uint32_t nj = __shfl(n, j);
+        if (sj >= 0 && sj < s) before += nj;
+        total += nj;
     }
-    new_count[t] = run + far_count[t];
+    if (s >= 0) offset[static_cast<int64_t>(s) * kNear + cd] = before;
+    if (cd == 0) new_count[t] = total + far_count[t];
 }
 
 // exclusive scan of one value per thread over a 1024-thread block
@@ -358,79 +382,100 @@ struct LsPayload {
     unsigned* far_val;
 };
 
+// Per pass of kLsChunk slots: every load first, then each wave's rank of
+// its particles among the same code (ballots), the per-code counts of
+// each (sub-chunk, wave) turned into exclusive starts in slot order by one
+// thread per code, then the stores.  The count table is double-buffered by
+// pass parity, so a pass needs two barriers.
 template <typename R>
 __global__ void __launch_bounds__(kLsBlock) k_ls_scatter(LsGeom g, const int32_t* __restrict__ old_start,
                                                          const int32_t* __restrict__ new_start,
                                                          const uint32_t* __restrict__ offset,
                                                          const uint32_t* __restrict__ far_base,
                                                          LsPayload<R> a) {
+    constexpr int U = kLsUnroll;
     __shared__ uint32_t s_run[kCodes];
-    __shared__ uint32_t s_wave[kLsWaves][kCodes];
+    __shared__ uint32_t s_tab[2][U][kLsWaves][kCodes];
     const int s = blockIdx.x;
     const int sx = s % g.k.ntx, sy = (s / g.k.ntx) % g.k.nty, sz = s / (g.k.ntx * g.k.nty);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (threadIdx.x < kCodes) s_run[threadIdx.x] = 0u;
-    for (int q = threadIdx.x; q < kLsWaves * kCodes; q += kLsBlock) (&s_wave[0][0])[q] = 0u;
+    for (int q = threadIdx.x; q < 2 * U * kLsWaves * kCodes; q += kLsBlock)
+        (&s_tab[0][0][0][0])[q] = 0u;
     __syncthreads();
     const int p0 = old_start[s], p1 = old_start[s + 1];
     const uint32_t fb = far_base[s];
     const uint32_t* off_s = offset + static_cast<int64_t>(s) * kNear;
-    for (int base = p0; base < p1; base += kLsBlock) {
-        const int p = base + static_cast<int>(threadIdx.x);
-        int t = -1, code = -1;
-        int32_t c = -1;
-        if (p < p1) {
-            c = a.cell[p];
-            code = ls_code(g, c, sx, sy, sz, t);
-        }
-        // rank among the wave's particles of the same code (slot order)
-        uint32_t rank = 0u;
-        bool pending = code >= 0;
-        for (;;) {
-            const uint64_t mp = __ballot(pending);
-            if (mp == 0) break;
-            const int leader = __builtin_ctzll(mp);
-            const int lc = __shfl(code, leader);
-            const bool mine = pending && code == lc;
-            const uint64_t m = __ballot(mine);
-            if (mine) {
-                rank = lanes_below(m);
-                pending = false;
-            }
-            if (lane == leader) s_wave[w][lc] = static_cast<uint32_t>(__popcll(m));
-        }
-        __syncthreads();
-        if (code >= 0 && code != kDead) {
-            uint32_t r = s_run[code] + rank;
-            for (int v = 0; v < w; ++v) r += s_wave[v][code];
-            if (code == kFar) {
-                const uint32_t f = fb + r;
-                a.st_cell[f] = c;
-                a.st_uid[f] = a.uid[p];
+    int buf = 0;
+    for (int base = p0; base < p1; base += kLsChunk, buf ^= 1) {
+        int32_t c[U];
+        uint32_t id[U];
+        R v[U][6];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = base + u * kLsBlock + static_cast<int>(threadIdx.x);
+            if (p < p1) {
+                c[u] = a.cell[p];
+                id[u] = a.uid[p];
 #pragma unroll
                 for (int k = 0; k < 6; ++k)
-                    if (k < a.count) a.st[k][f] = a.src[k][p];
-                a.far_key[f] = static_cast<unsigned>(t);
-                a.far_val[f] = f;
+                    if (k < a.count) v[u][k] = a.src[k][p];
             } else {
-                const int64_t q = static_cast<int64_t>(new_start[t]) + off_s[code] + r;
-                a.cell2[q] = c;
-                a.uid2[q] = a.uid[p];
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    if (k < a.count) a.dst[k][q] = a.src[k][p];
+                c[u] = -1;
             }
+        }
+        int code[U], t[U];
+        uint32_t rank[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int p = base + u * kLsBlock + static_cast<int>(threadIdx.x);
+            code[u] = -1;
+            t[u] = -1;
+            rank[u] = 0u;
+            if (p < p1) code[u] = ls_code(g, c[u], sx, sy, sz, t[u]);
+            for_each_code(code[u], [&](int lc, uint64_t m, bool mine, int leader) {
+                if (mine) rank[u] = lanes_below(m);
+                if (lane == leader) s_tab[buf][u][w][lc] = static_cast<uint32_t>(__popcll(m));
+            });
         }
         __syncthreads();
         if (threadIdx.x < kCodes) {
-            uint32_t sum = 0u;
-            for (int v = 0; v < kLsWaves; ++v) {
-                sum += s_wave[v][threadIdx.x];
-                s_wave[v][threadIdx.x] = 0u;
-            }
-            s_run[threadIdx.x] += sum;
+            // exclusive starts in slot order: sub-chunk, then wave
+            uint32_t run = s_run[threadIdx.x];
+            for (int u = 0; u < U; ++u)
+                for (int v2 = 0; v2 < kLsWaves; ++v2) {
+                    const uint32_t n = s_tab[buf][u][v2][threadIdx.x];
+                    s_tab[buf][u][v2][threadIdx.x] = run;
+                    run += n;
+                }
+            s_run[threadIdx.x] = run;
         }
+        // the other buffer's last readers finished before this pass's first barrier
+        for (int q = threadIdx.x; q < U * kLsWaves * kCodes; q += kLsBlock)
+            (&s_tab[buf ^ 1][0][0][0])[q] = 0u;
         __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (code[u] < 0 || code[u] == kDead) continue;
+            const uint32_t r = s_tab[buf][u][w][code[u]] + rank[u];
+            if (code[u] == kFar) {
+                const uint32_t f = fb + r;
+                a.st_cell[f] = c[u];
+                a.st_uid[f] = id[u];
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                    if (k < a.count) a.st[k][f] = v[u][k];
+                a.far_key[f] = static_cast<unsigned>(t[u]);
+                a.far_val[f] = f;
+            } else {
+                const int64_t q = static_cast<int64_t>(new_start[t[u]]) + off_s[code[u]] + r;
+                a.cell2[q] = c[u];
+                a.uid2[q] = id[u];
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                    if (k < a.count) a.dst[k][q] = v[u][k];
+            }
+        }
     }
 }
 
@@ -479,8 +524,8 @@ bool local_sort(Context& c, Species& sp, const KeyArgs& k, int ntz, unsigned bit
     k_fill_u32<<<(T + 255) / 256, 256, 0, c.stream>>>(c.ls_far_count.ptr, T);
     k_ls_count<<<T, kLsBlock, 0, c.stream>>>(g, sp.tile_start.ptr, sp.cell.ptr, c.ls_count.ptr,
                                             c.ls_far_count.ptr);
-    k_ls_offsets<<<(T + 255) / 256, 256, 0, c.stream>>>(g, c.ls_count.ptr, c.ls_far_count.ptr,
-                                                        c.ls_offset.ptr, c.ls_new_count.ptr);
+    k_ls_offsets<<<(T + 3) / 4, 256, 0, c.stream>>>(g, c.ls_count.ptr, c.ls_far_count.ptr,
+                                                    c.ls_offset.ptr, c.ls_new_count.ptr);
     k_ls_scan<<<1, 1024, 0, c.stream>>>(T, c.ls_new_count.ptr, c.ls_count.ptr,
                                         c.ls_new_start.ptr, c.ls_far_base.ptr, c.ls_summary.ptr);
     AUR_HIP(hipGetLastError());

@@ -388,8 +388,10 @@ def main():
     ke, fe, l1, live = ctx.energy(dt)
     log(f"live {live}, kinetic {ke.sum():.6e} J, field {fe:.6e} J")
     # the attainable HBM bandwidth of hand-written streaming kernels
-    stream_peak = {"copy_gbs": ctx.stream_bandwidth(0),
-                   "soa7_in_place_gbs": ctx.stream_bandwidth(1)}
+    streams = {k: ctx.stream_bandwidth(k) for k in range(6)}
+    stream_peak = {"copy_gbs": max(streams[k] for k in (0, 2, 3)),
+                   "soa7_in_place_gbs": max(streams[k] for k in (1, 4, 5)),
+                   "patterns_gbs": [round(streams[k], 1) for k in range(6)]}
     log("stream peak:", json.dumps(stream_peak))
     ctx.close()
 

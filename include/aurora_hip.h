@@ -217,9 +217,11 @@ int aurora_hip_set_tiling(aurora_hip_ctx* ctx, int tx, int ty, int tz,
  * flags (bit 0: the next particle's state is loaded while the current one
  * is processed; bit 1: registers capped at 64 so four 512-thread workgroups
  * share a CU; bit 2: the field box in LDS by component instead of float4
- * nodes).  Built: 512/0 for every shape; 512/1, 256/1, 256/0, 512/3, 512/5
- * for 8x8x4 halo 2; 512/3 for 8x8x4 halo 1; 512/1, 256/1, 512/5 for 8x8x8
- * halo 2; 512/1, 512/5 for 8x8x8 halo 1.  Results do not depend on it. */
+ * nodes; bit 3: non-temporal particle stores).  Built: 512/0 for every
+ * shape; 512/1, 256/1, 256/0, 512/3, 512/5, 512/7, 512/11, 512/13, 512/15
+ * for 8x8x4 halo 2; 512/3 for 8x8x4 halo 1; 512/1, 256/1, 512/5, 512/13 for 8x8x8
+ * halo 2; 512/1, 512/3, 512/5, 512/7, 512/13, 512/15 for 8x8x8 halo 1.
+ * Results do not depend on it. */
 int aurora_hip_set_push_variant(aurora_hip_ctx* ctx, int block, int prefetch);
 /* Cell sort every interval steps (0 = never) and an immediate sort. */
 int aurora_hip_set_sort_interval(aurora_hip_ctx* ctx, int64_t interval);
@@ -508,10 +510,12 @@ int aurora_hip_ensemble_destroy(aurora_hip_ctx* ctx, int64_t handle);
 /* -------------------------------------------------------------- timing */
 /* The attainable HBM bandwidth (GB/s, bytes read + written per second) of
  * hand-written streaming kernels on this device, best of `repeats` timed
- * launches over `bytes` of traffic: pattern 0 copies one array with 16-byte
- * accesses; pattern 1 reads and rewrites seven 4-byte arrays in place (the
- * 3D push's particle streams).  No reference counterpart: it calibrates
- * the HBM roofline the kernels are measured against. */
+ * launches over `bytes` of traffic: patterns 0, 2, 3 copy one array with
+ * 16-byte accesses (grid-stride; one element per thread; the same with
+ * non-temporal loads and stores); patterns 1, 4, 5 read and rewrite seven
+ * 4-byte arrays in place (the 3D push's particle streams) the same three
+ * ways.  No reference counterpart: it calibrates the HBM roofline the
+ * kernels are measured against. */
 int aurora_hip_stream_bandwidth(aurora_hip_ctx* ctx, int pattern, int64_t bytes,
                                 int repeats, double* gbs);
 /* Per-kernel HIP-event timing on the context stream (off by default). */
