@@ -111,7 +111,8 @@ def work_items_exceeded(request, work_item_limit=WORK_ITEM_LIMIT):
 
 
 def serve(ctx, requests, arrivals_s, hbm_budget, steps_per_launch=1000, clock=time.perf_counter,
-          on_launch=None, work_item_limit=WORK_ITEM_LIMIT):
+          on_launch=None, work_item_limit=WORK_ITEM_LIMIT, progress=None, progress_every_s=30.0,
+          max_active_particles=None):
     """Run the request loop until every request has a result.
 
     ctx: hip.Context with the swarm gas loaded.  arrivals_s: arrival time of
@@ -122,13 +123,19 @@ def serve(ctx, requests, arrivals_s, hbm_budget, steps_per_launch=1000, clock=ti
     holds back the requests that arrived after it.  A request above
     work_item_limit (particles x steps) is rejected with the reference's
     message; the default is the reference's 1e8, and a GPU deployment raises
-    it explicitly.
+    it explicitly.  max_active_particles, when given, caps the electrons of
+    the admitted requests together (a request larger than the cap runs
+    alone): once the GPU is full, admitting more only shares it more thinly,
+    so a cap of a few GPU-fulls keeps service first come, first served and
+    the latency of early arrivals short, at the same throughput.
 
     Swarm launches are queued without waiting (aurora_hip_swarm_launch), so
     while one runs the loop keeps admitting arrivals (they join the next
     launch) and calls on_launch, e.g. to step a 1D ensemble on its own
     context and stream beside the swarms; it then collects the launch and
-    the requests it finished.  Returns a ServingReport."""
+    the requests it finished.  progress(served, total, seconds), when
+    given, is called about every progress_every_s seconds.  Returns a
+    ServingReport."""
     order = np.argsort(np.asarray(arrivals_s), kind="stable")
     pending = collections.deque(Served(int(i), float(arrivals_s[i]), requests[i]) for i in order)
     queued = collections.deque()
@@ -138,6 +145,7 @@ def serve(ctx, requests, arrivals_s, hbm_budget, steps_per_launch=1000, clock=ti
     electron_steps = 0
     in_flight = False
     t0 = clock()
+    next_report = progress_every_s
 
     def admit():
         nonlocal refused
@@ -155,6 +163,10 @@ def serve(ctx, requests, arrivals_s, hbm_budget, steps_per_launch=1000, clock=ti
             queued.append(s)
         while queued:
             s = queued[0]
+            if max_active_particles is not None and active and \
+                    sum(a.request.particles for a in active.values()) + s.request.particles > \
+                    max_active_particles:
+                break
             try:
                 h = ctx.swarm_submit(s.request, hbm_budget=hbm_budget)
             except hip.HipError as e:
@@ -195,6 +207,9 @@ def serve(ctx, requests, arrivals_s, hbm_budget, steps_per_launch=1000, clock=ti
         if in_flight:
             continue
         t_done = clock() - t0
+        if progress is not None and t_done >= next_report:
+            progress(len(served), len(requests), t_done)
+            next_report = t_done + progress_every_s
         for h in list(active):
             done, r = ctx.swarm_poll(h)
             if done:

@@ -70,6 +70,16 @@ struct MccTally {
     uint32_t err;
 };
 
+// the counters alone (fixed-population swarms keep no products or energy
+// ledger; the draws are the same)
+template <int NCH>
+struct MccCounts {
+    uint32_t candidates;
+    uint32_t nulls;
+    uint32_t chan[NCH];
+    uint32_t err;
+};
+
 // Host-side dispatch on the channel count: f.template operator()<NCH>().
 template <typename F>
 void with_nchan(int nchan, F&& f) {
@@ -304,99 +314,140 @@ __device__ void apply_channel(const MccParams& m, const ChannelSel& ch, float* v
     }
 }
 
-// One particle-step of null-collision MCC on velocity v (in place).
+// The candidate loop of one particle-step, in two resumable pieces (the
+// swarm kernel interleaves particles' candidates): mcc_next_candidate is the
+// loop top -- the rate-bound checks at the current speed, then the next
+// waiting time -- and returns true when another candidate falls inside the
+// step; mcc_candidate handles that candidate and returns false when the
+// particle stops colliding in this step (attachment, or an error in t.err).
+template <int NCH, typename Tally>
+__device__ __forceinline__ bool mcc_next_candidate(const MccParams& m, const XsView& X,
+                                                   const float* v, float dt, Philox& rng,
+                                                   float& elapsed, Tally& t) {
+    float rate[NCH];
+    const float speed = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (m.neutral_vstd == 0.f) {
+        // validate_frequency_bound (Collision.cpp:627-632): below
+        // safe_speed the check cannot fail, so it is evaluated only above
+        if (speed > m.safe_speed && above_bound(m, mcc_rates<NCH>(m, X, speed, rate))) {
+            t.err = DEV_ERR_RATE;
+            return false;
+        }
+    } else if (thermal_bound_exceeded<NCH>(m, X, speed)) {
+        t.err = DEV_ERR_RATE;
+        return false;
+    }
+    const float waiting = -logf(rng.u01()) * m.inv_max_frequency;
+    if (!(waiting < dt - elapsed)) return false;
+    elapsed += waiting;
+    ++t.candidates;
+    if (t.candidates > static_cast<uint32_t>(m.max_candidates)) {
+        t.err = DEV_ERR_CANDIDATES;
+        return false;
+    }
+    return true;
+}
+
+template <int NCH, typename Tally>
+__device__ __forceinline__ bool mcc_candidate(const MccParams& m, const XsView& X,
+                                              const ChannelSel* chan, float* v, Philox& rng,
+                                              Tally& t) {
+    constexpr bool kLedger = requires { t.energy_change[0]; };
+    float rate[NCH];
+    float w[3];
+    neutral_velocity(m, rng, w);
+    const float g[3] = {v[0] - w[0], v[1] - w[1], v[2] - w[2]};
+    const float gs = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    // rates at the relative speed (Collision.cpp:943-944); with a cold
+    // neutral gs equals the speed checked above
+    const float total = mcc_rates<NCH>(m, X, gs, rate);
+    if (m.neutral_vstd != 0.f && above_bound(m, total)) {
+        t.err = DEV_ERR_RATE;
+        return false;
+    }
+    const float selection = rng.u01() * m.max_frequency;
+    float cumulative = 0.f;
+    int accepted = -1;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        cumulative += rate[c];
+        if (accepted < 0 && selection < cumulative) accepted = c;
+    }
+    if (accepted < 0) {
+        ++t.nulls;
+        return true;
+    }
+    const int c = accepted;
+    const ChannelSel ch = chan[c];
+    const float e_before = m.half_mass * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    apply_channel(m, ch, v, w, g, gs, rng);
+    const float de = m.half_mass * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) - e_before;
+#pragma unroll
+    for (int k = 0; k < NCH; ++k)
+        if (k == c) {
+            if constexpr (kLedger) t.energy_change[k] += de;
+            ++t.chan[k];
+        }
+    if (ch.kind == AURORA_IONIZATION) {
+        const float r0 = v[0] - w[0], r1 = v[1] - w[1], r2 = v[2] - w[2];
+        float sec[3];
+        isotropic(sqrtf(r0 * r0 + r1 * r1 + r2 * r2), rng, sec);
+        if constexpr (!kLedger) return ch.kind != AURORA_ATTACHMENT;   // draws consumed
+        else {
+        const float s0 = w[0] + sec[0], s1 = w[1] + sec[1], s2 = w[2] + sec[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            if (t.nsec == k) {
+                t.sec[k][0] = s0;
+                t.sec[k][1] = s1;
+                t.sec[k][2] = s2;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) t.ion[k][a] = w[a];
+            }
+        ++t.nsec;
+        if (m.half_sec_mass != 0.f) {
+            const float added = m.half_sec_mass * (s0 * s0 + s1 * s1 + s2 * s2) +
+                                m.half_ion_mass * (w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+#pragma unroll
+            for (int k = 0; k < NCH; ++k)
+                if (k == c) t.energy_change[k] += added;
+        }
+        }
+    }
+    return ch.kind != AURORA_ATTACHMENT;
+}
+
 template <int NCH>
-__device__ void mcc_collide(const MccParams& m, const XsView& X, const ChannelSel* chan, float* v,
-                            float dt, Philox& rng, MccTally<NCH>& t) {
+__device__ __forceinline__ void mcc_tally_reset(MccTally<NCH>& t) {
     t.candidates = 0;
     t.nulls = 0;
 #pragma unroll
     for (int c = 0; c < NCH; ++c) { t.chan[c] = 0; t.energy_change[c] = 0.f; }
     t.nsec = 0;
     t.err = 0;
+}
+
+template <int NCH>
+__device__ __forceinline__ void mcc_tally_reset(MccCounts<NCH>& t) {
+    t.candidates = 0;
+    t.nulls = 0;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) t.chan[c] = 0;
+    t.err = 0;
+}
+
+// One particle-step of null-collision MCC on velocity v (in place).
+template <int NCH>
+__device__ void mcc_collide(const MccParams& m, const XsView& X, const ChannelSel* chan, float* v,
+                            float dt, Philox& rng, MccTally<NCH>& t) {
+    mcc_tally_reset(t);
     if (!isfinite(v[0]) || !isfinite(v[1]) || !isfinite(v[2])) {
         t.err = DEV_ERR_NONFINITE;
         return;
     }
     float elapsed = 0.f;
-    float rate[NCH];
-    for (;;) {
-        const float speed = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        if (m.neutral_vstd == 0.f) {
-            // validate_frequency_bound (Collision.cpp:627-632): below
-            // safe_speed the check cannot fail, so it is evaluated only above
-            if (speed > m.safe_speed && above_bound(m, mcc_rates<NCH>(m, X, speed, rate))) {
-                t.err = DEV_ERR_RATE;
-                return;
-            }
-        } else if (thermal_bound_exceeded<NCH>(m, X, speed)) {
-            t.err = DEV_ERR_RATE;
-            return;
-        }
-        const float waiting = -logf(rng.u01()) * m.inv_max_frequency;
-        if (!(waiting < dt - elapsed)) break;
-        elapsed += waiting;
-        ++t.candidates;
-        if (t.candidates > static_cast<uint32_t>(m.max_candidates)) {
-            t.err = DEV_ERR_CANDIDATES;
-            return;
-        }
-        float w[3];
-        neutral_velocity(m, rng, w);
-        const float g[3] = {v[0] - w[0], v[1] - w[1], v[2] - w[2]};
-        const float gs = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-        // rates at the relative speed (Collision.cpp:943-944); with a cold
-        // neutral gs equals the speed checked above
-        const float total = mcc_rates<NCH>(m, X, gs, rate);
-        if (m.neutral_vstd != 0.f && above_bound(m, total)) {
-            t.err = DEV_ERR_RATE;
-            return;
-        }
-        const float selection = rng.u01() * m.max_frequency;
-        float cumulative = 0.f;
-        int accepted = -1;
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-            cumulative += rate[c];
-            if (accepted < 0 && selection < cumulative) accepted = c;
-        }
-        if (accepted < 0) { ++t.nulls; continue; }
-        const int c = accepted;
-        const ChannelSel ch = chan[c];
-        const float e_before = m.half_mass * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-        apply_channel(m, ch, v, w, g, gs, rng);
-        const float de = m.half_mass * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) - e_before;
-#pragma unroll
-        for (int k = 0; k < NCH; ++k)
-            if (k == c) {
-                t.energy_change[k] += de;
-                ++t.chan[k];
-            }
-        if (ch.kind == AURORA_IONIZATION) {
-            const float r0 = v[0] - w[0], r1 = v[1] - w[1], r2 = v[2] - w[2];
-            float sec[3];
-            isotropic(sqrtf(r0 * r0 + r1 * r1 + r2 * r2), rng, sec);
-            const float s0 = w[0] + sec[0], s1 = w[1] + sec[1], s2 = w[2] + sec[2];
-#pragma unroll
-            for (int k = 0; k < 2; ++k)
-                if (t.nsec == k) {
-                    t.sec[k][0] = s0;
-                    t.sec[k][1] = s1;
-                    t.sec[k][2] = s2;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) t.ion[k][a] = w[a];
-                }
-            ++t.nsec;
-            if (m.half_sec_mass != 0.f) {
-                const float added = m.half_sec_mass * (s0 * s0 + s1 * s1 + s2 * s2) +
-                                    m.half_ion_mass * (w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-#pragma unroll
-                for (int k = 0; k < NCH; ++k)
-                    if (k == c) t.energy_change[k] += added;
-            }
-        }
-        if (ch.kind == AURORA_ATTACHMENT) break;
-    }
+    while (mcc_next_candidate<NCH>(m, X, v, dt, rng, elapsed, t))
+        if (!mcc_candidate<NCH>(m, X, chan, v, rng, t)) break;
 }
 
 // bounded Maxwellian neutral velocity of the 1V collide: one normal from

@@ -67,8 +67,36 @@ struct WorkItem {
     int chunk;
 };
 
+// lanes of a wave that wait with a collision candidate before the wave
+// runs the candidate code for them together
+constexpr int kCandidateBatch = 32;
+
+// the rest of a swarm step after its candidate loop: the second half kick,
+// the energy checks, and (after burn-in) the counters and block sums
 template <int NCH>
-__global__ void __launch_bounds__(kChunk, 5) k_swarm(MccParams m, const RequestDesc* __restrict__ descs,
+__device__ __forceinline__ void swarm_end_step(const RequestDesc& d, bool sampling,
+                                               const MccCounts<NCH>& t, float* v, uint32_t& err,
+                                               float& emax, uint32_t& cand, uint32_t& nulls,
+                                               uint32_t* chan, long long& acc_v,
+                                               long long& acc_e, int& s) {
+    err |= t.err;
+    v[0] += d.half_kick;
+    const float e = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    emax = fmaxf(emax, e);
+    if (e > d.max_energy_ev) err |= DEV_ERR_ENERGY;
+    if (sampling) {
+        cand += t.candidates;
+        nulls += t.nulls;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) chan[c] += t.chan[c];
+        acc_v += __float2ll_rn(v[0] * d.scale_v);
+        acc_e += __float2ll_rn(e * d.scale_e);
+    }
+    ++s;
+}
+
+template <int NCH>
+__global__ void __launch_bounds__(kChunk, 4) k_swarm(MccParams m, const RequestDesc* __restrict__ descs,
                                                   const WorkItem* __restrict__ items, int nitems,
                                                   int* head) {
     __shared__ MccLds lds;
@@ -104,7 +132,9 @@ __global__ void __launch_bounds__(kChunk, 5) k_swarm(MccParams m, const RequestD
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(head, 1);
         __syncthreads();
-        const int item = item_s;
+        // uniform across the workgroup: the work item and its request
+        // descriptor go to scalar registers
+        const int item = __builtin_amdgcn_readfirstlane(item_s);
         __syncthreads();
         if (item >= nitems) break;
         const WorkItem w = items[item];
@@ -123,61 +153,88 @@ __global__ void __launch_bounds__(kChunk, 5) k_swarm(MccParams m, const RequestD
         uint32_t err = 0;
         float emax = 0.f;
         long long acc_v = 0, acc_e = 0;
-        int group = -1;               // uncertainty block being accumulated
-        int64_t group_end = 0;        // first sample index of the next block
-        for (int s = d.step0; s < d.step0 + d.nsteps; ++s) {
-            const bool sampling = s >= d.burn_in;
-            if (active && !err) {
-                v[0] += d.half_kick;
-                const float e_pre = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                emax = fmaxf(emax, e_pre);
-                if (e_pre > d.max_energy_ev) err |= DEV_ERR_ENERGY;
-                if (sampling) {
-                    x[0] += v[0] * d.dt;
-                    x[1] += v[1] * d.dt;
-                    x[2] += v[2] * d.dt;
+        // The launch's steps in segments that end at the burn-in end, at
+        // every uncertainty-block boundary and at the launch end (the
+        // chunk's block sums are flushed per segment).  Inside a segment
+        // every lane steps on its own: a lane whose step holds a collision
+        // candidate waits until kCandidateBatch lanes of its wave do (or no
+        // lane can step on), then the wave runs the candidate code for all
+        // of them at once -- a rare event no longer costs the whole wave
+        // every step.  Each lane's operations and draws are those of the
+        // step-synchronous loop, so results are unchanged.
+        const int s_stop = d.step0 + d.nsteps;
+        for (int s0 = d.step0; s0 < s_stop;) {
+            int seg_end, group = -1;
+            if (s0 < d.burn_in) {
+                seg_end = min(s_stop, d.burn_in);
+            } else {
+                // uncertainty block b covers samples [floor(b S / B),
+                // floor((b + 1) S / B)); blocks are at least one sample long
+                const int64_t k = s0 - d.burn_in;
+                group = static_cast<int>(((k + 1) * d.blocks - 1) / d.sampling);
+                const int64_t gend = (static_cast<int64_t>(group) + 1) * d.sampling / d.blocks;
+                seg_end = static_cast<int>(min(static_cast<int64_t>(s_stop), d.burn_in + gend));
+            }
+            const bool sampling = group >= 0;
+            int s = s0;
+            bool pending = false;
+            float elapsed = 0.f;
+            Philox rng;
+            MccCounts<NCH> t;
+            mcc_tally_reset(t);
+            for (;;) {
+                if (__ballot(active && s < seg_end) == 0ull) break;
+                if (active && s < seg_end && !pending) {
+                    if (err) {
+                        // a failed particle keeps its velocity; the block
+                        // sums still take it every step
+                        if (sampling) {
+                            const float e = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                            acc_v += __float2ll_rn(v[0] * d.scale_v);
+                            acc_e += __float2ll_rn(e * d.scale_e);
+                        }
+                        ++s;
+                    } else {
+                        v[0] += d.half_kick;
+                        const float e_pre = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                        emax = fmaxf(emax, e_pre);
+                        if (e_pre > d.max_energy_ev) err |= DEV_ERR_ENERGY;
+                        if (sampling) {
+                            x[0] += v[0] * d.dt;
+                            x[1] += v[1] * d.dt;
+                            x[2] += v[2] * d.dt;
+                        }
+                        rng.init(d.seed, 0u, static_cast<uint32_t>(p), static_cast<uint32_t>(s));
+                        mcc_tally_reset(t);
+                        elapsed = 0.f;
+                        if (!isfinite(v[0]) || !isfinite(v[1]) || !isfinite(v[2]))
+                            t.err = DEV_ERR_NONFINITE;
+                        else
+                            pending = mcc_next_candidate<NCH>(m, X, v, d.dt, rng, elapsed, t);
+                        if (!pending)
+                            swarm_end_step<NCH>(d, sampling, t, v, err, emax, cand, nulls, chan, acc_v, acc_e, s);
+                    }
                 }
-                Philox rng;
-                rng.init(d.seed, 0u, static_cast<uint32_t>(p), static_cast<uint32_t>(s));
-                MccTally<NCH> t;
-                mcc_collide<NCH>(m, X, lds.chan, v, d.dt, rng, t);
-                err |= t.err;
-                v[0] += d.half_kick;
-                const float e = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                emax = fmaxf(emax, e);
-                if (e > d.max_energy_ev) err |= DEV_ERR_ENERGY;
-                if (sampling) {
-                    cand += t.candidates;
-                    nulls += t.nulls;
-#pragma unroll
-                    for (int c = 0; c < NCH; ++c) chan[c] += t.chan[c];
+                const uint64_t waiting_lanes = __ballot(pending);
+                if (waiting_lanes == 0ull) continue;
+                const uint64_t stepping = __ballot(active && s < seg_end && !pending);
+                if (__popcll(waiting_lanes) < kCandidateBatch && stepping != 0ull) continue;
+                if (pending) {
+                    if (!(mcc_candidate<NCH>(m, X, lds.chan, v, rng, t) &&
+                          mcc_next_candidate<NCH>(m, X, v, d.dt, rng, elapsed, t))) {
+                        pending = false;
+                        swarm_end_step<NCH>(d, sampling, t, v, err, emax, cand, nulls, chan, acc_v, acc_e, s);
+                    }
                 }
             }
             if (sampling) {
-                // uncertainty block b covers samples [floor(b S / B),
-                // floor((b + 1) S / B)); blocks are at least one sample long
-                const int64_t k = s - d.burn_in;
-                if (group < 0) {
-                    group = static_cast<int>(((k + 1) * d.blocks - 1) / d.sampling);
-                    group_end = (static_cast<int64_t>(group) + 1) * d.sampling / d.blocks;
-                } else if (k >= group_end) {
-                    add_chunk_sums(acc_v, acc_e,
-                                   d.part + (static_cast<int64_t>(w.chunk) * d.blocks + group) * 2);
-                    acc_v = 0;
-                    acc_e = 0;
-                    group += 1;
-                    group_end = (static_cast<int64_t>(group) + 1) * d.sampling / d.blocks;
-                }
-                if (active) {
-                    const float e = d.ev_per_v2 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                    acc_v += __float2ll_rn(v[0] * d.scale_v);
-                    acc_e += __float2ll_rn(e * d.scale_e);
-                }
+                add_chunk_sums(acc_v, acc_e,
+                               d.part + (static_cast<int64_t>(w.chunk) * d.blocks + group) * 2);
+                acc_v = 0;
+                acc_e = 0;
             }
+            s0 = seg_end;
         }
-        if (group >= 0)
-            add_chunk_sums(acc_v, acc_e,
-                           d.part + (static_cast<int64_t>(w.chunk) * d.blocks + group) * 2);
         const bool last = d.step0 + d.nsteps == d.burn_in + d.sampling;
         if (last) {
             // displacement moments of the chunk, reduced in a fixed order

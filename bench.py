@@ -207,8 +207,11 @@ def serving_leg(args):
     # explicitly for the GPU: a 1e6-electron request of 20,000 steps is 2e10
     work_items = 10 ** 12
     # capacity: every request present at t = 0, advanced together
+    def progress(done, total, secs):
+        log(f"  {done}/{total} requests served after {secs:.0f} s")
+
     sat = serving.serve(ctx, reqs, np.zeros(R), budget, steps_per_launch=args.serving_launch_steps,
-                        work_item_limit=work_items)
+                        work_item_limit=work_items, progress=progress)
     log(f"capacity: {R} requests in {sat.wall_s:.3f} s batched")
     # the same requests one at a time, each alone on the GPU
     if args.serving_full:
@@ -223,11 +226,13 @@ def serving_leg(args):
     electron_steps = sum(r.steps * r.particles for r in reqs)
     log(f"sequential ({seq_kind}): {seq_wall:.3f} s")
     log(f"serving {R} swarm requests x {steps} steps (arrivals 200/s, budget "
-        f"{args.serving_hbm_gib} GiB) + {args.ensemble_instances} discharge instances")
+        f"{args.serving_hbm_gib} GiB, at most {args.serving_active_electrons:.3g} electrons "
+        f"admitted) + {args.ensemble_instances} discharge instances")
     t0 = time.perf_counter()
+    cap = int(args.serving_active_electrons) or None
     rep = serving.serve(ctx, reqs, cfg["arrival_s"][:R], budget,
                         steps_per_launch=args.serving_launch_steps, on_launch=ensemble_tick,
-                        work_item_limit=work_items)
+                        work_item_limit=work_items, progress=progress, max_active_particles=cap)
     wall = time.perf_counter() - t0
     summ = rep.summary()
     ens_updates = ens_steps[0] * args.ensemble_instances * args.ensemble_particles
@@ -254,6 +259,7 @@ def serving_leg(args):
         "requests_per_s_offered_load": summ["requests_per_s"],
         "wall_s": wall,
         "max_active": summ["max_active"],
+        "active_electron_cap": cap,
         "launches": summ["launches"],
         "budget_refusals": summ["budget_refusals"],
         "capacity_requests_per_s": R / sat.wall_s,
@@ -287,6 +293,11 @@ def main():
                     help="fixed tile shape tx,ty,tz,halo (skips the autotuner)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--counter-pass", default=None, metavar="VARIANTS",
+                    help="counter-collection driver for rocprofv3 --pmc: a flat device copy "
+                         "(1 GiB read + 1 GiB written per launch, two launches), then two "
+                         "steps of each push variant 'tx,ty,tz,halo,block,prefetch;...' right "
+                         "after a sort; prints no JSON line")
     ap.add_argument("--no-discharge", action="store_true")
     ap.add_argument("--discharge-steps", type=int, default=1000)
     ap.add_argument("--discharge-warmup", type=int, default=100)
@@ -297,6 +308,8 @@ def main():
     ap.add_argument("--serving-steps", type=int, default=4000)
     ap.add_argument("--serving-launch-steps", type=int, default=500)
     ap.add_argument("--serving-hbm-gib", type=int, default=8)
+    ap.add_argument("--serving-active-electrons", type=float, default=2e6,
+                    help="cap on the electrons of the admitted requests together (0: none)")
     ap.add_argument("--ensemble-instances", type=int, default=64)
     ap.add_argument("--ensemble-particles", type=int, default=100_000)
     ap.add_argument("--ensemble-steps-per-launch", type=int, default=50)
@@ -334,6 +347,18 @@ def main():
     total = sum(s["particles"] for s in cfg["species"])
     ctx.sort()
     ctx.initialize(dt)
+    if args.counter_pass:
+        log("copy calibration:", ctx.stream_bandwidth(2, 2 << 30, 1), "GB/s")
+        for v in args.counter_pass.split(";"):
+            shape = [int(x) for x in v.split(",")]
+            ctx.tiling(*shape[:4])
+            ctx.push_variant(shape[4], shape[5])
+            ctx.sort()
+            ctx.step(dt, 2)
+            ctx.sync()
+            log("counter pass: variant", v)
+        ctx.close()
+        return
     ctx.sort_interval(cfg["sort_interval"])
     adaptive = args.sort == "adaptive"
     if adaptive:
@@ -346,7 +371,8 @@ def main():
         if len(shape) > 4:
             ctx.push_variant(shape[4], shape[5])
         ctx.sort()
-        tuned = {"tile": shape[:3], "halo": shape[3], "sort": sort_desc, "fixed": True}
+        tuned = {"tile": shape[:3], "halo": shape[3], "block": shape[4] if len(shape) > 4 else 512,
+                 "prefetch": shape[5] if len(shape) > 4 else 0, "sort": sort_desc, "fixed": True}
     elif not args.no_autotune:
         from aurora_pic import autotune
         best, table = autotune.tune(ctx, dt, sort_intervals=(cfg["sort_interval"],), log=log,
@@ -395,13 +421,21 @@ def main():
     log("stream peak:", json.dumps(stream_peak))
     ctx.close()
 
+    # HBM bytes per launch of the variant that ran, from the counter runs
+    # recorded per variant (null when that variant was not measured)
     traffic = None
+    variant = None
+    if tuned is not None:
+        variant = ",".join(str(v) for v in (*tuned["tile"], tuned["halo"],
+                                            tuned.get("block", 512), tuned.get("prefetch", 0)))
     tfile = os.path.join(REPO, "profiles", "push_deposit_plasma3d_traffic.txt")
-    if os.path.exists(tfile):
+    if variant and os.path.exists(tfile):
         with open(tfile) as f:
             for line in f:
-                if line.startswith("bytes_per_launch:"):
-                    traffic = int(line.split(":", 1)[1])
+                parts = line.split()
+                if len(parts) >= 4 and parts[0] == "variant" and parts[1] == variant and \
+                        parts[2] == "bytes_per_launch":
+                    traffic = int(parts[3])
 
     discharge_out = None
     if not args.no_discharge:
@@ -451,6 +485,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "push_deposit (gather+Boris+drift+wrap+int64 deposit)",
+                     "variant": variant,
                      "bytes_per_update": BYTES_PER_UPDATE,
                      "attainable": stream_peak},
         "cpu_baseline": base,

@@ -98,3 +98,26 @@ def test_serve_rejects_work_above_the_item_limit():
         else:
             assert s.result is not None
     ctx.close()
+
+
+def test_active_electron_cap_serves_first_come_first_served():
+    """With a cap on the admitted electrons, requests wait in arrival order
+    and never run more than the cap allows together; results still equal
+    the lone runs."""
+    from aurora_pic import hip
+    count, steps = 8, 400
+    cfg, reqs = _requests(count, steps, seed=9)
+    ctx = hip.Context(0)
+    ctx.swarm_gas(cfg["gas"])
+    cap = max(r.particles for r in reqs) * 2
+    rep = serving.serve(ctx, reqs, np.zeros(count), 0, steps_per_launch=100,
+                        work_item_limit=10 ** 12, max_active_particles=cap)
+    print("capped:", rep.summary(), flush=True)
+    assert 1 <= rep.max_active < count
+    served = sorted(rep.served, key=lambda s: s.index)
+    admits = [s.admit_s for s in served]
+    assert admits == sorted(admits)            # admitted in arrival order
+    lone, _, _ = serving.run_sequential(ctx, reqs)
+    for i in range(count):
+        assert _same(served[i].result, lone[i]), i
+    ctx.close()
