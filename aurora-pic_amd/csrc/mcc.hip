@@ -33,7 +33,7 @@ constexpr int kBlk = 256;
 
 // standalone collide of count velocities (parity checks): uid = index
 template <int NCH>
-__global__ void __launch_bounds__(kBlk, 5) k_collide(MccParams m, float* v, int64_t count,
+__global__ void __launch_bounds__(kBlk, 4) k_collide(MccParams m, float* v, int64_t count,
                                                   float dt, uint64_t seed, uint32_t stream,
                                                   uint32_t step, double* rows, uint32_t* err,
                                                   int one_v) {
@@ -334,7 +334,9 @@ int64_t mcc_collide_pass(Context& c, int model, double dt) {
     a.err = c.dev_error.ptr;
     {
         KernelScope ks(c, "mcc");
-        const int g = blocks(c, n);
+        // one round of resident workgroups (grid-stride): fewer tally rows
+        // for the one-block fold that follows
+        const int g = std::min(blocks(c, n), 4 * c.cu_count);
         const int64_t rows = energy_rows(g);
         if (h.energy_rows.count < static_cast<size_t>(rows * kTallyStride))
             h.energy_rows.alloc(static_cast<size_t>(rows * kTallyStride));
@@ -424,6 +426,13 @@ void mcc_step(Context& c, double dt) {
     AUR_REQUIRE(c.dim == 1, AURORA_ERR_UNSUPPORTED,
                 "MCC inside the particle step is built for 1D meshes");
     const int models = c.mcc_models();
+    // Simulation.cpp:437-439
+    for (int k = 0; k < models; ++k) {
+        const MccModelHost& h = c.mcc_model(k);
+        for (int ch = 0; ch < h.nchan; ++ch)
+            AUR_REQUIRE(h.kind[ch] != AURORA_ATTACHMENT, AURORA_ERR_INVALID_ARGUMENT,
+                        "1D MCC does not support attachment");
+    }
     if (c.velocity_dims == 1)
         for (int k = 0; k < models; ++k) {
             const MccModelHost& h = c.mcc_model(k);

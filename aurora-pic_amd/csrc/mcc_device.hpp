@@ -199,7 +199,7 @@ __device__ __forceinline__ void isotropic(float speed, Philox& rng, float* out) 
 
 // Henyey-Greenstein deflection of the relative velocity g0 (Collision.cpp:
 // 66-134): inverse-CDF cosine for mean cosine mc, uniform azimuth about g0.
-__device__ void henyey_greenstein(const float* g0, float speed, float mc, Philox& rng,
+__device__ __forceinline__ void henyey_greenstein(const float* g0, float speed, float mc, Philox& rng,
                                   float* out) {
     const float inc = sqrtf(g0[0] * g0[0] + g0[1] * g0[1] + g0[2] * g0[2]);
     if (!(inc > 0.f)) {
@@ -267,7 +267,7 @@ struct ChannelSel {
 
 // apply_channel (Collision.cpp:738-829) for the accepted channel ch:
 // relative velocity g (speed gs) against neutral velocity w
-__device__ void apply_channel(const MccParams& m, const ChannelSel& ch, float* v, const float* w,
+__device__ __forceinline__ void apply_channel(const MccParams& m, const ChannelSel& ch, float* v, const float* w,
                               const float* g, float gs, Philox& rng) {
     const int kind = ch.kind;
     if (kind == AURORA_ATTACHMENT) {
@@ -395,15 +395,17 @@ __device__ __forceinline__ bool mcc_candidate(const MccParams& m, const XsView& 
         if constexpr (!kLedger) return ch.kind != AURORA_ATTACHMENT;   // draws consumed
         else {
         const float s0 = w[0] + sec[0], s1 = w[1] + sec[1], s2 = w[2] + sec[2];
+        // selects, not a store through a computed index (which would put the
+        // tally in scratch memory)
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
-            if (t.nsec == k) {
-                t.sec[k][0] = s0;
-                t.sec[k][1] = s1;
-                t.sec[k][2] = s2;
+        for (int k = 0; k < 2; ++k) {
+            const bool here = t.nsec == k;
+            t.sec[k][0] = here ? s0 : t.sec[k][0];
+            t.sec[k][1] = here ? s1 : t.sec[k][1];
+            t.sec[k][2] = here ? s2 : t.sec[k][2];
 #pragma unroll
-                for (int a = 0; a < 3; ++a) t.ion[k][a] = w[a];
-            }
+            for (int a = 0; a < 3; ++a) t.ion[k][a] = here ? w[a] : t.ion[k][a];
+        }
         ++t.nsec;
         if (m.half_sec_mass != 0.f) {
             const float added = m.half_sec_mass * (s0 * s0 + s1 * s1 + s2 * s2) +

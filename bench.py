@@ -141,13 +141,35 @@ def discharge_leg(args):
     counts, _ = ctx.mcc_stats()
     ctx.close()
     updates = 0.5 * (live0 + live1) * args.discharge_steps
+    # the same step through the batched path with one instance: steps run in
+    # HIP-graph-captured segments of 50, with no host round trip per step
+    # (an instance equals the context's run: particles, fields and counts
+    # bit for bit, ledgers to rounding)
+    from aurora_pic import serving
+    ectx, ens, edt = serving.discharge_ensemble(instances=1, particles=2 * cfg["species"][0]
+                                                ["particles"], capacity_factor=4, graph_steps=50)
+    ens.step(edt, args.discharge_warmup)
+    ectx.sync()
+    e_live0 = sum(ens.count(0, sid)[1] for sid in range(2))
+    t0 = time.perf_counter()
+    ens.step(edt, args.discharge_steps)
+    ectx.sync()
+    e_wall = time.perf_counter() - t0
+    e_live1 = sum(ens.count(0, sid)[1] for sid in range(2))
+    ectx.close()
+    e_updates = 0.5 * (e_live0 + e_live1) * args.discharge_steps
     return {"workload": "1D3V RF discharge: 513 nodes, 5e5 electrons + 5e5 ions (FP32), 200 V "
                         "at 13.56 MHz, synthetic gas at 1e21 m^-3 with ionization products",
             "particle_updates_per_s": updates / wall,
             "ms_per_step": 1e3 * wall / args.discharge_steps,
             "steps": args.discharge_steps, "warmup": args.discharge_warmup,
             "live_particles": [live0, live1], "ionizations": int(counts[4]),
-            "kernels_ms": kernels}
+            "kernels_ms": kernels,
+            "graph_path": {"particle_updates_per_s": e_updates / e_wall,
+                           "ms_per_step": 1e3 * e_wall / args.discharge_steps,
+                           "live_particles": [e_live0, e_live1],
+                           "note": "one-instance ensemble: the context's step captured in "
+                                   "HIP graphs of 50 steps"}}
 
 
 def sequential_estimate(ctx, reqs, strata, log_each=False):
@@ -195,8 +217,15 @@ def serving_leg(args):
     ens_ctx, ens, ens_dt = serving.discharge_ensemble(args.ensemble_instances,
                                                       args.ensemble_particles)
     ens_steps = [0]
+    last_tick = [-1e30]
 
     def ensemble_tick():
+        # a steady background load: one block of ensemble steps per
+        # --ensemble-interval seconds, whatever the swarm launches' length
+        now = time.perf_counter()
+        if now - last_tick[0] < args.ensemble_interval:
+            return
+        last_tick[0] = now
         ens.step(ens_dt, args.ensemble_steps_per_launch)
         ens_steps[0] += args.ensemble_steps_per_launch
 
@@ -253,7 +282,8 @@ def serving_leg(args):
                     f"({burn} burn-in), synthetic elastic/excitation/ionization gas at 1e21 "
                     f"m^-3; mixed with {args.ensemble_instances} distinct-seed 1D3V discharge "
                     f"instances of {args.ensemble_particles} particles (HIP-graph ensemble on "
-                    f"its own stream, stepped while swarm launches run)",
+                    f"its own stream, {args.ensemble_steps_per_launch} steps every "
+                    f"{args.ensemble_interval} s while swarm launches run)",
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "requests_per_s_offered_load": summ["requests_per_s"],
@@ -298,6 +328,10 @@ def main():
                          "(1 GiB read + 1 GiB written per launch, two launches), then two "
                          "steps of each push variant 'tx,ty,tz,halo,block,prefetch;...' right "
                          "after a sort; prints no JSON line")
+    ap.add_argument("--swarm-counter-pass", action="store_true",
+                    help="counter-collection driver for the swarm kernel: 64 requests of the "
+                         "served stream's distribution, two launches of 500 steps with every "
+                         "request active; prints no JSON line")
     ap.add_argument("--no-discharge", action="store_true")
     ap.add_argument("--discharge-steps", type=int, default=1000)
     ap.add_argument("--discharge-warmup", type=int, default=100)
@@ -313,6 +347,8 @@ def main():
     ap.add_argument("--ensemble-instances", type=int, default=64)
     ap.add_argument("--ensemble-particles", type=int, default=100_000)
     ap.add_argument("--ensemble-steps-per-launch", type=int, default=50)
+    ap.add_argument("--ensemble-interval", type=float, default=0.1,
+                    help="seconds between blocks of ensemble steps while swarms are served")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the per-step loop of one 3D3V instance runs on a "
@@ -322,6 +358,22 @@ def main():
 
     from aurora_pic import hip
     from aurora_pic import workloads as wl
+
+    if args.swarm_counter_pass:
+        from aurora_pic import serving
+        scfg = wl.config_swarm_requests(4096)
+        reqs = serving.make_requests(scfg, count=64, steps=1000, burn_in=500)
+        sctx = hip.Context(0)
+        sctx.swarm_gas(scfg["gas"])
+        for r in reqs:
+            sctx.swarm_submit(r, hbm_budget=0)
+        for k in range(2):
+            sctx.swarm_advance(500)
+            sctx.sync()
+            log(f"swarm counter pass: launch {k}, "
+                f"{sum(r.particles for r in reqs) * 500} electron-steps")
+        sctx.close()
+        return
 
     if args.copy_calibration:
         import torch

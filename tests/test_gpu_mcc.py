@@ -418,3 +418,26 @@ def test_second_model_targets_another_species(ctx):
         assert np.array_equal(c.mcc_model_stats(1)[0], after)   # the restart repeats
     finally:
         c.close()
+
+
+def test_1d_step_refuses_attachment():
+    """The 1D step refuses an attaching channel with the reference's message
+    (Simulation.cpp:437-439); the standalone collide keeps it."""
+    from aurora_pic import hip
+    e = np.geomspace(1e-3, 1e3, 50) * workloads.EV
+    gas = dict(channels=[(workloads.ELASTIC, 0.0, e, np.full(50, 1e-20)),
+                         (workloads.ATTACHMENT, 0.0, e, np.full(50, 1e-21))],
+               neutral_density=1e21, max_frequency=workloads.max_collision_frequency(
+                   [(0, 0.0, e, np.full(50, 1.1e-20))], 1e21, margin=1.5),
+               neutral_mass=0.0)
+    c = hip.Context(0)
+    try:
+        c.mesh([33], [0.01], hip.DIRICHLET, workloads.EPS0)
+        se = c.add_species(-workloads.EV, workloads.ME, 1e6, 1000)
+        c.init_maxwellian(se, 100, math.sqrt(2 * workloads.EV / workloads.ME), seed=3)
+        c.mcc_load(se, gas, seed=4)
+        c.initialize(1e-12)
+        with pytest.raises(hip.HipError, match="1D MCC does not support attachment"):
+            c.step(1e-12, 1)
+    finally:
+        c.close()
