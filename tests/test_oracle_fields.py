@@ -327,3 +327,42 @@ def test_fixed_point_3d_weights_exact(oracle):
                 for a in range(2):
                     exact = Fraction(wx[a]) * Fraction(wyz) * 2 ** 40
                     assert acc[(c * 4 + b) * 4 + a] == round(exact), (o, a, b, c)
+
+
+def test_step3d_is_the_pinned_pieces_in_order(oracle):
+    """or_step3d (the CPU baseline's Simulation3D::step, Simulation3D.cpp:
+    342-365) is the pinned push, deposit, periodic solve and gather in the
+    reference's order: one step equals that composition bit for bit, and the
+    OpenMP synchronisation loop gives the same state on 1 and 4 threads."""
+    import copy
+    n, L, eps, dt = [8, 6, 4], [1.0, 0.75, 0.5], 1.0, 0.01
+    species = [(-1.0, 1.0, 1e-3, 600, 0.3), (1.0, 50.0, 1e-3, 400, 0.05)]
+    st = oracle.plasma3d_init(n, L, eps, dt, species, seed=11)
+    st0 = copy.deepcopy(st)
+    st4 = copy.deepcopy(st)
+    oracle.step3d(st, 1)
+    oracle.step3d(st4, 4)
+    for key in ("pos", "vh", "v", "alive"):
+        for a, b in zip(st[key], st4[key]):
+            assert np.array_equal(a, b)
+    for key in ("rho", "phi", "E"):
+        assert np.array_equal(st[key], st4[key])
+    # the composition from the same start
+    rho = np.zeros(int(np.prod(n)))
+    E0 = st0["E"].reshape(3, -1)
+    moved = []
+    for k, (q, m, w, count, _) in enumerate(species):
+        pos, vh, alive, _ = oracle.push(n, L, 0, st0["pos"][k].reshape(-1, 3),
+                                        st0["vh"][k].reshape(-1, 3), st0["alive"][k], E0,
+                                        q / m, dt)
+        assert np.array_equal(pos.reshape(-1), st["pos"][k])
+        assert np.array_equal(vh.reshape(-1), st["vh"][k])
+        rho += oracle.deposit_cic(n, L, 0, pos, q, w)
+        moved.append((pos, vh, q / m))
+    # one accumulator over both species there, a sum of per-species arrays here
+    assert np.max(np.abs(rho - st["rho"])) <= 1e-14 * np.max(np.abs(rho))
+    _, E = oracle.poisson(n, L, 0, eps, st["rho"])
+    assert np.array_equal(E.reshape(-1), st["E"])
+    for k, (pos, vh, qm) in enumerate(moved):
+        e = oracle.gather(n, L, 0, pos, E)
+        assert np.array_equal(vh + 0.5 * qm * e * dt, st["v"][k].reshape(-1, 3))
