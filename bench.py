@@ -172,6 +172,70 @@ def discharge_leg(args):
                                    "HIP graphs of 50 steps"}}
 
 
+def other_configs_leg(args):
+    """Particle-updates/s of the two smaller configurations, timed the same
+    way (state resident, whole steps): [0] the 1D1V two-stream (25,600 FP64
+    electrons, 256 periodic nodes, spectral solve, 2,000 steps) and [2] the
+    2D3V Langmuir wave (4,194,304 FP32 electrons, 256^2 periodic nodes, FFT,
+    uniform B_z, 1,000 steps)."""
+    import numpy as np
+    from aurora_pic import hip
+    from aurora_pic import workloads as wl
+
+    out = {}
+    c0 = wl.config_two_stream()
+    rng = np.random.default_rng(c0["seed"])
+    s0 = c0["species"][0]
+    N = s0["particles"]
+    L = c0["L"][0]
+    u, spread = c0["beams"]
+    x = rng.uniform(0.0, L, N)
+    x = np.mod(x + c0["perturbation"] * np.sin(2.0 * np.pi * x / L) * L / (2.0 * np.pi), L)
+    v = np.zeros((N, 3))
+    v[:, 0] = np.where(np.arange(N) % 2 == 0, u, -u) + rng.normal(0.0, spread, N)
+    ctx = hip.Context(0)
+    ctx.mesh(c0["n"], c0["L"], hip.PERIODIC, c0["eps"])
+    sid = ctx.add_species(s0["charge"], s0["mass"], s0["weight"], N, 64)
+    ctx.upload(sid, x[:, None], v, kind=1)
+    ctx.initialize(c0["dt"])
+    ctx.step(c0["dt"], 100)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(c0["dt"], c0["steps"])
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ctx.close()
+    out["two_stream_1d1v"] = {"particles": N, "steps": c0["steps"],
+                              "particle_updates_per_s": N * c0["steps"] / wall,
+                              "ms_per_step": 1e3 * wall / c0["steps"], "precision": "f64"}
+    c2 = wl.config_langmuir_2d()
+    s2 = c2["species"][0]
+    N2 = s2["particles"]
+    Lx, Ly = c2["L"]
+    rng = np.random.default_rng(c2["seed"])
+    pos = np.stack([rng.uniform(0.0, Lx, N2), rng.uniform(0.0, Ly, N2)], axis=1)
+    k = 2.0 * np.pi / Lx
+    pos[:, 0] = np.mod(pos[:, 0] + c2["perturbation"] / k * np.sin(k * pos[:, 0]), Lx)
+    vel = rng.normal(0.0, wl.maxwellian_sigma(1.0, s2["mass"]), (N2, 3))
+    ctx = hip.Context(0)
+    ctx.mesh(c2["n"], c2["L"], hip.PERIODIC, c2["eps"], c2["depth"])
+    ctx.uniform_b(c2["B"])
+    sid = ctx.add_species(s2["charge"], s2["mass"], s2["weight"], N2)
+    ctx.upload(sid, pos, vel, kind=1)
+    ctx.initialize(c2["dt"])
+    ctx.step(c2["dt"], 50)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(c2["dt"], c2["steps"])
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ctx.close()
+    out["langmuir_2d3v"] = {"particles": N2, "steps": c2["steps"],
+                            "particle_updates_per_s": N2 * c2["steps"] / wall,
+                            "ms_per_step": 1e3 * wall / c2["steps"], "precision": "f32"}
+    return out
+
+
 def sequential_estimate(ctx, reqs, strata, log_each=False):
     """The sequential single-instance baseline of a request stream, from a
     stratified sample: requests ordered by particle count, cut into `strata`
@@ -498,6 +562,15 @@ def main():
             log("discharge leg failed:", repr(e))
             discharge_out = {"error": repr(e)}
 
+    others = None
+    if not args.no_discharge:
+        try:
+            others = other_configs_leg(args)
+            log("other configs:", json.dumps(others))
+        except Exception as e:  # a secondary leg must not hide the headline result
+            log("other configs failed:", repr(e))
+            others = {"error": repr(e)}
+
     serving_out = None
     if not args.no_serving:
         try:
@@ -542,6 +615,7 @@ def main():
                      "attainable": stream_peak},
         "cpu_baseline": base,
         "discharge_1d3v": discharge_out,
+        "other_configs": others,
         "serving": serving_out,
     }
     print(json.dumps(out), flush=True)
