@@ -732,8 +732,10 @@ int aurora_hip_set_tiling(aurora_hip_ctx* h, int tx, int ty, int tz, int halo) {
             c.tiled_enabled = false;
             return;
         }
+        // 3D shapes, then 2D ones (z extent 1)
         static const int shapes[][4] = {{8, 8, 8, 2}, {8, 8, 8, 1}, {8, 8, 4, 2}, {8, 8, 4, 1}, {4, 4, 4, 2},
-                                        {16, 8, 4, 2}};
+                                        {16, 8, 4, 2}, {8, 8, 1, 2}, {8, 8, 1, 1}, {16, 16, 1, 2},
+                                        {16, 8, 1, 2}};
         bool known = false;
         for (const auto& s : shapes)
             known = known || (s[0] == tx && s[1] == ty && s[2] == tz && s[3] == halo);

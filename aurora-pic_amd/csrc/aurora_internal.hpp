@@ -336,13 +336,13 @@ struct Context {
     int64_t ntiles() const {
         return static_cast<int64_t>(tiles_axis(0)) * tiles_axis(1) * tiles_axis(2);
     }
-    // the LDS-tiled 3D push applies on periodic meshes whose extents the
-    // tiles divide and that hold a whole tile box without self-overlap
+    // the LDS-tiled 2D and 3D pushes apply on periodic meshes whose extents
+    // the tiles divide and that hold a whole tile box without self-overlap
     bool tiled_push_ok() const {
-        if (!tiled_enabled || dim != 3 || bc != AURORA_PERIODIC) return false;
-        for (int f = 0; f < 6; ++f)
+        if (!tiled_enabled || (dim != 3 && dim != 2) || bc != AURORA_PERIODIC) return false;
+        for (int f = 0; f < 2 * dim; ++f)
             if (pb[f] != AURORA_PB_PERIODIC) return false;
-        for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < dim; ++a)
             if (n[a] % tile[a] != 0 || n[a] < tile[a] + 1 + 2 * halo) return false;
         return true;
     }

@@ -1015,6 +1015,164 @@ bool launch_tile3d(Context& c, Species& sp, double dt) {
            launch_tile3d_t<8, 8, 4, 2, 512, 15>(c, sp, a, mode);
 }
 
+// ------------------------------------------------------- LDS-tiled 2D push
+// The 2D form of k_push_tile3d: one workgroup per TX x TY tile of a
+// tile-sorted species; the tile's float2 E box (halo H) and int64 charge box
+// in LDS; particles that left the box since the last sort take the global
+// gather and atomics.  The same FP32 operations as the untiled k_push<2>
+// (bilinear gather in its order, leapfrog or Boris, periodic drift, the
+// exact 2D fixed-point weights), so particles and charge are identical.
+template <int TX, int TY, int H, int BMODE, int BLOCK, int PF>
+__global__ void __launch_bounds__(BLOCK) k_push_tile2d(PushArgs<float> a,
+                                                       const int32_t* __restrict__ tile_start,
+                                                       int ntx) {
+    constexpr int BX = TX + 1 + 2 * H, BY = TY + 1 + 2 * H;
+    constexpr int BN = BX * BY;
+    __shared__ float2 sE[BN];
+    __shared__ unsigned long long sQ[BN];
+    __shared__ unsigned outside_s;
+    const MeshView& m = a.m;
+    const SpeciesView<float>& sp = a.sp;
+    const int t = blockIdx.x;
+    const int ox = (t % ntx) * TX - H;
+    const int oy = (t / ntx) * TY - H;
+    const float2* E2 = reinterpret_cast<const float2*>(a.E32);
+    for (int q = threadIdx.x; q < BN; q += BLOCK) {
+        const int gx = wrap_near(ox + q % BX, m.nx);
+        const int gy = wrap_near(oy + q / BX, m.ny);
+        sE[q] = E2[gy * m.nx + gx];
+        sQ[q] = 0ull;
+    }
+    if (threadIdx.x == 0) outside_s = 0u;
+    __syncthreads();
+    unsigned outside = 0;
+    struct State {
+        int c;
+        float o0, o1, v0, v1, v2;
+    };
+    const int p0 = tile_start[t], p1 = tile_start[t + 1];
+    const unsigned cnt = static_cast<unsigned>(p1 - p0);
+    int32_t* __restrict__ cellp = sp.cell + p0;
+    float* __restrict__ o0p = sp.off0 + p0;
+    float* __restrict__ o1p = sp.off1 + p0;
+    float* __restrict__ v0p = sp.v0 + p0;
+    float* __restrict__ v1p = sp.v1 + p0;
+    float* __restrict__ v2p = sp.v2 + p0;
+    auto load = [&](unsigned k) {
+        State st;
+        st.c = cellp[k];
+        st.o0 = o0p[k]; st.o1 = o1p[k];
+        st.v0 = v0p[k]; st.v1 = v1p[k]; st.v2 = v2p[k];
+        return st;
+    };
+    constexpr bool kPrefetch = (PF & 1) != 0;
+    unsigned k = threadIdx.x;
+    State next{};
+    if (kPrefetch && k < cnt) next = load(k);
+    for (; k < cnt; k += BLOCK) {
+        const State st = kPrefetch ? next : load(k);
+        if (kPrefetch && k + BLOCK < cnt) next = load(k + BLOCK);
+        if (st.c < 0) continue;
+        int idx[3] = {0, 0, 0};
+        decompose<2>(st.c, m, idx);
+        float off[3] = {st.o0, st.o1, 0.f};
+        float v[3] = {st.v0, st.v1, st.v2};
+        const int rx = rel_index(idx[0], ox, m.nx);
+        const int ry = rel_index(idx[1], oy, m.ny);
+        float E[3] = {0.f, 0.f, 0.f};
+        if (rx <= BX - 2 && ry <= BY - 2) {
+            // gather<2, float>'s expression and order on the staged box
+            const int b00 = ry * BX + rx;
+            const float wx0 = 1.0f - off[0], wx1 = off[0];
+            const float wy0 = 1.0f - off[1], wy1 = off[1];
+            const float w00 = wx0 * wy0, w10 = wx1 * wy0, w01 = wx0 * wy1, w11 = wx1 * wy1;
+            const float2 e00 = sE[b00], e10 = sE[b00 + 1], e01 = sE[b00 + BX],
+                         e11 = sE[b00 + BX + 1];
+            E[0] = e00.x * w00 + e10.x * w10 + e01.x * w01 + e11.x * w11;
+            E[1] = e00.y * w00 + e10.y * w10 + e01.y * w01 + e11.y * w11;
+        } else {
+            gather<2, float>(a, idx, off, E);
+        }
+        if constexpr (BMODE == 0) {
+            v[0] += a.qm_dt * E[0];
+            v[1] += a.qm_dt * E[1];
+        } else {
+            float Bx, By, Bz;
+            magnetic<2, float>(a.b, idx, off, Bx, By, Bz, a.err);
+            boris<float>(v, E, Bx, By, Bz, a.half_qm_dt);
+        }
+        int sh[2];
+        drift_periodic(idx[0], off[0], v[0] * a.dtd0, m.nx, sh[0]);
+        drift_periodic(idx[1], off[1], v[1] * a.dtd1, m.ny, sh[1]);
+        v0p[k] = v[0];
+        v1p[k] = v[1];
+        v2p[k] = v[2];
+        cellp[k] = compose<2>(m, idx);
+        o0p[k] = off[0];
+        o1p[k] = off[1];
+        const unsigned qx = static_cast<unsigned>(rx + sh[0]);
+        const unsigned qy = static_cast<unsigned>(ry + sh[1]);
+        const double o[3] = {static_cast<double>(off[0]), static_cast<double>(off[1]), 0.0};
+        if (qx <= BX - 2 && qy <= BY - 2) {
+            // deposit_one<2>'s weights
+            const double wx[2] = {1.0 - o[0], o[0]};
+            const double wy[2] = {1.0 - o[1], o[1]};
+            const int base = qy * BX + qx;
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int aa = 0; aa < 2; ++aa)
+                    atomicAdd(&sQ[base + b * BX + aa],
+                              static_cast<unsigned long long>(quantise(wx[aa] * wy[b])));
+        } else {
+            deposit_one<2>(a.acc, m, idx, o);
+            ++outside;
+        }
+    }
+    if (outside) atomicAdd(&outside_s, outside);
+    __syncthreads();
+    if (threadIdx.x == 0 && outside_s)
+        atomicAdd(a.absorbed + 6, static_cast<unsigned long long>(outside_s));
+    for (int q = threadIdx.x; q < BN; q += BLOCK) {
+        const unsigned long long val = sQ[q];
+        if (val == 0ull) continue;
+        const int gx = wrap_near(ox + q % BX, m.nx);
+        const int gy = wrap_near(oy + q / BX, m.ny);
+        atomicAdd(reinterpret_cast<unsigned long long*>(a.acc) + gy * m.nx + gx, val);
+    }
+}
+
+template <int TX, int TY, int H, int BLOCK, int PF>
+bool launch_tile2d_t(Context& c, Species& sp, const PushArgs<float>& a, int mode) {
+    if (c.tile[0] != TX || c.tile[1] != TY || c.halo != H || c.push_block != BLOCK ||
+        (c.push_prefetch & 1) != PF)
+        return false;
+    const int ntx = c.tiles_axis(0);
+    const unsigned grid = static_cast<unsigned>(c.ntiles());
+    KernelScope ks(c, "push_deposit");
+    const int32_t* ts = sp.tile_start.ptr;
+    if (mode == 0)
+        k_push_tile2d<TX, TY, H, 0, BLOCK, PF><<<grid, BLOCK, 0, c.stream>>>(a, ts, ntx);
+    else if (mode == 1)
+        k_push_tile2d<TX, TY, H, 1, BLOCK, PF><<<grid, BLOCK, 0, c.stream>>>(a, ts, ntx);
+    else
+        k_push_tile2d<TX, TY, H, 2, BLOCK, PF><<<grid, BLOCK, 0, c.stream>>>(a, ts, ntx);
+    AUR_HIP(hipGetLastError());
+    return true;
+}
+
+// the 2D tile shapes built into the library (tile z extent ignored)
+bool launch_tile2d(Context& c, Species& sp, double dt) {
+    auto a = push_args<float>(c, sp, dt);
+    const int mode = a.b.mode;
+    return launch_tile2d_t<8, 8, 2, 512, 1>(c, sp, a, mode) ||
+           launch_tile2d_t<8, 8, 2, 512, 0>(c, sp, a, mode) ||
+           launch_tile2d_t<8, 8, 1, 512, 1>(c, sp, a, mode) ||
+           launch_tile2d_t<8, 8, 2, 256, 1>(c, sp, a, mode) ||
+           launch_tile2d_t<16, 16, 2, 512, 1>(c, sp, a, mode) ||
+           launch_tile2d_t<16, 8, 2, 512, 1>(c, sp, a, mode);
+}
+
 // Workgroups of a push launch: 1D deposits flush one LDS copy of the mesh
 // per workgroup, so their launches are capped at two workgroups per CU.
 int push_blocks(const Context& c, int64_t slots, bool deposit) {
@@ -1274,7 +1432,7 @@ void push_all(Context& c, double dt, bool fuse_deposit, bool defer_ledger) {
             const bool timed = c.sort_policy == AURORA_SORT_ADAPTIVE;
             EventPair& ev = sp.push_ev[sp.push_ev_cur];
             if (timed) ev.record_start(c.stream);
-            if (launch_tile3d(c, sp, dt)) {
+            if (c.dim == 3 ? launch_tile3d(c, sp, dt) : launch_tile2d(c, sp, dt)) {
                 if (timed) {
                     ev.record_end(c.stream);
                     sp.push_ev_cur ^= 1;

@@ -206,10 +206,11 @@ int aurora_hip_boundary_losses(aurora_hip_ctx* ctx, int species_id,
 int aurora_hip_power_transfer(aurora_hip_ctx* ctx, int species_id, double* electric_work);
 /* Simulation time and step count kept by the context. */
 int aurora_hip_time(aurora_hip_ctx* ctx, double* time, int64_t* step);
-/* Tile shape (cells per axis) of the cell sort and of the LDS-tiled 3D
- * push, and its halo in cells; tx = 0 keeps the sort but disables the tiled
- * push.  Built shapes: 8x8x8 (halo 1, 2), 8x8x4 (halo 1, 2), 4x4x4,
- * 16x8x4 (halo 2).
+/* Tile shape (cells per axis) of the cell sort and of the LDS-tiled 2D and
+ * 3D pushes, and its halo in cells; tx = 0 keeps the sort but disables the
+ * tiled push.  Built 3D shapes: 8x8x8 (halo 1, 2), 8x8x4 (halo 1, 2),
+ * 4x4x4, 16x8x4 (halo 2); 2D (tz ignored; 1 names them): 8x8 (halo 1, 2;
+ * 512 threads, or 256 with prefetch), 16x16 and 16x8 (halo 2).
  * No reference counterpart (the reference keeps particles unsorted). */
 int aurora_hip_set_tiling(aurora_hip_ctx* ctx, int tx, int ty, int tz,
                           int halo);

@@ -511,6 +511,55 @@ def test_tiled_push_bit_identical_to_untiled(shape):
         assert np.array_equal(results[0][2], other[2])
 
 
+@pytest.mark.parametrize("shape", [(8, 8, 1, 2, 512, 1), (8, 8, 1, 2, 512, 0),
+                                   (8, 8, 1, 1, 512, 1), (8, 8, 1, 2, 256, 1),
+                                   (16, 16, 1, 2, 512, 1), (16, 8, 1, 2, 512, 1)])
+@pytest.mark.parametrize("bfield", ["none", "uniform", "table"])
+def test_tiled_push_2d_bit_identical_to_untiled(shape, bfield):
+    """The LDS-tiled 2D push performs the untiled k_push<2>'s operations:
+    particle state and fixed-point charge agree bit for bit with no field,
+    a uniform B and a tabulated B(x), across re-sorts (fixed interval and
+    the adaptive policy) and with particles leaving the tile halo."""
+    from aurora_pic import hip
+    print(f"case={shape} {bfield}", flush=True)
+    rng = np.random.default_rng(23)
+    n, L = [64, 48], [1.0, 0.75]
+    N = 64 * 48 * 8
+    pos = rng.uniform(0, 1, (N, 2)) * np.asarray(L)
+    vel = rng.normal(0, 0.5, (N, 3))
+    x = np.linspace(0, 1.0, 65)
+    bvals = np.stack([0.1 * np.ones(65), 3.0 * np.exp(-((x - 0.5) / 0.2) ** 2),
+                      1.5 * np.ones(65)], axis=1)
+    results = []
+    for tiled, policy in ((True, None), (True, 5), (False, None)):
+        with hip.Context(0) as c:
+            c.mesh(n, L, 0, 1.0)
+            if bfield == "uniform":
+                c.uniform_b([0.0, 0.0, 2.0])
+            elif bfield == "table":
+                c.b_profile(0, x, bvals)
+            if tiled:
+                c.tiling(*shape[:4])
+                c.push_variant(*shape[4:])
+            else:
+                c.tiling(0)
+            if policy is not None:
+                c.sort_policy(hip.SORT_ADAPTIVE, policy)
+            sid = c.add_species(-1.0, 1.0, 1.0 / N, N)
+            c.upload(sid, pos, vel, kind=1)
+            c.sort()
+            c.initialize(0.01)
+            c.sort_interval(7)
+            c.step(0.01, 16)
+            p_, v_, a_, u_ = c.download(sid)
+            o = np.argsort(u_)
+            results.append((c.charge_fixed(sid), p_[o], v_[o]))
+    for other in results[1:]:
+        assert np.array_equal(results[0][0], other[0])
+        assert np.array_equal(results[0][1], other[1])
+        assert np.array_equal(results[0][2], other[2])
+
+
 def test_full_size_3d_invariants(hip_ctx):
     """The 3D3V workload at full size (128^3 cells, 16 + 16 particles per
     cell): charge conservation of the fixed-point deposit, fused deposit equal
