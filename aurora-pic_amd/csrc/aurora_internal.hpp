@@ -87,6 +87,27 @@ struct DevBuf {
     }
 };
 
+// A small pinned host buffer (hipHostMalloc), for read-backs the host waits on.
+template <typename T>
+struct PinnedHost {
+    T* ptr = nullptr;
+    size_t count = 0;
+    void alloc(size_t n) {
+        release();
+        AUR_HIP(hipHostMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T), hipHostMallocDefault));
+        count = n;
+    }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    ~PinnedHost() { release(); }
+    PinnedHost() = default;
+    PinnedHost(const PinnedHost&) = delete;
+    PinnedHost& operator=(const PinnedHost&) = delete;
+};
+
 // A start/end event pair for timing one launch on the context stream (the
 // adaptive sort policy's measurements); created on first use.
 struct EventPair {
@@ -298,6 +319,9 @@ struct Context {
     DevBuf<int32_t> ls_new_start;
     DevBuf<unsigned> ls_far_key, ls_far_key2, ls_far_val, ls_far_val2;
     DevBuf<uint8_t> ls_stage;
+    // the product count of an MCC pass, read back through pinned memory
+    DevBuf<long long> product_total;
+    PinnedHost<long long> product_total_host;
 
     std::vector<Species> species;
     MccModelHost mcc;                          // null-collision model 0 of the step

@@ -120,6 +120,12 @@ __global__ void __launch_bounds__(kBlk) k_fold_rows(const double* rows, int64_t 
 }
 
 
+// products of all parents: the last exclusive offset plus the last count
+__global__ void k_product_total(const int32_t* offs, const int32_t* count, int64_t n,
+                                long long* total) {
+    total[0] = static_cast<long long>(offs[n - 1]) + count[n - 1];
+}
+
 // write the products of parent p at slots base + offset[p] + k
 __global__ void k_append_products(const int32_t* pcell, const float* poff,
                                   const int32_t* nprod, const int32_t* offset, const float* prod,
@@ -356,13 +362,15 @@ int64_t mcc_collide_pass(Context& c, int model, double dt) {
     if (c.sort_temp.count < temp || !c.sort_temp.ptr) c.sort_temp.alloc(temp > 0 ? temp : 1);
     AUR_HIP(rocprim::exclusive_scan(c.sort_temp.ptr, temp, h.product_count.ptr, offs, 0, n,
                                     rocprim::plus<int32_t>(), c.stream));
-    int32_t last_off = 0, last_cnt = 0;
-    AUR_HIP(hipMemcpyAsync(&last_off, offs + (n - 1), sizeof last_off, hipMemcpyDeviceToHost,
-                           c.stream));
-    AUR_HIP(hipMemcpyAsync(&last_cnt, h.product_count.ptr + (n - 1), sizeof last_cnt,
+    // the total on the device, one copy into pinned memory, one wait
+    if (!c.product_total.ptr) c.product_total.alloc(1);
+    if (!c.product_total_host.ptr) c.product_total_host.alloc(1);
+    k_product_total<<<1, 1, 0, c.stream>>>(offs, h.product_count.ptr, n, c.product_total.ptr);
+    AUR_HIP(hipGetLastError());
+    AUR_HIP(hipMemcpyAsync(c.product_total_host.ptr, c.product_total.ptr, sizeof(long long),
                            hipMemcpyDeviceToHost, c.stream));
     AUR_HIP(hipStreamSynchronize(c.stream));
-    return static_cast<int64_t>(last_off) + last_cnt;
+    return static_cast<int64_t>(c.product_total_host.ptr[0]);
 }
 
 // Places the `total` product pairs staged by one model's pass after the
