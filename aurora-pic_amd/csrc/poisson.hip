@@ -458,6 +458,37 @@ struct ClearArgs {
     int n;
 };
 
+// One 1D step's field solve from the deposit: the charge density from the
+// species' fixed-point sums, the next step's accumulators zeroed, then the
+// Dirichlet or the periodic solve -- one launch where the charge
+// conversion, the clears and the solve took one each.
+__device__ __forceinline__ void rho_from_acc_1d(const RhoArgs& ra, const ClearArgs& cl,
+                                                double* rho, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        rho[i] = rho_at(ra, i);
+        for (int k = 0; k < cl.n; ++k) cl.p[k][i] = 0;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_thomas_1d_acc(RhoArgs ra, ClearArgs cl, double* rho,
+                                                       double* phi, double* E, float* E32,
+                                                       double* s, int n, double dx, double eps,
+                                                       double phi_left, double phi_right) {
+    rho_from_acc_1d(ra, cl, rho, n);
+    dirichlet_solve_1d(rho, phi, E, E32, s, n, dx, eps, phi_left, phi_right);
+}
+
+__global__ void __launch_bounds__(1024) k_spectral_1d_acc(RhoArgs ra, ClearArgs cl, double* rho,
+                                                          double* phi, double* E, float* E32,
+                                                          const double* cs, const double* sn,
+                                                          int n, double L, double eps) {
+    extern __shared__ double sh[];
+    __shared__ double mean_s;
+    rho_from_acc_1d(ra, cl, rho, n);
+    spectral_solve_1d(rho, phi, E, E32, cs, sn, n, L, eps, sh, &mean_s);
+}
+
 template <int LOG2N>
 __global__ void __launch_bounds__(256) k_fwd_x_acc(RhoArgs ra, ClearArgs cl,
                                                    double2* __restrict__ out, int64_t nlines,
@@ -933,6 +964,36 @@ void solve_step_fields(Context& c, double phi_left, double phi_right) {
         c.rho_valid = false;
         c.phi_valid = false;
         c.e64_valid = e64;
+        return;
+    }
+    if (c.dim == 1) {
+        // the charge from the accumulators, the next ones zeroed, the solve
+        const RhoArgs ra = rho_args(c);
+        ClearArgs cl{};
+        cl.n = static_cast<int>(c.species.size());
+        for (int s = 0; s < cl.n; ++s) {
+            Species& sp = c.species[s];
+            if (!sp.acc_next.ptr) sp.acc_next.alloc(c.nnodes);
+            cl.p[s] = reinterpret_cast<long long*>(sp.acc_next.ptr);
+        }
+        const int n = static_cast<int>(c.n[0]);
+        if (c.bc == AURORA_PERIODIC) {
+            KernelScope ks(c, "solve_spectral_1d");
+            const size_t shmem = 3 * static_cast<size_t>(n) * sizeof(double);
+            k_spectral_1d_acc<<<1, 1024, shmem, c.stream>>>(ra, cl, c.rho.ptr, c.phi.ptr,
+                                                            c.E64.ptr, c.E32.ptr, c.dft_cos.ptr,
+                                                            c.dft_sin.ptr, n, c.L[0], c.eps);
+        } else {
+            KernelScope ks(c, "solve_thomas_1d");
+            k_thomas_1d_acc<<<1, 256, 0, c.stream>>>(ra, cl, c.rho.ptr, c.phi.ptr, c.E64.ptr,
+                                                     c.E32.ptr, c.thomas_inv.ptr, n, c.d[0],
+                                                     c.eps, phi_left, phi_right);
+        }
+        AUR_HIP(hipGetLastError());
+        for (auto& sp : c.species) sp.acc_next_clear = true;
+        c.rho_valid = true;
+        c.phi_valid = true;
+        c.e64_valid = true;
         return;
     }
     convert_rho(c);
