@@ -235,9 +235,9 @@ struct MccModelHost {
     DevBuf<double> energy_change;       // per channel, cumulative
     DevBuf<double> energy_rows;         // per wave of the last MCC launch, per channel
     DevBuf<int32_t> product_count;      // per target particle
-    DevBuf<int64_t> product_offset;
+    DevBuf<int32_t> product_loff;       // per target particle: products before it in its chunk
+    DevBuf<int32_t> chunk_tot, chunk_off;   // per 256-slot chunk: products, exclusive offset
     DevBuf<float> product_vel;          // secondaries: up to 2 per particle x 6 floats
-    int64_t product_slots = 0;
 };
 
 struct TimerEntry {
@@ -319,9 +319,10 @@ struct Context {
     DevBuf<int32_t> ls_new_start;
     DevBuf<unsigned> ls_far_key, ls_far_key2, ls_far_val, ls_far_val2;
     DevBuf<uint8_t> ls_stage;
-    // the product count of an MCC pass, read back through pinned memory
+    // the product count of each model's MCC pass, read back through pinned memory
     DevBuf<long long> product_total;
     PinnedHost<long long> product_total_host;
+    int64_t product_total_models = 0;
 
     std::vector<Species> species;
     MccModelHost mcc;                          // null-collision model 0 of the step

@@ -250,7 +250,8 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_
 // offsets, and the energy change goes to one row per wave (summed in a
 // fixed order by k_ens_products).
 template <int NCH>
-__global__ void __launch_bounds__(kEnsThreads, 4) k_ens_mcc(EnsArgs a, MccParams m) {
+__global__ void __launch_bounds__(kEnsThreads, mcc_min_waves(NCH))
+    k_ens_mcc(EnsArgs a, MccParams m) {
     __shared__ MccLds lds;
     __shared__ int scan_sh[kEnsThreads];
     const XsView X = stage_tables(m, lds);

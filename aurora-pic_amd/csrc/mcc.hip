@@ -19,7 +19,6 @@
 #include <string.h>
 #include <vector>
 
-#include <rocprim/device/device_scan.hpp>
 
 #include "mcc_device.hpp"
 #include "push_device.hpp"
@@ -33,10 +32,9 @@ constexpr int kBlk = 256;
 
 // standalone collide of count velocities (parity checks): uid = index
 template <int NCH>
-__global__ void __launch_bounds__(kBlk, 4) k_collide(MccParams m, float* v, int64_t count,
-                                                  float dt, uint64_t seed, uint32_t stream,
-                                                  uint32_t step, double* rows, uint32_t* err,
-                                                  int one_v) {
+__global__ void __launch_bounds__(kBlk, mcc_min_waves(NCH))
+    k_collide(MccParams m, float* v, int64_t count, float dt, uint64_t seed, uint32_t stream,
+              uint32_t step, double* rows, uint32_t* err, int one_v) {
     __shared__ MccLds lds;
     const XsView X = stage_tables(m, lds);
     WaveTally<NCH> w{};
@@ -82,33 +80,92 @@ struct Mcc1dArgs {
     int32_t* nprod;         // per particle ionization events (0..2)
     float* prod;            // per particle: 2 x (secondary v[3], ion v[3])
     uint32_t* err;
+    int32_t* loff;          // per particle: products before it in its 256-slot chunk
+    int32_t* chunk_tot;     // per chunk: its products (null: the model has none)
 };
 
+// Exclusive scan of one int per thread over a kBlk-thread block: the wave's
+// prefix by shuffles, the wave totals through LDS; sets total.
+__device__ __forceinline__ int block_scan_excl(int x, int* wsum, int& total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int incl = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+#pragma unroll
+    for (int k = 0; k < kBlk / 64; ++k) {
+        const int sk = wsum[k];
+        before += k < wid ? sk : 0;
+        total += sk;
+    }
+    __syncthreads();
+    return before + incl - x;
+}
+
 // sync -> collide -> rebuild v_half for every live target particle
+// (with products: each 256-slot chunk's event counts scanned in the block,
+// so a parent's product index is its chunk's offset plus its own prefix)
 template <int NCH>
 __global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, double* rows) {
     __shared__ MccLds lds;
+    __shared__ int wsum[kBlk / 64];
     const XsView X = stage_tables(m, lds);
     WaveTally<NCH> w{};
-    for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < a.slots;
-         base += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBlk; base < a.slots;
+         base += static_cast<int64_t>(gridDim.x) * kBlk) {
         const int64_t p = base + threadIdx.x;
-        bool active = p < a.slots;
-        int c = -1;
-        if (active) {
-            c = a.cell[p];
-            a.nprod[p] = 0;
-            active = c >= 0;
-        }
+        const bool in = p < a.slots;
+        const int c = in ? a.cell[p] : -1;
+        const bool active = c >= 0;
         MccTally<NCH> t{};
+        int np = 0;
         if (active)
-            a.nprod[p] = mcc_1d_particle<NCH>(m, X, lds.chan, c, a.off[p], a.v0 + p, a.v1 + p,
-                                              a.v2 + p, a.uid[p], a.E32, a.nx, a.half_qm_dt,
-                                              a.dt, a.seed, a.step, a.prod + 12 * p, t, a.err,
-                                              a.stream, a.one_v != 0);
+            np = mcc_1d_particle<NCH>(m, X, lds.chan, c, a.off[p], a.v0 + p, a.v1 + p, a.v2 + p,
+                                      a.uid[p], a.E32, a.nx, a.half_qm_dt, a.dt, a.seed, a.step,
+                                      a.prod + 12 * p, t, a.err, a.stream, a.one_v != 0);
+        if (in) a.nprod[p] = np;
         flush_tally_rows<NCH>(t, active, w);
+        if (a.chunk_tot) {
+            int total = 0;
+            const int ex = block_scan_excl(np, wsum, total);
+            if (in) a.loff[p] = ex;
+            if (threadIdx.x == 0) a.chunk_tot[base / kBlk] = total;
+        }
     }
     store_wave_row<NCH>(w, rows);
+}
+
+// After one model's k_mcc_1d: its tally rows added in row order (as
+// k_fold_rows), then, with products, the chunk totals scanned into chunk
+// offsets and the model's product count written to *total.
+template <int NCH>
+__global__ void __launch_bounds__(kBlk) k_mcc_finish(const double* rows, int64_t nrows,
+                                                     double* energy, unsigned long long* counts,
+                                                     const int32_t* chunk_tot, int32_t* chunk_off,
+                                                     int64_t nchunks, long long* total) {
+    fold_tally_rows<NCH, kBlk>(rows, nrows, energy, counts);
+    if (!chunk_tot) {
+        if (threadIdx.x == 0) *total = 0;
+        return;
+    }
+    __shared__ int wsum[kBlk / 64];
+    const int64_t per = (nchunks + kBlk - 1) / kBlk;
+    const int64_t c0 = threadIdx.x * per;
+    int mine = 0;
+    for (int64_t k = 0; k < per && c0 + k < nchunks; ++k) mine += chunk_tot[c0 + k];
+    int all = 0;
+    int run = block_scan_excl(mine, wsum, all);
+    for (int64_t k = 0; k < per && c0 + k < nchunks; ++k) {
+        chunk_off[c0 + k] = run;
+        run += chunk_tot[c0 + k];
+    }
+    if (threadIdx.x == 0) *total = all;
 }
 
 // Adds the tally rows of one launch, in row order, to the energy change
@@ -120,22 +177,23 @@ __global__ void __launch_bounds__(kBlk) k_fold_rows(const double* rows, int64_t 
 }
 
 
-// products of all parents: the last exclusive offset plus the last count
-__global__ void k_product_total(const int32_t* offs, const int32_t* count, int64_t n,
-                                long long* total) {
-    total[0] = static_cast<long long>(offs[n - 1]) + count[n - 1];
-}
+__global__ void k_zero_total(long long* total) { *total = 0; }
 
-// write the products of parent p at slots base + offset[p] + k
-__global__ void k_append_products(const int32_t* pcell, const float* poff,
-                                  const int32_t* nprod, const int32_t* offset, const float* prod,
-                                  int64_t parents, const float* E32, int nx, AppendSide sec,
-                                  AppendSide ion) {
-    for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < parents;
-         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+// write the products of parent p at product index chunk_off[p / kBlk] +
+// loff[p] + k (grid-stride over 256-slot chunks)
+__global__ void __launch_bounds__(kBlk) k_append_products(
+    const int32_t* pcell, const float* poff, const int32_t* nprod, const int32_t* loff,
+    const int32_t* chunk_off, const float* prod, int64_t parents, const float* E32, int nx,
+    AppendSide sec, AppendSide ion) {
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBlk; base < parents;
+         base += static_cast<int64_t>(gridDim.x) * kBlk) {
+        const int64_t p = base + threadIdx.x;
+        if (p >= parents) continue;
         const int np = nprod[p];
         if (np == 0) continue;
-        append_products_of(pcell[p], poff[p], np, offset[p], prod + 12 * p, E32, nx, sec, ion);
+        append_products_of(pcell[p], poff[p], np, static_cast<int64_t>(chunk_off[base / kBlk]) +
+                                                      loff[p],
+                           prod + 12 * p, E32, nx, sec, ion);
     }
 }
 
@@ -305,19 +363,31 @@ MccParams mcc_step_params(const Context& c, int model) {
 namespace {
 
 // One model's collide pass over its target species (Simulation.cpp:835-927):
-// synchronise, collide, rebuild v_half, stage the products of each parent;
-// returns the number of ionization events (0 without products).
-int64_t mcc_collide_pass(Context& c, int model, double dt) {
+// synchronise, collide, rebuild v_half, stage the products of each parent
+// and write the model's product count to product_total[model] on the
+// device (0 without products).
+void mcc_collide_pass(Context& c, int model, double dt) {
     MccModelHost& h = c.mcc_model(model);
     Species& sp = c.species[h.target];
     AUR_REQUIRE(sp.precision == 32, AURORA_ERR_UNSUPPORTED, "MCC targets FP32 species");
     const int64_t n = sp.slots;
-    if (n == 0) return 0;
+    long long* total = c.product_total.ptr + model;
+    if (n == 0) {
+        k_zero_total<<<1, 1, 0, c.stream>>>(total);
+        AUR_HIP(hipGetLastError());
+        return;
+    }
+    const int64_t nchunks = (n + kBlk - 1) / kBlk;
     if (h.product_count.count < static_cast<size_t>(n)) {
         h.product_count.alloc(n);
-        h.product_offset.alloc(n + 1);
+        h.product_loff.alloc(n);
         h.product_vel.alloc(static_cast<size_t>(n) * 12);
     }
+    if (h.chunk_tot.count < static_cast<size_t>(nchunks)) {
+        h.chunk_tot.alloc(nchunks);
+        h.chunk_off.alloc(nchunks);
+    }
+    const bool products = h.secondary_species >= 0;
     const MccParams m = mcc_step_params(c, model);
     Mcc1dArgs a{};
     a.cell = sp.cell.ptr;
@@ -338,39 +408,22 @@ int64_t mcc_collide_pass(Context& c, int model, double dt) {
     a.nprod = h.product_count.ptr;
     a.prod = h.product_vel.ptr;
     a.err = c.dev_error.ptr;
-    {
-        KernelScope ks(c, "mcc");
-        // one round of resident workgroups (grid-stride): fewer tally rows
-        // for the one-block fold that follows
-        const int g = std::min(blocks(c, n), 4 * c.cu_count);
-        const int64_t rows = energy_rows(g);
-        if (h.energy_rows.count < static_cast<size_t>(rows * kTallyStride))
-            h.energy_rows.alloc(static_cast<size_t>(rows * kTallyStride));
-        with_nchan(m.nchan, [&]<int NCH>() {
-            k_mcc_1d<NCH><<<g, kBlk, 0, c.stream>>>(m, a, h.energy_rows.ptr);
-            k_fold_rows<NCH><<<1, kBlk, 0, c.stream>>>(h.energy_rows.ptr, rows,
-                                                       h.energy_change.ptr, h.counts.ptr);
-        });
-        AUR_HIP(hipGetLastError());
-    }
-    if (h.secondary_species < 0) return 0;
-    // exclusive prefix sum of event counts -> product slots
-    int32_t* offs = reinterpret_cast<int32_t*>(h.product_offset.ptr);
-    size_t temp = 0;
-    AUR_HIP(rocprim::exclusive_scan(nullptr, temp, h.product_count.ptr, offs, 0, n + 0,
-                                    rocprim::plus<int32_t>(), c.stream));
-    if (c.sort_temp.count < temp || !c.sort_temp.ptr) c.sort_temp.alloc(temp > 0 ? temp : 1);
-    AUR_HIP(rocprim::exclusive_scan(c.sort_temp.ptr, temp, h.product_count.ptr, offs, 0, n,
-                                    rocprim::plus<int32_t>(), c.stream));
-    // the total on the device, one copy into pinned memory, one wait
-    if (!c.product_total.ptr) c.product_total.alloc(1);
-    if (!c.product_total_host.ptr) c.product_total_host.alloc(1);
-    k_product_total<<<1, 1, 0, c.stream>>>(offs, h.product_count.ptr, n, c.product_total.ptr);
+    a.loff = products ? h.product_loff.ptr : nullptr;
+    a.chunk_tot = products ? h.chunk_tot.ptr : nullptr;
+    KernelScope ks(c, "mcc");
+    // one round of resident workgroups (grid-stride): fewer tally rows for
+    // the one-block fold that follows
+    const int g = std::min(blocks(c, n), 4 * c.cu_count);
+    const int64_t rows = energy_rows(g);
+    if (h.energy_rows.count < static_cast<size_t>(rows * kTallyStride))
+        h.energy_rows.alloc(static_cast<size_t>(rows * kTallyStride));
+    with_nchan(m.nchan, [&]<int NCH>() {
+        k_mcc_1d<NCH><<<g, kBlk, 0, c.stream>>>(m, a, h.energy_rows.ptr);
+        k_mcc_finish<NCH><<<1, kBlk, 0, c.stream>>>(h.energy_rows.ptr, rows,
+                                                    h.energy_change.ptr, h.counts.ptr,
+                                                    a.chunk_tot, h.chunk_off.ptr, nchunks, total);
+    });
     AUR_HIP(hipGetLastError());
-    AUR_HIP(hipMemcpyAsync(c.product_total_host.ptr, c.product_total.ptr, sizeof(long long),
-                           hipMemcpyDeviceToHost, c.stream));
-    AUR_HIP(hipStreamSynchronize(c.stream));
-    return static_cast<int64_t>(c.product_total_host.ptr[0]);
 }
 
 // Places the `total` product pairs staged by one model's pass after the
@@ -379,7 +432,6 @@ void mcc_append(Context& c, int model, int64_t total, double dt) {
     MccModelHost& h = c.mcc_model(model);
     Species& sp = c.species[h.target];
     const int64_t n = sp.slots;
-    const int32_t* offs = reinterpret_cast<const int32_t*>(h.product_offset.ptr);
     Species& se = c.species[h.secondary_species];
     Species& si = c.species[h.ion_species];
     auto side = [&](Species& s, int64_t base_slot) {
@@ -405,9 +457,10 @@ void mcc_append(Context& c, int model, int64_t total, double dt) {
     }
     {
         KernelScope ks(c, "append_products");
-        k_append_products<<<blocks(c, n), kBlk, 0, c.stream>>>(
+        k_append_products<<<std::min(blocks(c, n), 4 * c.cu_count), kBlk, 0, c.stream>>>(
             sp.cell.ptr, reinterpret_cast<const float*>(sp.off[0].ptr), h.product_count.ptr,
-            offs, h.product_vel.ptr, n, c.E32.ptr, static_cast<int>(c.n[0]), sec, ion);
+            h.product_loff.ptr, h.chunk_off.ptr, h.product_vel.ptr, n, c.E32.ptr,
+            static_cast<int>(c.n[0]), sec, ion);
         AUR_HIP(hipGetLastError());
     }
     if (h.secondary_species == h.ion_species) {
@@ -449,8 +502,25 @@ void mcc_step(Context& c, double dt) {
                             AURORA_ERR_INVALID_ARGUMENT,
                             "ionization and anisotropic MCC require the 3V collision interface");
         }
+    if (c.product_total_models < models) {
+        c.product_total.alloc(static_cast<size_t>(models));
+        c.product_total_host.alloc(static_cast<size_t>(models));
+        c.product_total_models = models;
+    }
+    bool products = false;
+    for (int k = 0; k < models; ++k) {
+        mcc_collide_pass(c, k, dt);
+        products = products || c.mcc_model(k).secondary_species >= 0;
+    }
+    if (!products) return;
+    // every model's product count in one copy into pinned memory, one wait
+    AUR_HIP(hipMemcpyAsync(c.product_total_host.ptr, c.product_total.ptr,
+                           static_cast<size_t>(models) * sizeof(long long), hipMemcpyDeviceToHost,
+                           c.stream));
+    AUR_HIP(hipStreamSynchronize(c.stream));
     std::vector<int64_t> total(static_cast<size_t>(models), 0);
-    for (int k = 0; k < models; ++k) total[static_cast<size_t>(k)] = mcc_collide_pass(c, k, dt);
+    for (int k = 0; k < models; ++k)
+        total[static_cast<size_t>(k)] = static_cast<int64_t>(c.product_total_host.ptr[k]);
     std::vector<int64_t> need(c.species.size(), 0);
     for (int k = 0; k < models; ++k) {
         const MccModelHost& h = c.mcc_model(k);

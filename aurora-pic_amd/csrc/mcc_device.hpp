@@ -600,6 +600,11 @@ __device__ void flush_tally(const MccTally<NCH>& t, TallyOut out, bool active) {
 // contend for one counter.
 constexpr int kTallyStride = 2 * kMaxChannels + 3;
 
+// Waves per SIMD that the MCC kernels over particles ask of the register
+// allocator: four (at most 128 VGPRs) for three channels, where that fits
+// without spilling; no cap for other channel counts, which would spill.
+constexpr int mcc_min_waves(int nch) { return nch == 3 ? 4 : 1; }
+
 template <int NCH>
 struct WaveTally {
     double energy[NCH];
@@ -645,26 +650,43 @@ __device__ __forceinline__ void store_wave_row(const WaveTally<NCH>& w, double* 
 
 // Adds rows [0, nrows) in row order: energies into energy[NCH], counts into
 // counts[NCH + 3].  Block of BLOCK threads; every thread calls it.
+// A thread's rows r, r + BLOCK, ... are loaded kFoldBatch at a time (the
+// loads in flight together) and added in that same order.
 template <int NCH, int BLOCK>
 __device__ void fold_tally_rows(const double* rows, int64_t nrows, double* energy,
                                 unsigned long long* counts) {
+    constexpr int kFoldBatch = NCH <= 4 ? 8 : 4;
     double v[NCH] = {};
-    for (int64_t r = threadIdx.x; r < nrows; r += BLOCK)
+    unsigned long long u[NCH + 3] = {};
+    for (int64_t r0 = threadIdx.x; r0 < nrows; r0 += kFoldBatch * BLOCK) {
+        double e[kFoldBatch][NCH];
+        unsigned long long n[kFoldBatch][NCH + 3];
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) v[c] += rows[r * kTallyStride + c];
+        for (int b = 0; b < kFoldBatch; ++b) {
+            const int64_t r = r0 + static_cast<int64_t>(b) * BLOCK;
+            const double* row = rows + (r < nrows ? r : 0) * kTallyStride;
+            const unsigned long long* cnt =
+                reinterpret_cast<const unsigned long long*>(row + kMaxChannels);
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) e[b][c] = row[c];
+#pragma unroll
+            for (int k = 0; k < NCH + 3; ++k) n[b][k] = cnt[k];
+        }
+#pragma unroll
+        for (int b = 0; b < kFoldBatch; ++b) {
+            if (r0 + static_cast<int64_t>(b) * BLOCK >= nrows) break;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) v[c] += e[b][c];
+#pragma unroll
+            for (int k = 0; k < NCH + 3; ++k) u[k] += n[b][k];
+        }
+    }
     const double s = block_sum<NCH, BLOCK>(v);
     if (threadIdx.x < NCH) energy[threadIdx.x] += s;
     // counts: integer sums, any order gives the same value
     __shared__ unsigned long long cs[NCH + 3];
     if (threadIdx.x < NCH + 3) cs[threadIdx.x] = 0;
     __syncthreads();
-    unsigned long long u[NCH + 3] = {};
-    for (int64_t r = threadIdx.x; r < nrows; r += BLOCK) {
-        const unsigned long long* n =
-            reinterpret_cast<const unsigned long long*>(rows + r * kTallyStride + kMaxChannels);
-#pragma unroll
-        for (int k = 0; k < NCH + 3; ++k) u[k] += n[k];
-    }
 #pragma unroll
     for (int k = 0; k < NCH + 3; ++k) {
         const unsigned long long x = wave_sum_u64(u[k]);
