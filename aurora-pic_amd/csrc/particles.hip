@@ -473,13 +473,15 @@ __global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
 
 // 1D survivors after the field solve at t+dt (Simulation.cpp:1094-1131):
 // v_x synchronised with E^{n+1}, and the work 0.5 m w (v^2 - v_old^2)
-// since the start of the step.  Per-block sums into part[blockIdx.x].
+// since the start of the step.  Block b of nb sums its particles into
+// part[b].
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_work_1d(PushArgs<R> a, double* part) {
+__device__ __forceinline__ void work_1d_blocks(const PushArgs<R>& a, double* part, int b,
+                                               int nb) {
     const SpeciesView<R>& sp = a.sp;
     double w[1] = {0.0};
-    for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < sp.slots;
-         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    for (int64_t p = b * static_cast<int64_t>(kBlock) + threadIdx.x; p < sp.slots;
+         p += static_cast<int64_t>(nb) * kBlock) {
         const int c = sp.cell[p];
         if (c < 0) continue;
         int idx[3] = {c, 0, 0};
@@ -492,24 +494,52 @@ __global__ void __launch_bounds__(kBlock) k_work_1d(PushArgs<R> a, double* part)
         w[0] += a.half_mw * (vn * vn - vo * vo);
     }
     const double s = block_sum<1, kBlock>(w);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (threadIdx.x == 0) part[b] = s;
 }
 
-// Folds a species' ledger partials in block order into its totals: push
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_work_1d(PushArgs<R> a, double* part) {
+    work_1d_blocks(a, part, blockIdx.x, gridDim.x);
+}
+
+// Two species of one precision in one launch: blocks [0, split) the first,
+// the rest the second (each species' partials as in k_work_1d).
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_work_1d_pair(PushArgs<R> a0, double* part0,
+                                                         PushArgs<R> a1, double* part1,
+                                                         int split) {
+    if (static_cast<int>(blockIdx.x) < split)
+        work_1d_blocks(a0, part0, blockIdx.x, split);
+    else
+        work_1d_blocks(a1, part1, blockIdx.x - split, gridDim.x - split);
+}
+
+// A species' ledger partials, folded in block order into its totals: push
 // partials (3 per block) and, when present, work partials (1 per block).
-__global__ void __launch_bounds__(kBlock) k_ledger_1d(const double* push_part, int push_blocks,
-                                                      const double* work_part, int work_blocks,
-                                                      double* ledger) {
+struct LedgerFold {
+    const double* push_part;
+    int push_blocks;
+    const double* work_part;
+    int work_blocks;
+    double* ledger;
+};
+
+__device__ __forceinline__ void fold_ledger(const LedgerFold& f) {
     double v[3] = {0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < push_blocks; b += kBlock) {
-        v[0] += push_part[3 * b];
-        v[1] += push_part[3 * b + 1];
-        v[2] += push_part[3 * b + 2];
+    for (int b = threadIdx.x; b < f.push_blocks; b += kBlock) {
+        v[0] += f.push_part[3 * b];
+        v[1] += f.push_part[3 * b + 1];
+        v[2] += f.push_part[3 * b + 2];
     }
-    if (work_part)
-        for (int b = threadIdx.x; b < work_blocks; b += kBlock) v[2] += work_part[b];
+    if (f.work_part)
+        for (int b = threadIdx.x; b < f.work_blocks; b += kBlock) v[2] += f.work_part[b];
     const double s = block_sum<3, kBlock>(v);
-    if (threadIdx.x < 3) ledger[threadIdx.x] += s;
+    if (threadIdx.x < 3) f.ledger[threadIdx.x] += s;
+}
+
+// one workgroup per species: block 0 folds f0, block 1 (when launched) f1
+__global__ void __launch_bounds__(kBlock) k_ledger_1d(LedgerFold f0, LedgerFold f1) {
+    fold_ledger(blockIdx.x == 0 ? f0 : f1);
 }
 
 // one atomic per workgroup (a few per CU): per-wave atomics on one address
@@ -1364,10 +1394,15 @@ void ensure_ledger_1d(Context& c, Species& sp) {
     if (!sp.vold.ptr) sp.vold.alloc(static_cast<size_t>(sp.capacity > 0 ? sp.capacity : 1));
 }
 
+LedgerFold ledger_fold(Species& sp, const double* work_part, int work_blocks) {
+    return LedgerFold{sp.ledger_part.ptr, sp.ledger_push_blocks, work_part, work_blocks,
+                      sp.ledger.ptr};
+}
+
 void fold_ledger_1d(Context& c, Species& sp, const double* work_part, int work_blocks) {
     KernelScope ks(c, "ledger");
-    k_ledger_1d<<<1, kBlock, 0, c.stream>>>(sp.ledger_part.ptr, sp.ledger_push_blocks, work_part,
-                                            work_blocks, sp.ledger.ptr);
+    const LedgerFold f = ledger_fold(sp, work_part, work_blocks);
+    k_ledger_1d<<<1, kBlock, 0, c.stream>>>(f, f);
     AUR_HIP(hipGetLastError());
     sp.ledger_pending = false;
 }
@@ -1376,6 +1411,33 @@ void fold_ledger_1d(Context& c, Species& sp, const double* work_part, int work_b
 
 void ledger_work_1d(Context& c, double dt) {
     if (c.dim != 1) return;
+    // the usual pair (electrons and ions of one precision): both species'
+    // work in one launch, both folds in another
+    std::vector<Species*> pending;
+    for (auto& sp : c.species)
+        if (sp.ledger_pending) pending.push_back(&sp);
+    if (pending.size() == 2 && pending[0]->precision == pending[1]->precision) {
+        Species& s0 = *pending[0];
+        Species& s1 = *pending[1];
+        const int g0 = launch_blocks(c, s0.slots), g1 = launch_blocks(c, s1.slots);
+        double* part0 = s0.ledger_part.ptr + 3 * static_cast<int64_t>(c.cu_count) * 32;
+        double* part1 = s1.ledger_part.ptr + 3 * static_cast<int64_t>(c.cu_count) * 32;
+        auto launch = [&]<typename R>() {
+            KernelScope ks(c, "work");
+            k_work_1d_pair<R><<<g0 + g1, kBlock, 0, c.stream>>>(
+                push_args<R>(c, s0, dt), part0, push_args<R>(c, s1, dt), part1, g0);
+            AUR_HIP(hipGetLastError());
+        };
+        if (s0.precision == 64) launch.template operator()<double>();
+        else launch.template operator()<float>();
+        KernelScope ks(c, "ledger");
+        k_ledger_1d<<<2, kBlock, 0, c.stream>>>(ledger_fold(s0, part0, g0),
+                                                ledger_fold(s1, part1, g1));
+        AUR_HIP(hipGetLastError());
+        s0.ledger_pending = false;
+        s1.ledger_pending = false;
+        return;
+    }
     for (auto& sp : c.species) {
         if (!sp.ledger_pending) continue;
         const int g = launch_blocks(c, sp.slots);
