@@ -373,22 +373,24 @@ __global__ void __launch_bounds__(kBlock) k_deposit(SpeciesView<R> sp, MeshView 
 // (a 1D mesh has few nodes, so global atomics would all land on them)
 constexpr int kLds1dNodes = 4096;
 
+// Block b of the nb pushing one species (lacc: the block's LDS charge sums
+// on small 1D meshes, else null).
 template <int DIM, typename R, int BMODE, bool DEPOSIT>
-__global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
+__device__ __forceinline__ void push_blocks_of(const PushArgs<R>& a, int b, int nb,
+                                               long long* lacc) {
     const SpeciesView<R>& sp = a.sp;
     double led[3] = {0.0, 0.0, 0.0};   // 1D: left / right wall energy, electric work
     constexpr bool kLds = DIM == 1 && DEPOSIT;
     long long* acc = a.acc;
     if constexpr (kLds) {
-        __shared__ long long lacc[kLds1dNodes];
         if (a.m.nx <= kLds1dNodes) {
-            for (int i = threadIdx.x; i < a.m.nx; i += blockDim.x) lacc[i] = 0;
+            for (int i = threadIdx.x; i < a.m.nx; i += kBlock) lacc[i] = 0;
             __syncthreads();
             acc = lacc;
         }
     }
-    for (int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; p < sp.slots;
-         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    for (int64_t p = b * static_cast<int64_t>(kBlock) + threadIdx.x; p < sp.slots;
+         p += static_cast<int64_t>(nb) * kBlock) {
         const int c = sp.cell[p];
         if (c < 0) continue;
         int idx[3] = {0, 0, 0};
@@ -459,7 +461,7 @@ __global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
         if (acc != a.acc) {
             // the block's integer sums into the species accumulator
             __syncthreads();
-            for (int i = threadIdx.x; i < a.m.nx; i += blockDim.x)
+            for (int i = threadIdx.x; i < a.m.nx; i += kBlock)
                 if (acc[i] != 0)
                     atomicAdd(reinterpret_cast<unsigned long long*>(a.acc + i),
                               static_cast<unsigned long long>(acc[i]));
@@ -467,9 +469,33 @@ __global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
     }
     if constexpr (DIM == 1) {
         const double s = block_sum<3, kBlock>(led);
-        if (threadIdx.x < 3) a.ledger_part[3 * blockIdx.x + threadIdx.x] = s;
+        if (threadIdx.x < 3) a.ledger_part[3 * b + threadIdx.x] = s;
     }
 }
+
+template <int DIM, typename R, int BMODE, bool DEPOSIT>
+__global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
+    if constexpr (DIM == 1 && DEPOSIT) {
+        __shared__ long long lacc[kLds1dNodes];
+        push_blocks_of<DIM, R, BMODE, DEPOSIT>(a, blockIdx.x, gridDim.x, lacc);
+    } else {
+        push_blocks_of<DIM, R, BMODE, DEPOSIT>(a, blockIdx.x, gridDim.x, nullptr);
+    }
+}
+
+// Two 1D species of one precision pushed (with their deposits) in one
+// launch: blocks [0, split) the first, the rest the second, each species'
+// blocks exactly those of its own k_push launch.
+template <typename R, int BMODE>
+__global__ void __launch_bounds__(kBlock) k_push_pair_1d(PushArgs<R> a0, PushArgs<R> a1,
+                                                         int split) {
+    __shared__ long long lacc[kLds1dNodes];
+    if (static_cast<int>(blockIdx.x) < split)
+        push_blocks_of<1, R, BMODE, true>(a0, blockIdx.x, split, lacc);
+    else
+        push_blocks_of<1, R, BMODE, true>(a1, blockIdx.x - split, gridDim.x - split, lacc);
+}
+
 
 // 1D survivors after the field solve at t+dt (Simulation.cpp:1094-1131):
 // v_x synchronised with E^{n+1}, and the work 0.5 m w (v^2 - v_old^2)
@@ -1225,6 +1251,23 @@ void launch_push_b(Context& c, Species& sp, double dt, bool deposit) {
     AUR_HIP(hipGetLastError());
 }
 
+template <typename R, int BMODE>
+void launch_push_pair_1d_b(Context& c, Species& s0, Species& s1, double dt) {
+    const int g0 = push_blocks(c, s0.slots, true), g1 = push_blocks(c, s1.slots, true);
+    KernelScope ks(c, "push_deposit");
+    k_push_pair_1d<R, BMODE><<<g0 + g1, kBlock, 0, c.stream>>>(push_args<R>(c, s0, dt),
+                                                               push_args<R>(c, s1, dt), g0);
+    AUR_HIP(hipGetLastError());
+}
+
+template <typename R>
+void launch_push_pair_1d(Context& c, Species& s0, Species& s1, double dt) {
+    const int mode = b_view(c).mode;
+    if (mode == 0) launch_push_pair_1d_b<R, 0>(c, s0, s1, dt);
+    else if (mode == 1) launch_push_pair_1d_b<R, 1>(c, s0, s1, dt);
+    else launch_push_pair_1d_b<R, 2>(c, s0, s1, dt);
+}
+
 template <int DIM, typename R>
 void launch_push(Context& c, Species& sp, double dt, bool deposit) {
     const int mode = b_view(c).mode;
@@ -1481,6 +1524,24 @@ void push_all(Context& c, double dt, bool fuse_deposit, bool defer_ledger) {
             k_fill_i64<<<launch_blocks(c, c.nnodes), kBlock, 0, c.stream>>>(
                 reinterpret_cast<long long*>(sp.acc.ptr), c.nnodes);
         }
+    }
+    if (c.dim == 1 && fuse_deposit && c.species.size() == 2 && c.species[0].slots > 0 &&
+        c.species[1].slots > 0 && c.species[0].precision == c.species[1].precision) {
+        // the usual pair (electrons and ions): one launch
+        Species& s0 = c.species[0];
+        Species& s1 = c.species[1];
+        for (Species* sp : {&s0, &s1}) {
+            ensure_ledger_1d(c, *sp);
+            if (sp->ledger_pending) fold_ledger_1d(c, *sp, nullptr, 0);
+        }
+        if (s0.precision == 64) launch_push_pair_1d<double>(c, s0, s1, dt);
+        else launch_push_pair_1d<float>(c, s0, s1, dt);
+        for (Species* sp : {&s0, &s1}) {
+            sp->ledger_push_blocks = push_blocks(c, sp->slots, true);
+            sp->ledger_pending = true;
+            if (!defer_ledger) fold_ledger_1d(c, *sp, nullptr, 0);
+        }
+        return;
     }
     for (auto& sp : c.species) {
         if (sp.slots == 0) continue;
