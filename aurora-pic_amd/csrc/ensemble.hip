@@ -681,7 +681,7 @@ CompactArgs compact_args(Ensemble& e, int s) {
 
 // Queue one step (or the compaction) of every instance on the stream.
 void enqueue_step(Context& c, Ensemble& e, const EnsArgs& a, const MccParams& m) {
-    const size_t shmem = (a.periodic ? 3 * static_cast<size_t>(a.nx) : 0) * sizeof(double);
+    const size_t shmem = a.periodic ? spectral_1d_lds(a.nx) : 0;
     k_ens_push<true><<<push_grid(a), kEnsThreads, 0, c.stream>>>(a);
     k_ens_field<<<a.M, kEnsThreads, shmem, c.stream>>>(a, 1);
     if (a.target >= 0) {

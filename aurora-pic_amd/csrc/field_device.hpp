@@ -65,62 +65,133 @@ __device__ __forceinline__ double rho_at_strided(const RhoArgs& a, const long lo
     return r;
 }
 
-// 1D periodic: direct spectral sum (FieldSolver.cpp:407-440) by one block;
-// sh holds 3 n doubles of LDS, mean_s one.
+// 1D periodic: the direct spectral sums (FieldSolver.cpp:407-440) by one
+// block.  Each output (a mode k, then a node j) is summed by a group of
+// kSpecLanes lanes, lane l taking every kSpecLanes-th term from its own on,
+// and the group's partial sums are combined by a butterfly: the order of the
+// additions depends on n alone, not on the block size (a multiple of
+// kSpecLanes).  sh holds spectral_1d_lds(n) bytes of LDS, mean_s one: on
+// meshes up to kSpecLdsTables nodes the cos/sin tables are staged there
+// too (a lane's table loads then wait on LDS, not on the cache hierarchy).
+constexpr int kSpecLanes = 8;
+constexpr int kSpecLdsTables = 1024;
+
+__host__ __device__ constexpr size_t spectral_1d_lds(int n) {
+    return static_cast<size_t>(n <= kSpecLdsTables ? 5 : 3) * static_cast<size_t>(n) *
+           sizeof(double);
+}
+
+template <typename T>
+__device__ __forceinline__ T spec_group_sum(T v) {
+#pragma unroll
+    for (int o = kSpecLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kSpecLanes);
+    return v;
+}
+
+// the forward and inverse sums of spectral_solve_1d with the tables at cs, sn
+__device__ __forceinline__ void spectral_sums_1d(double* x, double* pr, double* pim,
+                                                 const double* cs, const double* sn, double* phi,
+                                                 double* E, float* E32, int n, double L,
+                                                 double eps) {
+#pragma clang fp contract(off)
+    const double twopi = 6.283185307179586476925286766559;
+    const int lane = static_cast<int>(threadIdx.x) % kSpecLanes;
+    const int group = static_cast<int>(threadIdx.x) / kSpecLanes;
+    const int ngroups = static_cast<int>(blockDim.x) / kSpecLanes;
+    // forward: mode k from x_j cos/sin(2 pi j k / n)
+    for (int k0 = 0; k0 < n; k0 += ngroups) {
+        const int k = k0 + group;
+        double rr = 0.0, ri = 0.0;
+        if (k < n) {
+            int m = (lane * k) % n;
+            const int step = (kSpecLanes * k) % n;
+#pragma unroll 4
+            for (int j = lane; j < n; j += kSpecLanes) {
+                rr += x[j] * cs[m];
+                ri -= x[j] * sn[m];
+                m += step;
+                if (m >= n) m -= n;
+            }
+        }
+        rr = spec_group_sum(rr);
+        ri = spec_group_sum(ri);
+        if (k < n && lane == 0) {
+            if (k == 0) {
+                pr[0] = 0.0;
+                pim[0] = 0.0;
+            } else {
+                rr /= static_cast<double>(n);
+                ri /= static_cast<double>(n);
+                const double kk = (k <= n / 2) ? static_cast<double>(k) : -static_cast<double>(n - k);
+                const double wave = twopi * kk / L;
+                const double den = eps * wave * wave;
+                pr[k] = rr / den;
+                pim[k] = ri / den;
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const double kk = (k <= n / 2) ? static_cast<double>(k) : -static_cast<double>(n - k);
+        x[k] = twopi * kk / L;
+    }
+    __syncthreads();
+    // inverse: phi and E_k = -i wave phi_k = (wave pim, -wave pr) at node j
+    for (int j0 = 0; j0 < n; j0 += ngroups) {
+        const int j = j0 + group;
+        double ph = 0.0, e = 0.0;
+        if (j < n) {
+            int m = (j * (1 + lane)) % n;
+            const int step = (j * kSpecLanes) % n;
+#pragma unroll 4
+            for (int k = 1 + lane; k < n; k += kSpecLanes) {
+                const double c = cs[m], s = sn[m];
+                const double wave = x[k];
+                ph += pr[k] * c - pim[k] * s;
+                e += (wave * pim[k]) * c - (-wave * pr[k]) * s;
+                m += step;
+                if (m >= n) m -= n;
+            }
+        }
+        ph = spec_group_sum(ph);
+        e = spec_group_sum(e);
+        if (j < n && lane == 0) {
+            phi[j] = ph;
+            E[j] = e;
+            E32[j] = static_cast<float>(e);
+        }
+    }
+}
+
 __device__ inline void spectral_solve_1d(const double* rho, double* phi, double* E, float* E32,
                                   const double* cs, const double* sn, int n, double L, double eps,
                                   double* sh, double* mean_s) {
 #pragma clang fp contract(off)
-    double* r = sh;              // n
+    double* x = sh;              // n: rho - mean, then the wave number of each mode
     double* pr = sh + n;         // phi_k re
     double* pim = sh + 2 * n;    // phi_k im
-    for (int j = threadIdx.x; j < n; j += blockDim.x) r[j] = rho[j];
+    const bool lds_tables = n <= kSpecLdsTables;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        x[j] = rho[j];
+        if (lds_tables) {
+            sh[3 * n + j] = cs[j];
+            sh[4 * n + j] = sn[j];
+        }
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = 0.0;
-        for (int j = 0; j < n; ++j) m += r[j];
+        for (int j = 0; j < n; ++j) m += x[j];
         *mean_s = m / static_cast<double>(n);
     }
     __syncthreads();
     const double mean = *mean_s;
-    const double twopi = 6.283185307179586476925286766559;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        if (k == 0) { pr[0] = 0.0; pim[0] = 0.0; continue; }
-        double rr = 0.0, ri = 0.0;
-        int m = 0;
-        for (int j = 0; j < n; ++j) {
-            const double x = r[j] - mean;
-            rr += x * cs[m];
-            ri -= x * sn[m];
-            m += k;
-            if (m >= n) m -= n;
-        }
-        rr /= static_cast<double>(n);
-        ri /= static_cast<double>(n);
-        const double kk = (k <= n / 2) ? static_cast<double>(k) : -static_cast<double>(n - k);
-        const double wave = twopi * kk / L;
-        const double den = eps * wave * wave;
-        pr[k] = rr / den;
-        pim[k] = ri / den;
-    }
+    for (int j = threadIdx.x; j < n; j += blockDim.x) x[j] = x[j] - mean;
     __syncthreads();
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        double ph = 0.0, e = 0.0;
-        int m = j;
-        for (int k = 1; k < n; ++k) {
-            const double c = cs[m], s = sn[m];
-            const double kk = (k <= n / 2) ? static_cast<double>(k) : -static_cast<double>(n - k);
-            const double wave = twopi * kk / L;
-            ph += pr[k] * c - pim[k] * s;
-            // E_k = -i wave phi_k = (wave pim, -wave pr)
-            e += (wave * pim[k]) * c - (-wave * pr[k]) * s;
-            m += j;
-            if (m >= n) m -= n;
-        }
-        phi[j] = ph;
-        E[j] = e;
-        E32[j] = static_cast<float>(e);
-    }
+    if (lds_tables)
+        spectral_sums_1d(x, pr, pim, sh + 3 * n, sh + 4 * n, phi, E, E32, n, L, eps);
+    else
+        spectral_sums_1d(x, pr, pim, cs, sn, phi, E, E32, n, L, eps);
 }
 
 

@@ -979,7 +979,7 @@ void solve_step_fields(Context& c, double phi_left, double phi_right) {
         const int n = static_cast<int>(c.n[0]);
         if (c.bc == AURORA_PERIODIC) {
             KernelScope ks(c, "solve_spectral_1d");
-            const size_t shmem = 3 * static_cast<size_t>(n) * sizeof(double);
+            const size_t shmem = spectral_1d_lds(n);
             k_spectral_1d_acc<<<1, 1024, shmem, c.stream>>>(ra, cl, c.rho.ptr, c.phi.ptr,
                                                             c.E64.ptr, c.E32.ptr, c.dft_cos.ptr,
                                                             c.dft_sin.ptr, n, c.L[0], c.eps);
@@ -1006,7 +1006,7 @@ void solve_fields(Context& c, double phi_left, double phi_right, bool want_phi, 
         const int n = static_cast<int>(c.n[0]);
         if (c.bc == AURORA_PERIODIC) {
             KernelScope ks(c, "solve_spectral_1d");
-            const size_t shmem = 3 * static_cast<size_t>(n) * sizeof(double);
+            const size_t shmem = spectral_1d_lds(n);
             k_spectral_1d<<<1, 1024, shmem, c.stream>>>(c.rho.ptr, c.phi.ptr, c.E64.ptr, c.E32.ptr,
                                                         c.dft_cos.ptr, c.dft_sin.ptr, n, c.L[0],
                                                         c.eps);
