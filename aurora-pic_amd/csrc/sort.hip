@@ -21,7 +21,11 @@ struct KeyArgs {
     int tx, ty, tz;        // tile extent in cells
     int ntx, nty;          // tiles per axis
     unsigned sentinel;
+    int ltx, lty, ltz;     // log2 of the tile extents (-1: not a power of two)
 };
+
+// i / t, a shift when t = 2^l
+__device__ __forceinline__ int tile_of(int i, int t, int l) { return l >= 0 ? i >> l : i / t; }
 
 // key = tile id; a stable sort keeps the previous order inside a tile, so
 // neighbouring slots of a tile hold particles of different cells
@@ -35,7 +39,8 @@ __global__ void k_sort_keys(const int32_t* cell, unsigned* keys, unsigned* vals,
             const int i = c & k.mask;
             const int j = (c >> k.shift) & k.mask;
             const int kk = c >> (2 * k.shift);
-            key = static_cast<unsigned>(((kk / k.tz) * k.nty + j / k.ty) * k.ntx + i / k.tx);
+            key = static_cast<unsigned>((tile_of(kk, k.tz, k.ltz) * k.nty + tile_of(j, k.ty, k.lty)) *
+                                            k.ntx + tile_of(i, k.tx, k.ltx));
         }
         keys[p] = key;
         vals[p] = static_cast<unsigned>(p);
@@ -174,9 +179,9 @@ __device__ __forceinline__ void cell_tile(const LsGeom& g, int c, int& tx, int& 
         j = (c >> g.k.shift) & g.k.mask;
         if (g.dim > 2) kk = c >> (2 * g.k.shift);
     }
-    tx = i / g.k.tx;
-    ty = j / g.k.ty;
-    tz = kk / g.k.tz;
+    tx = tile_of(i, g.k.tx, g.k.ltx);
+    ty = tile_of(j, g.k.ty, g.k.lty);
+    tz = tile_of(kk, g.k.tz, g.k.ltz);
 }
 
 // step from source tile s to tile d along an axis of nt tiles (periodic
@@ -223,25 +228,31 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n) {
         p[i] = 0u;
 }
 
-// distinct codes of a wave, one at a time (a wave-uniform loop): calls
-// f(code, mask of the lanes holding it) with every lane present
-template <typename F>
-__device__ __forceinline__ void for_each_code(int code, F&& f) {
-    bool pending = code >= 0;
-    for (;;) {
-        const uint64_t mp = __ballot(pending);
-        if (mp == 0) break;
-        const int lc = __shfl(code, __builtin_ctzll(mp));
-        const bool mine = pending && code == lc;
-        const uint64_t m = __ballot(mine);
-        f(lc, m, mine, __builtin_ctzll(mp));
-        if (mine) pending = false;
+// the lanes of the wave that hold the same code as this one (codes below
+// 32; none for code < 0): five bit ballots in place of a loop over the
+// distinct codes of the wave
+__device__ __forceinline__ uint64_t code_peers(int code) {
+    const bool has = code >= 0;
+    uint64_t m = __ballot(has);
+    if (m == 0ull) return 0ull;
+    // one code in the whole wave (the common case): its lanes
+    const int c0 = __shfl(code, __builtin_ctzll(m));
+    if (__ballot(has && code == c0) == m) return has ? m : 0ull;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+        const bool bit = has && ((code >> b) & 1) != 0;
+        const uint64_t mb = __ballot(bit);
+        m &= bit ? mb : ~mb;
     }
+    return has ? m : 0ull;
 }
 
 // each thread takes kLsUnroll particles per pass (loads issued together)
 constexpr int kLsUnroll = 4;
 constexpr int kLsChunk = kLsUnroll * kLsBlock;
+// the scatter's pass: longer runs of writes per destination
+constexpr int kLsScatterUnroll = 8;
+constexpr int kLsScatterChunk = kLsScatterUnroll * kLsBlock;
 
 __global__ void __launch_bounds__(kLsBlock) k_ls_count(LsGeom g, const int32_t* __restrict__ old_start,
                                                        const int32_t* __restrict__ cell,
@@ -267,10 +278,10 @@ __global__ void __launch_bounds__(kLsBlock) k_ls_count(LsGeom g, const int32_t* 
             int t = -1, code = -1;
             if (p < p1) code = ls_code(g, cv[u], sx, sy, sz, t);
             if (code == kFar) atomicAdd(far_count + t, 1u);
-            // one LDS add per distinct code of the wave
-            for_each_code(code, [&](int lc, uint64_t m, bool, int leader) {
-                if (lane == leader) atomicAdd(&s_cnt[lc], static_cast<uint32_t>(__popcll(m)));
-            });
+            // one LDS add per distinct code of the wave, by its lowest lane
+            const uint64_t peers = code_peers(code);
+            if (peers != 0ull && lane == __builtin_ctzll(peers))
+                atomicAdd(&s_cnt[code], static_cast<uint32_t>(__popcll(peers)));
         }
     }
     __syncthreads();
@@ -343,6 +354,7 @@ __global__ void __launch_bounds__(1024) k_ls_scan(int ntiles, const uint32_t* __
     const int per = (ntiles + 1023) / 1024;
     const int i0 = threadIdx.x * per, i1 = min(ntiles, i0 + per);
     uint32_t a = 0u, f = 0u, d = 0u;
+#pragma unroll 8
     for (int i = i0; i < i1; ++i) {
         a += new_count[i];
         f += count[static_cast<int64_t>(i) * kCodes + kFar];
@@ -352,6 +364,7 @@ __global__ void __launch_bounds__(1024) k_ls_scan(int ntiles, const uint32_t* __
     uint32_t ea = block_scan_u32(a, lds, ta);
     uint32_t ef = block_scan_u32(f, lds, tf);
     (void)block_scan_u32(d, lds, td);
+#pragma unroll 8
     for (int i = i0; i < i1; ++i) {
         new_start[i] = static_cast<int32_t>(ea);
         ea += new_count[i];
@@ -393,7 +406,7 @@ __global__ void __launch_bounds__(kLsBlock) k_ls_scatter(LsGeom g, const int32_t
                                                          const uint32_t* __restrict__ offset,
                                                          const uint32_t* __restrict__ far_base,
                                                          LsPayload<R> a) {
-    constexpr int U = kLsUnroll;
+    constexpr int U = kLsScatterUnroll;
     __shared__ uint32_t s_run[kCodes];
     __shared__ uint32_t s_tab[2][U][kLsWaves][kCodes];
     const int s = blockIdx.x;
@@ -407,7 +420,7 @@ __global__ void __launch_bounds__(kLsBlock) k_ls_scatter(LsGeom g, const int32_t
     const uint32_t fb = far_base[s];
     const uint32_t* off_s = offset + static_cast<int64_t>(s) * kNear;
     int buf = 0;
-    for (int base = p0; base < p1; base += kLsChunk, buf ^= 1) {
+    for (int base = p0; base < p1; base += kLsScatterChunk, buf ^= 1) {
         int32_t c[U];
         uint32_t id[U];
         R v[U][6];
@@ -433,10 +446,12 @@ __global__ void __launch_bounds__(kLsBlock) k_ls_scatter(LsGeom g, const int32_t
             t[u] = -1;
             rank[u] = 0u;
             if (p < p1) code[u] = ls_code(g, c[u], sx, sy, sz, t[u]);
-            for_each_code(code[u], [&](int lc, uint64_t m, bool mine, int leader) {
-                if (mine) rank[u] = lanes_below(m);
-                if (lane == leader) s_tab[buf][u][w][lc] = static_cast<uint32_t>(__popcll(m));
-            });
+            const uint64_t peers = code_peers(code[u]);
+            if (peers != 0ull) {
+                rank[u] = lanes_below(peers);
+                if (lane == __builtin_ctzll(peers))
+                    s_tab[buf][u][w][code[u]] = static_cast<uint32_t>(__popcll(peers));
+            }
         }
         __syncthreads();
         if (threadIdx.x < kCodes) {
@@ -618,6 +633,14 @@ void sort_species(Context& c, Species& sp) {
     k.tz = c.dim > 2 ? (c.tile[2] <= k.nz ? c.tile[2] : k.nz) : 1;
     k.ntx = (k.nx + k.tx - 1) / k.tx;
     k.nty = (k.ny + k.ty - 1) / k.ty;
+    auto log2_or = [](int t) {
+        int l = 0;
+        while ((1 << l) < t) ++l;
+        return (1 << l) == t ? l : -1;
+    };
+    k.ltx = log2_or(k.tx);
+    k.lty = log2_or(k.ty);
+    k.ltz = log2_or(k.tz);
     const int ntz = (k.nz + k.tz - 1) / k.tz;
     const uint64_t keyspace = static_cast<uint64_t>(k.ntx) * k.nty * ntz;
     AUR_REQUIRE(keyspace < 0x7FFFFFFFull, AURORA_ERR_UNSUPPORTED, "sort key space too large");
