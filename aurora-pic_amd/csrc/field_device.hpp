@@ -70,9 +70,11 @@ __device__ __forceinline__ double rho_at_strided(const RhoArgs& a, const long lo
 // kSpecLanes lanes, lane l taking every kSpecLanes-th term from its own on,
 // and the group's partial sums are combined by a butterfly: the order of the
 // additions depends on n alone, not on the block size (a multiple of
-// kSpecLanes).  sh holds spectral_1d_lds(n) bytes of LDS, mean_s one: on
-// meshes up to kSpecLdsTables nodes the cos/sin tables are staged there
-// too (a lane's table loads then wait on LDS, not on the cache hierarchy).
+// kSpecLanes).  A lane takes the cos/sin of its first term from the tables
+// and advances them by the rotation of its stride (the tables' values to
+// about 1e-14 over 256 terms).  sh holds spectral_1d_lds(n) bytes of LDS,
+// mean_s one: on meshes up to kSpecLdsTables nodes the tables are staged
+// there too.
 constexpr int kSpecLanes = 8;
 constexpr int kSpecLdsTables = 1024;
 
@@ -103,14 +105,19 @@ __device__ __forceinline__ void spectral_sums_1d(double* x, double* pr, double* 
         const int k = k0 + group;
         double rr = 0.0, ri = 0.0;
         if (k < n) {
-            int m = (lane * k) % n;
+            // cos/sin(2 pi j k / n) from the lane's first angle, advanced by
+            // a rotation of 2 pi 8k / n per term
+            const int m = (lane * k) % n;
             const int step = (kSpecLanes * k) % n;
+            double c = cs[m], s = sn[m];
+            const double dc = cs[step], ds = sn[step];
 #pragma unroll 4
             for (int j = lane; j < n; j += kSpecLanes) {
-                rr += x[j] * cs[m];
-                ri -= x[j] * sn[m];
-                m += step;
-                if (m >= n) m -= n;
+                rr += x[j] * c;
+                ri -= x[j] * s;
+                const double c2 = c * dc - s * ds;
+                s = s * dc + c * ds;
+                c = c2;
             }
         }
         rr = spec_group_sum(rr);
@@ -141,16 +148,18 @@ __device__ __forceinline__ void spectral_sums_1d(double* x, double* pr, double* 
         const int j = j0 + group;
         double ph = 0.0, e = 0.0;
         if (j < n) {
-            int m = (j * (1 + lane)) % n;
+            const int m = (j * (1 + lane)) % n;
             const int step = (j * kSpecLanes) % n;
+            double c = cs[m], s = sn[m];
+            const double dc = cs[step], ds = sn[step];
 #pragma unroll 4
             for (int k = 1 + lane; k < n; k += kSpecLanes) {
-                const double c = cs[m], s = sn[m];
                 const double wave = x[k];
                 ph += pr[k] * c - pim[k] * s;
                 e += (wave * pim[k]) * c - (-wave * pr[k]) * s;
-                m += step;
-                if (m >= n) m -= n;
+                const double c2 = c * dc - s * ds;
+                s = s * dc + c * ds;
+                c = c2;
             }
         }
         ph = spec_group_sum(ph);
