@@ -98,23 +98,6 @@ struct EnsArgs {
     float dt;
 };
 
-// Block-wide exclusive scan of one int per thread (kEnsThreads threads);
-// returns the exclusive prefix and sets total.
-__device__ __forceinline__ int block_exclusive_scan(int x, int* sh, int& total) {
-    sh[threadIdx.x] = x;
-    __syncthreads();
-    for (int d = 1; d < kEnsThreads; d <<= 1) {
-        const int y = threadIdx.x >= d ? sh[threadIdx.x - d] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += y;
-        __syncthreads();
-    }
-    total = sh[kEnsThreads - 1];
-    const int incl = sh[threadIdx.x];
-    __syncthreads();
-    return incl - x;
-}
-
 // kick + drift + boundaries + deposit (PUSH), or the deposit alone
 template <bool PUSH>
 __global__ void __launch_bounds__(kEnsThreads) k_ens_push(EnsArgs a) {
@@ -282,7 +265,7 @@ __global__ void __launch_bounds__(kEnsThreads, mcc_min_waves(NCH))
                                       a.prod + 12 * p, t, a.err + inst);
         flush_tally_rows<NCH>(t, active, w);
         int total = 0;
-        const int ex = block_exclusive_scan(np, scan_sh, total);
+        const int ex = block_scan_int<kEnsThreads>(np, scan_sh, total);
         if (in) {
             a.nprod[p] = np;
             a.loff[p] = ex;
@@ -309,18 +292,9 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_products(EnsArgs a) {
             a.counts + static_cast<int64_t>(inst) * (kMaxChannels + 3));
     }
     if (a.sec < 0) return;
-    const int per = (a.mchunks + kEnsThreads - 1) / kEnsThreads;
-    const int c0 = threadIdx.x * per;
     const int32_t* tot = a.chunk_tot + static_cast<int64_t>(inst) * a.mchunks;
     int32_t* off = a.chunk_off + static_cast<int64_t>(inst) * a.mchunks;
-    int mine = 0;
-    for (int k = 0; k < per && c0 + k < a.mchunks; ++k) mine += tot[c0 + k];
-    int total = 0;
-    int run = block_exclusive_scan(mine, scan_sh, total);
-    for (int k = 0; k < per && c0 + k < a.mchunks; ++k) {
-        off[c0 + k] = run;
-        run += tot[c0 + k];
-    }
+    const int total = chunk_offsets<kEnsThreads>(tot, off, a.mchunks, scan_sh);
     if (threadIdx.x != 0) return;
     int64_t* b = a.base + static_cast<int64_t>(inst) * kBaseFields;
     const int M = a.M;
@@ -428,7 +402,7 @@ __global__ void __launch_bounds__(kEnsThreads) k_compact_count(CompactArgs a) {
     const int64_t p = static_cast<int64_t>(inst) * a.cap + q;
     const int live = (q < n && a.cell[p] >= 0) ? 1 : 0;
     int total = 0;
-    const int ex = block_exclusive_scan(live, scan_sh, total);
+    const int ex = block_scan_int<kEnsThreads>(live, scan_sh, total);
     if (q < n) a.loff[p] = ex;
     if (threadIdx.x == 0) a.tot[blockIdx.x] = total;
 }
@@ -436,18 +410,9 @@ __global__ void __launch_bounds__(kEnsThreads) k_compact_count(CompactArgs a) {
 __global__ void __launch_bounds__(kEnsThreads) k_compact_scan(CompactArgs a) {
     __shared__ int scan_sh[kEnsThreads];
     const int inst = blockIdx.x;
-    const int per = (a.chunks + kEnsThreads - 1) / kEnsThreads;
-    const int c0 = threadIdx.x * per;
     const int32_t* tot = a.tot + static_cast<int64_t>(inst) * a.chunks;
     int32_t* off = a.coff + static_cast<int64_t>(inst) * a.chunks;
-    int mine = 0;
-    for (int k = 0; k < per && c0 + k < a.chunks; ++k) mine += tot[c0 + k];
-    int total = 0;
-    int run = block_exclusive_scan(mine, scan_sh, total);
-    for (int k = 0; k < per && c0 + k < a.chunks; ++k) {
-        off[c0 + k] = run;
-        run += tot[c0 + k];
-    }
+    const int total = chunk_offsets<kEnsThreads>(tot, off, a.chunks, scan_sh);
     if (threadIdx.x == 0) a.cnew[inst] = total;
 }
 

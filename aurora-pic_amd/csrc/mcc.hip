@@ -84,30 +84,6 @@ struct Mcc1dArgs {
     int32_t* chunk_tot;     // per chunk: its products (null: the model has none)
 };
 
-// Exclusive scan of one int per thread over a kBlk-thread block: the wave's
-// prefix by shuffles, the wave totals through LDS; sets total.
-__device__ __forceinline__ int block_scan_excl(int x, int* wsum, int& total) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int incl = x;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += y;
-    }
-    if (lane == 63) wsum[wid] = incl;
-    __syncthreads();
-    int before = 0;
-    total = 0;
-#pragma unroll
-    for (int k = 0; k < kBlk / 64; ++k) {
-        const int sk = wsum[k];
-        before += k < wid ? sk : 0;
-        total += sk;
-    }
-    __syncthreads();
-    return before + incl - x;
-}
-
 // sync -> collide -> rebuild v_half for every live target particle
 // (with products: each 256-slot chunk's event counts scanned in the block,
 // so a parent's product index is its chunk's offset plus its own prefix)
@@ -133,7 +109,7 @@ __global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, doubl
         flush_tally_rows<NCH>(t, active, w);
         if (a.chunk_tot) {
             int total = 0;
-            const int ex = block_scan_excl(np, wsum, total);
+            const int ex = block_scan_int<kBlk>(np, wsum, total);
             if (in) a.loff[p] = ex;
             if (threadIdx.x == 0) a.chunk_tot[base / kBlk] = total;
         }
@@ -155,16 +131,7 @@ __global__ void __launch_bounds__(kBlk) k_mcc_finish(const double* rows, int64_t
         return;
     }
     __shared__ int wsum[kBlk / 64];
-    const int64_t per = (nchunks + kBlk - 1) / kBlk;
-    const int64_t c0 = threadIdx.x * per;
-    int mine = 0;
-    for (int64_t k = 0; k < per && c0 + k < nchunks; ++k) mine += chunk_tot[c0 + k];
-    int all = 0;
-    int run = block_scan_excl(mine, wsum, all);
-    for (int64_t k = 0; k < per && c0 + k < nchunks; ++k) {
-        chunk_off[c0 + k] = run;
-        run += chunk_tot[c0 + k];
-    }
+    const int all = chunk_offsets<kBlk>(chunk_tot, chunk_off, nchunks, wsum);
     if (threadIdx.x == 0) *total = all;
 }
 

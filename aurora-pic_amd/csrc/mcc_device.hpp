@@ -650,6 +650,65 @@ __device__ __forceinline__ void store_wave_row(const WaveTally<NCH>& w, double* 
 
 // Adds rows [0, nrows) in row order: energies into energy[NCH], counts into
 // counts[NCH + 3].  Block of BLOCK threads; every thread calls it.
+// Exclusive scan of one int per thread over a BLOCK-thread block: the
+// wave's prefix by shuffles, the wave totals through LDS (wsum: BLOCK / 64
+// ints); sets total.
+template <int BLOCK>
+__device__ __forceinline__ int block_scan_int(int x, int* wsum, int& total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int incl = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+#pragma unroll
+    for (int k = 0; k < BLOCK / 64; ++k) {
+        const int sk = wsum[k];
+        before += k < wid ? sk : 0;
+        total += sk;
+    }
+    __syncthreads();
+    return before + incl - x;
+}
+
+// off[c] = tot[0] + ... + tot[c - 1] for c < nchunks, by one BLOCK-thread
+// block: each thread sums a consecutive run, its loads issued eight at a
+// time; returns the sum of all totals.
+template <int BLOCK>
+__device__ int chunk_offsets(const int32_t* tot, int32_t* off, int64_t nchunks, int* wsum) {
+    constexpr int kB = 8;
+    const int64_t per = (nchunks + BLOCK - 1) / BLOCK;
+    const int64_t c0 = static_cast<int64_t>(threadIdx.x) * per;
+    const int64_t c1 = c0 + per < nchunks ? c0 + per : nchunks;
+    int mine = 0;
+    for (int64_t c = c0; c < c1; c += kB) {
+        int v[kB];
+#pragma unroll
+        for (int b = 0; b < kB; ++b) v[b] = c + b < c1 ? tot[c + b] : 0;
+#pragma unroll
+        for (int b = 0; b < kB; ++b) mine += v[b];
+    }
+    int total = 0;
+    int run = block_scan_int<BLOCK>(mine, wsum, total);
+    for (int64_t c = c0; c < c1; c += kB) {
+        int v[kB];
+#pragma unroll
+        for (int b = 0; b < kB; ++b) v[b] = c + b < c1 ? tot[c + b] : 0;
+#pragma unroll
+        for (int b = 0; b < kB; ++b)
+            if (c + b < c1) {
+                off[c + b] = run;
+                run += v[b];
+            }
+    }
+    return total;
+}
+
 // A thread's rows r, r + BLOCK, ... are loaded kFoldBatch at a time (the
 // loads in flight together) and added in that same order.
 template <int NCH, int BLOCK>
