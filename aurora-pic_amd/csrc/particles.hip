@@ -1230,10 +1230,13 @@ bool launch_tile2d(Context& c, Species& sp, double dt) {
 }
 
 // Workgroups of a push launch: 1D deposits flush one LDS copy of the mesh
-// per workgroup, so their launches are capped at two workgroups per CU.
+// per workgroup with global atomics on the same few nodes, so their
+// launches are capped at one workgroup per CU (measured on 513 nodes and
+// 5e5 + 5e5 particles: 21.5 us per pair push against 25.7 with two per CU
+// and 24.2 with one per two CUs).
 int push_blocks(const Context& c, int64_t slots, bool deposit) {
     const int g = launch_blocks(c, slots);
-    if (c.dim == 1 && deposit && c.n[0] <= kLds1dNodes) return std::min(g, 2 * c.cu_count);
+    if (c.dim == 1 && deposit && c.n[0] <= kLds1dNodes) return std::min(g, c.cu_count);
     return g;
 }
 
