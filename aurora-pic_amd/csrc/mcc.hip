@@ -378,13 +378,21 @@ void mcc_collide_pass(Context& c, int model, double dt) {
     a.loff = products ? h.product_loff.ptr : nullptr;
     a.chunk_tot = products ? h.chunk_tot.ptr : nullptr;
     KernelScope ks(c, "mcc");
-    // one round of resident workgroups (grid-stride): fewer tally rows for
-    // the one-block fold that follows
-    const int g = std::min(blocks(c, n), 4 * c.cu_count);
-    const int64_t rows = energy_rows(g);
-    if (h.energy_rows.count < static_cast<size_t>(rows * kTallyStride))
-        h.energy_rows.alloc(static_cast<size_t>(rows * kTallyStride));
     with_nchan(m.nchan, [&]<int NCH>() {
+        // one round of resident workgroups (grid-stride; a partial second
+        // round would leave most of the chip idle at its end), which also
+        // keeps the tally rows few for the one-block fold that follows
+        static const int per_cu = [] {
+            int v = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_mcc_1d<NCH>, kBlk, 0) !=
+                hipSuccess)
+                v = 1;
+            return std::max(1, v);
+        }();
+        const int g = std::min(blocks(c, n), per_cu * c.cu_count);
+        const int64_t rows = energy_rows(g);
+        if (h.energy_rows.count < static_cast<size_t>(rows * kTallyStride))
+            h.energy_rows.alloc(static_cast<size_t>(rows * kTallyStride));
         k_mcc_1d<NCH><<<g, kBlk, 0, c.stream>>>(m, a, h.energy_rows.ptr);
         k_mcc_finish<NCH><<<1, kBlk, 0, c.stream>>>(h.energy_rows.ptr, rows,
                                                     h.energy_change.ptr, h.counts.ptr,
