@@ -827,8 +827,17 @@ int aurora_hip_ensemble_create(aurora_hip_ctx* h, int64_t instances, const int64
             e->prod.alloc(static_cast<size_t>(12 * n));
             e->chunk_tot.alloc(static_cast<size_t>(M * mch));
             e->chunk_off.alloc(static_cast<size_t>(M * mch));
-            // about two rounds of resident MCC workgroups (five per CU)
-            const int64_t want = static_cast<int64_t>(c.cu_count) * 10;
+            // one round of resident MCC workgroups over all instances (a
+            // partial further round would leave most of the chip idle)
+            int per_cu = 1;
+            with_nchan(c.mcc.nchan, [&]<int NCH>() {
+                int v = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_ens_mcc<NCH>, kEnsThreads,
+                                                                 0) == hipSuccess &&
+                    v > 0)
+                    per_cu = v;
+            });
+            const int64_t want = static_cast<int64_t>(c.cu_count) * per_cu;
             e->mgroups = static_cast<int>(std::clamp<int64_t>((want + M - 1) / M, 1, mch));
             e->energy_rows.alloc(static_cast<size_t>(M * e->mgroups * (kEnsThreads / 64) *
                                                      kTallyStride));
