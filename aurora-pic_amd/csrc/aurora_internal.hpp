@@ -233,7 +233,7 @@ struct MccModelHost {
     std::vector<double> sigma_host[kMaxChannels];
     DevBuf<unsigned long long> counts;  // [candidates, nulls, chan...]
     DevBuf<double> energy_change;       // per channel, cumulative
-    DevBuf<double> energy_rows;         // per wave of the last MCC launch, per channel
+    DevBuf<double> energy_rows;         // per workgroup of the last 1D MCC launch
     DevBuf<int32_t> product_count;      // per target particle
     DevBuf<int32_t> product_loff;       // per target particle: products before it in its chunk
     DevBuf<int32_t> chunk_tot, chunk_off;   // per 256-slot chunk: products, exclusive offset

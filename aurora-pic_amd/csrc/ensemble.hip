@@ -91,7 +91,7 @@ struct EnsArgs {
     int32_t* chunk_off;
     int mchunks;
     int mgroups;                    // MCC workgroups per instance (each walks chunks mgroups apart)
-    double* energy_rows;            // [M][mgroups * waves][kTallyStride]: per-wave tallies
+    double* energy_rows;            // [M][mgroups][kTallyStride]: per-workgroup tallies
     int64_t* base;                  // [M][kBaseFields]
     unsigned long long* counts;     // [M][kMaxChannels + 3]
     double* ech;                    // [M][kMaxChannels]
@@ -230,8 +230,8 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_
 // MCC on the target species.  mgroups workgroups per instance walk its
 // 256-slot chunks mgroups apart, so the cross-section tables are staged in
 // LDS once per workgroup; product pairs are staged per parent with in-chunk
-// offsets, and the energy change goes to one row per wave (summed in a
-// fixed order by k_ens_products).
+// offsets, and the energy change goes to one row per workgroup (its waves
+// in wave order; the rows summed in a fixed order by k_ens_products).
 template <int NCH>
 __global__ void __launch_bounds__(kEnsThreads, mcc_min_waves(NCH))
     k_ens_mcc(EnsArgs a, MccParams m) {
@@ -272,9 +272,8 @@ __global__ void __launch_bounds__(kEnsThreads, mcc_min_waves(NCH))
         }
         if (threadIdx.x == 0) a.chunk_tot[ci] = total;
     }
-    store_tally_row<NCH>(w, a.energy_rows,
-                         (static_cast<int64_t>(inst) * a.mgroups + group) * (kEnsThreads / 64) +
-                             (threadIdx.x >> 6));
+    store_block_row<NCH, kEnsThreads>(w, a.energy_rows,
+                                      static_cast<int64_t>(inst) * a.mgroups + group);
 }
 
 // Per instance: chunk offsets, all-or-nothing capacity check
@@ -284,8 +283,8 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_products(EnsArgs a) {
     __shared__ int scan_sh[kEnsThreads];
     const int inst = blockIdx.x;
     {
-        // this step's MCC tallies: the instance's wave rows, in order
-        const int64_t nrows = static_cast<int64_t>(a.mgroups) * (kEnsThreads / 64);
+        // this step's MCC tallies: the instance's workgroup rows, in order
+        const int64_t nrows = a.mgroups;
         fold_tally_rows<NCH, kEnsThreads>(
             a.energy_rows + static_cast<int64_t>(inst) * nrows * kTallyStride, nrows,
             a.ech + static_cast<int64_t>(inst) * kMaxChannels,
@@ -839,8 +838,7 @@ int aurora_hip_ensemble_create(aurora_hip_ctx* h, int64_t instances, const int64
             });
             const int64_t want = static_cast<int64_t>(c.cu_count) * per_cu;
             e->mgroups = static_cast<int>(std::clamp<int64_t>((want + M - 1) / M, 1, mch));
-            e->energy_rows.alloc(static_cast<size_t>(M * e->mgroups * (kEnsThreads / 64) *
-                                                     kTallyStride));
+            e->energy_rows.alloc(static_cast<size_t>(M * e->mgroups * kTallyStride));
             e->base.alloc(static_cast<size_t>(M * kBaseFields));
             AUR_HIP(hipMemsetAsync(e->base.ptr, 0, M * kBaseFields * 8, c.stream));
         }

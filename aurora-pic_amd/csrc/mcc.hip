@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, doubl
             if (threadIdx.x == 0) a.chunk_tot[base / kBlk] = total;
         }
     }
-    store_wave_row<NCH>(w, rows);
+    store_block_row<NCH, kBlk>(w, rows, blockIdx.x);
 }
 
 // After one model's k_mcc_1d: its tally rows added in row order (as
@@ -390,7 +390,7 @@ void mcc_collide_pass(Context& c, int model, double dt) {
             return std::max(1, v);
         }();
         const int g = std::min(blocks(c, n), per_cu * c.cu_count);
-        const int64_t rows = energy_rows(g);
+        const int64_t rows = g;   // one per workgroup
         if (h.energy_rows.count < static_cast<size_t>(rows * kTallyStride))
             h.energy_rows.alloc(static_cast<size_t>(rows * kTallyStride));
         k_mcc_1d<NCH><<<g, kBlk, 0, c.stream>>>(m, a, h.energy_rows.ptr);

@@ -648,6 +648,42 @@ __device__ __forceinline__ void store_wave_row(const WaveTally<NCH>& w, double* 
                          static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 
+// One row per workgroup (BLOCK threads, every thread calls it): the waves'
+// tallies added in wave order, stored at rows + row * kTallyStride.
+template <int NCH, int BLOCK>
+__device__ __forceinline__ void store_block_row(const WaveTally<NCH>& w, double* rows,
+                                                int64_t row) {
+    constexpr int W = BLOCK / 64;
+    __shared__ double se[W][NCH];
+    __shared__ unsigned long long sc[W][NCH + 3];
+    const int wid = static_cast<int>(threadIdx.x) >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) se[wid][c] = w.energy[c];
+#pragma unroll
+        for (int k = 0; k < NCH + 3; ++k) sc[wid][k] = w.count[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        WaveTally<NCH> t;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            double e = se[0][c];
+#pragma unroll
+            for (int v = 1; v < W; ++v) e += se[v][c];
+            t.energy[c] = e;
+        }
+#pragma unroll
+        for (int k = 0; k < NCH + 3; ++k) {
+            unsigned long long n = 0;
+#pragma unroll
+            for (int v = 0; v < W; ++v) n += sc[v][k];
+            t.count[k] = n;
+        }
+        store_tally_row<NCH>(t, rows, row);
+    }
+}
+
 // Adds rows [0, nrows) in row order: energies into energy[NCH], counts into
 // counts[NCH + 3].  Block of BLOCK threads; every thread calls it.
 // Exclusive scan of one int per thread over a BLOCK-thread block: the

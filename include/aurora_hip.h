@@ -493,8 +493,8 @@ int aurora_hip_ensemble_download(aurora_hip_ctx* ctx, int64_t handle,
 /* MCC counters [candidates, nulls, per channel..., secondaries], per
  * channel energy change (J, weighted by the species weight), wall counts and
  * energies [species][left,right] of one instance (CollisionStepStatistics,
- * BoundaryLoss1D).  The FP64 sums are formed in a fixed order (per-wave and
- * per-workgroup rows folded in row order), so they repeat to the bit from
+ * BoundaryLoss1D).  The FP64 sums are formed in a fixed order (per-workgroup
+ * rows folded in row order), so they repeat to the bit from
  * run to run; the order differs from a lone context's (whose slots are not
  * compacted), so they equal a lone run's to rounding. */
 int aurora_hip_ensemble_stats(aurora_hip_ctx* ctx, int64_t handle,
