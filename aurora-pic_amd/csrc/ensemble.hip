@@ -342,7 +342,9 @@ __device__ __forceinline__ AppendSide ens_side(const EnsSpecies& sp, int inst, i
     return d;
 }
 
+// (and advances the step counter: nothing after it in the step reads it)
 __global__ void __launch_bounds__(kEnsThreads) k_ens_append(EnsArgs a) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.step += 1;
     const int inst = blockIdx.x / a.mchunks;
     const int chunk = blockIdx.x - inst * a.mchunks;
     const int64_t* b = a.base + static_cast<int64_t>(inst) * kBaseFields;
@@ -654,7 +656,10 @@ void enqueue_step(Context& c, Ensemble& e, const EnsArgs& a, const MccParams& m)
             k_ens_mcc<NCH><<<a.M * a.mgroups, kEnsThreads, 0, c.stream>>>(a, m);
             k_ens_products<NCH><<<a.M, kEnsThreads, 0, c.stream>>>(a);
         });
-        if (a.sec >= 0) k_ens_append<<<g, kEnsThreads, 0, c.stream>>>(a);
+        if (a.sec >= 0) {
+            k_ens_append<<<g, kEnsThreads, 0, c.stream>>>(a);
+            return;
+        }
     }
     k_ens_tick<<<1, 64, 0, c.stream>>>(e.step.ptr);
 }
