@@ -663,6 +663,21 @@ int aurora_hip_step(aurora_hip_ctx* h, double dt, int64_t nsteps) {
         AUR_REQUIRE(c.dim > 0 && finite_positive(dt), AURORA_ERR_INVALID_ARGUMENT,
                     "simulation dt must be positive and finite");
         AUR_REQUIRE(nsteps >= 0, AURORA_ERR_INVALID_ARGUMENT, "step count must be non-negative");
+        // 1D MCC with products: the slot counts stay on the device for the
+        // call (no host wait per step); the host's copies come back at its
+        // end, before a sort, and on an error
+        const bool dev_counts = nsteps > 0 && device_counts_apply(c);
+        struct Leave {
+            Context& c;
+            ~Leave() {
+                try {
+                    device_counts_leave(c);
+                } catch (...) {
+                    c.counts_on_device = false;
+                }
+            }
+        } leave{c};
+        if (dev_counts) device_counts_enter(c);
         for (int64_t s = 0; s < nsteps; ++s) {
             // kick + drift + boundary + deposit at x^{n+1} (fused)
             push_all(c, dt, true, c.dim == 1);
@@ -674,12 +689,18 @@ int aurora_hip_step(aurora_hip_ctx* h, double dt, int64_t nsteps) {
             c.step_count += 1;
             if (c.sort_policy == AURORA_SORT_ADAPTIVE) {
                 for (auto& sp : c.species)
-                    if (adaptive_sort_due(c, sp)) sort_species(c, sp);
+                    if (adaptive_sort_due(c, sp)) {
+                        device_counts_leave(c);
+                        sort_species(c, sp);
+                    }
             } else if (c.sort_interval > 0 && c.step_count % c.sort_interval == 0) {
+                device_counts_leave(c);
                 for (auto& sp : c.species)
                     if (sort_needed(c, sp)) sort_species(c, sp);
             }
+            if (dev_counts && !c.counts_on_device && s + 1 < nsteps) device_counts_enter(c);
         }
+        device_counts_leave(c);
         check_device_error(c);
         flush_timers(c);
     });

@@ -49,6 +49,9 @@ enum : uint32_t {
 };
 
 constexpr int kMaxChannels = 8;
+// per model placement of products on the device: secondary slot, ion slot,
+// secondary uid, ion uid, parents of the pass, placed (0/1)
+constexpr int kPlaceFields = 6;
 
 // Particle cell = packed corner-node coordinates: 1D i; 2D i | j << 15
 // (extents <= 32768); 3D i | j << 10 | k << 20 (extents <= 1024).
@@ -323,6 +326,16 @@ struct Context {
     DevBuf<long long> product_total;
     PinnedHost<long long> product_total_host;
     int64_t product_total_models = 0;
+    // 1D MCC steps inside one aurora_hip_step call keep the slot counts and
+    // next uids on the device (products are reserved and placed without a
+    // host wait; kernels read the counts, launches are sized for the
+    // capacity): [species] slots then [species] next uid, and per model
+    // the placement (kPlaceFields values); the host copies are refreshed at
+    // the end of the call or before a sort
+    DevBuf<long long> dev_counts;
+    DevBuf<long long> mcc_place;
+    PinnedHost<long long> dev_counts_host;
+    bool counts_on_device = false;
 
     std::vector<Species> species;
     MccModelHost mcc;                          // null-collision model 0 of the step
@@ -374,6 +387,12 @@ struct Context {
 };
 
 // Scoped kernel timer: records HIP events around a launch when enabled.
+// The slot count that sizes a species' launches: the capacity while the
+// counts live on the device (1D MCC steps of aurora_hip_step), else the slots.
+inline int64_t slot_bound(const Context& c, const Species& sp) {
+    return c.counts_on_device ? sp.capacity : sp.slots;
+}
+
 struct KernelScope {
     Context& ctx;
     const char* name;
@@ -434,6 +453,12 @@ bool sort_needed(Context& ctx, Species& sp);
 // adaptive policy: the species' push now costs more than its cycle average
 bool adaptive_sort_due(Context& ctx, Species& sp);
 void mcc_step(Context& ctx, double dt);
+// 1D MCC steps with the slot counts and next uids on the device: whether a
+// context qualifies (1D, MCC with products, at most kMaxChannels species and
+// models), the upload of the host's counts, and the read-back that ends it
+bool device_counts_apply(const Context& ctx);
+void device_counts_enter(Context& ctx);
+void device_counts_leave(Context& ctx);
 void collide_scratch_init(int device);
 void init_half_all(Context& ctx, double dt);
 void set_b_profile_device(Context& ctx);

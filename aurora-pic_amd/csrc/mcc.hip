@@ -82,6 +82,7 @@ struct Mcc1dArgs {
     uint32_t* err;
     int32_t* loff;          // per particle: products before it in its 256-slot chunk
     int32_t* chunk_tot;     // per chunk: its products (null: the model has none)
+    const long long* nslots;   // the slot count on the device (slots is then the capacity), or null
 };
 
 // sync -> collide -> rebuild v_half for every live target particle
@@ -93,10 +94,11 @@ __global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, doubl
     __shared__ int wsum[kBlk / 64];
     const XsView X = stage_tables(m, lds);
     WaveTally<NCH> w{};
-    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBlk; base < a.slots;
+    const int64_t nslot = a.nslots ? static_cast<int64_t>(*a.nslots) : a.slots;
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBlk; base < nslot;
          base += static_cast<int64_t>(gridDim.x) * kBlk) {
         const int64_t p = base + threadIdx.x;
-        const bool in = p < a.slots;
+        const bool in = p < nslot;
         const int c = in ? a.cell[p] : -1;
         const bool active = c >= 0;
         MccTally<NCH> t{};
@@ -124,12 +126,15 @@ template <int NCH>
 __global__ void __launch_bounds__(kBlk) k_mcc_finish(const double* rows, int64_t nrows,
                                                      double* energy, unsigned long long* counts,
                                                      const int32_t* chunk_tot, int32_t* chunk_off,
-                                                     int64_t nchunks, long long* total) {
+                                                     int64_t nchunks, const long long* nslots,
+                                                     long long* total) {
     fold_tally_rows<NCH, kBlk>(rows, nrows, energy, counts);
     if (!chunk_tot) {
         if (threadIdx.x == 0) *total = 0;
         return;
     }
+    // the chunks the collide wrote (its slot count on the device, if there)
+    if (nslots) nchunks = (static_cast<int64_t>(*nslots) + kBlk - 1) / kBlk;
     __shared__ int wsum[kBlk / 64];
     const int all = chunk_offsets<kBlk>(chunk_tot, chunk_off, nchunks, wsum);
     if (threadIdx.x == 0) *total = all;
@@ -146,12 +151,75 @@ __global__ void __launch_bounds__(kBlk) k_fold_rows(const double* rows, int64_t 
 
 __global__ void k_zero_total(long long* total) { *total = 0; }
 
+// All models' products reserved together on the device (the counts kept
+// there during aurora_hip_step): the all-or-nothing capacity check over
+// every model (Simulation.cpp:929-964), then slots and uids per model in
+// model order, as mcc_append reserves them on the host.  One thread.
+struct CommitArgs {
+    int models, nspecies;
+    int target[kMaxChannels], sec[kMaxChannels], ion[kMaxChannels];
+    long long capacity[kMaxChannels];
+};
+
+__global__ void k_mcc_commit(CommitArgs a, const long long* total, long long* counts,
+                             long long* place, uint32_t* err) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    long long* slots = counts;
+    long long* uid = counts + a.nspecies;
+    __shared__ long long need[kMaxChannels];
+    for (int s = 0; s < kMaxChannels; ++s) need[s] = 0;
+    for (int k = 0; k < a.models; ++k) {
+        long long* pk = place + static_cast<int64_t>(k) * kPlaceFields;
+        pk[4] = slots[a.target[k]];   // the parents of the pass
+        pk[5] = 0;
+        if (a.sec[k] < 0 || total[k] == 0) continue;
+        need[a.sec[k]] += total[k];
+        need[a.ion[k]] += total[k];
+    }
+    for (int s = 0; s < a.nspecies; ++s)
+        if (need[s] && slots[s] + need[s] > a.capacity[s]) {
+            atomicOr(err, DEV_ERR_CAPACITY);
+            return;
+        }
+    for (int k = 0; k < a.models; ++k) {
+        const long long t = total[k];
+        if (a.sec[k] < 0 || t == 0) continue;
+        long long* pk = place + static_cast<int64_t>(k) * kPlaceFields;
+        const int se = a.sec[k], si = a.ion[k];
+        pk[0] = slots[se];
+        pk[2] = uid[se];
+        if (se == si) {
+            pk[1] = slots[se] + t;
+            pk[3] = uid[se] + t;
+            slots[se] += 2 * t;
+            uid[se] += 2 * t;
+        } else {
+            pk[1] = slots[si];
+            pk[3] = uid[si];
+            slots[se] += t;
+            uid[se] += t;
+            slots[si] += t;
+            uid[si] += t;
+        }
+        pk[5] = 1;
+    }
+}
+
 // write the products of parent p at product index chunk_off[p / kBlk] +
-// loff[p] + k (grid-stride over 256-slot chunks)
+// loff[p] + k (grid-stride over 256-slot chunks); with place (device
+// counts), the parents and the bases come from k_mcc_commit's placement
 __global__ void __launch_bounds__(kBlk) k_append_products(
     const int32_t* pcell, const float* poff, const int32_t* nprod, const int32_t* loff,
     const int32_t* chunk_off, const float* prod, int64_t parents, const float* E32, int nx,
-    AppendSide sec, AppendSide ion) {
+    AppendSide sec, AppendSide ion, const long long* place) {
+    if (place) {
+        if (place[5] == 0) return;
+        sec.base_slot = place[0];
+        ion.base_slot = place[1];
+        sec.base_uid = static_cast<uint32_t>(place[2]);
+        ion.base_uid = static_cast<uint32_t>(place[3]);
+        parents = place[4];
+    }
     for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBlk; base < parents;
          base += static_cast<int64_t>(gridDim.x) * kBlk) {
         const int64_t p = base + threadIdx.x;
@@ -337,7 +405,10 @@ void mcc_collide_pass(Context& c, int model, double dt) {
     MccModelHost& h = c.mcc_model(model);
     Species& sp = c.species[h.target];
     AUR_REQUIRE(sp.precision == 32, AURORA_ERR_UNSUPPORTED, "MCC targets FP32 species");
-    const int64_t n = sp.slots;
+    // with the counts on the device: sized for the capacity, the kernels
+    // read the slot count
+    const int64_t n = slot_bound(c, sp);
+    const long long* nslots = c.counts_on_device ? c.dev_counts.ptr + h.target : nullptr;
     long long* total = c.product_total.ptr + model;
     if (n == 0) {
         k_zero_total<<<1, 1, 0, c.stream>>>(total);
@@ -377,6 +448,7 @@ void mcc_collide_pass(Context& c, int model, double dt) {
     a.err = c.dev_error.ptr;
     a.loff = products ? h.product_loff.ptr : nullptr;
     a.chunk_tot = products ? h.chunk_tot.ptr : nullptr;
+    a.nslots = nslots;
     KernelScope ks(c, "mcc");
     with_nchan(m.nchan, [&]<int NCH>() {
         // one round of resident workgroups (grid-stride; a partial second
@@ -396,17 +468,20 @@ void mcc_collide_pass(Context& c, int model, double dt) {
         k_mcc_1d<NCH><<<g, kBlk, 0, c.stream>>>(m, a, h.energy_rows.ptr);
         k_mcc_finish<NCH><<<1, kBlk, 0, c.stream>>>(h.energy_rows.ptr, rows,
                                                     h.energy_change.ptr, h.counts.ptr,
-                                                    a.chunk_tot, h.chunk_off.ptr, nchunks, total);
+                                                    a.chunk_tot, h.chunk_off.ptr, nchunks, nslots,
+                                                    total);
     });
     AUR_HIP(hipGetLastError());
 }
 
 // Places the `total` product pairs staged by one model's pass after the
 // live particles of the secondary and ion species (Simulation.cpp:965-1005).
-void mcc_append(Context& c, int model, int64_t total, double dt) {
+// place: the model's placement reserved on the device by k_mcc_commit (the
+// counts on the device; total and the host's counts are then unused).
+void mcc_append(Context& c, int model, int64_t total, double dt, const long long* place) {
     MccModelHost& h = c.mcc_model(model);
     Species& sp = c.species[h.target];
-    const int64_t n = sp.slots;
+    const int64_t n = slot_bound(c, sp);
     Species& se = c.species[h.secondary_species];
     Species& si = c.species[h.ion_species];
     auto side = [&](Species& s, int64_t base_slot) {
@@ -435,9 +510,12 @@ void mcc_append(Context& c, int model, int64_t total, double dt) {
         k_append_products<<<std::min(blocks(c, n), 4 * c.cu_count), kBlk, 0, c.stream>>>(
             sp.cell.ptr, reinterpret_cast<const float*>(sp.off[0].ptr), h.product_count.ptr,
             h.product_loff.ptr, h.chunk_off.ptr, h.product_vel.ptr, n, c.E32.ptr,
-            static_cast<int>(c.n[0]), sec, ion);
+            static_cast<int>(c.n[0]), sec, ion, place);
         AUR_HIP(hipGetLastError());
     }
+    se.tiled = false;
+    si.tiled = false;
+    if (place) return;
     if (h.secondary_species == h.ion_species) {
         se.slots += 2 * total;
         se.next_uid += static_cast<uint32_t>(2 * total);
@@ -447,8 +525,6 @@ void mcc_append(Context& c, int model, int64_t total, double dt) {
         si.slots += total;
         si.next_uid += static_cast<uint32_t>(total);
     }
-    se.tiled = false;
-    si.tiled = false;
 }
 
 }  // namespace
@@ -488,6 +564,28 @@ void mcc_step(Context& c, double dt) {
         products = products || c.mcc_model(k).secondary_species >= 0;
     }
     if (!products) return;
+    if (c.counts_on_device) {
+        // reserve on the device, place, no host wait
+        CommitArgs ca{};
+        ca.models = models;
+        ca.nspecies = static_cast<int>(c.species.size());
+        for (int k = 0; k < models; ++k) {
+            const MccModelHost& h = c.mcc_model(k);
+            ca.target[k] = h.target;
+            ca.sec[k] = h.secondary_species;
+            ca.ion[k] = h.ion_species;
+        }
+        for (int s = 0; s < ca.nspecies; ++s) ca.capacity[s] = c.species[s].capacity;
+        const size_t pn = static_cast<size_t>(models) * kPlaceFields;
+        if (c.mcc_place.count < pn) c.mcc_place.alloc(pn);
+        k_mcc_commit<<<1, 64, 0, c.stream>>>(ca, c.product_total.ptr, c.dev_counts.ptr,
+                                            c.mcc_place.ptr, c.dev_error.ptr);
+        AUR_HIP(hipGetLastError());
+        for (int k = 0; k < models; ++k)
+            if (c.mcc_model(k).secondary_species >= 0)
+                mcc_append(c, k, 0, dt, c.mcc_place.ptr + static_cast<int64_t>(k) * kPlaceFields);
+        return;
+    }
     // every model's product count in one copy into pinned memory, one wait
     AUR_HIP(hipMemcpyAsync(c.product_total_host.ptr, c.product_total.ptr,
                            static_cast<size_t>(models) * sizeof(long long), hipMemcpyDeviceToHost,
@@ -508,7 +606,64 @@ void mcc_step(Context& c, double dt) {
             throw Failure{AURORA_ERR_CAPACITY,
                           "reactive collisions exceeded max_particles_per_species for species"};
     for (int k = 0; k < models; ++k)
-        if (total[static_cast<size_t>(k)] > 0) mcc_append(c, k, total[static_cast<size_t>(k)], dt);
+        if (total[static_cast<size_t>(k)] > 0)
+            mcc_append(c, k, total[static_cast<size_t>(k)], dt, nullptr);
+}
+
+namespace {
+
+struct CountsArgs {
+    long long v[2 * kMaxChannels];   // [species] slots, then [species] next uid
+    int n;
+};
+
+__global__ void k_set_counts(CountsArgs a, long long* d) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+#pragma unroll
+    for (int i = 0; i < 2 * kMaxChannels; ++i)
+        if (i < 2 * a.n) d[i] = a.v[i];
+}
+
+}  // namespace
+
+bool device_counts_apply(const Context& c) {
+    if (c.dim != 1 || !c.mcc.loaded || c.species.size() > static_cast<size_t>(kMaxChannels) ||
+        c.mcc_models() > kMaxChannels)
+        return false;
+    for (int k = 0; k < c.mcc_models(); ++k)
+        if (c.mcc_model(k).secondary_species >= 0) return true;
+    return false;
+}
+
+void device_counts_enter(Context& c) {
+    const int n = static_cast<int>(c.species.size());
+    if (c.dev_counts.count < static_cast<size_t>(2 * kMaxChannels)) {
+        c.dev_counts.alloc(2 * kMaxChannels);
+        c.dev_counts_host.alloc(2 * kMaxChannels);
+    }
+    CountsArgs a{};
+    a.n = n;
+    for (int s = 0; s < n; ++s) {
+        a.v[s] = c.species[s].slots;
+        a.v[n + s] = c.species[s].next_uid;
+    }
+    k_set_counts<<<1, 64, 0, c.stream>>>(a, c.dev_counts.ptr);
+    AUR_HIP(hipGetLastError());
+    c.counts_on_device = true;
+}
+
+void device_counts_leave(Context& c) {
+    if (!c.counts_on_device) return;
+    c.counts_on_device = false;
+    const int n = static_cast<int>(c.species.size());
+    AUR_HIP(hipMemcpyAsync(c.dev_counts_host.ptr, c.dev_counts.ptr,
+                           static_cast<size_t>(2 * n) * sizeof(long long), hipMemcpyDeviceToHost,
+                           c.stream));
+    AUR_HIP(hipStreamSynchronize(c.stream));
+    for (int s = 0; s < n; ++s) {
+        c.species[s].slots = c.dev_counts_host.ptr[s];
+        c.species[s].next_uid = static_cast<uint32_t>(c.dev_counts_host.ptr[n + s]);
+    }
 }
 
 // Resolves this device's address of the collide scratch (once per device,

@@ -31,7 +31,14 @@ struct SpeciesView {
     R* v1;
     R* v2;
     int64_t slots;
+    // the live slot count on the device (1D MCC steps of aurora_hip_step,
+    // where slots is then the capacity), else null
+    const long long* nslots;
 };
+
+__device__ __forceinline__ int64_t slot_count(const int64_t slots, const long long* nslots) {
+    return nslots ? static_cast<int64_t>(*nslots) : slots;
+}
 
 struct MeshView {
     int nx, ny, nz;
@@ -389,7 +396,8 @@ __device__ __forceinline__ void push_blocks_of(const PushArgs<R>& a, int b, int 
             acc = lacc;
         }
     }
-    for (int64_t p = b * static_cast<int64_t>(kBlock) + threadIdx.x; p < sp.slots;
+    const int64_t nslot = slot_count(sp.slots, sp.nslots);
+    for (int64_t p = b * static_cast<int64_t>(kBlock) + threadIdx.x; p < nslot;
          p += static_cast<int64_t>(nb) * kBlock) {
         const int c = sp.cell[p];
         if (c < 0) continue;
@@ -506,7 +514,8 @@ __device__ __forceinline__ void work_1d_blocks(const PushArgs<R>& a, double* par
                                                int nb) {
     const SpeciesView<R>& sp = a.sp;
     double w[1] = {0.0};
-    for (int64_t p = b * static_cast<int64_t>(kBlock) + threadIdx.x; p < sp.slots;
+    const int64_t nslot = slot_count(sp.slots, sp.nslots);
+    for (int64_t p = b * static_cast<int64_t>(kBlock) + threadIdx.x; p < nslot;
          p += static_cast<int64_t>(nb) * kBlock) {
         const int c = sp.cell[p];
         if (c < 0) continue;
@@ -702,6 +711,7 @@ SpeciesView<R> view(Species& sp) {
     v.v1 = reinterpret_cast<R*>(sp.vel[1].ptr);
     v.v2 = reinterpret_cast<R*>(sp.vel[2].ptr);
     v.slots = sp.slots;
+    v.nslots = nullptr;
     return v;
 }
 
@@ -745,6 +755,10 @@ template <typename R>
 PushArgs<R> push_args(Context& c, Species& sp, double dt) {
     PushArgs<R> a{};
     a.sp = view<R>(sp);
+    if (c.counts_on_device) {
+        a.sp.slots = sp.capacity;
+        a.sp.nslots = c.dev_counts.ptr + (&sp - c.species.data());
+    }
     a.m = mesh_view(c);
     a.b = b_view(c);
     a.E32 = c.E32.ptr;
@@ -1243,7 +1257,7 @@ int push_blocks(const Context& c, int64_t slots, bool deposit) {
 template <int DIM, typename R, int BMODE>
 void launch_push_b(Context& c, Species& sp, double dt, bool deposit) {
     auto a = push_args<R>(c, sp, dt);
-    const int g = push_blocks(c, sp.slots, deposit);
+    const int g = push_blocks(c, slot_bound(c, sp), deposit);
     if (deposit) {
         KernelScope ks(c, "push_deposit");
         k_push<DIM, R, BMODE, true><<<g, kBlock, 0, c.stream>>>(a);
@@ -1256,7 +1270,8 @@ void launch_push_b(Context& c, Species& sp, double dt, bool deposit) {
 
 template <typename R, int BMODE>
 void launch_push_pair_1d_b(Context& c, Species& s0, Species& s1, double dt) {
-    const int g0 = push_blocks(c, s0.slots, true), g1 = push_blocks(c, s1.slots, true);
+    const int g0 = push_blocks(c, slot_bound(c, s0), true);
+    const int g1 = push_blocks(c, slot_bound(c, s1), true);
     KernelScope ks(c, "push_deposit");
     k_push_pair_1d<R, BMODE><<<g0 + g1, kBlock, 0, c.stream>>>(push_args<R>(c, s0, dt),
                                                                push_args<R>(c, s1, dt), g0);
@@ -1465,7 +1480,8 @@ void ledger_work_1d(Context& c, double dt) {
     if (pending.size() == 2 && pending[0]->precision == pending[1]->precision) {
         Species& s0 = *pending[0];
         Species& s1 = *pending[1];
-        const int g0 = launch_blocks(c, s0.slots), g1 = launch_blocks(c, s1.slots);
+        const int g0 = launch_blocks(c, slot_bound(c, s0));
+        const int g1 = launch_blocks(c, slot_bound(c, s1));
         double* part0 = s0.ledger_part.ptr + 3 * static_cast<int64_t>(c.cu_count) * 32;
         double* part1 = s1.ledger_part.ptr + 3 * static_cast<int64_t>(c.cu_count) * 32;
         auto launch = [&]<typename R>() {
@@ -1486,7 +1502,7 @@ void ledger_work_1d(Context& c, double dt) {
     }
     for (auto& sp : c.species) {
         if (!sp.ledger_pending) continue;
-        const int g = launch_blocks(c, sp.slots);
+        const int g = launch_blocks(c, slot_bound(c, sp));
         double* part = sp.ledger_part.ptr + 3 * static_cast<int64_t>(c.cu_count) * 32;
         dispatch_dim_prec(c, sp, [&]<int DIM, typename R>() {
             if constexpr (DIM == 1) {
@@ -1528,8 +1544,8 @@ void push_all(Context& c, double dt, bool fuse_deposit, bool defer_ledger) {
                 reinterpret_cast<long long*>(sp.acc.ptr), c.nnodes);
         }
     }
-    if (c.dim == 1 && fuse_deposit && c.species.size() == 2 && c.species[0].slots > 0 &&
-        c.species[1].slots > 0 && c.species[0].precision == c.species[1].precision) {
+    if (c.dim == 1 && fuse_deposit && c.species.size() == 2 && slot_bound(c, c.species[0]) > 0 &&
+        slot_bound(c, c.species[1]) > 0 && c.species[0].precision == c.species[1].precision) {
         // the usual pair (electrons and ions): one launch
         Species& s0 = c.species[0];
         Species& s1 = c.species[1];
@@ -1540,14 +1556,14 @@ void push_all(Context& c, double dt, bool fuse_deposit, bool defer_ledger) {
         if (s0.precision == 64) launch_push_pair_1d<double>(c, s0, s1, dt);
         else launch_push_pair_1d<float>(c, s0, s1, dt);
         for (Species* sp : {&s0, &s1}) {
-            sp->ledger_push_blocks = push_blocks(c, sp->slots, true);
+            sp->ledger_push_blocks = push_blocks(c, slot_bound(c, *sp), true);
             sp->ledger_pending = true;
             if (!defer_ledger) fold_ledger_1d(c, *sp, nullptr, 0);
         }
         return;
     }
     for (auto& sp : c.species) {
-        if (sp.slots == 0) continue;
+        if (slot_bound(c, sp) == 0) continue;
         if (c.dim == 1) {
             ensure_ledger_1d(c, sp);
             // a push whose partials were never folded (push without a step)
@@ -1571,7 +1587,7 @@ void push_all(Context& c, double dt, bool fuse_deposit, bool defer_ledger) {
             launch_push<DIM, R>(c, sp, dt, fuse_deposit);
         });
         if (c.dim == 1) {
-            sp.ledger_push_blocks = push_blocks(c, sp.slots, fuse_deposit);
+            sp.ledger_push_blocks = push_blocks(c, slot_bound(c, sp), fuse_deposit);
             sp.ledger_pending = true;
             if (!defer_ledger) fold_ledger_1d(c, sp, nullptr, 0);
         }

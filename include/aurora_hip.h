@@ -182,7 +182,12 @@ int aurora_hip_initialize(aurora_hip_ctx* ctx, double dt);
  * Simulation2D.cpp:1337-1364, Simulation.cpp:1008-1138): fused push and
  * deposit, field solve at t+dt, MCC where a model is loaded.  The
  * diagnostic synchronisation is computed on demand instead of every step
- * (aurora_hip_species_velocity, aurora_hip_energy). */
+ * (aurora_hip_species_velocity, aurora_hip_energy).  On 1D meshes with an
+ * MCC model that produces ionization products, the slot counts stay on the
+ * device for the call: products are reserved and placed there, and an
+ * overflow of a species' capacity (all models' products checked at once;
+ * that step places none) returns AURORA_ERR_CAPACITY at the end of the
+ * call, as the other errors the device detects do. */
 int aurora_hip_step(aurora_hip_ctx* ctx, double dt, int64_t nsteps);
 /* Simulation3D::sample (Simulation3D.cpp:551-579) and its 1D/2D
  * counterparts (Diagnostics.cpp:37-59, 114-140), FP64: kinetic energy per

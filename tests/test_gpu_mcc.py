@@ -441,3 +441,46 @@ def test_1d_step_refuses_attachment():
             c.step(1e-12, 1)
     finally:
         c.close()
+
+
+def test_1d_step_capacity_error_with_counts_on_device():
+    """A 1D MCC step whose products overflow max_particles_per_species fails
+    with the reference's message (Simulation.cpp:929-964) when the slot
+    counts are kept on the device for the call: the device-side commit
+    checks every model's products at once and places none of that step's,
+    and the host's counts come back within the capacity.  With room, the
+    same run places one secondary electron and one ion per ionization."""
+    from aurora_pic import hip
+    cfg = workloads.config_discharge()
+    hot = math.sqrt(100.0 * workloads.EV / workloads.ME)   # ionizing electrons
+
+    def run(extra, steps):
+        c = hip.Context(0)
+        try:
+            c.mesh([65], cfg["L"], hip.DIRICHLET, cfg["eps"])
+            n = 4000
+            se = c.add_species(-workloads.EV, workloads.ME, 1e6, n + extra)
+            si = c.add_species(workloads.EV, 6.67e-27, 1e6, n + extra)
+            c.init_maxwellian(se, n, hot, seed=11)
+            c.init_maxwellian(si, n, workloads.maxwellian_sigma(0.026, 6.67e-27), seed=12)
+            c.mcc_load(se, cfg["gas"], product_species=(se, si), seed=13)
+            c.initialize(cfg["dt"])
+            err = None
+            try:
+                c.step(cfg["dt"], steps)
+            except hip.HipError as e:
+                err = e
+            counts = [c.count(s) for s in (se, si)]
+            stats = c.mcc_stats()
+            return err, counts, stats, n
+        finally:
+            c.close()
+
+    err, counts, stats, n = run(2, 400)
+    assert err is not None and "reactive collisions exceeded max_particles_per_species" in str(err)
+    assert all(slots <= n + 2 for slots, _ in counts)
+    err, counts, stats, n = run(100000, 200)
+    assert err is None
+    ionizations = int(stats[0][2 + 2])
+    assert ionizations > 0
+    assert counts[0][0] == n + ionizations and counts[1][0] == n + ionizations
