@@ -484,3 +484,44 @@ def test_1d_step_capacity_error_with_counts_on_device():
     ionizations = int(stats[0][2 + 2])
     assert ionizations > 0
     assert counts[0][0] == n + ionizations and counts[1][0] == n + ionizations
+
+
+def test_1d_mcc_steps_with_sorts_inside_one_call():
+    """Sorts inside one aurora_hip_step call (the slot counts leave the
+    device for the sort and return after it) give the particles, counts and
+    collision statistics of the same steps taken one call at a time."""
+    from aurora_pic import hip
+    cfg = workloads.config_discharge()
+    hot = math.sqrt(30.0 * workloads.EV / workloads.ME)
+
+    def run(calls):
+        c = hip.Context(0)
+        try:
+            c.mesh([65], cfg["L"], hip.DIRICHLET, cfg["eps"])
+            n = 6000
+            se = c.add_species(-workloads.EV, workloads.ME, 1e6, 4 * n)
+            si = c.add_species(workloads.EV, 6.67e-27, 1e6, 4 * n)
+            c.init_maxwellian(se, n, hot, seed=21)
+            c.init_maxwellian(si, n, workloads.maxwellian_sigma(0.026, 6.67e-27), seed=22)
+            c.mcc_load(se, cfg["gas"], product_species=(se, si), seed=23)
+            c.initialize(cfg["dt"])
+            c.sort_interval(7)
+            for k in calls:
+                c.step(cfg["dt"], k)
+            out = []
+            for s in (se, si):
+                p, v, a, u = c.download(s)
+                o = np.argsort(u)
+                out.append((c.count(s), p[o], v[o], a[o], u[o]))
+            return out, c.mcc_stats()[0]
+        finally:
+            c.close()
+
+    one, stats_one = run([30])
+    many, stats_many = run([1] * 30)
+    assert np.array_equal(stats_one, stats_many)
+    assert stats_one[2 + 2] > 0          # ionizations happened
+    for a, b in zip(one, many):
+        assert a[0] == b[0]
+        for x, y in zip(a[1:], b[1:]):
+            assert np.array_equal(x, y)
