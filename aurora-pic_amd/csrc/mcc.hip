@@ -88,8 +88,11 @@ struct Mcc1dArgs {
 // sync -> collide -> rebuild v_half for every live target particle
 // (with products: each 256-slot chunk's event counts scanned in the block,
 // so a parent's product index is its chunk's offset plus its own prefix)
-template <int NCH>
-__global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, double* rows) {
+// ONE_V: the 1D1V scalar collide (a.one_v), a separate build so that the
+// 3V kernel carries no registers of the other path
+template <int NCH, bool ONE_V>
+__global__ void __launch_bounds__(kBlk, mcc_min_waves(NCH))
+    k_mcc_1d(MccParams m, Mcc1dArgs a, double* rows) {
     __shared__ MccLds lds;
     __shared__ int wsum[kBlk / 64];
     const XsView X = stage_tables(m, lds);
@@ -106,7 +109,7 @@ __global__ void __launch_bounds__(kBlk) k_mcc_1d(MccParams m, Mcc1dArgs a, doubl
         if (active)
             np = mcc_1d_particle<NCH>(m, X, lds.chan, c, a.off[p], a.v0 + p, a.v1 + p, a.v2 + p,
                                       a.uid[p], a.E32, a.nx, a.half_qm_dt, a.dt, a.seed, a.step,
-                                      a.prod + 12 * p, t, a.err, a.stream, a.one_v != 0);
+                                      a.prod + 12 * p, t, a.err, a.stream, ONE_V);
         if (in) a.nprod[p] = np;
         flush_tally_rows<NCH>(t, active, w);
         if (a.chunk_tot) {
@@ -454,18 +457,22 @@ void mcc_collide_pass(Context& c, int model, double dt) {
         // one round of resident workgroups (grid-stride; a partial second
         // round would leave most of the chip idle at its end), which also
         // keeps the tally rows few for the one-block fold that follows
-        static const int per_cu = [] {
+        auto occupancy = [](auto kernel) {
             int v = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_mcc_1d<NCH>, kBlk, 0) !=
-                hipSuccess)
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, kBlk, 0) != hipSuccess)
                 v = 1;
             return std::max(1, v);
-        }();
-        const int g = std::min(blocks(c, n), per_cu * c.cu_count);
+        };
+        static const int per_cu[2] = {occupancy(k_mcc_1d<NCH, false>),
+                                      occupancy(k_mcc_1d<NCH, true>)};
+        const int g = std::min(blocks(c, n), per_cu[a.one_v ? 1 : 0] * c.cu_count);
         const int64_t rows = g;   // one per workgroup
         if (h.energy_rows.count < static_cast<size_t>(rows * kTallyStride))
             h.energy_rows.alloc(static_cast<size_t>(rows * kTallyStride));
-        k_mcc_1d<NCH><<<g, kBlk, 0, c.stream>>>(m, a, h.energy_rows.ptr);
+        if (a.one_v)
+            k_mcc_1d<NCH, true><<<g, kBlk, 0, c.stream>>>(m, a, h.energy_rows.ptr);
+        else
+            k_mcc_1d<NCH, false><<<g, kBlk, 0, c.stream>>>(m, a, h.energy_rows.ptr);
         k_mcc_finish<NCH><<<1, kBlk, 0, c.stream>>>(h.energy_rows.ptr, rows,
                                                     h.energy_change.ptr, h.counts.ptr,
                                                     a.chunk_tot, h.chunk_off.ptr, nchunks, nslots,
