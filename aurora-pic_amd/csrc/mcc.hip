@@ -122,72 +122,42 @@ __global__ void __launch_bounds__(kBlk, mcc_min_waves(NCH))
     store_block_row<NCH, kBlk>(w, rows, blockIdx.x);
 }
 
-// After one model's k_mcc_1d: its tally rows added in row order (as
-// k_fold_rows), then, with products, the chunk totals scanned into chunk
-// offsets and the model's product count written to *total.
-template <int NCH>
-__global__ void __launch_bounds__(kBlk) k_mcc_finish(const double* rows, int64_t nrows,
-                                                     double* energy, unsigned long long* counts,
-                                                     const int32_t* chunk_tot, int32_t* chunk_off,
-                                                     int64_t nchunks, const long long* nslots,
-                                                     long long* total) {
-    fold_tally_rows<NCH, kBlk>(rows, nrows, energy, counts);
-    if (!chunk_tot) {
-        if (threadIdx.x == 0) *total = 0;
-        return;
-    }
-    // the chunks the collide wrote (its slot count on the device, if there)
-    if (nslots) nchunks = (static_cast<int64_t>(*nslots) + kBlk - 1) / kBlk;
-    __shared__ int wsum[kBlk / 64];
-    const int all = chunk_offsets<kBlk>(chunk_tot, chunk_off, nchunks, wsum);
-    if (threadIdx.x == 0) *total = all;
-}
-
-// Adds the tally rows of one launch, in row order, to the energy change
-// [NCH] and the counters [NCH + 3].
-template <int NCH>
-__global__ void __launch_bounds__(kBlk) k_fold_rows(const double* rows, int64_t nrows,
-                                                    double* energy, unsigned long long* counts) {
-    fold_tally_rows<NCH, kBlk>(rows, nrows, energy, counts);
-}
-
-
-__global__ void k_zero_total(long long* total) { *total = 0; }
-
 // All models' products reserved together on the device (the counts kept
 // there during aurora_hip_step): the all-or-nothing capacity check over
 // every model (Simulation.cpp:929-964), then slots and uids per model in
-// model order, as mcc_append reserves them on the host.  One thread.
+// model order, as mcc_append reserves them on the host.  Run by one thread
+// (the last model's k_mcc_finish, after every model's total is known).
 struct CommitArgs {
     int models, nspecies;
     int target[kMaxChannels], sec[kMaxChannels], ion[kMaxChannels];
     long long capacity[kMaxChannels];
+    long long* counts;   // [species] slots, then [species] next uid
+    long long* place;    // per model: kPlaceFields values
+    const long long* total;
+    uint32_t* err;
 };
 
-__global__ void k_mcc_commit(CommitArgs a, const long long* total, long long* counts,
-                             long long* place, uint32_t* err) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    long long* slots = counts;
-    long long* uid = counts + a.nspecies;
-    __shared__ long long need[kMaxChannels];
+__device__ void commit_products(const CommitArgs& a, long long* need) {
+    long long* slots = a.counts;
+    long long* uid = a.counts + a.nspecies;
     for (int s = 0; s < kMaxChannels; ++s) need[s] = 0;
     for (int k = 0; k < a.models; ++k) {
-        long long* pk = place + static_cast<int64_t>(k) * kPlaceFields;
+        long long* pk = a.place + static_cast<int64_t>(k) * kPlaceFields;
         pk[4] = slots[a.target[k]];   // the parents of the pass
         pk[5] = 0;
-        if (a.sec[k] < 0 || total[k] == 0) continue;
-        need[a.sec[k]] += total[k];
-        need[a.ion[k]] += total[k];
+        if (a.sec[k] < 0 || a.total[k] == 0) continue;
+        need[a.sec[k]] += a.total[k];
+        need[a.ion[k]] += a.total[k];
     }
     for (int s = 0; s < a.nspecies; ++s)
         if (need[s] && slots[s] + need[s] > a.capacity[s]) {
-            atomicOr(err, DEV_ERR_CAPACITY);
+            atomicOr(a.err, DEV_ERR_CAPACITY);
             return;
         }
     for (int k = 0; k < a.models; ++k) {
-        const long long t = total[k];
+        const long long t = a.total[k];
         if (a.sec[k] < 0 || t == 0) continue;
-        long long* pk = place + static_cast<int64_t>(k) * kPlaceFields;
+        long long* pk = a.place + static_cast<int64_t>(k) * kPlaceFields;
         const int se = a.sec[k], si = a.ion[k];
         pk[0] = slots[se];
         pk[2] = uid[se];
@@ -208,9 +178,52 @@ __global__ void k_mcc_commit(CommitArgs a, const long long* total, long long* co
     }
 }
 
+// After one model's k_mcc_1d: its tally rows added in row order (as
+// k_fold_rows), then, with products, the chunk totals scanned into chunk
+// offsets and the model's product count written to *total; with commit
+// (the last model while the counts are on the device), every model's
+// products reserved.
+template <int NCH>
+__global__ void __launch_bounds__(kBlk) k_mcc_finish(const double* rows, int64_t nrows,
+                                                     double* energy, unsigned long long* counts,
+                                                     const int32_t* chunk_tot, int32_t* chunk_off,
+                                                     int64_t nchunks, const long long* nslots,
+                                                     long long* total, CommitArgs commit) {
+    fold_tally_rows<NCH, kBlk>(rows, nrows, energy, counts);
+    __shared__ int wsum[kBlk / 64];
+    __shared__ long long need[kMaxChannels];
+    int all = 0;
+    if (chunk_tot) {
+        // the chunks the collide wrote (its slot count on the device, if there)
+        if (nslots) nchunks = (static_cast<int64_t>(*nslots) + kBlk - 1) / kBlk;
+        all = chunk_offsets<kBlk>(chunk_tot, chunk_off, nchunks, wsum);
+    }
+    if (threadIdx.x == 0) {
+        *total = all;
+        if (commit.models > 0) commit_products(commit, need);
+    }
+}
+
+// Adds the tally rows of one launch, in row order, to the energy change
+// [NCH] and the counters [NCH + 3].
+template <int NCH>
+__global__ void __launch_bounds__(kBlk) k_fold_rows(const double* rows, int64_t nrows,
+                                                    double* energy, unsigned long long* counts) {
+    fold_tally_rows<NCH, kBlk>(rows, nrows, energy, counts);
+}
+
+
+__global__ void k_zero_total(long long* total) { *total = 0; }
+
+// the reservation alone (a last model whose target holds no slots)
+__global__ void k_mcc_commit(CommitArgs a) {
+    __shared__ long long need[kMaxChannels];
+    if (blockIdx.x == 0 && threadIdx.x == 0) commit_products(a, need);
+}
+
 // write the products of parent p at product index chunk_off[p / kBlk] +
 // loff[p] + k (grid-stride over 256-slot chunks); with place (device
-// counts), the parents and the bases come from k_mcc_commit's placement
+// counts), the parents and the bases come from commit_products' placement
 __global__ void __launch_bounds__(kBlk) k_append_products(
     const int32_t* pcell, const float* poff, const int32_t* nprod, const int32_t* loff,
     const int32_t* chunk_off, const float* prod, int64_t parents, const float* E32, int nx,
@@ -403,8 +416,9 @@ namespace {
 // One model's collide pass over its target species (Simulation.cpp:835-927):
 // synchronise, collide, rebuild v_half, stage the products of each parent
 // and write the model's product count to product_total[model] on the
-// device (0 without products).
-void mcc_collide_pass(Context& c, int model, double dt) {
+// device (0 without products); commit (models > 0): the reservation of
+// every model's products, run after this model's total.
+void mcc_collide_pass(Context& c, int model, double dt, const CommitArgs& commit) {
     MccModelHost& h = c.mcc_model(model);
     Species& sp = c.species[h.target];
     AUR_REQUIRE(sp.precision == 32, AURORA_ERR_UNSUPPORTED, "MCC targets FP32 species");
@@ -415,6 +429,7 @@ void mcc_collide_pass(Context& c, int model, double dt) {
     long long* total = c.product_total.ptr + model;
     if (n == 0) {
         k_zero_total<<<1, 1, 0, c.stream>>>(total);
+        if (commit.models > 0) k_mcc_commit<<<1, 64, 0, c.stream>>>(commit);
         AUR_HIP(hipGetLastError());
         return;
     }
@@ -476,14 +491,14 @@ void mcc_collide_pass(Context& c, int model, double dt) {
         k_mcc_finish<NCH><<<1, kBlk, 0, c.stream>>>(h.energy_rows.ptr, rows,
                                                     h.energy_change.ptr, h.counts.ptr,
                                                     a.chunk_tot, h.chunk_off.ptr, nchunks, nslots,
-                                                    total);
+                                                    total, commit);
     });
     AUR_HIP(hipGetLastError());
 }
 
 // Places the `total` product pairs staged by one model's pass after the
 // live particles of the secondary and ion species (Simulation.cpp:965-1005).
-// place: the model's placement reserved on the device by k_mcc_commit (the
+// place: the model's placement reserved on the device by commit_products (the
 // counts on the device; total and the host's counts are then unused).
 void mcc_append(Context& c, int model, int64_t total, double dt, const long long* place) {
     MccModelHost& h = c.mcc_model(model);
@@ -566,14 +581,11 @@ void mcc_step(Context& c, double dt) {
         c.product_total_models = models;
     }
     bool products = false;
-    for (int k = 0; k < models; ++k) {
-        mcc_collide_pass(c, k, dt);
-        products = products || c.mcc_model(k).secondary_species >= 0;
-    }
-    if (!products) return;
-    if (c.counts_on_device) {
-        // reserve on the device, place, no host wait
-        CommitArgs ca{};
+    for (int k = 0; k < models; ++k) products = products || c.mcc_model(k).secondary_species >= 0;
+    // with the counts on the device: the last model's finishing kernel
+    // reserves every model's products (no host wait)
+    CommitArgs ca{};
+    if (products && c.counts_on_device) {
         ca.models = models;
         ca.nspecies = static_cast<int>(c.species.size());
         for (int k = 0; k < models; ++k) {
@@ -585,9 +597,15 @@ void mcc_step(Context& c, double dt) {
         for (int s = 0; s < ca.nspecies; ++s) ca.capacity[s] = c.species[s].capacity;
         const size_t pn = static_cast<size_t>(models) * kPlaceFields;
         if (c.mcc_place.count < pn) c.mcc_place.alloc(pn);
-        k_mcc_commit<<<1, 64, 0, c.stream>>>(ca, c.product_total.ptr, c.dev_counts.ptr,
-                                            c.mcc_place.ptr, c.dev_error.ptr);
-        AUR_HIP(hipGetLastError());
+        ca.counts = c.dev_counts.ptr;
+        ca.place = c.mcc_place.ptr;
+        ca.total = c.product_total.ptr;
+        ca.err = c.dev_error.ptr;
+    }
+    const CommitArgs none{};
+    for (int k = 0; k < models; ++k) mcc_collide_pass(c, k, dt, k == models - 1 ? ca : none);
+    if (!products) return;
+    if (c.counts_on_device) {
         for (int k = 0; k < models; ++k)
             if (c.mcc_model(k).secondary_species >= 0)
                 mcc_append(c, k, 0, dt, c.mcc_place.ptr + static_cast<int64_t>(k) * kPlaceFields);
