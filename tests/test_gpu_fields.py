@@ -714,3 +714,44 @@ def test_local_sort_when_every_particle_dies():
         c.push(0.01)
         c.sort()
         assert c.count(sid) == (0, 0)
+
+
+def test_autotune_picks_a_refined_candidate():
+    """The autotuner times every candidate, re-times the fastest over a
+    longer window and leaves the best of those configured; the tuned
+    context's steps equal an untiled context's bit for bit."""
+    from aurora_pic import hip
+    from aurora_pic import autotune
+    rng = np.random.default_rng(31)
+    n, L = [32, 32, 32], [1.0, 1.0, 1.0]
+    N = 32 ** 3 * 4
+    pos = rng.uniform(0, 1, (N, 3))
+    vel = rng.normal(0, 0.3, (N, 3))
+    shapes = [(8, 8, 8, 2, 512, 0), (8, 8, 4, 2, 512, 1), (8, 8, 4, 2, 512, 5)]
+    results = []
+    for tiled in (True, False):
+        with hip.Context(0) as c:
+            c.mesh(n, L, 0, 1.0)
+            sid = c.add_species(-1.0, 1.0, 1.0 / N, N)
+            c.upload(sid, pos, vel, kind=1)
+            c.initialize(0.01)
+            if tiled:
+                logs = []
+                best, table = autotune.tune(c, 0.01, shapes=shapes, steps=4, refine=2,
+                                            adaptive=True, log=logs.append)
+                assert len(table) == 3 and best[1] in shapes
+                assert sum("refined" in line for line in logs) == 2
+                assert best in sorted(table)[:3]
+                steps_done = 3 * 5 + 2 * 11
+            else:
+                c.tiling(0)
+                c.step(0.01, 3 * 5 + 2 * 11)
+                steps_done = 3 * 5 + 2 * 11
+            c.step(0.01, 5)
+            p_, v_, a_, u_ = c.download(sid)
+            o = np.argsort(u_)
+            results.append((steps_done, c.charge_fixed(sid), p_[o], v_[o]))
+    assert results[0][0] == results[1][0]
+    assert np.array_equal(results[0][1], results[1][1])
+    assert np.array_equal(results[0][2], results[1][2])
+    assert np.array_equal(results[0][3], results[1][3])
