@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(kEnsThreads) k_ens_push(EnsArgs a) {
 // rho from the fixed-point sums (clearing them), then the field solve at
 // solve index (*step + solve_offset): one workgroup per instance
 __global__ void __launch_bounds__(kEnsThreads) k_ens_field(EnsArgs a, int solve_offset) {
-    extern __shared__ double sh[];
+    extern __shared__ __align__(16) double sh[];
     __shared__ double mean_s;
     const int inst = blockIdx.x;
     const int64_t o = static_cast<int64_t>(inst) * a.nx;

@@ -172,6 +172,79 @@ __device__ __forceinline__ void spectral_sums_1d(double* x, double* pr, double* 
     }
 }
 
+// Radix-2 complex FFT of n = 2^log2n points in LDS (z in bit-reversed
+// order on entry, natural order on exit): sign -1 sums z_j e^(-2 pi i jk/n),
+// sign +1 z_j e^(+2 pi i jk/n), with cos/sin(2 pi m / n) from cs, sn.
+__device__ __forceinline__ void fft_radix2_1d(double2* z, int n, int log2n, const double* cs,
+                                              const double* sn, double sign) {
+#pragma clang fp contract(off)
+    for (int lh = 0; lh < log2n; ++lh) {
+        const int half = 1 << lh;
+        const int tshift = log2n - lh - 1;   // twiddle index stride n / (2 half)
+        for (int b = threadIdx.x; b < n / 2; b += blockDim.x) {
+            const int k = b & (half - 1);
+            const int i0 = ((b >> lh) << (lh + 1)) + k, i1 = i0 + half;
+            const int m = k << tshift;
+            const double wr = cs[m], wi = sign * sn[m];
+            const double2 a = z[i0], c = z[i1];
+            const double tr = c.x * wr - c.y * wi;
+            const double ti = c.x * wi + c.y * wr;
+            z[i0] = make_double2(a.x + tr, a.y + ti);
+            z[i1] = make_double2(a.x - tr, a.y - ti);
+        }
+        __syncthreads();
+    }
+}
+
+// The same solve on a power-of-two mesh through FFTs: the forward
+// transform of rho - mean, phi_k and E_k = -i wave phi_k per mode, and one
+// inverse transform of phi_k + i E_k, whose real part is phi and imaginary
+// part E (both sums are real; the Nyquist mode's E term has no real part
+// and is left out, as the direct sums' real part leaves it).  x holds
+// rho - mean; z is 2 n doubles of LDS.
+__device__ __forceinline__ void spectral_fft_1d(const double* x, double2* z, const double* cs,
+                                                const double* sn, double* phi, double* E,
+                                                float* E32, int n, int log2n, double L,
+                                                double eps) {
+#pragma clang fp contract(off)
+    const double twopi = 6.283185307179586476925286766559;
+    for (int j = threadIdx.x; j < n; j += blockDim.x)
+        z[__brev(static_cast<unsigned>(j)) >> (32 - log2n)] = make_double2(x[j], 0.0);
+    __syncthreads();
+    fft_radix2_1d(z, n, log2n, cs, sn, -1.0);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        double2 zk = make_double2(0.0, 0.0);
+        if (k > 0) {
+            const double rr = z[k].x / static_cast<double>(n);
+            const double ri = z[k].y / static_cast<double>(n);
+            const double kk = (k <= n / 2) ? static_cast<double>(k) : -static_cast<double>(n - k);
+            const double wave = twopi * kk / L;
+            const double den = eps * wave * wave;
+            const double pr = rr / den, pim = ri / den;
+            // + i E_k = i (wave pim - i wave pr) = wave pr + i wave pim
+            zk = (2 * k == n) ? make_double2(pr, pim)
+                              : make_double2(pr + wave * pr, pim + wave * pim);
+        }
+        z[k] = zk;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int r = static_cast<int>(__brev(static_cast<unsigned>(k)) >> (32 - log2n));
+        if (k < r) {
+            const double2 t = z[k];
+            z[k] = z[r];
+            z[r] = t;
+        }
+    }
+    __syncthreads();
+    fft_radix2_1d(z, n, log2n, cs, sn, 1.0);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        phi[j] = z[j].x;
+        E[j] = z[j].y;
+        E32[j] = static_cast<float>(z[j].y);
+    }
+}
+
 __device__ inline void spectral_solve_1d(const double* rho, double* phi, double* E, float* E32,
                                   const double* cs, const double* sn, int n, double L, double eps,
                                   double* sh, double* mean_s) {
@@ -197,6 +270,16 @@ __device__ inline void spectral_solve_1d(const double* rho, double* phi, double*
     const double mean = *mean_s;
     for (int j = threadIdx.x; j < n; j += blockDim.x) x[j] = x[j] - mean;
     __syncthreads();
+    if (n >= 4 && (n & (n - 1)) == 0) {
+        int log2n = 0;
+        while ((1 << log2n) < n) ++log2n;
+        double2* z = reinterpret_cast<double2*>(sh + n);   // pr and pim's 2 n doubles
+        if (lds_tables)
+            spectral_fft_1d(x, z, sh + 3 * n, sh + 4 * n, phi, E, E32, n, log2n, L, eps);
+        else
+            spectral_fft_1d(x, z, cs, sn, phi, E, E32, n, log2n, L, eps);
+        return;
+    }
     if (lds_tables)
         spectral_sums_1d(x, pr, pim, sh + 3 * n, sh + 4 * n, phi, E, E32, n, L, eps);
     else

@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(1024) k_spectral_1d(const double* rho, double*
                                                       float* E32, const double* cs,
                                                       const double* sn, int n, double L,
                                                       double eps) {
-    extern __shared__ double sh[];
+    extern __shared__ __align__(16) double sh[];
     __shared__ double mean_s;
     spectral_solve_1d(rho, phi, E, E32, cs, sn, n, L, eps, sh, &mean_s);
 }
@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(1024) k_spectral_1d_acc(RhoArgs ra, ClearArgs 
                                                           double* phi, double* E, float* E32,
                                                           const double* cs, const double* sn,
                                                           int n, double L, double eps) {
-    extern __shared__ double sh[];
+    extern __shared__ __align__(16) double sh[];
     __shared__ double mean_s;
     rho_from_acc_1d(ra, cl, rho, n);
     spectral_solve_1d(rho, phi, E, E32, cs, sn, n, L, eps, sh, &mean_s);

@@ -755,3 +755,22 @@ def test_autotune_picks_a_refined_candidate():
     assert np.array_equal(results[0][1], results[1][1])
     assert np.array_equal(results[0][2], results[1][2])
     assert np.array_equal(results[0][3], results[1][3])
+
+
+@pytest.mark.parametrize("n", [4, 96, 100, 256, 1024, 2048])
+def test_spectral_1d_solve_sizes(hip_ctx, n):
+    """The 1D periodic solve on power-of-two meshes (radix-2 FFTs in LDS,
+    tables in LDS up to 1024 nodes) and on others (direct sums), against
+    the reference's formula evaluated with numpy's FFT: phi and E within
+    1e-11 relative."""
+    print(f"n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    L = [2.7]
+    rho = rng.normal(size=n) + 0.3 * np.sin(np.linspace(0, 5, n))
+    hip_ctx.mesh([n], L, 0, 1.3)
+    hip_ctx.set_rho(rho)
+    hip_ctx.solve(0.0, 0.0, True)
+    _, phi, E = hip_ctx.fields()
+    phi_ref, E_ref = _spectral_reference(rho, [n], L, 1.3)
+    assert np.max(np.abs(phi - phi_ref)) <= 1e-11 * np.max(np.abs(phi_ref))
+    assert np.max(np.abs(E[0] - E_ref[0])) <= 1e-11 * np.max(np.abs(E_ref))
