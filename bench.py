@@ -422,7 +422,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the per-step loop of one 3D3V instance runs on a "
-              "single MI355X; multi-GPU sharding is a design note only (see DESIGN.md).",
+              "single MI355X; sharding across GPUs is not built.",
               file=sys.stderr)
         sys.exit(2)
 
