@@ -165,46 +165,35 @@ __device__ __forceinline__ void green_mode(const GreenArgs& g, int ix, int iy, i
     Bv = g.want_phi ? make_double2(ezr - pi, ezi + pr) : make_double2(ezr, ezi);
 }
 
-// green_mode's spectral step with the fields kept apart, in two parts: the
-// division phi_hat = rho_hat / (eps k^2) once per mode (green_phi, zero for
-// the mean mode), then each field from phi_hat (green_field: F = 0 E_x,
-// 1 E_y, 2 E_z, 3 phi; zero for the mean mode, E_z zero outside 3D, the
-// own-axis Nyquist plane removed) -- green_mode's operations, so its values.
-__device__ __forceinline__ double2 green_phi(const GreenArgs& g, int ix, int iy, int iz,
-                                             double2 r) {
+// green_mode's spectral step with the fields kept apart: E_x, E_y, E_z and
+// phi of the mode (E_z zero outside 3D); ex + i ey and ez + i ph are
+// green_mode's A and Bv.  Returned by value: no local's address is taken,
+// so every value stays in registers.
+struct ModeFields {
+    double2 ex, ey, ez, ph;
+};
+
+__device__ __forceinline__ ModeFields green_split(const GreenArgs& g, int ix, int iy, int iz,
+                                                  double2 r) {
 #pragma clang fp contract(off)
-    if (ix == 0 && iy == 0 && iz == 0) return make_double2(0.0, 0.0);
+    ModeFields f;
+    f.ex = f.ey = f.ez = f.ph = make_double2(0.0, 0.0);
+    if (ix == 0 && iy == 0 && iz == 0) return f;
     const double kx = g.twopi_over_L[0] * mode_of(ix, g.nx);
     const double ky = g.dim > 1 ? g.twopi_over_L[1] * mode_of(iy, g.ny) : 0.0;
     const double kz = g.dim > 2 ? g.twopi_over_L[2] * mode_of(iz, g.nz) : 0.0;
     double k2 = kx * kx + ky * ky;
     if (g.dim > 2) k2 += kz * kz;
     const double den = g.eps * k2;
-    return make_double2(r.x / den, r.y / den);
-}
-
-template <int F>
-__device__ __forceinline__ double2 green_field(const GreenArgs& g, int ix, int iy, int iz,
-                                               double2 ph) {
-#pragma clang fp contract(off)
-    const double2 zero = make_double2(0.0, 0.0);
-    if (ix == 0 && iy == 0 && iz == 0) return zero;
-    const double pr = ph.x, pi = ph.y;
-    if constexpr (F == 0) {
-        const double kx = g.twopi_over_L[0] * mode_of(ix, g.nx);
-        const bool nyx = (g.nx % 2 == 0) && ix == g.nx / 2;
-        return nyx ? zero : make_double2(kx * pi, -kx * pr);
-    } else if constexpr (F == 1) {
-        const double ky = g.dim > 1 ? g.twopi_over_L[1] * mode_of(iy, g.ny) : 0.0;
-        const bool nyy = (g.ny % 2 == 0) && iy == g.ny / 2;
-        return nyy ? zero : make_double2(ky * pi, -ky * pr);
-    } else if constexpr (F == 2) {
-        const double kz = g.dim > 2 ? g.twopi_over_L[2] * mode_of(iz, g.nz) : 0.0;
-        const bool nyz = (g.nz % 2 == 0) && iz == g.nz / 2;
-        return g.dim > 2 && !nyz ? make_double2(kz * pi, -kz * pr) : zero;
-    } else {
-        return ph;
-    }
+    const double pr = r.x / den, pi = r.y / den;
+    const bool nyx = (g.nx % 2 == 0) && ix == g.nx / 2;
+    const bool nyy = (g.ny % 2 == 0) && iy == g.ny / 2;
+    const bool nyz = (g.nz % 2 == 0) && iz == g.nz / 2;
+    if (!nyx) f.ex = make_double2(kx * pi, -kx * pr);
+    if (!nyy) f.ey = make_double2(ky * pi, -ky * pr);
+    if (g.dim > 2 && !nyz) f.ez = make_double2(kz * pi, -kz * pr);
+    f.ph = make_double2(pr, pi);
+    return f;
 }
 
 // fa: rho_hat in, (Ex + i Ey)_hat out; fb: (Ez + i phi)_hat out
@@ -361,14 +350,16 @@ template <int LOG2N>
 constexpr int kLds = (1 << lines_shift(kLastLds, 1 << LOG2N)) * Lines<LOG2N>::LS;
 
 // The radix-2 stages over `lines` lines of N (bit-reversed order in LDS),
-// two stages per pass: a thread loads the 4 elements i0 + k*2^S that
-// stages S and S+1 combine, runs those butterflies in registers (the same
+// R stages per pass: a thread loads the 2^R elements i0 + k*2^S that
+// stages S .. S+R-1 combine, runs those butterflies in registers (the same
 // twiddles, products and sums as one stage at a time, so the result is the
-// same to the bit) and writes them back -- ceil(LOG2N / 2) LDS round trips
-// and barriers instead of LOG2N.  Two stages rather than three: a block's
-// B * N / 4 = 256 groups keep all 256 threads busy (8-element groups left
-// half of them idle), and the 4-element registers let a CU hold a full
-// round of the fused kernels' blocks (40 VGPRs against 67 at N = 128).
+// same to the bit) and writes them back -- ceil(LOG2N / R) LDS round trips
+// and barriers instead of LOG2N.  The forward x kernels take two stages
+// per pass: a block's B * N / 4 = 256 groups keep all 256 threads busy
+// (8-element groups leave half of them idle) and at 40 VGPRs (67 with
+// three stages, N = 128) a CU holds 8 blocks, the whole 128^3 grid in one
+// round.  The y, last-axis and inverse x kernels keep three (fewer passes
+// measured faster there).
 template <int LOG2N, int S, int R>
 __device__ __forceinline__ void radix_pass(double2* lds, int lines, const double2* s_tw,
                                            bool inverse) {
@@ -404,21 +395,22 @@ __device__ __forceinline__ void radix_pass(double2* lds, int lines, const double
     }
 }
 
-template <int LOG2N, int S>
+template <int LOG2N, int S, int MAXR>
 __device__ __forceinline__ void radix_passes(double2* lds, int lines, const double2* s_tw,
                                              bool inverse) {
     if constexpr (S < LOG2N) {
-        constexpr int R = LOG2N - S >= 2 ? 2 : LOG2N - S;
+        constexpr int R = LOG2N - S >= MAXR ? MAXR : LOG2N - S;
         radix_pass<LOG2N, S, R>(lds, lines, s_tw, inverse);
         __syncthreads();
-        radix_passes<LOG2N, S + R>(lds, lines, s_tw, inverse);
+        radix_passes<LOG2N, S + R, MAXR>(lds, lines, s_tw, inverse);
     }
 }
 
-template <int LOG2N>
+// MAXR stages per pass (2 or 3; see radix_pass)
+template <int LOG2N, int MAXR>
 __device__ __forceinline__ void fft_lines(double2* lds, int lines, const double2* s_tw,
                                           bool inverse) {
-    radix_passes<LOG2N, 0>(lds, lines, s_tw, inverse);
+    radix_passes<LOG2N, 0, MAXR>(lds, lines, s_tw, inverse);
 }
 
 // frequencies 0 .. N/2 of the B lines in LDS to the half layout: element
@@ -460,7 +452,7 @@ __global__ void __launch_bounds__(256) k_fwd_x_real(const double* __restrict__ r
             lds[L::at(t >> LOG2N, bitrev(t & (N - 1), LOG2N))] = make_double2(r[u], 0.0);
     }
     __syncthreads();
-    fft_lines<LOG2N>(lds, lines, s_tw, false);
+    fft_lines<LOG2N, 2>(lds, lines, s_tw, false);
     store_half_x<LOG2N>(lds, out, line0, lines, LB, nlines);
 }
 
@@ -529,7 +521,7 @@ __global__ void __launch_bounds__(256) k_fwd_x_acc(RhoArgs ra, ClearArgs cl,
         lds[L::at(t >> LOG2N, bitrev(t & (N - 1), LOG2N))] = make_double2(r[u], 0.0);
     }
     __syncthreads();
-    fft_lines<LOG2N>(lds, lines, s_tw, false);
+    fft_lines<LOG2N, 2>(lds, lines, s_tw, false);
     store_half_x<LOG2N>(lds, out, line0, lines, LB, nlines);
 }
 
@@ -573,7 +565,7 @@ __global__ void __launch_bounds__(256) k_rows(Halves arr, int64_t nlines, int LB
         if (t < lines * N) lds[L::at(t >> LOG2N, bitrev(t & (N - 1), LOG2N))] = r[u];
     }
     __syncthreads();
-    fft_lines<LOG2N>(lds, lines, s_tw, inverse != 0);
+    fft_lines<LOG2N, 3>(lds, lines, s_tw, inverse != 0);
     const double scale = inverse ? 1.0 / static_cast<double>(N) : 1.0;
     for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
         double2 v = lds[L::at(t >> LOG2N, t & (N - 1))];
@@ -583,24 +575,24 @@ __global__ void __launch_bounds__(256) k_rows(Halves arr, int64_t nlines, int LB
     }
 }
 
-// forward transform of the last axis of H0, Green step, inverse transforms
-// along the same axis (scaled 1/N) of Ex, Ey, Ez (3D) and phi (when wanted)
-// to H0 .. H3.  A thread keeps phi_hat of its (at most 4: B*N <= 1024)
-// elements in registers (green_phi: the division once per mode); the
-// fields then go one at a time through the one LDS buffer of B lines
-// (green_field), each written over the transformed input.  One buffer of B
-// lines (not two, as a pair of fields would take) lets seven blocks share a
-// CU at N = 128, so the (N/2 + 1) * ny / B blocks of a cube run in one
-// round.
+// forward transform of the last axis of H0, Green step (green_split),
+// inverse transforms along the same axis (scaled 1/N) of Ex, Ey, Ez (3D)
+// and phi (when wanted) to H0 .. H3.  The fields go two at a time through
+// one LDS buffer of 2B lines (field a in lines 0 .. B-1, field b in B ..
+// 2B-1), so one pass of radix stages transforms both and keeps every thread
+// busy.  A thread keeps only its (at most 4: B*N <= 1024) elements of
+// rho_hat in registers and evaluates the Green step again for each pair
+// (the same operations, so the same values).  The fields are written over
+// the transformed input once every thread holds its share of rho_hat.
 // sy, sp: log2 of ny and of P = ny * nz (the half-layout node is
 // (kx << sp) + (z << sy) + y).
 template <int LOG2N>
 __global__ void __launch_bounds__(256) k_last(Halves arr, AxisPlan p, int LB,
-                                                 const double2* __restrict__ tw, GreenArgs g,
-                                                 int sy, int sp, int want_ez) {
+                                              const double2* __restrict__ tw, GreenArgs g,
+                                              int sy, int sp, int want_ez) {
     using L = Lines<LOG2N>;
     constexpr int N = 1 << LOG2N;
-    __shared__ double2 X[kLds<LOG2N>];
+    __shared__ double2 X[2 * kLds<LOG2N>];
     __shared__ double2 s_tw[N / 2 > 0 ? N / 2 : 1];
     stage_twiddles<LOG2N>(tw, s_tw);
     const int B = 1 << LB;
@@ -623,55 +615,60 @@ __global__ void __launch_bounds__(256) k_last(Halves arr, AxisPlan p, int LB,
         if (t < B * N) X[L::at(t & (B - 1), t >> LB)] = r[u];
     }
     __syncthreads();
-    fft_lines<LOG2N>(X, B, s_tw, false);
-    // mode indices of element t (lines past the end give zero fields)
-    auto mode = [&](int t, int& ix, int& iy, int& iz) {
-        const int64_t node = line_node(p, line0 + (t >> LOG2N), t & (N - 1));
-        ix = static_cast<int>(node >> sp);
-        iy = static_cast<int>(node & ((int64_t{1} << sy) - 1));
-        iz = static_cast<int>((node & ((int64_t{1} << sp) - 1)) >> sy);
-    };
-    double2 ph[4];
+    fft_lines<LOG2N, 3>(X, B, s_tw, false);
+    double2 rh[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int t = threadIdx.x + q * 256;
         const int tc = t < B * N ? t : 0;
         const double2 v = X[L::at(tc >> LOG2N, tc & (N - 1))];
         const bool ok = t < B * N && line0 + (t >> LOG2N) < p.nlines;
-        int ix, iy, iz;
-        mode(tc, ix, iy, iz);
-        ph[q] = green_phi(g, ix, iy, iz, make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0));
+        rh[q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
     }
-    // field F of every element into X in bit-reversed line order, its
-    // inverse transform written to out
-    auto field = [&]<int F>(double2* out) {
+    // fields FA (and FB >= 0) of every element (0 Ex, 1 Ey, 2 Ez, 3 phi)
+    // into X in bit-reversed line order, inverse transforms written to fa
+    // (and fb)
+    auto pair = [&]<int FA, int FB>(double2* fa, double2* fb) {
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int t = threadIdx.x + q * 256;
             if (t >= B * N) continue;
-            int ix, iy, iz;
-            mode(t, ix, iy, iz);
-            X[L::at(t >> LOG2N, bitrev(t & (N - 1), LOG2N))] =
-                green_field<F>(g, ix, iy, iz, ph[q]);
+            const int bb = t >> LOG2N, e = t & (N - 1);
+            // lines past the end (rh zero) give zero fields
+            const int64_t node = line_node(p, line0 + bb, e);
+            const int ix = static_cast<int>(node >> sp);
+            const int iy = static_cast<int>(node & ((int64_t{1} << sy) - 1));
+            const int iz = static_cast<int>((node & ((int64_t{1} << sp) - 1)) >> sy);
+            const ModeFields mf = green_split(g, ix, iy, iz, rh[q]);
+            const int at = L::at(bb, bitrev(e, LOG2N));
+            X[at] = FA == 0 ? mf.ex : FA == 1 ? mf.ey : FA == 2 ? mf.ez : mf.ph;
+            if constexpr (FB >= 0)
+                X[B * L::LS + at] = FB == 0 ? mf.ex : FB == 1 ? mf.ey : FB == 2 ? mf.ez : mf.ph;
         }
         __syncthreads();
-        fft_lines<LOG2N>(X, B, s_tw, true);
+        constexpr int NF = FB >= 0 ? 2 : 1;
+        fft_lines<LOG2N, 3>(X, NF * B, s_tw, true);
         const double scale = 1.0 / static_cast<double>(N);
-        for (int t = threadIdx.x; t < B * N; t += blockDim.x) {
-            const int bb = t & (B - 1), e = t >> LB;
+        for (int t = threadIdx.x; t < NF * B * N; t += blockDim.x) {
+            const int f = t >= B * N ? 1 : 0;
+            const int tt = t - f * B * N;
+            const int bb = tt & (B - 1), e = tt >> LB;
             const int64_t line = line0 + bb;
             if (line >= p.nlines) continue;
-            double2 v = X[L::at(bb, e)];
+            double2 v = X[L::at(f * B + bb, e)];
             v.x *= scale;
             v.y *= scale;
-            out[line_node(p, line, e)] = v;
+            (f ? fb : fa)[line_node(p, line, e)] = v;
         }
     };
-    field.template operator()<0>(arr.h[0]);
-    field.template operator()<1>(arr.h[1]);
-    if (want_ez) field.template operator()<2>(arr.h[2]);
-    if (g.want_phi) field.template operator()<3>(arr.h[3]);
+    pair.template operator()<0, 1>(arr.h[0], arr.h[1]);
+    if (want_ez && g.want_phi)
+        pair.template operator()<2, 3>(arr.h[2], arr.h[3]);
+    else if (want_ez)
+        pair.template operator()<2, -1>(arr.h[2], nullptr);
+    else if (g.want_phi)
+        pair.template operator()<3, -1>(arr.h[3], nullptr);
 }
 
 // One x line's full spectrum of (a + i b) from the half spectra a_k, b_k
@@ -733,8 +730,8 @@ __global__ void __launch_bounds__(256) k_inv_x_c2r(Halves arr, int64_t nlines, i
         if (use_b) put_pair<LOG2N>(Bb, b, k, vc[u], vd[u]);
     }
     __syncthreads();
-    fft_lines<LOG2N>(A, lines, s_tw, true);
-    if (use_b) fft_lines<LOG2N>(Bb, lines, s_tw, true);
+    fft_lines<LOG2N, 3>(A, lines, s_tw, true);
+    if (use_b) fft_lines<LOG2N, 3>(Bb, lines, s_tw, true);
     const double scale = 1.0 / static_cast<double>(N);
     for (int t = threadIdx.x; t < lines * N; t += blockDim.x) {
         const int64_t i = line0 * N + t;
