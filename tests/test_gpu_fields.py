@@ -177,12 +177,15 @@ def _spectral_reference(rho, n, L, eps):
 
 
 @pytest.mark.parametrize("n", [[4, 4], [4, 1024], [1024, 4], [64, 8], [4, 4, 4], [4, 8, 64],
-                               [128, 4, 4], [8, 4, 16]])
+                               [128, 4, 4], [8, 4, 16], [32, 4, 8], [256, 16], [512, 8]])
 def test_half_spectrum_solve_edge_shapes(hip_ctx, n):
     """The power-of-two solve on half spectra at the edges of its blocking:
     4-point axes (64 lines per block, partial last blocks) and 1024-point
-    axes (one line per block), against the reference's formula evaluated
-    with numpy's full complex FFT; E and phi within 1e-11 relative."""
+    axes (one line per block), and x axes of every power of two from 4 to
+    1024 (the forward x transform's passes of two radix-2 stages end in a
+    one-stage pass on odd powers), against the reference's formula
+    evaluated with numpy's full complex FFT; E and phi within 1e-11
+    relative."""
     print(f"n={n}", flush=True)
     rng = np.random.default_rng(len(n) * 100 + n[0])
     L = [1.0, 1.3, 0.7][:len(n)]
